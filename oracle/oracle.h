@@ -1,0 +1,128 @@
+/*
+ * CPU oracle for the chunk-extraction path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's per-chunk extraction
+ * (Lin20/BinaryMeshFitting), used by tests/ as the parity checker and by
+ * bench.py as the CPU baseline.  Nothing in the product links or calls it.
+ * Each function cites the reference code it restates.  Compiled with
+ * -ffp-contract=off (no FMA contraction, the reference builds with
+ * /fp:strict, BinaryMeshFitting.vcxproj:138); the noise sampler uses
+ * explicit fmaf() at fixed places, mirrored exactly by the HIP kernels.
+ *
+ * Parity pins (tests/test_oracle_*.py): every function here is checked
+ * against an independent formulation -- numpy brute force over corners,
+ * edges, cells and adjacency, closed forms of the smoothing operator,
+ * the reference-held MC table, exhaustive single-cell cases, an FP64 twin,
+ * and special cases of the QEF solver.  The FBM noise has no reference
+ * counterpart (the reference calls the absent FastNoiseSIMD library), so it
+ * is pinned to its own published definition (Gustavson 2005 simplex noise)
+ * by a brute-force all-lattice-points evaluation in numpy.
+ */
+#ifndef BMF_ORACLE_H
+#define BMF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    OR_SAMPLER_TERRAIN = 0, /* d = amp*fbm(p) - (p.y - y0) */
+    OR_SAMPLER_SPHERE = 1,  /* ImplicitSampler.cpp:46-50 */
+    OR_SAMPLER_CUBOID = 2,  /* ImplicitSampler.cpp:52-60 */
+    OR_SAMPLER_TORUS_Z = 3, /* ImplicitSampler.cpp:27-34 */
+    OR_SAMPLER_PLANE_Y = 4, /* ImplicitSampler.cpp:62-65 */
+    OR_SAMPLER_CSG = 5      /* union of spheres and boxes, positive inside */
+};
+
+typedef struct {
+    int32_t kind;
+    float world_size;   /* Sampler::world_size (Sampler.hpp:25) */
+    float amp, y0;      /* terrain: d = amp*fbm - (y - y0) */
+    float freq, lacunarity, gain;
+    int32_t octaves;
+    uint32_t seed;
+    const float* spheres; /* CSG: n_spheres x (cx, cy, cz, r) */
+    int32_t n_spheres;
+    const float* boxes;   /* CSG: n_boxes x (cx, cy, cz, hx, hy, hz) */
+    int32_t n_boxes;
+} or_sampler_t;
+
+typedef struct {
+    float pos[3];   /* DMCChunk::pos */
+    float size;     /* DMCChunk::size */
+    int32_t dim;    /* samples per axis (DMCChunk::dim) */
+    float overlap;  /* ChunkGenerator.cpp:98 result */
+    int32_t iters;  /* WorldProperties::process_iters */
+    int32_t boundary_processing;
+} or_chunk_t;
+
+typedef struct {
+    float* pos;          /* V x 3, grid units */
+    uint8_t* boundary;   /* V */
+    uint32_t* edge;      /* V: packed (x,y,z,axis) of the lattice edge: ((x*dim+y)*dim+z)*3+axis */
+    uint32_t* idx;       /* nI triangle indices */
+    int64_t V, nI;
+    int32_t contains_mesh;
+    float delta;         /* DMCChunk::scale */
+    float opos[3];       /* DMCChunk::overlap_pos */
+} or_mesh_t;
+
+/* ---- geometry (DMCChunk.cpp:94-101) ---- */
+void or_chunk_geometry(const or_chunk_t* c, float* delta, float opos[3]);
+
+/* ---- samplers: fill dim^3 [x][y][z] at opos + i*delta ---- */
+float or_snoise3_f32(float x, float y, float z, uint32_t seed);
+double or_snoise3_f64(double x, double y, double z, uint32_t seed);
+uint32_t or_grad_index(int32_t i, int32_t j, int32_t k, uint32_t seed);
+void or_fbm_params(const or_sampler_t* s, float* fo, float* ao, float* inv_norm);
+float or_terrain_value_f32(const or_sampler_t* s, float x, float y, float z);
+double or_terrain_value_f64(const or_sampler_t* s, double x, double y, double z);
+void or_sample_block_f32(const or_sampler_t* s, int dim, const float opos[3], float delta, float* out);
+void or_sample_block_f64(const or_sampler_t* s, int dim, const double opos[3], double delta, double* out);
+float or_csg_value_f32(const or_sampler_t* s, float x, float y, float z, float grad[3]);
+double or_csg_value_f64(const or_sampler_t* s, double x, double y, double z, double grad[3]);
+
+/* ---- DMCChunk::label_grid threshold (DMCChunk.cpp:113-162) ---- */
+int or_label_grid(const float* dens, int dim, uint32_t* words);
+
+/* ---- DMCChunk::label_edges mask passes (DMCChunk.cpp:200-438) ----
+ * masks must hold dim*dim*ceil(dim/8) + 4 words. */
+void or_cell_masks(const uint32_t* words, int dim, uint64_t* masks);
+
+/* ---- label_edges cell loop + polygonize (DMCChunk.cpp:440-576, 593-689) ----
+ * Writes up to capV vertices and capI indices; returns 0, or -1 when a
+ * capacity is too small (V/nI then hold the required counts). */
+int or_mesh_from_masks(const float* dens, const uint64_t* masks, int dim,
+                       float* pos, uint8_t* boundary, uint32_t* edge, uint8_t* valence, int64_t capV,
+                       uint32_t* idx, int64_t capI, int64_t* V, int64_t* nI);
+
+/* ---- MeshProcessor<3> smoothing as driven by ChunkGenerator.cpp:112-124 ---- */
+int or_smooth_f32(float* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI,
+                  int iters, int boundary_processing);
+int or_smooth_f64(double* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI,
+                  int iters, int boundary_processing);
+
+/* ---- ChunkGenerator::extract_chunk for one chunk (ChunkGenerator.cpp:80-147) ----
+ * dens_out (dim^3) is optional.  The mesh arrays are malloc'd; free with or_free_mesh. */
+int or_extract_chunk(const or_sampler_t* s, const or_chunk_t* c, or_mesh_t* out, float* dens_out);
+void or_free_mesh(or_mesh_t* m);
+
+/* FP64 twin of vertex placement + smoothing on a fixed topology (the FP32
+ * mesh's edges and indices): positions from FP64 densities. */
+int or_positions_f64(const or_sampler_t* s, const or_chunk_t* c, const or_mesh_t* topo, double* pos_out);
+
+/* ---- batch driver: OpenMP parallel for over chunks (ChunkGenerator.cpp:91) ----
+ * Returns the number of chunks processed; per-chunk V, nI, contains_mesh
+ * and an order-sensitive checksum of positions and indices. */
+int or_extract_batch(const or_sampler_t* s, const or_chunk_t* chunks, int n, int threads,
+                     int64_t* V, int64_t* nI, int32_t* contains, uint64_t* checksum);
+
+/* ---- QEF (qef_simd.h:392-579): FP32 with exact 1/sqrt, and FP64 twin ---- */
+float or_qef_solve_f32(const float* p, const float* n, int count, float tol, float out[3]);
+double or_qef_solve_f64(const double* p, const double* n, int count, double tol, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

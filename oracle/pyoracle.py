@@ -1,0 +1,269 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Imported by tests/, by the smoke check in __graft_entry__.py and by the
+CPU-baseline leg of bench.py -- never by the product package.
+"""
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+SAMPLER_TERRAIN, SAMPLER_SPHERE, SAMPLER_CUBOID, SAMPLER_TORUS_Z, SAMPLER_PLANE_Y, SAMPLER_CSG = range(6)
+
+
+class Sampler(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("world_size", C.c_float),
+        ("amp", C.c_float),
+        ("y0", C.c_float),
+        ("freq", C.c_float),
+        ("lacunarity", C.c_float),
+        ("gain", C.c_float),
+        ("octaves", C.c_int32),
+        ("seed", C.c_uint32),
+        ("spheres", C.POINTER(C.c_float)),
+        ("n_spheres", C.c_int32),
+        ("boxes", C.POINTER(C.c_float)),
+        ("n_boxes", C.c_int32),
+    ]
+
+
+class Chunk(C.Structure):
+    _fields_ = [
+        ("pos", C.c_float * 3),
+        ("size", C.c_float),
+        ("dim", C.c_int32),
+        ("overlap", C.c_float),
+        ("iters", C.c_int32),
+        ("boundary_processing", C.c_int32),
+    ]
+
+
+class Mesh(C.Structure):
+    _fields_ = [
+        ("pos", C.POINTER(C.c_float)),
+        ("boundary", C.POINTER(C.c_uint8)),
+        ("edge", C.POINTER(C.c_uint32)),
+        ("idx", C.POINTER(C.c_uint32)),
+        ("V", C.c_int64),
+        ("nI", C.c_int64),
+        ("contains_mesh", C.c_int32),
+        ("delta", C.c_float),
+        ("opos", C.c_float * 3),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run: make -C oracle)")
+        L = C.CDLL(str(LIB_PATH))
+        P = C.POINTER
+        f32p, f64p, u32p, u64p, u8p, i64p, i32p = (P(C.c_float), P(C.c_double), P(C.c_uint32), P(C.c_uint64),
+                                                  P(C.c_uint8), P(C.c_int64), P(C.c_int32))
+        L.or_chunk_geometry.argtypes = [P(Chunk), f32p, f32p]
+        L.or_snoise3_f32.argtypes = [C.c_float, C.c_float, C.c_float, C.c_uint32]
+        L.or_snoise3_f32.restype = C.c_float
+        L.or_snoise3_f64.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32]
+        L.or_snoise3_f64.restype = C.c_double
+        L.or_grad_index.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
+        L.or_grad_index.restype = C.c_uint32
+        L.or_fbm_params.argtypes = [P(Sampler), f32p, f32p, f32p]
+        L.or_terrain_value_f32.argtypes = [P(Sampler), C.c_float, C.c_float, C.c_float]
+        L.or_terrain_value_f32.restype = C.c_float
+        L.or_terrain_value_f64.argtypes = [P(Sampler), C.c_double, C.c_double, C.c_double]
+        L.or_terrain_value_f64.restype = C.c_double
+        L.or_sample_block_f32.argtypes = [P(Sampler), C.c_int, f32p, C.c_float, f32p]
+        L.or_sample_block_f64.argtypes = [P(Sampler), C.c_int, f64p, C.c_double, f64p]
+        L.or_csg_value_f32.argtypes = [P(Sampler), C.c_float, C.c_float, C.c_float, f32p]
+        L.or_csg_value_f32.restype = C.c_float
+        L.or_csg_value_f64.argtypes = [P(Sampler), C.c_double, C.c_double, C.c_double, f64p]
+        L.or_csg_value_f64.restype = C.c_double
+        L.or_label_grid.argtypes = [f32p, C.c_int, u32p]
+        L.or_label_grid.restype = C.c_int
+        L.or_cell_masks.argtypes = [u32p, C.c_int, u64p]
+        L.or_mesh_from_masks.argtypes = [f32p, u64p, C.c_int, f32p, u8p, u32p, u8p, C.c_int64, u32p, C.c_int64,
+                                         i64p, i64p]
+        L.or_mesh_from_masks.restype = C.c_int
+        L.or_smooth_f32.argtypes = [f32p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int]
+        L.or_smooth_f64.argtypes = [f64p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int]
+        L.or_extract_chunk.argtypes = [P(Sampler), P(Chunk), P(Mesh), f32p]
+        L.or_extract_chunk.restype = C.c_int
+        L.or_free_mesh.argtypes = [P(Mesh)]
+        L.or_positions_f64.argtypes = [P(Sampler), P(Chunk), P(Mesh), f64p]
+        L.or_extract_batch.argtypes = [P(Sampler), P(Chunk), C.c_int, C.c_int, i64p, i64p, i32p, u64p]
+        L.or_extract_batch.restype = C.c_int
+        L.or_qef_solve_f32.argtypes = [f32p, f32p, C.c_int, C.c_float, f32p]
+        L.or_qef_solve_f32.restype = C.c_float
+        L.or_qef_solve_f64.argtypes = [f64p, f64p, C.c_int, C.c_double, f64p]
+        L.or_qef_solve_f64.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class SamplerSpec:
+    """Keeps the numpy arrays a Sampler struct points into alive."""
+
+    def __init__(self, kind=SAMPLER_TERRAIN, world_size=256.0, amp=24.0, y0=32.0, freq=0.02, lacunarity=2.0,
+                 gain=0.5, octaves=5, seed=1337, spheres=None, boxes=None):
+        self.spheres = np.ascontiguousarray(spheres if spheres is not None else np.zeros((0, 4)), np.float32)
+        self.boxes = np.ascontiguousarray(boxes if boxes is not None else np.zeros((0, 6)), np.float32)
+        self.s = Sampler(kind, world_size, amp, y0, freq, lacunarity, gain, octaves, seed & 0xFFFFFFFF,
+                         _ptr(self.spheres, C.c_float), len(self.spheres), _ptr(self.boxes, C.c_float),
+                         len(self.boxes))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def make_chunk(pos, size, dim, overlap=0.0, iters=0, boundary_processing=0):
+    return Chunk((C.c_float * 3)(*pos), size, dim, overlap, iters, boundary_processing)
+
+
+def geometry(chunk):
+    d = C.c_float()
+    o = (C.c_float * 3)()
+    lib().or_chunk_geometry(C.byref(chunk), C.byref(d), o)
+    return d.value, np.array(list(o), np.float32)
+
+
+def sample_block(spec, dim, opos, delta):
+    out = np.empty(dim ** 3, np.float32)
+    o = np.ascontiguousarray(opos, np.float32)
+    lib().or_sample_block_f32(spec.ref, dim, _ptr(o, C.c_float), delta, _ptr(out, C.c_float))
+    return out.reshape(dim, dim, dim)
+
+
+def sample_block_f64(spec, dim, opos, delta):
+    out = np.empty(dim ** 3, np.float64)
+    o = np.ascontiguousarray(opos, np.float64)
+    lib().or_sample_block_f64(spec.ref, dim, _ptr(o, C.c_double), delta, _ptr(out, C.c_double))
+    return out.reshape(dim, dim, dim)
+
+
+def label_grid(dens):
+    dim = dens.shape[0]
+    zc = (dim + 31) // 32
+    words = np.zeros(dim * dim * zc, np.uint32)
+    d = np.ascontiguousarray(dens, np.float32)
+    cm = lib().or_label_grid(_ptr(d, C.c_float), dim, _ptr(words, C.c_uint32))
+    return words.reshape(dim, dim, zc), bool(cm)
+
+
+def cell_masks(words, dim):
+    z8 = (dim + 7) // 8
+    masks = np.zeros(dim * dim * z8 + 4, np.uint64)
+    w = np.ascontiguousarray(words, np.uint32)
+    lib().or_cell_masks(_ptr(w, C.c_uint32), dim, _ptr(masks, C.c_uint64))
+    return masks[: dim * dim * z8].reshape(dim, dim, z8)
+
+
+def mesh_from_masks(dens, masks):
+    dim = dens.shape[0]
+    d = np.ascontiguousarray(dens, np.float32)
+    m = np.zeros(dim * dim * ((dim + 7) // 8) + 4, np.uint64)
+    m[: masks.size] = masks.ravel()
+    capV, capI = 3 * dim ** 3, 15 * max(dim - 1, 1) ** 3
+    pos = np.zeros((capV, 3), np.float32)
+    bnd = np.zeros(capV, np.uint8)
+    edge = np.zeros(capV, np.uint32)
+    val = np.zeros(capV, np.uint8)
+    idx = np.zeros(capI, np.uint32)
+    V, nI = C.c_int64(), C.c_int64()
+    rc = lib().or_mesh_from_masks(_ptr(d, C.c_float), _ptr(m, C.c_uint64), dim, _ptr(pos, C.c_float),
+                                  _ptr(bnd, C.c_uint8), _ptr(edge, C.c_uint32), _ptr(val, C.c_uint8), capV,
+                                  _ptr(idx, C.c_uint32), capI, C.byref(V), C.byref(nI))
+    if rc != 0:
+        raise RuntimeError(f"or_mesh_from_masks rc={rc}")
+    V, nI = V.value, nI.value
+    return dict(pos=pos[:V].copy(), boundary=bnd[:V].copy(), edge=edge[:V].copy(), valence=val[:V].copy(),
+                idx=idx[:nI].copy())
+
+
+def smooth(pos, boundary, idx, iters, boundary_processing=False, f64=False):
+    p = np.ascontiguousarray(pos, np.float64 if f64 else np.float32).copy()
+    b = np.ascontiguousarray(boundary, np.uint8)
+    i = np.ascontiguousarray(idx, np.uint32)
+    fn = lib().or_smooth_f64 if f64 else lib().or_smooth_f32
+    fn(_ptr(p, C.c_double if f64 else C.c_float), _ptr(b, C.c_uint8), len(b), _ptr(i, C.c_uint32), len(i), iters,
+       int(boundary_processing))
+    return p
+
+
+def extract_chunk(spec, chunk, want_density=False, want_f64=False):
+    m = Mesh()
+    dens = np.empty(chunk.dim ** 3, np.float32) if want_density else None
+    rc = lib().or_extract_chunk(spec.ref, C.byref(chunk), C.byref(m),
+                                _ptr(dens, C.c_float) if dens is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"or_extract_chunk rc={rc}")
+    V, nI = m.V, m.nI
+    out = dict(V=V, nI=nI, contains_mesh=bool(m.contains_mesh), delta=m.delta, opos=np.array(list(m.opos)))
+    if V:
+        out["pos"] = np.ctypeslib.as_array(m.pos, shape=(V * 3,)).reshape(V, 3).copy()
+        out["boundary"] = np.ctypeslib.as_array(m.boundary, shape=(V,)).copy()
+        out["edge"] = np.ctypeslib.as_array(m.edge, shape=(V,)).copy()
+    else:
+        out["pos"] = np.zeros((0, 3), np.float32)
+        out["boundary"] = np.zeros(0, np.uint8)
+        out["edge"] = np.zeros(0, np.uint32)
+    out["idx"] = np.ctypeslib.as_array(m.idx, shape=(nI,)).copy() if nI else np.zeros(0, np.uint32)
+    if want_f64 and V:
+        p64 = np.empty((V, 3), np.float64)
+        lib().or_positions_f64(spec.ref, C.byref(chunk), C.byref(m), _ptr(p64, C.c_double))
+        out["pos_f64"] = p64
+    lib().or_free_mesh(C.byref(m))
+    if dens is not None:
+        out["density"] = dens.reshape(chunk.dim, chunk.dim, chunk.dim)
+    return out
+
+
+def extract_batch(spec, chunks, threads=0):
+    n = len(chunks)
+    arr = (Chunk * n)(*chunks)
+    V = np.zeros(n, np.int64)
+    nI = np.zeros(n, np.int64)
+    cm = np.zeros(n, np.int32)
+    ck = np.zeros(n, np.uint64)
+    lib().or_extract_batch(spec.ref, arr, n, threads, _ptr(V, C.c_int64), _ptr(nI, C.c_int64), _ptr(cm, C.c_int32),
+                           _ptr(ck, C.c_uint64))
+    return V, nI, cm, ck
+
+
+def qef_solve(p, n, tol, f64=False):
+    dt, ct = (np.float64, C.c_double) if f64 else (np.float32, C.c_float)
+    p = np.ascontiguousarray(p, dt)
+    n = np.ascontiguousarray(n, dt)
+    out = np.zeros(3, dt)
+    fn = lib().or_qef_solve_f64 if f64 else lib().or_qef_solve_f32
+    err = fn(_ptr(p, ct), _ptr(n, ct), len(p), tol, _ptr(out, ct))
+    return out, err
+
+
+def checksum_mesh(pos, idx):
+    """Same order-sensitive checksum as or_extract_batch."""
+    h = 0
+    M = (1 << 64) - 1
+
+    def mix(h, v):
+        return (h ^ ((v + 0x9E3779B97F4A7C15 + (h << 6) + (h >> 2)) & M)) & M
+
+    for b in np.ascontiguousarray(pos, np.float32).ravel().view(np.uint32):
+        h = mix(h, int(b))
+    for v in np.asarray(idx, np.uint32):
+        h = mix(h, int(v))
+    return h
