@@ -1,0 +1,205 @@
+"""ctypes binding of libbmf.so (include/bmf.h).
+
+This is the Python face of the C ABI used by the tests, the smoke check and
+bench.py.  It loads only the in-tree HIP library and raises if it is missing:
+there is no CPU fallback.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libbmf.so"
+
+OK, EINVAL, ECAPACITY, EHIP, ENODEV, ESTATE = range(6)
+SAMPLER_TERRAIN, SAMPLER_SPHERE, SAMPLER_CUBOID, SAMPLER_TORUS_Z, SAMPLER_PLANE_Y, SAMPLER_CSG = range(6)
+MODE_PLAIN, MODE_BATCHED = 0, 1
+MAX_DIM = 256
+
+EXPORTS = [
+    "bmf_create", "bmf_destroy", "bmf_strerror", "bmf_version", "bmf_set_sampler", "bmf_validate_chunks",
+    "bmf_extract", "bmf_result", "bmf_mesh_view_get", "bmf_mesh_to_host", "bmf_sync", "bmf_sample_block",
+    "bmf_label_block", "bmf_cell_masks", "bmf_last_timing",
+]
+
+
+class Limits(C.Structure):
+    _fields_ = [("max_chunks", C.c_int32), ("max_samples", C.c_int64), ("max_vertices", C.c_int64),
+                ("max_indices", C.c_int64)]
+
+
+class SamplerDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("world_size", C.c_float), ("amp", C.c_float), ("y0", C.c_float),
+        ("freq", C.c_float), ("lacunarity", C.c_float), ("gain", C.c_float), ("octaves", C.c_int32),
+        ("seed", C.c_uint32), ("spheres", C.POINTER(C.c_float)), ("n_spheres", C.c_int32),
+        ("boxes", C.POINTER(C.c_float)), ("n_boxes", C.c_int32),
+    ]
+
+
+class ChunkDesc(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("size", C.c_float), ("dim", C.c_int32), ("overlap", C.c_float),
+                ("iters", C.c_int32), ("boundary_processing", C.c_int32)]
+
+
+class ChunkResult(C.Structure):
+    _fields_ = [("vertex_offset", C.c_int64), ("index_offset", C.c_int64), ("vertex_count", C.c_int32),
+                ("index_count", C.c_int32), ("contains_mesh", C.c_int32), ("status", C.c_int32),
+                ("delta", C.c_float), ("overlap_pos", C.c_float * 3)]
+
+
+class MeshView(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("indices", C.c_void_p), ("boundary", C.c_void_p),
+                ("total_vertices", C.c_int64), ("total_indices", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libbmf.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        P = C.POINTER
+        ctxp = C.c_void_p
+        L.bmf_create.argtypes = [P(ctxp), C.c_int, P(Limits)]
+        L.bmf_destroy.argtypes = [ctxp]
+        L.bmf_destroy.restype = None
+        L.bmf_strerror.argtypes = [C.c_int]
+        L.bmf_strerror.restype = C.c_char_p
+        L.bmf_version.restype = C.c_char_p
+        L.bmf_set_sampler.argtypes = [ctxp, P(SamplerDesc)]
+        L.bmf_validate_chunks.argtypes = [P(ChunkDesc), C.c_int, P(Limits)]
+        L.bmf_extract.argtypes = [ctxp, P(ChunkDesc), C.c_int, C.c_int]
+        L.bmf_result.argtypes = [ctxp, P(ChunkResult), C.c_int, P(C.c_int64), P(C.c_int64)]
+        L.bmf_mesh_view_get.argtypes = [ctxp, P(MeshView)]
+        L.bmf_mesh_to_host.argtypes = [ctxp, C.c_int, P(C.c_float), P(C.c_uint32), P(C.c_uint8), C.c_int64,
+                                       C.c_int64, P(C.c_int64), P(C.c_int64)]
+        L.bmf_sync.argtypes = [ctxp]
+        L.bmf_sample_block.argtypes = [ctxp, P(ChunkDesc), P(C.c_float)]
+        L.bmf_label_block.argtypes = [ctxp, P(ChunkDesc), P(C.c_uint32), P(C.c_int32)]
+        L.bmf_cell_masks.argtypes = [ctxp, P(ChunkDesc), P(C.c_uint64)]
+        L.bmf_last_timing.argtypes = [ctxp, P(C.c_float), P(C.c_float)]
+        _lib = L
+    return _lib
+
+
+class BmfError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__(f"{what}: {lib().bmf_strerror(code).decode()} ({code})")
+
+
+def check(code, what):
+    if code != OK:
+        raise BmfError(code, what)
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def chunk_desc(pos, size, dim, overlap=0.0, iters=0, boundary_processing=0):
+    return ChunkDesc((C.c_float * 3)(*pos), size, dim, overlap, iters, boundary_processing)
+
+
+def chunk_array(chunks):
+    return (ChunkDesc * len(chunks))(*chunks)
+
+
+class Context:
+    """A bmf_ctx with its arenas; one per host thread."""
+
+    def __init__(self, max_chunks=1024, max_samples=64 * 65 ** 3, max_vertices=1 << 22, max_indices=1 << 24,
+                 device=0):
+        self.limits = Limits(max_chunks, max_samples, max_vertices, max_indices)
+        h = C.c_void_p()
+        check(lib().bmf_create(C.byref(h), device, C.byref(self.limits)), "bmf_create")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib().bmf_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_sampler(self, kind=SAMPLER_TERRAIN, world_size=256.0, amp=24.0, y0=32.0, freq=0.02, lacunarity=2.0,
+                    gain=0.5, octaves=5, seed=1337, spheres=None, boxes=None):
+        sp = np.ascontiguousarray(spheres if spheres is not None else np.zeros((0, 4)), np.float32)
+        bx = np.ascontiguousarray(boxes if boxes is not None else np.zeros((0, 6)), np.float32)
+        d = SamplerDesc(kind, world_size, amp, y0, freq, lacunarity, gain, octaves, seed & 0xFFFFFFFF,
+                        _ptr(sp, C.c_float), len(sp), _ptr(bx, C.c_float), len(bx))
+        check(lib().bmf_set_sampler(self.h, C.byref(d)), "bmf_set_sampler")
+
+    def extract(self, chunks, mode=MODE_BATCHED):
+        arr = chunk_array(chunks)
+        self._keep = [arr]
+        check(lib().bmf_extract(self.h, arr, len(chunks), mode), "bmf_extract")
+        return len(chunks)
+
+    def extract_raw(self, arr, n, mode=MODE_BATCHED):
+        return lib().bmf_extract(self.h, arr, n, mode)
+
+    def sync(self):
+        check(lib().bmf_sync(self.h), "bmf_sync")
+
+    def results(self, n):
+        out = (ChunkResult * n)()
+        nv, ni = C.c_int64(), C.c_int64()
+        rc = lib().bmf_result(self.h, out, n, C.byref(nv), C.byref(ni))
+        return rc, list(out), nv.value, ni.value
+
+    def mesh(self, chunk, cap_v=None, cap_i=None):
+        nv, ni = C.c_int64(), C.c_int64()
+        rc = lib().bmf_mesh_to_host(self.h, chunk, None, None, None, 0, 0, C.byref(nv), C.byref(ni))
+        if rc not in (OK, ECAPACITY):
+            check(rc, "bmf_mesh_to_host")
+        V, I = nv.value, ni.value
+        pos = np.zeros((V, 3), np.float32)
+        idx = np.zeros(I, np.uint32)
+        bnd = np.zeros(V, np.uint8)
+        check(lib().bmf_mesh_to_host(self.h, chunk, _ptr(pos, C.c_float), _ptr(idx, C.c_uint32),
+                                     _ptr(bnd, C.c_uint8), V, I, C.byref(nv), C.byref(ni)), "bmf_mesh_to_host")
+        return pos, idx, bnd
+
+    def sample_block(self, chunk):
+        out = np.zeros(chunk.dim ** 3, np.float32)
+        check(lib().bmf_sample_block(self.h, C.byref(chunk), _ptr(out, C.c_float)), "bmf_sample_block")
+        return out.reshape(chunk.dim, chunk.dim, chunk.dim)
+
+    def label_block(self, chunk):
+        dim = chunk.dim
+        out = np.zeros(dim * dim * ((dim + 31) // 32), np.uint32)
+        cm = C.c_int32()
+        check(lib().bmf_label_block(self.h, C.byref(chunk), _ptr(out, C.c_uint32), C.byref(cm)), "bmf_label_block")
+        return out.reshape(dim, dim, (dim + 31) // 32), bool(cm.value)
+
+    def cell_masks(self, chunk):
+        dim = chunk.dim
+        out = np.zeros(dim * dim * ((dim + 7) // 8), np.uint64)
+        check(lib().bmf_cell_masks(self.h, C.byref(chunk), _ptr(out, C.c_uint64)), "bmf_cell_masks")
+        return out.reshape(dim, dim, (dim + 7) // 8)
+
+    def last_timing(self):
+        t, s = C.c_float(), C.c_float()
+        check(lib().bmf_last_timing(self.h, C.byref(t), C.byref(s)), "bmf_last_timing")
+        return t.value, s.value
+
+
+def validate_chunks(chunks, limits=None):
+    arr = chunk_array(chunks) if chunks else None
+    return lib().bmf_validate_chunks(arr, len(chunks), C.byref(limits) if limits is not None else None)

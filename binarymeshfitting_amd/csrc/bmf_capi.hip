@@ -1,0 +1,495 @@
+// C ABI of libbmf.so (declared in include/bmf.h): contexts, device arenas,
+// the PLAIN and BATCHED execution paths and stage inspection.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/bmf.h"
+#include "bmf_internal.h"
+
+using namespace bmf;
+
+namespace {
+
+struct GraphEntry {
+    int n = -1;
+    int maxk = -1;
+    int sample_blocks = 0;
+    hipGraphExec_t exec = nullptr;
+};
+
+}  // namespace
+
+struct bmf_ctx {
+    int device = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    bmf_limits lim{};
+    Arenas A{};
+    bmf_chunk_desc* h_descs = nullptr;      // pinned
+    bmf_chunk_result* h_results = nullptr;  // pinned
+    Counters* h_ctr = nullptr;              // pinned
+    SamplerDev h_sampler{};
+    float* d_spheres = nullptr;
+    float* d_boxes = nullptr;
+    bool sampler_set = false;
+    int last_n = 0;
+    bool have_extract = false;
+    bool results_valid = false;
+    std::vector<GraphEntry> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;
+    bool sample_timed = false;
+    // last sampling-kernel layout
+    int persistent_blocks = 2048;
+};
+
+namespace {
+
+#define BMF_HIP(call)                                  \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return BMF_EHIP;         \
+    } while (0)
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+bool finite(float v) { return std::isfinite(v); }
+
+template <class T>
+hipError_t dalloc(T** p, int64_t count) {
+    return hipMalloc((void**)p, (size_t)std::max<int64_t>(count, 1) * sizeof(T));
+}
+
+void free_arenas(bmf_ctx* c) {
+    Arenas& A = c->A;
+    void* ptrs[] = {A.meta, A.descs, A.results, A.tile_prefix, A.row_prefix, A.vtile_prefix, A.ttile_prefix,
+                    A.dens, A.labels, A.row_v, A.row_t, A.pos, A.boundary, A.valence, A.adj, A.idx, A.cen,
+                    A.ctr, A.sampler, c->d_spheres, c->d_boxes};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    A = Arenas{};
+    c->d_spheres = c->d_boxes = nullptr;
+}
+
+// The whole device pipeline for chunks [0, n) of the uploaded descriptors.
+int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int allow_cull, int reset, int sample_blocks,
+                     int row_blocks, int scan_blocks, int smooth_blocks, bool time_sample) {
+    hipStream_t s = c->stream;
+    BMF_HIP(launch_plan(A, n, allow_cull, reset, s));
+    if (time_sample) BMF_HIP(hipEventRecord(c->evs0, s));
+    BMF_HIP(launch_sample(A, n, sample_blocks, s));
+    if (time_sample) BMF_HIP(hipEventRecord(c->evs1, s));
+    BMF_HIP(launch_count(A, n, row_blocks, s));
+    BMF_HIP(launch_scan(A, n, scan_blocks, s));
+    BMF_HIP(launch_emit(A, n, row_blocks, s));
+    if (maxk > 0) {
+        BMF_HIP(launch_sort_adj(A, n, smooth_blocks, s));
+        for (int m = 0; m < maxk; m++) {
+            BMF_HIP(launch_dual(A, n, m, smooth_blocks, s));
+            BMF_HIP(launch_primal(A, n, m, smooth_blocks, s));
+        }
+    }
+    BMF_HIP(launch_finalize(A, n, s));
+    return BMF_OK;
+}
+
+int upload_descs(bmf_ctx* c, const bmf_chunk_desc* chunks, int n) {
+    std::memcpy(c->h_descs, chunks, sizeof(bmf_chunk_desc) * (size_t)n);
+    BMF_HIP(hipMemcpyAsync(c->A.descs, c->h_descs, sizeof(bmf_chunk_desc) * (size_t)n, hipMemcpyHostToDevice,
+                           c->stream));
+    return BMF_OK;
+}
+
+int check_desc(const bmf_chunk_desc& d) {
+    if (d.dim < 2 || d.dim > BMF_MAX_DIM) return BMF_EINVAL;
+    if (!finite(d.size) || !(d.size > 0.0f)) return BMF_EINVAL;
+    if (!finite(d.overlap) || d.overlap < 0.0f || d.overlap > 1.0f) return BMF_EINVAL;
+    if (!finite(d.pos[0]) || !finite(d.pos[1]) || !finite(d.pos[2])) return BMF_EINVAL;
+    if (d.iters < 0 || d.iters > 64) return BMF_EINVAL;
+    if (d.boundary_processing != 0 && d.boundary_processing != 1) return BMF_EINVAL;
+    return BMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bmf_strerror(int code) {
+    switch (code) {
+        case BMF_OK: return "ok";
+        case BMF_EINVAL: return "invalid argument";
+        case BMF_ECAPACITY: return "arena capacity exceeded";
+        case BMF_EHIP: return "HIP runtime error";
+        case BMF_ENODEV: return "no usable GPU";
+        case BMF_ESTATE: return "call out of order";
+    }
+    return "unknown error";
+}
+
+const char* bmf_version(void) { return "bmf 0.1 (gfx950)"; }
+
+int bmf_validate_chunks(const bmf_chunk_desc* chunks, int n, const bmf_limits* lim) {
+    if (n < 0 || (n > 0 && !chunks)) return BMF_EINVAL;
+    if (lim && n > lim->max_chunks) return BMF_EINVAL;
+    for (int i = 0; i < n; i++) {
+        int rc = check_desc(chunks[i]);
+        if (rc) return rc;
+    }
+    return BMF_OK;
+}
+
+int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
+    if (!out || !lim) return BMF_EINVAL;
+    *out = nullptr;
+    if (lim->max_chunks < 1 || lim->max_chunks > (1 << 20) || lim->max_samples < 8 || lim->max_vertices < 1 ||
+        lim->max_indices < 3)
+        return BMF_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BMF_ENODEV;
+    if (device < 0 || device >= ndev) return BMF_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return BMF_EHIP;
+    bmf_ctx* c = new (std::nothrow) bmf_ctx();
+    if (!c) return BMF_EHIP;
+    c->device = device;
+    c->lim = *lim;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->cu_count = prop.multiProcessorCount;
+    c->persistent_blocks = c->cu_count * 8;
+    Arenas& A = c->A;
+    const int64_t nc = lim->max_chunks;
+    A.cap_chunks = (int32_t)nc;
+    A.cap_samples = lim->max_samples;
+    A.cap_rows = lim->max_samples / 2 + nc;
+    A.cap_vertices = lim->max_vertices;
+    A.cap_indices = lim->max_indices;
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && dalloc(&A.meta, nc) == hipSuccess && dalloc(&A.descs, nc) == hipSuccess &&
+         dalloc(&A.results, nc) == hipSuccess && dalloc(&A.tile_prefix, nc + 1) == hipSuccess &&
+         dalloc(&A.row_prefix, nc + 1) == hipSuccess && dalloc(&A.vtile_prefix, nc + 1) == hipSuccess &&
+         dalloc(&A.ttile_prefix, nc + 1) == hipSuccess && dalloc(&A.dens, A.cap_samples) == hipSuccess &&
+         dalloc(&A.labels, A.cap_samples / 64 + nc) == hipSuccess && dalloc(&A.row_v, A.cap_rows) == hipSuccess &&
+         dalloc(&A.row_t, A.cap_rows) == hipSuccess && dalloc(&A.pos, 3 * A.cap_vertices) == hipSuccess &&
+         dalloc(&A.boundary, A.cap_vertices) == hipSuccess && dalloc(&A.valence, A.cap_vertices) == hipSuccess &&
+         dalloc(&A.adj, (int64_t)BMF_MAX_VALENCE * A.cap_vertices) == hipSuccess &&
+         dalloc(&A.idx, A.cap_indices) == hipSuccess && dalloc(&A.cen, A.cap_indices) == hipSuccess &&
+         dalloc(&A.ctr, 1) == hipSuccess && dalloc(&A.sampler, 1) == hipSuccess;
+    ok = ok && hipHostMalloc((void**)&c->h_descs, sizeof(bmf_chunk_desc) * nc) == hipSuccess &&
+         hipHostMalloc((void**)&c->h_results, sizeof(bmf_chunk_result) * nc) == hipSuccess &&
+         hipHostMalloc((void**)&c->h_ctr, sizeof(Counters)) == hipSuccess;
+    ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
+         hipEventCreate(&c->evs0) == hipSuccess && hipEventCreate(&c->evs1) == hipSuccess;
+    if (ok) {
+        // counters start defined (plan resets them before use)
+        std::memset(c->h_ctr, 0, sizeof(Counters));
+        ok = hipMemcpy(A.ctr, c->h_ctr, sizeof(Counters), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!ok) {
+        bmf_destroy(c);
+        return BMF_EHIP;
+    }
+    *out = c;
+    return BMF_OK;
+}
+
+void bmf_destroy(bmf_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    free_arenas(c);
+    if (c->h_descs) (void)hipHostFree(c->h_descs);
+    if (c->h_results) (void)hipHostFree(c->h_results);
+    if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+    hipEvent_t evs[] = {c->ev0, c->ev1, c->evs0, c->evs1};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int bmf_set_sampler(bmf_ctx* c, const bmf_sampler_desc* s) {
+    if (!c || !s) return BMF_EINVAL;
+    if (s->kind < BMF_SAMPLER_TERRAIN || s->kind > BMF_SAMPLER_CSG) return BMF_EINVAL;
+    SamplerDev d{};
+    d.kind = s->kind;
+    d.world_size = s->world_size;
+    d.amp = s->amp;
+    d.y0 = s->y0;
+    d.seed = s->seed;
+    if (s->kind == BMF_SAMPLER_TERRAIN) {
+        if (s->octaves < 1 || s->octaves > BMF_MAX_OCTAVES) return BMF_EINVAL;
+        if (!finite(s->amp) || !finite(s->y0) || !finite(s->freq) || !finite(s->lacunarity) || !finite(s->gain))
+            return BMF_EINVAL;
+        d.octaves = s->octaves;
+        // per-octave constants: FP32 product chains, normaliser = sum of amplitudes
+        float f = s->freq, a = 1.0f, norm = 0.0f;
+        for (int o = 0; o < s->octaves; o++) {
+            d.fo[o] = f;
+            d.ao[o] = a;
+            norm = norm + a;
+            f = f * s->lacunarity;
+            a = a * s->gain;
+        }
+        if (!(norm > 0.0f) || !finite(norm)) return BMF_EINVAL;
+        d.inv_norm = 1.0f / norm;
+    }
+    BMF_HIP(hipSetDevice(c->device));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    if (c->d_spheres) (void)hipFree(c->d_spheres);
+    if (c->d_boxes) (void)hipFree(c->d_boxes);
+    c->d_spheres = c->d_boxes = nullptr;
+    if (s->kind == BMF_SAMPLER_CSG) {
+        if (s->n_spheres < 0 || s->n_boxes < 0 || (s->n_spheres && !s->spheres) || (s->n_boxes && !s->boxes))
+            return BMF_EINVAL;
+        d.n_spheres = s->n_spheres;
+        d.n_boxes = s->n_boxes;
+        BMF_HIP(dalloc(&c->d_spheres, 4 * (int64_t)s->n_spheres));
+        BMF_HIP(dalloc(&c->d_boxes, 6 * (int64_t)s->n_boxes));
+        if (s->n_spheres)
+            BMF_HIP(hipMemcpy(c->d_spheres, s->spheres, sizeof(float) * 4 * s->n_spheres, hipMemcpyHostToDevice));
+        if (s->n_boxes)
+            BMF_HIP(hipMemcpy(c->d_boxes, s->boxes, sizeof(float) * 6 * s->n_boxes, hipMemcpyHostToDevice));
+        d.spheres = c->d_spheres;
+        d.boxes = c->d_boxes;
+    }
+    c->h_sampler = d;
+    BMF_HIP(hipMemcpy(c->A.sampler, &d, sizeof(SamplerDev), hipMemcpyHostToDevice));
+    c->sampler_set = true;
+    return BMF_OK;
+}
+
+int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
+    if (!c) return BMF_EINVAL;
+    if (!c->sampler_set) return BMF_ESTATE;
+    if (mode != BMF_MODE_PLAIN && mode != BMF_MODE_BATCHED) return BMF_EINVAL;
+    int rc = bmf_validate_chunks(chunks, n, &c->lim);
+    if (rc) return rc;
+    BMF_HIP(hipSetDevice(c->device));
+    // the previous call's results and pinned descriptors must be consumed first
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    c->results_valid = false;
+    c->last_n = n;
+    c->have_extract = true;
+    if (n == 0) return BMF_OK;
+    rc = upload_descs(c, chunks, n);
+    if (rc) return rc;
+    int maxk = 0;
+    for (int i = 0; i < n; i++) maxk = std::max(maxk, chunks[i].iters);
+    const Arenas& A = c->A;
+    const int pb = c->persistent_blocks;
+    c->sample_timed = false;
+    if (mode == BMF_MODE_PLAIN) {
+        // one launch sequence per chunk; scratch arenas reused from offset 0,
+        // output arenas bump-allocated across the chunks (ChunkGenerator.cpp:91
+        // loop body, one chunk at a time)
+        BMF_HIP(hipEventRecord(c->ev0, c->stream));
+        for (int j = 0; j < n; j++) {
+            Arenas Aj = A;
+            Aj.descs = A.descs + j;
+            Aj.meta = A.meta + j;
+            Aj.results = A.results + j;
+            const int64_t dim = chunks[j].dim;
+            const int64_t ns = dim * dim * dim;
+            const int sb = (int)std::min<int64_t>(ceil_div(ns, kSampleTile), pb);
+            const int rb = (int)std::min<int64_t>(ceil_div(dim * dim, kRowThreads / 64), pb);
+            rc = enqueue_pipeline(c, Aj, 1, chunks[j].iters, 1, j == 0, sb, rb, 1, std::min(pb, 1024), false);
+            if (rc) return rc;
+        }
+        BMF_HIP(hipEventRecord(c->ev1, c->stream));
+        return BMF_OK;
+    }
+    // BATCHED: persistent kernels over the whole batch, captured once per
+    // (n, max iterations) in a hipGraph and replayed.
+    GraphEntry* ge = nullptr;
+    for (auto& g : c->graphs)
+        if (g.n == n && g.maxk == maxk) ge = &g;
+    if (!ge) {
+        hipGraph_t graph = nullptr;
+        BMF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_pipeline(c, A, n, maxk, 1, 1, pb, pb, std::min(n, c->cu_count), pb, false);
+        hipGraph_t captured = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &captured);
+        if (rc) {
+            if (captured) (void)hipGraphDestroy(captured);
+            return rc;
+        }
+        if (e != hipSuccess) return BMF_EHIP;
+        graph = captured;
+        GraphEntry g;
+        g.n = n;
+        g.maxk = maxk;
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return BMF_EHIP;
+        c->graphs.push_back(g);
+        ge = &c->graphs.back();
+    }
+    BMF_HIP(hipEventRecord(c->ev0, c->stream));
+    BMF_HIP(hipGraphLaunch(ge->exec, c->stream));
+    BMF_HIP(hipEventRecord(c->ev1, c->stream));
+    return BMF_OK;
+}
+
+int bmf_sync(bmf_ctx* c) {
+    if (!c) return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(c->device));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    return BMF_OK;
+}
+
+static int fetch_results(bmf_ctx* c) {
+    if (c->results_valid) return BMF_OK;
+    if (!c->have_extract) return BMF_ESTATE;
+    const int n = c->last_n;
+    if (n > 0)
+        BMF_HIP(hipMemcpyAsync(c->h_results, c->A.results, sizeof(bmf_chunk_result) * n, hipMemcpyDeviceToHost,
+                               c->stream));
+    BMF_HIP(hipMemcpyAsync(c->h_ctr, c->A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    c->results_valid = true;
+    return BMF_OK;
+}
+
+int bmf_result(bmf_ctx* c, bmf_chunk_result* out, int n, int64_t* need_v, int64_t* need_i) {
+    if (!c) return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(c->device));
+    int rc = fetch_results(c);
+    if (rc) return rc;
+    if (out) {
+        if (n < 0 || n > c->last_n) return BMF_EINVAL;
+        std::memcpy(out, c->h_results, sizeof(bmf_chunk_result) * (size_t)n);
+    }
+    if (need_v) *need_v = (int64_t)c->h_ctr->need_v;
+    if (need_i) *need_i = (int64_t)c->h_ctr->need_i;
+    if (c->h_ctr->error) return BMF_EHIP;
+    return c->h_ctr->overflow ? BMF_ECAPACITY : BMF_OK;
+}
+
+int bmf_mesh_view_get(bmf_ctx* c, bmf_mesh_view* v) {
+    if (!c || !v) return BMF_EINVAL;
+    int rc = fetch_results(c);
+    if (rc) return rc;
+    v->positions = c->A.pos;
+    v->indices = c->A.idx;
+    v->boundary = c->A.boundary;
+    v->total_vertices = std::min<int64_t>((int64_t)c->h_ctr->v_bump, c->A.cap_vertices);
+    v->total_indices = std::min<int64_t>((int64_t)c->h_ctr->i_bump, c->A.cap_indices);
+    return BMF_OK;
+}
+
+int bmf_mesh_to_host(bmf_ctx* c, int chunk, float* positions, uint32_t* indices, uint8_t* boundary, int64_t cap_v,
+                     int64_t cap_i, int64_t* nv, int64_t* ni) {
+    if (!c) return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(c->device));
+    int rc = fetch_results(c);
+    if (rc) return rc;
+    if (chunk < 0 || chunk >= c->last_n) return BMF_EINVAL;
+    const bmf_chunk_result& r = c->h_results[chunk];
+    if (nv) *nv = r.vertex_count;
+    if (ni) *ni = r.index_count;
+    if (r.status != BMF_OK) return r.status;
+    if (cap_v < r.vertex_count || cap_i < r.index_count) return BMF_ECAPACITY;
+    if (r.vertex_count) {
+        if (positions)
+            BMF_HIP(hipMemcpyAsync(positions, c->A.pos + 3 * r.vertex_offset, sizeof(float) * 3 * r.vertex_count,
+                                   hipMemcpyDeviceToHost, c->stream));
+        if (boundary)
+            BMF_HIP(hipMemcpyAsync(boundary, c->A.boundary + r.vertex_offset, r.vertex_count, hipMemcpyDeviceToHost,
+                                   c->stream));
+    }
+    if (r.index_count && indices)
+        BMF_HIP(hipMemcpyAsync(indices, c->A.idx + r.index_offset, sizeof(uint32_t) * r.index_count,
+                               hipMemcpyDeviceToHost, c->stream));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    return BMF_OK;
+}
+
+// Runs plan + sample (+ count/scan for the flag) for one chunk with culling
+// off, so the density block and labels are materialised.
+static int stage_one(bmf_ctx* c, const bmf_chunk_desc* d, bool scan) {
+    if (!c || !d) return BMF_EINVAL;
+    if (!c->sampler_set) return BMF_ESTATE;
+    int rc = check_desc(*d);
+    if (rc) return rc;
+    const int64_t ns = (int64_t)d->dim * d->dim * d->dim;
+    if (ns > c->A.cap_samples) return BMF_ECAPACITY;
+    BMF_HIP(hipSetDevice(c->device));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    c->have_extract = false;
+    c->results_valid = false;
+    rc = upload_descs(c, d, 1);
+    if (rc) return rc;
+    const Arenas& A = c->A;
+    BMF_HIP(launch_plan(A, 1, 0, 1, c->stream));
+    BMF_HIP(launch_sample(A, 1, (int)std::min<int64_t>(ceil_div(ns, kSampleTile), c->persistent_blocks), c->stream));
+    if (scan) {
+        const int rb = (int)std::min<int64_t>(ceil_div((int64_t)d->dim * d->dim, 4), c->persistent_blocks);
+        BMF_HIP(launch_count(A, 1, rb, c->stream));
+        BMF_HIP(launch_scan(A, 1, 1, c->stream));
+    }
+    return BMF_OK;
+}
+
+int bmf_sample_block(bmf_ctx* c, const bmf_chunk_desc* d, float* dens) {
+    int rc = stage_one(c, d, false);
+    if (rc) return rc;
+    if (!dens) return BMF_EINVAL;
+    const int64_t ns = (int64_t)d->dim * d->dim * d->dim;
+    BMF_HIP(hipMemcpyAsync(dens, c->A.dens, sizeof(float) * ns, hipMemcpyDeviceToHost, c->stream));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    return BMF_OK;
+}
+
+int bmf_label_block(bmf_ctx* c, const bmf_chunk_desc* d, uint32_t* words, int32_t* contains) {
+    int rc = stage_one(c, d, true);
+    if (rc) return rc;
+    if (!words) return BMF_EINVAL;
+    const int64_t nw = (int64_t)d->dim * d->dim * ((d->dim + 31) / 32);
+    uint32_t* tmp = nullptr;
+    BMF_HIP(dalloc(&tmp, nw));
+    hipError_t e = launch_ref_labels(c->A, tmp, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(words, tmp, sizeof(uint32_t) * nw, hipMemcpyDeviceToHost, c->stream);
+    ChunkMeta m{};
+    if (e == hipSuccess) e = hipMemcpyAsync(&m, c->A.meta, sizeof(ChunkMeta), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return BMF_EHIP;
+    if (contains) *contains = m.contains_mesh;
+    return BMF_OK;
+}
+
+int bmf_cell_masks(bmf_ctx* c, const bmf_chunk_desc* d, uint64_t* masks) {
+    int rc = stage_one(c, d, false);
+    if (rc) return rc;
+    if (!masks) return BMF_EINVAL;
+    const int64_t nw = (int64_t)d->dim * d->dim * ((d->dim + 7) / 8);
+    uint64_t* tmp = nullptr;
+    BMF_HIP(dalloc(&tmp, nw));
+    hipError_t e = launch_ref_masks(c->A, tmp, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(masks, tmp, sizeof(uint64_t) * nw, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    return e == hipSuccess ? BMF_OK : BMF_EHIP;
+}
+
+int bmf_last_timing(bmf_ctx* c, float* total_ms, float* sample_ms) {
+    if (!c) return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(c->device));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    if (total_ms) {
+        float ms = 0.0f;
+        BMF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *total_ms = ms;
+    }
+    if (sample_ms) *sample_ms = -1.0f;
+    return BMF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// Device density samplers.  FP32 with an exact, fixed operation sequence:
+// the library is compiled with -ffp-contract=off and every fused multiply-add
+// below is an explicit __builtin_fmaf, so the values are reproducible on any
+// IEEE FP32 implementation that follows the same sequence.
+//
+// Terrain: d = amp * fbm(p) - (p.y - y0), the sampler family of
+// NoiseSamplers::terrain3d_block (NoiseSampler.cpp:196-227) with the noise
+// being the project's own FBM of 3D simplex noise (the reference calls
+// FastNoiseSIMD there, which is not part of the reference tree).
+//
+// Simplex noise after S. Gustavson, "Simplex noise demystified" (2005):
+// skew 1/3, unskew 1/6, kernel (0.5 - |d|^2)^4, gradient set and selection of
+// K. Perlin, "Improving Noise" (2002), a lattice hash of 24-bit multiplies
+// and xor-shifts (v_mul_u32_u24 runs at full rate), output x76.8 clamped to
+// [-1, 1].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bmf_internal.h"
+
+namespace bmf {
+
+constexpr uint32_t kP1 = 0x5BD1E9u, kP2 = 0x27D4EBu, kP3 = 0x165667u;
+constexpr uint32_t kM1 = 0x9E3779u, kM2 = 0x85EBCBu;
+constexpr float kNoiseScale = 76.8f;
+constexpr float kNoiseR2 = 0.5f;
+constexpr float kCsgFloor = -16.0f;
+
+__device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) { return __umul24(a, b); }
+
+__device__ __forceinline__ uint32_t grad_index(int32_t i, int32_t j, int32_t k, uint32_t seed) {
+    uint32_t h = mul24((uint32_t)i, kP1) ^ mul24((uint32_t)j, kP2) ^ mul24((uint32_t)k, kP3) ^ seed;
+    h ^= h >> 13;
+    h = mul24(h ^ (h >> 24), kM1);
+    h ^= h >> 11;
+    h = mul24(h ^ (h >> 24), kM2);
+    h ^= h >> 16;
+    return (h >> 4) & 15u;
+}
+
+__device__ __forceinline__ float grad_dot(uint32_t h, float x, float y, float z) {
+    float u = h < 8u ? x : y;
+    float v = h < 4u ? y : ((h == 12u || h == 14u) ? x : z);
+    float su = (h & 1u) ? -u : u;
+    float sv = (h & 2u) ? -v : v;
+    return su + sv;
+}
+
+__device__ __forceinline__ float corner(float x, float y, float z, uint32_t h) {
+    float t = __builtin_fmaf(-x, x, kNoiseR2);
+    t = __builtin_fmaf(-y, y, t);
+    t = __builtin_fmaf(-z, z, t);
+    t = fmaxf(t, 0.0f);
+    t = t * t;
+    return (t * t) * grad_dot(h, x, y, z);
+}
+
+__device__ __forceinline__ float snoise3(float x, float y, float z, uint32_t seed) {
+    const float F3 = 1.0f / 3.0f;
+    const float G3 = 1.0f / 6.0f;
+    const float G3x2 = 2.0f * G3;
+    float s = ((x + y) + z) * F3;
+    float fi = floorf(x + s), fj = floorf(y + s), fk = floorf(z + s);
+    int32_t i = (int32_t)fi, j = (int32_t)fj, k = (int32_t)fk;
+    float t = (float)(i + j + k) * G3;
+    float x0 = x - (fi - t), y0 = y - (fj - t), z0 = z - (fk - t);
+    bool gxy = x0 >= y0, gyz = y0 >= z0, gxz = x0 >= z0;
+    int i1 = gxy & gxz, j1 = (!gxy) & gyz, k1 = (!gxz) & (!gyz);
+    int i2 = gxy | gxz, j2 = (!gxy) | gyz, k2 = (!gxz) | (!gyz);
+    float x1 = (x0 - (float)i1) + G3, y1 = (y0 - (float)j1) + G3, z1 = (z0 - (float)k1) + G3;
+    float x2 = (x0 - (float)i2) + G3x2, y2 = (y0 - (float)j2) + G3x2, z2 = (z0 - (float)k2) + G3x2;
+    float x3 = (x0 - 1.0f) + 0.5f, y3 = (y0 - 1.0f) + 0.5f, z3 = (z0 - 1.0f) + 0.5f;
+    float n0 = corner(x0, y0, z0, grad_index(i, j, k, seed));
+    float n1 = corner(x1, y1, z1, grad_index(i + i1, j + j1, k + k1, seed));
+    float n2 = corner(x2, y2, z2, grad_index(i + i2, j + j2, k + k2, seed));
+    float n3 = corner(x3, y3, z3, grad_index(i + 1, j + 1, k + 1, seed));
+    float n = ((n0 + n1) + n2) + n3;
+    float r = kNoiseScale * n;
+    return fminf(fmaxf(r, -1.0f), 1.0f);
+}
+
+// FP32 op count of snoise3 as written (add/sub/mul/min/max/floor/convert
+// and selects on float data = 1, fma = 2); integer hash work excluded.
+// skew 4 + floor 6 + convert 4 + unskew 7 + offsets 18 + 4 corners x 14
+// (3 fma = 6, max, 2 mul, mul, grad add + 2 negates) + sum 3 + scale/clamp 3.
+constexpr int kSimplexFlops = 101;
+// Per octave outside snoise3: 3 coordinate multiplies + the fma accumulate.
+constexpr int kOctaveFlops = 5;
+// Per sample outside the octave loop: 2 coordinate ops x 3 axes, the
+// normalise, clamp (2), amp multiply, y - y0 and the final subtract.
+constexpr int kTerrainFlops = 12;
+
+__device__ __forceinline__ float terrain_density(const SamplerDev& S, float x, float y, float z) {
+    float acc = 0.0f;
+    for (int o = 0; o < S.octaves; o++) {
+        const float f = S.fo[o];
+        float n = snoise3(x * f, y * f, z * f, S.seed + (uint32_t)o);
+        acc = __builtin_fmaf(S.ao[o], n, acc);
+    }
+    float fb = acc * S.inv_norm;
+    fb = fminf(fmaxf(fb, -1.0f), 1.0f);
+    float t1 = S.amp * fb;
+    float t2 = y - S.y0;
+    return t1 - t2;
+}
+
+// ImplicitFunctions (ImplicitSampler.cpp:27-65), positive inside.
+__device__ __forceinline__ float implicit_density(int kind, float res, float x, float y, float z) {
+    switch (kind) {
+        case BMF_SAMPLER_SPHERE: {
+            float r = res * 0.25f;
+            float l = sqrtf((x * x + y * y) + z * z);
+            return -(l - r);
+        }
+        case BMF_SAMPLER_CUBOID: {
+            float r = res / 8.0f;
+            float dx = fabsf(x) - r, dy = fabsf(y) - r, dz = fabsf(z) - r;
+            float m = fmaxf(dx, fmaxf(dy, dz));
+            float l = sqrtf((dx * dx + dy * dy) + dz * dz);
+            return -fminf(m, l);
+        }
+        case BMF_SAMPLER_TORUS_Z: {
+            float r1 = res / 4.0f, r2 = res / 10.0f;
+            float qx = fabsf(sqrtf(x * x + y * y)) - r1;
+            float len = sqrtf(qx * qx + z * z);
+            return -(len - r2);
+        }
+        case BMF_SAMPLER_PLANE_Y:
+            return -y;
+    }
+    return 0.0f;
+}
+
+// CSG union of spheres r - |p - c| and exact boxes -sdf, floored.
+__device__ __forceinline__ float csg_density(const SamplerDev& S, float x, float y, float z) {
+    float best = kCsgFloor;
+    for (int i = 0; i < S.n_spheres; i++) {
+        const float* sp = S.spheres + 4 * i;
+        float dx = x - sp[0], dy = y - sp[1], dz = z - sp[2];
+        float l = sqrtf((dx * dx + dy * dy) + dz * dz);
+        best = fmaxf(best, sp[3] - l);
+    }
+    for (int i = 0; i < S.n_boxes; i++) {
+        const float* b = S.boxes + 6 * i;
+        float px = x - b[0], py = y - b[1], pz = z - b[2];
+        float qx = fabsf(px) - b[3], qy = fabsf(py) - b[4], qz = fabsf(pz) - b[5];
+        float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+        float outside = sqrtf((mx * mx + my * my) + mz * mz);
+        float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+        best = fmaxf(best, -(outside + inside));
+    }
+    return best;
+}
+
+__device__ __forceinline__ float sample_density(const SamplerDev& S, float x, float y, float z) {
+    if (S.kind == BMF_SAMPLER_TERRAIN) return terrain_density(S, x, y, z);
+    if (S.kind == BMF_SAMPLER_CSG) return csg_density(S, x, y, z);
+    return implicit_density(S.kind, S.world_size, x, y, z);
+}
+
+}  // namespace bmf

@@ -33,8 +33,8 @@ for (dx, dy, dz, a) in LOCAL_EDGE:
 
 def _parse(name):
     text = _HEADER.read_text()
-    body = text[text.index(name):]
-    body = body[body.index("{") + 1: body.index("};")]
+    body = text[text.index("#define " + name):]
+    body = body[body.index("{") + 1: body.index("}")]
     return [int(v, 16) for v in re.findall(r"0x([0-9a-fA-F]+)ull", body)]
 
 
@@ -58,8 +58,8 @@ def _decode_dmc(v):
     return polys
 
 
-TRI_PACKED = _parse("bmf_tri_table")
-DMC_PACKED = _parse("bmf_dmc_poly_table")
+TRI_PACKED = _parse("BMF_TRI_TABLE_INIT")
+DMC_PACKED = _parse("BMF_DMC_POLY_TABLE_INIT")
 assert len(TRI_PACKED) == 256 and len(DMC_PACKED) == 256
 TRI_TABLE = [_decode_tri(v) for v in TRI_PACKED]
 DMC_POLYS = [_decode_dmc(v) for v in DMC_PACKED]

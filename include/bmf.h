@@ -1,0 +1,166 @@
+/*
+ * bmf.h -- C ABI of the MI355X chunk-extraction library (libbmf.so).
+ *
+ * Drop-in boundary for the per-chunk extraction path of BinaryMeshFitting:
+ * the reference's ChunkGenerator::process_queue (ChunkGenerator.cpp:27-60)
+ * runs ChunkGenerator::extract_chunk (ChunkGenerator.cpp:80-147) over a batch
+ * of chunks: DMCChunk::label_grid (DMCChunk.cpp:79-166) -> label_edges
+ * (:168-508) -> polygonize (:514-576) -> MeshProcessor<3> smoothing
+ * (MeshProcessor.cpp:25-306).  Here one bmf_extract call takes the batch's
+ * chunk descriptors and runs that whole path on the GPU.
+ *
+ * Conventions (same as the reference):
+ *   - dim = samples per axis; cells emitted = (dim-1)^3; 2 <= dim <= 256.
+ *   - sample (i,j,k) sits at overlap_pos + (i,j,k)*delta, with
+ *     delta = size*(1+2*overlap)/(dim-1), overlap_pos = pos - size*overlap.
+ *   - a sample is "air" (label bit set) when its density is < 0; densities
+ *     are positive inside the solid.
+ *   - vertex positions are in grid units (world = p*delta + overlap_pos);
+ *     indices are chunk-local triangle lists (3 per triangle).
+ *
+ * All functions return BMF_OK (0) or a positive BMF_E* code; nothing throws
+ * or aborts across the boundary.  A context is thread-compatible, not
+ * thread-safe (one per host thread, like the reference's single watcher
+ * thread).  Work is ordered on the context's HIP stream.
+ */
+#ifndef BMF_H
+#define BMF_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMF_OK 0
+#define BMF_EINVAL 1       /* bad argument or descriptor */
+#define BMF_ECAPACITY 2    /* an arena was too small; bmf_result reports what was needed */
+#define BMF_EHIP 3         /* HIP runtime error */
+#define BMF_ENODEV 4       /* no usable GPU */
+#define BMF_ESTATE 5       /* call out of order (e.g. mesh access before extraction) */
+
+#define BMF_MAX_DIM 256
+#define BMF_MAX_OCTAVES 16
+#define BMF_MAX_VALENCE 20 /* 4 cells around an edge x at most 5 uses per cell */
+
+typedef struct bmf_ctx bmf_ctx;
+
+/* Arena sizes fixed at creation (the reference's ResourceAllocator pools,
+ * ResourceAllocator.hpp:34-62, become preallocated device arenas). */
+typedef struct {
+    int32_t max_chunks;     /* descriptors per bmf_extract call */
+    int64_t max_samples;    /* sum of dim^3 over the sampled chunks of one call */
+    int64_t max_vertices;   /* output vertices over all chunks of one call */
+    int64_t max_indices;    /* output indices over all chunks of one call */
+} bmf_limits;
+
+/* Samplers (Sampler.hpp:24-34 and its create_sampler_* factories). */
+enum {
+    BMF_SAMPLER_TERRAIN = 0, /* d = amp*fbm(p) - (p.y - y0); fbm = FBM of 3D simplex noise */
+    BMF_SAMPLER_SPHERE = 1,  /* ImplicitFunctions::sphere, ImplicitSampler.cpp:46-50 */
+    BMF_SAMPLER_CUBOID = 2,  /* ImplicitFunctions::cuboid, ImplicitSampler.cpp:52-60 */
+    BMF_SAMPLER_TORUS_Z = 3, /* ImplicitFunctions::torus_z, ImplicitSampler.cpp:27-34 */
+    BMF_SAMPLER_PLANE_Y = 4, /* ImplicitFunctions::plane_y, ImplicitSampler.cpp:62-65 */
+    BMF_SAMPLER_CSG = 5      /* max-union of spheres and boxes, floored at -16 */
+};
+
+typedef struct {
+    int32_t kind;
+    float world_size;                 /* Sampler::world_size (implicit radii derive from it) */
+    float amp, y0;                    /* terrain */
+    float freq, lacunarity, gain;     /* terrain FBM */
+    int32_t octaves;                  /* 1..BMF_MAX_OCTAVES */
+    uint32_t seed;
+    const float* spheres;             /* CSG, host memory: n_spheres x (cx, cy, cz, r) */
+    int32_t n_spheres;
+    const float* boxes;               /* CSG, host memory: n_boxes x (cx, cy, cz, hx, hy, hz) */
+    int32_t n_boxes;
+} bmf_sampler_desc;
+
+/* One chunk: DMCChunk fields (DMCChunk.hpp:24-60) plus the per-chunk
+ * WorldProperties the generator applies (ChunkGenerator.cpp:85-98). */
+typedef struct {
+    float pos[3];                 /* DMCChunk::pos */
+    float size;                   /* DMCChunk::size */
+    int32_t dim;                  /* DMCChunk::dim (samples per axis) */
+    float overlap;                /* result of the overlap rule, ChunkGenerator.cpp:98 */
+    int32_t iters;                /* smoothing iterations K (process_iters) */
+    int32_t boundary_processing;  /* WorldProperties::boundary_processing */
+} bmf_chunk_desc;
+
+enum {
+    BMF_MODE_PLAIN = 0,   /* one launch sequence per chunk, host loop over chunks */
+    BMF_MODE_BATCHED = 1  /* whole batch: persistent kernels + device work queue, captured in a hipGraph */
+};
+
+/* Per-chunk result (after bmf_result). */
+typedef struct {
+    int64_t vertex_offset;   /* first vertex of the chunk in the view's arrays */
+    int64_t index_offset;    /* first index of the chunk */
+    int32_t vertex_count;
+    int32_t index_count;     /* 3 x triangles */
+    int32_t contains_mesh;   /* DMCChunk::contains_mesh (DMCChunk.cpp:159-162) */
+    int32_t status;          /* BMF_OK, or BMF_ECAPACITY when the chunk did not fit */
+    float delta;             /* DMCChunk::scale */
+    float overlap_pos[3];    /* DMCChunk::overlap_pos */
+} bmf_chunk_result;
+
+/* Device pointers into the context's output arenas; valid until the next
+ * bmf_extract on the context. */
+typedef struct {
+    const float* positions;  /* [vertices][3], grid units */
+    const uint32_t* indices; /* chunk-local vertex ids */
+    const uint8_t* boundary; /* DualVertex::boundary (DMCChunk.cpp:673) */
+    int64_t total_vertices;
+    int64_t total_indices;
+} bmf_mesh_view;
+
+/* ---- lifetime ---------------------------------------------------------- */
+int bmf_create(bmf_ctx** out, int device, const bmf_limits* limits);
+void bmf_destroy(bmf_ctx* ctx);
+const char* bmf_strerror(int code);
+const char* bmf_version(void);
+
+/* Replaces Sampler::block + NoiseSamplerProperties (Sampler.hpp:24-34,
+ * NoiseSampler.hpp:7-22); the sampler applies to later extractions. */
+int bmf_set_sampler(bmf_ctx* ctx, const bmf_sampler_desc* sampler);
+
+/* Checks descriptors on the host only (no GPU needed). */
+int bmf_validate_chunks(const bmf_chunk_desc* chunks, int n, const bmf_limits* limits);
+
+/* ---- the path ---------------------------------------------------------- */
+/* ChunkGenerator::process_queue -> extract_chunk over n chunks (host
+ * descriptor array).  Enqueues work on the context stream and returns. */
+int bmf_extract(bmf_ctx* ctx, const bmf_chunk_desc* chunks, int n, int mode);
+
+/* Waits for the last bmf_extract and copies per-chunk results to `out`
+ * (n entries, may be NULL).  Returns BMF_ECAPACITY if any chunk did not fit;
+ * `need_vertices/need_indices` (may be NULL) then hold the totals needed. */
+int bmf_result(bmf_ctx* ctx, bmf_chunk_result* out, int n, int64_t* need_vertices, int64_t* need_indices);
+
+int bmf_mesh_view_get(bmf_ctx* ctx, bmf_mesh_view* view);
+
+/* Copies one chunk's mesh to host memory (the VerticesIndicesBlock of
+ * ChunkBlocks.hpp:161-179, as SoA).  Returns BMF_ECAPACITY if cap_* is too
+ * small (counts are still written). */
+int bmf_mesh_to_host(bmf_ctx* ctx, int chunk, float* positions, uint32_t* indices, uint8_t* boundary,
+                     int64_t cap_vertices, int64_t cap_indices, int64_t* vertices, int64_t* indices_out);
+
+int bmf_sync(bmf_ctx* ctx);
+
+/* Stage inspection for one chunk, to host memory.  Densities: dim^3 FP32
+ * [x][y][z] (Sampler::block output).  Labels: the reference's u32 words
+ * [x][y][ceil(dim/32)] of DMCChunk::label_grid.  Masks: the reference's
+ * u64 cell-mask words [x][y][ceil(dim/8)] of DMCChunk::label_edges. */
+int bmf_sample_block(bmf_ctx* ctx, const bmf_chunk_desc* chunk, float* densities);
+int bmf_label_block(bmf_ctx* ctx, const bmf_chunk_desc* chunk, uint32_t* words, int32_t* contains_mesh);
+int bmf_cell_masks(bmf_ctx* ctx, const bmf_chunk_desc* chunk, uint64_t* masks);
+
+/* Timing of the last bmf_extract on the device (HIP events, ms), and of
+ * the sampling kernel inside it. */
+int bmf_last_timing(bmf_ctx* ctx, float* total_ms, float* sample_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

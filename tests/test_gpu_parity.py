@@ -1,0 +1,232 @@
+"""GPU path vs the CPU oracle, through the C ABI.
+
+Bar: topology (labels, cell masks, vertex ids, triangle lists, boundary
+flags, contains_mesh) bit-exact; densities and positions within 1e-5 (the
+kernels follow the oracle's FP32 operation sequence, so they are in fact
+expected to match bit for bit, and the tests report the max difference).
+"""
+import numpy as np
+import pytest
+
+from binarymeshfitting_amd import capi, workloads
+
+pytestmark = pytest.mark.gpu
+
+TERRAIN = workloads.terrain_sampler()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with capi.Context(max_chunks=2048, max_samples=40 * 65 ** 3, max_vertices=1 << 22, max_indices=1 << 24) as c:
+        yield c
+
+
+def _ospec(oracle, sampler):
+    return oracle.SamplerSpec(**sampler)
+
+
+def _ochunk(oracle, c):
+    return oracle.make_chunk(c.pos, c.size, c.dim, c.overlap, c.iters, c.boundary_processing)
+
+
+def _check_mesh(oracle, spec, c, pos, idx, bnd, res=None, tol=1e-5):
+    ref = oracle.extract_chunk(spec, _ochunk(oracle, c))
+    if res is not None:
+        assert bool(res.contains_mesh) == ref["contains_mesh"], c
+        assert res.vertex_count == ref["V"] and res.index_count == ref["nI"], c
+    assert np.array_equal(idx, ref["idx"]), f"triangles differ for {c}"
+    assert np.array_equal(bnd, ref["boundary"]), f"boundary flags differ for {c}"
+    if len(pos):
+        err = np.abs(pos.astype(np.float64) - ref["pos"]).max()
+        assert err <= tol * (c.dim - 1), f"positions differ by {err} for {c}"
+        return err
+    return 0.0
+
+
+CHUNKS = [
+    workloads.Chunk((0.0, 0.0, 0.0), 64.0, 65, 0.0, 4, 0),        # config-1 chunk
+    workloads.Chunk((64.0, 0.0, -64.0), 64.0, 65, 0.0, 0, 0),     # no smoothing
+    workloads.Chunk((10.0, 7.5, 3.0), 32.0, 33, 0.055, 4, 1),     # overlap + boundary processing
+    workloads.Chunk((-40.0, 0.0, 5.0), 128.0, 129, 0.0, 8, 0),    # 128 cells, spacing 1
+    workloads.Chunk((0.0, 16.0, 0.0), 40.0, 41, 0.0, 2, 0),       # dim % 8 != 1
+    workloads.Chunk((3.0, 20.0, 1.0), 16.0, 17, 0.0, 3, 0),
+    workloads.Chunk((5.0, 30.0, 9.0), 1.0, 2, 0.0, 1, 0),         # a single cell
+    workloads.Chunk((0.0, 0.0, 0.0), 200.0, 101, 0.0, 1, 0),      # dim 101, spacing 2
+]
+
+
+@pytest.mark.parametrize("c", CHUNKS[:6], ids=lambda c: f"dim{c.dim}")
+def test_sample_block_matches_oracle(ctx, oracle, c):
+    ctx.set_sampler(**TERRAIN)
+    d = ctx.sample_block(workloads.to_capi([c])[0])
+    spec = _ospec(oracle, TERRAIN)
+    delta, opos = oracle.geometry(_ochunk(oracle, c))
+    ref = oracle.sample_block(spec, c.dim, opos, delta)
+    diff = np.abs(d.astype(np.float64) - ref).max()
+    assert diff <= 1e-5, diff
+    # labels (d < 0) identical
+    assert np.array_equal(d < 0, ref < 0)
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3, 4])
+def test_implicit_sample_block(ctx, oracle, kind):
+    s = dict(kind=kind, world_size=48.0)
+    ctx.set_sampler(**s)
+    c = workloads.Chunk((-20.0, -19.0, -21.5), 40.0, 41, 0.0, 0, 0)
+    d = ctx.sample_block(workloads.to_capi([c])[0])
+    delta, opos = oracle.geometry(_ochunk(oracle, c))
+    ref = oracle.sample_block(oracle.SamplerSpec(**s), c.dim, opos, delta)
+    assert np.abs(d - ref).max() <= 1e-5
+
+
+@pytest.mark.parametrize("c", CHUNKS, ids=lambda c: f"dim{c.dim}")
+def test_labels_and_masks_match_oracle(ctx, oracle, c):
+    ctx.set_sampler(**TERRAIN)
+    cd = workloads.to_capi([c])[0]
+    words, cm = ctx.label_block(cd)
+    masks = ctx.cell_masks(cd)
+    spec = _ospec(oracle, TERRAIN)
+    delta, opos = oracle.geometry(_ochunk(oracle, c))
+    dens = oracle.sample_block(spec, c.dim, opos, delta)
+    rw, rcm = oracle.label_grid(dens)
+    assert np.array_equal(words, rw)
+    assert cm == rcm
+    assert np.array_equal(masks, oracle.cell_masks(rw, c.dim))
+
+
+@pytest.mark.parametrize("mode", [capi.MODE_PLAIN, capi.MODE_BATCHED], ids=["plain", "batched"])
+def test_extract_matches_oracle(ctx, oracle, mode):
+    ctx.set_sampler(**TERRAIN)
+    ctx.extract(workloads.to_capi(CHUNKS), mode)
+    rc, res, _, _ = ctx.results(len(CHUNKS))
+    assert rc == capi.OK
+    spec = _ospec(oracle, TERRAIN)
+    worst = 0.0
+    for j, c in enumerate(CHUNKS):
+        pos, idx, bnd = ctx.mesh(j)
+        worst = max(worst, _check_mesh(oracle, spec, c, pos, idx, bnd, res[j]))
+    print(f"max position difference {worst:.3g}")
+
+
+def test_culled_chunks_match_oracle(ctx, oracle):
+    """Chunks entirely above / below the terrain band are classified without
+    sampling; their flags must equal the oracle's full evaluation."""
+    ctx.set_sampler(**TERRAIN)
+    cs = [workloads.Chunk((0.0, 64.0, 0.0), 64.0, 65, 0.0, 4, 0),    # air, dim % 32 != 0 -> contains_mesh
+          workloads.Chunk((0.0, -64.0, 0.0), 64.0, 65, 0.0, 4, 0),   # solid
+          workloads.Chunk((0.0, 64.0, 0.0), 31.0, 32, 0.0, 0, 0),    # air, dim % 32 == 0
+          workloads.Chunk((0.0, 57.0, 0.0), 8.0, 9, 0.0, 0, 0)]      # just above the band
+    ctx.extract(workloads.to_capi(cs), capi.MODE_BATCHED)
+    rc, res, _, _ = ctx.results(len(cs))
+    assert rc == capi.OK
+    spec = _ospec(oracle, TERRAIN)
+    for j, c in enumerate(cs):
+        ref = oracle.extract_chunk(spec, _ochunk(oracle, c))
+        assert bool(res[j].contains_mesh) == ref["contains_mesh"]
+        assert res[j].vertex_count == 0 and ref["V"] == 0
+
+
+def test_uniform_grid_batch(ctx, oracle):
+    """A 4x4x4 slice of the 512-chunk uniform world (config 2 layout)."""
+    ctx.set_sampler(**TERRAIN)
+    wl = workloads.uniform_grid(n_side=4, c=64, iters=4)
+    ctx.extract(workloads.to_capi(wl.chunks), capi.MODE_BATCHED)
+    rc, res, _, _ = ctx.results(len(wl.chunks))
+    assert rc == capi.OK
+    spec = _ospec(oracle, TERRAIN)
+    nonempty = 0
+    for j, c in enumerate(wl.chunks):
+        if c.pos[1] != 0.0:
+            ref = oracle.extract_chunk(spec, _ochunk(oracle, c))
+            assert res[j].vertex_count == ref["V"] == 0
+            assert bool(res[j].contains_mesh) == ref["contains_mesh"]
+            continue
+        pos, idx, bnd = ctx.mesh(j)
+        _check_mesh(oracle, spec, c, pos, idx, bnd, res[j])
+        nonempty += 1
+    assert nonempty == 16
+
+
+def test_graph_replays_are_identical(ctx):
+    ctx.set_sampler(**TERRAIN)
+    wl = workloads.uniform_grid(n_side=2, c=64, iters=3)
+    cd = workloads.to_capi(wl.chunks)
+    meshes = []
+    for rep in range(4):
+        ctx.extract(cd, capi.MODE_BATCHED)
+        rc, res, _, _ = ctx.results(len(cd))
+        assert rc == capi.OK
+        meshes.append([ctx.mesh(j) for j in range(len(cd))])
+    for rep in range(1, 4):
+        for j in range(len(cd)):
+            for a, b in zip(meshes[0][j], meshes[rep][j]):
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def test_plain_equals_batched_mixed_iters(ctx):
+    ctx.set_sampler(**TERRAIN)
+    cs = [workloads.Chunk((float(33 * i), 0.0, 0.0), 32.0, 33, 0.0, i % 9, 0) for i in range(12)]
+    cd = workloads.to_capi(cs)
+    out = {}
+    for mode in (capi.MODE_PLAIN, capi.MODE_BATCHED):
+        ctx.extract(cd, mode)
+        rc, res, _, _ = ctx.results(len(cd))
+        assert rc == capi.OK
+        out[mode] = [ctx.mesh(j) for j in range(len(cd))]
+    for j in range(len(cd)):
+        for a, b in zip(out[capi.MODE_PLAIN][j], out[capi.MODE_BATCHED][j]):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def test_sphere_sampler_closed_surface(ctx, oracle):
+    s = dict(kind=1, world_size=48.0)
+    ctx.set_sampler(**s)
+    c = workloads.Chunk((-16.0, -16.0, -16.0), 32.0, 33, 0.0, 2, 0)
+    ctx.extract(workloads.to_capi([c]), capi.MODE_BATCHED)
+    rc, res, _, _ = ctx.results(1)
+    assert rc == capi.OK
+    pos, idx, bnd = ctx.mesh(0)
+    _check_mesh(oracle, oracle.SamplerSpec(**s), c, pos, idx, bnd, res[0])
+    edges = {}
+    for t in idx.reshape(-1, 3):
+        for k in range(3):
+            a, b = sorted((int(t[k]), int(t[(k + 1) % 3])))
+            edges[(a, b)] = edges.get((a, b), 0) + 1
+    assert all(v == 2 for v in edges.values())
+    assert len(pos) - len(edges) + len(idx) // 3 == 2
+
+
+def test_capacity_overflow_reports_needs():
+    with capi.Context(max_chunks=8, max_samples=4 * 65 ** 3, max_vertices=1000, max_indices=3000) as small:
+        small.set_sampler(**TERRAIN)
+        c = workloads.Chunk((0.0, 0.0, 0.0), 64.0, 65, 0.0, 2, 0)
+        small.extract(workloads.to_capi([c]), capi.MODE_BATCHED)
+        rc, res, need_v, need_i = small.results(1)
+        assert rc == capi.ECAPACITY
+        assert res[0].status == capi.ECAPACITY
+        assert need_v > 1000 and need_i > 3000
+    with capi.Context(max_chunks=8, max_samples=4 * 65 ** 3, max_vertices=need_v, max_indices=need_i) as fit:
+        fit.set_sampler(**TERRAIN)
+        fit.extract(workloads.to_capi([c]), capi.MODE_BATCHED)
+        rc, res, _, _ = fit.results(1)
+        assert rc == capi.OK and res[0].vertex_count == need_v
+
+
+def test_sample_arena_overflow():
+    with capi.Context(max_chunks=8, max_samples=65 ** 3, max_vertices=1 << 20, max_indices=1 << 22) as small:
+        small.set_sampler(**TERRAIN)
+        cs = [workloads.Chunk((0.0, 0.0, 0.0), 64.0, 65), workloads.Chunk((64.0, 0.0, 0.0), 64.0, 65)]
+        small.extract(workloads.to_capi(cs), capi.MODE_BATCHED)
+        rc, res, _, _ = small.results(2)
+        assert rc == capi.ECAPACITY
+        assert all(r.status == capi.ECAPACITY for r in res)
+
+
+def test_invalid_descriptors_rejected(ctx):
+    ctx.set_sampler(**TERRAIN)
+    for bad in [workloads.Chunk((0.0, 0.0, 0.0), 64.0, 1), workloads.Chunk((0.0, 0.0, 0.0), 64.0, 300),
+                workloads.Chunk((0.0, 0.0, 0.0), -1.0, 33), workloads.Chunk((0.0, 0.0, 0.0), 8.0, 9, 0.0, -1)]:
+        assert ctx.extract_raw(capi.chunk_array(workloads.to_capi([bad])), 1) == capi.EINVAL
+    # the context still works afterwards
+    ctx.extract(workloads.to_capi([CHUNKS[5]]), capi.MODE_BATCHED)
+    assert ctx.results(1)[0] == capi.OK
