@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Throughput of the chunk-extraction path on one MI355X.
+
+Workload (default): 512 chunks of 64^3 cells tiled 8x8x8 over the terrain
+d = 24*fbm - (y - 32) (FBM of 3D simplex noise, 5 octaves, base frequency
+0.02, lacunarity 2, gain 0.5, seed 1337), FP32, marching-cubes triangles
+plus 4 dual/primal smoothing iterations; the whole batch is one BATCHED
+extraction (persistent kernels, device work queue, hipGraph replay).  A step
+is one extraction of the batch; inputs (the descriptors) are uploaded inside
+the step, outputs stay in HBM.
+
+Printed JSON line: value = chunks/s of the batched path; beside it the plain
+per-chunk GPU path, the CPU oracle (multithreaded, the reference's algorithm)
+on a bounded sample, the roofline of the dominant kernel (sampling, FP32
+VALU bound) from HIP events on the library stream, and a parity check of
+every chunk's mesh against the CPU run.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "64^3-cell chunks meshed/sec, batched vs plain GPU vs CPU, at fixed tolerance"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path leg")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU, "
+              "and the extraction path shards by chunk with no collective on the hot path)", file=sys.stderr)
+        return 2
+
+    from binarymeshfitting_amd import capi, roofline, workloads
+
+    wl = workloads.uniform_grid(n_side=8, c=64, iters=4)
+    n = len(wl.chunks)
+    dim = wl.chunks[0].dim
+    cd = workloads.to_capi(wl.chunks)
+    arr = capi.chunk_array(cd)
+    ctx = capi.Context(max_chunks=n, max_samples=n * dim ** 3, max_vertices=1 << 22, max_indices=1 << 24)
+    ctx.set_sampler(**wl.sampler)
+
+    # ---- batched path: warmup, then K timed steps ----------------------------
+    log(f"[bench] warmup {args.warmup} batched steps of {n} chunks")
+    for _ in range(max(1, args.warmup)):
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+    rc, res, _, _ = ctx.results(n)
+    capi.check(rc, "bmf_result")
+    ctx.sync()
+    log(f"[bench] timing {args.steps} batched steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+    ctx.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = n * args.steps / wall
+    rc, res, _, _ = ctx.results(n)
+    capi.check(rc, "bmf_result")
+    sampled = sum(r.sampled for r in res)
+    V = sum(r.vertex_count for r in res)
+    T = sum(r.index_count for r in res) // 3
+    log(f"[bench] batched: {value:.1f} chunks/s ({1e3 * wall / args.steps:.3f} ms/step), "
+        f"{sampled}/{n} chunks sampled, V={V} T={T}")
+
+    # ---- per-stage device times (HIP events on the library stream) -------------
+    stages = None
+    for _ in range(max(1, args.profile_reps)):
+        st = ctx.profile_stages(cd)
+        stages = {k: v + (stages or {}).get(k, 0.0) for k, v in st.items()}
+    stages = {k: v / max(1, args.profile_reps) for k, v in stages.items()}
+    log("[bench] stages (ms): " + ", ".join(f"{k}={v:.4f}" for k, v in stages.items()))
+    octaves = wl.sampler["octaves"]
+    samples = sampled * dim ** 3
+    flops = samples * roofline.terrain_flops_per_sample(octaves)
+    sample_s = stages["sample"] * 1e-3
+    achieved_tflops = flops / sample_s / 1e12
+    traffic = None
+    tfile = ROOT / "profiles" / "sample_kernel_uniform512x64_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+    roof = {
+        "kernel": "k_sample",
+        "bound": "valu",
+        "achieved": round(achieved_tflops, 3),
+        "peak": roofline.PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved_tflops / roofline.PEAK_FP32_TFLOPS, 4),
+        "frac_of_unpacked_peak": round(achieved_tflops / roofline.PEAK_FP32_UNPACKED_TFLOPS, 4),
+        "traffic": traffic,
+        "algorithmic_flops_per_launch": flops,
+        "algorithmic_bytes_per_launch": int(samples * roofline.SAMPLE_BYTES),
+        "avg_launch_ms": round(stages["sample"], 4),
+        "hbm_frac": round(samples * roofline.SAMPLE_BYTES / sample_s / 1e9 / roofline.PEAK_HBM_GBS, 5),
+    }
+
+    # ---- plain path ---------------------------------------------------------------
+    plain = None
+    if not args.no_plain and args.plain_steps > 0:
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
+        ctx.sync()
+        log(f"[bench] timing {args.plain_steps} plain steps")
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
+        ctx.sync()
+        plain = n * args.plain_steps / (time.perf_counter() - t0)
+        log(f"[bench] plain: {plain:.1f} chunks/s")
+
+    # ---- CPU baseline + parity -----------------------------------------------------
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        from oracle import pyoracle
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        spec = pyoracle.SamplerSpec(**wl.sampler)
+        ochunks = [pyoracle.make_chunk(c.pos, c.size, c.dim, c.overlap, c.iters, c.boundary_processing)
+                   for c in wl.chunks]
+        log(f"[bench] CPU oracle on all {n} chunks with {threads} threads")
+        t0 = time.perf_counter()
+        oV, onI, ocm, ock = pyoracle.extract_batch(spec, ochunks, threads)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(n / cpu_s, 3), "unit": "chunks/s", "cores": threads, "kind": "port",
+               "sample": f"all {n} chunks of the workload, one pass ({cpu_s:.2f} s wall)"}
+        log(f"[bench] CPU: {n / cpu_s:.2f} chunks/s")
+        # the last batched step's meshes vs the CPU run
+        rc, res, _, _ = ctx.results(n)
+        bad = 0
+        for j in range(n):
+            r = res[j]
+            if r.vertex_count != oV[j] or r.index_count != onI[j] or bool(r.contains_mesh) != bool(ocm[j]):
+                bad += 1
+                continue
+            if r.vertex_count:
+                pos, idx, _ = ctx.mesh(j)
+                if pyoracle.checksum_mesh(pos, idx) != int(ock[j]):
+                    bad += 1
+        parity = {"chunks": n, "mismatched": bad, "check": "bit-exact positions and triangle lists vs CPU"}
+        log(f"[bench] parity: {bad} of {n} chunks differ")
+
+    ms_per_step = 1e3 * wall / args.steps
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "chunks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "uniform-512x64: 8x8x8 chunks of 64^3 cells, fbm terrain seed 1337, K=4, batched",
+                   "chunks": n, "cells_per_chunk": (dim - 1) ** 3, "smoothing_iters": 4, "octaves": octaves,
+                   "sampled_chunks": sampled, "vertices": V, "triangles": T},
+        "cells_per_s": round(value * (dim - 1) ** 3, 1),
+        "plain_chunks_per_s": round(plain, 2) if plain else None,
+        "batched_over_plain": round(value / plain, 2) if plain else None,
+        "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -19,8 +19,10 @@ MAX_DIM = 256
 EXPORTS = [
     "bmf_create", "bmf_destroy", "bmf_strerror", "bmf_version", "bmf_set_sampler", "bmf_validate_chunks",
     "bmf_extract", "bmf_result", "bmf_mesh_view_get", "bmf_mesh_to_host", "bmf_sync", "bmf_sample_block",
-    "bmf_label_block", "bmf_cell_masks", "bmf_last_timing",
+    "bmf_label_block", "bmf_cell_masks", "bmf_last_timing", "bmf_profile_stages",
 ]
+NUM_STAGES = 8
+STAGES = ["plan", "sample", "count", "scan", "emit", "adjacency", "smooth", "finalize"]
 
 
 class Limits(C.Structure):
@@ -45,7 +47,7 @@ class ChunkDesc(C.Structure):
 class ChunkResult(C.Structure):
     _fields_ = [("vertex_offset", C.c_int64), ("index_offset", C.c_int64), ("vertex_count", C.c_int32),
                 ("index_count", C.c_int32), ("contains_mesh", C.c_int32), ("status", C.c_int32),
-                ("delta", C.c_float), ("overlap_pos", C.c_float * 3)]
+                ("sampled", C.c_int32), ("delta", C.c_float), ("overlap_pos", C.c_float * 3)]
 
 
 class MeshView(C.Structure):
@@ -81,7 +83,8 @@ def lib():
         L.bmf_sample_block.argtypes = [ctxp, P(ChunkDesc), P(C.c_float)]
         L.bmf_label_block.argtypes = [ctxp, P(ChunkDesc), P(C.c_uint32), P(C.c_int32)]
         L.bmf_cell_masks.argtypes = [ctxp, P(ChunkDesc), P(C.c_uint64)]
-        L.bmf_last_timing.argtypes = [ctxp, P(C.c_float), P(C.c_float)]
+        L.bmf_last_timing.argtypes = [ctxp, P(C.c_float)]
+        L.bmf_profile_stages.argtypes = [ctxp, P(ChunkDesc), C.c_int, P(C.c_float)]
         _lib = L
     return _lib
 
@@ -195,9 +198,15 @@ class Context:
         return out.reshape(dim, dim, (dim + 7) // 8)
 
     def last_timing(self):
-        t, s = C.c_float(), C.c_float()
-        check(lib().bmf_last_timing(self.h, C.byref(t), C.byref(s)), "bmf_last_timing")
-        return t.value, s.value
+        t = C.c_float()
+        check(lib().bmf_last_timing(self.h, C.byref(t)), "bmf_last_timing")
+        return t.value
+
+    def profile_stages(self, chunks):
+        arr = chunk_array(chunks)
+        out = (C.c_float * NUM_STAGES)()
+        check(lib().bmf_profile_stages(self.h, arr, len(chunks), out), "bmf_profile_stages")
+        return dict(zip(STAGES, list(out)))
 
 
 def validate_chunks(chunks, limits=None):

@@ -31,7 +31,10 @@ struct bmf_ctx {
     hipStream_t stream = nullptr;
     bmf_limits lim{};
     Arenas A{};
-    bmf_chunk_desc* h_descs = nullptr;      // pinned
+    bmf_chunk_desc* h_descs[2] = {nullptr, nullptr};  // pinned, double-buffered uploads
+    hipEvent_t ev_upload[2] = {nullptr, nullptr};
+    int upload_slot = 0;
+    hipEvent_t ev_stage[BMF_NUM_STAGES + 1] = {};
     bmf_chunk_result* h_results = nullptr;  // pinned
     Counters* h_ctr = nullptr;              // pinned
     SamplerDev h_sampler{};
@@ -42,8 +45,7 @@ struct bmf_ctx {
     bool have_extract = false;
     bool results_valid = false;
     std::vector<GraphEntry> graphs;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evs0 = nullptr, evs1 = nullptr;
-    bool sample_timed = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // last sampling-kernel layout
     int persistent_blocks = 2048;
 };
@@ -77,31 +79,47 @@ void free_arenas(bmf_ctx* c) {
 }
 
 // The whole device pipeline for chunks [0, n) of the uploaded descriptors.
+// With `ev` (BMF_NUM_STAGES + 1 events) the stage boundaries are recorded.
 int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int allow_cull, int reset, int sample_blocks,
-                     int row_blocks, int scan_blocks, int smooth_blocks, bool time_sample) {
+                     int row_blocks, int scan_blocks, int smooth_blocks, hipEvent_t* ev = nullptr) {
     hipStream_t s = c->stream;
+#define BMF_MARK(i) \
+    if (ev) BMF_HIP(hipEventRecord(ev[i], s))
+    BMF_MARK(0);
     BMF_HIP(launch_plan(A, n, allow_cull, reset, s));
-    if (time_sample) BMF_HIP(hipEventRecord(c->evs0, s));
+    BMF_MARK(1);
     BMF_HIP(launch_sample(A, n, sample_blocks, s));
-    if (time_sample) BMF_HIP(hipEventRecord(c->evs1, s));
+    BMF_MARK(2);
     BMF_HIP(launch_count(A, n, row_blocks, s));
+    BMF_MARK(3);
     BMF_HIP(launch_scan(A, n, scan_blocks, s));
+    BMF_MARK(4);
     BMF_HIP(launch_emit(A, n, row_blocks, s));
-    if (maxk > 0) {
-        BMF_HIP(launch_sort_adj(A, n, smooth_blocks, s));
-        for (int m = 0; m < maxk; m++) {
-            BMF_HIP(launch_dual(A, n, m, smooth_blocks, s));
-            BMF_HIP(launch_primal(A, n, m, smooth_blocks, s));
-        }
+    BMF_MARK(5);
+    if (maxk > 0) BMF_HIP(launch_sort_adj(A, n, smooth_blocks, s));
+    BMF_MARK(6);
+    for (int m = 0; m < maxk; m++) {
+        BMF_HIP(launch_dual(A, n, m, smooth_blocks, s));
+        BMF_HIP(launch_primal(A, n, m, smooth_blocks, s));
     }
+    BMF_MARK(7);
     BMF_HIP(launch_finalize(A, n, s));
+    BMF_MARK(8);
+#undef BMF_MARK
     return BMF_OK;
 }
 
+// Host descriptors -> device through one of two pinned staging buffers; a
+// buffer is refilled only after its previous copy has executed, so frames
+// can be enqueued back to back without a host synchronisation.
 int upload_descs(bmf_ctx* c, const bmf_chunk_desc* chunks, int n) {
-    std::memcpy(c->h_descs, chunks, sizeof(bmf_chunk_desc) * (size_t)n);
-    BMF_HIP(hipMemcpyAsync(c->A.descs, c->h_descs, sizeof(bmf_chunk_desc) * (size_t)n, hipMemcpyHostToDevice,
-                           c->stream));
+    const int slot = c->upload_slot;
+    c->upload_slot ^= 1;
+    BMF_HIP(hipEventSynchronize(c->ev_upload[slot]));
+    std::memcpy(c->h_descs[slot], chunks, sizeof(bmf_chunk_desc) * (size_t)n);
+    BMF_HIP(hipMemcpyAsync(c->A.descs, c->h_descs[slot], sizeof(bmf_chunk_desc) * (size_t)n,
+                           hipMemcpyHostToDevice, c->stream));
+    BMF_HIP(hipEventRecord(c->ev_upload[slot], c->stream));
     return BMF_OK;
 }
 
@@ -179,11 +197,13 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
          dalloc(&A.adj, (int64_t)BMF_MAX_VALENCE * A.cap_vertices) == hipSuccess &&
          dalloc(&A.idx, A.cap_indices) == hipSuccess && dalloc(&A.cen, A.cap_indices) == hipSuccess &&
          dalloc(&A.ctr, 1) == hipSuccess && dalloc(&A.sampler, 1) == hipSuccess;
-    ok = ok && hipHostMalloc((void**)&c->h_descs, sizeof(bmf_chunk_desc) * nc) == hipSuccess &&
+    ok = ok && hipHostMalloc((void**)&c->h_descs[0], sizeof(bmf_chunk_desc) * nc) == hipSuccess &&
+         hipHostMalloc((void**)&c->h_descs[1], sizeof(bmf_chunk_desc) * nc) == hipSuccess &&
          hipHostMalloc((void**)&c->h_results, sizeof(bmf_chunk_result) * nc) == hipSuccess &&
          hipHostMalloc((void**)&c->h_ctr, sizeof(Counters)) == hipSuccess;
     ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
-         hipEventCreate(&c->evs0) == hipSuccess && hipEventCreate(&c->evs1) == hipSuccess;
+         hipEventCreate(&c->ev_upload[0]) == hipSuccess && hipEventCreate(&c->ev_upload[1]) == hipSuccess;
+    for (int i = 0; ok && i <= BMF_NUM_STAGES; i++) ok = hipEventCreate(&c->ev_stage[i]) == hipSuccess;
     if (ok) {
         // counters start defined (plan resets them before use)
         std::memset(c->h_ctr, 0, sizeof(Counters));
@@ -204,12 +224,15 @@ void bmf_destroy(bmf_ctx* c) {
     for (auto& g : c->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     free_arenas(c);
-    if (c->h_descs) (void)hipHostFree(c->h_descs);
+    for (int i = 0; i < 2; i++)
+        if (c->h_descs[i]) (void)hipHostFree(c->h_descs[i]);
     if (c->h_results) (void)hipHostFree(c->h_results);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
-    hipEvent_t evs[] = {c->ev0, c->ev1, c->evs0, c->evs1};
+    hipEvent_t evs[] = {c->ev0, c->ev1, c->ev_upload[0], c->ev_upload[1]};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i <= BMF_NUM_STAGES; i++)
+        if (c->ev_stage[i]) (void)hipEventDestroy(c->ev_stage[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -272,8 +295,6 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
     int rc = bmf_validate_chunks(chunks, n, &c->lim);
     if (rc) return rc;
     BMF_HIP(hipSetDevice(c->device));
-    // the previous call's results and pinned descriptors must be consumed first
-    BMF_HIP(hipStreamSynchronize(c->stream));
     c->results_valid = false;
     c->last_n = n;
     c->have_extract = true;
@@ -284,7 +305,6 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
     for (int i = 0; i < n; i++) maxk = std::max(maxk, chunks[i].iters);
     const Arenas& A = c->A;
     const int pb = c->persistent_blocks;
-    c->sample_timed = false;
     if (mode == BMF_MODE_PLAIN) {
         // one launch sequence per chunk; scratch arenas reused from offset 0,
         // output arenas bump-allocated across the chunks (ChunkGenerator.cpp:91
@@ -299,7 +319,7 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
             const int64_t ns = dim * dim * dim;
             const int sb = (int)std::min<int64_t>(ceil_div(ns, kSampleTile), pb);
             const int rb = (int)std::min<int64_t>(ceil_div(dim * dim, kRowThreads / 64), pb);
-            rc = enqueue_pipeline(c, Aj, 1, chunks[j].iters, 1, j == 0, sb, rb, 1, std::min(pb, 1024), false);
+            rc = enqueue_pipeline(c, Aj, 1, chunks[j].iters, 1, j == 0, sb, rb, 1, std::min(pb, 1024));
             if (rc) return rc;
         }
         BMF_HIP(hipEventRecord(c->ev1, c->stream));
@@ -313,7 +333,7 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
     if (!ge) {
         hipGraph_t graph = nullptr;
         BMF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_pipeline(c, A, n, maxk, 1, 1, pb, pb, std::min(n, c->cu_count), pb, false);
+        rc = enqueue_pipeline(c, A, n, maxk, 1, 1, pb, pb, std::min(n, c->cu_count), pb);
         hipGraph_t captured = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &captured);
         if (rc) {
@@ -479,16 +499,39 @@ int bmf_cell_masks(bmf_ctx* c, const bmf_chunk_desc* d, uint64_t* masks) {
     return e == hipSuccess ? BMF_OK : BMF_EHIP;
 }
 
-int bmf_last_timing(bmf_ctx* c, float* total_ms, float* sample_ms) {
-    if (!c) return BMF_EINVAL;
+int bmf_last_timing(bmf_ctx* c, float* total_ms) {
+    if (!c || !total_ms) return BMF_EINVAL;
+    if (!c->have_extract) return BMF_ESTATE;
     BMF_HIP(hipSetDevice(c->device));
     BMF_HIP(hipStreamSynchronize(c->stream));
-    if (total_ms) {
-        float ms = 0.0f;
-        BMF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        *total_ms = ms;
-    }
-    if (sample_ms) *sample_ms = -1.0f;
+    float ms = 0.0f;
+    BMF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *total_ms = ms;
+    return BMF_OK;
+}
+
+int bmf_profile_stages(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, float* stage_ms) {
+    if (!c || !stage_ms) return BMF_EINVAL;
+    if (!c->sampler_set) return BMF_ESTATE;
+    int rc = bmf_validate_chunks(chunks, n, &c->lim);
+    if (rc) return rc;
+    if (n == 0) return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(c->device));
+    c->results_valid = false;
+    c->last_n = n;
+    c->have_extract = true;
+    rc = upload_descs(c, chunks, n);
+    if (rc) return rc;
+    int maxk = 0;
+    for (int i = 0; i < n; i++) maxk = std::max(maxk, chunks[i].iters);
+    const int pb = c->persistent_blocks;
+    BMF_HIP(hipEventRecord(c->ev0, c->stream));
+    rc = enqueue_pipeline(c, c->A, n, maxk, 1, 1, pb, pb, std::min(n, c->cu_count), pb, c->ev_stage);
+    if (rc) return rc;
+    BMF_HIP(hipEventRecord(c->ev1, c->stream));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < BMF_NUM_STAGES; i++)
+        BMF_HIP(hipEventElapsedTime(&stage_ms[i], c->ev_stage[i], c->ev_stage[i + 1]));
     return BMF_OK;
 }
 

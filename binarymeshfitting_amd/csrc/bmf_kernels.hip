@@ -821,6 +821,7 @@ __global__ void k_finalize(Arenas A, int n) {
     r.index_count = ok ? 3 * M.T : 0;
     r.contains_mesh = M.contains_mesh;
     r.status = M.status;
+    r.sampled = M.state == kStateSampled;
     r.delta = M.delta;
     r.overlap_pos[0] = M.opos[0];
     r.overlap_pos[1] = M.opos[1];

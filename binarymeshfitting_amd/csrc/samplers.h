@@ -80,16 +80,24 @@ __device__ __forceinline__ float snoise3(float x, float y, float z, uint32_t see
     return fminf(fmaxf(r, -1.0f), 1.0f);
 }
 
-// FP32 op count of snoise3 as written (add/sub/mul/min/max/floor/convert
-// and selects on float data = 1, fma = 2); integer hash work excluded.
-// skew 4 + floor 6 + convert 4 + unskew 7 + offsets 18 + 4 corners x 14
-// (3 fma = 6, max, 2 mul, mul, grad add + 2 negates) + sum 3 + scale/clamp 3.
-constexpr int kSimplexFlops = 101;
+// FP32 operation count of snoise3 as written, one per add/sub/mul/min/max/
+// floor/convert/compare/select on float data, two per fma; the integer hash
+// work is excluded (reported separately as kHashIntOps):
+//   skew: 2 add + 1 mul = 3;  x+s: 3 add + 3 floor + 3 cvt = 9;
+//   t: 1 cvt + 1 mul = 2;  x0: 6 sub;  rank compares: 3;
+//   corners 1 and 2: 2 x (3 cvt + 3 sub + 3 add) = 18;  corner 3: 6;
+//   per corner: 3 fma (6) + max + 2 mul + final mul + 4 selects + add = 16,
+//   x 4 = 64;  sum 3 + scale 1 + clamp 2 = 6.   Total 117.
+constexpr int kSimplexFlops = 117;
+// Integer ops of the lattice hash per simplex evaluation: 4 corners x
+// (3 mul24 + 3 xor for the combine, 2 mul24 + 6 xor/shift for the mix,
+// 2 shift/and for the index) plus 9 adds for the corner lattice points.
+constexpr int kHashIntOps = 73;
 // Per octave outside snoise3: 3 coordinate multiplies + the fma accumulate.
 constexpr int kOctaveFlops = 5;
-// Per sample outside the octave loop: 2 coordinate ops x 3 axes, the
-// normalise, clamp (2), amp multiply, y - y0 and the final subtract.
-constexpr int kTerrainFlops = 12;
+// Per terrain sample outside the octave loop: 3 x (cvt + mul + add) for the
+// coordinate, normalise, 2 clamp, amp multiply, y - y0, final subtract.
+constexpr int kTerrainFlops = 15;
 
 __device__ __forceinline__ float terrain_density(const SamplerDev& S, float x, float y, float z) {
     float acc = 0.0f;

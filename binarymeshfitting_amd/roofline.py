@@ -1,0 +1,38 @@
+"""Algorithmic work of the extraction kernels, for roofline fractions.
+
+FLOP constants are read from csrc/samplers.h (counted from the kernel
+source there).  MI355X peaks are the published figures.
+"""
+import re
+from pathlib import Path
+
+_SAMPLERS = Path(__file__).resolve().parent / "csrc" / "samplers.h"
+_TEXT = _SAMPLERS.read_text()
+
+
+def _const(name):
+    return int(re.search(rf"constexpr int {name} = (\d+);", _TEXT).group(1))
+
+
+SIMPLEX_FLOPS = _const("kSimplexFlops")
+OCTAVE_FLOPS = _const("kOctaveFlops")
+TERRAIN_FLOPS = _const("kTerrainFlops")
+HASH_INT_OPS = _const("kHashIntOps")
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, packed (78.6 unpacked)
+PEAK_FP32_UNPACKED_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0      # HBM3E
+
+# Compulsory HBM bytes written by the sampling kernel per sample: the FP32
+# density and one label bit.
+SAMPLE_BYTES = 4.0 + 1.0 / 8.0
+
+
+def terrain_flops_per_sample(octaves):
+    return octaves * (SIMPLEX_FLOPS + OCTAVE_FLOPS) + TERRAIN_FLOPS
+
+
+def mesh_bytes(V, T):
+    """Compulsory output bytes of a mesh: 12 B positions + 1 B boundary per
+    vertex, 12 B indices per triangle, 16 B of counts."""
+    return 13 * V + 12 * T + 16

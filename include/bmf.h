@@ -101,6 +101,8 @@ typedef struct {
     int32_t index_count;     /* 3 x triangles */
     int32_t contains_mesh;   /* DMCChunk::contains_mesh (DMCChunk.cpp:159-162) */
     int32_t status;          /* BMF_OK, or BMF_ECAPACITY when the chunk did not fit */
+    int32_t sampled;         /* 1 when densities were evaluated, 0 when the chunk was
+                                classified uniform without sampling */
     float delta;             /* DMCChunk::scale */
     float overlap_pos[3];    /* DMCChunk::overlap_pos */
 } bmf_chunk_result;
@@ -156,9 +158,17 @@ int bmf_sample_block(bmf_ctx* ctx, const bmf_chunk_desc* chunk, float* densities
 int bmf_label_block(bmf_ctx* ctx, const bmf_chunk_desc* chunk, uint32_t* words, int32_t* contains_mesh);
 int bmf_cell_masks(bmf_ctx* ctx, const bmf_chunk_desc* chunk, uint64_t* masks);
 
-/* Timing of the last bmf_extract on the device (HIP events, ms), and of
- * the sampling kernel inside it. */
-int bmf_last_timing(bmf_ctx* ctx, float* total_ms, float* sample_ms);
+/* Device time of the last bmf_extract (HIP events on the context stream,
+ * ms).  Replaces the reference's clock() batch timer (WorldWatcher.cpp:69-73). */
+int bmf_last_timing(bmf_ctx* ctx, float* total_ms);
+
+/* Runs the BATCHED path once without a graph, with HIP events between the
+ * stages on the context stream, and returns the stage times in ms:
+ * [0] plan, [1] sample, [2] count, [3] scan, [4] emit, [5] adjacency sort,
+ * [6] smoothing (all iterations), [7] finalize.  The per-stage counterpart of
+ * the reference's stage timers (DMCChunk.cpp:785-840, ChunkGenerator.cpp:153-193). */
+#define BMF_NUM_STAGES 8
+int bmf_profile_stages(bmf_ctx* ctx, const bmf_chunk_desc* chunks, int n, float* stage_ms);
 
 #ifdef __cplusplus
 }

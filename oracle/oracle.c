@@ -681,10 +681,19 @@ int or_positions_f64(const or_sampler_t* s, const or_chunk_t* c, const or_mesh_t
 /* ------------------------------------------------------------------------ */
 /* Batch: #pragma omp parallel for over chunks (ChunkGenerator.cpp:91).      */
 /* ------------------------------------------------------------------------ */
-static uint64_t mix64(uint64_t h, uint64_t v)
+/* Order-sensitive checksum of a mesh: sum over the u32 stream (position
+ * bit patterns, then indices) of word_k * (2k + 1), times an odd constant,
+ * modulo 2^64. */
+static uint64_t mesh_checksum(const float* pos, int64_t V, const uint32_t* idx, int64_t nI)
 {
-    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
-    return h;
+    uint64_t h = 0, k = 0;
+    for (int64_t i = 0; i < 3 * V; i++, k++) {
+        uint32_t b;
+        memcpy(&b, &pos[i], 4);
+        h += (uint64_t)b * (2 * k + 1);
+    }
+    for (int64_t i = 0; i < nI; i++, k++) h += (uint64_t)idx[i] * (2 * k + 1);
+    return h * 0x9E3779B97F4A7C15ull;
 }
 
 int or_extract_batch(const or_sampler_t* s, const or_chunk_t* chunks, int n, int threads,
@@ -699,13 +708,7 @@ int or_extract_batch(const or_sampler_t* s, const or_chunk_t* chunks, int n, int
     for (i = 0; i < n; i++) {
         or_mesh_t m;
         or_extract_chunk(s, &chunks[i], &m, 0);
-        uint64_t h = 0;
-        for (int64_t k = 0; k < 3 * m.V; k++) {
-            uint32_t b;
-            memcpy(&b, &m.pos[k], 4);
-            h = mix64(h, b);
-        }
-        for (int64_t k = 0; k < m.nI; k++) h = mix64(h, m.idx[k]);
+        uint64_t h = mesh_checksum(m.pos, m.V, m.idx, m.nI);
         V[i] = m.V;
         nI[i] = m.nI;
         contains[i] = m.contains_mesh;

@@ -256,14 +256,9 @@ def qef_solve(p, n, tol, f64=False):
 
 def checksum_mesh(pos, idx):
     """Same order-sensitive checksum as or_extract_batch."""
-    h = 0
-    M = (1 << 64) - 1
-
-    def mix(h, v):
-        return (h ^ ((v + 0x9E3779B97F4A7C15 + (h << 6) + (h >> 2)) & M)) & M
-
-    for b in np.ascontiguousarray(pos, np.float32).ravel().view(np.uint32):
-        h = mix(h, int(b))
-    for v in np.asarray(idx, np.uint32):
-        h = mix(h, int(v))
-    return h
+    w = np.concatenate([np.ascontiguousarray(pos, np.float32).ravel().view(np.uint32),
+                        np.asarray(idx, np.uint32)]).astype(np.uint64)
+    k = np.arange(len(w), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        h = np.sum(w * (np.uint64(2) * k + np.uint64(1)), dtype=np.uint64)
+        return int(h * np.uint64(0x9E3779B97F4A7C15))
