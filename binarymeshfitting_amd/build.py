@@ -47,7 +47,30 @@ def build(verbose=False, extra=()):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+HOST = PKG / "host"
+HOST_LIB = PKG / "libbmf_host.so"
+HOST_DEMO = HOST / "bmf_host_demo"
+
+
+def build_host(verbose=False):
+    """The C++ host layer (g++) over the C ABI, and its demo driver."""
+    rocm_lib = "/opt/rocm/lib"
+    common = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-fPIC", f"-I{PKG.parent / 'include'}"]
+    link = [f"-L{PKG}", "-lbmf", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath-link,{rocm_lib}",
+            f"-L{rocm_lib}"]
+    cmds = [
+        [*common, "-shared", "-o", str(HOST_LIB), str(HOST / "bmf_host.cpp"), *link],
+        [*common, "-o", str(HOST_DEMO), str(HOST / "bmf_host_demo.cpp"), str(HOST / "bmf_host.cpp"), *link,
+         "-Wl,-rpath,$ORIGIN/.."],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

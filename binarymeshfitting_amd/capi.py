@@ -21,8 +21,8 @@ EXPORTS = [
     "bmf_extract", "bmf_result", "bmf_mesh_view_get", "bmf_mesh_to_host", "bmf_sync", "bmf_sample_block",
     "bmf_label_block", "bmf_cell_masks", "bmf_last_timing", "bmf_profile_stages",
 ]
-NUM_STAGES = 8
-STAGES = ["plan", "sample", "count", "scan", "emit", "adjacency", "smooth", "finalize"]
+NUM_STAGES = 7
+STAGES = ["plan", "sample", "count", "scan", "emit", "smooth", "finalize"]
 
 
 class Limits(C.Structure):

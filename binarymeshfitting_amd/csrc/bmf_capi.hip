@@ -19,7 +19,7 @@ namespace {
 struct GraphEntry {
     int n = -1;
     int maxk = -1;
-    int sample_blocks = 0;
+    int wclasses = 0;
     hipGraphExec_t exec = nullptr;
 };
 
@@ -69,9 +69,10 @@ hipError_t dalloc(T** p, int64_t count) {
 
 void free_arenas(bmf_ctx* c) {
     Arenas& A = c->A;
-    void* ptrs[] = {A.meta, A.descs, A.results, A.tile_prefix, A.row_prefix, A.vtile_prefix, A.ttile_prefix,
-                    A.dens, A.labels, A.row_v, A.row_t, A.pos, A.boundary, A.valence, A.adj, A.idx, A.cen,
-                    A.ctr, A.sampler, c->d_spheres, c->d_boxes};
+    void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plane_prefix, A.smooth_list, A.smooth_prefix,
+                    A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
+                    A.pos,     A.pos_tmp,   A.boundary, A.vedge,       A.idx,          A.ctr,
+                    A.sampler, c->d_spheres, c->d_boxes};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     A = Arenas{};
@@ -80,31 +81,26 @@ void free_arenas(bmf_ctx* c) {
 
 // The whole device pipeline for chunks [0, n) of the uploaded descriptors.
 // With `ev` (BMF_NUM_STAGES + 1 events) the stage boundaries are recorded.
-int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int allow_cull, int reset, int sample_blocks,
-                     int row_blocks, int scan_blocks, int smooth_blocks, hipEvent_t* ev = nullptr) {
+int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses, int allow_cull, int reset,
+                     int sample_blocks, int row_blocks, int scan_blocks, int smooth_blocks, hipEvent_t* ev = nullptr) {
     hipStream_t s = c->stream;
 #define BMF_MARK(i) \
     if (ev) BMF_HIP(hipEventRecord(ev[i], s))
     BMF_MARK(0);
     BMF_HIP(launch_plan(A, n, allow_cull, reset, s));
     BMF_MARK(1);
-    BMF_HIP(launch_sample(A, n, sample_blocks, s));
+    BMF_HIP(launch_sample(A, n, sample_blocks, 1, s));
     BMF_MARK(2);
-    BMF_HIP(launch_count(A, n, row_blocks, s));
+    BMF_HIP(launch_count(A, n, row_blocks, wclasses, s));
     BMF_MARK(3);
     BMF_HIP(launch_scan(A, n, scan_blocks, s));
     BMF_MARK(4);
-    BMF_HIP(launch_emit(A, n, row_blocks, s));
+    BMF_HIP(launch_emit(A, n, row_blocks, wclasses, s));
     BMF_MARK(5);
-    if (maxk > 0) BMF_HIP(launch_sort_adj(A, n, smooth_blocks, s));
+    for (int m = 0; m < maxk; m++) BMF_HIP(launch_smooth(A, m, smooth_blocks, s));
     BMF_MARK(6);
-    for (int m = 0; m < maxk; m++) {
-        BMF_HIP(launch_dual(A, n, m, smooth_blocks, s));
-        BMF_HIP(launch_primal(A, n, m, smooth_blocks, s));
-    }
-    BMF_MARK(7);
     BMF_HIP(launch_finalize(A, n, s));
-    BMF_MARK(8);
+    BMF_MARK(7);
 #undef BMF_MARK
     return BMF_OK;
 }
@@ -189,14 +185,16 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && dalloc(&A.meta, nc) == hipSuccess && dalloc(&A.descs, nc) == hipSuccess &&
          dalloc(&A.results, nc) == hipSuccess && dalloc(&A.tile_prefix, nc + 1) == hipSuccess &&
-         dalloc(&A.row_prefix, nc + 1) == hipSuccess && dalloc(&A.vtile_prefix, nc + 1) == hipSuccess &&
-         dalloc(&A.ttile_prefix, nc + 1) == hipSuccess && dalloc(&A.dens, A.cap_samples) == hipSuccess &&
-         dalloc(&A.labels, A.cap_samples / 64 + nc) == hipSuccess && dalloc(&A.row_v, A.cap_rows) == hipSuccess &&
-         dalloc(&A.row_t, A.cap_rows) == hipSuccess && dalloc(&A.pos, 3 * A.cap_vertices) == hipSuccess &&
-         dalloc(&A.boundary, A.cap_vertices) == hipSuccess && dalloc(&A.valence, A.cap_vertices) == hipSuccess &&
-         dalloc(&A.adj, (int64_t)BMF_MAX_VALENCE * A.cap_vertices) == hipSuccess &&
-         dalloc(&A.idx, A.cap_indices) == hipSuccess && dalloc(&A.cen, A.cap_indices) == hipSuccess &&
-         dalloc(&A.ctr, 1) == hipSuccess && dalloc(&A.sampler, 1) == hipSuccess;
+         dalloc(&A.plane_prefix, nc + 1) == hipSuccess && dalloc(&A.smooth_list, nc) == hipSuccess &&
+         dalloc(&A.smooth_prefix, nc + 1) == hipSuccess &&
+         dalloc(&A.dens, A.cap_samples) == hipSuccess && dalloc(&A.labels, A.cap_samples / 64 + nc) == hipSuccess &&
+         dalloc(&A.exact, A.cap_samples / 64 + nc) == hipSuccess &&
+         dalloc(&A.row_v, A.cap_rows) == hipSuccess && dalloc(&A.row_t, A.cap_rows) == hipSuccess &&
+         dalloc(&A.cell_tb, A.cap_samples) == hipSuccess && dalloc(&A.cell_mask, A.cap_samples) == hipSuccess &&
+         dalloc(&A.pos, 3 * A.cap_vertices) == hipSuccess && dalloc(&A.pos_tmp, 3 * A.cap_vertices) == hipSuccess &&
+         dalloc(&A.boundary, A.cap_vertices) == hipSuccess && dalloc(&A.vedge, A.cap_vertices) == hipSuccess &&
+         dalloc(&A.idx, A.cap_indices) == hipSuccess && dalloc(&A.ctr, 1) == hipSuccess &&
+         dalloc(&A.sampler, 1) == hipSuccess;
     ok = ok && hipHostMalloc((void**)&c->h_descs[0], sizeof(bmf_chunk_desc) * nc) == hipSuccess &&
          hipHostMalloc((void**)&c->h_descs[1], sizeof(bmf_chunk_desc) * nc) == hipSuccess &&
          hipHostMalloc((void**)&c->h_results, sizeof(bmf_chunk_result) * nc) == hipSuccess &&
@@ -301,8 +299,11 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
     if (n == 0) return BMF_OK;
     rc = upload_descs(c, chunks, n);
     if (rc) return rc;
-    int maxk = 0;
-    for (int i = 0; i < n; i++) maxk = std::max(maxk, chunks[i].iters);
+    int maxk = 0, wclasses = 0;
+    for (int i = 0; i < n; i++) {
+        maxk = std::max(maxk, chunks[i].iters);
+        wclasses |= row_word_class(chunks[i].dim);
+    }
     const Arenas& A = c->A;
     const int pb = c->persistent_blocks;
     if (mode == BMF_MODE_PLAIN) {
@@ -318,8 +319,8 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
             const int64_t dim = chunks[j].dim;
             const int64_t ns = dim * dim * dim;
             const int sb = (int)std::min<int64_t>(ceil_div(ns, kSampleTile), pb);
-            const int rb = (int)std::min<int64_t>(ceil_div(dim * dim, kRowThreads / 64), pb);
-            rc = enqueue_pipeline(c, Aj, 1, chunks[j].iters, 1, j == 0, sb, rb, 1, std::min(pb, 1024));
+            rc = enqueue_pipeline(c, Aj, 1, chunks[j].iters, row_word_class(chunks[j].dim), 1, j == 0, sb, (int)dim, 1,
+                                  std::min(pb, 1024));
             if (rc) return rc;
         }
         BMF_HIP(hipEventRecord(c->ev1, c->stream));
@@ -329,11 +330,11 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
     // (n, max iterations) in a hipGraph and replayed.
     GraphEntry* ge = nullptr;
     for (auto& g : c->graphs)
-        if (g.n == n && g.maxk == maxk) ge = &g;
+        if (g.n == n && g.maxk == maxk && g.wclasses == wclasses) ge = &g;
     if (!ge) {
         hipGraph_t graph = nullptr;
         BMF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_pipeline(c, A, n, maxk, 1, 1, pb, pb, std::min(n, c->cu_count), pb);
+        rc = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, pb, pb, std::min(n, c->cu_count), 4 * pb);
         hipGraph_t captured = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &captured);
         if (rc) {
@@ -345,6 +346,7 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
         GraphEntry g;
         g.n = n;
         g.maxk = maxk;
+        g.wclasses = wclasses;
         e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return BMF_EHIP;
@@ -448,10 +450,10 @@ static int stage_one(bmf_ctx* c, const bmf_chunk_desc* d, bool scan) {
     if (rc) return rc;
     const Arenas& A = c->A;
     BMF_HIP(launch_plan(A, 1, 0, 1, c->stream));
-    BMF_HIP(launch_sample(A, 1, (int)std::min<int64_t>(ceil_div(ns, kSampleTile), c->persistent_blocks), c->stream));
+    BMF_HIP(launch_sample(A, 1, (int)std::min<int64_t>(ceil_div(ns, kSampleTile), c->persistent_blocks), 0,
+                          c->stream));
     if (scan) {
-        const int rb = (int)std::min<int64_t>(ceil_div((int64_t)d->dim * d->dim, 4), c->persistent_blocks);
-        BMF_HIP(launch_count(A, 1, rb, c->stream));
+        BMF_HIP(launch_count(A, 1, d->dim, row_word_class(d->dim), c->stream));
         BMF_HIP(launch_scan(A, 1, 1, c->stream));
     }
     return BMF_OK;
@@ -522,11 +524,15 @@ int bmf_profile_stages(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, float* s
     c->have_extract = true;
     rc = upload_descs(c, chunks, n);
     if (rc) return rc;
-    int maxk = 0;
-    for (int i = 0; i < n; i++) maxk = std::max(maxk, chunks[i].iters);
+    int maxk = 0, wclasses = 0;
+    for (int i = 0; i < n; i++) {
+        maxk = std::max(maxk, chunks[i].iters);
+        wclasses |= row_word_class(chunks[i].dim);
+    }
     const int pb = c->persistent_blocks;
     BMF_HIP(hipEventRecord(c->ev0, c->stream));
-    rc = enqueue_pipeline(c, c->A, n, maxk, 1, 1, pb, pb, std::min(n, c->cu_count), pb, c->ev_stage);
+    rc = enqueue_pipeline(c, c->A, n, maxk, wclasses, 1, 1, pb, pb, std::min(n, c->cu_count), 4 * pb,
+                          c->ev_stage);
     if (rc) return rc;
     BMF_HIP(hipEventRecord(c->ev1, c->stream));
     BMF_HIP(hipStreamSynchronize(c->stream));

@@ -10,10 +10,10 @@ namespace bmf {
 // Samples handled by one work-queue tile of the sampling kernel: 256 threads
 // x 4 rounds; a multiple of 64 so that every wave owns whole label words.
 constexpr int kSampleThreads = 256;
-constexpr int kSampleRounds = 4;
+constexpr int kSampleRounds = 8;
 constexpr int kSampleTile = kSampleThreads * kSampleRounds;
-constexpr int kRowThreads = 256;      // 4 waves, one label row each
-constexpr int kSmoothThreads = 256;
+constexpr int kRowThreads = 256;      // 4 waves; one x-plane per block, one label row per wave
+constexpr int kSmoothThreads = 256;   // one vertex per thread
 constexpr int kMaxRowWords = 4;       // ceil(BMF_MAX_DIM / 64)
 
 enum ChunkState : int32_t {
@@ -49,7 +49,7 @@ struct ChunkMeta {
     int64_t dens_off;   // float index into the density arena
     int64_t lab_off;    // u64 index into the label bitmap arena
     int64_t lab_words;  // u64 words of this chunk's bitmap
-    int64_t row_off;    // index into the per-row arrays
+    int64_t row_off;    // index into the per-row arrays (rows (x, y) in x-major order)
     uint32_t any_air;   // set by the sampling kernel
     uint32_t any_solid;
     int32_t contains_mesh;
@@ -68,10 +68,10 @@ struct Counters {
     unsigned long long need_v;
     unsigned long long need_i;
     int64_t total_tiles;
-    int64_t total_rows;
+    int64_t total_planes;
     int64_t total_samples;
     int64_t total_vtiles;
-    int64_t total_ttiles;
+    int32_t n_smooth;
     int32_t overflow;
     int32_t error;
 };
@@ -81,42 +81,39 @@ struct Arenas {
     ChunkMeta* meta;           // [max_chunks]
     bmf_chunk_desc* descs;     // [max_chunks]
     bmf_chunk_result* results; // [max_chunks]
-    int64_t* tile_prefix;      // [max_chunks + 1]
-    int64_t* row_prefix;       // [max_chunks + 1]
-    int64_t* vtile_prefix;     // [max_chunks + 1]
-    int64_t* ttile_prefix;     // [max_chunks + 1]
+    int64_t* tile_prefix;      // [max_chunks + 1] sampling tiles
+    int64_t* plane_prefix;     // [max_chunks + 1] x-planes of the sampled chunks
+    int32_t* smooth_list;      // [max_chunks] chunks that run the smoothing
+    int64_t* smooth_prefix;    // [max_chunks + 1] vertex tiles of the listed chunks
     float* dens;               // [max_samples]
     uint64_t* labels;          // [max_samples / 64 + max_chunks]
-    uint32_t* row_v;           // [max_rows]
-    uint32_t* row_t;           // [max_rows]
+    uint64_t* exact;           // same layout as labels: density of the sample is final
+    uint32_t* row_v;           // [max_rows] per label row: vertex count, then base
+    uint32_t* row_t;           // [max_rows] per label row: triangle count, then base
+    uint32_t* cell_tb;         // [max_samples] first triangle of each surface cell
+    uint8_t* cell_mask;        // [max_samples] corner mask of each surface cell
     float* pos;                // [max_vertices * 3]
+    float* pos_tmp;            // [max_vertices * 3] smoothing ping-pong
     uint8_t* boundary;         // [max_vertices]
-    uint32_t* valence;         // [max_vertices]
-    uint32_t* adj;             // [max_vertices * BMF_MAX_VALENCE]
+    uint32_t* vedge;           // [max_vertices] lattice edge of each vertex: x | y<<8 | z<<16 | axis<<24
     uint32_t* idx;             // [max_indices]
-    float* cen;                // [max_indices] (one float3 per triangle)
     Counters* ctr;
     SamplerDev* sampler;
     int64_t cap_samples, cap_rows, cap_vertices, cap_indices;
     int32_t cap_chunks;
 };
 
-// Launch wrappers (bmf_kernels.hip).
-struct LaunchShape {
-    int sample_blocks;
-    int row_blocks;
-    int scan_blocks;
-    int smooth_blocks;
-};
+inline int row_word_class(int dim) { return 1 << (((dim + 63) >> 6) - 1); }
 
+// Launch wrappers (bmf_kernels.hip).
 hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, hipStream_t s);
-hipError_t launch_sample(const Arenas& A, int n, int blocks, hipStream_t s);
-hipError_t launch_count(const Arenas& A, int n, int blocks, hipStream_t s);
+// adaptive: terrain samples stop adding octaves once their label is decided
+hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, hipStream_t s);
+// wclasses: bit W-1 set when the batch holds chunks with ceil(dim/64) == W.
+hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s);
-hipError_t launch_emit(const Arenas& A, int n, int blocks, hipStream_t s);
-hipError_t launch_sort_adj(const Arenas& A, int n, int blocks, hipStream_t s);
-hipError_t launch_dual(const Arenas& A, int n, int m, int blocks, hipStream_t s);
-hipError_t launch_primal(const Arenas& A, int n, int m, int blocks, hipStream_t s);
+hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
+hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s);
 hipError_t launch_ref_labels(const Arenas& A, uint32_t* words, hipStream_t s);
 hipError_t launch_ref_masks(const Arenas& A, uint64_t* masks, hipStream_t s);

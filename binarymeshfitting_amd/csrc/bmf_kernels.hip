@@ -1,7 +1,6 @@
 // HIP kernels of the chunk-extraction path (gfx950, wave64).
 //
-// Per batch:  plan -> sample -> count -> scan -> emit -> sort_adj ->
-//             K x (dual, primal) -> finalize
+// Per batch:  plan -> sample -> count -> scan -> emit -> smooth -> finalize
 //
 // Data layout in HBM (per sampled chunk, offsets from the plan kernel):
 //   densities   FP32 [x][y][z]                    (DMCChunk density_block)
@@ -10,10 +9,12 @@
 //               one word per wave of 64 consecutive samples (one ballot)
 //   row_v/row_t per label row (x, y): vertex / triangle counts, then
 //               exclusive bases after the scan
-//   output      positions float3, boundary u8, chunk-local triangle
-//               indices, valence u32, adjacency BMF_MAX_VALENCE x u32
+//   cell_tb/    per surface cell (same index as the density): first triangle
+//   cell_mask   id and corner mask, written only for chunks that smooth
+//   output      positions float3, boundary u8, vertex edge code u32,
+//               chunk-local triangle indices
 // Cell masks and the reference's u32 label words are never materialised on
-// the hot path: the row kernels rebuild them from the bitmap with shifts.
+// the hot path: the plane kernels rebuild them from the bitmap with shifts.
 #include <hip/hip_runtime.h>
 
 #include "../../include/bmf_mc_tables.h"
@@ -24,12 +25,19 @@ namespace bmf {
 
 __constant__ uint64_t c_tri_table[256] = BMF_TRI_TABLE_INIT;
 
-// Local edge e of a cell -> (dx, dy, dz, axis) of the lattice edge it is
-// (DMCChunk.cpp:543-565).
-__constant__ int8_t c_edge_dx[12] = {0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 1, 1};
-__constant__ int8_t c_edge_dy[12] = {0, 0, 1, 1, 0, 0, 0, 0, 0, 1, 0, 1};
-__constant__ int8_t c_edge_dz[12] = {0, 1, 0, 1, 0, 1, 0, 1, 0, 0, 0, 0};
-__constant__ int8_t c_edge_ax[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+// The four cells around a lattice edge along `axis`, in lexicographic cell
+// order, as (dx, dy, dz, local edge) offsets from the edge origin; local
+// edges numbered as in DMCChunk::polygonize_cell (DMCChunk.cpp:543-565).
+// Visiting them in this order and each cell's table slots in order lists
+// the edge's triangles in ascending id, the order in which
+// MeshProcessor::init_primitives appends them (MeshProcessor.cpp:114-127).
+__constant__ int8_t c_ring[3][4][4] = {
+    {{0, -1, -1, 3}, {0, -1, 0, 2}, {0, 0, -1, 1}, {0, 0, 0, 0}},
+    {{-1, 0, -1, 7}, {-1, 0, 0, 6}, {0, 0, -1, 5}, {0, 0, 0, 4}},
+    {{-1, -1, 0, 11}, {-1, 0, 0, 10}, {0, -1, 0, 9}, {0, 0, 0, 8}},
+};
+
+constexpr int kMaxPlaneWords = (BMF_MAX_DIM * BMF_MAX_DIM + 63) / 64 + 1;
 
 // ------------------------------------------------------------------------
 // small helpers
@@ -208,12 +216,12 @@ __device__ __forceinline__ uint32_t cell_mask(const RowBits<W>& r00, const RowBi
 constexpr int kPlanThreads = 1024;
 
 __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allow_cull, int reset_bumps) {
-    __shared__ int64_t s_scan[4][kPlanThreads];
+    __shared__ int64_t s_scan[5][kPlanThreads];
     const int tid = threadIdx.x;
     const SamplerDev* __restrict__ S = A.sampler;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
     const int j0 = tid * per, j1 = min(n, j0 + per);
-    int64_t sum[4] = {0, 0, 0, 0};  // samples, label words, rows, tiles
+    int64_t sum[5] = {0, 0, 0, 0, 0};  // samples, label words, rows, tiles, planes
 
     for (int j = j0; j < j1; j++) {
         const bmf_chunk_desc d = A.descs[j];
@@ -257,42 +265,44 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         sum[1] += m.lab_words;
         sum[2] += sampled ? (int64_t)d.dim * d.dim : 0;
         sum[3] += sampled ? (ns + kSampleTile - 1) / kSampleTile : 0;
+        sum[4] += sampled ? d.dim : 0;
         A.meta[j] = m;
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) s_scan[q][tid] = sum[q];
+    for (int q = 0; q < 5; q++) s_scan[q][tid] = sum[q];
     __syncthreads();
     // Hillis-Steele inclusive scan across threads
     for (int off = 1; off < kPlanThreads; off <<= 1) {
-        int64_t v[4];
+        int64_t v[5];
 #pragma unroll
-        for (int q = 0; q < 4; q++) v[q] = tid >= off ? s_scan[q][tid - off] : 0;
+        for (int q = 0; q < 5; q++) v[q] = tid >= off ? s_scan[q][tid - off] : 0;
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; q++) s_scan[q][tid] += v[q];
+        for (int q = 0; q < 5; q++) s_scan[q][tid] += v[q];
         __syncthreads();
     }
-    int64_t base[4];
+    int64_t base[5];
 #pragma unroll
-    for (int q = 0; q < 4; q++) base[q] = s_scan[q][tid] - sum[q];
+    for (int q = 0; q < 5; q++) base[q] = s_scan[q][tid] - sum[q];
     const int64_t tot_samples = s_scan[0][kPlanThreads - 1];
     const int64_t tot_words = s_scan[1][kPlanThreads - 1];
     const int64_t tot_rows = s_scan[2][kPlanThreads - 1];
     const int64_t tot_tiles = s_scan[3][kPlanThreads - 1];
+    const int64_t tot_planes = s_scan[4][kPlanThreads - 1];
     const bool overflow = tot_samples > A.cap_samples || tot_words > A.cap_samples / 64 + A.cap_chunks ||
                           tot_rows > A.cap_rows;
-    int64_t tp = base[3], rp = base[2];
+    int64_t tp = base[3], pp = base[4];
     for (int j = j0; j < j1; j++) {
         ChunkMeta& m = A.meta[j];
         m.dens_off += base[0];
         m.lab_off += base[1];
         m.row_off += base[2];
         A.tile_prefix[j] = overflow ? 0 : tp;
-        A.row_prefix[j] = overflow ? 0 : rp;
+        A.plane_prefix[j] = overflow ? 0 : pp;
         if (m.state == kStateSampled) {
             const int64_t ns = (int64_t)m.dim * m.dim * m.dim;
             tp += (ns + kSampleTile - 1) / kSampleTile;
-            rp += (int64_t)m.dim * m.dim;
+            pp += m.dim;
             if (overflow) {
                 m.state = kStateSkipped;
                 m.status = BMF_ECAPACITY;
@@ -304,7 +314,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
     }
     if (tid == 0) {
         A.tile_prefix[n] = overflow ? 0 : tot_tiles;
-        A.row_prefix[n] = overflow ? 0 : tot_rows;
+        A.plane_prefix[n] = overflow ? 0 : tot_planes;
         Counters* c = A.ctr;
         c->sample_head = 0;
         if (reset_bumps) {
@@ -317,80 +327,290 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         }
         if (overflow) c->overflow = 1;
         c->total_tiles = overflow ? 0 : tot_tiles;
-        c->total_rows = overflow ? 0 : tot_rows;
+        c->total_planes = overflow ? 0 : tot_planes;
+        c->n_smooth = 0;
         c->total_samples = tot_samples;
     }
 }
 
 // ------------------------------------------------------------------------
-// sample: persistent; work-queue tiles of kSampleTile consecutive samples
+// sample: persistent; workgroups pull tiles of kSampleTile consecutive
+// samples of one chunk from an atomic work queue.
+//
+// Terrain chunks use exact adaptive octave evaluation.  After k octaves the
+// FP32 partial sum S_k bounds the final sum: each remaining step is
+// fma(a_o, n_o, S) with |n_o| <= 1 and rounding is monotone, so
+// S_O in [L, H] with L = (..(S_k - |a_k|) - ..), H = (..(S_k + |a_k|) + ..)
+// evaluated in FP32; the density is monotone in S, so d(L), d(H) bound the
+// final density and decide the label once they agree in sign.  Undecided
+// samples are compacted into an LDS queue (wave ballot + popcount) so each
+// octave runs on dense waves.  Samples that run all octaves get their exact
+// density written and their bit set in the `exact` bitmap; the others only
+// get their label (their densities are finished later, in k_count, when
+// they are endpoints of a sign-changing edge).  Without adaptation every
+// sample is evaluated fully (bit-identical values either way).
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
+__device__ __forceinline__ float density_from_sum(const SamplerDev& S, float sum, float t2) {
+    float fb = sum * S.inv_norm;
+    fb = fminf(fmaxf(fb, -1.0f), 1.0f);
+    return S.amp * fb - t2;
+}
+
+// 0: undecided, 1: certainly air (d < 0), 2: certainly solid.
+__device__ __forceinline__ int decide_label(const SamplerDev& S, int k, float sum, float t2) {
+    float lo = sum, hi = sum;
+    for (int o = k; o < S.octaves; o++) {
+        const float a = fabsf(S.ao[o]);
+        lo = lo - a;
+        hi = hi + a;
+    }
+    const float d0 = density_from_sum(S, lo, t2), d1 = density_from_sum(S, hi, t2);
+    if (fmaxf(d0, d1) < 0.0f) return 1;
+    if (!(fminf(d0, d1) < 0.0f)) return 2;
+    return 0;
+}
+
+__device__ __forceinline__ void sample_coords(const ChunkMeta& M, uint32_t i, uint32_t dim, uint32_t dim2, float rd,
+                                              float rd2, float& px, float& py, float& pz) {
+    const uint32_t x = udiv24(i, dim2, rd2);
+    const uint32_t rem = i - x * dim2;
+    const uint32_t y = udiv24(rem, dim, rd);
+    const uint32_t z = rem - y * dim;
+    // NoiseSampler.cpp:21-27: (float)i * scale + p, two roundings
+    px = (float)x * M.delta + M.opos[0];
+    py = (float)y * M.delta + M.opos[1];
+    pz = (float)z * M.delta + M.opos[2];
+}
+
+__global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int adaptive) {
     __shared__ int64_t s_tile;
+    __shared__ uint32_t s_qn[2];
+    __shared__ uint16_t s_qi[2][kSampleTile];
+    __shared__ float s_qs[2][kSampleTile];
+    __shared__ unsigned long long s_lab[kSampleTile / 64];
+    __shared__ unsigned long long s_exact[kSampleTile / 64];
+    __shared__ uint32_t s_flags;
     const SamplerDev* __restrict__ S = A.sampler;
     const int64_t total = A.ctr->total_tiles;
     const int lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    const bool adapt = adaptive && S->kind == BMF_SAMPLER_TERRAIN;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&A.ctr->sample_head, 1ull);
+        if (threadIdx.x == 0) {
+            s_tile = (int64_t)atomicAdd(&A.ctr->sample_head, 1ull);
+            s_qn[0] = 0;
+            s_qn[1] = 0;
+            s_flags = 0;
+        }
         __syncthreads();
         const int64_t tile = s_tile;
-        __syncthreads();
         if (tile >= total) break;
         const int j = find_chunk(A.tile_prefix, n, tile);
         const ChunkMeta& M = A.meta[j];
         const uint32_t dim = (uint32_t)M.dim, dim2 = dim * dim, ns = dim2 * dim;
         const float rd = 1.0f / (float)dim, rd2 = 1.0f / (float)dim2;
-        const float delta = M.delta, ox = M.opos[0], oy = M.opos[1], oz = M.opos[2];
         float* __restrict__ dens = A.dens + M.dens_off;
-        uint64_t* __restrict__ lab = A.labels + M.lab_off;
         const uint32_t first = (uint32_t)(tile - A.tile_prefix[j]) * kSampleTile;
-        bool air = false, solid = false;
+        // ---- phase 0: every sample of the tile ----
 #pragma unroll 1
         for (int r = 0; r < kSampleRounds; r++) {
-            const uint32_t i = first + r * kSampleThreads + threadIdx.x;
+            const uint32_t li = r * kSampleThreads + threadIdx.x;  // waves cover 64 consecutive samples
+            const uint32_t i = first + li;
             const bool valid = i < ns;
-            float d = 0.0f;
+            bool air = false, done = true;
             if (valid) {
-                const uint32_t x = udiv24(i, dim2, rd2);
-                const uint32_t rem = i - x * dim2;
-                const uint32_t y = udiv24(rem, dim, rd);
-                const uint32_t z = rem - y * dim;
-                // NoiseSampler.cpp:21-27: (float)i * scale + p, two roundings
-                const float px = (float)x * delta + ox;
-                const float py = (float)y * delta + oy;
-                const float pz = (float)z * delta + oz;
-                d = sample_density(*S, px, py, pz);
-                dens[i] = d;
+                float px, py, pz;
+                sample_coords(M, i, dim, dim2, rd, rd2, px, py, pz);
+                if (adapt) {
+                    const float t2 = py - S->y0;
+                    const int dec = decide_label(*S, 0, 0.0f, t2);
+                    done = dec != 0;
+                    air = dec == 1;
+                } else {
+                    const float d = sample_density(*S, px, py, pz);
+                    dens[i] = d;
+                    air = d < 0.0f;
+                }
             }
-            const uint64_t b = __ballot(valid && d < 0.0f);
-            const uint64_t sb = __ballot(valid && !(d < 0.0f));
-            if (lane == 0 && (i - (uint32_t)lane) < ns) lab[i >> 6] = b;
-            air = air || b != 0ull;
-            solid = solid || sb != 0ull;
+            const uint64_t bl = __ballot(valid && done && air);
+            const uint64_t push = __ballot(valid && !done);
+            const uint64_t vb = __ballot(valid);
+            if (lane == 0) {
+                s_lab[li >> 6] = bl;
+                s_exact[li >> 6] = adapt ? 0ull : vb;
+            }
+            if (push) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&s_qn[0], (uint32_t)__popcll(push));
+                base = __shfl(base, 0);
+                if (valid && !done) {
+                    const uint32_t q = base + (uint32_t)__popcll(push & lt);
+                    s_qi[0][q] = (uint16_t)li;
+                    s_qs[0][q] = 0.0f;
+                }
+            }
         }
-        if (lane == 0) {
-            if (air) atomicOr(&A.meta[j].any_air, 1u);
-            if (solid) atomicOr(&A.meta[j].any_solid, 1u);
+        __syncthreads();
+        // ---- octave rounds on the compacted queue ----
+        if (adapt) {
+            const int O = S->octaves;
+            for (int k = 0; k < O; k++) {
+                const int cur = k & 1, nxt = cur ^ 1;
+                const uint32_t nq = s_qn[cur];
+                if (nq == 0) break;
+                const float f = S->fo[k], a = S->ao[k];
+                const uint32_t seed = S->seed + (uint32_t)k;
+                for (uint32_t e0 = 0; e0 < nq; e0 += kSampleThreads) {
+                    const uint32_t e = e0 + threadIdx.x;
+                    const bool active = e < nq;
+                    uint32_t li = 0;
+                    float sum = 0.0f, t2 = 0.0f;
+                    int dec = 2;
+                    if (active) {
+                        li = s_qi[cur][e];
+                        sum = s_qs[cur][e];
+                        float px, py, pz;
+                        sample_coords(M, first + li, dim, dim2, rd, rd2, px, py, pz);
+                        t2 = py - S->y0;
+                        const float nz = snoise3(px * f, py * f, pz * f, seed);
+                        sum = __builtin_fmaf(a, nz, sum);
+                        if (k == O - 1) {
+                            const float d = density_from_sum(*S, sum, t2);
+                            dens[first + li] = d;
+                            dec = d < 0.0f ? 1 : 2;
+                            atomicOr(&s_exact[li >> 6], 1ull << (li & 63));
+                        } else {
+                            dec = decide_label(*S, k + 1, sum, t2);
+                        }
+                        if (dec == 1) atomicOr(&s_lab[li >> 6], 1ull << (li & 63));
+                    }
+                    const uint64_t push = __ballot(active && dec == 0);
+                    if (push) {
+                        uint32_t base = 0;
+                        if (lane == 0) base = atomicAdd(&s_qn[nxt], (uint32_t)__popcll(push));
+                        base = __shfl(base, 0);
+                        if (active && dec == 0) {
+                            const uint32_t q = base + (uint32_t)__popcll(push & lt);
+                            s_qi[nxt][q] = (uint16_t)li;
+                            s_qs[nxt][q] = sum;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) s_qn[cur] = 0;
+                __syncthreads();
+            }
         }
+        // ---- label and exactness words of the tile ----
+        if (threadIdx.x < kSampleTile / 64) {
+            const uint32_t w0 = first + 64u * threadIdx.x;
+            if (w0 < ns) {
+                const uint32_t nb = ns - w0;
+                const uint64_t vm = nb >= 64 ? ~0ull : ((1ull << nb) - 1ull);
+                const uint64_t lb = s_lab[threadIdx.x];
+                A.labels[M.lab_off + (w0 >> 6)] = lb;
+                A.exact[M.lab_off + (w0 >> 6)] = s_exact[threadIdx.x];
+                uint32_t fl = 0;
+                if (lb & vm) fl |= 1u;
+                if (~lb & vm) fl |= 2u;
+                if (fl) atomicOr(&s_flags, fl);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (s_flags & 1u) atomicOr(&A.meta[j].any_air, 1u);
+            if (s_flags & 2u) atomicOr(&A.meta[j].any_solid, 1u);
+        }
+        __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------------
-// count: one wave per label row -> vertices and triangles of the row
+// plane staging: three consecutive label planes of a chunk, realigned
+// to u64 words in LDS (coalesced funnel-shifted loads); rows are then read
+// from LDS.  Word pw of each plane is a zero pad for the funnel reads.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void stage_planes(uint64_t (*s_lab)[kMaxPlaneWords], const uint64_t* __restrict__ L,
+                                             int64_t nwords, int dim, int x0) {
+    const int64_t pbits = (int64_t)dim * dim;
+    const int pw = (int)((pbits + 63) >> 6);
+    for (int p = 0; p < 3; p++) {
+        const int xp = x0 + p;
+        for (int k = threadIdx.x; k <= pw; k += blockDim.x) {
+            uint64_t v = 0ull;
+            if (xp >= 0 && xp < dim && k < pw) {
+                const int64_t bit = (int64_t)xp * pbits + 64ll * k;
+                const int64_t w = bit >> 6;
+                const int sh = (int)(bit & 63);
+                v = L[w];
+                if (sh) {
+                    const uint64_t hi = (w + 1 < nwords) ? L[w + 1] : 0ull;
+                    v = (v >> sh) | (hi << (64 - sh));
+                }
+                const int64_t rem = pbits - 64ll * k;
+                if (rem < 64) v &= (1ull << rem) - 1ull;
+            }
+            s_lab[p][k] = v;
+        }
+    }
+}
+
+// Row y of a staged plane.
+template <int W>
+__device__ __forceinline__ RowBits<W> lds_row(const uint64_t* P, int dim, int y, bool valid) {
+    RowBits<W> r = zero_row<W>();
+    if (!valid) return r;
+    const int base = y * dim;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const int bit = base + 64 * k;
+        const int w = bit >> 6;
+        const int sh = bit & 63;
+        uint64_t v = P[w];
+        if (sh) v = (v >> sh) | (P[w + 1] << (64 - sh));
+        const int nb = dim - 64 * k;
+        if (nb < 64) v &= (1ull << nb) - 1ull;
+        r.w[k] = v;
+    }
+    return r;
+}
+
+// ------------------------------------------------------------------------
+// count: one block per (chunk, x-plane), one wave per label row ->
+// vertices and triangles of each row
 // ------------------------------------------------------------------------
 template <int W>
-__device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, int64_t r, const uint8_t* s_ntri,
-                                          uint32_t& nv, uint32_t& nt) {
+__device__ __forceinline__ RowBits<W> rshl1(const RowBits<W>& a) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) r.w[k] = (a.w[k] << 1) | (k > 0 ? (a.w[k - 1] >> 63) : 0ull);
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ RowBits<W> ror(const RowBits<W>& a, const RowBits<W>& b) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) r.w[k] = a.w[k] | b.w[k];
+    return r;
+}
+
+// Staged planes: s_lab[0] = x-1, s_lab[1] = x, s_lab[2] = x+1.  Counts the
+// row's vertices and triangles, and finishes the densities of the row's
+// samples that are endpoints of sign-changing edges but were labelled
+// before their last octave (k_sample's adaptive evaluation).
+template <int W>
+__device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, const uint64_t (*s_lab)[kMaxPlaneWords],
+                                          const uint8_t* s_ntri, int x, int y, uint32_t& nv, uint32_t& nt) {
     const int dim = M.dim;
     const int lane = lane_id();
-    const int x = (int)(r / dim), y = (int)(r - (int64_t)x * dim);
-    const uint64_t* L = A.labels + M.lab_off;
-    const RowBits<W> r00 = load_row<W>(L, M.lab_words, dim, x, y);
-    const RowBits<W> r10 = load_row<W>(L, M.lab_words, dim, x + 1, y);
-    const RowBits<W> r01 = load_row<W>(L, M.lab_words, dim, x, y + 1);
-    const RowBits<W> r11 = load_row<W>(L, M.lab_words, dim, x + 1, y + 1);
-    nv = total_pop<W>(rxor<W>(r00, r10, x + 1 < dim)) + total_pop<W>(rxor<W>(r00, r01, y + 1 < dim)) +
-         total_pop<W>(zedges<W>(r00, dim));
+    const bool x1 = x + 1 < dim, y1 = y + 1 < dim;
+    const RowBits<W> r00 = lds_row<W>(s_lab[1], dim, y, true);
+    const RowBits<W> r01 = lds_row<W>(s_lab[1], dim, y + 1, y1);
+    const RowBits<W> r10 = lds_row<W>(s_lab[2], dim, y, x1);
+    const RowBits<W> r11 = lds_row<W>(s_lab[2], dim, y + 1, x1 && y1);
+    const RowBits<W> ex = rxor<W>(r00, r10, x1), ey = rxor<W>(r00, r01, y1), ez = zedges<W>(r00, dim);
+    nv = total_pop<W>(ex) + total_pop<W>(ey) + total_pop<W>(ez);
     nt = 0;
     if (x < dim - 1 && y < dim - 1 && !quad_uniform<W>(r00, r01, r10, r11, dim)) {
         for (int z0 = 0; z0 < dim - 1; z0 += 64) {
@@ -400,32 +620,62 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, i
                   4u * (uint32_t)__popcll(__ballot(c & 4u));
         }
     }
+    // endpoints of the sign-changing edges incident to the row's samples
+    const RowBits<W> rxm = lds_row<W>(s_lab[0], dim, y, x >= 1);
+    const RowBits<W> rym = lds_row<W>(s_lab[1], dim, y - 1, y >= 1);
+    RowBits<W> ep = ror<W>(ror<W>(ex, ey), ror<W>(ez, rshl1<W>(ez)));
+    ep = ror<W>(ep, ror<W>(rxor<W>(r00, rxm, x >= 1), rxor<W>(r00, rym, y >= 1)));
+    if (total_pop<W>(ep) == 0) return;
+    const RowBits<W> done = load_row<W>(A.exact + M.lab_off, M.lab_words, dim, x, y);
+    RowBits<W> need;
+#pragma unroll
+    for (int k = 0; k < W; k++) need.w[k] = ep.w[k] & ~done.w[k];
+    if (total_pop<W>(need) == 0) return;
+    const SamplerDev* __restrict__ S = A.sampler;
+    float* __restrict__ D = A.dens + M.dens_off + ((int64_t)x * dim + y) * dim;
+    const float px = (float)x * M.delta + M.opos[0];
+    const float py = (float)y * M.delta + M.opos[1];
+    for (int z0 = 0; z0 < dim; z0 += 64) {
+        const int z = z0 + lane;
+        if (z < dim && bit_at<W>(need, z)) {
+            const float pz = (float)z * M.delta + M.opos[2];
+            D[z] = sample_density(*S, px, py, pz);
+        }
+    }
 }
 
+// Instantiated per row-word class W = ceil(dim/64); blocks of other classes
+// skip their items (the host launches only the classes present).
+template <int W>
 __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
     __shared__ uint8_t s_ntri[256];
+    __shared__ uint64_t s_lab[3][kMaxPlaneWords];
     for (int t = threadIdx.x; t < 256; t += blockDim.x) s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
-    __syncthreads();
     const int wave = uniform_int(threadIdx.x >> 6);
     const int lane = lane_id();
-    const int64_t nw = (int64_t)gridDim.x * (kRowThreads / 64);
-    const int64_t total = A.ctr->total_rows;
-    for (int64_t g = (int64_t)blockIdx.x * (kRowThreads / 64) + wave; g < total; g += nw) {
-        const int j = find_chunk(A.row_prefix, n, g);
+    const int64_t total = A.ctr->total_planes;
+    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const int j = find_chunk(A.plane_prefix, n, item);
         const ChunkMeta& M = A.meta[j];
-        const int64_t r = g - A.row_prefix[j];
-        uint32_t nv = 0, nt = 0;
-        if (M.any_air && M.any_solid) {
-            switch ((M.dim + 63) >> 6) {
-                case 1: count_row<1>(A, M, r, s_ntri, nv, nt); break;
-                case 2: count_row<2>(A, M, r, s_ntri, nv, nt); break;
-                case 3: count_row<3>(A, M, r, s_ntri, nv, nt); break;
-                default: count_row<4>(A, M, r, s_ntri, nv, nt); break;
-            }
+        const int dim = M.dim;
+        if (((dim + 63) >> 6) != W) continue;
+        const int x = (int)(item - A.plane_prefix[j]);
+        uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
+        uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
+        if (!(M.any_air && M.any_solid)) {  // no surface in this chunk
+            for (int y = threadIdx.x; y < dim; y += blockDim.x) rv[y] = rt[y] = 0u;
+            continue;
         }
-        if (lane == 0) {
-            A.row_v[M.row_off + r] = nv;
-            A.row_t[M.row_off + r] = nt;
+        __syncthreads();  // previous item's rows are done with s_lab
+        stage_planes(s_lab, A.labels + M.lab_off, M.lab_words, dim, x - 1);
+        __syncthreads();
+        for (int y = wave; y < dim; y += kRowThreads / 64) {
+            uint32_t nv = 0, nt = 0;
+            count_row<W>(A, M, s_lab, s_ntri, x, y, nv, nt);
+            if (lane == 0) {
+                rv[y] = nv;
+                rt[y] = nt;
+            }
         }
     }
 }
@@ -437,8 +687,6 @@ constexpr int kScanThreads = 1024;
 
 __global__ void __launch_bounds__(kScanThreads) k_scan(Arenas A, int n) {
     __shared__ uint32_t s_v[kScanThreads], s_t[kScanThreads];
-    __shared__ int64_t s_off[2];
-    __shared__ int s_ok;
     const int tid = threadIdx.x;
     for (int j = blockIdx.x; j < n; j += gridDim.x) {
         ChunkMeta& M = A.meta[j];
@@ -491,111 +739,106 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(Arenas A, int n) {
                 M.v_off = vo;
                 M.i_off = io;
             }
-            s_ok = ok;
-            s_off[0] = M.v_off;
-        }
-        __syncthreads();
-        // valence counters start at zero (filled by atomics in k_emit)
-        if (s_ok && M.iters > 0) {
-            const int64_t V = M.V, vo = s_off[0];
-            for (int64_t v = tid; v < V; v += kScanThreads) A.valence[vo + v] = 0u;
+            (void)ok;
         }
         __syncthreads();
     }
 }
 
-// tile prefixes for the smoothing kernels (one block)
+// Ordered list of the chunks that run the smoothing (K > 0, a mesh, fitted)
+// and the prefix of their vertex tiles, for k_smooth (one block).
+__device__ __forceinline__ bool smooths(const ChunkMeta& M) {
+    return M.iters > 0 && M.V > 0 && M.T > 0 && M.status == BMF_OK && M.state == kStateSampled;
+}
+
 __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
     __shared__ int64_t s_scan[2][kPlanThreads];
     const int tid = threadIdx.x;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
     const int j0 = tid * per, j1 = min(n, j0 + per);
-    int64_t sum[2] = {0, 0};
+    int64_t cnt = 0, tiles = 0;
     for (int j = j0; j < j1; j++) {
         const ChunkMeta& M = A.meta[j];
-        const bool smooth = M.iters > 0 && M.V > 0 && M.status == BMF_OK && M.state == kStateSampled;
-        sum[0] += smooth ? (M.V + kSmoothThreads - 1) / kSmoothThreads : 0;
-        sum[1] += smooth ? (M.T + kSmoothThreads - 1) / kSmoothThreads : 0;
+        if (smooths(M)) {
+            cnt++;
+            tiles += (M.V + kSmoothThreads - 1) / kSmoothThreads;
+        }
     }
-    s_scan[0][tid] = sum[0];
-    s_scan[1][tid] = sum[1];
+    s_scan[0][tid] = cnt;
+    s_scan[1][tid] = tiles;
     __syncthreads();
     for (int off = 1; off < kPlanThreads; off <<= 1) {
-        int64_t a = tid >= off ? s_scan[0][tid - off] : 0, b = tid >= off ? s_scan[1][tid - off] : 0;
+        const int64_t a = tid >= off ? s_scan[0][tid - off] : 0, b = tid >= off ? s_scan[1][tid - off] : 0;
         __syncthreads();
         s_scan[0][tid] += a;
         s_scan[1][tid] += b;
         __syncthreads();
     }
-    int64_t bv = s_scan[0][tid] - sum[0], bt = s_scan[1][tid] - sum[1];
+    int64_t k = s_scan[0][tid] - cnt, tp = s_scan[1][tid] - tiles;
     for (int j = j0; j < j1; j++) {
         const ChunkMeta& M = A.meta[j];
-        const bool smooth = M.iters > 0 && M.V > 0 && M.status == BMF_OK && M.state == kStateSampled;
-        A.vtile_prefix[j] = bv;
-        A.ttile_prefix[j] = bt;
-        bv += smooth ? (M.V + kSmoothThreads - 1) / kSmoothThreads : 0;
-        bt += smooth ? (M.T + kSmoothThreads - 1) / kSmoothThreads : 0;
+        if (!smooths(M)) continue;
+        A.smooth_list[k] = j;
+        A.smooth_prefix[k] = tp;
+        k++;
+        tp += (M.V + kSmoothThreads - 1) / kSmoothThreads;
     }
-    if (tid == 0) {
-        A.vtile_prefix[n] = s_scan[0][kPlanThreads - 1];
-        A.ttile_prefix[n] = s_scan[1][kPlanThreads - 1];
-        A.ctr->total_vtiles = s_scan[0][kPlanThreads - 1];
-        A.ctr->total_ttiles = s_scan[1][kPlanThreads - 1];
+    if (tid == kPlanThreads - 1) {
+        A.ctr->n_smooth = (int32_t)s_scan[0][tid];
+        A.smooth_prefix[s_scan[0][tid]] = s_scan[1][tid];
+        A.ctr->total_vtiles = s_scan[1][tid];
     }
 }
 
 // ------------------------------------------------------------------------
-// emit: one wave per label row; lanes = z.  Vertices of the row's samples
-// (ids by popcount prefix over the row's edge masks), triangles of the
-// row's interior cells (ids by a bit-sliced ballot prefix of the per-cell
-// triangle counts), adjacency by atomic append (sorted afterwards).
+// emit: one block per (chunk, x-plane), one wave per label row, lanes = z.
+// Vertices of the row's samples (ids by popcount prefix over the row's edge
+// masks), triangles of the row's interior cells (ids by a bit-sliced ballot
+// prefix of the per-cell triangle counts), and for chunks that smooth the
+// per-cell triangle base and mask.
 // ------------------------------------------------------------------------
 template <int W>
-__device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, int64_t r, const uint8_t* s_ntri,
-                                         uint32_t (*s_vid)[64]) {
+__device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, const uint64_t (*s_lab)[kMaxPlaneWords],
+                                         const uint8_t* s_ntri, const uint32_t (*s_base)[BMF_MAX_DIM + 1],
+                                         uint32_t (*s_vid)[64], int x, int y) {
     const int dim = M.dim;
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
-    const int x = (int)(r / dim), y = (int)(r - (int64_t)x * dim);
-    const uint64_t* L = A.labels + M.lab_off;
-    const int64_t nwd = M.lab_words;
-    // label rows (x+a, y+b)
-    const RowBits<W> R00 = load_row<W>(L, nwd, dim, x, y), R01 = load_row<W>(L, nwd, dim, x, y + 1);
-    const RowBits<W> R10 = load_row<W>(L, nwd, dim, x + 1, y), R11 = load_row<W>(L, nwd, dim, x + 1, y + 1);
+    const bool x1 = x + 1 < dim, x2 = x + 2 < dim, y1 = y + 1 < dim, y2 = y + 2 < dim;
+    const RowBits<W> R00 = lds_row<W>(s_lab[0], dim, y, true), R01 = lds_row<W>(s_lab[0], dim, y + 1, y1);
+    const RowBits<W> R10 = lds_row<W>(s_lab[1], dim, y, x1), R11 = lds_row<W>(s_lab[1], dim, y + 1, x1 && y1);
     const bool tris = x < dim - 1 && y < dim - 1 && !quad_uniform<W>(R00, R01, R10, R11, dim);
-    // edge masks of row (x, y)
-    RowBits<W> EX00 = rxor<W>(R00, R10, x + 1 < dim), EY00 = rxor<W>(R00, R01, y + 1 < dim);
-    RowBits<W> EZ00 = zedges<W>(R00, dim);
+    const RowBits<W> EX00 = rxor<W>(R00, R10, x1), EY00 = rxor<W>(R00, R01, y1), EZ00 = zedges<W>(R00, dim);
     const uint32_t nvr = total_pop<W>(EX00) + total_pop<W>(EY00) + total_pop<W>(EZ00);
     if (nvr == 0 && !tris) return;
-    // edge masks of rows (x, y+1), (x+1, y), (x+1, y+1)
+    // edge masks of rows (x, y+1), (x+1, y), (x+1, y+1); only needed for triangles
     RowBits<W> EX01 = zero_row<W>(), EY01 = zero_row<W>(), EZ01 = zero_row<W>();
     RowBits<W> EX10 = zero_row<W>(), EY10 = zero_row<W>(), EZ10 = zero_row<W>();
     RowBits<W> EX11 = zero_row<W>(), EY11 = zero_row<W>(), EZ11 = zero_row<W>();
-    uint32_t vb00 = A.row_v[M.row_off + r], vb01 = 0, vb10 = 0, vb11 = 0;
-    if (tris) {
-        const RowBits<W> R02 = load_row<W>(L, nwd, dim, x, y + 2), R12 = load_row<W>(L, nwd, dim, x + 1, y + 2);
-        const RowBits<W> R20 = load_row<W>(L, nwd, dim, x + 2, y), R21 = load_row<W>(L, nwd, dim, x + 2, y + 1);
-        EX01 = rxor<W>(R01, R11, x + 1 < dim);
-        EY01 = rxor<W>(R01, R02, y + 2 < dim);
+    if (tris) {  // here x + 1 < dim and y + 1 < dim
+        const RowBits<W> R02 = lds_row<W>(s_lab[0], dim, y + 2, y2), R12 = lds_row<W>(s_lab[1], dim, y + 2, y2);
+        const RowBits<W> R20 = lds_row<W>(s_lab[2], dim, y, x2), R21 = lds_row<W>(s_lab[2], dim, y + 1, x2);
+        EX01 = rxor<W>(R01, R11, true);
+        EY01 = rxor<W>(R01, R02, y2);
         EZ01 = zedges<W>(R01, dim);
-        EX10 = rxor<W>(R10, R20, x + 2 < dim);
-        EY10 = rxor<W>(R10, R11, y + 1 < dim);
+        EX10 = rxor<W>(R10, R20, x2);
+        EY10 = rxor<W>(R10, R11, true);
         EZ10 = zedges<W>(R10, dim);
-        EX11 = rxor<W>(R11, R21, x + 2 < dim);
-        EY11 = rxor<W>(R11, R12, y + 2 < dim);
+        EX11 = rxor<W>(R11, R21, x2);
+        EY11 = rxor<W>(R11, R12, y2);
         EZ11 = zedges<W>(R11, dim);
-        // x, y < dim-1 here, so all four rows exist
-        vb01 = A.row_v[M.row_off + r + 1];
-        vb10 = A.row_v[M.row_off + r + dim];
-        vb11 = A.row_v[M.row_off + r + dim + 1];
     }
+    const uint32_t vb00 = s_base[0][y], vb01 = s_base[0][y + 1], vb10 = s_base[1][y], vb11 = s_base[1][y + 1];
     const int64_t vo = M.v_off, io = M.i_off;
-    float* __restrict__ P = A.pos + vo * 3;
+    const uint32_t Vc = (uint32_t)M.V, Tc = (uint32_t)M.T;
+    // chunks with an odd iteration count start from the scratch buffer so that
+    // the last smoothing step lands in the output (see k_smooth)
+    float* __restrict__ P = ((M.iters & 1) ? A.pos_tmp : A.pos) + vo * 3;
     uint8_t* __restrict__ B = A.boundary + vo;
+    uint32_t* __restrict__ E = A.vedge + vo;
     const float* __restrict__ D = A.dens + M.dens_off;
-    const bool adjacency = M.iters > 0;
-    uint32_t tcarry = A.row_t[M.row_off + r];
+    const bool cells = M.iters > 0;
+    uint32_t tcarry = s_base[2][y];
     for (int z0 = 0; z0 < dim; z0 += 64) {
         const int z = z0 + lane;
         // ---- vertices owned by sample (x, y, z): x, y, z edges in order ----
@@ -608,26 +851,27 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, in
                 for (int ax = 0; ax < 3; ax++) {
                     const uint32_t f = ax == 0 ? fx : (ax == 1 ? fy : fz);
                     if (!f) continue;
-                    if (id >= (uint32_t)M.V) {  // counts and ids disagree: report, never write past the chunk
+                    if (id >= Vc) {  // counts and ids disagree: report, never write past the chunk
                         A.ctr->error = 2;
                         break;
                     }
-                    const int x1 = x + (ax == 0), y1 = y + (ax == 1), z1 = z + (ax == 2);
-                    const float s1 = D[((int64_t)x1 * dim + y1) * dim + z1];
+                    const int xe = x + (ax == 0), ye = y + (ax == 1), ze = z + (ax == 2);
+                    const float s1 = D[((int64_t)xe * dim + ye) * dim + ze];
                     // DMCChunk::calculate_isovertex / _get_intersection (DMCChunk.cpp:657-674)
                     const float mu = (0.0f - s0) / (s1 - s0);
-                    const float dx = (float)x1 - (float)x, dy = (float)y1 - (float)y, dz = (float)z1 - (float)z;
+                    const float dx = (float)xe - (float)x, dy = (float)ye - (float)y, dz = (float)ze - (float)z;
                     P[3 * (int64_t)id + 0] = dx * mu + (float)x;
                     P[3 * (int64_t)id + 1] = dy * mu + (float)y;
                     P[3 * (int64_t)id + 2] = dz * mu + (float)z;
                     B[id] = (uint8_t)(x == 0 || y == 0 || z == 0 || x == dim - 1 || y == dim - 1 || z == dim - 1 ||
-                                      x1 == dim - 1 || y1 == dim - 1 || z1 == dim - 1);
+                                      xe == dim - 1 || ye == dim - 1 || ze == dim - 1);
+                    E[id] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | ((uint32_t)ax << 24);
                     id++;
                 }
             }
         }
         // ---- triangles of interior cell (x, y, z) ----
-        if (tris) {
+        if (tris && z0 < dim - 1) {
             const uint32_t mask = z < dim - 1 ? cell_mask<W>(R00, R01, R10, R11, z) : 0u;
             const uint32_t nt = s_ntri[mask];
             const uint64_t b0 = __ballot(nt & 1u), b1 = __ballot(nt & 2u), b2 = __ballot(nt & 4u);
@@ -637,20 +881,20 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, in
                 // vertex ids of the 12 local edges (DMCChunk.cpp:543-565):
                 // x-edges e = 2*dy + dz on rows (x, y+dy); y-edges e = 4 + 2*dx + dz
                 // on rows (x+dx, y); z-edges e = 8 + 2*dx + dy on rows (x+dx, y+dy)
-#define BMF_VIDS(EX, EY, EZ, VB, DX, DY)                                                          \
-    {                                                                                            \
+#define BMF_VIDS(EX, EY, EZ, VB, DX, DY)                                                              \
+    {                                                                                                \
         const uint32_t p0 = VB + prefix_pop<W>(EX, z) + prefix_pop<W>(EY, z) + prefix_pop<W>(EZ, z); \
-        const uint32_t ex0 = bit_at<W>(EX, z), ey0 = bit_at<W>(EY, z), ez0 = bit_at<W>(EZ, z);   \
-        const uint32_t p1 = p0 + ex0 + ey0 + ez0;                                                \
-        if (DX == 0) {                                                                           \
-            s_vid[2 * DY + 0][lane] = p0;                                                        \
-            s_vid[2 * DY + 1][lane] = p1;                                                        \
-        }                                                                                        \
-        if (DY == 0) {                                                                           \
-            s_vid[4 + 2 * DX + 0][lane] = p0 + ex0;                                              \
-            s_vid[4 + 2 * DX + 1][lane] = p1 + bit_at<W>(EX, z + 1);                             \
-        }                                                                                        \
-        s_vid[8 + 2 * DX + DY][lane] = p0 + ex0 + ey0;                                           \
+        const uint32_t ex0 = bit_at<W>(EX, z), ey0 = bit_at<W>(EY, z), ez0 = bit_at<W>(EZ, z);       \
+        const uint32_t p1 = p0 + ex0 + ey0 + ez0;                                                    \
+        if (DX == 0) {                                                                               \
+            s_vid[2 * DY + 0][lane] = p0;                                                            \
+            s_vid[2 * DY + 1][lane] = p1;                                                            \
+        }                                                                                            \
+        if (DY == 0) {                                                                               \
+            s_vid[4 + 2 * DX + 0][lane] = p0 + ex0;                                                  \
+            s_vid[4 + 2 * DX + 1][lane] = p1 + bit_at<W>(EX, z + 1);                                 \
+        }                                                                                            \
+        s_vid[8 + 2 * DX + DY][lane] = p0 + ex0 + ey0;                                               \
     }
                 BMF_VIDS(EX00, EY00, EZ00, vb00, 0, 0)
                 BMF_VIDS(EX01, EY01, EZ01, vb01, 0, 1)
@@ -660,23 +904,22 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, in
                 const uint32_t tb = tcarry + pre;
                 const uint64_t row = c_tri_table[mask];
                 const int cnt = (int)(row >> 60);
-                if (tb + (uint32_t)(cnt / 3) > (uint32_t)M.T) {
+                if (tb + (uint32_t)(cnt / 3) > Tc) {
                     A.ctr->error = 3;
-                    continue;
-                }
-                uint32_t* __restrict__ I = A.idx + io + 3ll * tb;
-                for (int s = 0; s < cnt; s++) {
-                    const int e = (int)((row >> (4 * s)) & 15ull);
-                    const uint32_t v = s_vid[e][lane];
-                    if (v >= (uint32_t)M.V) {
-                        A.ctr->error = 4;
-                        break;
+                } else {
+                    uint32_t* __restrict__ I = A.idx + io + 3ll * tb;
+                    for (int s = 0; s < cnt; s++) {
+                        const uint32_t v = s_vid[(row >> (4 * s)) & 15ull][lane];
+                        if (v >= Vc) {
+                            A.ctr->error = 4;
+                            break;
+                        }
+                        I[s] = v;
                     }
-                    I[s] = v;
-                    if (adjacency) {
-                        const uint32_t k = atomicAdd(&A.valence[vo + v], 1u);
-                        if (k < BMF_MAX_VALENCE) A.adj[(vo + v) * BMF_MAX_VALENCE + k] = tb + (uint32_t)(s / 3);
-                        else A.ctr->error = 1;
+                    if (cells) {
+                        const int64_t ci = M.dens_off + ((int64_t)x * dim + y) * dim + z;
+                        A.cell_tb[ci] = tb;
+                        A.cell_mask[ci] = (uint8_t)mask;
                     }
                 }
             }
@@ -685,124 +928,119 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, in
     }
 }
 
+template <int W>
 __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
     __shared__ uint8_t s_ntri[256];
+    __shared__ uint64_t s_lab[3][kMaxPlaneWords];
+    __shared__ uint32_t s_base[3][BMF_MAX_DIM + 1];  // row vertex bases of planes x, x+1; triangle bases of plane x
     __shared__ uint32_t s_vid[kRowThreads / 64][12][64];
     for (int t = threadIdx.x; t < 256; t += blockDim.x) s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
-    __syncthreads();
     const int wave = uniform_int(threadIdx.x >> 6);
-    const int64_t nw = (int64_t)gridDim.x * (kRowThreads / 64);
-    const int64_t total = A.ctr->total_rows;
-    for (int64_t g = (int64_t)blockIdx.x * (kRowThreads / 64) + wave; g < total; g += nw) {
-        const int j = find_chunk(A.row_prefix, n, g);
+    const int64_t total = A.ctr->total_planes;
+    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const int j = find_chunk(A.plane_prefix, n, item);
         const ChunkMeta& M = A.meta[j];
-        if (M.status != BMF_OK || M.V == 0) continue;
-        const int64_t r = g - A.row_prefix[j];
-        switch ((M.dim + 63) >> 6) {
-            case 1: emit_row<1>(A, M, r, s_ntri, s_vid[wave]); break;
-            case 2: emit_row<2>(A, M, r, s_ntri, s_vid[wave]); break;
-            case 3: emit_row<3>(A, M, r, s_ntri, s_vid[wave]); break;
-            default: emit_row<4>(A, M, r, s_ntri, s_vid[wave]); break;
+        const int dim = M.dim;
+        if (((dim + 63) >> 6) != W) continue;
+        if (M.status != BMF_OK || M.V == 0) continue;  // uniform across the block
+        const int x = (int)(item - A.plane_prefix[j]);
+        __syncthreads();  // previous item is done with the shared arrays
+        stage_planes(s_lab, A.labels + M.lab_off, M.lab_words, dim, x);
+        const uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
+        const uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
+        for (int y = threadIdx.x; y <= dim; y += blockDim.x) {
+            s_base[0][y] = y < dim ? rv[y] : 0u;
+            s_base[1][y] = (y < dim && x + 1 < dim) ? rv[dim + y] : 0u;
+            s_base[2][y] = y < dim ? rt[y] : 0u;
         }
+        __syncthreads();
+        for (int y = wave; y < dim; y += kRowThreads / 64) emit_row<W>(A, M, s_lab, s_ntri, s_base, s_vid[wave], x, y);
     }
 }
 
 // ------------------------------------------------------------------------
-// sort each vertex's adjacency list into ascending triangle id
-// (MeshProcessor::init_primitives appends in triangle order, :114-127)
+// smooth: iteration m of MeshProcessor<3>'s dual/primal smoothing
+// (MeshProcessor.cpp:130-306 as driven by ChunkGenerator.cpp:112-124) for
+// every vertex of the chunks with K > m, one thread per vertex.  The
+// vertex's adjacent triangles come in ascending id from the ring of cells
+// around its lattice edge (c_ring) and each cell's table slots; per
+// triangle the centroid ((0 + p0) + p1) + p2 then / 3, summed from +0 in
+// that order, then / count.  Boundary vertices stay unless
+// boundary_processing.  Positions ping-pong between the output and a
+// scratch buffer: chunk j reads buffer (m + K_j) & 1 and writes the other,
+// so its last iteration writes the output.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kSmoothThreads) k_sort_adj(Arenas A, int n) {
+__global__ void __launch_bounds__(kSmoothThreads) k_smooth(Arenas A, int m) {
     const int64_t total = A.ctr->total_vtiles;
+    const int nsm = A.ctr->n_smooth;
     for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
-        const int j = find_chunk(A.vtile_prefix, n, tile);
-        const ChunkMeta& M = A.meta[j];
-        const int64_t v = (tile - A.vtile_prefix[j]) * kSmoothThreads + threadIdx.x;
-        if (v >= M.V) continue;
-        const int64_t gv = M.v_off + v;
-        const uint32_t cnt = min(A.valence[gv], (uint32_t)BMF_MAX_VALENCE);
-        uint32_t* __restrict__ lst = A.adj + gv * BMF_MAX_VALENCE;
-        uint32_t a[BMF_MAX_VALENCE];
-#pragma unroll
-        for (int k = 0; k < BMF_MAX_VALENCE; k++) a[k] = (uint32_t)k < cnt ? lst[k] : 0xFFFFFFFFu;
-        // odd-even transposition sort with compile-time indices
-#pragma unroll
-        for (int rnd = 0; rnd < BMF_MAX_VALENCE; rnd++)
-#pragma unroll
-            for (int k = rnd & 1; k + 1 < BMF_MAX_VALENCE; k += 2) {
-                const uint32_t lo = min(a[k], a[k + 1]), hi = max(a[k], a[k + 1]);
-                a[k] = lo;
-                a[k + 1] = hi;
+        const int k = find_chunk(A.smooth_prefix, nsm, tile);
+        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int K = M.iters;
+        if (K <= m) continue;
+        const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
+        const int64_t V = M.V;
+        if (v >= V) continue;
+        const int dim = M.dim;
+        const int64_t T = M.T, vo = M.v_off;
+        const float* __restrict__ src = (((m + K) & 1) ? A.pos_tmp : A.pos) + vo * 3;
+        float* __restrict__ dst = (((m + K) & 1) ? A.pos : A.pos_tmp) + vo * 3;
+        float px = src[3 * v + 0], py = src[3 * v + 1], pz = src[3 * v + 2];
+        if (M.bp || !A.boundary[vo + v]) {
+            const uint32_t* __restrict__ I = A.idx + M.i_off;
+            const uint32_t* __restrict__ CT = A.cell_tb + M.dens_off;
+            const uint8_t* __restrict__ CM = A.cell_mask + M.dens_off;
+            const uint32_t code = A.vedge[vo + v];
+            const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
+            const int ax = (int)(code >> 24);
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            uint32_t cnt = 0;
+            for (int c = 0; c < 4; c++) {
+                const int cx = ex + c_ring[ax][c][0], cy = ey + c_ring[ax][c][1], cz = ez + c_ring[ax][c][2];
+                if (cx < 0 || cy < 0 || cz < 0 || cx > dim - 2 || cy > dim - 2 || cz > dim - 2) continue;
+                const int64_t ci = ((int64_t)cx * dim + cy) * dim + cz;
+                const uint32_t le = (uint32_t)c_ring[ax][c][3];
+                const uint32_t tb = CT[ci];
+                const uint64_t row = c_tri_table[CM[ci]];
+                const int ns = (int)(row >> 60);
+                for (int s = 0; s < ns; s++) {
+                    if (((row >> (4 * s)) & 15ull) != le) continue;
+                    const int64_t t = (int64_t)tb + s / 3;
+                    if (t >= T) {
+                        A.ctr->error = 7;
+                        break;
+                    }
+                    const uint32_t i0 = I[3 * t], i1 = I[3 * t + 1], i2 = I[3 * t + 2];
+                    if (i0 >= V || i1 >= V || i2 >= V) {
+                        A.ctr->error = 5;
+                        break;
+                    }
+                    float qx = 0.0f, qy = 0.0f, qz = 0.0f;
+                    qx = qx + src[3 * (int64_t)i0 + 0];
+                    qy = qy + src[3 * (int64_t)i0 + 1];
+                    qz = qz + src[3 * (int64_t)i0 + 2];
+                    qx = qx + src[3 * (int64_t)i1 + 0];
+                    qy = qy + src[3 * (int64_t)i1 + 1];
+                    qz = qz + src[3 * (int64_t)i1 + 2];
+                    qx = qx + src[3 * (int64_t)i2 + 0];
+                    qy = qy + src[3 * (int64_t)i2 + 1];
+                    qz = qz + src[3 * (int64_t)i2 + 2];
+                    sx = sx + qx / 3.0f;
+                    sy = sy + qy / 3.0f;
+                    sz = sz + qz / 3.0f;
+                    cnt++;
+                }
             }
-#pragma unroll
-        for (int k = 0; k < BMF_MAX_VALENCE; k++)
-            if ((uint32_t)k < cnt) lst[k] = a[k];
-    }
-}
-
-// ------------------------------------------------------------------------
-// smoothing: dual step per triangle, primal step per vertex
-// (MeshProcessor.cpp:130-306), iteration m of chunks with iters > m
-// ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int n, int m) {
-    const int64_t total = A.ctr->total_ttiles;
-    for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
-        const int j = find_chunk(A.ttile_prefix, n, tile);
-        const ChunkMeta& M = A.meta[j];
-        if (M.iters <= m) continue;
-        const int64_t t = (tile - A.ttile_prefix[j]) * kSmoothThreads + threadIdx.x;
-        if (t >= M.T) continue;
-        const uint32_t* __restrict__ I = A.idx + M.i_off + 3 * t;
-        const float* __restrict__ P = A.pos + M.v_off * 3;
-        const uint32_t i0 = I[0], i1 = I[1], i2 = I[2];
-        if (i0 >= (uint32_t)M.V || i1 >= (uint32_t)M.V || i2 >= (uint32_t)M.V) {
-            A.ctr->error = 5;
-            continue;
-        }
-        float* __restrict__ C = A.cen + M.i_off + 3 * t;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            float acc = 0.0f;
-            acc = acc + P[3 * (int64_t)i0 + c];
-            acc = acc + P[3 * (int64_t)i1 + c];
-            acc = acc + P[3 * (int64_t)i2 + c];
-            C[c] = acc / 3.0f;
-        }
-    }
-}
-
-__global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int n, int m) {
-    const int64_t total = A.ctr->total_vtiles;
-    for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
-        const int j = find_chunk(A.vtile_prefix, n, tile);
-        const ChunkMeta& M = A.meta[j];
-        if (M.iters <= m) continue;
-        const int64_t v = (tile - A.vtile_prefix[j]) * kSmoothThreads + threadIdx.x;
-        if (v >= M.V) continue;
-        const int64_t gv = M.v_off + v;
-        const uint32_t cnt = A.valence[gv];
-        if (cnt == 0 || (!M.bp && A.boundary[gv])) continue;
-        const uint32_t* __restrict__ lst = A.adj + gv * BMF_MAX_VALENCE;
-        const float* __restrict__ C = A.cen + M.i_off;
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        if (cnt > BMF_MAX_VALENCE) {
-            A.ctr->error = 6;
-            continue;
-        }
-        for (uint32_t k = 0; k < cnt; k++) {
-            const int64_t t = lst[k];
-            if (t >= M.T) {
-                A.ctr->error = 7;
-                break;
+            if (cnt) {
+                const float fc = (float)cnt;
+                px = sx / fc;
+                py = sy / fc;
+                pz = sz / fc;
             }
-            sx = sx + C[3 * t + 0];
-            sy = sy + C[3 * t + 1];
-            sz = sz + C[3 * t + 2];
         }
-        const float fc = (float)cnt;
-        float* __restrict__ P = A.pos + gv * 3;
-        P[0] = sx / fc;
-        P[1] = sy / fc;
-        P[2] = sz / fc;
+        dst[3 * v + 0] = px;
+        dst[3 * v + 1] = py;
+        dst[3 * v + 2] = pz;
     }
 }
 
@@ -899,12 +1137,15 @@ hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, 
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, s, A, n, allow_cull, reset_bumps);
     return hipGetLastError();
 }
-hipError_t launch_sample(const Arenas& A, int n, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_sample, dim3(blocks), dim3(kSampleThreads), 0, s, A, n);
+hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, hipStream_t s) {
+    hipLaunchKernelGGL(k_sample, dim3(blocks), dim3(kSampleThreads), 0, s, A, n, adaptive);
     return hipGetLastError();
 }
-hipError_t launch_count(const Arenas& A, int n, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_count, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
+    if (wclasses & 1) hipLaunchKernelGGL(k_count<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_count<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_count<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_count<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
     return hipGetLastError();
 }
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s) {
@@ -912,20 +1153,15 @@ hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_plan_smooth, dim3(1), dim3(kPlanThreads), 0, s, A, n);
     return hipGetLastError();
 }
-hipError_t launch_emit(const Arenas& A, int n, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
+    if (wclasses & 1) hipLaunchKernelGGL(k_emit<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_emit<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_emit<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_emit<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
     return hipGetLastError();
 }
-hipError_t launch_sort_adj(const Arenas& A, int n, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_sort_adj, dim3(blocks), dim3(kSmoothThreads), 0, s, A, n);
-    return hipGetLastError();
-}
-hipError_t launch_dual(const Arenas& A, int n, int m, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_dual, dim3(blocks), dim3(kSmoothThreads), 0, s, A, n, m);
-    return hipGetLastError();
-}
-hipError_t launch_primal(const Arenas& A, int n, int m, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_primal, dim3(blocks), dim3(kSmoothThreads), 0, s, A, n, m);
+hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_smooth, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
     return hipGetLastError();
 }
 hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s) {

@@ -1,0 +1,71 @@
+// Runs ChunkGenerator::process_queue over a batch of terrain chunks (the
+// reference's world-update path, headless) and prints one JSON line per
+// chunk: contains_mesh, vertex and index counts, and the mesh checksum
+// (sum over the u32 stream of position bit patterns then indices of
+// word_k * (2k + 1), times 0x9E3779B97F4A7C15, modulo 2^64).
+//   bmf_host_demo <n_side> <resolution> <iters> <mode: 0 plain, 1 batched>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "bmf_host.hpp"
+
+using namespace bmf_host;
+
+int main(int argc, char** argv) {
+    const int n_side = argc > 1 ? std::atoi(argv[1]) : 2;
+    const int res = argc > 2 ? std::atoi(argv[2]) : 33;
+    const int iters = argc > 3 ? std::atoi(argv[3]) : 2;
+    const int mode = argc > 4 ? std::atoi(argv[4]) : BMF_MODE_BATCHED;
+    WorldProperties props;
+    props.chunk_resolution = res;
+    props.process_iters = iters;
+    Sampler sampler;
+    NoiseSamplers::create_sampler_terrain_fbm_3d(&sampler, NoiseSamplers::TerrainProperties{});
+    ChunkGenerator gen;
+    gen.mode = mode;
+    int rc = gen.init(props, sampler);
+    if (rc) {
+        std::fprintf(stderr, "init: %s\n", bmf_strerror(rc));
+        return 1;
+    }
+    const float size = (float)(res - 1);
+    std::vector<std::unique_ptr<DMCChunk>> chunks;
+    std::vector<std::unique_ptr<WorldOctreeNode>> nodes;
+    std::vector<WorldOctreeNode*> batch;
+    for (int i = 0; i < n_side; i++)
+        for (int j = 0; j < n_side; j++)
+            for (int k = 0; k < n_side; k++) {
+                chunks.emplace_back(new DMCChunk());
+                chunks.back()->init(vec3(size * i, size * j, size * k), size, props.max_level, sampler);
+                chunks.back()->dim = (uint32_t)res;
+                nodes.emplace_back(new WorldOctreeNode());
+                nodes.back()->chunk = chunks.back().get();
+                nodes.back()->level = props.max_level;
+                batch.push_back(nodes.back().get());
+            }
+    rc = gen.process_queue(batch);
+    if (rc) {
+        std::fprintf(stderr, "process_queue: %s\n", bmf_strerror(rc));
+        return 1;
+    }
+    for (size_t c = 0; c < batch.size(); c++) {
+        const DMCChunk* ch = batch[c]->chunk;
+        uint64_t h = 0, k = 0;
+        size_t nv = ch->vi ? ch->vi->vertices.size() : 0, ni = ch->vi ? ch->vi->mesh_indexes.size() : 0;
+        for (size_t v = 0; v < nv; v++) {
+            const float* f = &ch->vi->vertices[v].p.x;
+            for (int q = 0; q < 3; q++, k++) {
+                uint32_t b;
+                std::memcpy(&b, &f[q], 4);
+                h += (uint64_t)b * (2 * k + 1);
+            }
+        }
+        for (size_t q = 0; q < ni; q++, k++) h += (uint64_t)ch->vi->mesh_indexes[q] * (2 * k + 1);
+        h *= 0x9E3779B97F4A7C15ull;
+        std::printf("{\"chunk\": %zu, \"contains_mesh\": %d, \"V\": %zu, \"I\": %zu, \"stage\": %d, \"checksum\": %llu}\n",
+                    c, (int)ch->contains_mesh, nv, ni, batch[c]->generation_stage, (unsigned long long)h);
+    }
+    return 0;
+}

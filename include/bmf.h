@@ -41,7 +41,6 @@ extern "C" {
 
 #define BMF_MAX_DIM 256
 #define BMF_MAX_OCTAVES 16
-#define BMF_MAX_VALENCE 20 /* 4 cells around an edge x at most 5 uses per cell */
 
 typedef struct bmf_ctx bmf_ctx;
 
@@ -164,10 +163,10 @@ int bmf_last_timing(bmf_ctx* ctx, float* total_ms);
 
 /* Runs the BATCHED path once without a graph, with HIP events between the
  * stages on the context stream, and returns the stage times in ms:
- * [0] plan, [1] sample, [2] count, [3] scan, [4] emit, [5] adjacency sort,
- * [6] smoothing (all iterations), [7] finalize.  The per-stage counterpart of
- * the reference's stage timers (DMCChunk.cpp:785-840, ChunkGenerator.cpp:153-193). */
-#define BMF_NUM_STAGES 8
+ * [0] plan, [1] sample, [2] count, [3] scan, [4] emit, [5] smoothing (all
+ * iterations), [6] finalize.  The per-stage counterpart of the reference's
+ * stage timers (DMCChunk.cpp:785-840, ChunkGenerator.cpp:153-193). */
+#define BMF_NUM_STAGES 7
 int bmf_profile_stages(bmf_ctx* ctx, const bmf_chunk_desc* chunks, int n, float* stage_ms);
 
 #ifdef __cplusplus

@@ -230,3 +230,34 @@ def test_invalid_descriptors_rejected(ctx):
     # the context still works afterwards
     ctx.extract(workloads.to_capi([CHUNKS[5]]), capi.MODE_BATCHED)
     assert ctx.results(1)[0] == capi.OK
+
+
+@pytest.mark.parametrize("params", [
+    dict(octaves=1),
+    dict(octaves=7, freq=0.005),
+    dict(octaves=16, freq=0.01, gain=0.6),
+    dict(amp=-24.0),                       # density decreasing in fbm
+    dict(gain=-0.5),                       # alternating amplitude signs
+    dict(amp=3.0, y0=20.0, freq=0.1, lacunarity=2.7, seed=99),
+], ids=["o1", "o7", "o16", "neg-amp", "neg-gain", "steep"])
+def test_adaptive_octaves_match_full_evaluation(ctx, oracle, params):
+    """The extraction path decides labels before the last octave where the
+    remaining amplitude cannot flip them and finishes only the densities of
+    crossing-edge endpoints; the mesh must equal the oracle's full
+    evaluation bit for bit."""
+    s = dict(TERRAIN)
+    s.update(params)
+    ctx.set_sampler(**s)
+    cs = [workloads.Chunk((0.0, 0.0, 0.0), 64.0, 65, 0.0, 2, 0),
+          workloads.Chunk((100.0, 10.0, -30.0), 96.0, 49, 0.0, 1, 0),
+          workloads.Chunk((7.0, 25.0, 3.0), 32.0, 33, 0.0, 0, 0)]
+    ctx.extract(workloads.to_capi(cs), capi.MODE_BATCHED)
+    rc, res, _, _ = ctx.results(len(cs))
+    assert rc == capi.OK
+    spec = oracle.SamplerSpec(**s)
+    for j, c in enumerate(cs):
+        pos, idx, bnd = ctx.mesh(j)
+        _check_mesh(oracle, spec, c, pos, idx, bnd, res[j])
+        if len(pos):
+            ref = oracle.extract_chunk(spec, _ochunk(oracle, c))
+            assert np.array_equal(pos.view(np.uint32), ref["pos"].view(np.uint32))
