@@ -71,6 +71,8 @@ struct Counters {
     int64_t total_planes;
     int64_t total_samples;
     int64_t total_vtiles;
+    int64_t total_ttiles;
+    unsigned long long refine_n;
     int32_t n_smooth;
     int32_t overflow;
     int32_t error;
@@ -85,6 +87,8 @@ struct Arenas {
     int64_t* plane_prefix;     // [max_chunks + 1] x-planes of the sampled chunks
     int32_t* smooth_list;      // [max_chunks] chunks that run the smoothing
     int64_t* smooth_prefix;    // [max_chunks + 1] vertex tiles of the listed chunks
+    int64_t* smooth_tprefix;   // [max_chunks + 1] triangle tiles of the listed chunks
+    uint64_t* refine;          // [max_samples] queued samples: chunk << 32 | sample index
     float* dens;               // [max_samples]
     uint64_t* labels;          // [max_samples / 64 + max_chunks]
     uint64_t* exact;           // same layout as labels: density of the sample is final
@@ -93,7 +97,8 @@ struct Arenas {
     uint32_t* cell_tb;         // [max_samples] first triangle of each surface cell
     uint8_t* cell_mask;        // [max_samples] corner mask of each surface cell
     float* pos;                // [max_vertices * 3]
-    float* pos_tmp;            // [max_vertices * 3] smoothing ping-pong
+    float4* pos4;              // [max_vertices] positions during the smoothing
+    float4* cen4;              // [max_indices / 3] triangle centroids
     uint8_t* boundary;         // [max_vertices]
     uint32_t* vedge;           // [max_vertices] lattice edge of each vertex: x | y<<8 | z<<16 | axis<<24
     uint32_t* idx;             // [max_indices]
@@ -113,6 +118,7 @@ hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, hipSt
 hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
+hipError_t launch_refine(const Arenas& A, int blocks, hipStream_t s);
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s);
 hipError_t launch_ref_labels(const Arenas& A, uint32_t* words, hipStream_t s);

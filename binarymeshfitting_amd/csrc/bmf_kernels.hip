@@ -329,6 +329,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         c->total_tiles = overflow ? 0 : tot_tiles;
         c->total_planes = overflow ? 0 : tot_planes;
         c->n_smooth = 0;
+        c->refine_n = 0;
         c->total_samples = tot_samples;
     }
 }
@@ -596,12 +597,13 @@ __device__ __forceinline__ RowBits<W> ror(const RowBits<W>& a, const RowBits<W>&
 }
 
 // Staged planes: s_lab[0] = x-1, s_lab[1] = x, s_lab[2] = x+1.  Counts the
-// row's vertices and triangles, and finishes the densities of the row's
-// samples that are endpoints of sign-changing edges but were labelled
-// before their last octave (k_sample's adaptive evaluation).
+// row's vertices and triangles, and queues for k_refine the row's samples
+// that are endpoints of sign-changing edges but were labelled before their
+// last octave (k_sample's adaptive evaluation).
 template <int W>
 __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, const uint64_t (*s_lab)[kMaxPlaneWords],
-                                          const uint8_t* s_ntri, int x, int y, uint32_t& nv, uint32_t& nt) {
+                                          const uint8_t* s_ntri, int x, int y, uint32_t* s_q, uint32_t* s_qn,
+                                          uint32_t& nv, uint32_t& nt) {
     const int dim = M.dim;
     const int lane = lane_id();
     const bool x1 = x + 1 < dim, y1 = y + 1 < dim;
@@ -631,16 +633,38 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
 #pragma unroll
     for (int k = 0; k < W; k++) need.w[k] = ep.w[k] & ~done.w[k];
     if (total_pop<W>(need) == 0) return;
-    const SamplerDev* __restrict__ S = A.sampler;
-    float* __restrict__ D = A.dens + M.dens_off + ((int64_t)x * dim + y) * dim;
-    const float px = (float)x * M.delta + M.opos[0];
-    const float py = (float)y * M.delta + M.opos[1];
+    // stage them in the block's LDS queue; k_count flushes it to the global
+    // refine queue with one reservation per flush, and k_refine evaluates
+    // them on dense waves
+    const uint64_t lt = lanemask_lt();
+    const uint32_t rowbase = (uint32_t)(((int64_t)x * dim + y) * dim);
     for (int z0 = 0; z0 < dim; z0 += 64) {
         const int z = z0 + lane;
-        if (z < dim && bit_at<W>(need, z)) {
-            const float pz = (float)z * M.delta + M.opos[2];
-            D[z] = sample_density(*S, px, py, pz);
-        }
+        const bool want = z < dim && bit_at<W>(need, z);
+        const uint64_t b = __ballot(want);
+        if (!b) continue;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(s_qn, (uint32_t)__popcll(b));
+        base = __shfl(base, 0);
+        if (want) s_q[base + __popcll(b & lt)] = rowbase + (uint32_t)z;
+    }
+}
+
+// ------------------------------------------------------------------------
+// refine: full density of the queued samples (endpoints of sign-changing
+// edges labelled early by k_sample), one sample per thread.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A) {
+    const SamplerDev* __restrict__ S = A.sampler;
+    const uint64_t total = A.ctr->refine_n;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t code = A.refine[e];
+        const ChunkMeta& M = A.meta[(int)(code >> 32)];
+        const uint32_t i = (uint32_t)code;
+        const uint32_t dim = (uint32_t)M.dim, dim2 = dim * dim;
+        float px, py, pz;
+        sample_coords(M, i, dim, dim2, 1.0f / (float)dim, 1.0f / (float)dim2, px, py, pz);
+        A.dens[M.dens_off + i] = sample_density(*S, px, py, pz);
     }
 }
 
@@ -650,6 +674,14 @@ template <int W>
 __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
     __shared__ uint8_t s_ntri[256];
     __shared__ uint64_t s_lab[3][kMaxPlaneWords];
+    // refine staging: one round of rows (4 rows of <= 256 samples) never
+    // adds more than kRefineRound entries, so flushing when the queue holds
+    // more than its capacity minus one round keeps it from overflowing
+    constexpr int kRefineRound = (kRowThreads / 64) * BMF_MAX_DIM;
+    constexpr int kRefineCap = 4 * kRefineRound;
+    __shared__ uint32_t s_q[kRefineCap];
+    __shared__ uint32_t s_qn;
+    __shared__ unsigned long long s_base;
     for (int t = threadIdx.x; t < 256; t += blockDim.x) s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
     const int wave = uniform_int(threadIdx.x >> 6);
     const int lane = lane_id();
@@ -666,15 +698,31 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
             for (int y = threadIdx.x; y < dim; y += blockDim.x) rv[y] = rt[y] = 0u;
             continue;
         }
-        __syncthreads();  // previous item's rows are done with s_lab
+        __syncthreads();  // previous item's rows are done with s_lab and s_q
+        if (threadIdx.x == 0) s_qn = 0;
         stage_planes(s_lab, A.labels + M.lab_off, M.lab_words, dim, x - 1);
         __syncthreads();
-        for (int y = wave; y < dim; y += kRowThreads / 64) {
-            uint32_t nv = 0, nt = 0;
-            count_row<W>(A, M, s_lab, s_ntri, x, y, nv, nt);
-            if (lane == 0) {
-                rv[y] = nv;
-                rt[y] = nt;
+        const int rounds = (dim + kRowThreads / 64 - 1) / (kRowThreads / 64);
+        for (int rd = 0; rd < rounds; rd++) {
+            const int y = rd * (kRowThreads / 64) + wave;
+            if (y < dim) {
+                uint32_t nv = 0, nt = 0;
+                count_row<W>(A, M, s_lab, s_ntri, x, y, s_q, &s_qn, nv, nt);
+                if (lane == 0) {
+                    rv[y] = nv;
+                    rt[y] = nt;
+                }
+            }
+            __syncthreads();
+            const uint32_t q = s_qn;
+            if (q > 0 && (q > kRefineCap - kRefineRound || rd == rounds - 1)) {
+                if (threadIdx.x == 0) s_base = atomicAdd(&A.ctr->refine_n, (unsigned long long)q);
+                __syncthreads();
+                const uint64_t tag = (uint64_t)j << 32;
+                for (uint32_t e = threadIdx.x; e < q; e += blockDim.x) A.refine[s_base + e] = tag | s_q[e];
+                __syncthreads();
+                if (threadIdx.x == 0) s_qn = 0;
+                __syncthreads();
             }
         }
     }
@@ -752,41 +800,48 @@ __device__ __forceinline__ bool smooths(const ChunkMeta& M) {
 }
 
 __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
-    __shared__ int64_t s_scan[2][kPlanThreads];
+    __shared__ int64_t s_scan[3][kPlanThreads];
     const int tid = threadIdx.x;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
     const int j0 = tid * per, j1 = min(n, j0 + per);
-    int64_t cnt = 0, tiles = 0;
+    int64_t cnt = 0, tiles = 0, ttiles = 0;
     for (int j = j0; j < j1; j++) {
         const ChunkMeta& M = A.meta[j];
         if (smooths(M)) {
             cnt++;
             tiles += (M.V + kSmoothThreads - 1) / kSmoothThreads;
+            ttiles += (M.T + kSmoothThreads - 1) / kSmoothThreads;
         }
     }
     s_scan[0][tid] = cnt;
     s_scan[1][tid] = tiles;
+    s_scan[2][tid] = ttiles;
     __syncthreads();
     for (int off = 1; off < kPlanThreads; off <<= 1) {
-        const int64_t a = tid >= off ? s_scan[0][tid - off] : 0, b = tid >= off ? s_scan[1][tid - off] : 0;
+        int64_t v[3];
+        for (int q = 0; q < 3; q++) v[q] = tid >= off ? s_scan[q][tid - off] : 0;
         __syncthreads();
-        s_scan[0][tid] += a;
-        s_scan[1][tid] += b;
+        for (int q = 0; q < 3; q++) s_scan[q][tid] += v[q];
         __syncthreads();
     }
-    int64_t k = s_scan[0][tid] - cnt, tp = s_scan[1][tid] - tiles;
+    int64_t k = s_scan[0][tid] - cnt, tp = s_scan[1][tid] - tiles, tt = s_scan[2][tid] - ttiles;
     for (int j = j0; j < j1; j++) {
         const ChunkMeta& M = A.meta[j];
         if (!smooths(M)) continue;
         A.smooth_list[k] = j;
         A.smooth_prefix[k] = tp;
+        A.smooth_tprefix[k] = tt;
         k++;
         tp += (M.V + kSmoothThreads - 1) / kSmoothThreads;
+        tt += (M.T + kSmoothThreads - 1) / kSmoothThreads;
     }
     if (tid == kPlanThreads - 1) {
-        A.ctr->n_smooth = (int32_t)s_scan[0][tid];
-        A.smooth_prefix[s_scan[0][tid]] = s_scan[1][tid];
+        const int64_t ns = s_scan[0][tid];
+        A.ctr->n_smooth = (int32_t)ns;
+        A.smooth_prefix[ns] = s_scan[1][tid];
+        A.smooth_tprefix[ns] = s_scan[2][tid];
         A.ctr->total_vtiles = s_scan[1][tid];
+        A.ctr->total_ttiles = s_scan[2][tid];
     }
 }
 
@@ -831,9 +886,8 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, co
     const uint32_t vb00 = s_base[0][y], vb01 = s_base[0][y + 1], vb10 = s_base[1][y], vb11 = s_base[1][y + 1];
     const int64_t vo = M.v_off, io = M.i_off;
     const uint32_t Vc = (uint32_t)M.V, Tc = (uint32_t)M.T;
-    // chunks with an odd iteration count start from the scratch buffer so that
-    // the last smoothing step lands in the output (see k_smooth)
-    float* __restrict__ P = ((M.iters & 1) ? A.pos_tmp : A.pos) + vo * 3;
+    float* __restrict__ P = A.pos + vo * 3;
+    float4* __restrict__ P4 = M.iters > 0 ? A.pos4 + vo : nullptr;  // smoothing state
     uint8_t* __restrict__ B = A.boundary + vo;
     uint32_t* __restrict__ E = A.vedge + vo;
     const float* __restrict__ D = A.dens + M.dens_off;
@@ -860,9 +914,11 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, co
                     // DMCChunk::calculate_isovertex / _get_intersection (DMCChunk.cpp:657-674)
                     const float mu = (0.0f - s0) / (s1 - s0);
                     const float dx = (float)xe - (float)x, dy = (float)ye - (float)y, dz = (float)ze - (float)z;
-                    P[3 * (int64_t)id + 0] = dx * mu + (float)x;
-                    P[3 * (int64_t)id + 1] = dy * mu + (float)y;
-                    P[3 * (int64_t)id + 2] = dz * mu + (float)z;
+                    const float qx = dx * mu + (float)x, qy = dy * mu + (float)y, qz = dz * mu + (float)z;
+                    P[3 * (int64_t)id + 0] = qx;
+                    P[3 * (int64_t)id + 1] = qy;
+                    P[3 * (int64_t)id + 2] = qz;
+                    if (P4) P4[id] = make_float4(qx, qy, qz, 0.0f);
                     B[id] = (uint8_t)(x == 0 || y == 0 || z == 0 || x == dim - 1 || y == dim - 1 || z == dim - 1 ||
                                       xe == dim - 1 || ye == dim - 1 || ze == dim - 1);
                     E[id] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | ((uint32_t)ax << 24);
@@ -959,18 +1015,53 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
 }
 
 // ------------------------------------------------------------------------
-// smooth: iteration m of MeshProcessor<3>'s dual/primal smoothing
-// (MeshProcessor.cpp:130-306 as driven by ChunkGenerator.cpp:112-124) for
-// every vertex of the chunks with K > m, one thread per vertex.  The
-// vertex's adjacent triangles come in ascending id from the ring of cells
-// around its lattice edge (c_ring) and each cell's table slots; per
-// triangle the centroid ((0 + p0) + p1) + p2 then / 3, summed from +0 in
-// that order, then / count.  Boundary vertices stay unless
-// boundary_processing.  Positions ping-pong between the output and a
-// scratch buffer: chunk j reads buffer (m + K_j) & 1 and writes the other,
-// so its last iteration writes the output.
+// smoothing, iteration m of MeshProcessor<3> (MeshProcessor.cpp:130-306 as
+// driven by ChunkGenerator.cpp:112-124), over the chunks with K > m:
+//   k_dual   per triangle: centroid ((0 + p0) + p1) + p2, then / 3
+//   k_primal per vertex: sum of the adjacent centroids from +0 in ascending
+//            triangle id, / count; boundary vertices stay unless
+//            boundary_processing.  The adjacent triangles come from the ring
+//            of cells around the vertex's lattice edge (c_ring) and each
+//            cell's table slots, which is ascending id order.
+// Positions live in a float4 buffer during the iterations (updated in place
+// by k_primal: k_dual has consumed them); the last iteration also writes the
+// float3 output.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kSmoothThreads) k_smooth(Arenas A, int m) {
+__global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int m) {
+    const int64_t total = A.ctr->total_ttiles;
+    const int nsm = A.ctr->n_smooth;
+    for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+        const int k = find_chunk(A.smooth_tprefix, nsm, tile);
+        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        if (M.iters <= m) continue;
+        const int64_t t = (tile - A.smooth_tprefix[k]) * kSmoothThreads + threadIdx.x;
+        if (t >= M.T) continue;
+        const uint32_t* __restrict__ I = A.idx + M.i_off + 3 * t;
+        const float4* __restrict__ P = A.pos4 + M.v_off;
+        const uint32_t i0 = I[0], i1 = I[1], i2 = I[2];
+        if (i0 >= (uint32_t)M.V || i1 >= (uint32_t)M.V || i2 >= (uint32_t)M.V) {
+            A.ctr->error = 5;
+            continue;
+        }
+        const float4 a = P[i0], b = P[i1], c = P[i2];
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        cx = cx + a.x;
+        cy = cy + a.y;
+        cz = cz + a.z;
+        cx = cx + b.x;
+        cy = cy + b.y;
+        cz = cz + b.z;
+        cx = cx + c.x;
+        cy = cy + c.y;
+        cz = cz + c.z;
+        A.cen4[M.i_off / 3 + t] = make_float4(cx / 3.0f, cy / 3.0f, cz / 3.0f, 0.0f);
+    }
+}
+
+__global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
+    __shared__ uint64_t s_tri[256];
+    for (int q = threadIdx.x; q < 256; q += blockDim.x) s_tri[q] = c_tri_table[q];
+    __syncthreads();
     const int64_t total = A.ctr->total_vtiles;
     const int nsm = A.ctr->n_smooth;
     for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
@@ -979,68 +1070,50 @@ __global__ void __launch_bounds__(kSmoothThreads) k_smooth(Arenas A, int m) {
         const int K = M.iters;
         if (K <= m) continue;
         const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
-        const int64_t V = M.V;
-        if (v >= V) continue;
+        if (v >= M.V) continue;
+        const int64_t vo = M.v_off, T = M.T;
+        if (!M.bp && A.boundary[vo + v]) continue;  // stays at its position in both buffers
         const int dim = M.dim;
-        const int64_t T = M.T, vo = M.v_off;
-        const float* __restrict__ src = (((m + K) & 1) ? A.pos_tmp : A.pos) + vo * 3;
-        float* __restrict__ dst = (((m + K) & 1) ? A.pos : A.pos_tmp) + vo * 3;
-        float px = src[3 * v + 0], py = src[3 * v + 1], pz = src[3 * v + 2];
-        if (M.bp || !A.boundary[vo + v]) {
-            const uint32_t* __restrict__ I = A.idx + M.i_off;
-            const uint32_t* __restrict__ CT = A.cell_tb + M.dens_off;
-            const uint8_t* __restrict__ CM = A.cell_mask + M.dens_off;
-            const uint32_t code = A.vedge[vo + v];
-            const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
-            const int ax = (int)(code >> 24);
-            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-            uint32_t cnt = 0;
-            for (int c = 0; c < 4; c++) {
-                const int cx = ex + c_ring[ax][c][0], cy = ey + c_ring[ax][c][1], cz = ez + c_ring[ax][c][2];
-                if (cx < 0 || cy < 0 || cz < 0 || cx > dim - 2 || cy > dim - 2 || cz > dim - 2) continue;
-                const int64_t ci = ((int64_t)cx * dim + cy) * dim + cz;
-                const uint32_t le = (uint32_t)c_ring[ax][c][3];
-                const uint32_t tb = CT[ci];
-                const uint64_t row = c_tri_table[CM[ci]];
-                const int ns = (int)(row >> 60);
-                for (int s = 0; s < ns; s++) {
-                    if (((row >> (4 * s)) & 15ull) != le) continue;
-                    const int64_t t = (int64_t)tb + s / 3;
-                    if (t >= T) {
-                        A.ctr->error = 7;
-                        break;
-                    }
-                    const uint32_t i0 = I[3 * t], i1 = I[3 * t + 1], i2 = I[3 * t + 2];
-                    if (i0 >= V || i1 >= V || i2 >= V) {
-                        A.ctr->error = 5;
-                        break;
-                    }
-                    float qx = 0.0f, qy = 0.0f, qz = 0.0f;
-                    qx = qx + src[3 * (int64_t)i0 + 0];
-                    qy = qy + src[3 * (int64_t)i0 + 1];
-                    qz = qz + src[3 * (int64_t)i0 + 2];
-                    qx = qx + src[3 * (int64_t)i1 + 0];
-                    qy = qy + src[3 * (int64_t)i1 + 1];
-                    qz = qz + src[3 * (int64_t)i1 + 2];
-                    qx = qx + src[3 * (int64_t)i2 + 0];
-                    qy = qy + src[3 * (int64_t)i2 + 1];
-                    qz = qz + src[3 * (int64_t)i2 + 2];
-                    sx = sx + qx / 3.0f;
-                    sy = sy + qy / 3.0f;
-                    sz = sz + qz / 3.0f;
-                    cnt++;
+        const float4* __restrict__ C = A.cen4 + M.i_off / 3;
+        const uint32_t* __restrict__ CT = A.cell_tb + M.dens_off;
+        const uint8_t* __restrict__ CM = A.cell_mask + M.dens_off;
+        const uint32_t code = A.vedge[vo + v];
+        const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
+        const int ax = (int)(code >> 24);
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        uint32_t cnt = 0;
+        for (int c = 0; c < 4; c++) {
+            const int cx = ex + c_ring[ax][c][0], cy = ey + c_ring[ax][c][1], cz = ez + c_ring[ax][c][2];
+            if (cx < 0 || cy < 0 || cz < 0 || cx > dim - 2 || cy > dim - 2 || cz > dim - 2) continue;
+            const int64_t ci = ((int64_t)cx * dim + cy) * dim + cz;
+            const uint32_t le = (uint32_t)c_ring[ax][c][3];
+            const uint32_t tb = CT[ci];
+            const uint64_t row = s_tri[CM[ci]];
+            const int ns = (int)(row >> 60);
+            for (int s = 0; s < ns; s++) {
+                if (((row >> (4 * s)) & 15ull) != le) continue;
+                const int64_t t = (int64_t)tb + s / 3;
+                if (t >= T) {
+                    A.ctr->error = 7;
+                    break;
                 }
-            }
-            if (cnt) {
-                const float fc = (float)cnt;
-                px = sx / fc;
-                py = sy / fc;
-                pz = sz / fc;
+                const float4 q = C[t];
+                sx = sx + q.x;
+                sy = sy + q.y;
+                sz = sz + q.z;
+                cnt++;
             }
         }
-        dst[3 * v + 0] = px;
-        dst[3 * v + 1] = py;
-        dst[3 * v + 2] = pz;
+        if (cnt == 0) continue;  // MeshProcessor.cpp:247: no adjacency, unchanged
+        const float fc = (float)cnt;
+        const float px = sx / fc, py = sy / fc, pz = sz / fc;
+        A.pos4[vo + v] = make_float4(px, py, pz, 0.0f);
+        if (m == K - 1) {
+            float* __restrict__ P = A.pos + (vo + v) * 3;
+            P[0] = px;
+            P[1] = py;
+            P[2] = pz;
+        }
     }
 }
 
@@ -1160,8 +1233,13 @@ hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStre
     if (wclasses & 8) hipLaunchKernelGGL(k_emit<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
     return hipGetLastError();
 }
+hipError_t launch_refine(const Arenas& A, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kSampleThreads), 0, s, A);
+    return hipGetLastError();
+}
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_smooth, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    hipLaunchKernelGGL(k_dual, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    hipLaunchKernelGGL(k_primal, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
     return hipGetLastError();
 }
 hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s) {
