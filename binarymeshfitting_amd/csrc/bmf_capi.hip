@@ -70,7 +70,7 @@ hipError_t dalloc(T** p, int64_t count) {
 void free_arenas(bmf_ctx* c) {
     Arenas& A = c->A;
     void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plane_prefix, A.smooth_list, A.smooth_prefix,
-                    A.smooth_tprefix, A.refine, A.cen4,
+                    A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
                     A.pos,     A.pos4,      A.boundary, A.vedge,       A.idx,          A.ctr,
                     A.sampler, c->d_spheres, c->d_boxes};
@@ -93,7 +93,7 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
     BMF_HIP(launch_sample(A, n, sample_blocks, 1, s));
     BMF_MARK(2);
     BMF_HIP(launch_count(A, n, row_blocks, wclasses, s));
-    BMF_HIP(launch_refine(A, sample_blocks, s));
+    BMF_HIP(launch_refine(A, n, row_blocks, s));
     BMF_MARK(3);
     BMF_HIP(launch_scan(A, n, scan_blocks, s));
     BMF_MARK(4);
@@ -189,7 +189,9 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
          dalloc(&A.results, nc) == hipSuccess && dalloc(&A.tile_prefix, nc + 1) == hipSuccess &&
          dalloc(&A.plane_prefix, nc + 1) == hipSuccess && dalloc(&A.smooth_list, nc) == hipSuccess &&
          dalloc(&A.smooth_prefix, nc + 1) == hipSuccess && dalloc(&A.smooth_tprefix, nc + 1) == hipSuccess &&
-         dalloc(&A.refine, A.cap_samples) == hipSuccess && dalloc(&A.cen4, A.cap_indices / 3 + 1) == hipSuccess &&
+         dalloc(&A.smooth_xcd_v, kXcds + 1) == hipSuccess && dalloc(&A.smooth_xcd_t, kXcds + 1) == hipSuccess &&
+         dalloc(&A.refine, A.cap_samples) == hipSuccess && dalloc(&A.plane_refine_n, nc * BMF_MAX_DIM) == hipSuccess &&
+         dalloc(&A.cen4, A.cap_indices / 3 + 1) == hipSuccess &&
          dalloc(&A.dens, A.cap_samples) == hipSuccess && dalloc(&A.labels, A.cap_samples / 64 + nc) == hipSuccess &&
          dalloc(&A.exact, A.cap_samples / 64 + nc) == hipSuccess &&
          dalloc(&A.row_v, A.cap_rows) == hipSuccess && dalloc(&A.row_t, A.cap_rows) == hipSuccess &&

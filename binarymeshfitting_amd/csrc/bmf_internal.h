@@ -15,6 +15,7 @@ constexpr int kSampleTile = kSampleThreads * kSampleRounds;
 constexpr int kRowThreads = 256;      // 4 waves; one x-plane per block, one label row per wave
 constexpr int kSmoothThreads = 256;   // one vertex per thread
 constexpr int kMaxRowWords = 4;       // ceil(BMF_MAX_DIM / 64)
+constexpr int kXcds = 8;              // MI355X: 8 XCDs, each with its own L2
 
 enum ChunkState : int32_t {
     kStateAir = 0,      // culled: every sample is air (d < 0 everywhere)
@@ -72,7 +73,6 @@ struct Counters {
     int64_t total_samples;
     int64_t total_vtiles;
     int64_t total_ttiles;
-    unsigned long long refine_n;
     int32_t n_smooth;
     int32_t overflow;
     int32_t error;
@@ -88,7 +88,10 @@ struct Arenas {
     int32_t* smooth_list;      // [max_chunks] chunks that run the smoothing
     int64_t* smooth_prefix;    // [max_chunks + 1] vertex tiles of the listed chunks
     int64_t* smooth_tprefix;   // [max_chunks + 1] triangle tiles of the listed chunks
-    uint64_t* refine;          // [max_samples] queued samples: chunk << 32 | sample index
+    int64_t* smooth_xcd_v;     // [kXcds + 1] vertex-tile range of each XCD group
+    int64_t* smooth_xcd_t;     // [kXcds + 1] triangle-tile range of each XCD group
+    uint32_t* refine;          // [max_samples] per x-plane queue of samples (y*dim + z) to finish
+    uint32_t* plane_refine_n;  // [max_planes] queue length of each x-plane item
     float* dens;               // [max_samples]
     uint64_t* labels;          // [max_samples / 64 + max_chunks]
     uint64_t* exact;           // same layout as labels: density of the sample is final
@@ -118,7 +121,7 @@ hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, hipSt
 hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
-hipError_t launch_refine(const Arenas& A, int blocks, hipStream_t s);
+hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s);
 hipError_t launch_ref_labels(const Arenas& A, uint32_t* words, hipStream_t s);

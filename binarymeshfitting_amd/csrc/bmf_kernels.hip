@@ -37,7 +37,9 @@ __constant__ int8_t c_ring[3][4][4] = {
     {{-1, -1, 0, 11}, {-1, 0, 0, 10}, {0, -1, 0, 9}, {0, 0, 0, 8}},
 };
 
-constexpr int kMaxPlaneWords = (BMF_MAX_DIM * BMF_MAX_DIM + 63) / 64 + 1;
+// u64 words of one staged label plane for row-word class W (dim <= 64 W),
+// plus the zero pad word.
+__host__ __device__ constexpr int plane_words(int W) { return 64 * W * W + 1; }
 
 // ------------------------------------------------------------------------
 // small helpers
@@ -329,7 +331,6 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         c->total_tiles = overflow ? 0 : tot_tiles;
         c->total_planes = overflow ? 0 : tot_planes;
         c->n_smooth = 0;
-        c->refine_n = 0;
         c->total_samples = tot_samples;
     }
 }
@@ -531,29 +532,33 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int 
 // to u64 words in LDS (coalesced funnel-shifted loads); rows are then read
 // from LDS.  Word pw of each plane is a zero pad for the funnel reads.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ void stage_planes(uint64_t (*s_lab)[kMaxPlaneWords], const uint64_t* __restrict__ L,
-                                             int64_t nwords, int dim, int x0) {
+__device__ __forceinline__ void stage_plane(uint64_t* s_p, const uint64_t* __restrict__ L, int64_t nwords, int dim,
+                                            int xp) {
     const int64_t pbits = (int64_t)dim * dim;
     const int pw = (int)((pbits + 63) >> 6);
-    for (int p = 0; p < 3; p++) {
-        const int xp = x0 + p;
-        for (int k = threadIdx.x; k <= pw; k += blockDim.x) {
-            uint64_t v = 0ull;
-            if (xp >= 0 && xp < dim && k < pw) {
-                const int64_t bit = (int64_t)xp * pbits + 64ll * k;
-                const int64_t w = bit >> 6;
-                const int sh = (int)(bit & 63);
-                v = L[w];
-                if (sh) {
-                    const uint64_t hi = (w + 1 < nwords) ? L[w + 1] : 0ull;
-                    v = (v >> sh) | (hi << (64 - sh));
-                }
-                const int64_t rem = pbits - 64ll * k;
-                if (rem < 64) v &= (1ull << rem) - 1ull;
+    for (int k = threadIdx.x; k <= pw; k += blockDim.x) {
+        uint64_t v = 0ull;
+        if (xp >= 0 && xp < dim && k < pw) {
+            const int64_t bit = (int64_t)xp * pbits + 64ll * k;
+            const int64_t w = bit >> 6;
+            const int sh = (int)(bit & 63);
+            v = L[w];
+            if (sh) {
+                const uint64_t hi = (w + 1 < nwords) ? L[w + 1] : 0ull;
+                v = (v >> sh) | (hi << (64 - sh));
             }
-            s_lab[p][k] = v;
+            const int64_t rem = pbits - 64ll * k;
+            if (rem < 64) v &= (1ull << rem) - 1ull;
         }
+        s_p[k] = v;
     }
+}
+
+__device__ __forceinline__ void stage_planes(uint64_t* p0, uint64_t* p1, uint64_t* p2, const uint64_t* __restrict__ L,
+                                             int64_t nwords, int dim, int x0) {
+    stage_plane(p0, L, nwords, dim, x0);
+    stage_plane(p1, L, nwords, dim, x0 + 1);
+    stage_plane(p2, L, nwords, dim, x0 + 2);
 }
 
 // Row y of a staged plane.
@@ -596,14 +601,14 @@ __device__ __forceinline__ RowBits<W> ror(const RowBits<W>& a, const RowBits<W>&
     return r;
 }
 
-// Staged planes: s_lab[0] = x-1, s_lab[1] = x, s_lab[2] = x+1.  Counts the
-// row's vertices and triangles, and queues for k_refine the row's samples
-// that are endpoints of sign-changing edges but were labelled before their
-// last octave (k_sample's adaptive evaluation).
+// Staged planes: s_lab[0..2] = labels of x-1, x, x+1; s_lab[3] = exactness
+// of x.  Counts the row's vertices and triangles, and queues for k_refine
+// the row's samples that are endpoints of sign-changing edges but were
+// labelled before their last octave (k_sample's adaptive evaluation).
 template <int W>
-__device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, const uint64_t (*s_lab)[kMaxPlaneWords],
-                                          const uint8_t* s_ntri, int x, int y, uint32_t* s_q, uint32_t* s_qn,
-                                          uint32_t& nv, uint32_t& nt) {
+__device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, const uint64_t* const* s_lab,
+                                          const uint8_t* s_ntri, int x, int y, uint32_t* __restrict__ q,
+                                          uint32_t* s_qn, uint32_t& nv, uint32_t& nt) {
     const int dim = M.dim;
     const int lane = lane_id();
     const bool x1 = x + 1 < dim, y1 = y + 1 < dim;
@@ -628,16 +633,15 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
     RowBits<W> ep = ror<W>(ror<W>(ex, ey), ror<W>(ez, rshl1<W>(ez)));
     ep = ror<W>(ep, ror<W>(rxor<W>(r00, rxm, x >= 1), rxor<W>(r00, rym, y >= 1)));
     if (total_pop<W>(ep) == 0) return;
-    const RowBits<W> done = load_row<W>(A.exact + M.lab_off, M.lab_words, dim, x, y);
+    const RowBits<W> done = lds_row<W>(s_lab[3], dim, y, true);
     RowBits<W> need;
 #pragma unroll
     for (int k = 0; k < W; k++) need.w[k] = ep.w[k] & ~done.w[k];
     if (total_pop<W>(need) == 0) return;
-    // stage them in the block's LDS queue; k_count flushes it to the global
-    // refine queue with one reservation per flush, and k_refine evaluates
-    // them on dense waves
+    // append them to this plane's region of the refine queue (an LDS counter
+    // per plane, no global atomics); k_refine evaluates them on dense waves
     const uint64_t lt = lanemask_lt();
-    const uint32_t rowbase = (uint32_t)(((int64_t)x * dim + y) * dim);
+    const uint32_t rowbase = (uint32_t)(y * dim);
     for (int z0 = 0; z0 < dim; z0 += 64) {
         const int z = z0 + lane;
         const bool want = z < dim && bit_at<W>(need, z);
@@ -646,25 +650,34 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(s_qn, (uint32_t)__popcll(b));
         base = __shfl(base, 0);
-        if (want) s_q[base + __popcll(b & lt)] = rowbase + (uint32_t)z;
+        if (want) q[base + __popcll(b & lt)] = rowbase + (uint32_t)z;
     }
 }
 
 // ------------------------------------------------------------------------
 // refine: full density of the queued samples (endpoints of sign-changing
-// edges labelled early by k_sample), one sample per thread.
+// edges labelled early by k_sample); one block per x-plane queue.
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A) {
+__global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A, int n) {
     const SamplerDev* __restrict__ S = A.sampler;
-    const uint64_t total = A.ctr->refine_n;
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t code = A.refine[e];
-        const ChunkMeta& M = A.meta[(int)(code >> 32)];
-        const uint32_t i = (uint32_t)code;
-        const uint32_t dim = (uint32_t)M.dim, dim2 = dim * dim;
-        float px, py, pz;
-        sample_coords(M, i, dim, dim2, 1.0f / (float)dim, 1.0f / (float)dim2, px, py, pz);
-        A.dens[M.dens_off + i] = sample_density(*S, px, py, pz);
+    const int64_t total = A.ctr->total_planes;
+    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const uint32_t cnt = A.plane_refine_n[item];
+        if (cnt == 0) continue;
+        const int j = find_chunk(A.plane_prefix, n, item);
+        const ChunkMeta& M = A.meta[j];
+        const uint32_t dim = (uint32_t)M.dim;
+        const uint32_t x = (uint32_t)(item - A.plane_prefix[j]);
+        const uint32_t* __restrict__ q = A.refine + M.dens_off + (int64_t)x * dim * dim;
+        float* __restrict__ D = A.dens + M.dens_off + (int64_t)x * dim * dim;
+        const float px = (float)x * M.delta + M.opos[0];
+        for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) {
+            const uint32_t r = q[e];  // y * dim + z within the plane
+            const uint32_t y = r / dim, z = r - y * dim;
+            const float py = (float)y * M.delta + M.opos[1];
+            const float pz = (float)z * M.delta + M.opos[2];
+            D[r] = sample_density(*S, px, py, pz);
+        }
     }
 }
 
@@ -673,15 +686,9 @@ __global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A) {
 template <int W>
 __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
     __shared__ uint8_t s_ntri[256];
-    __shared__ uint64_t s_lab[3][kMaxPlaneWords];
-    // refine staging: one round of rows (4 rows of <= 256 samples) never
-    // adds more than kRefineRound entries, so flushing when the queue holds
-    // more than its capacity minus one round keeps it from overflowing
-    constexpr int kRefineRound = (kRowThreads / 64) * BMF_MAX_DIM;
-    constexpr int kRefineCap = 4 * kRefineRound;
-    __shared__ uint32_t s_q[kRefineCap];
+    __shared__ uint64_t s_planes[4][plane_words(W)];
+    const uint64_t* const s_lab[4] = {s_planes[0], s_planes[1], s_planes[2], s_planes[3]};
     __shared__ uint32_t s_qn;
-    __shared__ unsigned long long s_base;
     for (int t = threadIdx.x; t < 256; t += blockDim.x) s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
     const int wave = uniform_int(threadIdx.x >> 6);
     const int lane = lane_id();
@@ -696,35 +703,25 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
         uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
         if (!(M.any_air && M.any_solid)) {  // no surface in this chunk
             for (int y = threadIdx.x; y < dim; y += blockDim.x) rv[y] = rt[y] = 0u;
+            if (threadIdx.x == 0) A.plane_refine_n[item] = 0;
             continue;
         }
-        __syncthreads();  // previous item's rows are done with s_lab and s_q
+        __syncthreads();  // previous item is done with the shared arrays
         if (threadIdx.x == 0) s_qn = 0;
-        stage_planes(s_lab, A.labels + M.lab_off, M.lab_words, dim, x - 1);
+        stage_planes(s_planes[0], s_planes[1], s_planes[2], A.labels + M.lab_off, M.lab_words, dim, x - 1);
+        stage_plane(s_planes[3], A.exact + M.lab_off, M.lab_words, dim, x);
         __syncthreads();
-        const int rounds = (dim + kRowThreads / 64 - 1) / (kRowThreads / 64);
-        for (int rd = 0; rd < rounds; rd++) {
-            const int y = rd * (kRowThreads / 64) + wave;
-            if (y < dim) {
-                uint32_t nv = 0, nt = 0;
-                count_row<W>(A, M, s_lab, s_ntri, x, y, s_q, &s_qn, nv, nt);
-                if (lane == 0) {
-                    rv[y] = nv;
-                    rt[y] = nt;
-                }
-            }
-            __syncthreads();
-            const uint32_t q = s_qn;
-            if (q > 0 && (q > kRefineCap - kRefineRound || rd == rounds - 1)) {
-                if (threadIdx.x == 0) s_base = atomicAdd(&A.ctr->refine_n, (unsigned long long)q);
-                __syncthreads();
-                const uint64_t tag = (uint64_t)j << 32;
-                for (uint32_t e = threadIdx.x; e < q; e += blockDim.x) A.refine[s_base + e] = tag | s_q[e];
-                __syncthreads();
-                if (threadIdx.x == 0) s_qn = 0;
-                __syncthreads();
+        uint32_t* q = A.refine + M.dens_off + (int64_t)x * dim * dim;
+        for (int y = wave; y < dim; y += kRowThreads / 64) {
+            uint32_t nv = 0, nt = 0;
+            count_row<W>(A, M, s_lab, s_ntri, x, y, q, &s_qn, nv, nt);
+            if (lane == 0) {
+                rv[y] = nv;
+                rt[y] = nt;
             }
         }
+        __syncthreads();
+        if (threadIdx.x == 0) A.plane_refine_n[item] = s_qn;
     }
 }
 
@@ -799,49 +796,84 @@ __device__ __forceinline__ bool smooths(const ChunkMeta& M) {
     return M.iters > 0 && M.V > 0 && M.T > 0 && M.status == BMF_OK && M.state == kStateSampled;
 }
 
+// The list is grouped by XCD: the k-th smoothing chunk goes to XCD k % 8,
+// so that all tiles of a chunk run on one XCD and its positions, indices and
+// centroids stay in that XCD's L2 (workgroups are dispatched to the 8 XCDs
+// round-robin by block index; a different dispatch order only costs
+// locality, not correctness).
 __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
     __shared__ int64_t s_scan[3][kPlanThreads];
+    __shared__ int32_t s_total;
     const int tid = threadIdx.x;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
     const int j0 = tid * per, j1 = min(n, j0 + per);
-    int64_t cnt = 0, tiles = 0, ttiles = 0;
-    for (int j = j0; j < j1; j++) {
-        const ChunkMeta& M = A.meta[j];
-        if (smooths(M)) {
-            cnt++;
-            tiles += (M.V + kSmoothThreads - 1) / kSmoothThreads;
-            ttiles += (M.T + kSmoothThreads - 1) / kSmoothThreads;
-        }
-    }
+    // ---- rank of each smoothing chunk (ordered compaction) ----
+    int64_t cnt = 0;
+    for (int j = j0; j < j1; j++) cnt += smooths(A.meta[j]);
     s_scan[0][tid] = cnt;
-    s_scan[1][tid] = tiles;
-    s_scan[2][tid] = ttiles;
     __syncthreads();
     for (int off = 1; off < kPlanThreads; off <<= 1) {
-        int64_t v[3];
-        for (int q = 0; q < 3; q++) v[q] = tid >= off ? s_scan[q][tid - off] : 0;
+        const int64_t a = tid >= off ? s_scan[0][tid - off] : 0;
         __syncthreads();
-        for (int q = 0; q < 3; q++) s_scan[q][tid] += v[q];
+        s_scan[0][tid] += a;
         __syncthreads();
     }
-    int64_t k = s_scan[0][tid] - cnt, tp = s_scan[1][tid] - tiles, tt = s_scan[2][tid] - ttiles;
+    const int ns = (int)s_scan[0][kPlanThreads - 1];
+    int k = (int)(s_scan[0][tid] - cnt);
     for (int j = j0; j < j1; j++) {
-        const ChunkMeta& M = A.meta[j];
-        if (!smooths(M)) continue;
-        A.smooth_list[k] = j;
-        A.smooth_prefix[k] = tp;
-        A.smooth_tprefix[k] = tt;
+        if (!smooths(A.meta[j])) continue;
+        // grouped position: groups g = k % 8 in order, rank k / 8 inside
+        const int g = k & (kXcds - 1), q = k / kXcds;
+        int pos = q;
+        for (int h = 0; h < g; h++) pos += (ns - h + kXcds - 1) / kXcds;
+        A.smooth_list[pos] = j;
         k++;
-        tp += (M.V + kSmoothThreads - 1) / kSmoothThreads;
+    }
+    __syncthreads();  // smooth_list written by this block is visible to it
+    // ---- tile prefixes in grouped order ----
+    const int per2 = (ns + kPlanThreads - 1) / kPlanThreads;
+    const int p0 = tid * per2, p1 = min(ns, p0 + per2);
+    int64_t tv = 0, tt = 0;
+    for (int p = p0; p < p1; p++) {
+        const ChunkMeta& M = A.meta[A.smooth_list[p]];
+        tv += (M.V + kSmoothThreads - 1) / kSmoothThreads;
         tt += (M.T + kSmoothThreads - 1) / kSmoothThreads;
     }
+    __syncthreads();
+    s_scan[1][tid] = tv;
+    s_scan[2][tid] = tt;
+    __syncthreads();
+    for (int off = 1; off < kPlanThreads; off <<= 1) {
+        const int64_t a = tid >= off ? s_scan[1][tid - off] : 0, b = tid >= off ? s_scan[2][tid - off] : 0;
+        __syncthreads();
+        s_scan[1][tid] += a;
+        s_scan[2][tid] += b;
+        __syncthreads();
+    }
+    int64_t bv = s_scan[1][tid] - tv, bt = s_scan[2][tid] - tt;
+    for (int p = p0; p < p1; p++) {
+        const ChunkMeta& M = A.meta[A.smooth_list[p]];
+        A.smooth_prefix[p] = bv;
+        A.smooth_tprefix[p] = bt;
+        bv += (M.V + kSmoothThreads - 1) / kSmoothThreads;
+        bt += (M.T + kSmoothThreads - 1) / kSmoothThreads;
+    }
+    if (tid == 0) s_total = ns;
+    __syncthreads();
     if (tid == kPlanThreads - 1) {
-        const int64_t ns = s_scan[0][tid];
-        A.ctr->n_smooth = (int32_t)ns;
+        A.ctr->n_smooth = ns;
         A.smooth_prefix[ns] = s_scan[1][tid];
         A.smooth_tprefix[ns] = s_scan[2][tid];
         A.ctr->total_vtiles = s_scan[1][tid];
         A.ctr->total_ttiles = s_scan[2][tid];
+    }
+    __syncthreads();
+    // ---- tile range of each XCD group ----
+    if (tid <= kXcds) {
+        int pos = 0;
+        for (int h = 0; h < tid; h++) pos += (ns - h + kXcds - 1) / kXcds;
+        A.smooth_xcd_v[tid] = pos < ns ? A.smooth_prefix[pos] : A.smooth_prefix[ns];
+        A.smooth_xcd_t[tid] = pos < ns ? A.smooth_tprefix[pos] : A.smooth_tprefix[ns];
     }
 }
 
@@ -853,7 +885,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
 // per-cell triangle base and mask.
 // ------------------------------------------------------------------------
 template <int W>
-__device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, const uint64_t (*s_lab)[kMaxPlaneWords],
+__device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, const uint64_t* const* s_lab,
                                          const uint8_t* s_ntri, const uint32_t (*s_base)[BMF_MAX_DIM + 1],
                                          uint32_t (*s_vid)[64], int x, int y) {
     const int dim = M.dim;
@@ -987,7 +1019,8 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, co
 template <int W>
 __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
     __shared__ uint8_t s_ntri[256];
-    __shared__ uint64_t s_lab[3][kMaxPlaneWords];
+    __shared__ uint64_t s_planes[3][plane_words(W)];
+    const uint64_t* const s_lab[3] = {s_planes[0], s_planes[1], s_planes[2]};
     __shared__ uint32_t s_base[3][BMF_MAX_DIM + 1];  // row vertex bases of planes x, x+1; triangle bases of plane x
     __shared__ uint32_t s_vid[kRowThreads / 64][12][64];
     for (int t = threadIdx.x; t < 256; t += blockDim.x) s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
@@ -1001,7 +1034,7 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
         if (M.status != BMF_OK || M.V == 0) continue;  // uniform across the block
         const int x = (int)(item - A.plane_prefix[j]);
         __syncthreads();  // previous item is done with the shared arrays
-        stage_planes(s_lab, A.labels + M.lab_off, M.lab_words, dim, x);
+        stage_planes(s_planes[0], s_planes[1], s_planes[2], A.labels + M.lab_off, M.lab_words, dim, x);
         const uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
         const uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
         for (int y = threadIdx.x; y <= dim; y += blockDim.x) {
@@ -1028,9 +1061,10 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
 // float3 output.
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int m) {
-    const int64_t total = A.ctr->total_ttiles;
     const int nsm = A.ctr->n_smooth;
-    for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t end = A.smooth_xcd_t[xcd + 1];
+    for (int64_t tile = A.smooth_xcd_t[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
         const int k = find_chunk(A.smooth_tprefix, nsm, tile);
         const ChunkMeta& M = A.meta[A.smooth_list[k]];
         if (M.iters <= m) continue;
@@ -1062,9 +1096,10 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
     __shared__ uint64_t s_tri[256];
     for (int q = threadIdx.x; q < 256; q += blockDim.x) s_tri[q] = c_tri_table[q];
     __syncthreads();
-    const int64_t total = A.ctr->total_vtiles;
     const int nsm = A.ctr->n_smooth;
-    for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t end = A.smooth_xcd_v[xcd + 1];
+    for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
         const int k = find_chunk(A.smooth_prefix, nsm, tile);
         const ChunkMeta& M = A.meta[A.smooth_list[k]];
         const int K = M.iters;
@@ -1233,11 +1268,12 @@ hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStre
     if (wclasses & 8) hipLaunchKernelGGL(k_emit<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
     return hipGetLastError();
 }
-hipError_t launch_refine(const Arenas& A, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kSampleThreads), 0, s, A);
+hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kSampleThreads), 0, s, A, n);
     return hipGetLastError();
 }
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
+    blocks = blocks < kXcds ? kXcds : blocks - blocks % kXcds;  // whole XCD groups
     hipLaunchKernelGGL(k_dual, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
     hipLaunchKernelGGL(k_primal, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
     return hipGetLastError();
