@@ -40,7 +40,27 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 launches) and exit; its FETCH_SIZE and "
+                         "WRITE_SIZE readings scale the counters of the extraction kernels to HBM bytes")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size, for scaling the L2-to-memory
+    byte counters of the other kernels to bytes."""
+    import torch
+
+    nbytes = mib << 20
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda:0").fill_(7)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert int(dst[-1]) == 7
+    print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": 3}), flush=True)
+    return 0
 
 
 def log(msg):
@@ -53,6 +73,8 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU, "
               "and the extraction path shards by chunk with no collective on the hot path)", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     from binarymeshfitting_amd import capi, roofline, workloads
 
@@ -116,9 +138,23 @@ def main():
         "frac_of_unpacked_peak": round(achieved_tflops / roofline.PEAK_FP32_UNPACKED_TFLOPS, 4),
         "traffic": traffic,
         "algorithmic_flops_per_launch": flops,
-        "algorithmic_bytes_per_launch": int(samples * roofline.SAMPLE_BYTES),
+        "algorithmic_bytes_per_launch": int(samples * roofline.LABEL_BYTES),
         "avg_launch_ms": round(stages["sample"], 4),
-        "hbm_frac": round(samples * roofline.SAMPLE_BYTES / sample_s / 1e9 / roofline.PEAK_HBM_GBS, 5),
+        "hbm_frac": round((traffic or samples * roofline.LABEL_BYTES) / sample_s / 1e9 / roofline.PEAK_HBM_GBS, 5),
+    }
+    # whole frame: full-octave sampling + smoothing FLOPs and the compulsory
+    # mesh bytes over the step time
+    frame_s = wall / args.steps
+    path_flops = flops + sum(roofline.smoothing_flops(r.vertex_count, r.index_count // 3, c.iters)
+                              for r, c in zip(res, wl.chunks))
+    path_bytes = sum(roofline.mesh_bytes(r.vertex_count, r.index_count // 3) for r in res)
+    path = {
+        "flops_per_step": path_flops,
+        "bytes_per_step": path_bytes,
+        "tflops": round(path_flops / frame_s / 1e12, 3),
+        "fp32_frac": round(path_flops / frame_s / 1e12 / roofline.PEAK_FP32_TFLOPS, 4),
+        "hbm_gbs": round(path_bytes / frame_s / 1e9, 2),
+        "hbm_frac": round(path_bytes / frame_s / 1e9 / roofline.PEAK_HBM_GBS, 5),
     }
 
     # ---- plain path ---------------------------------------------------------------
@@ -188,6 +224,7 @@ def main():
         "batched_over_plain": round(value / plain, 2) if plain else None,
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roof,
+        "path_roofline": path,
         "cpu_baseline": cpu,
         "parity": parity,
     }

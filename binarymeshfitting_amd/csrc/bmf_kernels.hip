@@ -803,7 +803,6 @@ __device__ __forceinline__ bool smooths(const ChunkMeta& M) {
 // locality, not correctness).
 __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
     __shared__ int64_t s_scan[3][kPlanThreads];
-    __shared__ int32_t s_total;
     const int tid = threadIdx.x;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
     const int j0 = tid * per, j1 = min(n, j0 + per);
@@ -858,7 +857,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
         bv += (M.V + kSmoothThreads - 1) / kSmoothThreads;
         bt += (M.T + kSmoothThreads - 1) / kSmoothThreads;
     }
-    if (tid == 0) s_total = ns;
+
     __syncthreads();
     if (tid == kPlanThreads - 1) {
         A.ctr->n_smooth = ns;
@@ -1037,13 +1036,22 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
         stage_planes(s_planes[0], s_planes[1], s_planes[2], A.labels + M.lab_off, M.lab_words, dim, x);
         const uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
         const uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
+        // row bases of planes x and x+1 and the triangle bases of plane x; the
+        // entry after a plane's last row is the next row's base (or the chunk
+        // total), so base[y+1] - base[y] is the row's count
+        const int64_t R = (int64_t)dim * dim, r0 = (int64_t)x * dim;
         for (int y = threadIdx.x; y <= dim; y += blockDim.x) {
-            s_base[0][y] = y < dim ? rv[y] : 0u;
-            s_base[1][y] = (y < dim && x + 1 < dim) ? rv[dim + y] : 0u;
-            s_base[2][y] = y < dim ? rt[y] : 0u;
+            const int64_t ra = r0 + y, rb = r0 + dim + y;
+            s_base[0][y] = ra < R ? rv[y] : (uint32_t)M.V;
+            s_base[1][y] = y < dim ? (rb < R ? rv[dim + y] : (uint32_t)M.V) : 0u;
+            s_base[2][y] = ra < R ? rt[y] : (uint32_t)M.T;
         }
         __syncthreads();
-        for (int y = wave; y < dim; y += kRowThreads / 64) emit_row<W>(A, M, s_lab, s_ntri, s_base, s_vid[wave], x, y);
+        for (int y = wave; y < dim; y += kRowThreads / 64) {
+            // rows that own no vertex and hold no triangle skip the label work
+            if (s_base[0][y + 1] == s_base[0][y] && s_base[2][y + 1] == s_base[2][y]) continue;
+            emit_row<W>(A, M, s_lab, s_ntri, s_base, s_vid[wave], x, y);
+        }
     }
 }
 

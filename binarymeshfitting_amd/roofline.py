@@ -23,13 +23,23 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, packed (78.6 unpacked)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0      # HBM3E
 
-# Compulsory HBM bytes written by the sampling kernel per sample: the FP32
-# density and one label bit.
-SAMPLE_BYTES = 4.0 + 1.0 / 8.0
+# Compulsory HBM bytes written by the sampling kernel per sample: one label
+# bit.  Densities are stored only where a vertex needs them (both ends of a
+# sign-changing edge), a data-dependent fraction that is not counted here.
+LABEL_BYTES = 1.0 / 8.0
 
 
 def terrain_flops_per_sample(octaves):
+    """FP32 operations of one full-octave density evaluation (the reference
+    evaluates every octave at every sample)."""
     return octaves * (SIMPLEX_FLOPS + OCTAVE_FLOPS) + TERRAIN_FLOPS
+
+
+def smoothing_flops(V, T, K):
+    """Vertex placement (4 per vertex) and K dual/primal smoothing iterations:
+    per triangle its centroid (6 adds + 3 divides) and its share of the three
+    corner ring sums (3 x 3 adds), per vertex the 3 divides by the ring size."""
+    return 4 * V + K * (9 * T + 3 * (3 * T) + 3 * V)
 
 
 def mesh_bytes(V, T):
