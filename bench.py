@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
+    ap.add_argument("--diff-steps", type=int, default=5,
+                    help="timed steps of the forward+loss+backward leg (one 128^3-cell chunk, K=8); 0 skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 launches) and exit; its FETCH_SIZE and "
                          "WRITE_SIZE readings scale the counters of the extraction kernels to HBM bytes")
@@ -65,6 +67,49 @@ def copy_calibration(mib):
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def diff_leg(capi, roofline, workloads, steps):
+    """Differentiable path on one 128^3-cell terrain chunk (129^3 samples),
+    8 smoothing iterations, checkpoints every 2, loss = mean squared distance
+    to the nearest of 200,000 targets on a sphere (r 40, centre 64^3)."""
+    dim, K = 129, 8
+    with capi.Context(max_chunks=4, max_samples=dim ** 3, max_vertices=1 << 20, max_indices=1 << 22) as ctx:
+        ctx.set_sampler(**workloads.terrain_sampler())
+        with capi.Tape(ctx, max_dim=dim, max_vertices=1 << 20, max_indices=1 << 22, checkpoint_every=2) as tape:
+            cd = capi.chunk_desc((0.0, 0.0, 0.0), float(dim - 1), dim, 0.0, K, 0)
+            targets = capi.DeviceArray.from_numpy(workloads.sphere_targets())
+            info = tape.forward(cd)
+            L, g, _ = tape.loss_nearest(targets)
+            grad = capi.DeviceArray((dim, dim, dim), np.float32)
+            capi.check(capi.lib().bmf_backward_diff(tape.h, g.ptr, grad.ptr), "bmf_backward_diff")
+            t_f = t_l = t_b = 0.0
+            for _ in range(steps):
+                t0 = time.perf_counter()
+                capi.check(capi.lib().bmf_forward_diff(ctx.h, capi.C.byref(cd), None, tape.h), "bmf_forward_diff")
+                t1 = time.perf_counter()
+                capi.check(capi.lib().bmf_loss_nearest(tape.h, targets.ptr, targets.shape[0], g.ptr, None, None),
+                           "bmf_loss_nearest")
+                t2 = time.perf_counter()
+                capi.check(capi.lib().bmf_backward_diff(tape.h, g.ptr, grad.ptr), "bmf_backward_diff")
+                t3 = time.perf_counter()
+                t_f, t_l, t_b = t_f + t1 - t0, t_l + t2 - t1, t_b + t3 - t2
+            V = info.vertex_count
+            T = info.index_count // 3
+            n3 = dim ** 3
+            per = (t_f + t_l + t_b) / steps
+            # compulsory bytes of the backward: dL/dp in, the gradient out
+            bwd_bytes = 12 * V + 4 * n3
+            fwd_flops = n3 * roofline.terrain_flops_per_sample(5) + roofline.smoothing_flops(V, T, K)
+            log(f"[bench] fwd+bwd: {1e3 * per:.3f} ms/step (forward {1e3 * t_f / steps:.3f}, loss {1e3 * t_l / steps:.3f},"
+                f" backward {1e3 * t_b / steps:.3f}), V={V}, loss {L:.4f}")
+            return {"workload": "one 128^3-cell fbm terrain chunk, K=8, 200k sphere targets",
+                    "steps": steps, "ms_per_step": round(1e3 * per, 4), "steps_per_s": round(1.0 / per, 2),
+                    "forward_ms": round(1e3 * t_f / steps, 4), "loss_ms": round(1e3 * t_l / steps, 4),
+                    "backward_ms": round(1e3 * t_b / steps, 4), "vertices": V, "triangles": T, "loss": L,
+                    "forward_fp32_frac": round(fwd_flops / (t_f / steps) / 1e12 / roofline.PEAK_FP32_TFLOPS, 4),
+                    "backward_hbm_frac": round(bwd_bytes / (t_b / steps) / 1e9 / roofline.PEAK_HBM_GBS, 5),
+                    "timing": "host wall clock per call; each call ends with a stream synchronisation"}
 
 
 def main():
@@ -170,6 +215,11 @@ def main():
         plain = n * args.plain_steps / (time.perf_counter() - t0)
         log(f"[bench] plain: {plain:.1f} chunks/s")
 
+    # ---- forward + loss + backward: one 128^3-cell chunk, K=8, 200k targets ---------
+    diff = None
+    if args.diff_steps > 0:
+        diff = diff_leg(capi, roofline, workloads, args.diff_steps)
+
     # ---- CPU baseline + parity -----------------------------------------------------
     cpu = None
     parity = None
@@ -225,6 +275,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roof,
         "path_roofline": path,
+        "fwd_bwd": diff,
         "cpu_baseline": cpu,
         "parity": parity,
     }

@@ -12,14 +12,16 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libbmf.so"
 
 OK, EINVAL, ECAPACITY, EHIP, ENODEV, ESTATE = range(6)
-SAMPLER_TERRAIN, SAMPLER_SPHERE, SAMPLER_CUBOID, SAMPLER_TORUS_Z, SAMPLER_PLANE_Y, SAMPLER_CSG = range(6)
+SAMPLER_TERRAIN, SAMPLER_SPHERE, SAMPLER_CUBOID, SAMPLER_TORUS_Z, SAMPLER_PLANE_Y, SAMPLER_CSG, SAMPLER_EXTERNAL = range(7)
 MODE_PLAIN, MODE_BATCHED = 0, 1
 MAX_DIM = 256
 
 EXPORTS = [
     "bmf_create", "bmf_destroy", "bmf_strerror", "bmf_version", "bmf_set_sampler", "bmf_validate_chunks",
     "bmf_extract", "bmf_result", "bmf_mesh_view_get", "bmf_mesh_to_host", "bmf_sync", "bmf_sample_block",
-    "bmf_label_block", "bmf_cell_masks", "bmf_last_timing", "bmf_profile_stages",
+    "bmf_label_block", "bmf_cell_masks", "bmf_last_timing", "bmf_profile_stages", "bmf_tape_create",
+    "bmf_tape_destroy", "bmf_forward_diff", "bmf_tape_get", "bmf_tape_positions", "bmf_loss_nearest",
+    "bmf_backward_diff",
 ]
 NUM_STAGES = 7
 STAGES = ["plan", "sample", "count", "scan", "emit", "smooth", "finalize"]
@@ -35,7 +37,7 @@ class SamplerDesc(C.Structure):
         ("kind", C.c_int32), ("world_size", C.c_float), ("amp", C.c_float), ("y0", C.c_float),
         ("freq", C.c_float), ("lacunarity", C.c_float), ("gain", C.c_float), ("octaves", C.c_int32),
         ("seed", C.c_uint32), ("spheres", C.POINTER(C.c_float)), ("n_spheres", C.c_int32),
-        ("boxes", C.POINTER(C.c_float)), ("n_boxes", C.c_int32),
+        ("boxes", C.POINTER(C.c_float)), ("n_boxes", C.c_int32), ("densities", C.c_void_p),
     ]
 
 
@@ -53,6 +55,13 @@ class ChunkResult(C.Structure):
 class MeshView(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("indices", C.c_void_p), ("boundary", C.c_void_p),
                 ("total_vertices", C.c_int64), ("total_indices", C.c_int64)]
+
+
+class TapeInfo(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("iters", C.c_int32), ("checkpoint_every", C.c_int32),
+                ("vertex_count", C.c_int32), ("index_count", C.c_int32), ("contains_mesh", C.c_int32),
+                ("positions", C.c_void_p), ("indices", C.c_void_p), ("boundary", C.c_void_p),
+                ("densities", C.c_void_p)]
 
 
 _lib = None
@@ -85,6 +94,15 @@ def lib():
         L.bmf_cell_masks.argtypes = [ctxp, P(ChunkDesc), P(C.c_uint64)]
         L.bmf_last_timing.argtypes = [ctxp, P(C.c_float)]
         L.bmf_profile_stages.argtypes = [ctxp, P(ChunkDesc), C.c_int, P(C.c_float)]
+        tp = C.c_void_p
+        L.bmf_tape_create.argtypes = [P(tp), ctxp, C.c_int32, C.c_int64, C.c_int64, C.c_int32]
+        L.bmf_tape_destroy.argtypes = [tp]
+        L.bmf_tape_destroy.restype = None
+        L.bmf_forward_diff.argtypes = [ctxp, P(ChunkDesc), C.c_void_p, tp]
+        L.bmf_tape_get.argtypes = [tp, P(TapeInfo)]
+        L.bmf_tape_positions.argtypes = [tp, C.c_int32, C.c_void_p]
+        L.bmf_loss_nearest.argtypes = [tp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, P(C.c_float)]
+        L.bmf_backward_diff.argtypes = [tp, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -141,11 +159,14 @@ class Context:
             pass
 
     def set_sampler(self, kind=SAMPLER_TERRAIN, world_size=256.0, amp=24.0, y0=32.0, freq=0.02, lacunarity=2.0,
-                    gain=0.5, octaves=5, seed=1337, spheres=None, boxes=None):
+                    gain=0.5, octaves=5, seed=1337, spheres=None, boxes=None, densities=None):
+        """densities: a DeviceArray of packed dim^3 blocks for SAMPLER_EXTERNAL."""
         sp = np.ascontiguousarray(spheres if spheres is not None else np.zeros((0, 4)), np.float32)
         bx = np.ascontiguousarray(boxes if boxes is not None else np.zeros((0, 6)), np.float32)
         d = SamplerDesc(kind, world_size, amp, y0, freq, lacunarity, gain, octaves, seed & 0xFFFFFFFF,
-                        _ptr(sp, C.c_float), len(sp), _ptr(bx, C.c_float), len(bx))
+                        _ptr(sp, C.c_float), len(sp), _ptr(bx, C.c_float), len(bx),
+                        densities.ptr if densities is not None else None)
+        self._sampler_keep = densities
         check(lib().bmf_set_sampler(self.h, C.byref(d)), "bmf_set_sampler")
 
     def extract(self, chunks, mode=MODE_BATCHED):
@@ -212,3 +233,131 @@ class Context:
 def validate_chunks(chunks, limits=None):
     arr = chunk_array(chunks) if chunks else None
     return lib().bmf_validate_chunks(arr, len(chunks), C.byref(limits) if limits is not None else None)
+
+
+# ---- device buffers (HIP runtime through ctypes; the library's own runtime) ----------------
+_hip = None
+_H2D, _D2H = 1, 2
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7")
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipDeviceSynchronize.argtypes = []
+        _hip = H
+    return _hip
+
+
+class DeviceArray:
+    """A device allocation holding a numpy array's bytes."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        if hip().hipMalloc(C.byref(p), max(self.nbytes, 1)) != 0:
+            raise BmfError(EHIP, "hipMalloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if a.nbytes and hip().hipMemcpy(d.ptr, a.ctypes.data, a.nbytes, _H2D) != 0:
+            raise BmfError(EHIP, "hipMemcpy")
+        return d
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes and hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, _D2H) != 0:
+            raise BmfError(EHIP, "hipMemcpy")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_to_numpy(ptr, shape, dtype):
+    out = np.empty(shape, dtype)
+    if out.nbytes and hip().hipMemcpy(out.ctypes.data, ptr, out.nbytes, _D2H) != 0:
+        raise BmfError(EHIP, "hipMemcpy")
+    return out
+
+
+class Tape:
+    """bmf_tape: forward / loss / backward of one chunk on a context."""
+
+    def __init__(self, ctx, max_dim=129, max_vertices=1 << 20, max_indices=1 << 22, checkpoint_every=2):
+        h = C.c_void_p()
+        check(lib().bmf_tape_create(C.byref(h), ctx.h, max_dim, max_vertices, max_indices, checkpoint_every),
+              "bmf_tape_create")
+        self.h = h
+        self.ctx = ctx
+
+    def close(self):
+        if self.h:
+            lib().bmf_tape_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def forward(self, chunk, densities=None):
+        """densities: DeviceArray (dim^3 FP32) or None for the context's sampler."""
+        check(lib().bmf_forward_diff(self.ctx.h, C.byref(chunk), densities.ptr if densities is not None else None,
+                                     self.h), "bmf_forward_diff")
+        return self.info()
+
+    def info(self):
+        i = TapeInfo()
+        check(lib().bmf_tape_get(self.h, C.byref(i)), "bmf_tape_get")
+        return i
+
+    def mesh(self):
+        i = self.info()
+        V, nI = i.vertex_count, i.index_count
+        return (device_to_numpy(i.positions, (V, 3), np.float32), device_to_numpy(i.indices, (nI,), np.uint32),
+                device_to_numpy(i.boundary, (V,), np.uint8))
+
+    def densities(self):
+        i = self.info()
+        return device_to_numpy(i.densities, (i.dim,) * 3, np.float32)
+
+    def positions(self, iteration):
+        i = self.info()
+        out = DeviceArray((max(i.vertex_count, 1), 3), np.float32)
+        check(lib().bmf_tape_positions(self.h, iteration, out.ptr), "bmf_tape_positions")
+        return out.numpy()[: i.vertex_count]
+
+    def loss_nearest(self, targets):
+        """targets: DeviceArray [n][3] FP32.  Returns (loss, dL/dp DeviceArray, nearest numpy)."""
+        i = self.info()
+        g = DeviceArray((max(i.vertex_count, 1), 3), np.float32)
+        near = DeviceArray((max(i.vertex_count, 1),), np.int32)
+        L = C.c_float()
+        check(lib().bmf_loss_nearest(self.h, targets.ptr, targets.shape[0], g.ptr, near.ptr, C.byref(L)),
+              "bmf_loss_nearest")
+        return L.value, g, near.numpy()[: i.vertex_count]
+
+    def backward(self, dldp):
+        i = self.info()
+        out = DeviceArray((i.dim,) * 3, np.float32)
+        check(lib().bmf_backward_diff(self.h, dldp.ptr if dldp is not None else None, out.ptr), "bmf_backward_diff")
+        return out.numpy()

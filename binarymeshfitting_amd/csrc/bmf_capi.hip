@@ -10,45 +10,10 @@
 #include <vector>
 
 #include "../../include/bmf.h"
+#include "bmf_context.h"
 #include "bmf_internal.h"
 
 using namespace bmf;
-
-namespace {
-
-struct GraphEntry {
-    int n = -1;
-    int maxk = -1;
-    int wclasses = 0;
-    hipGraphExec_t exec = nullptr;
-};
-
-}  // namespace
-
-struct bmf_ctx {
-    int device = 0;
-    int cu_count = 256;
-    hipStream_t stream = nullptr;
-    bmf_limits lim{};
-    Arenas A{};
-    bmf_chunk_desc* h_descs[2] = {nullptr, nullptr};  // pinned, double-buffered uploads
-    hipEvent_t ev_upload[2] = {nullptr, nullptr};
-    int upload_slot = 0;
-    hipEvent_t ev_stage[BMF_NUM_STAGES + 1] = {};
-    bmf_chunk_result* h_results = nullptr;  // pinned
-    Counters* h_ctr = nullptr;              // pinned
-    SamplerDev h_sampler{};
-    float* d_spheres = nullptr;
-    float* d_boxes = nullptr;
-    bool sampler_set = false;
-    int last_n = 0;
-    bool have_extract = false;
-    bool results_valid = false;
-    std::vector<GraphEntry> graphs;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // last sampling-kernel layout
-    int persistent_blocks = 2048;
-};
 
 namespace {
 
@@ -107,6 +72,10 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
     return BMF_OK;
 }
 
+}  // namespace
+
+namespace bmf {
+
 // Host descriptors -> device through one of two pinned staging buffers; a
 // buffer is refilled only after its previous copy has executed, so frames
 // can be enqueued back to back without a host synchronisation.
@@ -131,7 +100,7 @@ int check_desc(const bmf_chunk_desc& d) {
     return BMF_OK;
 }
 
-}  // namespace
+}  // namespace bmf
 
 extern "C" {
 
@@ -242,7 +211,8 @@ void bmf_destroy(bmf_ctx* c) {
 
 int bmf_set_sampler(bmf_ctx* c, const bmf_sampler_desc* s) {
     if (!c || !s) return BMF_EINVAL;
-    if (s->kind < BMF_SAMPLER_TERRAIN || s->kind > BMF_SAMPLER_CSG) return BMF_EINVAL;
+    if (s->kind < BMF_SAMPLER_TERRAIN || s->kind > BMF_SAMPLER_EXTERNAL) return BMF_EINVAL;
+    if (s->kind == BMF_SAMPLER_EXTERNAL && !s->densities) return BMF_EINVAL;
     SamplerDev d{};
     d.kind = s->kind;
     d.world_size = s->world_size;
@@ -285,6 +255,7 @@ int bmf_set_sampler(bmf_ctx* c, const bmf_sampler_desc* s) {
         d.spheres = c->d_spheres;
         d.boxes = c->d_boxes;
     }
+    if (s->kind == BMF_SAMPLER_EXTERNAL) d.ext = s->densities;
     c->h_sampler = d;
     BMF_HIP(hipMemcpy(c->A.sampler, &d, sizeof(SamplerDev), hipMemcpyHostToDevice));
     c->sampler_set = true;
@@ -316,8 +287,11 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
         // output arenas bump-allocated across the chunks (ChunkGenerator.cpp:91
         // loop body, one chunk at a time)
         BMF_HIP(hipEventRecord(c->ev0, c->stream));
+        int64_t ext_base = 0;
         for (int j = 0; j < n; j++) {
             Arenas Aj = A;
+            Aj.ext_base = ext_base;
+            ext_base += (int64_t)chunks[j].dim * chunks[j].dim * chunks[j].dim;
             Aj.descs = A.descs + j;
             Aj.meta = A.meta + j;
             Aj.results = A.results + j;

@@ -1,0 +1,53 @@
+// The context object behind the C ABI, shared by the extraction entry
+// points (bmf_capi.hip) and the differentiable path (bmf_diff.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/bmf.h"
+#include "bmf_internal.h"
+
+namespace bmf {
+
+struct GraphEntry {
+    int n = -1;
+    int maxk = -1;
+    int wclasses = 0;
+    hipGraphExec_t exec = nullptr;
+};
+
+}  // namespace bmf
+
+struct bmf_ctx {
+    int device = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    bmf_limits lim{};
+    bmf::Arenas A{};
+    bmf_chunk_desc* h_descs[2] = {nullptr, nullptr};  // pinned, double-buffered uploads
+    hipEvent_t ev_upload[2] = {nullptr, nullptr};
+    int upload_slot = 0;
+    hipEvent_t ev_stage[BMF_NUM_STAGES + 1] = {};
+    bmf_chunk_result* h_results = nullptr;  // pinned
+    bmf::Counters* h_ctr = nullptr;         // pinned
+    bmf::SamplerDev h_sampler{};
+    float* d_spheres = nullptr;
+    float* d_boxes = nullptr;
+    bool sampler_set = false;
+    int last_n = 0;
+    bool have_extract = false;
+    bool results_valid = false;
+    std::vector<bmf::GraphEntry> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int persistent_blocks = 2048;  // workgroups of the persistent kernels
+};
+
+namespace bmf {
+
+// Copies n descriptors to the device arena on the context stream.
+int upload_descs(bmf_ctx* c, const bmf_chunk_desc* chunks, int n);
+// BMF_OK or BMF_EINVAL for one descriptor (host only).
+int check_desc(const bmf_chunk_desc& d);
+
+}  // namespace bmf

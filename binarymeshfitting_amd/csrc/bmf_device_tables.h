@@ -1,0 +1,26 @@
+// Marching-cubes tables in constant memory, one copy per translation unit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bmf_mc_tables.h"
+
+namespace bmf {
+
+// MarchingCubes::tri_table (MCTable.h:27-283), packed: bits 60..63 hold the
+// entry count, nibble s the local edge of slot s.
+static __constant__ uint64_t c_tri_table[256] = BMF_TRI_TABLE_INIT;
+
+// The four cells around a lattice edge along `axis`, in lexicographic cell
+// order, as (dx, dy, dz, local edge) offsets from the edge origin; local
+// edges numbered as in DMCChunk::polygonize_cell (DMCChunk.cpp:543-565).
+// Visiting them in this order and each cell's table slots in order lists
+// the edge's triangles in ascending id, the order in which
+// MeshProcessor::init_primitives appends them (MeshProcessor.cpp:114-127).
+static __constant__ int8_t c_ring[3][4][4] = {
+    {{0, -1, -1, 3}, {0, -1, 0, 2}, {0, 0, -1, 1}, {0, 0, 0, 0}},
+    {{-1, 0, -1, 7}, {-1, 0, 0, 6}, {0, 0, -1, 5}, {0, 0, 0, 4}},
+    {{-1, -1, 0, 11}, {-1, 0, 0, 10}, {0, -1, 0, 9}, {0, 0, 0, 8}},
+};
+
+}  // namespace bmf

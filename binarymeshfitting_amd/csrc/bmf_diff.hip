@@ -1,0 +1,809 @@
+// Differentiable extraction of one chunk (C ABI in include/bmf.h):
+//   forward   the extraction kernels with every octave evaluated, then the
+//             smoothing on the tape's own copy of the mesh, keeping the
+//             positions every `checkpoint_every` iterations;
+//   loss      mean squared distance to the nearest of a set of targets
+//             (exact nearest neighbour through a two-level AABB hierarchy
+//             over Morton-sorted targets);
+//   backward  the transpose of the K smoothing steps (linear in the
+//             positions for the frozen topology) and of vertex placement
+//             mu = (0 - s0) / (s1 - s0) (DMCChunk.cpp:657-674), gathered per
+//             density sample in a fixed order (deterministic, no atomics).
+//
+// Tape layout in HBM (one chunk, V vertices, T triangles):
+//   dens  FP32 [dim^3]      the densities of the forward
+//   vid0  u32 [dim^3]       first vertex id owned by each sample (valid
+//                           where the sample owns a sign-changing edge)
+//   vedge u32 [V], boundary u8 [V], idx u32 [3T]   copies of the mesh
+//   adj   u32 [kMaxRing][V] ring of each vertex: its triangles in ascending
+//                           id (the adjacency of MeshProcessor::set_adj,
+//                           MeshProcessor.cpp:98-128), slot-major so that a
+//                           wave's loads of one slot are contiguous
+//   deg u8 [V], mv u8 [V]   ring size; whether the primal step moves the
+//                           vertex (MeshProcessor.cpp:247)
+//   p4 float4 [V]           working positions; ckpt float4 [slots][V]
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "../../include/bmf.h"
+#include "bmf_context.h"
+#include "bmf_device_tables.h"
+#include "bmf_internal.h"
+
+using namespace bmf;
+
+namespace {
+
+constexpr int kThreads = 256;
+// A vertex's lattice edge borders 4 cells and one cell's triangulation uses
+// one edge in at most 5 table slots (MCTable.h:27-283), so a ring holds at
+// most 20 triangles.
+constexpr int kMaxRing = 20;
+constexpr int kCluster = 32;  // targets per leaf box of the nearest-target search
+constexpr int kSuper = 32;    // leaf boxes per top-level box
+
+#define BMF_HIP(call)                          \
+    do {                                       \
+        hipError_t e_ = (call);                \
+        if (e_ != hipSuccess) return BMF_EHIP; \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, int64_t count) {
+    return hipMalloc((void**)p, (size_t)std::max<int64_t>(count, 1) * sizeof(T));
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kThreads - 1) / kThreads); }
+
+// ------------------------------------------------------------------------
+// forward
+// ------------------------------------------------------------------------
+__global__ void k_tape_init(const float* __restrict__ pos, int V, float4* __restrict__ p4) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    p4[v] = make_float4(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 0.0f);
+}
+
+// Ring of each vertex from the per-cell triangle bases and masks written by
+// the emit kernel: the 4 cells around its edge in lexicographic order and,
+// in each, the table slots on its local edge in slot order.
+__global__ void k_tape_ring(const uint32_t* __restrict__ vedge, const uint8_t* __restrict__ boundary,
+                            const uint32_t* __restrict__ cell_tb, const uint8_t* __restrict__ cell_mask, int dim,
+                            int V, int T, int bp, int stride, uint32_t* __restrict__ adj, uint8_t* __restrict__ deg,
+                            uint8_t* __restrict__ mv, int* __restrict__ err) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint32_t code = vedge[v];
+    const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
+    const int ax = (int)(code >> 24);
+    int n = 0;
+    for (int c = 0; c < 4; c++) {
+        const int cx = ex + c_ring[ax][c][0], cy = ey + c_ring[ax][c][1], cz = ez + c_ring[ax][c][2];
+        if (cx < 0 || cy < 0 || cz < 0 || cx > dim - 2 || cy > dim - 2 || cz > dim - 2) continue;
+        const int64_t ci = ((int64_t)cx * dim + cy) * dim + cz;
+        const uint32_t le = (uint32_t)c_ring[ax][c][3];
+        const uint32_t tb = cell_tb[ci];
+        const uint64_t row = c_tri_table[cell_mask[ci]];
+        const int ns = (int)(row >> 60);
+        for (int s = 0; s < ns; s++) {
+            if (((row >> (4 * s)) & 15ull) != le) continue;
+            const uint32_t t = tb + (uint32_t)(s / 3);
+            if (t >= (uint32_t)T || n >= kMaxRing) {
+                *err = 1;
+                continue;
+            }
+            adj[(int64_t)n * stride + v] = t;
+            n++;
+        }
+    }
+    deg[v] = (uint8_t)n;
+    mv[v] = (uint8_t)((bp || !boundary[v]) && n > 0);
+}
+
+// vid0[sample] = first vertex owned by the sample (vertices are numbered in
+// sample order, x-, y-, z-edge within a sample).
+__global__ void k_tape_vid0(const uint32_t* __restrict__ vedge, int V, int dim, uint32_t* __restrict__ vid0) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint32_t s = vedge[v] & 0xFFFFFFu;
+    if (v > 0 && (vedge[v - 1] & 0xFFFFFFu) == s) return;
+    const int x = (int)(s & 255u), y = (int)((s >> 8) & 255u), z = (int)(s >> 16);
+    vid0[((int64_t)x * dim + y) * dim + z] = (uint32_t)v;
+}
+
+// Dual step (MeshProcessor.cpp:130-236): centroid ((0 + p0) + p1) + p2, / 3.
+__global__ void k_tape_dual(const uint32_t* __restrict__ idx, const float4* __restrict__ p4, int T,
+                            float4* __restrict__ cen) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const float4 a = p4[idx[3 * t]], b = p4[idx[3 * t + 1]], c = p4[idx[3 * t + 2]];
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    cx = cx + a.x;
+    cy = cy + a.y;
+    cz = cz + a.z;
+    cx = cx + b.x;
+    cy = cy + b.y;
+    cz = cz + b.z;
+    cx = cx + c.x;
+    cy = cy + c.y;
+    cz = cz + c.z;
+    cen[t] = make_float4(cx / 3.0f, cy / 3.0f, cz / 3.0f, 0.0f);
+}
+
+// Primal step (MeshProcessor.cpp:238-306): ring sum in ascending triangle
+// order from +0, divided by the ring size.
+__global__ void k_tape_primal(const uint32_t* __restrict__ adj, int stride, const uint8_t* __restrict__ deg,
+                              const uint8_t* __restrict__ mv, const float4* __restrict__ cen, int V,
+                              float4* __restrict__ p4) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V || !mv[v]) return;
+    const int n = deg[v];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int k = 0; k < n; k++) {
+        const float4 q = cen[adj[(int64_t)k * stride + v]];
+        sx = sx + q.x;
+        sy = sy + q.y;
+        sz = sz + q.z;
+    }
+    const float fc = (float)n;
+    p4[v] = make_float4(sx / fc, sy / fc, sz / fc, 0.0f);
+}
+
+__global__ void k_tape_out(const float4* __restrict__ p4, int V, float* __restrict__ pos) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const float4 q = p4[v];
+    pos[3 * v] = q.x;
+    pos[3 * v + 1] = q.y;
+    pos[3 * v + 2] = q.z;
+}
+
+// ------------------------------------------------------------------------
+// backward
+// ------------------------------------------------------------------------
+__global__ void k_bwd_init(const float* __restrict__ g3, int V, float4* __restrict__ g) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    g[v] = make_float4(g3[3 * v], g3[3 * v + 1], g3[3 * v + 2], 0.0f);
+}
+
+// dL/dc_t = sum over the corners u of t that the primal step moves of
+// dL/dp'_u / |ring(u)|.
+__global__ void k_bwd_tri(const uint32_t* __restrict__ idx, const float4* __restrict__ g,
+                          const uint8_t* __restrict__ deg, const uint8_t* __restrict__ mv, int T,
+                          float4* __restrict__ gc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint32_t u = idx[3 * t + k];
+        if (!mv[u]) continue;
+        const float4 q = g[u];
+        const float n = (float)deg[u];
+        ax = ax + q.x / n;
+        ay = ay + q.y / n;
+        az = az + q.z / n;
+    }
+    gc[t] = make_float4(ax, ay, az, 0.0f);
+}
+
+// dL/dp_v = [v not moved] dL/dp'_v + (sum over ring(v) of dL/dc_t) / 3.
+__global__ void k_bwd_vert(const uint32_t* __restrict__ adj, int stride, const uint8_t* __restrict__ deg,
+                           const uint8_t* __restrict__ mv, const float4* __restrict__ gc, int V,
+                           float4* __restrict__ g) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int n = deg[v];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int k = 0; k < n; k++) {
+        const float4 q = gc[adj[(int64_t)k * stride + v]];
+        sx = sx + q.x;
+        sy = sy + q.y;
+        sz = sz + q.z;
+    }
+    const float4 own = mv[v] ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : g[v];
+    g[v] = make_float4(own.x + sx / 3.0f, own.y + sy / 3.0f, own.z + sz / 3.0f, 0.0f);
+}
+
+// dL/d(density) per sample: the sample is s0 of its own sign-changing edges
+// (+x, +y, +z, vertex ids vid0[i] + 0, 1, ...) and s1 of the sign-changing
+// edges of its -x, -y, -z neighbours.  mu = -s0 / (s1 - s0), so
+// dmu/ds0 = -s1 / (s1 - s0)^2 and dmu/ds1 = s0 / (s1 - s0)^2; the vertex
+// moves along its edge axis by mu (grid units).
+__global__ void k_bwd_place(const float* __restrict__ dens, const uint32_t* __restrict__ vid0,
+                            const float4* __restrict__ g, int dim, float* __restrict__ out) {
+    const int64_t n3 = (int64_t)dim * dim * dim;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const int z = (int)(i % dim), y = (int)((i / dim) % dim), x = (int)(i / ((int64_t)dim * dim));
+    const int64_t st[3] = {(int64_t)dim * dim, (int64_t)dim, 1};
+    const int co[3] = {x, y, z};
+    const float di = dens[i];
+    const bool li = di < 0.0f;
+    float acc = 0.0f;
+    int k = 0;
+    for (int a = 0; a < 3; a++) {  // own edges: this sample is s0
+        if (co[a] + 1 >= dim) continue;
+        const float dj = dens[i + st[a]];
+        if ((dj < 0.0f) == li) continue;
+        const float4 q = g[vid0[i] + (uint32_t)k];
+        k++;
+        const float ds = dj - di;
+        const float ga = a == 0 ? q.x : (a == 1 ? q.y : q.z);
+        acc = acc + ga * (-dj / (ds * ds));
+    }
+    for (int a = 0; a < 3; a++) {  // edges of the -a neighbour: this sample is s1
+        if (co[a] == 0) continue;
+        const int64_t j = i - st[a];
+        const float dj = dens[j];
+        const bool lj = dj < 0.0f;
+        if (lj == li) continue;
+        // rank of j's axis-a edge among j's sign-changing edges
+        uint32_t r = 0;
+        for (int b = 0; b < a; b++) {
+            const int cb = b == 0 ? x - (a == 0) : (b == 1 ? y - (a == 1) : z);
+            if (cb + 1 < dim && ((dens[j + st[b]] < 0.0f) != lj)) r++;
+        }
+        const float4 q = g[vid0[j] + r];
+        const float ds = di - dj;
+        const float ga = a == 0 ? q.x : (a == 1 ? q.y : q.z);
+        acc = acc + ga * (dj / (ds * ds));
+    }
+    out[i] = acc;
+}
+
+// ------------------------------------------------------------------------
+// nearest-target loss
+// ------------------------------------------------------------------------
+__device__ __forceinline__ int float_order(float f) {
+    const int b = __float_as_int(f);
+    return b >= 0 ? b : b ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float order_float(int b) { return __int_as_float(b >= 0 ? b : b ^ 0x7FFFFFFF); }
+
+__global__ void k_nn_bbox_init(int* bb) {
+    if (threadIdx.x < 3) bb[threadIdx.x] = 0x7FFFFFFF;
+    else if (threadIdx.x < 6) bb[threadIdx.x] = (int)0x80000000;
+}
+
+__global__ void k_nn_bbox(const float* __restrict__ t, int64_t n, int* bb) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fminf(lo[a], t[3 * i + a]);
+            hi[a] = fmaxf(hi[a], t[3 * i + a]);
+        }
+    for (int a = 0; a < 3; a++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[a] = fminf(lo[a], __shfl_xor(lo[a], off));
+            hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&bb[a], float_order(lo[a]));
+            atomicMax(&bb[3 + a], float_order(hi[a]));
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton_key(const int* bb, float x, float y, float z) {
+    const float p[3] = {x, y, z};
+    uint32_t q[3];
+    for (int a = 0; a < 3; a++) {
+        const float lo = order_float(bb[a]), hi = order_float(bb[3 + a]);
+        const float ext = hi - lo;
+        float u = ext > 0.0f ? (p[a] - lo) * (1024.0f / ext) : 0.0f;
+        u = fminf(fmaxf(u, 0.0f), 1023.0f);
+        q[a] = (uint32_t)u;
+    }
+    return (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+}
+
+__global__ void k_nn_keys(const float* __restrict__ t, int64_t n, const int* __restrict__ bb,
+                          uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = morton_key(bb, t[3 * i], t[3 * i + 1], t[3 * i + 2]);
+    vals[i] = (int32_t)i;
+}
+
+__global__ void k_nn_gather(const float* __restrict__ t, const int32_t* __restrict__ perm, int64_t n,
+                            float4* __restrict__ st) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t p = perm[i];
+    st[i] = make_float4(t[3 * (int64_t)p], t[3 * (int64_t)p + 1], t[3 * (int64_t)p + 2], __int_as_float(p));
+}
+
+// Boxes over groups of `group` consecutive entries (points or boxes).
+__global__ void k_nn_boxes(const float4* __restrict__ lo_in, const float4* __restrict__ hi_in, int64_t n,
+                           int group, int64_t nbox, float4* __restrict__ lo, float4* __restrict__ hi) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+    const int64_t e = std::min<int64_t>(n, (b + 1) * group);
+    for (int64_t i = b * group; i < e; i++) {
+        const float4 a = lo_in[i], c = hi_in[i];
+        l.x = fminf(l.x, a.x);
+        l.y = fminf(l.y, a.y);
+        l.z = fminf(l.z, a.z);
+        h.x = fmaxf(h.x, c.x);
+        h.y = fmaxf(h.y, c.y);
+        h.z = fmaxf(h.z, c.z);
+    }
+    lo[b] = l;
+    hi[b] = h;
+}
+
+__device__ __forceinline__ float dist2(float px, float py, float pz, float4 q) {
+    const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+// Squared distance from p to the box (0 inside); never above the distance to
+// any point in the box in FP32 (every step rounds monotonically).
+__device__ __forceinline__ float box_lb2(float px, float py, float pz, float4 l, float4 h) {
+    const float dx = fmaxf(fmaxf(l.x - px, px - h.x), 0.0f);
+    const float dy = fmaxf(fmaxf(l.y - py, py - h.y), 0.0f);
+    const float dz = fmaxf(fmaxf(l.z - pz, pz - h.z), 0.0f);
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+__device__ __forceinline__ void nn_consider(float px, float py, float pz, float4 q, float& best, int& bi,
+                                            float4& bq) {
+    const float d = dist2(px, py, pz, q);
+    const int id = __float_as_int(q.w);
+    if (d < best || (d == best && id < bi)) {
+        best = d;
+        bi = id;
+        bq = q;
+    }
+}
+
+// One vertex per lane.  Seed: the leaf box holding the vertex's Morton key;
+// then every top-level box, and inside it every leaf box, whose lower bound
+// does not exceed the lane's best distance (wave-uniform skips).
+__global__ void __launch_bounds__(kThreads) k_nn_query(const float* __restrict__ pos, int V,
+                                                       const float4* __restrict__ st, const uint32_t* __restrict__ skeys,
+                                                       int64_t n, const int* __restrict__ bb,
+                                                       const float4* __restrict__ clo, const float4* __restrict__ chi,
+                                                       int64_t ncl, const float4* __restrict__ slo,
+                                                       const float4* __restrict__ shi, int64_t nsup,
+                                                       int32_t* __restrict__ nearest, float* __restrict__ dldp,
+                                                       double* __restrict__ d2) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = v < V;
+    const float px = active ? pos[3 * v] : 0.0f, py = active ? pos[3 * v + 1] : 0.0f, pz = active ? pos[3 * v + 2] : 0.0f;
+    float best = INFINITY;
+    int bi = 0x7FFFFFFF;
+    float4 bq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (active) {
+        const uint32_t key = morton_key(bb, px, py, pz);
+        int64_t lo = 0, hi = n;  // first sorted key >= key
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (skeys[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        const int64_t c0 = std::min<int64_t>(lo / kCluster, ncl - 1);
+        const int64_t e = std::min<int64_t>(n, (c0 + 1) * kCluster);
+        for (int64_t i = c0 * kCluster; i < e; i++) nn_consider(px, py, pz, st[i], best, bi, bq);
+    }
+    for (int64_t s = 0; s < nsup; s++) {
+        const bool need_s = active && box_lb2(px, py, pz, slo[s], shi[s]) <= best;
+        if (!__any(need_s)) continue;
+        const int64_t ce = std::min<int64_t>(ncl, (s + 1) * kSuper);
+        for (int64_t c = s * kSuper; c < ce; c++) {
+            const bool need = need_s && box_lb2(px, py, pz, clo[c], chi[c]) <= best;
+            if (!__any(need)) continue;
+            const int64_t e = std::min<int64_t>(n, (c + 1) * kCluster);
+            for (int64_t i = c * kCluster; i < e; i++) {
+                const float4 q = st[i];
+                if (need) nn_consider(px, py, pz, q, best, bi, bq);
+            }
+        }
+    }
+    if (!active) return;
+    nearest[v] = bi;
+    const float sc = 2.0f / (float)V;
+    dldp[3 * v] = (px - bq.x) * sc;
+    dldp[3 * v + 1] = (py - bq.y) * sc;
+    dldp[3 * v + 2] = (pz - bq.z) * sc;
+    d2[v] = (double)best;
+}
+
+// Deterministic sum: each thread a contiguous slice, then a fixed tree.
+__global__ void __launch_bounds__(1024) k_sum(const double* __restrict__ x, int n, double* __restrict__ out) {
+    __shared__ double s[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    double a = 0.0;
+    for (int i = tid * per; i < std::min(n, (tid + 1) * per); i++) a += x[i];
+    s[tid] = a;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (tid < w) s[tid] += s[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) out[0] = s[0] / (double)n;
+}
+
+}  // namespace
+
+struct bmf_tape {
+    bmf_ctx* ctx = nullptr;
+    int32_t max_dim = 0;
+    int64_t cap_v = 0, cap_i = 0;
+    int32_t ckpt_every = 1;
+    bool valid = false;
+    int32_t dim = 0, K = 0, bp = 0, V = 0, T = 0, contains_mesh = 0;
+    float* dens = nullptr;
+    uint32_t* vid0 = nullptr;
+    uint32_t* vedge = nullptr;
+    uint8_t* boundary = nullptr;
+    uint32_t* idx = nullptr;
+    uint32_t* adj = nullptr;
+    uint8_t* deg = nullptr;
+    uint8_t* mv = nullptr;
+    float4* p4 = nullptr;
+    float4* q4 = nullptr;
+    float4* cen = nullptr;
+    float* pos = nullptr;
+    float4* g = nullptr;
+    int* err = nullptr;
+    float4* ckpt = nullptr;
+    int64_t ckpt_slots = 0;
+    // nearest-target search
+    int64_t nn_cap = 0;
+    uint32_t* keys[2] = {nullptr, nullptr};
+    int32_t* vals[2] = {nullptr, nullptr};
+    float4* st = nullptr;
+    float4* clo = nullptr;
+    float4* chi = nullptr;
+    float4* slo = nullptr;
+    float4* shi = nullptr;
+    int* bb = nullptr;
+    void* cub_tmp = nullptr;
+    size_t cub_bytes = 0;
+    double* d2 = nullptr;
+    double* loss = nullptr;
+    int32_t* nearest_tmp = nullptr;
+};
+
+namespace {
+
+void free_nn(bmf_tape* t) {
+    for (int i = 0; i < 2; i++) {
+        dfree(t->keys[i]);
+        dfree(t->vals[i]);
+    }
+    dfree(t->st);
+    dfree(t->clo);
+    dfree(t->chi);
+    dfree(t->slo);
+    dfree(t->shi);
+    if (t->cub_tmp) (void)hipFree(t->cub_tmp);
+    t->cub_tmp = nullptr;
+    t->cub_bytes = 0;
+    t->nn_cap = 0;
+}
+
+int reserve_nn(bmf_tape* t, int64_t n) {
+    if (n <= t->nn_cap) return BMF_OK;
+    free_nn(t);
+    const int64_t ncl = (n + kCluster - 1) / kCluster, nsup = (ncl + kSuper - 1) / kSuper;
+    bool ok = true;
+    for (int i = 0; i < 2; i++)
+        ok = ok && dalloc(&t->keys[i], n) == hipSuccess && dalloc(&t->vals[i], n) == hipSuccess;
+    ok = ok && dalloc(&t->st, n) == hipSuccess && dalloc(&t->clo, ncl) == hipSuccess &&
+         dalloc(&t->chi, ncl) == hipSuccess && dalloc(&t->slo, nsup) == hipSuccess && dalloc(&t->shi, nsup) == hipSuccess;
+    if (ok) {
+        hipcub::DoubleBuffer<uint32_t> kb(t->keys[0], t->keys[1]);
+        hipcub::DoubleBuffer<int32_t> vb(t->vals[0], t->vals[1]);
+        size_t bytes = 0;
+        ok = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kb, vb, (int)n, 0, 30) == hipSuccess;
+        ok = ok && hipMalloc(&t->cub_tmp, std::max<size_t>(bytes, 1)) == hipSuccess;
+        t->cub_bytes = bytes;
+    }
+    if (!ok) {
+        free_nn(t);
+        return BMF_EHIP;
+    }
+    t->nn_cap = n;
+    return BMF_OK;
+}
+
+// Runs k smoothing iterations on `state` (float4 [V]) with the tape's ring.
+int smooth_steps(bmf_tape* t, float4* state, int k, hipStream_t s, int ckpt_from = -1) {
+    for (int m = 0; m < k; m++) {
+        hipLaunchKernelGGL(k_tape_dual, dim3(blocks_for(t->T)), dim3(kThreads), 0, s, t->idx, state, t->T, t->cen);
+        hipLaunchKernelGGL(k_tape_primal, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->adj, (int)t->cap_v, t->deg,
+                           t->mv, t->cen, t->V, state);
+        BMF_HIP(hipGetLastError());
+        if (ckpt_from >= 0 && (ckpt_from + m + 1) % t->ckpt_every == 0) {
+            const int64_t slot = (ckpt_from + m + 1) / t->ckpt_every;
+            BMF_HIP(hipMemcpyAsync(t->ckpt + slot * t->cap_v, state, sizeof(float4) * (size_t)t->V,
+                                   hipMemcpyDeviceToDevice, s));
+        }
+    }
+    return BMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bmf_tape_create(bmf_tape** out, bmf_ctx* ctx, int32_t max_dim, int64_t max_vertices, int64_t max_indices,
+                    int32_t checkpoint_every) {
+    if (!out || !ctx) return BMF_EINVAL;
+    *out = nullptr;
+    if (max_dim < 2 || max_dim > BMF_MAX_DIM || max_vertices < 1 || max_indices < 3 || checkpoint_every < 1)
+        return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(ctx->device));
+    bmf_tape* t = new (std::nothrow) bmf_tape();
+    if (!t) return BMF_EHIP;
+    t->ctx = ctx;
+    t->max_dim = max_dim;
+    t->cap_v = max_vertices;
+    t->cap_i = max_indices;
+    t->ckpt_every = checkpoint_every;
+    const int64_t n3 = (int64_t)max_dim * max_dim * max_dim;
+    const int64_t nt = max_indices / 3 + 1;
+    const bool ok = dalloc(&t->dens, n3) == hipSuccess && dalloc(&t->vid0, n3) == hipSuccess &&
+                    dalloc(&t->vedge, max_vertices) == hipSuccess && dalloc(&t->boundary, max_vertices) == hipSuccess &&
+                    dalloc(&t->idx, max_indices) == hipSuccess &&
+                    dalloc(&t->adj, (int64_t)kMaxRing * max_vertices) == hipSuccess &&
+                    dalloc(&t->deg, max_vertices) == hipSuccess && dalloc(&t->mv, max_vertices) == hipSuccess &&
+                    dalloc(&t->p4, max_vertices) == hipSuccess && dalloc(&t->q4, max_vertices) == hipSuccess &&
+                    dalloc(&t->cen, nt) == hipSuccess && dalloc(&t->pos, 3 * max_vertices) == hipSuccess &&
+                    dalloc(&t->g, max_vertices) == hipSuccess && dalloc(&t->err, 1) == hipSuccess &&
+                    dalloc(&t->bb, 8) == hipSuccess && dalloc(&t->d2, max_vertices) == hipSuccess &&
+                    dalloc(&t->loss, 1) == hipSuccess && dalloc(&t->nearest_tmp, max_vertices) == hipSuccess;
+    if (!ok) {
+        bmf_tape_destroy(t);
+        return BMF_EHIP;
+    }
+    *out = t;
+    return BMF_OK;
+}
+
+void bmf_tape_destroy(bmf_tape* t) {
+    if (!t) return;
+    (void)hipSetDevice(t->ctx->device);
+    (void)hipStreamSynchronize(t->ctx->stream);
+    dfree(t->dens);
+    dfree(t->vid0);
+    dfree(t->vedge);
+    dfree(t->boundary);
+    dfree(t->idx);
+    dfree(t->adj);
+    dfree(t->deg);
+    dfree(t->mv);
+    dfree(t->p4);
+    dfree(t->q4);
+    dfree(t->cen);
+    dfree(t->pos);
+    dfree(t->g);
+    dfree(t->err);
+    dfree(t->ckpt);
+    dfree(t->bb);
+    dfree(t->d2);
+    dfree(t->loss);
+    dfree(t->nearest_tmp);
+    free_nn(t);
+    delete t;
+}
+
+int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities, bmf_tape* t) {
+    if (!c || !d || !t || t->ctx != c) return BMF_EINVAL;
+    int rc = check_desc(*d);
+    if (rc) return rc;
+    if (d->dim > t->max_dim) return BMF_ECAPACITY;
+    if (!densities && !c->sampler_set) return BMF_ESTATE;
+    const int dim = d->dim;
+    const int64_t n3 = (int64_t)dim * dim * dim;
+    if (n3 > c->A.cap_samples) return BMF_ECAPACITY;
+    BMF_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    BMF_HIP(hipStreamSynchronize(s));
+    t->valid = false;
+    c->have_extract = false;
+    c->results_valid = false;
+    // the sampler of this call: the context's, or the caller's densities
+    if (densities) {
+        SamplerDev ext{};
+        ext.kind = BMF_SAMPLER_EXTERNAL;
+        ext.ext = densities;
+        BMF_HIP(hipMemcpy(c->A.sampler, &ext, sizeof(SamplerDev), hipMemcpyHostToDevice));
+    }
+    rc = upload_descs(c, d, 1);
+    if (rc) return rc;
+    Arenas A = c->A;
+    A.ext_base = 0;
+    const int wcls = row_word_class(dim);
+    // extraction with every octave evaluated (densities exact everywhere),
+    // no smoothing: the tape smooths its own copy
+    hipError_t e = launch_plan(A, 1, 0, 1, s);
+    if (e == hipSuccess)
+        e = launch_sample(A, 1, (int)std::min<int64_t>((n3 + kSampleTile - 1) / kSampleTile, c->persistent_blocks), 0, s);
+    if (e == hipSuccess) e = launch_count(A, 1, dim, wcls, s);
+    if (e == hipSuccess) e = launch_refine(A, 1, dim, s);
+    if (e == hipSuccess) e = launch_scan(A, 1, 1, s);
+    if (e == hipSuccess) e = launch_emit(A, 1, dim, wcls, s);
+    if (e == hipSuccess) e = launch_finalize(A, 1, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_results, A.results, sizeof(bmf_chunk_result), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (densities) (void)hipMemcpy(c->A.sampler, &c->h_sampler, sizeof(SamplerDev), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return BMF_EHIP;
+    const bmf_chunk_result r = c->h_results[0];
+    if (c->h_ctr->error) return BMF_EHIP;
+    if (r.status != BMF_OK) return r.status;
+    if (r.vertex_count > t->cap_v || r.index_count > t->cap_i) return BMF_ECAPACITY;
+    t->dim = dim;
+    t->K = d->iters;
+    t->bp = d->boundary_processing;
+    t->V = r.vertex_count;
+    t->T = r.index_count / 3;
+    t->contains_mesh = r.contains_mesh;
+    const int64_t slots = t->K / t->ckpt_every + 1;
+    if (slots > t->ckpt_slots) {
+        dfree(t->ckpt);
+        BMF_HIP(dalloc(&t->ckpt, slots * t->cap_v));
+        t->ckpt_slots = slots;
+    }
+    const int V = t->V, T = t->T;
+    BMF_HIP(hipMemcpyAsync(t->dens, A.dens, sizeof(float) * n3, hipMemcpyDeviceToDevice, s));
+    if (V > 0) {
+        BMF_HIP(hipMemcpyAsync(t->vedge, A.vedge + r.vertex_offset, sizeof(uint32_t) * V, hipMemcpyDeviceToDevice, s));
+        BMF_HIP(hipMemcpyAsync(t->boundary, A.boundary + r.vertex_offset, V, hipMemcpyDeviceToDevice, s));
+        BMF_HIP(hipMemcpyAsync(t->idx, A.idx + r.index_offset, sizeof(uint32_t) * 3 * T, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_tape_init, dim3(blocks_for(V)), dim3(kThreads), 0, s, A.pos + 3 * r.vertex_offset, V,
+                           t->p4);
+        hipLaunchKernelGGL(k_tape_vid0, dim3(blocks_for(V)), dim3(kThreads), 0, s, t->vedge, V, dim, t->vid0);
+        BMF_HIP(hipMemsetAsync(t->err, 0, sizeof(int), s));
+        if (t->K > 0 && T > 0) {
+            // per-cell triangle bases and masks exist for chunks with K > 0
+            hipLaunchKernelGGL(k_tape_ring, dim3(blocks_for(V)), dim3(kThreads), 0, s, t->vedge, t->boundary, A.cell_tb,
+                               A.cell_mask, dim, V, T, t->bp, (int)t->cap_v, t->adj, t->deg, t->mv, t->err);
+        } else {
+            BMF_HIP(hipMemsetAsync(t->deg, 0, V, s));
+            BMF_HIP(hipMemsetAsync(t->mv, 0, V, s));
+        }
+        BMF_HIP(hipGetLastError());
+        BMF_HIP(hipMemcpyAsync(t->ckpt, t->p4, sizeof(float4) * V, hipMemcpyDeviceToDevice, s));
+        if (t->K > 0 && T > 0) {
+            rc = smooth_steps(t, t->p4, t->K, s, 0);
+            if (rc) return rc;
+        } else {
+            for (int64_t q = 1; q < slots; q++)
+                BMF_HIP(hipMemcpyAsync(t->ckpt + q * t->cap_v, t->p4, sizeof(float4) * V, hipMemcpyDeviceToDevice, s));
+        }
+        hipLaunchKernelGGL(k_tape_out, dim3(blocks_for(V)), dim3(kThreads), 0, s, t->p4, V, t->pos);
+        BMF_HIP(hipGetLastError());
+    }
+    int herr = 0;
+    BMF_HIP(hipMemcpyAsync(&herr, t->err, sizeof(int), hipMemcpyDeviceToHost, s));
+    BMF_HIP(hipStreamSynchronize(s));
+    if (V > 0 && herr) return BMF_EHIP;
+    t->valid = true;
+    return BMF_OK;
+}
+
+int bmf_tape_get(bmf_tape* t, bmf_tape_info* info) {
+    if (!t || !info) return BMF_EINVAL;
+    if (!t->valid) return BMF_ESTATE;
+    info->dim = t->dim;
+    info->iters = t->K;
+    info->checkpoint_every = t->ckpt_every;
+    info->vertex_count = t->V;
+    info->index_count = 3 * t->T;
+    info->contains_mesh = t->contains_mesh;
+    info->positions = t->pos;
+    info->indices = t->idx;
+    info->boundary = t->boundary;
+    info->densities = t->dens;
+    return BMF_OK;
+}
+
+int bmf_tape_positions(bmf_tape* t, int32_t iteration, float* out) {
+    if (!t || !out) return BMF_EINVAL;
+    if (!t->valid) return BMF_ESTATE;
+    if (iteration < 0 || iteration > t->K) return BMF_EINVAL;
+    if (t->V == 0) return BMF_OK;
+    hipStream_t s = t->ctx->stream;
+    BMF_HIP(hipSetDevice(t->ctx->device));
+    const int64_t slot = iteration / t->ckpt_every;
+    BMF_HIP(hipMemcpyAsync(t->q4, t->ckpt + slot * t->cap_v, sizeof(float4) * t->V, hipMemcpyDeviceToDevice, s));
+    const int rest = t->T > 0 ? iteration - (int)(slot * t->ckpt_every) : 0;
+    int rc = smooth_steps(t, t->q4, rest, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_tape_out, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->q4, t->V, out);
+    BMF_HIP(hipGetLastError());
+    BMF_HIP(hipStreamSynchronize(s));
+    return BMF_OK;
+}
+
+int bmf_loss_nearest(bmf_tape* t, const float* targets, int64_t n, float* dldp, int32_t* nearest, float* loss) {
+    if (!t || !targets || n < 1 || n > (int64_t)0x7FFFFFFF || !dldp) return BMF_EINVAL;
+    if (!t->valid) return BMF_ESTATE;
+    BMF_HIP(hipSetDevice(t->ctx->device));
+    hipStream_t s = t->ctx->stream;
+    if (t->V == 0) {
+        if (loss) *loss = 0.0f;
+        return BMF_OK;
+    }
+    int rc = reserve_nn(t, n);
+    if (rc) return rc;
+    const int64_t ncl = (n + kCluster - 1) / kCluster, nsup = (ncl + kSuper - 1) / kSuper;
+    hipLaunchKernelGGL(k_nn_bbox_init, dim3(1), dim3(64), 0, s, t->bb);
+    hipLaunchKernelGGL(k_nn_bbox, dim3((unsigned)std::min<int64_t>(1024, blocks_for(n))), dim3(kThreads), 0, s, targets,
+                       n, t->bb);
+    hipLaunchKernelGGL(k_nn_keys, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, n, t->bb, t->keys[0], t->vals[0]);
+    BMF_HIP(hipGetLastError());
+    hipcub::DoubleBuffer<uint32_t> kb(t->keys[0], t->keys[1]);
+    hipcub::DoubleBuffer<int32_t> vb(t->vals[0], t->vals[1]);
+    size_t bytes = t->cub_bytes;
+    BMF_HIP(hipcub::DeviceRadixSort::SortPairs(t->cub_tmp, bytes, kb, vb, (int)n, 0, 30, s));
+    hipLaunchKernelGGL(k_nn_gather, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, vb.Current(), n, t->st);
+    hipLaunchKernelGGL(k_nn_boxes, dim3(blocks_for(ncl)), dim3(kThreads), 0, s, t->st, t->st, n, kCluster, ncl, t->clo,
+                       t->chi);
+    hipLaunchKernelGGL(k_nn_boxes, dim3(blocks_for(nsup)), dim3(kThreads), 0, s, t->clo, t->chi, ncl, kSuper, nsup,
+                       t->slo, t->shi);
+    hipLaunchKernelGGL(k_nn_query, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->pos, t->V, t->st, kb.Current(), n,
+                       t->bb, t->clo, t->chi, ncl, t->slo, t->shi, nsup, nearest ? nearest : t->nearest_tmp, dldp,
+                       t->d2);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, t->d2, t->V, t->loss);
+    BMF_HIP(hipGetLastError());
+    double l = 0.0;
+    BMF_HIP(hipMemcpyAsync(&l, t->loss, sizeof(double), hipMemcpyDeviceToHost, s));
+    BMF_HIP(hipStreamSynchronize(s));
+    if (loss) *loss = (float)l;
+    return BMF_OK;
+}
+
+int bmf_backward_diff(bmf_tape* t, const float* dldp, float* dldd) {
+    if (!t || !dldd || (!dldp && t->V > 0)) return BMF_EINVAL;
+    if (!t->valid) return BMF_ESTATE;
+    BMF_HIP(hipSetDevice(t->ctx->device));
+    hipStream_t s = t->ctx->stream;
+    const int64_t n3 = (int64_t)t->dim * t->dim * t->dim;
+    if (t->V == 0) {
+        BMF_HIP(hipMemsetAsync(dldd, 0, sizeof(float) * n3, s));
+        BMF_HIP(hipStreamSynchronize(s));
+        return BMF_OK;
+    }
+    hipLaunchKernelGGL(k_bwd_init, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, dldp, t->V, t->g);
+    if (t->T > 0)
+        for (int m = t->K - 1; m >= 0; m--) {
+            hipLaunchKernelGGL(k_bwd_tri, dim3(blocks_for(t->T)), dim3(kThreads), 0, s, t->idx, t->g, t->deg, t->mv,
+                               t->T, t->cen);
+            hipLaunchKernelGGL(k_bwd_vert, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->adj, (int)t->cap_v, t->deg,
+                               t->mv, t->cen, t->V, t->g);
+        }
+    hipLaunchKernelGGL(k_bwd_place, dim3(blocks_for(n3)), dim3(kThreads), 0, s, t->dens, t->vid0, t->g, t->dim, dldd);
+    BMF_HIP(hipGetLastError());
+    BMF_HIP(hipStreamSynchronize(s));
+    return BMF_OK;
+}
+
+}  // extern "C"

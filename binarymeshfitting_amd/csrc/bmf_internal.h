@@ -37,6 +37,7 @@ struct SamplerDev {
     float ao[BMF_MAX_OCTAVES];  // per-octave amplitude g^o
     const float* spheres;       // device, n_spheres x 4
     const float* boxes;         // device, n_boxes x 6
+    const float* ext;           // EXTERNAL: caller's densities, chunks packed in descriptor order
 };
 
 // Per-chunk state in device memory.
@@ -109,6 +110,7 @@ struct Arenas {
     SamplerDev* sampler;
     int64_t cap_samples, cap_rows, cap_vertices, cap_indices;
     int32_t cap_chunks;
+    int64_t ext_base;          // EXTERNAL: offset of this launch's first chunk in the caller's densities
 };
 
 inline int row_word_class(int dim) { return 1 << (((dim + 63) >> 6) - 1); }

@@ -17,25 +17,11 @@
 // the hot path: the plane kernels rebuild them from the bitmap with shifts.
 #include <hip/hip_runtime.h>
 
-#include "../../include/bmf_mc_tables.h"
+#include "bmf_device_tables.h"
 #include "bmf_internal.h"
 #include "samplers.h"
 
 namespace bmf {
-
-__constant__ uint64_t c_tri_table[256] = BMF_TRI_TABLE_INIT;
-
-// The four cells around a lattice edge along `axis`, in lexicographic cell
-// order, as (dx, dy, dz, local edge) offsets from the edge origin; local
-// edges numbered as in DMCChunk::polygonize_cell (DMCChunk.cpp:543-565).
-// Visiting them in this order and each cell's table slots in order lists
-// the edge's triangles in ascending id, the order in which
-// MeshProcessor::init_primitives appends them (MeshProcessor.cpp:114-127).
-__constant__ int8_t c_ring[3][4][4] = {
-    {{0, -1, -1, 3}, {0, -1, 0, 2}, {0, 0, -1, 1}, {0, 0, 0, 0}},
-    {{-1, 0, -1, 7}, {-1, 0, 0, 6}, {0, 0, -1, 5}, {0, 0, 0, 4}},
-    {{-1, -1, 0, 11}, {-1, 0, 0, 10}, {0, -1, 0, 9}, {0, 0, 0, 8}},
-};
 
 // u64 words of one staged label plane for row-word class W (dim <= 64 W),
 // plus the zero pad word.
@@ -429,7 +415,8 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int 
                     done = dec != 0;
                     air = dec == 1;
                 } else {
-                    const float d = sample_density(*S, px, py, pz);
+                    const float d = S->kind == BMF_SAMPLER_EXTERNAL ? S->ext[A.ext_base + M.dens_off + i]
+                                                                    : sample_density(*S, px, py, pz);
                     dens[i] = d;
                     air = d < 0.0f;
                 }

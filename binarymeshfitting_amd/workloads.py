@@ -32,6 +32,17 @@ class SplitMix64:
         """integer in [lo, hi] inclusive."""
         return lo + int(self.next() % (hi - lo + 1))
 
+    def uniforms(self, n):
+        """The next n uniform() values at once (same stream, vectorised)."""
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.state) + k * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        self.state = (self.state + n * 0x9E3779B97F4A7C15) & MASK64
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
 
 @dataclass
 class TerrainParams:
@@ -127,7 +138,7 @@ def sphere_targets(n=200_000, seed=3, radius=40.0, centre=(64.0, 64.0, 64.0)):
     """Points uniform on a sphere (normalised Gaussian directions from
     Box-Muller over splitmix64 uniforms)."""
     rng = SplitMix64(seed)
-    u = np.array([rng.uniform() for _ in range(4 * n)]).reshape(n, 4)
+    u = rng.uniforms(4 * n).reshape(n, 4)
     u1 = np.maximum(u[:, 0], 1e-300)
     r = np.sqrt(-2.0 * np.log(u1))
     g0 = r * np.cos(2 * np.pi * u[:, 1])

@@ -60,7 +60,9 @@ enum {
     BMF_SAMPLER_CUBOID = 2,  /* ImplicitFunctions::cuboid, ImplicitSampler.cpp:52-60 */
     BMF_SAMPLER_TORUS_Z = 3, /* ImplicitFunctions::torus_z, ImplicitSampler.cpp:27-34 */
     BMF_SAMPLER_PLANE_Y = 4, /* ImplicitFunctions::plane_y, ImplicitSampler.cpp:62-65 */
-    BMF_SAMPLER_CSG = 5      /* max-union of spheres and boxes, floored at -16 */
+    BMF_SAMPLER_CSG = 5,     /* max-union of spheres and boxes, floored at -16 */
+    BMF_SAMPLER_EXTERNAL = 6 /* densities supplied by the caller in device memory (any
+                                host sampler's Sampler::block output, Sampler.hpp:20-34) */
 };
 
 typedef struct {
@@ -74,6 +76,9 @@ typedef struct {
     int32_t n_spheres;
     const float* boxes;               /* CSG, host memory: n_boxes x (cx, cy, cz, hx, hy, hz) */
     int32_t n_boxes;
+    const float* densities;           /* EXTERNAL, device memory: the dim^3 FP32 [x][y][z] blocks
+                                         of the chunks of each later bmf_extract, packed in
+                                         descriptor order; must stay valid while work is queued */
 } bmf_sampler_desc;
 
 /* One chunk: DMCChunk fields (DMCChunk.hpp:24-60) plus the per-chunk
@@ -168,6 +173,65 @@ int bmf_last_timing(bmf_ctx* ctx, float* total_ms);
  * stage timers (DMCChunk.cpp:785-840, ChunkGenerator.cpp:153-193). */
 #define BMF_NUM_STAGES 7
 int bmf_profile_stages(bmf_ctx* ctx, const bmf_chunk_desc* chunks, int n, float* stage_ms);
+
+/* ---- differentiable extraction --------------------------------------- */
+/* One chunk's density-positioned vertices plus K smoothing iterations as a
+ * function of its dim^3 densities, with the topology (labels, cells,
+ * triangles) frozen by the forward pass.  The reference has no adjoint; the
+ * forward is ChunkGenerator::extract_chunk for one chunk (ChunkGenerator.cpp:
+ * 80-147) and the backward is its transpose: smoothing (MeshProcessor.cpp:
+ * 130-306) is linear in the positions for a fixed topology, and vertex
+ * placement (DMCChunk.cpp:657-674) depends on the two densities of its edge
+ * through mu = (0 - s0) / (s1 - s0).
+ *
+ * A tape belongs to one context and uses its stream and arenas for the
+ * forward extraction; it owns copies of everything the backward reads, so
+ * later bmf_extract calls on the context do not disturb it. */
+typedef struct bmf_tape bmf_tape;
+
+typedef struct {
+    int32_t dim;
+    int32_t iters;            /* K */
+    int32_t checkpoint_every; /* smoothing states kept every this many iterations */
+    int32_t vertex_count;
+    int32_t index_count;
+    int32_t contains_mesh;
+    const float* positions;   /* device [V][3]: positions after K iterations, grid units */
+    const uint32_t* indices;  /* device [index_count] */
+    const uint8_t* boundary;  /* device [V] */
+    const float* densities;   /* device [dim^3]: the densities the forward used */
+} bmf_tape_info;
+
+/* max_dim bounds the chunk's samples per axis; max_vertices / max_indices
+ * bound its mesh; checkpoint_every >= 1. */
+int bmf_tape_create(bmf_tape** out, bmf_ctx* ctx, int32_t max_dim, int64_t max_vertices, int64_t max_indices,
+                    int32_t checkpoint_every);
+void bmf_tape_destroy(bmf_tape* tape);
+
+/* Forward pass for one chunk.  `densities` is a device pointer to its dim^3
+ * FP32 block, or NULL to evaluate the context's sampler at every sample
+ * (all octaves, no early exit).  Positions after iterations 0, c, 2c, ...
+ * (c = checkpoint_every) are kept on the tape.  Synchronises the stream. */
+int bmf_forward_diff(bmf_ctx* ctx, const bmf_chunk_desc* chunk, const float* densities, bmf_tape* tape);
+int bmf_tape_get(bmf_tape* tape, bmf_tape_info* info);
+
+/* Positions after smoothing iteration k (0 <= k <= K) into a device buffer
+ * [V][3], recomputed from the last checkpoint at or before k; bit-identical
+ * to the forward's own state at k. */
+int bmf_tape_positions(bmf_tape* tape, int32_t iteration, float* positions);
+
+/* Loss L = (1/V) sum_v |p_v - t(v)|^2 over the tape's final positions, t(v)
+ * the nearest of the n device targets [n][3] (squared FP32 distance
+ * ((dx*dx + dy*dy) + dz*dz); ties go to the lower target index).  Writes
+ * dL/dp [V][3] (device), the chosen target per vertex (device, may be NULL)
+ * and L to host memory (may be NULL). */
+int bmf_loss_nearest(bmf_tape* tape, const float* targets, int64_t n_targets, float* dloss_dpos, int32_t* nearest,
+                     float* loss);
+
+/* Backward pass: dL/d(density) [dim^3] (device) from dL/d(final positions)
+ * [V][3] (device).  Samples that are not an endpoint of a sign-changing
+ * edge get 0. */
+int bmf_backward_diff(bmf_tape* tape, const float* dloss_dpos, float* dloss_ddensity);
 
 #ifdef __cplusplus
 }

@@ -122,6 +122,25 @@ int or_extract_batch(const or_sampler_t* s, const or_chunk_t* chunks, int n, int
 float or_qef_solve_f32(const float* p, const float* n, int count, float tol, float out[3]);
 double or_qef_solve_f64(const double* p, const double* n, int count, double tol, double out[3]);
 
+/* ---- differentiable extraction, FP64 (oracle_diff.c) ----
+ * Topology frozen: edge[] / boundary[] / idx[] of an or_mesh_t. */
+void or_place_f64(const double* dens, int dim, const uint32_t* edge, int64_t V, double* pos);
+void or_forward_f64(const double* dens, int dim, const uint32_t* edge, const uint8_t* boundary, int64_t V,
+                    const uint32_t* idx, int64_t nI, int K, int bp, double* pos);
+void or_jvp_f64(const double* dens, int dim, const uint32_t* edge, const uint8_t* boundary, int64_t V,
+                const uint32_t* idx, int64_t nI, int K, int bp, const double* ddens, double* dpos);
+void or_adjoint_f64(const double* dens, int dim, const uint32_t* edge, const uint8_t* boundary, int64_t V,
+                    const uint32_t* idx, int64_t nI, int K, int bp, const double* dldp, double* dldd);
+typedef struct or_kdtree or_kdtree;
+or_kdtree* or_kd_build(const double* pts, int64_t n);
+void or_kd_free(or_kdtree* t);
+void or_kd_nearest(const or_kdtree* t, const double* q, int64_t m, int64_t* out, double* d2, int threads);
+double or_loss_nearest_f64(const double* pos, int64_t V, const double* targets, const int64_t* nearest,
+                           double* dldp);
+void or_fd_grad_f64(const double* dens, int dim, const uint32_t* edge, const uint8_t* boundary, int64_t V,
+                    const uint32_t* idx, int64_t nI, int K, int bp, const or_kdtree* kd, const int64_t* samples,
+                    int m, double h, int threads, double* out, int32_t* kink);
+
 #ifdef __cplusplus
 }
 #endif

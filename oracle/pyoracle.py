@@ -106,6 +106,18 @@ def lib():
         L.or_qef_solve_f32.restype = C.c_float
         L.or_qef_solve_f64.argtypes = [f64p, f64p, C.c_int, C.c_double, f64p]
         L.or_qef_solve_f64.restype = C.c_double
+        topo = [C.c_int, u32p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int]
+        L.or_place_f64.argtypes = [f64p, C.c_int, u32p, C.c_int64, f64p]
+        L.or_forward_f64.argtypes = [f64p] + topo + [f64p]
+        L.or_jvp_f64.argtypes = [f64p] + topo + [f64p, f64p]
+        L.or_adjoint_f64.argtypes = [f64p] + topo + [f64p, f64p]
+        L.or_kd_build.argtypes = [f64p, C.c_int64]
+        L.or_kd_build.restype = C.c_void_p
+        L.or_kd_free.argtypes = [C.c_void_p]
+        L.or_kd_nearest.argtypes = [C.c_void_p, f64p, C.c_int64, i64p, f64p, C.c_int]
+        L.or_loss_nearest_f64.argtypes = [f64p, C.c_int64, f64p, i64p, f64p]
+        L.or_loss_nearest_f64.restype = C.c_double
+        L.or_fd_grad_f64.argtypes = [f64p] + topo + [C.c_void_p, i64p, C.c_int, C.c_double, C.c_int, f64p, i32p]
         _lib = L
     return _lib
 
@@ -262,3 +274,87 @@ def checksum_mesh(pos, idx):
     with np.errstate(over="ignore"):
         h = np.sum(w * (np.uint64(2) * k + np.uint64(1)), dtype=np.uint64)
         return int(h * np.uint64(0x9E3779B97F4A7C15))
+
+
+# ---- differentiable extraction (FP64, frozen topology) -------------------------------------
+def _topo(mesh, dens):
+    dim = dens.shape[0]
+    e = np.ascontiguousarray(mesh["edge"], np.uint32)
+    b = np.ascontiguousarray(mesh["boundary"], np.uint8)
+    i = np.ascontiguousarray(mesh["idx"], np.uint32)
+    keep = (e, b, i)
+    return keep, [dim, _ptr(e, C.c_uint32), _ptr(b, C.c_uint8), len(e), _ptr(i, C.c_uint32), len(i)]
+
+
+def forward_f64(dens, mesh, K, bp=False):
+    """Final positions (V, 3) from FP64 densities on the mesh's topology."""
+    d = np.ascontiguousarray(dens, np.float64)
+    keep, t = _topo(mesh, d)
+    out = np.zeros((len(keep[0]), 3), np.float64)
+    lib().or_forward_f64(_ptr(d, C.c_double), *t, K, int(bp), _ptr(out, C.c_double))
+    return out
+
+
+def jvp_f64(dens, mesh, K, bp, ddens):
+    d = np.ascontiguousarray(dens, np.float64)
+    dd = np.ascontiguousarray(ddens, np.float64)
+    keep, t = _topo(mesh, d)
+    out = np.zeros((len(keep[0]), 3), np.float64)
+    lib().or_jvp_f64(_ptr(d, C.c_double), *t, K, int(bp), _ptr(dd, C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def adjoint_f64(dens, mesh, K, bp, dldp):
+    d = np.ascontiguousarray(dens, np.float64)
+    g = np.ascontiguousarray(dldp, np.float64)
+    keep, t = _topo(mesh, d)
+    out = np.zeros(d.shape, np.float64)
+    lib().or_adjoint_f64(_ptr(d, C.c_double), *t, K, int(bp), _ptr(g, C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+class KDTree:
+    """Exact nearest target in FP64; ties go to the lower target index."""
+
+    def __init__(self, targets):
+        self.targets = np.ascontiguousarray(targets, np.float64)
+        self.h = lib().or_kd_build(_ptr(self.targets, C.c_double), len(self.targets))
+
+    def nearest(self, q, threads=8):
+        q = np.ascontiguousarray(q, np.float64)
+        idx = np.zeros(len(q), np.int64)
+        d2 = np.zeros(len(q), np.float64)
+        lib().or_kd_nearest(self.h, _ptr(q, C.c_double), len(q), _ptr(idx, C.c_int64), _ptr(d2, C.c_double), threads)
+        return idx, d2
+
+    def close(self):
+        if self.h:
+            lib().or_kd_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def loss_nearest_f64(pos, targets, nearest):
+    p = np.ascontiguousarray(pos, np.float64)
+    t = np.ascontiguousarray(targets, np.float64)
+    n = np.ascontiguousarray(nearest, np.int64)
+    g = np.zeros_like(p)
+    L = lib().or_loss_nearest_f64(_ptr(p, C.c_double), len(p), _ptr(t, C.c_double), _ptr(n, C.c_int64),
+                                  _ptr(g, C.c_double))
+    return L, g
+
+
+def fd_grad_f64(dens, mesh, K, bp, kd, samples, h=1e-3, threads=8):
+    """Central differences of the FP64 nearest-target loss along single
+    densities, and a flag per sample where a nearest target switched
+    within the stencil (no derivative there)."""
+    d = np.ascontiguousarray(dens, np.float64)
+    keep, t = _topo(mesh, d)
+    smp = np.ascontiguousarray(samples, np.int64)
+    out = np.zeros(len(smp), np.float64)
+    kink = np.zeros(len(smp), np.int32)
+    lib().or_fd_grad_f64(_ptr(d, C.c_double), *t, K, int(bp), kd.h, _ptr(smp, C.c_int64), len(smp), h, threads,
+                         _ptr(out, C.c_double), _ptr(kink, C.c_int32))
+    return out, kink.astype(bool)
