@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
+    ap.add_argument("--lod-steps", type=int, default=10,
+                    help="timed batched steps of the uneven-LOD leg (1,536 chunks); 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=5,
                     help="timed steps of the forward+loss+backward leg (one 128^3-cell chunk, K=8); 0 skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -67,6 +69,46 @@ def copy_calibration(mib):
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def lod_leg(capi, workloads, steps):
+    """Uneven-LOD batch: 1,536 chunks (922 / 461 / 153 of 32^3 / 64^3 / 128^3
+    cells at spacing 1 / 2 / 4, origins from seed 7 in a 4096^3 world), 7-octave
+    field, K ~ U{0..8}; shapes from the autotuner, then batched and plain."""
+    wl = workloads.mixed_lod()
+    cd = workloads.to_capi(wl.chunks)
+    n = len(cd)
+    arr = capi.chunk_array(cd)
+    with capi.Context(max_chunks=2048, max_samples=96 * 129 ** 3, max_vertices=1 << 23, max_indices=1 << 25) as ctx:
+        ctx.set_sampler(**wl.sampler)
+        t0 = time.perf_counter()
+        tuned = ctx.autotune(cd, reps=3)
+        t_tune = time.perf_counter() - t0
+        for _ in range(2):
+            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        ctx.sync()
+        wall = (time.perf_counter() - t0) / steps
+        rc, res, _, _ = ctx.results(n)
+        capi.check(rc, "bmf_result")
+        sampled = sum(r.sampled for r in res)
+        nonempty = sum(1 for r in res if r.vertex_count)
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
+        ctx.sync()
+        t0 = time.perf_counter()
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
+        ctx.sync()
+        plain = time.perf_counter() - t0
+    log(f"[bench] uneven LOD: {n / wall:.1f} chunks/s batched ({1e3 * wall:.3f} ms/step), plain {n / plain:.1f} chunks/s,"
+        f" sampled {sampled}/{n}, non-empty {nonempty}/{n}, tuned {tuned} in {t_tune:.2f} s")
+    return {"workload": "uneven-lod-1536: 922/461/153 chunks of 32^3/64^3/128^3 cells, spacing 1/2/4, 7 octaves, K~U{0..8}",
+            "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4),
+            "cells_per_s": round(wl.cells / wall, 1), "plain_chunks_per_s": round(n / plain, 2),
+            "batched_over_plain": round(plain / wall, 2), "sampled_chunks": sampled, "nonempty_chunks": nonempty,
+            "nonempty_fraction": round(nonempty / n, 4), "tuning": tuned, "autotune_s": round(t_tune, 3)}
 
 
 def diff_leg(capi, roofline, workloads, steps):
@@ -215,6 +257,11 @@ def main():
         plain = n * args.plain_steps / (time.perf_counter() - t0)
         log(f"[bench] plain: {plain:.1f} chunks/s")
 
+    # ---- uneven LOD: 1,536 chunks of 32/64/128 cells, autotuned shapes ------------------
+    lod = None
+    if args.lod_steps > 0:
+        lod = lod_leg(capi, workloads, args.lod_steps)
+
     # ---- forward + loss + backward: one 128^3-cell chunk, K=8, 200k targets ---------
     diff = None
     if args.diff_steps > 0:
@@ -275,6 +322,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roof,
         "path_roofline": path,
+        "uneven_lod": lod,
         "fwd_bwd": diff,
         "cpu_baseline": cpu,
         "parity": parity,

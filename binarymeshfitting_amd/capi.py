@@ -21,8 +21,11 @@ EXPORTS = [
     "bmf_extract", "bmf_result", "bmf_mesh_view_get", "bmf_mesh_to_host", "bmf_sync", "bmf_sample_block",
     "bmf_label_block", "bmf_cell_masks", "bmf_last_timing", "bmf_profile_stages", "bmf_tape_create",
     "bmf_tape_destroy", "bmf_forward_diff", "bmf_tape_get", "bmf_tape_positions", "bmf_loss_nearest",
-    "bmf_backward_diff",
+    "bmf_backward_diff", "bmf_get_tuning", "bmf_set_tuning", "bmf_autotune", "bmf_set_mesh_options",
+    "bmf_get_mesh_options", "bmf_qef_error_to_host",
 ]
+MESH_TRIANGLES, MESH_QUADS = 0, 1
+PLACE_MASS_POINT, PLACE_QEF = 0, 1
 NUM_STAGES = 7
 STAGES = ["plan", "sample", "count", "scan", "emit", "smooth", "finalize"]
 
@@ -55,6 +58,19 @@ class ChunkResult(C.Structure):
 class MeshView(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("indices", C.c_void_p), ("boundary", C.c_void_p),
                 ("total_vertices", C.c_int64), ("total_indices", C.c_int64)]
+
+
+class MeshOptions(C.Structure):
+    _fields_ = [("mesh", C.c_int32), ("placement", C.c_int32), ("qef_tolerance", C.c_float),
+                ("quads_as_triangles", C.c_int32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("sample_tile", C.c_int32), ("sample_blocks", C.c_int32), ("row_blocks", C.c_int32),
+                ("smooth_blocks", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class TapeInfo(C.Structure):
@@ -94,6 +110,12 @@ def lib():
         L.bmf_cell_masks.argtypes = [ctxp, P(ChunkDesc), P(C.c_uint64)]
         L.bmf_last_timing.argtypes = [ctxp, P(C.c_float)]
         L.bmf_profile_stages.argtypes = [ctxp, P(ChunkDesc), C.c_int, P(C.c_float)]
+        L.bmf_set_mesh_options.argtypes = [ctxp, P(MeshOptions)]
+        L.bmf_get_mesh_options.argtypes = [ctxp, P(MeshOptions)]
+        L.bmf_qef_error_to_host.argtypes = [ctxp, C.c_int, P(C.c_float), C.c_int64]
+        L.bmf_get_tuning.argtypes = [ctxp, P(Tuning)]
+        L.bmf_set_tuning.argtypes = [ctxp, P(Tuning)]
+        L.bmf_autotune.argtypes = [ctxp, P(ChunkDesc), C.c_int, C.c_int, P(Tuning)]
         tp = C.c_void_p
         L.bmf_tape_create.argtypes = [P(tp), ctxp, C.c_int32, C.c_int64, C.c_int64, C.c_int32]
         L.bmf_tape_destroy.argtypes = [tp]
@@ -222,6 +244,38 @@ class Context:
         t = C.c_float()
         check(lib().bmf_last_timing(self.h, C.byref(t)), "bmf_last_timing")
         return t.value
+
+    def set_mesh_options(self, mesh=MESH_TRIANGLES, placement=PLACE_MASS_POINT, qef_tolerance=0.1,
+                         quads_as_triangles=False):
+        o = MeshOptions(mesh, placement, qef_tolerance, int(quads_as_triangles))
+        check(lib().bmf_set_mesh_options(self.h, C.byref(o)), "bmf_set_mesh_options")
+
+    def mesh_options(self):
+        o = MeshOptions()
+        check(lib().bmf_get_mesh_options(self.h, C.byref(o)), "bmf_get_mesh_options")
+        return {k: getattr(o, k) for k, _ in o._fields_}
+
+    def qef_errors(self, chunk):
+        rc, res, _, _ = self.results(chunk + 1)
+        V = res[chunk].vertex_count
+        out = np.zeros(V, np.float32)
+        check(lib().bmf_qef_error_to_host(self.h, chunk, _ptr(out, C.c_float), V), "bmf_qef_error_to_host")
+        return out
+
+    def tuning(self):
+        t = Tuning()
+        check(lib().bmf_get_tuning(self.h, C.byref(t)), "bmf_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, sample_tile=0, sample_blocks=0, row_blocks=0, smooth_blocks=0):
+        t = Tuning(sample_tile, sample_blocks, row_blocks, smooth_blocks)
+        check(lib().bmf_set_tuning(self.h, C.byref(t)), "bmf_set_tuning")
+
+    def autotune(self, chunks, reps=3):
+        arr = chunk_array(chunks)
+        t = Tuning()
+        check(lib().bmf_autotune(self.h, arr, len(chunks), reps, C.byref(t)), "bmf_autotune")
+        return t.as_dict()
 
     def profile_stages(self, chunks):
         arr = chunk_array(chunks)

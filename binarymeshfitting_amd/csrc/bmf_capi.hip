@@ -38,7 +38,7 @@ void free_arenas(bmf_ctx* c) {
                     A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
                     A.pos,     A.pos4,      A.boundary, A.vedge,       A.idx,          A.ctr,
-                    A.sampler, c->d_spheres, c->d_boxes};
+                    A.sampler, c->d_spheres, c->d_boxes, A.ring, A.ring_n, A.qef_err};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     A = Arenas{};
@@ -57,14 +57,20 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
     BMF_MARK(1);
     BMF_HIP(launch_sample(A, n, sample_blocks, 1, s));
     BMF_MARK(2);
+    const bool quads = A.mesh_mode == BMF_MESH_QUADS;
     BMF_HIP(launch_count(A, n, row_blocks, wclasses, s));
     BMF_HIP(launch_refine(A, n, row_blocks, s));
+    if (quads) BMF_HIP(launch_dmc_count(A, n, row_blocks, wclasses, s));
     BMF_MARK(3);
     BMF_HIP(launch_scan(A, n, scan_blocks, s));
     BMF_MARK(4);
-    BMF_HIP(launch_emit(A, n, row_blocks, wclasses, s));
+    if (quads) BMF_HIP(launch_dmc_emit(A, n, row_blocks, wclasses, s));
+    else BMF_HIP(launch_emit(A, n, row_blocks, wclasses, s));
     BMF_MARK(5);
-    for (int m = 0; m < maxk; m++) BMF_HIP(launch_smooth(A, m, smooth_blocks, s));
+    for (int m = 0; m < maxk; m++) {
+        if (quads) BMF_HIP(launch_dmc_smooth(A, m, smooth_blocks, s));
+        else BMF_HIP(launch_smooth(A, m, smooth_blocks, s));
+    }
     BMF_MARK(6);
     BMF_HIP(launch_finalize(A, n, s));
     BMF_MARK(7);
@@ -146,6 +152,10 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         c->cu_count = prop.multiProcessorCount;
     c->persistent_blocks = c->cu_count * 8;
+    c->tune.sample_tile = kSampleTileDefault;
+    c->tune.sample_blocks = c->persistent_blocks;
+    c->tune.row_blocks = c->persistent_blocks;
+    c->tune.smooth_blocks = 4 * c->persistent_blocks;
     Arenas& A = c->A;
     const int64_t nc = lim->max_chunks;
     A.cap_chunks = (int32_t)nc;
@@ -153,6 +163,7 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     A.cap_rows = lim->max_samples / 2 + nc;
     A.cap_vertices = lim->max_vertices;
     A.cap_indices = lim->max_indices;
+    A.sample_tile = kSampleTileDefault;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && dalloc(&A.meta, nc) == hipSuccess && dalloc(&A.descs, nc) == hipSuccess &&
          dalloc(&A.results, nc) == hipSuccess && dalloc(&A.tile_prefix, nc + 1) == hipSuccess &&
@@ -297,7 +308,7 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
             Aj.results = A.results + j;
             const int64_t dim = chunks[j].dim;
             const int64_t ns = dim * dim * dim;
-            const int sb = (int)std::min<int64_t>(ceil_div(ns, kSampleTile), pb);
+            const int sb = (int)std::min<int64_t>(ceil_div(ns, A.sample_tile), pb);
             rc = enqueue_pipeline(c, Aj, 1, chunks[j].iters, row_word_class(chunks[j].dim), 1, j == 0, sb, (int)dim, 1,
                                   std::min(pb, 1024));
             if (rc) return rc;
@@ -313,7 +324,8 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
     if (!ge) {
         hipGraph_t graph = nullptr;
         BMF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, pb, pb, std::min(n, c->cu_count), 4 * pb);
+        rc = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, c->tune.sample_blocks, c->tune.row_blocks,
+                              std::min(n, c->cu_count), c->tune.smooth_blocks);
         hipGraph_t captured = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &captured);
         if (rc) {
@@ -429,7 +441,7 @@ static int stage_one(bmf_ctx* c, const bmf_chunk_desc* d, bool scan) {
     if (rc) return rc;
     const Arenas& A = c->A;
     BMF_HIP(launch_plan(A, 1, 0, 1, c->stream));
-    BMF_HIP(launch_sample(A, 1, (int)std::min<int64_t>(ceil_div(ns, kSampleTile), c->persistent_blocks), 0,
+    BMF_HIP(launch_sample(A, 1, (int)std::min<int64_t>(ceil_div(ns, A.sample_tile), c->persistent_blocks), 0,
                           c->stream));
     if (scan) {
         BMF_HIP(launch_count(A, 1, d->dim, row_word_class(d->dim), c->stream));
@@ -508,15 +520,204 @@ int bmf_profile_stages(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, float* s
         maxk = std::max(maxk, chunks[i].iters);
         wclasses |= row_word_class(chunks[i].dim);
     }
-    const int pb = c->persistent_blocks;
     BMF_HIP(hipEventRecord(c->ev0, c->stream));
-    rc = enqueue_pipeline(c, c->A, n, maxk, wclasses, 1, 1, pb, pb, std::min(n, c->cu_count), 4 * pb,
-                          c->ev_stage);
+    rc = enqueue_pipeline(c, c->A, n, maxk, wclasses, 1, 1, c->tune.sample_blocks, c->tune.row_blocks,
+                          std::min(n, c->cu_count), c->tune.smooth_blocks, c->ev_stage);
     if (rc) return rc;
     BMF_HIP(hipEventRecord(c->ev1, c->stream));
     BMF_HIP(hipStreamSynchronize(c->stream));
     for (int i = 0; i < BMF_NUM_STAGES; i++)
         BMF_HIP(hipEventElapsedTime(&stage_ms[i], c->ev_stage[i], c->ev_stage[i + 1]));
+    return BMF_OK;
+}
+
+int bmf_set_mesh_options(bmf_ctx* c, const bmf_mesh_options* o) {
+    if (!c || !o) return BMF_EINVAL;
+    if (o->mesh != BMF_MESH_TRIANGLES && o->mesh != BMF_MESH_QUADS) return BMF_EINVAL;
+    if (o->placement != BMF_PLACE_MASS_POINT && o->placement != BMF_PLACE_QEF) return BMF_EINVAL;
+    if (!finite(o->qef_tolerance) || o->qef_tolerance < 0.0f) return BMF_EINVAL;
+    if (o->quads_as_triangles != 0 && o->quads_as_triangles != 1) return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(c->device));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    Arenas& A = c->A;
+    if (o->mesh == BMF_MESH_QUADS && !A.ring) {
+        if (dalloc(&A.ring, (int64_t)kRingSlots * A.cap_vertices) != hipSuccess ||
+            dalloc(&A.ring_n, A.cap_vertices) != hipSuccess || dalloc(&A.qef_err, A.cap_vertices) != hipSuccess) {
+            if (A.ring) (void)hipFree(A.ring);
+            if (A.ring_n) (void)hipFree(A.ring_n);
+            if (A.qef_err) (void)hipFree(A.qef_err);
+            A.ring = A.ring_n = nullptr;
+            A.qef_err = nullptr;
+            return BMF_EHIP;
+        }
+    }
+    c->mesh = *o;
+    A.mesh_mode = o->mesh;
+    A.placement = o->placement;
+    A.qef_tol = o->qef_tolerance;
+    A.quad_tris = o->quads_as_triangles;
+    for (auto& g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+    c->have_extract = false;
+    c->results_valid = false;
+    return BMF_OK;
+}
+
+int bmf_get_mesh_options(bmf_ctx* c, bmf_mesh_options* o) {
+    if (!c || !o) return BMF_EINVAL;
+    *o = c->mesh;
+    return BMF_OK;
+}
+
+int bmf_qef_error_to_host(bmf_ctx* c, int chunk, float* errors, int64_t cap_v) {
+    if (!c || !errors) return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(c->device));
+    int rc = fetch_results(c);
+    if (rc) return rc;
+    if (chunk < 0 || chunk >= c->last_n) return BMF_EINVAL;
+    const bmf_chunk_result& r = c->h_results[chunk];
+    if (r.status != BMF_OK) return r.status;
+    if (cap_v < r.vertex_count) return BMF_ECAPACITY;
+    if (!r.vertex_count) return BMF_OK;
+    if (c->A.qef_err && c->mesh.mesh == BMF_MESH_QUADS && c->mesh.placement == BMF_PLACE_QEF) {
+        BMF_HIP(hipMemcpyAsync(errors, c->A.qef_err + r.vertex_offset, sizeof(float) * r.vertex_count,
+                               hipMemcpyDeviceToHost, c->stream));
+        BMF_HIP(hipStreamSynchronize(c->stream));
+    } else {
+        std::memset(errors, 0, sizeof(float) * (size_t)r.vertex_count);
+    }
+    return BMF_OK;
+}
+
+int bmf_get_tuning(bmf_ctx* c, bmf_tuning* t) {
+    if (!c || !t) return BMF_EINVAL;
+    *t = c->tune;
+    return BMF_OK;
+}
+
+static void drop_graphs(bmf_ctx* c) {
+    for (auto& g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
+int bmf_set_tuning(bmf_ctx* c, const bmf_tuning* t) {
+    if (!c || !t) return BMF_EINVAL;
+    bmf_tuning n = c->tune;
+    if (t->sample_tile) {
+        if (t->sample_tile != 1024 && t->sample_tile != 2048 && t->sample_tile != 4096) return BMF_EINVAL;
+        n.sample_tile = t->sample_tile;
+    }
+    const int32_t cap = 64 * c->cu_count;  // at most 64 workgroups per CU for any kernel
+    if (t->sample_blocks < 0 || t->row_blocks < 0 || t->smooth_blocks < 0) return BMF_EINVAL;
+    if (t->sample_blocks > cap || t->row_blocks > cap || t->smooth_blocks > cap) return BMF_EINVAL;
+    if (t->sample_blocks) n.sample_blocks = t->sample_blocks;
+    if (t->row_blocks) n.row_blocks = t->row_blocks;
+    if (t->smooth_blocks) n.smooth_blocks = (t->smooth_blocks + kXcds - 1) / kXcds * kXcds;
+    BMF_HIP(hipSetDevice(c->device));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    c->tune = n;
+    c->A.sample_tile = n.sample_tile;
+    drop_graphs(c);
+    return BMF_OK;
+}
+
+int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_tuning* best) {
+    if (!c || reps < 1 || n < 1) return BMF_EINVAL;
+    if (!c->sampler_set) return BMF_ESTATE;
+    int rc = bmf_validate_chunks(chunks, n, &c->lim);
+    if (rc) return rc;
+    BMF_HIP(hipSetDevice(c->device));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    c->results_valid = false;
+    c->have_extract = false;
+    rc = upload_descs(c, chunks, n);
+    if (rc) return rc;
+    int maxk = 0, wclasses = 0;
+    for (int i = 0; i < n; i++) {
+        maxk = std::max(maxk, chunks[i].iters);
+        wclasses |= row_word_class(chunks[i].dim);
+    }
+    // stage time (ms, best of reps) of the pipeline under tuning t
+    float st[BMF_NUM_STAGES];
+    auto run = [&](const bmf_tuning& t, float* out) -> int {
+        Arenas A = c->A;
+        A.sample_tile = t.sample_tile;
+        for (int i = 0; i < BMF_NUM_STAGES; i++) out[i] = 1e30f;
+        for (int r = 0; r < reps; r++) {
+            int e = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, t.sample_blocks, t.row_blocks,
+                                     std::min(n, c->cu_count), t.smooth_blocks, c->ev_stage);
+            if (e) return e;
+            BMF_HIP(hipStreamSynchronize(c->stream));
+            for (int i = 0; i < BMF_NUM_STAGES; i++) {
+                float ms = 0.0f;
+                BMF_HIP(hipEventElapsedTime(&ms, c->ev_stage[i], c->ev_stage[i + 1]));
+                out[i] = std::min(out[i], ms);
+            }
+        }
+        return BMF_OK;
+    };
+    bmf_tuning cur = c->tune;
+    const int cu = c->cu_count;
+    // sampling: tile x persistent workgroups (stage 1)
+    {
+        float bt = 1e30f;
+        bmf_tuning b = cur;
+        const int tiles[3] = {1024, 2048, 4096};
+        const int mult[4] = {2, 4, 8, 16};
+        for (int ti : tiles)
+            for (int m : mult) {
+                bmf_tuning t = cur;
+                t.sample_tile = ti;
+                t.sample_blocks = m * cu;
+                rc = run(t, st);
+                if (rc) return rc;
+                if (st[1] < bt) {
+                    bt = st[1];
+                    b = t;
+                }
+            }
+        cur.sample_tile = b.sample_tile;
+        cur.sample_blocks = b.sample_blocks;
+    }
+    // plane kernels: count (+ refine) and emit (stages 2 and 4)
+    {
+        float bt = 1e30f;
+        int bb = cur.row_blocks;
+        const int mult[5] = {1, 2, 4, 8, 16};
+        for (int m : mult) {
+            bmf_tuning t = cur;
+            t.row_blocks = m * cu;
+            rc = run(t, st);
+            if (rc) return rc;
+            if (st[2] + st[4] < bt) {
+                bt = st[2] + st[4];
+                bb = t.row_blocks;
+            }
+        }
+        cur.row_blocks = bb;
+    }
+    // smoothing (stage 5)
+    if (maxk > 0) {
+        float bt = 1e30f;
+        int bb = cur.smooth_blocks;
+        const int mult[5] = {2, 4, 8, 16, 32};
+        for (int m : mult) {
+            bmf_tuning t = cur;
+            t.smooth_blocks = (m * cu + kXcds - 1) / kXcds * kXcds;
+            rc = run(t, st);
+            if (rc) return rc;
+            if (st[5] < bt) {
+                bt = st[5];
+                bb = t.smooth_blocks;
+            }
+        }
+        cur.smooth_blocks = bb;
+    }
+    rc = bmf_set_tuning(c, &cur);
+    if (rc) return rc;
+    if (best) *best = cur;
     return BMF_OK;
 }
 

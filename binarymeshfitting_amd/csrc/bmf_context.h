@@ -40,7 +40,9 @@ struct bmf_ctx {
     bool results_valid = false;
     std::vector<bmf::GraphEntry> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int persistent_blocks = 2048;  // workgroups of the persistent kernels
+    int persistent_blocks = 2048;  // default workgroups of the persistent kernels
+    bmf_tuning tune{};             // BATCHED launch shapes (bmf_set_tuning / bmf_autotune)
+    bmf_mesh_options mesh{0, 0, 0.1f, 0};  // bmf_set_mesh_options
 };
 
 namespace bmf {

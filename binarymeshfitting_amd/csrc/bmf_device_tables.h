@@ -11,6 +11,11 @@ namespace bmf {
 // entry count, nibble s the local edge of slot s.
 static __constant__ uint64_t c_tri_table[256] = BMF_TRI_TABLE_INIT;
 
+// Dual-marching-cubes polygons of each mask (Tables.hpp:96-354, EdgeTable up
+// to its end marker): bits 60..63 nibble count, nibbles = local edges with
+// 0xF between polygons.
+static __constant__ uint64_t c_dmc_table[256] = BMF_DMC_POLY_TABLE_INIT;
+
 // The four cells around a lattice edge along `axis`, in lexicographic cell
 // order, as (dx, dy, dz, local edge) offsets from the edge origin; local
 // edges numbered as in DMCChunk::polygonize_cell (DMCChunk.cpp:543-565).

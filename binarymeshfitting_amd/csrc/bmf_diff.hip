@@ -3,8 +3,8 @@
 //             smoothing on the tape's own copy of the mesh, keeping the
 //             positions every `checkpoint_every` iterations;
 //   loss      mean squared distance to the nearest of a set of targets
-//             (exact nearest neighbour through a two-level AABB hierarchy
-//             over Morton-sorted targets);
+//             (exact nearest neighbour: an implicit binary box hierarchy
+//             over Morton-sorted targets, traversed nearest child first);
 //   backward  the transpose of the K smoothing steps (linear in the
 //             positions for the frozen topology) and of vertex placement
 //             mu = (0 - s0) / (s1 - s0) (DMCChunk.cpp:657-674), gathered per
@@ -43,8 +43,9 @@ constexpr int kThreads = 256;
 // one edge in at most 5 table slots (MCTable.h:27-283), so a ring holds at
 // most 20 triangles.
 constexpr int kMaxRing = 20;
-constexpr int kCluster = 32;  // targets per leaf box of the nearest-target search
-constexpr int kSuper = 32;    // leaf boxes per top-level box
+constexpr int kLeaf = 8;       // targets per leaf box of the nearest-target hierarchy
+constexpr int kNNThreads = 128;
+constexpr int kNNStack = 32;   // per-lane traversal stack (tree height <= 28 for n < 2^31)
 
 #define BMF_HIP(call)                          \
     do {                                       \
@@ -334,24 +335,38 @@ __global__ void k_nn_gather(const float* __restrict__ t, const int32_t* __restri
     st[i] = make_float4(t[3 * (int64_t)p], t[3 * (int64_t)p + 1], t[3 * (int64_t)p + 2], __int_as_float(p));
 }
 
-// Boxes over groups of `group` consecutive entries (points or boxes).
-__global__ void k_nn_boxes(const float4* __restrict__ lo_in, const float4* __restrict__ hi_in, int64_t n,
-                           int group, int64_t nbox, float4* __restrict__ lo, float4* __restrict__ hi) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbox) return;
-    float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
-    const int64_t e = std::min<int64_t>(n, (b + 1) * group);
-    for (int64_t i = b * group; i < e; i++) {
-        const float4 a = lo_in[i], c = hi_in[i];
-        l.x = fminf(l.x, a.x);
-        l.y = fminf(l.y, a.y);
-        l.z = fminf(l.z, a.z);
-        h.x = fmaxf(h.x, c.x);
-        h.y = fmaxf(h.y, c.y);
-        h.z = fmaxf(h.z, c.z);
+// Leaf boxes of the implicit hierarchy: heap-numbered nodes 1 .. 2P-1 over
+// P (a power of two) leaves; leaf l is node P + l and holds sorted targets
+// [kLeaf l, kLeaf (l+1)); leaves past the targets are empty boxes.
+__global__ void k_nn_leaves(const float4* __restrict__ st, int64_t n, int64_t P, float4* __restrict__ lo,
+                            float4* __restrict__ hi) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= P) return;
+    float4 a = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), b = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+    const int64_t e = std::min<int64_t>(n, (l + 1) * kLeaf);
+    for (int64_t i = l * kLeaf; i < e; i++) {
+        const float4 q = st[i];
+        a.x = fminf(a.x, q.x);
+        a.y = fminf(a.y, q.y);
+        a.z = fminf(a.z, q.z);
+        b.x = fmaxf(b.x, q.x);
+        b.y = fmaxf(b.y, q.y);
+        b.z = fmaxf(b.z, q.z);
     }
-    lo[b] = l;
-    hi[b] = h;
+    lo[P + l] = a;
+    hi[P + l] = b;
+}
+
+// Inner boxes, level by level from the leaves up (one workgroup).
+__global__ void __launch_bounds__(1024) k_nn_inner(int64_t P, float4* __restrict__ lo, float4* __restrict__ hi) {
+    for (int64_t level = P >> 1; level >= 1; level >>= 1) {
+        for (int64_t i = level + threadIdx.x; i < 2 * level; i += blockDim.x) {
+            const float4 a = lo[2 * i], b = lo[2 * i + 1], c = hi[2 * i], d = hi[2 * i + 1];
+            lo[i] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+            hi[i] = make_float4(fmaxf(c.x, d.x), fmaxf(c.y, d.y), fmaxf(c.z, d.z), 0.0f);
+        }
+        __syncthreads();
+    }
 }
 
 __device__ __forceinline__ float dist2(float px, float py, float pz, float4 q) {
@@ -359,70 +374,79 @@ __device__ __forceinline__ float dist2(float px, float py, float pz, float4 q) {
     return (dx * dx + dy * dy) + dz * dz;
 }
 
-// Squared distance from p to the box (0 inside); never above the distance to
-// any point in the box in FP32 (every step rounds monotonically).
+// Squared distance from p to the box (0 inside, +inf for an empty box);
+// never above the distance to any point in the box in FP32 (every step
+// rounds monotonically).
 __device__ __forceinline__ float box_lb2(float px, float py, float pz, float4 l, float4 h) {
+    if (!(l.x <= h.x)) return INFINITY;
     const float dx = fmaxf(fmaxf(l.x - px, px - h.x), 0.0f);
     const float dy = fmaxf(fmaxf(l.y - py, py - h.y), 0.0f);
     const float dz = fmaxf(fmaxf(l.z - pz, pz - h.z), 0.0f);
     return (dx * dx + dy * dy) + dz * dz;
 }
 
-__device__ __forceinline__ void nn_consider(float px, float py, float pz, float4 q, float& best, int& bi,
-                                            float4& bq) {
-    const float d = dist2(px, py, pz, q);
-    const int id = __float_as_int(q.w);
-    if (d < best || (d == best && id < bi)) {
-        best = d;
-        bi = id;
-        bq = q;
-    }
-}
-
-// One vertex per lane.  Seed: the leaf box holding the vertex's Morton key;
-// then every top-level box, and inside it every leaf box, whose lower bound
-// does not exceed the lane's best distance (wave-uniform skips).
-__global__ void __launch_bounds__(kThreads) k_nn_query(const float* __restrict__ pos, int V,
-                                                       const float4* __restrict__ st, const uint32_t* __restrict__ skeys,
-                                                       int64_t n, const int* __restrict__ bb,
-                                                       const float4* __restrict__ clo, const float4* __restrict__ chi,
-                                                       int64_t ncl, const float4* __restrict__ slo,
-                                                       const float4* __restrict__ shi, int64_t nsup,
-                                                       int32_t* __restrict__ nearest, float* __restrict__ dldp,
-                                                       double* __restrict__ d2) {
+// One vertex per lane: depth-first traversal, nearer child first, a subtree
+// skipped when its box is strictly farther than the best distance so far
+// (an equally far target with a lower index is still found).
+__global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict__ pos, int V,
+                                                         const float4* __restrict__ st, int64_t n, int64_t P,
+                                                         const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                         int32_t* __restrict__ nearest, float* __restrict__ dldp,
+                                                         double* __restrict__ d2) {
+    __shared__ int32_t s_node[kNNStack][kNNThreads];
+    __shared__ float s_lb[kNNStack][kNNThreads];
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = v < V;
-    const float px = active ? pos[3 * v] : 0.0f, py = active ? pos[3 * v + 1] : 0.0f, pz = active ? pos[3 * v + 2] : 0.0f;
+    if (v >= V) return;
+    const float px = pos[3 * v], py = pos[3 * v + 1], pz = pos[3 * v + 2];
     float best = INFINITY;
     int bi = 0x7FFFFFFF;
     float4 bq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (active) {
-        const uint32_t key = morton_key(bb, px, py, pz);
-        int64_t lo = 0, hi = n;  // first sorted key >= key
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (skeys[mid] < key) lo = mid + 1;
-            else hi = mid;
-        }
-        const int64_t c0 = std::min<int64_t>(lo / kCluster, ncl - 1);
-        const int64_t e = std::min<int64_t>(n, (c0 + 1) * kCluster);
-        for (int64_t i = c0 * kCluster; i < e; i++) nn_consider(px, py, pz, st[i], best, bi, bq);
-    }
-    for (int64_t s = 0; s < nsup; s++) {
-        const bool need_s = active && box_lb2(px, py, pz, slo[s], shi[s]) <= best;
-        if (!__any(need_s)) continue;
-        const int64_t ce = std::min<int64_t>(ncl, (s + 1) * kSuper);
-        for (int64_t c = s * kSuper; c < ce; c++) {
-            const bool need = need_s && box_lb2(px, py, pz, clo[c], chi[c]) <= best;
-            if (!__any(need)) continue;
-            const int64_t e = std::min<int64_t>(n, (c + 1) * kCluster);
-            for (int64_t i = c * kCluster; i < e; i++) {
+    int sp = 0;
+    int64_t node = 1;
+    for (;;) {
+        if (node < P) {
+            int64_t a = 2 * node, b = a + 1;
+            float la = box_lb2(px, py, pz, lo[a], hi[a]), lb = box_lb2(px, py, pz, lo[b], hi[b]);
+            if (lb < la) {
+                const int64_t t = a;
+                a = b;
+                b = t;
+                const float u = la;
+                la = lb;
+                lb = u;
+            }
+            if (lb <= best && sp < kNNStack) {
+                s_node[sp][threadIdx.x] = (int32_t)b;
+                s_lb[sp][threadIdx.x] = lb;
+                sp++;
+            }
+            if (la <= best) {
+                node = a;
+                continue;
+            }
+        } else {
+            const int64_t e = std::min<int64_t>(n, (node - P + 1) * kLeaf);
+            for (int64_t i = (node - P) * kLeaf; i < e; i++) {
                 const float4 q = st[i];
-                if (need) nn_consider(px, py, pz, q, best, bi, bq);
+                const float d = dist2(px, py, pz, q);
+                const int id = __float_as_int(q.w);
+                if (d < best || (d == best && id < bi)) {
+                    best = d;
+                    bi = id;
+                    bq = q;
+                }
             }
         }
+        node = 0;
+        while (sp > 0) {
+            sp--;
+            if (s_lb[sp][threadIdx.x] <= best) {
+                node = s_node[sp][threadIdx.x];
+                break;
+            }
+        }
+        if (node == 0) break;
     }
-    if (!active) return;
     nearest[v] = bi;
     const float sc = 2.0f / (float)V;
     dldp[3 * v] = (px - bq.x) * sc;
@@ -477,10 +501,8 @@ struct bmf_tape {
     uint32_t* keys[2] = {nullptr, nullptr};
     int32_t* vals[2] = {nullptr, nullptr};
     float4* st = nullptr;
-    float4* clo = nullptr;
-    float4* chi = nullptr;
-    float4* slo = nullptr;
-    float4* shi = nullptr;
+    float4* blo = nullptr;  // hierarchy boxes, heap-numbered [2P]
+    float4* bhi = nullptr;
     int* bb = nullptr;
     void* cub_tmp = nullptr;
     size_t cub_bytes = 0;
@@ -491,16 +513,22 @@ struct bmf_tape {
 
 namespace {
 
+// Leaves of the target hierarchy, rounded up to a power of two.
+int64_t leaf_pow2(int64_t n) {
+    const int64_t leaves = (n + kLeaf - 1) / kLeaf;
+    int64_t P = 1;
+    while (P < leaves) P <<= 1;
+    return P;
+}
+
 void free_nn(bmf_tape* t) {
     for (int i = 0; i < 2; i++) {
         dfree(t->keys[i]);
         dfree(t->vals[i]);
     }
     dfree(t->st);
-    dfree(t->clo);
-    dfree(t->chi);
-    dfree(t->slo);
-    dfree(t->shi);
+    dfree(t->blo);
+    dfree(t->bhi);
     if (t->cub_tmp) (void)hipFree(t->cub_tmp);
     t->cub_tmp = nullptr;
     t->cub_bytes = 0;
@@ -510,12 +538,12 @@ void free_nn(bmf_tape* t) {
 int reserve_nn(bmf_tape* t, int64_t n) {
     if (n <= t->nn_cap) return BMF_OK;
     free_nn(t);
-    const int64_t ncl = (n + kCluster - 1) / kCluster, nsup = (ncl + kSuper - 1) / kSuper;
+    const int64_t P = leaf_pow2(n);
     bool ok = true;
     for (int i = 0; i < 2; i++)
         ok = ok && dalloc(&t->keys[i], n) == hipSuccess && dalloc(&t->vals[i], n) == hipSuccess;
-    ok = ok && dalloc(&t->st, n) == hipSuccess && dalloc(&t->clo, ncl) == hipSuccess &&
-         dalloc(&t->chi, ncl) == hipSuccess && dalloc(&t->slo, nsup) == hipSuccess && dalloc(&t->shi, nsup) == hipSuccess;
+    ok = ok && dalloc(&t->st, n) == hipSuccess && dalloc(&t->blo, 2 * P) == hipSuccess &&
+         dalloc(&t->bhi, 2 * P) == hipSuccess;
     if (ok) {
         hipcub::DoubleBuffer<uint32_t> kb(t->keys[0], t->keys[1]);
         hipcub::DoubleBuffer<int32_t> vb(t->vals[0], t->vals[1]);
@@ -639,12 +667,15 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
     if (rc) return rc;
     Arenas A = c->A;
     A.ext_base = 0;
+    A.mesh_mode = BMF_MESH_TRIANGLES;  // the adjoint is defined on the triangle path
+    A.quad_tris = 0;
     const int wcls = row_word_class(dim);
     // extraction with every octave evaluated (densities exact everywhere),
     // no smoothing: the tape smooths its own copy
     hipError_t e = launch_plan(A, 1, 0, 1, s);
     if (e == hipSuccess)
-        e = launch_sample(A, 1, (int)std::min<int64_t>((n3 + kSampleTile - 1) / kSampleTile, c->persistent_blocks), 0, s);
+        e = launch_sample(A, 1, (int)std::min<int64_t>((n3 + A.sample_tile - 1) / A.sample_tile, c->persistent_blocks),
+                          0, s);
     if (e == hipSuccess) e = launch_count(A, 1, dim, wcls, s);
     if (e == hipSuccess) e = launch_refine(A, 1, dim, s);
     if (e == hipSuccess) e = launch_scan(A, 1, 1, s);
@@ -754,7 +785,7 @@ int bmf_loss_nearest(bmf_tape* t, const float* targets, int64_t n, float* dldp, 
     }
     int rc = reserve_nn(t, n);
     if (rc) return rc;
-    const int64_t ncl = (n + kCluster - 1) / kCluster, nsup = (ncl + kSuper - 1) / kSuper;
+    const int64_t P = leaf_pow2(n);
     hipLaunchKernelGGL(k_nn_bbox_init, dim3(1), dim3(64), 0, s, t->bb);
     hipLaunchKernelGGL(k_nn_bbox, dim3((unsigned)std::min<int64_t>(1024, blocks_for(n))), dim3(kThreads), 0, s, targets,
                        n, t->bb);
@@ -765,13 +796,10 @@ int bmf_loss_nearest(bmf_tape* t, const float* targets, int64_t n, float* dldp, 
     size_t bytes = t->cub_bytes;
     BMF_HIP(hipcub::DeviceRadixSort::SortPairs(t->cub_tmp, bytes, kb, vb, (int)n, 0, 30, s));
     hipLaunchKernelGGL(k_nn_gather, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, vb.Current(), n, t->st);
-    hipLaunchKernelGGL(k_nn_boxes, dim3(blocks_for(ncl)), dim3(kThreads), 0, s, t->st, t->st, n, kCluster, ncl, t->clo,
-                       t->chi);
-    hipLaunchKernelGGL(k_nn_boxes, dim3(blocks_for(nsup)), dim3(kThreads), 0, s, t->clo, t->chi, ncl, kSuper, nsup,
-                       t->slo, t->shi);
-    hipLaunchKernelGGL(k_nn_query, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->pos, t->V, t->st, kb.Current(), n,
-                       t->bb, t->clo, t->chi, ncl, t->slo, t->shi, nsup, nearest ? nearest : t->nearest_tmp, dldp,
-                       t->d2);
+    hipLaunchKernelGGL(k_nn_leaves, dim3(blocks_for(P)), dim3(kThreads), 0, s, t->st, n, P, t->blo, t->bhi);
+    hipLaunchKernelGGL(k_nn_inner, dim3(1), dim3(1024), 0, s, P, t->blo, t->bhi);
+    hipLaunchKernelGGL(k_nn_query, dim3((unsigned)((t->V + kNNThreads - 1) / kNNThreads)), dim3(kNNThreads), 0, s,
+                       t->pos, t->V, t->st, n, P, t->blo, t->bhi, nearest ? nearest : t->nearest_tmp, dldp, t->d2);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, t->d2, t->V, t->loss);
     BMF_HIP(hipGetLastError());
     double l = 0.0;

@@ -1,0 +1,667 @@
+// Quad mode of the extraction path (BMF_MESH_QUADS): dual marching cubes
+// over the reference's polygon table (Tables.hpp:96-354) with mass-point or
+// QEF vertex placement (qef_simd.h:392-579), quads around the interior
+// sign-changing edges, and MeshProcessor<4>/<3> smoothing over per-vertex
+// rings.
+//
+// Per batch (after sample, count and refine of the triangle path, which
+// leave every edge endpoint's density exact):
+//   k_dmc_count    per label row: polygons of the row's cells, quads of the
+//                  row's interior sign-changing edges
+//   (k_scan)       row bases, arena offsets
+//   k_dmc_vertices per row: vertex positions; per surface cell its first
+//                  vertex id and mask (cell_tb / cell_mask arenas)
+//   k_dmc_quads    per row: quad (or two-triangle) indices; rings of the
+//                  chunks that smooth (slot per corner by atomic counter,
+//                  then sorted, so the order is the reference's ascending
+//                  primitive id)
+//   k_ring_sort, then K x (k_dual_n, k_primal_ring)
+#include <hip/hip_runtime.h>
+
+#include "bmf_device_tables.h"
+#include "bmf_internal.h"
+#include "bmf_rows.h"
+#include "samplers.h"
+
+namespace bmf {
+
+namespace {
+
+// Per block: polygon count of each mask, and for each mask and local edge
+// the polygon holding it (nibbles, 0xF = none).
+struct DmcTables {
+    uint8_t np[256];
+    uint64_t pe[256];
+};
+
+__device__ void load_dmc_tables(DmcTables& t) {
+    for (int m = threadIdx.x; m < 256; m += blockDim.x) {
+        const uint64_t row = c_dmc_table[m];
+        const int n = (int)(row >> 60);
+        uint64_t pe = ~0ull;
+        int p = 0, any = 0;
+        for (int s = 0; s < n; s++) {
+            const int e = (int)((row >> (4 * s)) & 15ull);
+            if (e == 15) {
+                p++;
+                any = 0;
+                continue;
+            }
+            pe = (pe & ~(15ull << (4 * e))) | ((uint64_t)p << (4 * e));
+            any = 1;
+        }
+        t.np[m] = (uint8_t)(p + any);
+        t.pe[m] = pe;
+    }
+}
+
+// Samples z in [1, dim-2]: the z-range where both cells beside an x- or
+// y-edge lie in the chunk.
+template <int W>
+__device__ __forceinline__ RowBits<W> inner_z(int dim) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        uint64_t v = ~0ull;
+        const int lo = 64 * k;
+        if (lo == 0) v &= ~1ull;
+        const int hi = dim - 2 - lo;  // last valid bit in this word
+        if (hi < 0) v = 0ull;
+        else if (hi < 63) v &= (2ull << hi) - 1ull;
+        r.w[k] = v;
+    }
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ RowBits<W> rand_(const RowBits<W>& a, const RowBits<W>& b) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) r.w[k] = a.w[k] & b.w[k];
+    return r;
+}
+
+// Interior sign-changing edges owned by the samples of row (x, y): the four
+// cells around each lie in the chunk.
+template <int W>
+__device__ __forceinline__ void interior_edges(const RowBits<W>& r00, const RowBits<W>& r01, const RowBits<W>& r10,
+                                               int dim, int x, int y, RowBits<W>& ex, RowBits<W>& ey,
+                                               RowBits<W>& ez) {
+    const RowBits<W> iz = inner_z<W>(dim);
+    const bool xin = x >= 1 && x <= dim - 2, yin = y >= 1 && y <= dim - 2;
+    ex = (yin && x + 1 < dim) ? rand_<W>(rxor<W>(r00, r10, true), iz) : zero_row<W>();
+    ey = (xin && y + 1 < dim) ? rand_<W>(rxor<W>(r00, r01, true), iz) : zero_row<W>();
+    ez = (xin && yin) ? zedges<W>(r00, dim) : zero_row<W>();
+}
+
+// Sum over the wave of a per-lane count c < 8 (bit-sliced ballots), and the
+// exclusive prefix of the lanes below.
+__device__ __forceinline__ uint32_t wave_sum3(uint32_t c, uint64_t lt, uint32_t& pre) {
+    const uint64_t b0 = __ballot(c & 1u), b1 = __ballot(c & 2u), b2 = __ballot(c & 4u);
+    pre = (uint32_t)__popcll(b0 & lt) + 2u * (uint32_t)__popcll(b1 & lt) + 4u * (uint32_t)__popcll(b2 & lt);
+    return (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
+}
+
+// ------------------------------------------------------------------------
+// CSG gradient and QEF (FP32, the CPU oracle's operation sequence)
+// ------------------------------------------------------------------------
+// Gradient of the winning primitive of the max-union (first index on ties),
+// zero on the floor.
+__device__ void csg_gradient(const SamplerDev& S, float x, float y, float z, float g[3]) {
+    float best = kCsgFloor;
+    g[0] = g[1] = g[2] = 0.0f;
+    for (int i = 0; i < S.n_spheres; i++) {
+        const float* sp = S.spheres + 4 * i;
+        const float dx = x - sp[0], dy = y - sp[1], dz = z - sp[2];
+        const float l = sqrtf((dx * dx + dy * dy) + dz * dz);
+        const float v = sp[3] - l;
+        if (v > best) {
+            best = v;
+            const float inv = l > 0.0f ? 1.0f / l : 0.0f;
+            g[0] = -dx * inv;
+            g[1] = -dy * inv;
+            g[2] = -dz * inv;
+        }
+    }
+    for (int i = 0; i < S.n_boxes; i++) {
+        const float* b = S.boxes + 6 * i;
+        const float px = x - b[0], py = y - b[1], pz = z - b[2];
+        const float qx = fabsf(px) - b[3], qy = fabsf(py) - b[4], qz = fabsf(pz) - b[5];
+        const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+        const float outside = sqrtf((mx * mx + my * my) + mz * mz);
+        const float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+        const float v = -(outside + inside);
+        if (v > best) {
+            best = v;
+            const float sx = px < 0.0f ? -1.0f : 1.0f, sy = py < 0.0f ? -1.0f : 1.0f, sz = pz < 0.0f ? -1.0f : 1.0f;
+            if (outside > 0.0f) {
+                const float inv = 1.0f / outside;
+                g[0] = -mx * sx * inv;
+                g[1] = -my * sy * inv;
+                g[2] = -mz * sz * inv;
+            } else {
+                g[0] = g[1] = g[2] = 0.0f;
+                if (qx >= qy && qx >= qz) g[0] = -sx;
+                else if (qy >= qz) g[1] = -sy;
+                else g[2] = -sz;
+            }
+        }
+    }
+}
+
+// QEF of `count` (point, unit normal) pairs (qef_simd.h:392-579): normal
+// equations about the mass point, 5 Jacobi sweeps over the pairs (0,1),
+// (0,2), (1,2) with an exact 1/sqrt Givens cosine, 1/sigma kept when
+// min(|sigma|, |1/sigma|) >= tol, residual with the uncentred ATb.
+__device__ float qef_solve(const float (*p)[3], const float (*n)[3], int count, float tol, float out[3]) {
+    if (count < 2 || count > 12) {
+        out[0] = out[1] = out[2] = 0.0f;
+        return 0.0f;
+    }
+    float ata[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, atb[3] = {0, 0, 0}, pa[4] = {0, 0, 0, 0};
+    for (int i = 0; i < count; i++) {
+        const float px = p[i][0], py = p[i][1], pz = p[i][2];
+        const float nn[3] = {n[i][0], n[i][1], n[i][2]};
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) ata[r][c] = ata[r][c] + nn[r] * nn[c];
+        const float d = (px * nn[0] + py * nn[1]) + pz * nn[2];
+#pragma unroll
+        for (int r = 0; r < 3; r++) atb[r] = atb[r] + d * nn[r];
+        pa[0] = pa[0] + px;
+        pa[1] = pa[1] + py;
+        pa[2] = pa[2] + pz;
+        pa[3] = pa[3] + 1.0f;
+    }
+    const float mp[3] = {pa[0] / pa[3], pa[1] / pa[3], pa[2] / pa[3]};
+    float rhs[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) rhs[c] = atb[c] - ((mp[0] * ata[0][c] + mp[1] * ata[1][c]) + mp[2] * ata[2][c]);
+    float a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) a[r][c] = ata[r][c];
+    for (int sw = 0; sw < 5; sw++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int i0 = q == 2 ? 1 : 0, i1 = q == 0 ? 1 : 2;
+            if (a[i0][i1] == 0.0f) continue;
+            const float pp = a[i0][i0], pq = a[i0][i1], qq = a[i1][i1];
+            const float tau = (qq - pp) / (pq * 2.0f);
+            const float stt = sqrtf(tau * tau + 1.0f);
+            const float tn = 1.0f / (tau >= 0.0f ? tau + stt : tau - stt);
+            const float cs = 1.0f / sqrtf(1.0f + tn * tn);
+            const float sn = tn * cs;
+            const float cc = cs * cs, ss = sn * sn, mx = ((2.0f * cs) * sn) * pq;
+            a[i0][i0] = (cc * pp - mx) + ss * qq;
+            a[i1][i1] = (ss * pp + mx) + cc * qq;
+            const int ua_r = 0, ua_c = 3 - i1, va_r = 1 - i0, va_c = 2;
+            const float u3 = a[ua_r][ua_c], v3 = a[va_r][va_c];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const float vu = v[r][i0], vv = v[r][i1];
+                v[r][i0] = cs * vu - sn * vv;
+                v[r][i1] = sn * vu + cs * vv;
+            }
+            a[ua_r][ua_c] = cs * u3 - sn * v3;
+            a[va_r][va_c] = sn * u3 + cs * v3;
+            a[i0][i1] = 0.0f;
+        }
+    const float sig[3] = {a[0][0], a[1][1], a[2][2]};
+    float inv[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float r1 = 1.0f / sig[k];
+        const float ax = sig[k] < 0.0f ? -sig[k] : sig[k], ar = r1 < 0.0f ? -r1 : r1;
+        inv[k] = ((ax < ar ? ax : ar) >= tol) ? r1 : 0.0f;
+    }
+    float pinv[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            pinv[r][c] = (v[c][0] * inv[0] * v[r][0] + v[c][1] * inv[1] * v[r][1]) + v[c][2] * inv[2] * v[r][2];
+    float x[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) x[c] = (rhs[0] * pinv[0][c] + rhs[1] * pinv[1][c]) + rhs[2] * pinv[2][c];
+    float e[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) e[c] = atb[c] - ((x[0] * ata[0][c] + x[1] * ata[1][c]) + x[2] * ata[2][c]);
+    const float err = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+    out[0] = x[0] + mp[0];
+    out[1] = x[1] + mp[1];
+    out[2] = x[2] + mp[2];
+    return err;
+}
+
+// Local edge -> offset of its origin sample from the cell origin and axis
+// (DMCChunk.cpp:543-565).
+__constant__ int8_t c_le[12][4] = {{0, 0, 0, 0}, {0, 0, 1, 0}, {0, 1, 0, 0}, {0, 1, 1, 0},
+                                   {0, 0, 0, 1}, {0, 0, 1, 1}, {1, 0, 0, 1}, {1, 0, 1, 1},
+                                   {0, 0, 0, 2}, {0, 1, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}};
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// count: polygons per row of cells, quads per row of owned edges
+// ------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(kRowThreads) k_dmc_count(Arenas A, int n) {
+    __shared__ DmcTables s_t;
+    __shared__ uint64_t s_planes[2][plane_words(W)];
+    load_dmc_tables(s_t);
+    const int wave = uniform_int(threadIdx.x >> 6);
+    const int lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    const int64_t total = A.ctr->total_planes;
+    const uint32_t ppq = A.quad_tris ? 2u : 1u;
+    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const int j = find_chunk(A.plane_prefix, n, item);
+        const ChunkMeta& M = A.meta[j];
+        const int dim = M.dim;
+        if (((dim + 63) >> 6) != W) continue;
+        const int x = (int)(item - A.plane_prefix[j]);
+        uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
+        uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
+        if (!(M.any_air && M.any_solid)) {
+            for (int y = threadIdx.x; y < dim; y += blockDim.x) rv[y] = rt[y] = 0u;
+            continue;
+        }
+        __syncthreads();
+        stage_plane(s_planes[0], A.labels + M.lab_off, M.lab_words, dim, x);
+        stage_plane(s_planes[1], A.labels + M.lab_off, M.lab_words, dim, x + 1);
+        __syncthreads();
+        for (int y = wave; y < dim; y += kRowThreads / 64) {
+            const bool x1 = x + 1 < dim, y1 = y + 1 < dim;
+            const RowBits<W> r00 = lds_row<W>(s_planes[0], dim, y, true);
+            const RowBits<W> r01 = lds_row<W>(s_planes[0], dim, y + 1, y1);
+            const RowBits<W> r10 = lds_row<W>(s_planes[1], dim, y, x1);
+            const RowBits<W> r11 = lds_row<W>(s_planes[1], dim, y + 1, x1 && y1);
+            uint32_t nv = 0;
+            if (x1 && y1 && !quad_uniform<W>(r00, r01, r10, r11, dim))
+                for (int z0 = 0; z0 < dim - 1; z0 += 64) {
+                    const int z = z0 + lane;
+                    const uint32_t c = z < dim - 1 ? s_t.np[cell_mask<W>(r00, r01, r10, r11, z)] : 0u;
+                    uint32_t pre;
+                    nv += wave_sum3(c, lt, pre);
+                }
+            RowBits<W> ex, ey, ez;
+            interior_edges<W>(r00, r01, r10, dim, x, y, ex, ey, ez);
+            const uint32_t nq = total_pop<W>(ex) + total_pop<W>(ey) + total_pop<W>(ez);
+            if (lane == 0) {
+                rv[y] = nv;
+                rt[y] = nq * ppq;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// vertices: one per polygon of each surface cell of the row
+// ------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
+    __shared__ DmcTables s_t;
+    __shared__ uint64_t s_planes[2][plane_words(W)];
+    load_dmc_tables(s_t);
+    const SamplerDev* __restrict__ S = A.sampler;
+    const int wave = uniform_int(threadIdx.x >> 6);
+    const int lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    const int64_t total = A.ctr->total_planes;
+    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const int j = find_chunk(A.plane_prefix, n, item);
+        const ChunkMeta& M = A.meta[j];
+        const int dim = M.dim;
+        if (((dim + 63) >> 6) != W) continue;
+        if (M.status != BMF_OK || M.V == 0) continue;
+        const int x = (int)(item - A.plane_prefix[j]);
+        if (x >= dim - 1) continue;
+        __syncthreads();
+        stage_plane(s_planes[0], A.labels + M.lab_off, M.lab_words, dim, x);
+        stage_plane(s_planes[1], A.labels + M.lab_off, M.lab_words, dim, x + 1);
+        __syncthreads();
+        const uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
+        const int64_t vo = M.v_off;
+        const uint32_t Vc = (uint32_t)M.V;
+        const bool smooth = M.iters > 0 && M.T > 0;
+        const bool qef = A.placement == BMF_PLACE_QEF && S->kind == BMF_SAMPLER_CSG;
+        const float* __restrict__ D = A.dens + M.dens_off;
+        for (int y = wave; y < dim - 1; y += kRowThreads / 64) {
+            const RowBits<W> r00 = lds_row<W>(s_planes[0], dim, y, true);
+            const RowBits<W> r01 = lds_row<W>(s_planes[0], dim, y + 1, true);
+            const RowBits<W> r10 = lds_row<W>(s_planes[1], dim, y, true);
+            const RowBits<W> r11 = lds_row<W>(s_planes[1], dim, y + 1, true);
+            if (quad_uniform<W>(r00, r01, r10, r11, dim)) continue;
+            uint32_t carry = rv[y];
+            for (int z0 = 0; z0 < dim - 1; z0 += 64) {
+                const int z = z0 + lane;
+                const uint32_t mask = z < dim - 1 ? cell_mask<W>(r00, r01, r10, r11, z) : 0u;
+                const uint32_t np = s_t.np[mask];
+                uint32_t pre;
+                const uint32_t tot = wave_sum3(np, lt, pre);
+                const uint32_t vb = carry + pre;
+                carry += tot;
+                if (!np) continue;
+                const int64_t ci = M.dens_off + ((int64_t)x * dim + y) * dim + z;
+                A.cell_tb[ci] = vb;
+                A.cell_mask[ci] = (uint8_t)mask;
+                const uint64_t row = c_dmc_table[mask];
+                const int nn = (int)(row >> 60);
+                int s = 0;
+                for (uint32_t p = 0; p < np; p++) {
+                    const uint32_t v = vb + p;
+                    // edges of polygon p
+                    float q[7][3];
+                    int k = 0;
+                    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+                    for (; s < nn; s++) {
+                        const int e = (int)((row >> (4 * s)) & 15ull);
+                        if (e == 15) {
+                            s++;
+                            break;
+                        }
+                        const int x0 = x + c_le[e][0], y0 = y + c_le[e][1], z0e = z + c_le[e][2], a = c_le[e][3];
+                        const int x1 = x0 + (a == 0), y1 = y0 + (a == 1), z1 = z0e + (a == 2);
+                        const float s0 = D[((int64_t)x0 * dim + y0) * dim + z0e];
+                        const float s1 = D[((int64_t)x1 * dim + y1) * dim + z1];
+                        // DMCChunk::calculate_isovertex (DMCChunk.cpp:657-674)
+                        const float mu = (0.0f - s0) / (s1 - s0);
+                        const float dx = (float)x1 - (float)x0, dy = (float)y1 - (float)y0, dz = (float)z1 - (float)z0e;
+                        if (k < 7) {
+                            q[k][0] = dx * mu + (float)x0;
+                            q[k][1] = dy * mu + (float)y0;
+                            q[k][2] = dz * mu + (float)z0e;
+                            ax = ax + q[k][0];
+                            ay = ay + q[k][1];
+                            az = az + q[k][2];
+                        }
+                        k++;
+                    }
+                    if (v >= Vc || k > 7 || k == 0) {
+                        A.ctr->error = 8;
+                        continue;
+                    }
+                    float px, py, pz, err = 0.0f;
+                    if (qef) {
+                        float nrm[7][3];
+                        for (int i = 0; i < k; i++) {
+                            const float wx = q[i][0] * M.delta + M.opos[0], wy = q[i][1] * M.delta + M.opos[1],
+                                        wz = q[i][2] * M.delta + M.opos[2];
+                            float g[3];
+                            csg_gradient(*S, wx, wy, wz, g);
+                            const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+                            for (int c = 0; c < 3; c++) nrm[i][c] = len > 0.0f ? g[c] / len : 0.0f;
+                        }
+                        float o[3];
+                        err = qef_solve(q, nrm, k, A.qef_tol, o);
+                        px = o[0];
+                        py = o[1];
+                        pz = o[2];
+                    } else {
+                        const float fk = (float)k;
+                        px = ax / fk;
+                        py = ay / fk;
+                        pz = az / fk;
+                    }
+                    float* P = A.pos + (vo + v) * 3;
+                    P[0] = px;
+                    P[1] = py;
+                    P[2] = pz;
+                    if (smooth) {
+                        A.pos4[vo + v] = make_float4(px, py, pz, 0.0f);
+                        A.ring_n[vo + v] = 0u;
+                    }
+                    if (A.qef_err) A.qef_err[vo + v] = err;
+                    A.boundary[vo + v] =
+                        (uint8_t)(x == 0 || y == 0 || z == 0 || x + 1 == dim - 1 || y + 1 == dim - 1 || z + 1 == dim - 1);
+                    A.vedge[vo + v] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | (p << 24);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// quads around the interior sign-changing edges owned by the row's samples
+// ------------------------------------------------------------------------
+template <int W>
+__global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
+    __shared__ DmcTables s_t;
+    __shared__ uint64_t s_planes[2][plane_words(W)];
+    load_dmc_tables(s_t);
+    const int wave = uniform_int(threadIdx.x >> 6);
+    const int lane = lane_id();
+    const int64_t total = A.ctr->total_planes;
+    const bool tris = A.quad_tris != 0;
+    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const int j = find_chunk(A.plane_prefix, n, item);
+        const ChunkMeta& M = A.meta[j];
+        const int dim = M.dim;
+        if (((dim + 63) >> 6) != W) continue;
+        if (M.status != BMF_OK || M.T == 0) continue;
+        const int x = (int)(item - A.plane_prefix[j]);
+        __syncthreads();
+        stage_plane(s_planes[0], A.labels + M.lab_off, M.lab_words, dim, x);
+        stage_plane(s_planes[1], A.labels + M.lab_off, M.lab_words, dim, x + 1);
+        __syncthreads();
+        const uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
+        const bool smooth = M.iters > 0;
+        const uint32_t Vc = (uint32_t)M.V, Tc = (uint32_t)M.T;
+        uint32_t* __restrict__ I = A.idx + M.i_off;
+        for (int y = wave; y < dim; y += kRowThreads / 64) {
+            const bool x1 = x + 1 < dim, y1 = y + 1 < dim;
+            const RowBits<W> r00 = lds_row<W>(s_planes[0], dim, y, true);
+            const RowBits<W> r01 = lds_row<W>(s_planes[0], dim, y + 1, y1);
+            const RowBits<W> r10 = lds_row<W>(s_planes[1], dim, y, x1);
+            RowBits<W> ex, ey, ez;
+            interior_edges<W>(r00, r01, r10, dim, x, y, ex, ey, ez);
+            if (total_pop<W>(ex) + total_pop<W>(ey) + total_pop<W>(ez) == 0) continue;
+            const uint32_t qbase = rt[y] / (tris ? 2u : 1u);
+            for (int z0 = 0; z0 < dim; z0 += 64) {
+                const int z = z0 + lane;
+                if (z >= dim) continue;
+                const uint32_t f[3] = {bit_at<W>(ex, z), bit_at<W>(ey, z), bit_at<W>(ez, z)};
+                if (!(f[0] | f[1] | f[2])) continue;
+                uint32_t qid = qbase + prefix_pop<W>(ex, z) + prefix_pop<W>(ey, z) + prefix_pop<W>(ez, z);
+                const bool solid_origin = bit_at<W>(r00, z) == 0u;
+                for (int a = 0; a < 3; a++) {
+                    if (!f[a]) continue;
+                    uint32_t v[4];
+                    bool ok = true;
+                    for (int r = 0; r < 4; r++) {
+                        const int cx = x + c_ring[a][r][0], cy = y + c_ring[a][r][1], cz = z + c_ring[a][r][2];
+                        const int64_t ci = M.dens_off + ((int64_t)cx * dim + cy) * dim + cz;
+                        const uint32_t le = (uint32_t)c_ring[a][r][3];
+                        const uint32_t pi = (uint32_t)((s_t.pe[A.cell_mask[ci]] >> (4 * le)) & 15ull);
+                        v[r] = A.cell_tb[ci] + pi;
+                        if (pi == 15u || v[r] >= Vc) ok = false;
+                    }
+                    const uint32_t prim = tris ? 2u * qid : qid;
+                    if (!ok || prim + (tris ? 1u : 0u) >= Tc) {
+                        A.ctr->error = 9;
+                        qid++;
+                        continue;
+                    }
+                    // cyclic c0 c1 c3 c2: right-hand normal along -x, +y, -z;
+                    // flipped so that it points to the solid end of the edge
+                    const bool toward_origin = a != 1;
+                    uint32_t o[4] = {v[0], v[1], v[3], v[2]};
+                    if (solid_origin != toward_origin) {
+                        o[1] = v[2];
+                        o[3] = v[1];
+                    }
+                    if (tris) {  // MeshProcessor::flush_to_tris (MeshProcessor.cpp:74-91)
+                        uint32_t* t = I + 3ll * prim;
+                        t[0] = o[0];
+                        t[1] = o[1];
+                        t[2] = o[2];
+                        t[3] = o[2];
+                        t[4] = o[3];
+                        t[5] = o[0];
+                    } else {
+                        uint32_t* t = I + 4ll * prim;
+                        t[0] = o[0];
+                        t[1] = o[1];
+                        t[2] = o[2];
+                        t[3] = o[3];
+                    }
+                    if (smooth) {
+                        const int64_t vo = M.v_off;
+                        for (int r = 0; r < 4; r++) {
+                            const uint32_t u = o[r];
+                            // corner r of the quad: triangles (0,1,2) and (2,3,0)
+                            const int reps = tris ? ((r == 0 || r == 2) ? 2 : 1) : 1;
+                            for (int h = 0; h < reps; h++) {
+                                const uint32_t pid = tris ? (r == 1 ? prim : (r == 3 ? prim + 1 : prim + (uint32_t)h)) : prim;
+                                const uint32_t slot = atomicAdd(&A.ring_n[vo + u], 1u);
+                                if (slot < (uint32_t)kRingSlots)
+                                    A.ring[(int64_t)slot * A.cap_vertices + vo + u] = pid;
+                                else
+                                    A.ctr->error = 10;
+                            }
+                        }
+                    }
+                    qid++;
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// smoothing over the rings (chunks of the smoothing list, XCD-grouped tiles
+// as in the triangle path)
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(kSmoothThreads) k_ring_sort(Arenas A) {
+    const int nsm = A.ctr->n_smooth;
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t end = A.smooth_xcd_v[xcd + 1];
+    for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
+        const int k = find_chunk(A.smooth_prefix, nsm, tile);
+        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
+        if (v >= M.V) continue;
+        const int64_t g = M.v_off + v;
+        const int n = (int)min(A.ring_n[g], (uint32_t)kRingSlots);
+        uint32_t r[kRingSlots];
+        for (int i = 0; i < n; i++) r[i] = A.ring[(int64_t)i * A.cap_vertices + g];
+        for (int i = 1; i < n; i++) {  // insertion sort: ascending primitive id
+            const uint32_t t = r[i];
+            int p = i - 1;
+            while (p >= 0 && r[p] > t) {
+                r[p + 1] = r[p];
+                p--;
+            }
+            r[p + 1] = t;
+        }
+        for (int i = 0; i < n; i++) A.ring[(int64_t)i * A.cap_vertices + g] = r[i];
+    }
+}
+
+// Dual step (MeshProcessor.cpp:130-236) for N-gons: ((0 + p0) + p1 ...) / N.
+__global__ void __launch_bounds__(kSmoothThreads) k_dual_n(Arenas A, int m) {
+    const int nsm = A.ctr->n_smooth;
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t end = A.smooth_xcd_t[xcd + 1];
+    for (int64_t tile = A.smooth_xcd_t[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
+        const int k = find_chunk(A.smooth_tprefix, nsm, tile);
+        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        if (M.iters <= m) continue;
+        const int64_t t = (tile - A.smooth_tprefix[k]) * kSmoothThreads + threadIdx.x;
+        if (t >= M.T) continue;
+        const int N = M.ipp;
+        const uint32_t* __restrict__ I = A.idx + M.i_off + (int64_t)N * t;
+        const float4* __restrict__ P = A.pos4 + M.v_off;
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        bool ok = true;
+        for (int c = 0; c < N; c++) {
+            const uint32_t u = I[c];
+            if (u >= (uint32_t)M.V) {
+                ok = false;
+                break;
+            }
+            const float4 q = P[u];
+            cx = cx + q.x;
+            cy = cy + q.y;
+            cz = cz + q.z;
+        }
+        if (!ok) {
+            A.ctr->error = 11;
+            continue;
+        }
+        const float fn = (float)N;
+        A.cen4[M.i_off / N + t] = make_float4(cx / fn, cy / fn, cz / fn, 0.0f);
+    }
+}
+
+// Primal step (MeshProcessor.cpp:238-306) over the sorted ring.
+__global__ void __launch_bounds__(kSmoothThreads) k_primal_ring(Arenas A, int m) {
+    const int nsm = A.ctr->n_smooth;
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t end = A.smooth_xcd_v[xcd + 1];
+    for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
+        const int k = find_chunk(A.smooth_prefix, nsm, tile);
+        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int K = M.iters;
+        if (K <= m) continue;
+        const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
+        if (v >= M.V) continue;
+        const int64_t g = M.v_off + v;
+        if (!M.bp && A.boundary[g]) continue;
+        const int n = (int)min(A.ring_n[g], (uint32_t)kRingSlots);
+        if (n == 0) continue;
+        const float4* __restrict__ C = A.cen4 + M.i_off / M.ipp;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (int i = 0; i < n; i++) {
+            const float4 q = C[A.ring[(int64_t)i * A.cap_vertices + g]];
+            sx = sx + q.x;
+            sy = sy + q.y;
+            sz = sz + q.z;
+        }
+        const float fc = (float)n;
+        const float px = sx / fc, py = sy / fc, pz = sz / fc;
+        A.pos4[g] = make_float4(px, py, pz, 0.0f);
+        if (m == K - 1) {
+            float* __restrict__ P = A.pos + g * 3;
+            P[0] = px;
+            P[1] = py;
+            P[2] = pz;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// launch wrappers
+// ------------------------------------------------------------------------
+hipError_t launch_dmc_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
+    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_count<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_count<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_count<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_count<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
+    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_vertices<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_vertices<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_vertices<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_vertices<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_quads<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_quads<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_quads<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_quads<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_dmc_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
+    blocks = blocks < kXcds ? kXcds : blocks - blocks % kXcds;  // whole XCD groups
+    if (m == 0) hipLaunchKernelGGL(k_ring_sort, dim3(blocks), dim3(kSmoothThreads), 0, s, A);
+    hipLaunchKernelGGL(k_dual_n, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    hipLaunchKernelGGL(k_primal_ring, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    return hipGetLastError();
+}
+
+}  // namespace bmf

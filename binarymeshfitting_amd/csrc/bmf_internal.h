@@ -8,10 +8,10 @@
 namespace bmf {
 
 // Samples handled by one work-queue tile of the sampling kernel: 256 threads
-// x 4 rounds; a multiple of 64 so that every wave owns whole label words.
+// x rounds (4, 8 or 16, chosen at run time: Arenas::sample_tile); a multiple
+// of 64 so that every wave owns whole label words.
 constexpr int kSampleThreads = 256;
-constexpr int kSampleRounds = 8;
-constexpr int kSampleTile = kSampleThreads * kSampleRounds;
+constexpr int kSampleTileDefault = 2048;
 constexpr int kRowThreads = 256;      // 4 waves; one x-plane per block, one label row per wave
 constexpr int kSmoothThreads = 256;   // one vertex per thread
 constexpr int kMaxRowWords = 4;       // ceil(BMF_MAX_DIM / 64)
@@ -57,7 +57,8 @@ struct ChunkMeta {
     int32_t contains_mesh;
     int32_t status;
     int32_t V;          // vertices
-    int32_t T;          // triangles
+    int32_t T;          // primitives (triangles, or quads in the quad mode)
+    int32_t ipp;        // indices per primitive: 3 or 4
     int64_t v_off;      // output offsets
     int64_t i_off;
 };
@@ -111,7 +112,20 @@ struct Arenas {
     int64_t cap_samples, cap_rows, cap_vertices, cap_indices;
     int32_t cap_chunks;
     int64_t ext_base;          // EXTERNAL: offset of this launch's first chunk in the caller's densities
+    int32_t sample_tile;       // samples per sampling work-queue tile: 1024, 2048 or 4096
+    // mesh options (BMF_MESH_*, BMF_PLACE_*): the quad mode's extra arenas
+    int32_t mesh_mode;
+    int32_t placement;
+    float qef_tol;
+    int32_t quad_tris;         // quads written as two triangles each
+    uint32_t* ring;            // [kRingSlots][cap_vertices] primitives around each vertex (quad mode)
+    uint32_t* ring_n;          // [cap_vertices] ring sizes
+    float* qef_err;            // [cap_vertices] QEF residual of each vertex (QEF placement)
 };
+
+// Quad mode: a vertex (one polygon of a cell) has at most 7 edges, so at most
+// 7 quads, or 14 triangles when quads are written as triangles.
+constexpr int kRingSlots = 16;
 
 inline int row_word_class(int dim) { return 1 << (((dim + 63) >> 6) - 1); }
 
@@ -125,6 +139,11 @@ hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
 hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
+// Quad mode (bmf_dmc.hip): counts after launch_count/launch_refine, emission
+// after launch_scan, smoothing over the per-vertex rings.
+hipError_t launch_dmc_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
+hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
+hipError_t launch_dmc_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s);
 hipError_t launch_ref_labels(const Arenas& A, uint32_t* words, hipStream_t s);
 hipError_t launch_ref_masks(const Arenas& A, uint64_t* masks, hipStream_t s);

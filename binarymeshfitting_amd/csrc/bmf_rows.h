@@ -1,0 +1,257 @@
+// Label-row helpers shared by the plane kernels: a row of the label bitmap
+// (samples (x, y, 0..dim-1)) as W u64 words held in registers, and the
+// staging of whole label planes into LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bmf_internal.h"
+
+namespace bmf {
+
+// u64 words of one staged label plane for row-word class W (dim <= 64 W),
+// plus the zero pad word.
+__host__ __device__ constexpr int plane_words(int W) { return 64 * W * W + 1; }
+
+// ------------------------------------------------------------------------
+// small helpers
+// ------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Chunk j whose half-open range [prefix[j], prefix[j+1]) holds g.
+__device__ __forceinline__ int find_chunk(const int64_t* __restrict__ prefix, int n, int64_t g) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// q = n / d for n < 2^24 (float estimate + exact correction).
+__device__ __forceinline__ uint32_t udiv24(uint32_t n, uint32_t d, float rcp) {
+    int32_t q = (int32_t)((float)n * rcp);
+    int32_t r = (int32_t)n - q * (int32_t)d;
+    while (r < 0) { q--; r += (int32_t)d; }
+    while (r >= (int32_t)d) { q++; r -= (int32_t)d; }
+    return (uint32_t)q;
+}
+
+// A label row: bits z = 0..dim-1 of row (x, y) in W u64 words, W = ceil(dim/64).
+template <int W>
+struct RowBits {
+    uint64_t w[W];
+};
+
+template <int W>
+__device__ __forceinline__ RowBits<W> zero_row() {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) r.w[k] = 0ull;
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ RowBits<W> full_row(int dim) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const int nb = dim - 64 * k;
+        r.w[k] = nb >= 64 ? ~0ull : ((1ull << nb) - 1ull);
+    }
+    return r;
+}
+
+// Row (x, y) of a chunk's flat label bitmap: bits start at (x*dim + y)*dim.
+template <int W>
+__device__ __forceinline__ RowBits<W> load_row(const uint64_t* __restrict__ L, int64_t nwords, int dim, int x, int y) {
+    RowBits<W> r = zero_row<W>();
+    if (x < 0 || y < 0 || x >= dim || y >= dim) return r;
+    const int64_t base = ((int64_t)x * dim + y) * dim;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const int nb = dim - 64 * k;  // >= 1 because W = ceil(dim/64)
+        const int64_t bit = base + 64 * k;
+        const int64_t w = bit >> 6;
+        const int sh = (int)(bit & 63);
+        uint64_t v = L[w];
+        if (sh) {
+            const uint64_t hi = (w + 1 < nwords) ? L[w + 1] : 0ull;
+            v = (v >> sh) | (hi << (64 - sh));
+        }
+        if (nb < 64) v &= (1ull << nb) - 1ull;
+        r.w[k] = v;
+    }
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ uint64_t word_at(const RowBits<W>& r, int k) {
+    uint64_t v = 0ull;
+#pragma unroll
+    for (int q = 0; q < W; q++) v = k == q ? r.w[q] : v;
+    return v;
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t bit_at(const RowBits<W>& r, int z) {
+    return (uint32_t)((word_at<W>(r, z >> 6) >> (z & 63)) & 1ull);
+}
+
+// popcount of bits [0, z)
+template <int W>
+__device__ __forceinline__ uint32_t prefix_pop(const RowBits<W>& r, int z) {
+    const int wz = z >> 6;
+    const uint64_t m = (1ull << (z & 63)) - 1ull;
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint64_t v = r.w[k];
+        c += k < wz ? (uint32_t)__popcll(v) : (k == wz ? (uint32_t)__popcll(v & m) : 0u);
+    }
+    return c;
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t total_pop(const RowBits<W>& r) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) c += (uint32_t)__popcll(r.w[k]);
+    return c;
+}
+
+template <int W>
+__device__ __forceinline__ RowBits<W> rxor(const RowBits<W>& a, const RowBits<W>& b, bool valid) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) r.w[k] = valid ? (a.w[k] ^ b.w[k]) : 0ull;
+    return r;
+}
+
+// z-edges of a row: bit z set when labels z and z+1 differ (z <= dim-2).
+template <int W>
+__device__ __forceinline__ RowBits<W> zedges(const RowBits<W>& a, int dim) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint64_t nxt = k + 1 < W ? a.w[k + 1] : 0ull;
+        r.w[k] = a.w[k] ^ ((a.w[k] >> 1) | (nxt << 63));
+    }
+    const int last = dim - 1;  // clear bit dim-1 and above
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const int lo = 64 * k;
+        if (last <= lo) r.w[k] = 0ull;
+        else if (last < lo + 64) r.w[k] &= (1ull << (last - lo)) - 1ull;
+    }
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ bool row_equal(const RowBits<W>& a, const RowBits<W>& b) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < W; k++) ok = ok && a.w[k] == b.w[k];
+    return ok;
+}
+
+// True when the four rows around a row of cells hold one constant label,
+// i.e. no cell of the row is mixed.
+template <int W>
+__device__ __forceinline__ bool quad_uniform(const RowBits<W>& r00, const RowBits<W>& r01, const RowBits<W>& r10,
+                                             const RowBits<W>& r11, int dim) {
+    const RowBits<W> z = zero_row<W>(), f = full_row<W>(dim);
+    const bool allz = row_equal(r00, z) && row_equal(r01, z) && row_equal(r10, z) && row_equal(r11, z);
+    const bool allf = row_equal(r00, f) && row_equal(r01, f) && row_equal(r10, f) && row_equal(r11, f);
+    return allz || allf;
+}
+
+// 8-corner mask of cell (x, y, z) from the four rows around it; corner bit
+// 4dx + 2dy + dz (DMCChunk.cpp:200-438).
+template <int W>
+__device__ __forceinline__ uint32_t cell_mask(const RowBits<W>& r00, const RowBits<W>& r01, const RowBits<W>& r10,
+                                              const RowBits<W>& r11, int z) {
+    return bit_at<W>(r00, z) | (bit_at<W>(r00, z + 1) << 1) | (bit_at<W>(r01, z) << 2) |
+           (bit_at<W>(r01, z + 1) << 3) | (bit_at<W>(r10, z) << 4) | (bit_at<W>(r10, z + 1) << 5) |
+           (bit_at<W>(r11, z) << 6) | (bit_at<W>(r11, z + 1) << 7);
+}
+
+// ------------------------------------------------------------------------
+// plane staging: three consecutive label planes of a chunk, realigned
+// to u64 words in LDS (coalesced funnel-shifted loads); rows are then read
+// from LDS.  Word pw of each plane is a zero pad for the funnel reads.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void stage_plane(uint64_t* s_p, const uint64_t* __restrict__ L, int64_t nwords, int dim,
+                                            int xp) {
+    const int64_t pbits = (int64_t)dim * dim;
+    const int pw = (int)((pbits + 63) >> 6);
+    for (int k = threadIdx.x; k <= pw; k += blockDim.x) {
+        uint64_t v = 0ull;
+        if (xp >= 0 && xp < dim && k < pw) {
+            const int64_t bit = (int64_t)xp * pbits + 64ll * k;
+            const int64_t w = bit >> 6;
+            const int sh = (int)(bit & 63);
+            v = L[w];
+            if (sh) {
+                const uint64_t hi = (w + 1 < nwords) ? L[w + 1] : 0ull;
+                v = (v >> sh) | (hi << (64 - sh));
+            }
+            const int64_t rem = pbits - 64ll * k;
+            if (rem < 64) v &= (1ull << rem) - 1ull;
+        }
+        s_p[k] = v;
+    }
+}
+
+__device__ __forceinline__ void stage_planes(uint64_t* p0, uint64_t* p1, uint64_t* p2, const uint64_t* __restrict__ L,
+                                             int64_t nwords, int dim, int x0) {
+    stage_plane(p0, L, nwords, dim, x0);
+    stage_plane(p1, L, nwords, dim, x0 + 1);
+    stage_plane(p2, L, nwords, dim, x0 + 2);
+}
+
+// Row y of a staged plane.
+template <int W>
+__device__ __forceinline__ RowBits<W> lds_row(const uint64_t* P, int dim, int y, bool valid) {
+    RowBits<W> r = zero_row<W>();
+    if (!valid) return r;
+    const int base = y * dim;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const int bit = base + 64 * k;
+        const int w = bit >> 6;
+        const int sh = bit & 63;
+        uint64_t v = P[w];
+        if (sh) v = (v >> sh) | (P[w + 1] << (64 - sh));
+        const int nb = dim - 64 * k;
+        if (nb < 64) v &= (1ull << nb) - 1ull;
+        r.w[k] = v;
+    }
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ RowBits<W> rshl1(const RowBits<W>& a) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) r.w[k] = (a.w[k] << 1) | (k > 0 ? (a.w[k - 1] >> 63) : 0ull);
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ RowBits<W> ror(const RowBits<W>& a, const RowBits<W>& b) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) r.w[k] = a.w[k] | b.w[k];
+    return r;
+}
+
+}  // namespace bmf

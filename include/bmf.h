@@ -174,6 +174,56 @@ int bmf_last_timing(bmf_ctx* ctx, float* total_ms);
 #define BMF_NUM_STAGES 7
 int bmf_profile_stages(bmf_ctx* ctx, const bmf_chunk_desc* chunks, int n, float* stage_ms);
 
+/* ---- mesh options ------------------------------------------------------- */
+/* Topology and vertex placement of the extraction (per context).
+ * TRIANGLES is the reference's live path: marching-cubes triangles with one
+ * vertex per sign-changing edge (DMCChunk.cpp:440-689).  QUADS is the dual
+ * marching cubes of the reference's polygon table (Tables.hpp:76-354, no
+ * live caller there): one vertex per EdgeTable polygon of each cell, one
+ * quad per sign-changing edge whose four cells lie in the chunk, wound like
+ * the triangles (right-hand normal toward the solid side), smoothed as
+ * MeshProcessor<4> (or, written as triangles, as MeshProcessor<3> after
+ * MeshProcessor::flush_to_tris, MeshProcessor.cpp:73-91). */
+enum { BMF_MESH_TRIANGLES = 0, BMF_MESH_QUADS = 1 };
+/* QUADS vertex placement: the mass point of the polygon's edge points, or
+ * the QEF of those points with unit normals from the sampler's analytic
+ * gradient (qef_simd.h:392-579: FP32 Jacobi SVD, 5 sweeps, singular values
+ * kept in [tol, 1/tol]; CSG sampler only). */
+enum { BMF_PLACE_MASS_POINT = 0, BMF_PLACE_QEF = 1 };
+
+typedef struct {
+    int32_t mesh;                /* BMF_MESH_* */
+    int32_t placement;           /* BMF_PLACE_* (QUADS only) */
+    float qef_tolerance;         /* truncation threshold of the QEF pseudo-inverse */
+    int32_t quads_as_triangles;  /* QUADS: write (a,b,c), (c,d,a) per quad */
+} bmf_mesh_options;
+
+int bmf_set_mesh_options(bmf_ctx* ctx, const bmf_mesh_options* options);
+int bmf_get_mesh_options(bmf_ctx* ctx, bmf_mesh_options* options);
+
+/* Per-vertex QEF residual |ATb - ATA x|^2 (qef_simd.h:434-440) of one
+ * chunk's vertices (QUADS with BMF_PLACE_QEF; zeros otherwise). */
+int bmf_qef_error_to_host(bmf_ctx* ctx, int chunk, float* errors, int64_t cap_vertices);
+
+/* ---- launch shapes ------------------------------------------------------ */
+/* Tile and workgroup shapes of the BATCHED path (the PLAIN path sizes its
+ * launches per chunk).  A field left 0 keeps its current value. */
+typedef struct {
+    int32_t sample_tile;    /* samples per work-queue tile of the sampling kernel: 1024, 2048, 4096 */
+    int32_t sample_blocks;  /* persistent workgroups of the sampling kernel */
+    int32_t row_blocks;     /* persistent workgroups of the plane kernels (count, refine, emit) */
+    int32_t smooth_blocks;  /* workgroups of the smoothing kernels (rounded up to a multiple of 8) */
+} bmf_tuning;
+
+int bmf_get_tuning(bmf_ctx* ctx, bmf_tuning* tuning);
+int bmf_set_tuning(bmf_ctx* ctx, const bmf_tuning* tuning);
+
+/* Autotuner: runs the BATCHED pipeline on these chunks with candidate
+ * shapes (HIP events around each stage, the best of `reps` runs per
+ * candidate), keeps the fastest shape of each stage in the context and
+ * writes it to `best` (may be NULL).  Outputs of the runs are discarded. */
+int bmf_autotune(bmf_ctx* ctx, const bmf_chunk_desc* chunks, int n, int reps, bmf_tuning* best);
+
 /* ---- differentiable extraction --------------------------------------- */
 /* One chunk's density-positioned vertices plus K smoothing iterations as a
  * function of its dim^3 densities, with the topology (labels, cells,

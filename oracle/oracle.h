@@ -141,6 +141,25 @@ void or_fd_grad_f64(const double* dens, int dim, const uint32_t* edge, const uin
                     const uint32_t* idx, int64_t nI, int K, int bp, const or_kdtree* kd, const int64_t* samples,
                     int m, double h, int threads, double* out, int32_t* kink);
 
+/* ---- dual-marching-cubes quad mode (oracle_dmc.c) ----
+ * polygons of EdgeTable[mask] (Tables.hpp:96-354): edges[] concatenated,
+ * polygon p = edges[start[p] .. start[p+1]); returns the polygon count. */
+int or_dmc_polygons(int mask, int* edges, int* start);
+int or_dmc_poly_of_edge(int mask, int local_edge);
+/* placement 0 = mass point, 1 = QEF with the CSG sampler's unit gradients.
+ * code[v] = x | y<<8 | z<<16 | polygon<<24 of the vertex's cell.  Returns 0,
+ * -1 when a capacity is too small (counts still written), -2 on a table
+ * inconsistency. */
+int or_dmc_mesh(const or_sampler_t* s, const float* dens, int dim, const float opos[3], float delta, int placement,
+                float tol, int as_triangles, float* pos, uint8_t* boundary, uint32_t* code, float* qerr,
+                int64_t capV, uint32_t* idx, int64_t capI, int64_t* V, int64_t* nI);
+int or_smooth_n_f32(float* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int N,
+                    int iters, int boundary_processing);
+int or_smooth_n_f64(double* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int N,
+                    int iters, int boundary_processing);
+void or_dmc_positions_f64(const or_sampler_t* s, const double* dens, int dim, const double opos[3], double delta,
+                          int placement, double tol, const uint32_t* code, int64_t V, double* pos, double* qerr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,6 +106,15 @@ def lib():
         L.or_qef_solve_f32.restype = C.c_float
         L.or_qef_solve_f64.argtypes = [f64p, f64p, C.c_int, C.c_double, f64p]
         L.or_qef_solve_f64.restype = C.c_double
+        L.or_dmc_polygons.argtypes = [C.c_int, i32p, i32p]
+        L.or_dmc_poly_of_edge.argtypes = [C.c_int, C.c_int]
+        L.or_dmc_mesh.argtypes = [P(Sampler), f32p, C.c_int, f32p, C.c_float, C.c_int, C.c_float, C.c_int, f32p, u8p,
+                                  u32p, f32p, C.c_int64, u32p, C.c_int64, i64p, i64p]
+        L.or_dmc_mesh.restype = C.c_int
+        L.or_smooth_n_f32.argtypes = [f32p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int, C.c_int]
+        L.or_smooth_n_f64.argtypes = [f64p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int, C.c_int]
+        L.or_dmc_positions_f64.argtypes = [P(Sampler), f64p, C.c_int, f64p, C.c_double, C.c_int, C.c_double, u32p,
+                                           C.c_int64, f64p, f64p]
         topo = [C.c_int, u32p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int]
         L.or_place_f64.argtypes = [f64p, C.c_int, u32p, C.c_int64, f64p]
         L.or_forward_f64.argtypes = [f64p] + topo + [f64p]
@@ -358,3 +367,53 @@ def fd_grad_f64(dens, mesh, K, bp, kd, samples, h=1e-3, threads=8):
     lib().or_fd_grad_f64(_ptr(d, C.c_double), *t, K, int(bp), kd.h, _ptr(smp, C.c_int64), len(smp), h, threads,
                          _ptr(out, C.c_double), _ptr(kink, C.c_int32))
     return out, kink.astype(bool)
+
+
+# ---- dual-marching-cubes quad mode -------------------------------------------------------
+PLACE_MASS_POINT, PLACE_QEF = 0, 1
+
+
+def dmc_mesh(dens, spec=None, opos=(0.0, 0.0, 0.0), delta=1.0, placement=PLACE_MASS_POINT, tol=0.1,
+             as_triangles=False):
+    d = np.ascontiguousarray(dens, np.float32)
+    dim = d.shape[0]
+    c = max(dim - 1, 1)
+    capV, capI = 4 * c ** 3 + 1, 6 * 3 * dim ** 3 + 6
+    pos = np.zeros((capV, 3), np.float32)
+    bnd = np.zeros(capV, np.uint8)
+    code = np.zeros(capV, np.uint32)
+    err = np.zeros(capV, np.float32)
+    idx = np.zeros(capI, np.uint32)
+    o = np.ascontiguousarray(opos, np.float32)
+    V, nI = C.c_int64(), C.c_int64()
+    rc = lib().or_dmc_mesh(spec.ref if spec is not None else None, _ptr(d, C.c_float), dim, _ptr(o, C.c_float),
+                           delta, placement, tol, int(as_triangles), _ptr(pos, C.c_float), _ptr(bnd, C.c_uint8),
+                           _ptr(code, C.c_uint32), _ptr(err, C.c_float), capV, _ptr(idx, C.c_uint32), capI,
+                           C.byref(V), C.byref(nI))
+    if rc != 0:
+        raise RuntimeError(f"or_dmc_mesh rc={rc}")
+    V, nI = V.value, nI.value
+    return dict(pos=pos[:V].copy(), boundary=bnd[:V].copy(), code=code[:V].copy(), qef_error=err[:V].copy(),
+                idx=idx[:nI].copy(), V=V, nI=nI)
+
+
+def smooth_n(pos, boundary, idx, N, iters, boundary_processing=False, f64=False):
+    p = np.ascontiguousarray(pos, np.float64 if f64 else np.float32).copy()
+    b = np.ascontiguousarray(boundary, np.uint8)
+    i = np.ascontiguousarray(idx, np.uint32)
+    fn = lib().or_smooth_n_f64 if f64 else lib().or_smooth_n_f32
+    fn(_ptr(p, C.c_double if f64 else C.c_float), _ptr(b, C.c_uint8), len(b), _ptr(i, C.c_uint32), len(i), N, iters,
+       int(boundary_processing))
+    return p
+
+
+def dmc_positions_f64(dens64, code, spec=None, opos=(0.0, 0.0, 0.0), delta=1.0, placement=PLACE_MASS_POINT, tol=0.1):
+    d = np.ascontiguousarray(dens64, np.float64)
+    cd = np.ascontiguousarray(code, np.uint32)
+    o = np.ascontiguousarray(opos, np.float64)
+    pos = np.zeros((len(cd), 3), np.float64)
+    err = np.zeros(len(cd), np.float64)
+    lib().or_dmc_positions_f64(spec.ref if spec is not None else None, _ptr(d, C.c_double), d.shape[0],
+                               _ptr(o, C.c_double), delta, placement, tol, _ptr(cd, C.c_uint32), len(cd),
+                               _ptr(pos, C.c_double), _ptr(err, C.c_double))
+    return pos, err

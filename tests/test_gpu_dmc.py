@@ -1,0 +1,133 @@
+"""Quad mode (dual marching cubes over the reference's polygon table) on the
+GPU against the CPU oracle, through the C ABI.
+
+Bar: vertex grouping, quad topology and winding, boundary flags bit-exact;
+mass-point and QEF positions and smoothed positions bit-exact with the FP32
+oracle (same operation sequence); QEF error against the FP64 twin reported.
+"""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from binarymeshfitting_amd import capi, workloads
+
+pytestmark = pytest.mark.gpu
+
+TERRAIN = workloads.terrain_sampler()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with capi.Context(max_chunks=128, max_samples=70 * 65 ** 3, max_vertices=1 << 22, max_indices=1 << 24) as c:
+        yield c
+
+
+def _oracle_dmc(oracle, spec, c, placement=0, tol=0.1, as_tris=False):
+    oc = oracle.make_chunk(c.pos, c.size, c.dim, c.overlap, c.iters, c.boundary_processing)
+    delta, opos = oracle.geometry(oc)
+    dens = oracle.sample_block(spec, c.dim, opos, delta)
+    m = oracle.dmc_mesh(dens, spec, opos, delta, placement=placement, tol=tol, as_triangles=as_tris)
+    m["dens"], m["opos"], m["delta"] = dens, opos, delta
+    if c.iters > 0 and m["V"] and m["nI"]:
+        m["pos"] = oracle.smooth_n(m["pos"], m["boundary"], m["idx"], 3 if as_tris else 4, c.iters,
+                                   c.boundary_processing)
+    return m
+
+
+def _run(ctx, chunks, mode):
+    ctx.extract(workloads.to_capi(chunks), mode)
+    rc, res, _, _ = ctx.results(len(chunks))
+    capi.check(rc, "bmf_result")
+    return res
+
+
+@pytest.mark.parametrize("mode", [capi.MODE_PLAIN, capi.MODE_BATCHED], ids=["plain", "batched"])
+@pytest.mark.parametrize("as_tris", [False, True], ids=["quads", "tris"])
+@pytest.mark.parametrize("K", [0, 4])
+def test_terrain_quads_match_oracle(ctx, oracle, mode, as_tris, K):
+    ctx.set_sampler(**TERRAIN)
+    ctx.set_mesh_options(mesh=capi.MESH_QUADS, quads_as_triangles=as_tris)
+    try:
+        chunks = [workloads.Chunk((0.0, 0.0, 0.0), 64.0, 65, 0.0, K, 0),
+                  workloads.Chunk((32.0, 8.0, 96.0), 32.0, 33, 0.0, K, 1)]
+        res = _run(ctx, chunks, mode)
+        spec = oracle.SamplerSpec(**TERRAIN)
+        for j, c in enumerate(chunks):
+            ref = _oracle_dmc(oracle, spec, c, as_tris=as_tris)
+            assert res[j].vertex_count == ref["V"] and res[j].index_count == ref["nI"], c
+            pos, idx, bnd = ctx.mesh(j)
+            assert np.array_equal(idx, ref["idx"]), c
+            assert np.array_equal(bnd, ref["boundary"]), c
+            assert np.array_equal(pos, ref["pos"]), (c, np.abs(pos - ref["pos"]).max())
+    finally:
+        ctx.set_mesh_options(mesh=capi.MESH_TRIANGLES)
+
+
+def test_sphere_quads_closed_and_oriented(ctx):
+    ctx.set_sampler(kind=capi.SAMPLER_SPHERE, world_size=40.0)
+    ctx.set_mesh_options(mesh=capi.MESH_QUADS)
+    try:
+        c = workloads.Chunk((-12.3, -12.3, -12.3), 24.0, 25, 0.0, 0, 0)
+        _run(ctx, [c], capi.MODE_BATCHED)
+        pos, idx, _ = ctx.mesh(0)
+        q = idx.reshape(-1, 4).astype(np.int64)
+        directed = set()
+        for quad in q:
+            for k in range(4):
+                e = (quad[k], quad[(k + 1) % 4])
+                assert e not in directed
+                directed.add(e)
+        assert all((b, a) in directed for a, b in directed)
+        assert len(pos) - len(directed) // 2 + len(q) == 2
+    finally:
+        ctx.set_mesh_options(mesh=capi.MESH_TRIANGLES)
+
+
+def test_hermite_qef_config4(ctx, oracle):
+    """Config 4: 64 chunks of 64^3 cells over [0,256)^3 of the CSG union of
+    200 spheres and 100 boxes (seed 42), QEF placement, tolerance 0.1."""
+    wl = workloads.csg_scene()
+    ctx.set_sampler(**wl.sampler)
+    ctx.set_mesh_options(mesh=capi.MESH_QUADS, placement=capi.PLACE_QEF, qef_tolerance=0.1)
+    try:
+        res = _run(ctx, wl.chunks, capi.MODE_BATCHED)
+        spec = oracle.SamplerSpec(**wl.sampler)
+        with ThreadPoolExecutor(16) as ex:
+            refs = list(ex.map(lambda c: _oracle_dmc(oracle, spec, c, placement=1, tol=0.1), wl.chunks))
+        worst32, errs64, resid = 0.0, [], []
+        for j, (c, ref) in enumerate(zip(wl.chunks, refs)):
+            assert res[j].vertex_count == ref["V"] and res[j].index_count == ref["nI"], c
+            if not ref["V"]:
+                continue
+            pos, idx, bnd = ctx.mesh(j)
+            assert np.array_equal(idx, ref["idx"]) and np.array_equal(bnd, ref["boundary"]), c
+            worst32 = max(worst32, float(np.abs(pos - ref["pos"]).max()))
+            e = ctx.qef_errors(j)
+            assert np.array_equal(e, ref["qef_error"])
+            d64 = oracle.sample_block_f64(spec, c.dim, ref["opos"].astype(np.float64), float(ref["delta"]))
+            p64, _ = oracle.dmc_positions_f64(d64, ref["code"], spec, ref["opos"], float(ref["delta"]),
+                                              oracle.PLACE_QEF, 0.1)
+            errs64.append(np.abs(pos.astype(np.float64) - p64).max(1))
+            resid.append(e)
+        errs64 = np.concatenate(errs64)
+        resid = np.concatenate(resid)
+        print(f"V={len(errs64)} max |GPU - FP32 oracle| {worst32:.3g}; |GPU FP32 - FP64| max {errs64.max():.3g} "
+              f"p99 {np.quantile(errs64, 0.99):.3g} median {np.median(errs64):.3g}; "
+              f"QEF residual median {np.median(resid):.3g} max {resid.max():.3g}")
+        assert worst32 == 0.0
+        assert np.quantile(errs64, 0.99) < 1e-3 * 64
+    finally:
+        ctx.set_mesh_options(mesh=capi.MESH_TRIANGLES)
+
+
+def test_mesh_options_rejects_and_round_trips(ctx):
+    with pytest.raises(capi.BmfError):
+        ctx.set_mesh_options(mesh=7)
+    with pytest.raises(capi.BmfError):
+        ctx.set_mesh_options(mesh=capi.MESH_QUADS, placement=5)
+    ctx.set_mesh_options(mesh=capi.MESH_QUADS, placement=capi.PLACE_QEF, qef_tolerance=0.25, quads_as_triangles=True)
+    o = ctx.mesh_options()
+    assert o["mesh"] == 1 and o["placement"] == 1 and abs(o["qef_tolerance"] - 0.25) < 1e-7
+    ctx.set_mesh_options()
+    assert ctx.mesh_options()["mesh"] == capi.MESH_TRIANGLES

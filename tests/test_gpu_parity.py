@@ -261,3 +261,79 @@ def test_adaptive_octaves_match_full_evaluation(ctx, oracle, params):
         if len(pos):
             ref = oracle.extract_chunk(spec, _ochunk(oracle, c))
             assert np.array_equal(pos.view(np.uint32), ref["pos"].view(np.uint32))
+
+
+def _mixed_lod_check_set(wl, extra=40):
+    """Chunks whose y range can meet the terrain band (|amp * fbm| <= amp)
+    plus a seeded sample of the others: the oracle samples every chunk in
+    full, so the culled ones are spot-checked."""
+    s = wl.sampler
+    band = (s["y0"] - abs(s["amp"]) - 1.0, s["y0"] + abs(s["amp"]) + 1.0)
+    near = [j for j, c in enumerate(wl.chunks) if c.pos[1] <= band[1] and c.pos[1] + c.size >= band[0]]
+    rng = workloads.SplitMix64(11)
+    far = [j for j in range(len(wl.chunks)) if j not in set(near)]
+    pick = sorted({far[rng.randint(0, len(far) - 1)] for _ in range(extra)})
+    return near, pick
+
+
+def test_mixed_lod_batch_matches_oracle(oracle):
+    """Config 3: 1,536 uneven-LOD chunks (33^3 / 65^3 / 129^3 samples at
+    spacing 1 / 2 / 4), K ~ U{0..8}, 7-octave field, autotuned shapes."""
+    wl = workloads.mixed_lod()
+    cd = workloads.to_capi(wl.chunks)
+    n = len(cd)
+    with capi.Context(max_chunks=2048, max_samples=96 * 129 ** 3, max_vertices=1 << 23, max_indices=1 << 25) as c:
+        c.set_sampler(**wl.sampler)
+        tuned = c.autotune(cd, reps=2)
+        print("autotuned", tuned)
+        c.extract(cd, capi.MODE_BATCHED)
+        rc, res, _, _ = c.results(n)
+        assert rc == capi.OK
+        near, far = _mixed_lod_check_set(wl)
+        check = near + far
+        spec = _ospec(oracle, wl.sampler)
+        V, nI, cm, ck = oracle.extract_batch(spec, [_ochunk(oracle, wl.chunks[j]) for j in check], threads=16)
+        sampled = sum(r.sampled for r in res)
+        nonempty = sum(1 for r in res if r.vertex_count)
+        print(f"sampled {sampled}/{n}, non-empty {nonempty}/{n}, oracle-checked {len(check)}")
+        for k, j in enumerate(check):
+            r = res[j]
+            assert r.vertex_count == V[k] and r.index_count == nI[k], wl.chunks[j]
+            assert bool(r.contains_mesh) == bool(cm[k]), wl.chunks[j]
+            if r.vertex_count:
+                pos, idx, _ = c.mesh(j)
+                assert oracle.checksum_mesh(pos, idx) == int(ck[k]), wl.chunks[j]
+        # every non-empty chunk is in the checked set
+        assert {j for j, r in enumerate(res) if r.vertex_count} <= set(check)
+
+
+def test_tuning_shapes_give_identical_meshes(ctx):
+    ctx.set_sampler(**TERRAIN)
+    wl = workloads.uniform_grid(n_side=3, c=64, iters=2)
+    cd = workloads.to_capi(wl.chunks)
+    default = ctx.tuning()
+
+    def run():
+        ctx.extract(cd, capi.MODE_BATCHED)
+        rc, res, _, _ = ctx.results(len(cd))
+        assert rc == capi.OK
+        return [ctx.mesh(j) for j in range(len(cd)) if res[j].vertex_count]
+
+    ref = run()
+    try:
+        for tile, sb, rb, smb in [(1024, 512, 256, 256), (4096, 4096, 1024, 2048), (2048, 256, 4096, 8)]:
+            ctx.set_tuning(tile, sb, rb, smb)
+            got = run()
+            assert len(got) == len(ref)
+            for a, b in zip(ref, got):
+                for x, y in zip(a, b):
+                    assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (tile, sb, rb, smb)
+    finally:
+        ctx.set_tuning(**default)
+
+
+def test_set_tuning_rejects_bad_shapes(ctx):
+    for bad in [dict(sample_tile=1000), dict(sample_blocks=-1), dict(row_blocks=1 << 30)]:
+        with pytest.raises(capi.BmfError) as e:
+            ctx.set_tuning(**bad)
+        assert e.value.code == capi.EINVAL
