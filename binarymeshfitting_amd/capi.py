@@ -22,7 +22,7 @@ EXPORTS = [
     "bmf_label_block", "bmf_cell_masks", "bmf_last_timing", "bmf_profile_stages", "bmf_tape_create",
     "bmf_tape_destroy", "bmf_forward_diff", "bmf_tape_get", "bmf_tape_positions", "bmf_loss_nearest",
     "bmf_backward_diff", "bmf_get_tuning", "bmf_set_tuning", "bmf_autotune", "bmf_set_mesh_options",
-    "bmf_get_mesh_options", "bmf_qef_error_to_host",
+    "bmf_get_mesh_options", "bmf_qef_error_to_host", "bmf_normals_to_host",
 ]
 MESH_TRIANGLES, MESH_QUADS = 0, 1
 PLACE_MASS_POINT, PLACE_QEF = 0, 1
@@ -57,12 +57,12 @@ class ChunkResult(C.Structure):
 
 class MeshView(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("indices", C.c_void_p), ("boundary", C.c_void_p),
-                ("total_vertices", C.c_int64), ("total_indices", C.c_int64)]
+                ("normals", C.c_void_p), ("total_vertices", C.c_int64), ("total_indices", C.c_int64)]
 
 
 class MeshOptions(C.Structure):
     _fields_ = [("mesh", C.c_int32), ("placement", C.c_int32), ("qef_tolerance", C.c_float),
-                ("quads_as_triangles", C.c_int32)]
+                ("quads_as_triangles", C.c_int32), ("normals", C.c_int32)]
 
 
 class Tuning(C.Structure):
@@ -113,6 +113,7 @@ def lib():
         L.bmf_set_mesh_options.argtypes = [ctxp, P(MeshOptions)]
         L.bmf_get_mesh_options.argtypes = [ctxp, P(MeshOptions)]
         L.bmf_qef_error_to_host.argtypes = [ctxp, C.c_int, P(C.c_float), C.c_int64]
+        L.bmf_normals_to_host.argtypes = [ctxp, C.c_int, P(C.c_float), C.c_int64]
         L.bmf_get_tuning.argtypes = [ctxp, P(Tuning)]
         L.bmf_set_tuning.argtypes = [ctxp, P(Tuning)]
         L.bmf_autotune.argtypes = [ctxp, P(ChunkDesc), C.c_int, C.c_int, P(Tuning)]
@@ -246,14 +247,21 @@ class Context:
         return t.value
 
     def set_mesh_options(self, mesh=MESH_TRIANGLES, placement=PLACE_MASS_POINT, qef_tolerance=0.1,
-                         quads_as_triangles=False):
-        o = MeshOptions(mesh, placement, qef_tolerance, int(quads_as_triangles))
+                         quads_as_triangles=False, normals=False):
+        o = MeshOptions(mesh, placement, qef_tolerance, int(quads_as_triangles), int(normals))
         check(lib().bmf_set_mesh_options(self.h, C.byref(o)), "bmf_set_mesh_options")
 
     def mesh_options(self):
         o = MeshOptions()
         check(lib().bmf_get_mesh_options(self.h, C.byref(o)), "bmf_get_mesh_options")
         return {k: getattr(o, k) for k, _ in o._fields_}
+
+    def normals(self, chunk):
+        rc, res, _, _ = self.results(chunk + 1)
+        V = res[chunk].vertex_count
+        out = np.zeros((V, 3), np.float32)
+        check(lib().bmf_normals_to_host(self.h, chunk, _ptr(out, C.c_float), V), "bmf_normals_to_host")
+        return out
 
     def qef_errors(self, chunk):
         rc, res, _, _ = self.results(chunk + 1)

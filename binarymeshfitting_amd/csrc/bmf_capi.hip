@@ -38,7 +38,7 @@ void free_arenas(bmf_ctx* c) {
                     A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
                     A.pos,     A.pos4,      A.boundary, A.vedge,       A.idx,          A.ctr,
-                    A.sampler, c->d_spheres, c->d_boxes, A.ring, A.ring_n, A.qef_err};
+                    A.sampler, c->d_spheres, c->d_boxes, A.ring, A.ring_n, A.qef_err, A.nrm4, A.fn4, A.nrm};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     A = Arenas{};
@@ -94,6 +94,13 @@ int upload_descs(bmf_ctx* c, const bmf_chunk_desc* chunks, int n) {
                            hipMemcpyHostToDevice, c->stream));
     BMF_HIP(hipEventRecord(c->ev_upload[slot], c->stream));
     return BMF_OK;
+}
+
+int max_steps(const bmf_ctx* c, const bmf_chunk_desc* chunks, int n) {
+    int k = 0;
+    for (int i = 0; i < n; i++) k = std::max(k, chunks[i].iters);
+    if (c->A.normals && n > 0) k = std::max(k, 1);
+    return k;
 }
 
 int check_desc(const bmf_chunk_desc& d) {
@@ -286,11 +293,8 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
     if (n == 0) return BMF_OK;
     rc = upload_descs(c, chunks, n);
     if (rc) return rc;
-    int maxk = 0, wclasses = 0;
-    for (int i = 0; i < n; i++) {
-        maxk = std::max(maxk, chunks[i].iters);
-        wclasses |= row_word_class(chunks[i].dim);
-    }
+    int maxk = max_steps(c, chunks, n), wclasses = 0;
+    for (int i = 0; i < n; i++) wclasses |= row_word_class(chunks[i].dim);
     const Arenas& A = c->A;
     const int pb = c->persistent_blocks;
     if (mode == BMF_MODE_PLAIN) {
@@ -309,7 +313,8 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
             const int64_t dim = chunks[j].dim;
             const int64_t ns = dim * dim * dim;
             const int sb = (int)std::min<int64_t>(ceil_div(ns, A.sample_tile), pb);
-            rc = enqueue_pipeline(c, Aj, 1, chunks[j].iters, row_word_class(chunks[j].dim), 1, j == 0, sb, (int)dim, 1,
+            rc = enqueue_pipeline(c, Aj, 1, max_steps(c, &chunks[j], 1), row_word_class(chunks[j].dim), 1, j == 0, sb,
+                                  (int)dim, 1,
                                   std::min(pb, 1024));
             if (rc) return rc;
         }
@@ -392,6 +397,7 @@ int bmf_mesh_view_get(bmf_ctx* c, bmf_mesh_view* v) {
     v->positions = c->A.pos;
     v->indices = c->A.idx;
     v->boundary = c->A.boundary;
+    v->normals = c->A.normals ? c->A.nrm : nullptr;
     v->total_vertices = std::min<int64_t>((int64_t)c->h_ctr->v_bump, c->A.cap_vertices);
     v->total_indices = std::min<int64_t>((int64_t)c->h_ctr->i_bump, c->A.cap_indices);
     return BMF_OK;
@@ -515,11 +521,8 @@ int bmf_profile_stages(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, float* s
     c->have_extract = true;
     rc = upload_descs(c, chunks, n);
     if (rc) return rc;
-    int maxk = 0, wclasses = 0;
-    for (int i = 0; i < n; i++) {
-        maxk = std::max(maxk, chunks[i].iters);
-        wclasses |= row_word_class(chunks[i].dim);
-    }
+    int maxk = max_steps(c, chunks, n), wclasses = 0;
+    for (int i = 0; i < n; i++) wclasses |= row_word_class(chunks[i].dim);
     BMF_HIP(hipEventRecord(c->ev0, c->stream));
     rc = enqueue_pipeline(c, c->A, n, maxk, wclasses, 1, 1, c->tune.sample_blocks, c->tune.row_blocks,
                           std::min(n, c->cu_count), c->tune.smooth_blocks, c->ev_stage);
@@ -537,6 +540,8 @@ int bmf_set_mesh_options(bmf_ctx* c, const bmf_mesh_options* o) {
     if (o->placement != BMF_PLACE_MASS_POINT && o->placement != BMF_PLACE_QEF) return BMF_EINVAL;
     if (!finite(o->qef_tolerance) || o->qef_tolerance < 0.0f) return BMF_EINVAL;
     if (o->quads_as_triangles != 0 && o->quads_as_triangles != 1) return BMF_EINVAL;
+    if (o->normals != 0 && o->normals != 1) return BMF_EINVAL;
+    if (o->normals && o->mesh != BMF_MESH_TRIANGLES) return BMF_EINVAL;
     BMF_HIP(hipSetDevice(c->device));
     BMF_HIP(hipStreamSynchronize(c->stream));
     Arenas& A = c->A;
@@ -551,7 +556,19 @@ int bmf_set_mesh_options(bmf_ctx* c, const bmf_mesh_options* o) {
             return BMF_EHIP;
         }
     }
+    if (o->normals && !A.nrm) {
+        if (dalloc(&A.nrm4, A.cap_vertices) != hipSuccess || dalloc(&A.fn4, A.cap_indices / 3 + 1) != hipSuccess ||
+            dalloc(&A.nrm, 3 * A.cap_vertices) != hipSuccess) {
+            if (A.nrm4) (void)hipFree(A.nrm4);
+            if (A.fn4) (void)hipFree(A.fn4);
+            if (A.nrm) (void)hipFree(A.nrm);
+            A.nrm4 = A.fn4 = nullptr;
+            A.nrm = nullptr;
+            return BMF_EHIP;
+        }
+    }
     c->mesh = *o;
+    A.normals = o->normals;
     A.mesh_mode = o->mesh;
     A.placement = o->placement;
     A.qef_tol = o->qef_tolerance;
@@ -561,6 +578,23 @@ int bmf_set_mesh_options(bmf_ctx* c, const bmf_mesh_options* o) {
     c->graphs.clear();
     c->have_extract = false;
     c->results_valid = false;
+    return BMF_OK;
+}
+
+int bmf_normals_to_host(bmf_ctx* c, int chunk, float* normals, int64_t cap_v) {
+    if (!c || !normals) return BMF_EINVAL;
+    if (!c->A.normals) return BMF_ESTATE;
+    BMF_HIP(hipSetDevice(c->device));
+    int rc = fetch_results(c);
+    if (rc) return rc;
+    if (chunk < 0 || chunk >= c->last_n) return BMF_EINVAL;
+    const bmf_chunk_result& r = c->h_results[chunk];
+    if (r.status != BMF_OK) return r.status;
+    if (cap_v < r.vertex_count) return BMF_ECAPACITY;
+    if (!r.vertex_count) return BMF_OK;
+    BMF_HIP(hipMemcpyAsync(normals, c->A.nrm + 3 * r.vertex_offset, sizeof(float) * 3 * r.vertex_count,
+                           hipMemcpyDeviceToHost, c->stream));
+    BMF_HIP(hipStreamSynchronize(c->stream));
     return BMF_OK;
 }
 
@@ -634,11 +668,8 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
     c->have_extract = false;
     rc = upload_descs(c, chunks, n);
     if (rc) return rc;
-    int maxk = 0, wclasses = 0;
-    for (int i = 0; i < n; i++) {
-        maxk = std::max(maxk, chunks[i].iters);
-        wclasses |= row_word_class(chunks[i].dim);
-    }
+    int maxk = max_steps(c, chunks, n), wclasses = 0;
+    for (int i = 0; i < n; i++) wclasses |= row_word_class(chunks[i].dim);
     // stage time (ms, best of reps) of the pipeline under tuning t
     float st[BMF_NUM_STAGES];
     auto run = [&](const bmf_tuning& t, float* out) -> int {

@@ -42,11 +42,13 @@ struct bmf_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int persistent_blocks = 2048;  // default workgroups of the persistent kernels
     bmf_tuning tune{};             // BATCHED launch shapes (bmf_set_tuning / bmf_autotune)
-    bmf_mesh_options mesh{0, 0, 0.1f, 0};  // bmf_set_mesh_options
+    bmf_mesh_options mesh{0, 0, 0.1f, 0, 0};  // bmf_set_mesh_options
 };
 
 namespace bmf {
 
+// Smoothing launches of a batch: the largest K, at least 1 with normals.
+int max_steps(const bmf_ctx* c, const bmf_chunk_desc* chunks, int n);
 // Copies n descriptors to the device arena on the context stream.
 int upload_descs(bmf_ctx* c, const bmf_chunk_desc* chunks, int n);
 // BMF_OK or BMF_EINVAL for one descriptor (host only).

@@ -669,6 +669,7 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
     A.ext_base = 0;
     A.mesh_mode = BMF_MESH_TRIANGLES;  // the adjoint is defined on the triangle path
     A.quad_tris = 0;
+    A.normals = 0;
     const int wcls = row_word_class(dim);
     // extraction with every octave evaluated (densities exact everywhere),
     // no smoothing: the tape smooths its own copy

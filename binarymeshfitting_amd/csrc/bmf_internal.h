@@ -121,6 +121,11 @@ struct Arenas {
     uint32_t* ring;            // [kRingSlots][cap_vertices] primitives around each vertex (quad mode)
     uint32_t* ring_n;          // [cap_vertices] ring sizes
     float* qef_err;            // [cap_vertices] QEF residual of each vertex (QEF placement)
+    // vertex normals of the triangle path (bmf_mesh_options::normals)
+    int32_t normals;
+    float4* nrm4;              // [cap_vertices] normal state during the smoothing
+    float4* fn4;               // [cap_indices / 3] per-triangle normal of the current step
+    float* nrm;                // [cap_vertices * 3] unit normals (output)
 };
 
 // Quad mode: a vertex (one polygon of a cell) has at most 7 edges, so at most

@@ -161,8 +161,9 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
 // samples are compacted into an LDS queue (wave ballot + popcount) so each
 // octave runs on dense waves.  Samples that run all octaves get their exact
 // density written and their bit set in the `exact` bitmap; the others only
-// get their label (their densities are finished later, in k_count, when
-// they are endpoints of a sign-changing edge).  Without adaptation every
+// get their label (k_count queues the ones that are endpoints of a
+// sign-changing edge and k_refine finishes their densities).  Without
+// adaptation every
 // sample is evaluated fully (bit-identical values either way).
 // ------------------------------------------------------------------------
 __device__ __forceinline__ float density_from_sum(const SamplerDev& S, float sum, float t2) {
@@ -539,8 +540,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(Arenas A, int n) {
 
 // Ordered list of the chunks that run the smoothing (K > 0, a mesh, fitted)
 // and the prefix of their vertex tiles, for k_smooth (one block).
-__device__ __forceinline__ bool smooths(const ChunkMeta& M) {
-    return M.iters > 0 && M.V > 0 && M.T > 0 && M.status == BMF_OK && M.state == kStateSampled;
+__device__ __forceinline__ bool smooths(const ChunkMeta& M, int normals) {
+    return (M.iters > 0 || normals) && M.V > 0 && M.T > 0 && M.status == BMF_OK && M.state == kStateSampled;
 }
 
 // The list is grouped by XCD: the k-th smoothing chunk goes to XCD k % 8,
@@ -555,7 +556,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
     const int j0 = tid * per, j1 = min(n, j0 + per);
     // ---- rank of each smoothing chunk (ordered compaction) ----
     int64_t cnt = 0;
-    for (int j = j0; j < j1; j++) cnt += smooths(A.meta[j]);
+    for (int j = j0; j < j1; j++) cnt += smooths(A.meta[j], A.normals);
     s_scan[0][tid] = cnt;
     __syncthreads();
     for (int off = 1; off < kPlanThreads; off <<= 1) {
@@ -567,7 +568,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
     const int ns = (int)s_scan[0][kPlanThreads - 1];
     int k = (int)(s_scan[0][tid] - cnt);
     for (int j = j0; j < j1; j++) {
-        if (!smooths(A.meta[j])) continue;
+        if (!smooths(A.meta[j], A.normals)) continue;
         // grouped position: groups g = k % 8 in order, rank k / 8 inside
         const int g = k & (kXcds - 1), q = k / kXcds;
         int pos = q;
@@ -665,11 +666,12 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, co
     const int64_t vo = M.v_off, io = M.i_off;
     const uint32_t Vc = (uint32_t)M.V, Tc = (uint32_t)M.T;
     float* __restrict__ P = A.pos + vo * 3;
-    float4* __restrict__ P4 = M.iters > 0 ? A.pos4 + vo : nullptr;  // smoothing state
+    const bool cells = M.iters > 0 || A.normals;  // the chunk runs the smoothing kernels
+    float4* __restrict__ P4 = cells ? A.pos4 + vo : nullptr;  // smoothing state
+    float4* __restrict__ N4 = cells && A.normals ? A.nrm4 + vo : nullptr;
     uint8_t* __restrict__ B = A.boundary + vo;
     uint32_t* __restrict__ E = A.vedge + vo;
     const float* __restrict__ D = A.dens + M.dens_off;
-    const bool cells = M.iters > 0;
     uint32_t tcarry = s_base[2][y];
     for (int z0 = 0; z0 < dim; z0 += 64) {
         const int z = z0 + lane;
@@ -697,6 +699,7 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, co
                     P[3 * (int64_t)id + 1] = qy;
                     P[3 * (int64_t)id + 2] = qz;
                     if (P4) P4[id] = make_float4(qx, qy, qz, 0.0f);
+                    if (N4) N4[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     B[id] = (uint8_t)(x == 0 || y == 0 || z == 0 || x == dim - 1 || y == dim - 1 || z == dim - 1 ||
                                       xe == dim - 1 || ye == dim - 1 || ze == dim - 1);
                     E[id] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | ((uint32_t)ax << 24);
@@ -907,6 +910,166 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
     }
 }
 
+
+// With vertex normals (the smooth_normals branch of MeshProcessor<3>,
+// MeshProcessor.cpp:159-216, 271-287, for N = 3): every chunk of the list
+// runs max(K, 1) steps; step m's face normal is
+// -cross(normalize(p0 - p1), normalize(p0 - p2)) while m == 0, m < max_norms
+// (min(K/2 - 3, 10)) or m < 3, else the sum of its corners' normals; the
+// primal step averages them over the ring for every vertex with a ring,
+// normalises when m == 3 or (m == 0 and K <= 3) and keeps the result when
+// its y is not 0; positions move as in k_dual/k_primal (K > 0 only); the
+// last step writes the unit normal.
+__global__ void __launch_bounds__(kSmoothThreads) k_dual_nrm(Arenas A, int m) {
+    const int nsm = A.ctr->n_smooth;
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t end = A.smooth_xcd_t[xcd + 1];
+    for (int64_t tile = A.smooth_xcd_t[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
+        const int k = find_chunk(A.smooth_tprefix, nsm, tile);
+        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int K = M.iters;
+        if (max(K, 1) <= m) continue;
+        const int64_t t = (tile - A.smooth_tprefix[k]) * kSmoothThreads + threadIdx.x;
+        if (t >= M.T) continue;
+        const uint32_t* __restrict__ I = A.idx + M.i_off + 3 * t;
+        const float4* __restrict__ P = A.pos4 + M.v_off;
+        const uint32_t i0 = I[0], i1 = I[1], i2 = I[2];
+        if (i0 >= (uint32_t)M.V || i1 >= (uint32_t)M.V || i2 >= (uint32_t)M.V) {
+            A.ctr->error = 5;
+            continue;
+        }
+        const float4 a = P[i0], b = P[i1], c = P[i2];
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        cx = cx + a.x;
+        cy = cy + a.y;
+        cz = cz + a.z;
+        cx = cx + b.x;
+        cy = cy + b.y;
+        cz = cz + b.z;
+        cx = cx + c.x;
+        cy = cy + c.y;
+        cz = cz + c.z;
+        const int64_t tt = M.i_off / 3 + t;
+        A.cen4[tt] = make_float4(cx / 3.0f, cy / 3.0f, cz / 3.0f, 0.0f);
+        const int max_norms = min(K / 2 - 3, 10);
+        float4 fn;
+        if (m == 0 || m < max_norms || m < 3) {
+            float ux = a.x - b.x, uy = a.y - b.y, uz = a.z - b.z;
+            float wx = a.x - c.x, wy = a.y - c.y, wz = a.z - c.z;
+            const float lu = sqrtf((ux * ux + uy * uy) + uz * uz), lw = sqrtf((wx * wx + wy * wy) + wz * wz);
+            ux = ux / lu;
+            uy = uy / lu;
+            uz = uz / lu;
+            wx = wx / lw;
+            wy = wy / lw;
+            wz = wz / lw;
+            fn = make_float4(-(uy * wz - wy * uz), -(uz * wx - wz * ux), -(ux * wy - wx * uy), 0.0f);
+        } else {
+            const float4* __restrict__ N = A.nrm4 + M.v_off;
+            const float4 na = N[i0], nb = N[i1], nc = N[i2];
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            sx = sx + na.x;
+            sy = sy + na.y;
+            sz = sz + na.z;
+            sx = sx + nb.x;
+            sy = sy + nb.y;
+            sz = sz + nb.z;
+            sx = sx + nc.x;
+            sy = sy + nc.y;
+            sz = sz + nc.z;
+            fn = make_float4(sx, sy, sz, 0.0f);
+        }
+        A.fn4[tt] = fn;
+    }
+}
+
+__global__ void __launch_bounds__(kSmoothThreads) k_primal_nrm(Arenas A, int m) {
+    __shared__ uint64_t s_tri[256];
+    for (int q = threadIdx.x; q < 256; q += blockDim.x) s_tri[q] = c_tri_table[q];
+    __syncthreads();
+    const int nsm = A.ctr->n_smooth;
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t end = A.smooth_xcd_v[xcd + 1];
+    for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
+        const int k = find_chunk(A.smooth_prefix, nsm, tile);
+        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int K = M.iters;
+        const int steps = max(K, 1);
+        if (steps <= m) continue;
+        const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
+        if (v >= M.V) continue;
+        const int64_t vo = M.v_off, T = M.T, g = vo + v;
+        const int dim = M.dim;
+        const float4* __restrict__ C = A.cen4 + M.i_off / 3;
+        const float4* __restrict__ F = A.fn4 + M.i_off / 3;
+        const uint32_t* __restrict__ CT = A.cell_tb + M.dens_off;
+        const uint8_t* __restrict__ CM = A.cell_mask + M.dens_off;
+        const uint32_t code = A.vedge[g];
+        const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
+        const int ax = (int)(code >> 24);
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+        uint32_t cnt = 0;
+        for (int c = 0; c < 4; c++) {
+            const int cx = ex + c_ring[ax][c][0], cy = ey + c_ring[ax][c][1], cz = ez + c_ring[ax][c][2];
+            if (cx < 0 || cy < 0 || cz < 0 || cx > dim - 2 || cy > dim - 2 || cz > dim - 2) continue;
+            const int64_t ci = ((int64_t)cx * dim + cy) * dim + cz;
+            const uint32_t le = (uint32_t)c_ring[ax][c][3];
+            const uint32_t tb = CT[ci];
+            const uint64_t row = s_tri[CM[ci]];
+            const int ns = (int)(row >> 60);
+            for (int s = 0; s < ns; s++) {
+                if (((row >> (4 * s)) & 15ull) != le) continue;
+                const int64_t t = (int64_t)tb + s / 3;
+                if (t >= T) {
+                    A.ctr->error = 7;
+                    break;
+                }
+                const float4 q = C[t], f = F[t];
+                sx = sx + q.x;
+                sy = sy + q.y;
+                sz = sz + q.z;
+                nx = nx + f.x;
+                ny = ny + f.y;
+                nz = nz + f.z;
+                cnt++;
+            }
+        }
+        float4 n = A.nrm4[g];
+        if (cnt > 0) {
+            const float fc = (float)cnt;
+            float ax2 = nx / fc, ay2 = ny / fc, az2 = nz / fc;
+            const bool unit = m != steps - 1 && (m == 3 || (m == 0 && K <= 3));
+            if (unit) {
+                const float l = sqrtf((ax2 * ax2 + ay2 * ay2) + az2 * az2);
+                ax2 = ax2 / l;
+                ay2 = ay2 / l;
+                az2 = az2 / l;
+            }
+            if (ay2 != 0.0f) {
+                n = make_float4(ax2, ay2, az2, 0.0f);
+                A.nrm4[g] = n;
+            }
+            if (K > 0 && (M.bp || !A.boundary[g])) {
+                const float px = sx / fc, py = sy / fc, pz = sz / fc;
+                A.pos4[g] = make_float4(px, py, pz, 0.0f);
+                if (m == K - 1) {
+                    float* __restrict__ P = A.pos + g * 3;
+                    P[0] = px;
+                    P[1] = py;
+                    P[2] = pz;
+                }
+            }
+        }
+        if (m == steps - 1) {
+            const float l = sqrtf((n.x * n.x + n.y * n.y) + n.z * n.z);
+            float* __restrict__ O = A.nrm + g * 3;
+            O[0] = l > 0.0f ? n.x / l : 0.0f;
+            O[1] = l > 0.0f ? n.y / l : 0.0f;
+            O[2] = l > 0.0f ? n.z / l : 0.0f;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------
 // finalize: per-chunk results for the host
 // ------------------------------------------------------------------------
@@ -1034,8 +1197,13 @@ hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s) {
 }
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
     blocks = blocks < kXcds ? kXcds : blocks - blocks % kXcds;  // whole XCD groups
-    hipLaunchKernelGGL(k_dual, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
-    hipLaunchKernelGGL(k_primal, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    if (A.normals) {
+        hipLaunchKernelGGL(k_dual_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+        hipLaunchKernelGGL(k_primal_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    } else {
+        hipLaunchKernelGGL(k_dual, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+        hipLaunchKernelGGL(k_primal, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    }
     return hipGetLastError();
 }
 hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s) {

@@ -117,6 +117,7 @@ typedef struct {
     const float* positions;  /* [vertices][3], grid units */
     const uint32_t* indices; /* chunk-local vertex ids */
     const uint8_t* boundary; /* DualVertex::boundary (DMCChunk.cpp:673) */
+    const float* normals;    /* [vertices][3] unit normals, or NULL when not enabled */
     int64_t total_vertices;
     int64_t total_indices;
 } bmf_mesh_view;
@@ -196,7 +197,20 @@ typedef struct {
     int32_t placement;           /* BMF_PLACE_* (QUADS only) */
     float qef_tolerance;         /* truncation threshold of the QEF pseudo-inverse */
     int32_t quads_as_triangles;  /* QUADS: write (a,b,c), (c,d,a) per quad */
+    int32_t normals;             /* TRIANGLES: also compute unit vertex normals (below) */
 } bmf_mesh_options;
+
+/* Vertex normals (TRIANGLES).  The reference computes them only with
+ * smooth_normals (MeshProcessor.cpp:159-216, 271-287), off on its live path
+ * (ChunkGenerator.cpp:116); here they follow that branch for N = 3: per
+ * smoothing step m the face normal -cross(normalize(p0-p1), normalize(p0-p2))
+ * while m == 0, m < min(K/2-3, 10) or m < 3, else the sum of the corners'
+ * normals; per primal step the ring average, normalised when m == 3 or
+ * (m == 0 and K <= 3), kept when its y is not 0, for every vertex with a ring
+ * (boundary vertices included; their positions stay pinned); K = 0 runs one
+ * such step without moving positions; the output is the final normal divided
+ * by its length (0 for a zero normal). */
+int bmf_normals_to_host(bmf_ctx* ctx, int chunk, float* normals, int64_t cap_vertices);
 
 int bmf_set_mesh_options(bmf_ctx* ctx, const bmf_mesh_options* options);
 int bmf_get_mesh_options(bmf_ctx* ctx, bmf_mesh_options* options);

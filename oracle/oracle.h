@@ -157,6 +157,13 @@ int or_smooth_n_f32(float* pos, const uint8_t* boundary, int64_t V, const uint32
                     int iters, int boundary_processing);
 int or_smooth_n_f64(double* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int N,
                     int iters, int boundary_processing);
+/* Triangle-path smoothing with vertex normals (the smooth_normals branch of
+ * MeshProcessor<3>, MeshProcessor.cpp:159-216, 271-287, as defined in
+ * oracle_dmc.c); positions as or_smooth_*, normals [V][3] unit or zero. */
+int or_smooth_normals_f32(float* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int iters,
+                          int boundary_processing, float* normals);
+int or_smooth_normals_f64(double* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int iters,
+                          int boundary_processing, double* normals);
 void or_dmc_positions_f64(const or_sampler_t* s, const double* dens, int dim, const double opos[3], double delta,
                           int placement, double tol, const uint32_t* code, int64_t V, double* pos, double* qerr);
 

@@ -297,3 +297,117 @@ void or_dmc_positions_f64(const or_sampler_t* s, const double* dens, int dim, co
         }
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Vertex normals of the triangle path.  The reference computes them only    */
+/* with smooth_normals (MeshProcessor.cpp:159-216, 271-287), which its live  */
+/* path leaves off (ChunkGenerator.cpp:116); they are defined here from that */
+/* branch for N = 3:                                                          */
+/*   dual step m: face normal -cross(normalize(p0-p1), normalize(p0-p2)) of   */
+/*     the positions at step m while m == 0 || m < max_norms || m < 3, with  */
+/*     max_norms = min(K/2 - 3, 10); later steps take the sum of the three    */
+/*     vertex normals;                                                        */
+/*   primal steps: n = (sum of the ring's face normals, ascending) / count,   */
+/*     normalised when m == 3 or (m == 0 and K <= 3), kept only if n.y != 0;  */
+/*     every vertex with a ring gets a normal (positions of boundary          */
+/*     vertices stay pinned as before);                                       */
+/*   K = 0: one face-normal step and one primal step, positions unchanged;    */
+/*   output: the final normal divided by its length (0 when zero).           */
+/* normalize(v) = v / sqrt((x*x + y*y) + z*z); cross(a, b) =                  */
+/* (a.y*b.z - b.y*a.z, a.z*b.x - b.z*a.x, a.x*b.y - b.x*a.y).                  */
+/* ------------------------------------------------------------------------ */
+#define NORMALS_BODY(T, SQRT)                                                                   \
+    if (V == 0 || nI < 3) return 0;                                                             \
+    const int64_t nT = nI / 3;                                                                  \
+    const int K = iters;                                                                        \
+    const int steps = K > 0 ? K : 1;                                                            \
+    const int max_norms = (K / 2 - 3 < 10) ? K / 2 - 3 : 10;                                    \
+    uint32_t* val = (uint32_t*)calloc((size_t)V + 1, sizeof(uint32_t));                         \
+    uint32_t* off = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(V + 1));                      \
+    uint32_t* nxt = (uint32_t*)calloc((size_t)V + 1, sizeof(uint32_t));                         \
+    uint32_t* adj = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(3 * nT + 1));                 \
+    T* cen = (T*)malloc(sizeof(T) * (size_t)(3 * nT + 3));                                      \
+    T* fn = (T*)malloc(sizeof(T) * (size_t)(3 * nT + 3));                                       \
+    T* vn = (T*)calloc((size_t)(3 * V + 3), sizeof(T));                                         \
+    for (int64_t i = 0; i < 3 * nT; i++) val[idx[i]]++;                                         \
+    off[0] = 0;                                                                                 \
+    for (int64_t v = 0; v < V; v++) off[v + 1] = off[v] + val[v];                               \
+    for (int64_t t = 0; t < nT; t++)                                                            \
+        for (int k = 0; k < 3; k++) {                                                           \
+            const uint32_t v = idx[3 * t + k];                                                  \
+            adj[off[v] + nxt[v]++] = (uint32_t)t;                                               \
+        }                                                                                       \
+    for (int m = 0; m < steps; m++) {                                                           \
+        for (int64_t t = 0; t < nT; t++) {                                                      \
+            const T* a = pos + 3 * idx[3 * t];                                                  \
+            const T* b = pos + 3 * idx[3 * t + 1];                                              \
+            const T* c = pos + 3 * idx[3 * t + 2];                                              \
+            for (int k = 0; k < 3; k++) {                                                       \
+                T acc = (T)0;                                                                   \
+                acc = acc + a[k];                                                               \
+                acc = acc + b[k];                                                               \
+                acc = acc + c[k];                                                               \
+                cen[3 * t + k] = acc / (T)3;                                                    \
+            }                                                                                   \
+            if (m == 0 || m < max_norms || m < 3) {                                             \
+                T u[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};                               \
+                T w[3] = {a[0] - c[0], a[1] - c[1], a[2] - c[2]};                               \
+                const T lu = SQRT((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);                   \
+                const T lw = SQRT((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);                   \
+                for (int k = 0; k < 3; k++) {                                                   \
+                    u[k] = u[k] / lu;                                                           \
+                    w[k] = w[k] / lw;                                                           \
+                }                                                                               \
+                fn[3 * t + 0] = -(u[1] * w[2] - w[1] * u[2]);                                   \
+                fn[3 * t + 1] = -(u[2] * w[0] - w[2] * u[0]);                                   \
+                fn[3 * t + 2] = -(u[0] * w[1] - w[0] * u[1]);                                   \
+            } else {                                                                            \
+                for (int k = 0; k < 3; k++) {                                                   \
+                    T s = (T)0;                                                                 \
+                    s = s + vn[3 * idx[3 * t] + k];                                             \
+                    s = s + vn[3 * idx[3 * t + 1] + k];                                         \
+                    s = s + vn[3 * idx[3 * t + 2] + k];                                         \
+                    fn[3 * t + k] = s;                                                          \
+                }                                                                               \
+            }                                                                                   \
+        }                                                                                       \
+        const int last = m == steps - 1;                                                        \
+        const int unit = !last && (m == 3 || (m == 0 && K <= 3));                               \
+        for (int64_t v = 0; v < V; v++) {                                                       \
+            if (nxt[v] == 0) continue;                                                          \
+            T ap[3] = {(T)0, (T)0, (T)0}, an[3] = {(T)0, (T)0, (T)0};                           \
+            for (uint32_t q = off[v]; q < off[v] + nxt[v]; q++)                                 \
+                for (int k = 0; k < 3; k++) {                                                   \
+                    ap[k] = ap[k] + cen[3 * adj[q] + k];                                        \
+                    an[k] = an[k] + fn[3 * adj[q] + k];                                         \
+                }                                                                               \
+            for (int k = 0; k < 3; k++) an[k] = an[k] / (T)nxt[v];                              \
+            if (unit) {                                                                         \
+                const T l = SQRT((an[0] * an[0] + an[1] * an[1]) + an[2] * an[2]);              \
+                for (int k = 0; k < 3; k++) an[k] = an[k] / l;                                  \
+            }                                                                                   \
+            if (an[1] != (T)0)                                                                  \
+                for (int k = 0; k < 3; k++) vn[3 * v + k] = an[k];                              \
+            if (K > 0 && (boundary_processing || !boundary[v]))                                 \
+                for (int k = 0; k < 3; k++) pos[3 * v + k] = ap[k] / (T)nxt[v];                 \
+        }                                                                                       \
+    }                                                                                           \
+    for (int64_t v = 0; v < V; v++) {                                                           \
+        const T* n = vn + 3 * v;                                                                \
+        const T l = SQRT((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);                            \
+        for (int k = 0; k < 3; k++) normals[3 * v + k] = l > (T)0 ? n[k] / l : (T)0;            \
+    }                                                                                           \
+    free(val); free(off); free(nxt); free(adj); free(cen); free(fn); free(vn);                  \
+    return 0;
+
+int or_smooth_normals_f32(float* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int iters,
+                          int boundary_processing, float* normals)
+{
+    NORMALS_BODY(float, sqrtf)
+}
+
+int or_smooth_normals_f64(double* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int iters,
+                          int boundary_processing, double* normals)
+{
+    NORMALS_BODY(double, sqrt)
+}

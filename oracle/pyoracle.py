@@ -115,6 +115,8 @@ def lib():
         L.or_smooth_n_f64.argtypes = [f64p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int, C.c_int]
         L.or_dmc_positions_f64.argtypes = [P(Sampler), f64p, C.c_int, f64p, C.c_double, C.c_int, C.c_double, u32p,
                                            C.c_int64, f64p, f64p]
+        L.or_smooth_normals_f32.argtypes = [f32p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int, f32p]
+        L.or_smooth_normals_f64.argtypes = [f64p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int, f64p]
         topo = [C.c_int, u32p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int]
         L.or_place_f64.argtypes = [f64p, C.c_int, u32p, C.c_int64, f64p]
         L.or_forward_f64.argtypes = [f64p] + topo + [f64p]
@@ -417,3 +419,16 @@ def dmc_positions_f64(dens64, code, spec=None, opos=(0.0, 0.0, 0.0), delta=1.0, 
                                _ptr(o, C.c_double), delta, placement, tol, _ptr(cd, C.c_uint32), len(cd),
                                _ptr(pos, C.c_double), _ptr(err, C.c_double))
     return pos, err
+
+
+def smooth_normals(pos, boundary, idx, iters, boundary_processing=False, f64=False):
+    """Smoothed positions and unit vertex normals of a triangle mesh."""
+    dt, ct = (np.float64, C.c_double) if f64 else (np.float32, C.c_float)
+    p = np.ascontiguousarray(pos, dt).copy()
+    b = np.ascontiguousarray(boundary, np.uint8)
+    i = np.ascontiguousarray(idx, np.uint32)
+    n = np.zeros_like(p)
+    fn = lib().or_smooth_normals_f64 if f64 else lib().or_smooth_normals_f32
+    fn(_ptr(p, ct), _ptr(b, C.c_uint8), len(b), _ptr(i, C.c_uint32), len(i), iters, int(boundary_processing),
+       _ptr(n, ct))
+    return p, n
