@@ -42,6 +42,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
     ap.add_argument("--lod-steps", type=int, default=10,
                     help="timed batched steps of the uneven-LOD leg (1,536 chunks); 0 skips it")
+    ap.add_argument("--hermite-steps", type=int, default=5,
+                    help="timed batched steps of the CSG quad/QEF leg (64 chunks of 64^3 cells); 0 skips it")
+    ap.add_argument("--single-steps", type=int, default=20,
+                    help="timed plain extractions of the single 64^3-cell chunk; 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=5,
                     help="timed steps of the forward+loss+backward leg (one 128^3-cell chunk, K=8); 0 skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -109,6 +113,73 @@ def lod_leg(capi, workloads, steps):
             "cells_per_s": round(wl.cells / wall, 1), "plain_chunks_per_s": round(n / plain, 2),
             "batched_over_plain": round(plain / wall, 2), "sampled_chunks": sampled, "nonempty_chunks": nonempty,
             "nonempty_fraction": round(nonempty / n, 4), "tuning": tuned, "autotune_s": round(t_tune, 3)}
+
+
+def hermite_leg(capi, workloads, steps):
+    """64 chunks of 64^3 cells tiled 4x4x4 over [0,256)^3 of the CSG union of
+    200 spheres and 100 boxes (seed 42); dual-marching-cubes quads with FP32
+    QEF placement (tolerance 0.1), batched."""
+    wl = workloads.csg_scene()
+    cd = workloads.to_capi(wl.chunks)
+    n = len(cd)
+    arr = capi.chunk_array(cd)
+    with capi.Context(max_chunks=n, max_samples=n * 65 ** 3, max_vertices=1 << 22, max_indices=1 << 24) as ctx:
+        ctx.set_sampler(**wl.sampler)
+        ctx.set_mesh_options(mesh=capi.MESH_QUADS, placement=capi.PLACE_QEF, qef_tolerance=0.1)
+        for _ in range(2):
+            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        ctx.sync()
+        wall = (time.perf_counter() - t0) / steps
+        rc, res, _, _ = ctx.results(n)
+        capi.check(rc, "bmf_result")
+        V = sum(r.vertex_count for r in res)
+        Q = sum(r.index_count for r in res) // 4
+        err = np.concatenate([ctx.qef_errors(j) for j in range(n) if res[j].vertex_count] or [np.zeros(1)])
+        stages = ctx.profile_stages(cd)
+    log(f"[bench] hermite/QEF: {n / wall:.1f} chunks/s ({1e3 * wall:.3f} ms/step), V={V} quads={Q}, "
+        f"stages {', '.join(f'{k}={v:.3f}' for k, v in stages.items())}")
+    return {"workload": "csg-64x64: 4x4x4 chunks of 64^3 cells, 200 spheres + 100 boxes (seed 42), DMC quads, QEF tol 0.1",
+            "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4), "vertices": V, "quads": Q,
+            "qef_residual_median": float(np.median(err)), "qef_residual_max": float(err.max()),
+            "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
+
+
+def single_leg(capi, workloads, steps, with_cpu):
+    """One 64^3-cell chunk (origin 0, unit spacing, K=4) through the plain
+    path, against one CPU thread of the oracle."""
+    wl = workloads.single_chunk(iters=4)
+    cd = workloads.to_capi(wl.chunks)
+    arr = capi.chunk_array(cd)
+    with capi.Context(max_chunks=1, max_samples=65 ** 3, max_vertices=1 << 20, max_indices=1 << 22) as ctx:
+        ctx.set_sampler(**wl.sampler)
+        for _ in range(3):
+            capi.check(ctx.extract_raw(arr, 1, capi.MODE_PLAIN), "bmf_extract")
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            capi.check(ctx.extract_raw(arr, 1, capi.MODE_PLAIN), "bmf_extract")
+            ctx.sync()
+        gpu_ms = 1e3 * (time.perf_counter() - t0) / steps
+    out = {"workload": "one 64^3-cell fbm terrain chunk, K=4, plain path", "gpu_ms": round(gpu_ms, 4)}
+    if with_cpu:
+        from oracle import pyoracle
+
+        spec = pyoracle.SamplerSpec(**wl.sampler)
+        c = wl.chunks[0]
+        oc = pyoracle.make_chunk(c.pos, c.size, c.dim, c.overlap, c.iters, c.boundary_processing)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            pyoracle.extract_batch(spec, [oc], 1)
+            times.append(time.perf_counter() - t0)
+        out["cpu_ms_1_thread"] = round(1e3 * float(np.median(times)), 3)
+    log(f"[bench] single chunk: GPU plain {gpu_ms:.3f} ms" +
+        (f", CPU 1 thread {out['cpu_ms_1_thread']:.1f} ms" if with_cpu else ""))
+    return out
 
 
 def diff_leg(capi, roofline, workloads, steps):
@@ -262,6 +333,16 @@ def main():
     if args.lod_steps > 0:
         lod = lod_leg(capi, workloads, args.lod_steps)
 
+    # ---- Hermite / QEF: 64 CSG chunks, dual quads placed by FP32 QEF ---------------------
+    hermite = None
+    if args.hermite_steps > 0:
+        hermite = hermite_leg(capi, workloads, args.hermite_steps)
+
+    # ---- single chunk, plain path: latency against one CPU thread ----------------------
+    single = None
+    if args.single_steps > 0:
+        single = single_leg(capi, workloads, args.single_steps, not args.no_cpu)
+
     # ---- forward + loss + backward: one 128^3-cell chunk, K=8, 200k targets ---------
     diff = None
     if args.diff_steps > 0:
@@ -323,6 +404,8 @@ def main():
         "roofline": roof,
         "path_roofline": path,
         "uneven_lod": lod,
+        "hermite_qef": hermite,
+        "single_chunk": single,
         "fwd_bwd": diff,
         "cpu_baseline": cpu,
         "parity": parity,

@@ -38,7 +38,8 @@ void free_arenas(bmf_ctx* c) {
                     A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
                     A.pos,     A.pos4,      A.boundary, A.vedge,       A.idx,          A.ctr,
-                    A.sampler, c->d_spheres, c->d_boxes, A.ring, A.ring_n, A.qef_err, A.nrm4, A.fn4, A.nrm};
+                    A.sampler, c->d_spheres, c->d_boxes, A.ring, A.ring_n, A.qef_err, A.nrm4, A.fn4, A.nrm,
+                    A.csg_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     A = Arenas{};
@@ -253,12 +254,27 @@ int bmf_set_sampler(bmf_ctx* c, const bmf_sampler_desc* s) {
         }
         if (!(norm > 0.0f) || !finite(norm)) return BMF_EINVAL;
         d.inv_norm = 1.0f / norm;
+        // adaptive-octave bounds (k_sample decide_label)
+        double total = 0.0;
+        for (int o = 0; o < s->octaves; o++) total += std::fabs((double)d.ao[o]);
+        const double ulp_total = total * std::ldexp(1.0, -23);
+        for (int k = 0; k <= s->octaves; k++) {
+            double r = 0.0;
+            for (int o = k; o < s->octaves; o++) r += std::fabs((double)d.ao[o]);
+            r += (double)(s->octaves - k + 2) * ulp_total;
+            float rf = (float)r;
+            if ((double)rf < r) rf = std::nextafter(rf, INFINITY);
+            d.reach[k] = rf;
+        }
     }
     BMF_HIP(hipSetDevice(c->device));
     BMF_HIP(hipStreamSynchronize(c->stream));
     if (c->d_spheres) (void)hipFree(c->d_spheres);
     if (c->d_boxes) (void)hipFree(c->d_boxes);
+    if (c->A.csg_list) (void)hipFree(c->A.csg_list);
     c->d_spheres = c->d_boxes = nullptr;
+    c->A.csg_list = nullptr;
+    c->A.csg_stride = 0;
     if (s->kind == BMF_SAMPLER_CSG) {
         if (s->n_spheres < 0 || s->n_boxes < 0 || (s->n_spheres && !s->spheres) || (s->n_boxes && !s->boxes))
             return BMF_EINVAL;
@@ -270,12 +286,23 @@ int bmf_set_sampler(bmf_ctx* c, const bmf_sampler_desc* s) {
             BMF_HIP(hipMemcpy(c->d_spheres, s->spheres, sizeof(float) * 4 * s->n_spheres, hipMemcpyHostToDevice));
         if (s->n_boxes)
             BMF_HIP(hipMemcpy(c->d_boxes, s->boxes, sizeof(float) * 6 * s->n_boxes, hipMemcpyHostToDevice));
+        // per-chunk culled primitive lists (u16 ids; without them every
+        // sample evaluates every primitive)
+        const int64_t np = (int64_t)s->n_spheres + s->n_boxes;
+        if (np > 0 && s->n_spheres < 65536 && s->n_boxes < 65536) {
+            BMF_HIP(dalloc(&c->A.csg_list, np * c->A.cap_chunks));
+            c->A.csg_stride = (int32_t)np;
+        }
         d.spheres = c->d_spheres;
         d.boxes = c->d_boxes;
     }
     if (s->kind == BMF_SAMPLER_EXTERNAL) d.ext = s->densities;
     c->h_sampler = d;
     BMF_HIP(hipMemcpy(c->A.sampler, &d, sizeof(SamplerDev), hipMemcpyHostToDevice));
+    // captured frames hold the arena pointers (the CSG lists among them)
+    for (auto& g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
     c->sampler_set = true;
     return BMF_OK;
 }

@@ -107,11 +107,18 @@ __device__ __forceinline__ uint32_t wave_sum3(uint32_t c, uint64_t lt, uint32_t&
 // ------------------------------------------------------------------------
 // Gradient of the winning primitive of the max-union (first index on ties),
 // zero on the floor.
-__device__ void csg_gradient(const SamplerDev& S, float x, float y, float z, float g[3]) {
+// The primitives are the chunk's culled list (sphere ids, then box ids, in
+// order) when there is one: the others never exceed the floor in the chunk.
+__device__ void csg_gradient(const SamplerDev& S, const uint16_t* list, int ns, int nb, float x, float y, float z,
+                             float g[3]) {
     float best = kCsgFloor;
     g[0] = g[1] = g[2] = 0.0f;
-    for (int i = 0; i < S.n_spheres; i++) {
-        const float* sp = S.spheres + 4 * i;
+    if (!list) {
+        ns = S.n_spheres;
+        nb = S.n_boxes;
+    }
+    for (int k = 0; k < ns; k++) {
+        const float* sp = S.spheres + 4 * (list ? list[k] : k);
         const float dx = x - sp[0], dy = y - sp[1], dz = z - sp[2];
         const float l = sqrtf((dx * dx + dy * dy) + dz * dz);
         const float v = sp[3] - l;
@@ -123,8 +130,8 @@ __device__ void csg_gradient(const SamplerDev& S, float x, float y, float z, flo
             g[2] = -dz * inv;
         }
     }
-    for (int i = 0; i < S.n_boxes; i++) {
-        const float* b = S.boxes + 6 * i;
+    for (int k = 0; k < nb; k++) {
+        const float* b = S.boxes + 6 * (list ? list[ns + k] : k);
         const float px = x - b[0], py = y - b[1], pz = z - b[2];
         const float qx = fabsf(px) - b[3], qy = fabsf(py) - b[4], qz = fabsf(pz) - b[5];
         const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
@@ -391,7 +398,8 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
                             const float wx = q[i][0] * M.delta + M.opos[0], wy = q[i][1] * M.delta + M.opos[1],
                                         wz = q[i][2] * M.delta + M.opos[2];
                             float g[3];
-                            csg_gradient(*S, wx, wy, wz, g);
+                            csg_gradient(*S, A.csg_list ? A.csg_list + (int64_t)j * A.csg_stride : nullptr,
+                                         M.csg_ns, M.csg_nb, wx, wy, wz, g);
                             const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
                             for (int c = 0; c < 3; c++) nrm[i][c] = len > 0.0f ? g[c] / len : 0.0f;
                         }

@@ -35,6 +35,9 @@ struct SamplerDev {
     int32_t n_spheres, n_boxes;
     float fo[BMF_MAX_OCTAVES];  // per-octave frequency f0 * L^o (FP32 product chain)
     float ao[BMF_MAX_OCTAVES];  // per-octave amplitude g^o
+    // reach[k] >= the most the FP32 partial sum after k octaves can still
+    // move: sum of |a_o| for o >= k plus the rounding slack of those steps
+    float reach[BMF_MAX_OCTAVES + 1];
     const float* spheres;       // device, n_spheres x 4
     const float* boxes;         // device, n_boxes x 6
     const float* ext;           // EXTERNAL: caller's densities, chunks packed in descriptor order
@@ -59,6 +62,8 @@ struct ChunkMeta {
     int32_t V;          // vertices
     int32_t T;          // primitives (triangles, or quads in the quad mode)
     int32_t ipp;        // indices per primitive: 3 or 4
+    int32_t csg_ns;     // CSG: spheres and boxes that can reach above the floor in this chunk,
+    int32_t csg_nb;     //      listed in Arenas::csg_list (sphere ids, then box ids, in order)
     int64_t v_off;      // output offsets
     int64_t i_off;
 };
@@ -113,6 +118,8 @@ struct Arenas {
     int32_t cap_chunks;
     int64_t ext_base;          // EXTERNAL: offset of this launch's first chunk in the caller's densities
     int32_t sample_tile;       // samples per sampling work-queue tile: 1024, 2048 or 4096
+    uint16_t* csg_list;        // CSG: [chunk][csg_stride] primitives per chunk (k_plan), or null
+    int32_t csg_stride;        // n_spheres + n_boxes
     // mesh options (BMF_MESH_*, BMF_PLACE_*): the quad mode's extra arenas
     int32_t mesh_mode;
     int32_t placement;

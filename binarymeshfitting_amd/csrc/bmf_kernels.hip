@@ -61,6 +61,36 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
             if (ylo - S->y0 > a) m.state = kStateAir;
             else if (yhi - S->y0 < -a) m.state = kStateSolid;
         }
+        m.csg_ns = m.csg_nb = 0;
+        if (S->kind == BMF_SAMPLER_CSG && A.csg_list) {
+            // primitives whose value can exceed the floor inside the chunk's
+            // sample box: spheres within r + |floor| of it, boxes whose box
+            // grown by |floor| meets it (0.5 of slack covers FP32 rounding)
+            const float lo[3] = {m.opos[0], m.opos[1], m.opos[2]};
+            const float ext = (float)(d.dim - 1) * m.delta;
+            const float reach = -kCsgFloor + 0.5f;
+            uint16_t* L = A.csg_list + (int64_t)j * A.csg_stride;
+            int k = 0;
+            for (int i = 0; i < S->n_spheres; i++) {
+                const float* sp = S->spheres + 4 * i;
+                float dd = 0.0f;
+                for (int a = 0; a < 3; a++) {
+                    const float t = fmaxf(fmaxf(lo[a] - sp[a], sp[a] - (lo[a] + ext)), 0.0f);
+                    dd += t * t;
+                }
+                const float r = sp[3] + reach;
+                if (dd < r * r) L[k++] = (uint16_t)i;
+            }
+            m.csg_ns = k;
+            for (int i = 0; i < S->n_boxes; i++) {
+                const float* b = S->boxes + 6 * i;
+                bool hit = true;
+                for (int a = 0; a < 3; a++)
+                    hit = hit && (b[a] - b[3 + a] - reach <= lo[a] + ext) && (b[a] + b[3 + a] + reach >= lo[a]);
+                if (hit) L[k++] = (uint16_t)i;
+            }
+            m.csg_nb = k - m.csg_ns;
+        }
         const int64_t ns = (int64_t)d.dim * d.dim * d.dim;
         const bool sampled = m.state == kStateSampled;
         m.lab_words = sampled ? (ns + 63) / 64 : 0;
@@ -156,7 +186,11 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
 // FP32 partial sum S_k bounds the final sum: each remaining step is
 // fma(a_o, n_o, S) with |n_o| <= 1 and rounding is monotone, so
 // S_O in [L, H] with L = (..(S_k - |a_k|) - ..), H = (..(S_k + |a_k|) + ..)
-// evaluated in FP32; the density is monotone in S, so d(L), d(H) bound the
+// evaluated in FP32; every value involved is at most A = sum |a_o| in
+// magnitude, so L and H differ from the exact S_k -/+ sum_{o>=k} |a_o| by at
+// most (O - k) half-ulps of A, and reach[k] = that sum + (O - k + 2) ulp(A)
+// (rounded up, host side) gives FP32 bounds S_k -/+ reach[k] outside [L, H].
+// The density is monotone in S, so d(S_k - reach), d(S_k + reach) bound the
 // final density and decide the label once they agree in sign.  Undecided
 // samples are compacted into an LDS queue (wave ballot + popcount) so each
 // octave runs on dense waves.  Samples that run all octaves get their exact
@@ -172,14 +206,12 @@ __device__ __forceinline__ float density_from_sum(const SamplerDev& S, float sum
     return S.amp * fb - t2;
 }
 
-// 0: undecided, 1: certainly air (d < 0), 2: certainly solid.
+// 0: undecided, 1: certainly air (d < 0), 2: certainly solid.  The final
+// FP32 sum lies in [sum - reach[k], sum + reach[k]] (see SamplerDev::reach:
+// the interval the remaining fma steps can reach, widened by their rounding
+// and that of the two operations here, so both ends are safe bounds).
 __device__ __forceinline__ int decide_label(const SamplerDev& S, int k, float sum, float t2) {
-    float lo = sum, hi = sum;
-    for (int o = k; o < S.octaves; o++) {
-        const float a = fabsf(S.ao[o]);
-        lo = lo - a;
-        hi = hi + a;
-    }
+    const float lo = sum - S.reach[k], hi = sum + S.reach[k];
     const float d0 = density_from_sum(S, lo, t2), d1 = density_from_sum(S, hi, t2);
     if (fmaxf(d0, d1) < 0.0f) return 1;
     if (!(fminf(d0, d1) < 0.0f)) return 2;
@@ -245,8 +277,12 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int 
                     done = dec != 0;
                     air = dec == 1;
                 } else {
-                    const float d = S->kind == BMF_SAMPLER_EXTERNAL ? S->ext[A.ext_base + M.dens_off + i]
-                                                                    : sample_density(*S, px, py, pz);
+                    float d;
+                    if (S->kind == BMF_SAMPLER_EXTERNAL) d = S->ext[A.ext_base + M.dens_off + i];
+                    else if (S->kind == BMF_SAMPLER_CSG && A.csg_list)
+                        d = csg_density_list(*S, A.csg_list + (int64_t)j * A.csg_stride, M.csg_ns, M.csg_nb, px, py,
+                                             pz);
+                    else d = sample_density(*S, px, py, pz);
                     dens[i] = d;
                     air = d < 0.0f;
                 }

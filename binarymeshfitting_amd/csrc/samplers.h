@@ -161,6 +161,30 @@ __device__ __forceinline__ float csg_density(const SamplerDev& S, float x, float
     return best;
 }
 
+// The same union over a chunk's culled primitive list: sphere ids, then box
+// ids, in their original order.  Primitives left out cannot exceed the floor
+// anywhere in the chunk, so the value is identical.
+__device__ __forceinline__ float csg_density_list(const SamplerDev& S, const uint16_t* __restrict__ list, int ns,
+                                                  int nb, float x, float y, float z) {
+    float best = kCsgFloor;
+    for (int k = 0; k < ns; k++) {
+        const float* sp = S.spheres + 4 * list[k];
+        float dx = x - sp[0], dy = y - sp[1], dz = z - sp[2];
+        float l = sqrtf((dx * dx + dy * dy) + dz * dz);
+        best = fmaxf(best, sp[3] - l);
+    }
+    for (int k = 0; k < nb; k++) {
+        const float* b = S.boxes + 6 * list[ns + k];
+        float px = x - b[0], py = y - b[1], pz = z - b[2];
+        float qx = fabsf(px) - b[3], qy = fabsf(py) - b[4], qz = fabsf(pz) - b[5];
+        float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+        float outside = sqrtf((mx * mx + my * my) + mz * mz);
+        float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+        best = fmaxf(best, -(outside + inside));
+    }
+    return best;
+}
+
 __device__ __forceinline__ float sample_density(const SamplerDev& S, float x, float y, float z) {
     if (S.kind == BMF_SAMPLER_TERRAIN) return terrain_density(S, x, y, z);
     if (S.kind == BMF_SAMPLER_CSG) return csg_density(S, x, y, z);
