@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores (at most 16)")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed passes of the CPU baseline (median)")
     ap.add_argument("--lod-steps", type=int, default=10,
                     help="timed batched steps of the uneven-LOD leg (1,536 chunks); 0 skips it")
     ap.add_argument("--hermite-steps", type=int, default=5,
@@ -52,6 +53,20 @@ def parse():
                     help="only run a plain device copy of MIB MiB (3 launches) and exit; its FETCH_SIZE and "
                          "WRITE_SIZE readings scale the counters of the extraction kernels to HBM bytes")
     return ap.parse_args()
+
+
+def host_cpu():
+    """CPU model and core counts of the host (the CPU baseline's hardware)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff}
 
 
 def copy_calibration(mib):
@@ -358,14 +373,22 @@ def main():
         spec = pyoracle.SamplerSpec(**wl.sampler)
         ochunks = [pyoracle.make_chunk(c.pos, c.size, c.dim, c.overlap, c.iters, c.boundary_processing)
                    for c in wl.chunks]
-        log(f"[bench] CPU oracle on all {n} chunks with {threads} threads")
-        t0 = time.perf_counter()
+        runs = max(1, args.cpu_runs)
+        log(f"[bench] CPU oracle on all {n} chunks with {threads} threads: 1 warm-up + {runs} timed passes")
         oV, onI, ocm, ock = pyoracle.extract_batch(spec, ochunks, threads)
-        cpu_s = time.perf_counter() - t0
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            pyoracle.extract_batch(spec, ochunks, threads)
+            times.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(times))
         cpu = {"value": round(n / cpu_s, 3), "unit": "chunks/s", "cores": threads, "kind": "port",
-               "sample": f"all {n} chunks of the workload, one pass ({cpu_s:.2f} s wall)"}
+               "sample": f"all {n} chunks of the workload; median of {runs} passes after a warm-up "
+                         f"({cpu_s:.2f} s wall per pass)",
+               "host": host_cpu()}
         log(f"[bench] CPU: {n / cpu_s:.2f} chunks/s")
-        # the last batched step's meshes vs the CPU run
+        # the batched path's meshes vs the CPU run
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
         rc, res, _, _ = ctx.results(n)
         bad = 0
         for j in range(n):

@@ -11,6 +11,27 @@ namespace bmf {
 // entry count, nibble s the local edge of slot s.
 static __constant__ uint64_t c_tri_table[256] = BMF_TRI_TABLE_INIT;
 
+// For each mask and local edge, bit q set when triangle q of the cell (table
+// slots 3q..3q+2) uses the edge: the cell's part of a vertex's ring, in
+// ascending triangle id.  Derived from the triangle table at compile time.
+struct EdgeTriTable {
+    uint8_t m[256][12];
+};
+constexpr uint64_t kTriRows[256] = BMF_TRI_TABLE_INIT;
+constexpr EdgeTriTable make_edge_tris() {
+    EdgeTriTable t{};
+    for (int m = 0; m < 256; m++) {
+        const uint64_t row = kTriRows[m];
+        const int n = (int)(row >> 60);
+        for (int s = 0; s < n; s++) {
+            const int e = (int)((row >> (4 * s)) & 15u);
+            t.m[m][e] = (uint8_t)(t.m[m][e] | (1u << (s / 3)));
+        }
+    }
+    return t;
+}
+static __constant__ EdgeTriTable c_edge_tris = make_edge_tris();
+
 // Dual-marching-cubes polygons of each mask (Tables.hpp:96-354, EdgeTable up
 // to its end marker): bits 60..63 nibble count, nibbles = local edges with
 // 0xF between polygons.

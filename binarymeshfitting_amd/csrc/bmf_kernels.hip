@@ -399,9 +399,19 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
     const RowBits<W> r01 = lds_row<W>(s_lab[1], dim, y + 1, y1);
     const RowBits<W> r10 = lds_row<W>(s_lab[2], dim, y, x1);
     const RowBits<W> r11 = lds_row<W>(s_lab[2], dim, y + 1, x1 && y1);
+    const RowBits<W> rxm = lds_row<W>(s_lab[0], dim, y, x >= 1);
+    const RowBits<W> rym = lds_row<W>(s_lab[1], dim, y - 1, y >= 1);
+    nv = 0;
+    nt = 0;
+    // a row inside one uniform region (the row, and its neighbours at
+    // x-1, x+1, y-1, y+1, x+1/y+1, all one constant label) has nothing
+    if ((row_equal<W>(r00, zero_row<W>()) || row_equal<W>(r00, full_row<W>(dim))) &&
+        (!y1 || row_equal<W>(r01, r00)) && (!x1 || row_equal<W>(r10, r00)) &&
+        (!(x1 && y1) || row_equal<W>(r11, r00)) && (x < 1 || row_equal<W>(rxm, r00)) &&
+        (y < 1 || row_equal<W>(rym, r00)))
+        return;
     const RowBits<W> ex = rxor<W>(r00, r10, x1), ey = rxor<W>(r00, r01, y1), ez = zedges<W>(r00, dim);
     nv = total_pop<W>(ex) + total_pop<W>(ey) + total_pop<W>(ez);
-    nt = 0;
     if (x < dim - 1 && y < dim - 1 && !quad_uniform<W>(r00, r01, r10, r11, dim)) {
         for (int z0 = 0; z0 < dim - 1; z0 += 64) {
             const int z = z0 + lane;
@@ -411,8 +421,6 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
         }
     }
     // endpoints of the sign-changing edges incident to the row's samples
-    const RowBits<W> rxm = lds_row<W>(s_lab[0], dim, y, x >= 1);
-    const RowBits<W> rym = lds_row<W>(s_lab[1], dim, y - 1, y >= 1);
     RowBits<W> ep = ror<W>(ror<W>(ex, ey), ror<W>(ez, rshl1<W>(ez)));
     ep = ror<W>(ep, ror<W>(rxor<W>(r00, rxm, x >= 1), rxor<W>(r00, rym, y >= 1)));
     if (total_pop<W>(ep) == 0) return;
@@ -887,9 +895,6 @@ __global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int m) {
 }
 
 __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
-    __shared__ uint64_t s_tri[256];
-    for (int q = threadIdx.x; q < 256; q += blockDim.x) s_tri[q] = c_tri_table[q];
-    __syncthreads();
     const int nsm = A.ctr->n_smooth;
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_v[xcd + 1];
@@ -917,11 +922,10 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
             const int64_t ci = ((int64_t)cx * dim + cy) * dim + cz;
             const uint32_t le = (uint32_t)c_ring[ax][c][3];
             const uint32_t tb = CT[ci];
-            const uint64_t row = s_tri[CM[ci]];
-            const int ns = (int)(row >> 60);
-            for (int s = 0; s < ns; s++) {
-                if (((row >> (4 * s)) & 15ull) != le) continue;
-                const int64_t t = (int64_t)tb + s / 3;
+            uint32_t tm = c_edge_tris.m[CM[ci]][le];
+            while (tm) {
+                const int64_t t = (int64_t)tb + (__ffs(tm) - 1);
+                tm &= tm - 1u;
                 if (t >= T) {
                     A.ctr->error = 7;
                     break;
@@ -1020,9 +1024,6 @@ __global__ void __launch_bounds__(kSmoothThreads) k_dual_nrm(Arenas A, int m) {
 }
 
 __global__ void __launch_bounds__(kSmoothThreads) k_primal_nrm(Arenas A, int m) {
-    __shared__ uint64_t s_tri[256];
-    for (int q = threadIdx.x; q < 256; q += blockDim.x) s_tri[q] = c_tri_table[q];
-    __syncthreads();
     const int nsm = A.ctr->n_smooth;
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_v[xcd + 1];
@@ -1051,11 +1052,10 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_nrm(Arenas A, int m) 
             const int64_t ci = ((int64_t)cx * dim + cy) * dim + cz;
             const uint32_t le = (uint32_t)c_ring[ax][c][3];
             const uint32_t tb = CT[ci];
-            const uint64_t row = s_tri[CM[ci]];
-            const int ns = (int)(row >> 60);
-            for (int s = 0; s < ns; s++) {
-                if (((row >> (4 * s)) & 15ull) != le) continue;
-                const int64_t t = (int64_t)tb + s / 3;
+            uint32_t tm = c_edge_tris.m[CM[ci]][le];
+            while (tm) {
+                const int64_t t = (int64_t)tb + (__ffs(tm) - 1);
+                tm &= tm - 1u;
                 if (t >= T) {
                     A.ctr->error = 7;
                     break;

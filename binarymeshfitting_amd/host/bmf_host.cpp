@@ -52,7 +52,13 @@ int ChunkGenerator::create_context(int64_t max_samples, int64_t max_vertices, in
     max_samples_ = max_samples;
     max_vertices_ = max_vertices;
     max_indices_ = max_indices;
-    return bmf_set_sampler(ctx_, &sampler_.desc);
+    rc = bmf_set_sampler(ctx_, &sampler_.desc);
+    if (rc) return rc;
+    bmf_mesh_options o{};
+    o.mesh = BMF_MESH_TRIANGLES;
+    o.qef_tolerance = 0.1f;
+    o.normals = properties.smooth_normals ? 1 : 0;
+    return bmf_set_mesh_options(ctx_, &o);
 }
 
 int ChunkGenerator::init(const WorldProperties& props, const Sampler& sampler, int device, int max_chunks) {
@@ -101,7 +107,7 @@ int ChunkGenerator::process_queue(std::vector<WorldOctreeNode*>& batch) {
         rc = run(batch, &need_v, &need_i);
     }
     if (rc) return rc;
-    std::vector<float> p;
+    std::vector<float> p, nrm;
     std::vector<uint8_t> b;
     for (int i = 0; i < n; i++) {
         DMCChunk* c = batch[i]->chunk;
@@ -128,6 +134,11 @@ int ChunkGenerator::process_queue(std::vector<WorldOctreeNode*>& batch) {
                 rc = bmf_mesh_to_host(ctx_, i, p.data(), c->vi->mesh_indexes.data(), b.data(), r.vertex_count,
                                       r.index_count, &nv, &ni);
                 if (rc) return rc;
+                if (properties.smooth_normals) {
+                    nrm.resize(3 * (size_t)r.vertex_count);
+                    rc = bmf_normals_to_host(ctx_, i, nrm.data(), r.vertex_count);
+                    if (rc) return rc;
+                }
                 c->vi->vertices.resize(r.vertex_count);
                 for (int v = 0; v < r.vertex_count; v++) {  // DMCChunk::calculate_dual_vertex (DMCChunk.cpp:676-689)
                     DualVertex& dv = c->vi->vertices[v];
@@ -135,6 +146,7 @@ int ChunkGenerator::process_queue(std::vector<WorldOctreeNode*>& batch) {
                     dv.p = vec3(p[3 * v], p[3 * v + 1], p[3 * v + 2]);
                     dv.color = vec3(1, 1, 1);
                     dv.boundary = b[v] != 0;
+                    if (properties.smooth_normals) dv.n = vec3(nrm[3 * v], nrm[3 * v + 1], nrm[3 * v + 2]);
                 }
             }
         }

@@ -89,6 +89,9 @@ struct WorldProperties {
     int chunk_resolution = 32;
     float overlap = 0.035f;
     bool boundary_processing = false;
+    // The reference's SMOOTH_NORMALS build switch (MeshProcessor<3>'s
+    // smooth_normals, off on its live path): fills DualVertex::n.
+    bool smooth_normals = false;
 };
 
 enum GenerationStage { GENERATION_STAGES_GENERATING = 0, GENERATION_STAGES_NEEDS_UPLOAD = 1, GENERATION_STAGES_DONE = 2 };

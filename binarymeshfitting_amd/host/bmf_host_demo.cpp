@@ -3,7 +3,8 @@
 // chunk: contains_mesh, vertex and index counts, and the mesh checksum
 // (sum over the u32 stream of position bit patterns then indices of
 // word_k * (2k + 1), times 0x9E3779B97F4A7C15, modulo 2^64).
-//   bmf_host_demo <n_side> <resolution> <iters> <mode: 0 plain, 1 batched>
+// With smooth_normals = 1 it also prints the same checksum over the normals.
+//   bmf_host_demo <n_side> <resolution> <iters> <mode: 0 plain, 1 batched> [smooth_normals]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -18,7 +19,9 @@ int main(int argc, char** argv) {
     const int res = argc > 2 ? std::atoi(argv[2]) : 33;
     const int iters = argc > 3 ? std::atoi(argv[3]) : 2;
     const int mode = argc > 4 ? std::atoi(argv[4]) : BMF_MODE_BATCHED;
+    const bool normals = argc > 5 && std::atoi(argv[5]) != 0;
     WorldProperties props;
+    props.smooth_normals = normals;
     props.chunk_resolution = res;
     props.process_iters = iters;
     Sampler sampler;
@@ -64,8 +67,20 @@ int main(int argc, char** argv) {
         }
         for (size_t q = 0; q < ni; q++, k++) h += (uint64_t)ch->vi->mesh_indexes[q] * (2 * k + 1);
         h *= 0x9E3779B97F4A7C15ull;
-        std::printf("{\"chunk\": %zu, \"contains_mesh\": %d, \"V\": %zu, \"I\": %zu, \"stage\": %d, \"checksum\": %llu}\n",
-                    c, (int)ch->contains_mesh, nv, ni, batch[c]->generation_stage, (unsigned long long)h);
+        uint64_t hn = 0, kn = 0;
+        for (size_t v = 0; v < nv; v++) {
+            const float* f = &ch->vi->vertices[v].n.x;
+            for (int q = 0; q < 3; q++, kn++) {
+                uint32_t b;
+                std::memcpy(&b, &f[q], 4);
+                hn += (uint64_t)b * (2 * kn + 1);
+            }
+        }
+        hn *= 0x9E3779B97F4A7C15ull;
+        std::printf("{\"chunk\": %zu, \"contains_mesh\": %d, \"V\": %zu, \"I\": %zu, \"stage\": %d, \"checksum\": %llu, "
+                    "\"normals\": %llu}\n",
+                    c, (int)ch->contains_mesh, nv, ni, batch[c]->generation_stage, (unsigned long long)h,
+                    (unsigned long long)hn);
     }
     return 0;
 }

@@ -33,3 +33,26 @@ def test_chunk_generator_process_queue(oracle, mode):
                 # ChunkGenerator.cpp:137-143: vi exists iff the chunk has the mesh flag
                 assert r["stage"] == (1 if ref["contains_mesh"] else 2)
                 c += 1
+
+
+def test_chunk_generator_smooth_normals(oracle):
+    """WorldProperties::smooth_normals fills DualVertex::n with the library's
+    normals; their checksum equals the oracle's."""
+    n_side, res, iters = 2, 33, 3
+    out = subprocess.run([str(build.HOST_DEMO), str(n_side), str(res), str(iters), "1", "1"], check=True,
+                         capture_output=True, text=True, timeout=120).stdout
+    rows = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
+    spec = oracle.SamplerSpec(**workloads.terrain_sampler())
+    size = float(res - 1)
+    c = 0
+    for i in range(n_side):
+        for j in range(n_side):
+            for k in range(n_side):
+                base = oracle.extract_chunk(spec, oracle.make_chunk((size * i, size * j, size * k), size, res, 0.0, 0, 0))
+                r = rows[c]
+                c += 1
+                if not base["V"]:
+                    continue
+                p, n = oracle.smooth_normals(base["pos"], base["boundary"], base["idx"], iters)
+                assert r["checksum"] == oracle.checksum_mesh(p, base["idx"])
+                assert r["normals"] == oracle.checksum_mesh(n, np.zeros(0, np.uint32))

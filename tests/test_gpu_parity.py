@@ -337,3 +337,24 @@ def test_set_tuning_rejects_bad_shapes(ctx):
         with pytest.raises(capi.BmfError) as e:
             ctx.set_tuning(**bad)
         assert e.value.code == capi.EINVAL
+
+
+def test_csg_culled_sampling_matches_full_union(ctx, oracle):
+    """The sampler evaluates only the primitives that can reach above the
+    floor in each chunk; densities and meshes equal the oracle's union over
+    all 300 primitives."""
+    wl = workloads.csg_scene()
+    ctx.set_sampler(**wl.sampler)
+    spec = _ospec(oracle, wl.sampler)
+    picks = [wl.chunks[k] for k in (0, 21, 42, 63)]
+    for c in picks:
+        d = ctx.sample_block(workloads.to_capi([c])[0])
+        delta, opos = oracle.geometry(_ochunk(oracle, c))
+        ref = oracle.sample_block(spec, c.dim, opos, delta)
+        assert np.array_equal(d, ref)
+    ctx.extract(workloads.to_capi(picks), capi.MODE_BATCHED)
+    rc, res, _, _ = ctx.results(len(picks))
+    assert rc == capi.OK
+    for j, c in enumerate(picks):
+        pos, idx, bnd = ctx.mesh(j)
+        _check_mesh(oracle, spec, c, pos, idx, bnd, res[j], tol=0.0)
