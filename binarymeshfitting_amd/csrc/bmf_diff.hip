@@ -45,6 +45,7 @@ constexpr int kThreads = 256;
 constexpr int kMaxRing = 20;
 constexpr int kLeaf = 8;       // targets per leaf box of the nearest-target hierarchy
 constexpr int kNNThreads = 128;
+constexpr int kNNLanes = 8;    // lanes per vertex in the nearest-target search
 constexpr int kNNStack = 32;   // per-lane traversal stack (tree height <= 28 for n < 2^31)
 
 #define BMF_HIP(call)                          \
@@ -278,22 +279,34 @@ __global__ void k_nn_bbox_init(int* bb) {
     else if (threadIdx.x < 6) bb[threadIdx.x] = (int)0x80000000;
 }
 
-__global__ void k_nn_bbox(const float* __restrict__ t, int64_t n, int* bb) {
+// Bounding box of the targets: per-thread, per-wave, per-block reductions,
+// then one atomic per block and coordinate (min/max are order-independent).
+__global__ void __launch_bounds__(kThreads) k_nn_bbox(const float* __restrict__ t, int64_t n, int* bb) {
+    __shared__ float s_r[6][kThreads / 64];
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         for (int a = 0; a < 3; a++) {
             lo[a] = fminf(lo[a], t[3 * i + a]);
             hi[a] = fmaxf(hi[a], t[3 * i + a]);
         }
+    const int wave = threadIdx.x >> 6;
     for (int a = 0; a < 3; a++) {
         for (int off = 32; off > 0; off >>= 1) {
             lo[a] = fminf(lo[a], __shfl_xor(lo[a], off));
             hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off));
         }
         if ((threadIdx.x & 63) == 0) {
-            atomicMin(&bb[a], float_order(lo[a]));
-            atomicMax(&bb[3 + a], float_order(hi[a]));
+            s_r[a][wave] = lo[a];
+            s_r[3 + a][wave] = hi[a];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        float v = s_r[a][0];
+        for (int w = 1; w < kThreads / 64; w++) v = a < 3 ? fminf(v, s_r[a][w]) : fmaxf(v, s_r[a][w]);
+        if (a < 3) atomicMin(&bb[a], float_order(v));
+        else atomicMax(&bb[a], float_order(v));
     }
 }
 
@@ -357,9 +370,18 @@ __global__ void k_nn_leaves(const float4* __restrict__ st, int64_t n, int64_t P,
     hi[P + l] = b;
 }
 
-// Inner boxes, level by level from the leaves up (one workgroup).
-__global__ void __launch_bounds__(1024) k_nn_inner(int64_t P, float4* __restrict__ lo, float4* __restrict__ hi) {
-    for (int64_t level = P >> 1; level >= 1; level >>= 1) {
+// Inner boxes of one wide level [level, 2 level).
+__global__ void k_nn_level(int64_t level, float4* __restrict__ lo, float4* __restrict__ hi) {
+    const int64_t i = level + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * level) return;
+    const float4 a = lo[2 * i], b = lo[2 * i + 1], c = hi[2 * i], d = hi[2 * i + 1];
+    lo[i] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+    hi[i] = make_float4(fmaxf(c.x, d.x), fmaxf(c.y, d.y), fmaxf(c.z, d.z), 0.0f);
+}
+
+// Inner boxes of the levels from `top` up to the root (one workgroup).
+__global__ void __launch_bounds__(1024) k_nn_inner(int64_t top, float4* __restrict__ lo, float4* __restrict__ hi) {
+    for (int64_t level = top; level >= 1; level >>= 1) {
         for (int64_t i = level + threadIdx.x; i < 2 * level; i += blockDim.x) {
             const float4 a = lo[2 * i], b = lo[2 * i + 1], c = hi[2 * i], d = hi[2 * i + 1];
             lo[i] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
@@ -385,9 +407,12 @@ __device__ __forceinline__ float box_lb2(float px, float py, float pz, float4 l,
     return (dx * dx + dy * dy) + dz * dz;
 }
 
-// One vertex per lane: depth-first traversal, nearer child first, a subtree
-// skipped when its box is strictly farther than the best distance so far
-// (an equally far target with a lower index is still found).
+// kNNLanes lanes per vertex (consecutive lanes of a wave): lane r walks the
+// subtree under heap node kNNLanes + r depth first, nearer child first; the
+// lanes of a vertex share their best distance every step, and a subtree is
+// skipped only when its box is strictly farther than that shared best (an
+// actual target distance), so an equally far target with a lower index is
+// still found.  The lanes' results are then merged by (distance, index).
 __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict__ pos, int V,
                                                          const float4* __restrict__ st, int64_t n, int64_t P,
                                                          const float4* __restrict__ lo, const float4* __restrict__ hi,
@@ -395,73 +420,101 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
                                                          double* __restrict__ d2) {
     __shared__ int32_t s_node[kNNStack][kNNThreads];
     __shared__ float s_lb[kNNStack][kNNThreads];
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    const float px = pos[3 * v], py = pos[3 * v + 1], pz = pos[3 * v + 2];
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = gid / kNNLanes, r = gid % kNNLanes;
+    const bool active = v < V;  // uniform across the lanes of a vertex
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (active) {
+        px = pos[3 * v];
+        py = pos[3 * v + 1];
+        pz = pos[3 * v + 2];
+    }
     float best = INFINITY;
     int bi = 0x7FFFFFFF;
     float4 bq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     int sp = 0;
-    int64_t node = 1;
+    // the subtree this lane walks (the whole tree on lane 0 when it is shallow)
+    int64_t node = !active ? 0 : (P >= kNNLanes ? kNNLanes + r : (r == 0 ? 1 : 0));
     for (;;) {
-        if (node < P) {
-            int64_t a = 2 * node, b = a + 1;
-            float la = box_lb2(px, py, pz, lo[a], hi[a]), lb = box_lb2(px, py, pz, lo[b], hi[b]);
-            if (lb < la) {
-                const int64_t t = a;
-                a = b;
-                b = t;
-                const float u = la;
-                la = lb;
-                lb = u;
+        bool work = node != 0;
+        if (work) {
+            if (node < P) {
+                int64_t a = 2 * node, b = a + 1;
+                float la = box_lb2(px, py, pz, lo[a], hi[a]), lb = box_lb2(px, py, pz, lo[b], hi[b]);
+                if (lb < la) {
+                    const int64_t t = a;
+                    a = b;
+                    b = t;
+                    const float u = la;
+                    la = lb;
+                    lb = u;
+                }
+                if (lb <= best && sp < kNNStack) {
+                    s_node[sp][threadIdx.x] = (int32_t)b;
+                    s_lb[sp][threadIdx.x] = lb;
+                    sp++;
+                }
+                node = la <= best ? a : 0;
+            } else {
+                const int64_t e = std::min<int64_t>(n, (node - P + 1) * kLeaf);
+                for (int64_t i = (node - P) * kLeaf; i < e; i++) {
+                    const float4 q = st[i];
+                    const float d = dist2(px, py, pz, q);
+                    const int id = __float_as_int(q.w);
+                    if (d < best || (d == best && id < bi)) {
+                        best = d;
+                        bi = id;
+                        bq = q;
+                    }
+                }
+                node = 0;
             }
-            if (lb <= best && sp < kNNStack) {
-                s_node[sp][threadIdx.x] = (int32_t)b;
-                s_lb[sp][threadIdx.x] = lb;
-                sp++;
-            }
-            if (la <= best) {
-                node = a;
-                continue;
-            }
-        } else {
-            const int64_t e = std::min<int64_t>(n, (node - P + 1) * kLeaf);
-            for (int64_t i = (node - P) * kLeaf; i < e; i++) {
-                const float4 q = st[i];
-                const float d = dist2(px, py, pz, q);
-                const int id = __float_as_int(q.w);
-                if (d < best || (d == best && id < bi)) {
-                    best = d;
-                    bi = id;
-                    bq = q;
+        }
+        // share the best distance among the lanes of this vertex
+#pragma unroll
+        for (int off = 1; off < kNNLanes; off <<= 1) best = fminf(best, __shfl_xor(best, off));
+        if (node == 0) {
+            while (sp > 0) {
+                sp--;
+                if (s_lb[sp][threadIdx.x] <= best) {
+                    node = s_node[sp][threadIdx.x];
+                    break;
                 }
             }
         }
-        node = 0;
-        while (sp > 0) {
-            sp--;
-            if (s_lb[sp][threadIdx.x] <= best) {
-                node = s_node[sp][threadIdx.x];
-                break;
-            }
-        }
-        if (node == 0) break;
+        if (!__any(node != 0)) break;
     }
+    // merge the lanes' candidates: lowest (distance, index)
+    float bx = bq.x, by = bq.y, bz = bq.z;
+#pragma unroll
+    for (int off = 1; off < kNNLanes; off <<= 1) {
+        const float od = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        const float ox = __shfl_xor(bx, off), oy = __shfl_xor(by, off), oz = __shfl_xor(bz, off);
+        if (od < best || (od == best && oi < bi)) {
+            best = od;
+            bi = oi;
+            bx = ox;
+            by = oy;
+            bz = oz;
+        }
+    }
+    if (!active || r != 0) return;
     nearest[v] = bi;
     const float sc = 2.0f / (float)V;
-    dldp[3 * v] = (px - bq.x) * sc;
-    dldp[3 * v + 1] = (py - bq.y) * sc;
-    dldp[3 * v + 2] = (pz - bq.z) * sc;
+    dldp[3 * v] = (px - bx) * sc;
+    dldp[3 * v + 1] = (py - by) * sc;
+    dldp[3 * v + 2] = (pz - bz) * sc;
     d2[v] = (double)best;
 }
 
-// Deterministic sum: each thread a contiguous slice, then a fixed tree.
+// Deterministic sum: thread t adds x[t], x[t + 1024], ... in order, then a
+// fixed tree.
 __global__ void __launch_bounds__(1024) k_sum(const double* __restrict__ x, int n, double* __restrict__ out) {
     __shared__ double s[1024];
     const int tid = threadIdx.x;
-    const int per = (n + 1023) / 1024;
     double a = 0.0;
-    for (int i = tid * per; i < std::min(n, (tid + 1) * per); i++) a += x[i];
+    for (int i = tid; i < n; i += 1024) a += x[i];
     s[tid] = a;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
@@ -788,7 +841,7 @@ int bmf_loss_nearest(bmf_tape* t, const float* targets, int64_t n, float* dldp, 
     if (rc) return rc;
     const int64_t P = leaf_pow2(n);
     hipLaunchKernelGGL(k_nn_bbox_init, dim3(1), dim3(64), 0, s, t->bb);
-    hipLaunchKernelGGL(k_nn_bbox, dim3((unsigned)std::min<int64_t>(1024, blocks_for(n))), dim3(kThreads), 0, s, targets,
+    hipLaunchKernelGGL(k_nn_bbox, dim3((unsigned)std::min<int64_t>(256, blocks_for(n))), dim3(kThreads), 0, s, targets,
                        n, t->bb);
     hipLaunchKernelGGL(k_nn_keys, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, n, t->bb, t->keys[0], t->vals[0]);
     BMF_HIP(hipGetLastError());
@@ -798,8 +851,12 @@ int bmf_loss_nearest(bmf_tape* t, const float* targets, int64_t n, float* dldp, 
     BMF_HIP(hipcub::DeviceRadixSort::SortPairs(t->cub_tmp, bytes, kb, vb, (int)n, 0, 30, s));
     hipLaunchKernelGGL(k_nn_gather, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, vb.Current(), n, t->st);
     hipLaunchKernelGGL(k_nn_leaves, dim3(blocks_for(P)), dim3(kThreads), 0, s, t->st, n, P, t->blo, t->bhi);
-    hipLaunchKernelGGL(k_nn_inner, dim3(1), dim3(1024), 0, s, P, t->blo, t->bhi);
-    hipLaunchKernelGGL(k_nn_query, dim3((unsigned)((t->V + kNNThreads - 1) / kNNThreads)), dim3(kNNThreads), 0, s,
+    int64_t level = P >> 1;
+    for (; level >= 2048; level >>= 1)
+        hipLaunchKernelGGL(k_nn_level, dim3(blocks_for(level)), dim3(kThreads), 0, s, level, t->blo, t->bhi);
+    if (level >= 1) hipLaunchKernelGGL(k_nn_inner, dim3(1), dim3(1024), 0, s, level, t->blo, t->bhi);
+    hipLaunchKernelGGL(k_nn_query, dim3((unsigned)(((int64_t)t->V * kNNLanes + kNNThreads - 1) / kNNThreads)),
+                       dim3(kNNThreads), 0, s,
                        t->pos, t->V, t->st, n, P, t->blo, t->bhi, nearest ? nearest : t->nearest_tmp, dldp, t->d2);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, t->d2, t->V, t->loss);
     BMF_HIP(hipGetLastError());

@@ -306,8 +306,71 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_count(Arenas A, int n) {
 }
 
 // ------------------------------------------------------------------------
-// vertices: one per polygon of each surface cell of the row
+// vertices: one per polygon of each surface cell.  Row pass (wave per row,
+// lane = z): per-cell first vertex id and mask, per vertex its cell and
+// polygon code and boundary flag.  The plane's vertices are a contiguous id
+// range, so a dense pass of the whole block then places them (mass point or
+// QEF), free of the row pass's idle lanes.
 // ------------------------------------------------------------------------
+__device__ void place_vertex(const Arenas& A, const ChunkMeta& M, const SamplerDev& S, const uint16_t* csg, int j,
+                             int x, uint32_t code, uint32_t mask, bool qef, float out[3], float& err) {
+    const int dim = M.dim;
+    const int y = (int)((code >> 8) & 255u), z = (int)((code >> 16) & 255u), p = (int)(code >> 24);
+    const float* __restrict__ D = A.dens + M.dens_off;
+    const uint64_t row = c_dmc_table[mask];
+    const int nn = (int)(row >> 60);
+    // skip to polygon p
+    int s = 0;
+    for (int q = 0; q < p && s < nn; s++)
+        if (((row >> (4 * s)) & 15ull) == 15ull) q++;
+    float q3[7][3];
+    int k = 0;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (; s < nn; s++) {
+        const int e = (int)((row >> (4 * s)) & 15ull);
+        if (e == 15) break;
+        const int x0 = x + c_le[e][0], y0 = y + c_le[e][1], z0 = z + c_le[e][2], a = c_le[e][3];
+        const int x1 = x0 + (a == 0), y1 = y0 + (a == 1), z1 = z0 + (a == 2);
+        const float s0 = D[((int64_t)x0 * dim + y0) * dim + z0];
+        const float s1 = D[((int64_t)x1 * dim + y1) * dim + z1];
+        // DMCChunk::calculate_isovertex (DMCChunk.cpp:657-674)
+        const float mu = (0.0f - s0) / (s1 - s0);
+        const float dx = (float)x1 - (float)x0, dy = (float)y1 - (float)y0, dz = (float)z1 - (float)z0;
+        if (k < 7) {
+            q3[k][0] = dx * mu + (float)x0;
+            q3[k][1] = dy * mu + (float)y0;
+            q3[k][2] = dz * mu + (float)z0;
+            ax = ax + q3[k][0];
+            ay = ay + q3[k][1];
+            az = az + q3[k][2];
+        }
+        k++;
+    }
+    err = 0.0f;
+    if (k == 0 || k > 7) {
+        A.ctr->error = 8;
+        out[0] = out[1] = out[2] = 0.0f;
+        return;
+    }
+    if (qef) {
+        float nrm[7][3];
+        for (int i = 0; i < k; i++) {
+            const float wx = q3[i][0] * M.delta + M.opos[0], wy = q3[i][1] * M.delta + M.opos[1],
+                        wz = q3[i][2] * M.delta + M.opos[2];
+            float g[3];
+            csg_gradient(S, csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
+            const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+            for (int c = 0; c < 3; c++) nrm[i][c] = len > 0.0f ? g[c] / len : 0.0f;
+        }
+        err = qef_solve(q3, nrm, k, A.qef_tol, out);
+    } else {
+        const float fk = (float)k;
+        out[0] = ax / fk;
+        out[1] = ay / fk;
+        out[2] = az / fk;
+    }
+}
+
 template <int W>
 __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
     __shared__ DmcTables s_t;
@@ -335,7 +398,7 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
         const uint32_t Vc = (uint32_t)M.V;
         const bool smooth = M.iters > 0 && M.T > 0;
         const bool qef = A.placement == BMF_PLACE_QEF && S->kind == BMF_SAMPLER_CSG;
-        const float* __restrict__ D = A.dens + M.dens_off;
+        // ---- row pass ----
         for (int y = wave; y < dim - 1; y += kRowThreads / 64) {
             const RowBits<W> r00 = lds_row<W>(s_planes[0], dim, y, true);
             const RowBits<W> r01 = lds_row<W>(s_planes[0], dim, y + 1, true);
@@ -355,79 +418,35 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
                 const int64_t ci = M.dens_off + ((int64_t)x * dim + y) * dim + z;
                 A.cell_tb[ci] = vb;
                 A.cell_mask[ci] = (uint8_t)mask;
-                const uint64_t row = c_dmc_table[mask];
-                const int nn = (int)(row >> 60);
-                int s = 0;
                 for (uint32_t p = 0; p < np; p++) {
                     const uint32_t v = vb + p;
-                    // edges of polygon p
-                    float q[7][3];
-                    int k = 0;
-                    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-                    for (; s < nn; s++) {
-                        const int e = (int)((row >> (4 * s)) & 15ull);
-                        if (e == 15) {
-                            s++;
-                            break;
-                        }
-                        const int x0 = x + c_le[e][0], y0 = y + c_le[e][1], z0e = z + c_le[e][2], a = c_le[e][3];
-                        const int x1 = x0 + (a == 0), y1 = y0 + (a == 1), z1 = z0e + (a == 2);
-                        const float s0 = D[((int64_t)x0 * dim + y0) * dim + z0e];
-                        const float s1 = D[((int64_t)x1 * dim + y1) * dim + z1];
-                        // DMCChunk::calculate_isovertex (DMCChunk.cpp:657-674)
-                        const float mu = (0.0f - s0) / (s1 - s0);
-                        const float dx = (float)x1 - (float)x0, dy = (float)y1 - (float)y0, dz = (float)z1 - (float)z0e;
-                        if (k < 7) {
-                            q[k][0] = dx * mu + (float)x0;
-                            q[k][1] = dy * mu + (float)y0;
-                            q[k][2] = dz * mu + (float)z0e;
-                            ax = ax + q[k][0];
-                            ay = ay + q[k][1];
-                            az = az + q[k][2];
-                        }
-                        k++;
-                    }
-                    if (v >= Vc || k > 7 || k == 0) {
+                    if (v >= Vc) {
                         A.ctr->error = 8;
-                        continue;
+                        break;
                     }
-                    float px, py, pz, err = 0.0f;
-                    if (qef) {
-                        float nrm[7][3];
-                        for (int i = 0; i < k; i++) {
-                            const float wx = q[i][0] * M.delta + M.opos[0], wy = q[i][1] * M.delta + M.opos[1],
-                                        wz = q[i][2] * M.delta + M.opos[2];
-                            float g[3];
-                            csg_gradient(*S, A.csg_list ? A.csg_list + (int64_t)j * A.csg_stride : nullptr,
-                                         M.csg_ns, M.csg_nb, wx, wy, wz, g);
-                            const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
-                            for (int c = 0; c < 3; c++) nrm[i][c] = len > 0.0f ? g[c] / len : 0.0f;
-                        }
-                        float o[3];
-                        err = qef_solve(q, nrm, k, A.qef_tol, o);
-                        px = o[0];
-                        py = o[1];
-                        pz = o[2];
-                    } else {
-                        const float fk = (float)k;
-                        px = ax / fk;
-                        py = ay / fk;
-                        pz = az / fk;
-                    }
-                    float* P = A.pos + (vo + v) * 3;
-                    P[0] = px;
-                    P[1] = py;
-                    P[2] = pz;
-                    if (smooth) {
-                        A.pos4[vo + v] = make_float4(px, py, pz, 0.0f);
-                        A.ring_n[vo + v] = 0u;
-                    }
-                    if (A.qef_err) A.qef_err[vo + v] = err;
                     A.boundary[vo + v] =
                         (uint8_t)(x == 0 || y == 0 || z == 0 || x + 1 == dim - 1 || y + 1 == dim - 1 || z + 1 == dim - 1);
                     A.vedge[vo + v] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | (p << 24);
+                    if (smooth) A.ring_n[vo + v] = 0u;
                 }
             }
+        }
+        __syncthreads();  // the plane's vertex codes are written
+        // ---- dense placement of the plane's vertices [rv[0], next plane's first) ----
+        const uint32_t v0 = rv[0], v1 = min(x + 1 < dim ? rv[dim] : Vc, Vc);
+        const uint16_t* csg = A.csg_list ? A.csg_list + (int64_t)j * A.csg_stride : nullptr;
+        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+            const uint32_t code = A.vedge[vo + v];
+            const int y = (int)((code >> 8) & 255u), z = (int)((code >> 16) & 255u);
+            const uint32_t mask = A.cell_mask[M.dens_off + ((int64_t)x * dim + y) * dim + z];
+            float o[3], err;
+            place_vertex(A, M, *S, csg, j, x, code, mask, qef, o, err);
+            float* P = A.pos + (vo + v) * 3;
+            P[0] = o[0];
+            P[1] = o[1];
+            P[2] = o[2];
+            if (smooth) A.pos4[vo + v] = make_float4(o[0], o[1], o[2], 0.0f);
+            if (A.qef_err) A.qef_err[vo + v] = err;
         }
     }
 }

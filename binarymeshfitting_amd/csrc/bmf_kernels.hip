@@ -61,36 +61,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
             if (ylo - S->y0 > a) m.state = kStateAir;
             else if (yhi - S->y0 < -a) m.state = kStateSolid;
         }
-        m.csg_ns = m.csg_nb = 0;
-        if (S->kind == BMF_SAMPLER_CSG && A.csg_list) {
-            // primitives whose value can exceed the floor inside the chunk's
-            // sample box: spheres within r + |floor| of it, boxes whose box
-            // grown by |floor| meets it (0.5 of slack covers FP32 rounding)
-            const float lo[3] = {m.opos[0], m.opos[1], m.opos[2]};
-            const float ext = (float)(d.dim - 1) * m.delta;
-            const float reach = -kCsgFloor + 0.5f;
-            uint16_t* L = A.csg_list + (int64_t)j * A.csg_stride;
-            int k = 0;
-            for (int i = 0; i < S->n_spheres; i++) {
-                const float* sp = S->spheres + 4 * i;
-                float dd = 0.0f;
-                for (int a = 0; a < 3; a++) {
-                    const float t = fmaxf(fmaxf(lo[a] - sp[a], sp[a] - (lo[a] + ext)), 0.0f);
-                    dd += t * t;
-                }
-                const float r = sp[3] + reach;
-                if (dd < r * r) L[k++] = (uint16_t)i;
-            }
-            m.csg_ns = k;
-            for (int i = 0; i < S->n_boxes; i++) {
-                const float* b = S->boxes + 6 * i;
-                bool hit = true;
-                for (int a = 0; a < 3; a++)
-                    hit = hit && (b[a] - b[3 + a] - reach <= lo[a] + ext) && (b[a] + b[3 + a] + reach >= lo[a]);
-                if (hit) L[k++] = (uint16_t)i;
-            }
-            m.csg_nb = k - m.csg_ns;
-        }
+        m.csg_ns = m.csg_nb = 0;  // k_csg_cull fills them
         const int64_t ns = (int64_t)d.dim * d.dim * d.dim;
         const bool sampled = m.state == kStateSampled;
         m.lab_words = sampled ? (ns + 63) / 64 : 0;
@@ -175,6 +146,68 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         c->total_planes = overflow ? 0 : tot_planes;
         c->n_smooth = 0;
         c->total_samples = tot_samples;
+    }
+}
+
+// ------------------------------------------------------------------------
+// CSG culling: per chunk, the primitives whose value can exceed the floor
+// inside its sample box (spheres within r + |floor| of the box, boxes whose
+// box grown by |floor| meets it; 0.5 of slack covers FP32 rounding), in
+// their original order (sphere ids, then box ids).  Block per chunk; order
+// kept by a ballot prefix per 256 primitives.
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_csg_cull(Arenas A, int n) {
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
+    const SamplerDev* __restrict__ S = A.sampler;
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+        ChunkMeta& M = A.meta[j];
+        const float lo[3] = {M.opos[0], M.opos[1], M.opos[2]};
+        const float ext = (float)(M.dim - 1) * M.delta;
+        const float reach = -kCsgFloor + 0.5f;
+        uint16_t* L = A.csg_list + (int64_t)j * A.csg_stride;
+        if (threadIdx.x == 0) s_base = 0;
+        __syncthreads();
+        for (int pass = 0; pass < 2; pass++) {
+            const int np = pass == 0 ? S->n_spheres : S->n_boxes;
+            for (int i0 = 0; i0 < np; i0 += 256) {
+                const int i = i0 + (int)threadIdx.x;
+                bool keep = false;
+                if (i < np) {
+                    if (pass == 0) {
+                        const float* sp = S->spheres + 4 * i;
+                        float dd = 0.0f;
+                        for (int a = 0; a < 3; a++) {
+                            const float t = fmaxf(fmaxf(lo[a] - sp[a], sp[a] - (lo[a] + ext)), 0.0f);
+                            dd += t * t;
+                        }
+                        const float r = sp[3] + reach;
+                        keep = dd < r * r;
+                    } else {
+                        const float* b = S->boxes + 6 * i;
+                        keep = true;
+                        for (int a = 0; a < 3; a++)
+                            keep = keep && (b[a] - b[3 + a] - reach <= lo[a] + ext) && (b[a] + b[3 + a] + reach >= lo[a]);
+                    }
+                }
+                const uint64_t bal = __ballot(keep);
+                if (lane == 0) s_wave[wave] = (uint32_t)__popcll(bal);
+                __syncthreads();
+                uint32_t off = s_base;
+                for (int w = 0; w < wave; w++) off += s_wave[w];
+                if (keep) L[off + (uint32_t)__popcll(bal & lt)] = (uint16_t)i;
+                __syncthreads();
+                if (threadIdx.x == 0) s_base += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                if (pass == 0) M.csg_ns = (int32_t)s_base;
+                else M.csg_nb = (int32_t)s_base - M.csg_ns;
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -1197,6 +1230,7 @@ __global__ void k_ref_masks(Arenas A, uint64_t* masks) {
 // ------------------------------------------------------------------------
 hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, hipStream_t s) {
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, s, A, n, allow_cull, reset_bumps);
+    if (A.csg_list) hipLaunchKernelGGL(k_csg_cull, dim3((unsigned)(n < 2048 ? n : 2048)), dim3(256), 0, s, A, n);
     return hipGetLastError();
 }
 hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, hipStream_t s) {
