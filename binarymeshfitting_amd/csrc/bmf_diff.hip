@@ -429,15 +429,15 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
         py = pos[3 * v + 1];
         pz = pos[3 * v + 2];
     }
-    float best = INFINITY;
+    float best = INFINITY;   // distance of this lane's own candidate
+    float bound = INFINITY;  // smallest candidate distance among the vertex's lanes (pruning)
     int bi = 0x7FFFFFFF;
     float4 bq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     int sp = 0;
     // the subtree this lane walks (the whole tree on lane 0 when it is shallow)
     int64_t node = !active ? 0 : (P >= kNNLanes ? kNNLanes + r : (r == 0 ? 1 : 0));
     for (;;) {
-        bool work = node != 0;
-        if (work) {
+        if (node != 0) {
             if (node < P) {
                 int64_t a = 2 * node, b = a + 1;
                 float la = box_lb2(px, py, pz, lo[a], hi[a]), lb = box_lb2(px, py, pz, lo[b], hi[b]);
@@ -449,12 +449,12 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
                     la = lb;
                     lb = u;
                 }
-                if (lb <= best && sp < kNNStack) {
+                if (lb <= bound && sp < kNNStack) {
                     s_node[sp][threadIdx.x] = (int32_t)b;
                     s_lb[sp][threadIdx.x] = lb;
                     sp++;
                 }
-                node = la <= best ? a : 0;
+                node = la <= bound ? a : 0;
             } else {
                 const int64_t e = std::min<int64_t>(n, (node - P + 1) * kLeaf);
                 for (int64_t i = (node - P) * kLeaf; i < e; i++) {
@@ -470,13 +470,14 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
                 node = 0;
             }
         }
-        // share the best distance among the lanes of this vertex
+        // the pruning bound: the best distance any lane of this vertex has
+        bound = best;
 #pragma unroll
-        for (int off = 1; off < kNNLanes; off <<= 1) best = fminf(best, __shfl_xor(best, off));
+        for (int off = 1; off < kNNLanes; off <<= 1) bound = fminf(bound, __shfl_xor(bound, off));
         if (node == 0) {
             while (sp > 0) {
                 sp--;
-                if (s_lb[sp][threadIdx.x] <= best) {
+                if (s_lb[sp][threadIdx.x] <= bound) {
                     node = s_node[sp][threadIdx.x];
                     break;
                 }
