@@ -44,9 +44,9 @@ constexpr int kThreads = 256;
 // most 20 triangles.
 constexpr int kMaxRing = 20;
 constexpr int kLeaf = 8;       // targets per leaf box of the nearest-target hierarchy
-constexpr int kNNThreads = 128;
-constexpr int kNNLanes = 8;    // lanes per vertex in the nearest-target search
-constexpr int kNNStack = 32;   // per-lane traversal stack (tree height <= 28 for n < 2^31)
+constexpr int kNNThreads = 256;  // four vertices per workgroup, one wave each
+constexpr int kNNFan = 6;        // a wave tests 2^kNNFan = 64 boxes of one subtree level at once
+constexpr int kNNFrames = 6;     // frames of the wave's walk: root + ceil(28 / kNNFan) levels for n < 2^31
 
 #define BMF_HIP(call)                          \
     do {                                       \
@@ -407,106 +407,146 @@ __device__ __forceinline__ float box_lb2(float px, float py, float pz, float4 l,
     return (dx * dx + dy * dy) + dz * dz;
 }
 
-// kNNLanes lanes per vertex (consecutive lanes of a wave): lane r walks the
-// subtree under heap node kNNLanes + r depth first, nearer child first; the
-// lanes of a vertex share their best distance every step, and a subtree is
-// skipped only when its box is strictly farther than that shared best (an
-// actual target distance), so an equally far target with a lower index is
-// still found.  The lanes' results are then merged by (distance, index).
+__device__ __forceinline__ float wave_min(float x) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x = fminf(x, __shfl_xor(x, off));
+    return x;
+}
+
+// Lowest lane of the (non-empty) set m whose value x is smallest.
+__device__ __forceinline__ int wave_argmin(float x, uint64_t m, int lane) {
+    const float y = (m >> lane) & 1 ? x : INFINITY;
+    const float mn = wave_min(y);
+    return __ffsll((unsigned long long)(__ballot(y == mn) & m)) - 1;
+}
+
+// Position of the k-th (from 0) set bit of m, or -1 when m has k bits or fewer.
+__device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
+    if (k >= __popcll(m)) return -1;
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __popcll(m & ((1ull << w) - 1));
+        if (k >= c) {
+            k -= c;
+            m >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+// Nearest target of each vertex, one wave per vertex.  The hierarchy (heap
+// nodes 1 .. 2P-1, leaves at depth D = log2 P) is walked in frames: frame 0
+// holds the root, and each later frame holds the 2^f descendants, f <=
+// kNNFan levels down, of a node popped from the frame above (the first step
+// takes D mod kNNFan levels so that the leaf frames are full).  Lane j
+// tests descendant j; the frame keeps the set whose box is no farther than
+// the best target distance U found so far, and pops its nearest box first.
+// A frame of leaves is evaluated kLeaf lanes per leaf, 64 / kLeaf leaves per
+// step, re-pruned against U between steps; the very first leaf frame starts
+// with its nearest leaf alone so that U is finite early.  Pruning drops a
+// box only when it is strictly farther than U, an actual target distance,
+// so every target at the minimum distance is evaluated and the result is
+// the lowest (distance, original index), as a brute-force scan gives.
 __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict__ pos, int V,
                                                          const float4* __restrict__ st, int64_t n, int64_t P,
                                                          const float4* __restrict__ lo, const float4* __restrict__ hi,
                                                          int32_t* __restrict__ nearest, float* __restrict__ dldp,
                                                          double* __restrict__ d2) {
-    __shared__ int32_t s_node[kNNStack][kNNThreads];
-    __shared__ float s_lb[kNNStack][kNNThreads];
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int v = gid / kNNLanes, r = gid % kNNLanes;
-    const bool active = v < V;  // uniform across the lanes of a vertex
-    float px = 0.0f, py = 0.0f, pz = 0.0f;
-    if (active) {
-        px = pos[3 * v];
-        py = pos[3 * v + 1];
-        pz = pos[3 * v + 2];
-    }
-    float best = INFINITY;   // distance of this lane's own candidate
-    float bound = INFINITY;  // smallest candidate distance among the vertex's lanes (pruning)
+    const int v = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (v >= V) return;  // uniform across the wave
+    const int lane = (int)(threadIdx.x & 63);
+    const float px = pos[3 * (int64_t)v], py = pos[3 * (int64_t)v + 1], pz = pos[3 * (int64_t)v + 2];
+    int D = 0;
+    while ((int64_t)1 << D < P) D++;
+    const int f0 = D == 0 ? 0 : D - kNNFan * ((D - 1) / kNNFan);
+    float bd = INFINITY;  // this lane's best (distance, original index, sorted position)
     int bi = 0x7FFFFFFF;
-    float4 bq = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    int sp = 0;
-    // the subtree this lane walks (the whole tree on lane 0 when it is shallow)
-    int64_t node = !active ? 0 : (P >= kNNLanes ? kNNLanes + r : (r == 0 ? 1 : 0));
-    for (;;) {
-        if (node != 0) {
-            if (node < P) {
-                int64_t a = 2 * node, b = a + 1;
-                float la = box_lb2(px, py, pz, lo[a], hi[a]), lb = box_lb2(px, py, pz, lo[b], hi[b]);
-                if (lb < la) {
-                    const int64_t t = a;
-                    a = b;
-                    b = t;
-                    const float u = la;
-                    la = lb;
-                    lb = u;
-                }
-                if (lb <= bound && sp < kNNStack) {
-                    s_node[sp][threadIdx.x] = (int32_t)b;
-                    s_lb[sp][threadIdx.x] = lb;
-                    sp++;
-                }
-                node = la <= bound ? a : 0;
-            } else {
-                const int64_t e = std::min<int64_t>(n, (node - P + 1) * kLeaf);
-                for (int64_t i = (node - P) * kLeaf; i < e; i++) {
-                    const float4 q = st[i];
-                    const float d = dist2(px, py, pz, q);
-                    const int id = __float_as_int(q.w);
-                    if (d < best || (d == best && id < bi)) {
-                        best = d;
-                        bi = id;
-                        bq = q;
-                    }
-                }
-                node = 0;
+    int64_t bs = -1;
+    float U = INFINITY;  // best distance over the wave
+    // the frames of this wave: per-lane box distances, the live set, heap index of descendant 0
+    __shared__ float s_lb[kNNThreads / 64][kNNFrames][64];
+    __shared__ unsigned long long s_mk[kNNThreads / 64][kNNFrames];
+    __shared__ int64_t s_base[kNNThreads / 64][kNNFrames];
+    const int w = (int)(threadIdx.x >> 6);
+    s_lb[w][0][lane] = lane == 0 ? box_lb2(px, py, pz, lo[1], hi[1]) : INFINITY;
+    s_mk[w][0] = 1;
+    s_base[w][0] = 1;
+    int depth = 1;
+    auto eval = [&](int64_t leaf) {
+        // this lane's target of `leaf` (heap index), if any
+        const int64_t i = (leaf - P) * kLeaf + (lane & (kLeaf - 1));
+        if (i < n) {
+            const float4 q = st[i];
+            const float d = dist2(px, py, pz, q);
+            const int id = __float_as_int(q.w);
+            if (d < bd || (d == bd && id < bi)) {
+                bd = d;
+                bi = id;
+                bs = i;
             }
         }
-        // the pruning bound: the best distance any lane of this vertex has
-        bound = best;
+    };
+    while (depth > 0) {
+        const int k = depth - 1;
+        const int lk = k == 0 ? 0 : f0 + kNNFan * (k - 1);
+        const float lb = s_lb[w][k][lane];
+        const int64_t b0 = s_base[w][k];
+        unsigned long long m = s_mk[w][k] & __ballot(lb <= U);
+        if (lk == D) {
+            if (U == INFINITY && m) {
+                const int j = wave_argmin(lb, m, lane);
+                if (lane < kLeaf) eval(b0 + j);
+                m &= ~(1ull << j);
+                U = wave_min(bd);
+                m &= __ballot(lb <= U);
+            }
+            while (m) {
+                const int j = kth_set_bit(m, lane / kLeaf);
+                if (j >= 0) eval(b0 + j);
 #pragma unroll
-        for (int off = 1; off < kNNLanes; off <<= 1) bound = fminf(bound, __shfl_xor(bound, off));
-        if (node == 0) {
-            while (sp > 0) {
-                sp--;
-                if (s_lb[sp][threadIdx.x] <= bound) {
-                    node = s_node[sp][threadIdx.x];
-                    break;
-                }
+                for (int t = 0; t < 64 / kLeaf; t++) m &= m - 1;
+                U = wave_min(bd);
+                m &= __ballot(lb <= U);
             }
+            depth--;
+            continue;
         }
-        if (!__any(node != 0)) break;
+        if (!m) {
+            depth--;
+            continue;
+        }
+        const int j = wave_argmin(lb, m, lane);
+        s_mk[w][k] = m & ~(1ull << j);
+        const int f = k == 0 ? f0 : kNNFan;
+        const int64_t c0 = (b0 + j) << f;
+        const float lc = lane < (1 << f) ? box_lb2(px, py, pz, lo[c0 + lane], hi[c0 + lane]) : INFINITY;
+        s_lb[w][k + 1][lane] = lc;
+        s_mk[w][k + 1] = __ballot(lane < (1 << f) && lc <= U);
+        s_base[w][k + 1] = c0;
+        depth++;
     }
-    // merge the lanes' candidates: lowest (distance, index)
-    float bx = bq.x, by = bq.y, bz = bq.z;
+    // the wave's lowest (distance, index)
 #pragma unroll
-    for (int off = 1; off < kNNLanes; off <<= 1) {
-        const float od = __shfl_xor(best, off);
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float od = __shfl_xor(bd, off);
         const int oi = __shfl_xor(bi, off);
-        const float ox = __shfl_xor(bx, off), oy = __shfl_xor(by, off), oz = __shfl_xor(bz, off);
-        if (od < best || (od == best && oi < bi)) {
-            best = od;
+        const int64_t os = __shfl_xor(bs, off);
+        if (od < bd || (od == bd && oi < bi)) {
+            bd = od;
             bi = oi;
-            bx = ox;
-            by = oy;
-            bz = oz;
+            bs = os;
         }
     }
-    if (!active || r != 0) return;
-    nearest[v] = bi;
+    if (lane != 0) return;
+    const float4 q = bs >= 0 ? st[bs] : make_float4(px, py, pz, 0.0f);
+    nearest[v] = bs >= 0 ? bi : -1;
     const float sc = 2.0f / (float)V;
-    dldp[3 * v] = (px - bx) * sc;
-    dldp[3 * v + 1] = (py - by) * sc;
-    dldp[3 * v + 2] = (pz - bz) * sc;
-    d2[v] = (double)best;
+    dldp[3 * (int64_t)v] = (px - q.x) * sc;
+    dldp[3 * (int64_t)v + 1] = (py - q.y) * sc;
+    dldp[3 * (int64_t)v + 2] = (pz - q.z) * sc;
+    d2[v] = bs >= 0 ? (double)bd : 0.0;
 }
 
 // Deterministic sum: thread t adds x[t], x[t + 1024], ... in order, then a
@@ -856,7 +896,7 @@ int bmf_loss_nearest(bmf_tape* t, const float* targets, int64_t n, float* dldp, 
     for (; level >= 2048; level >>= 1)
         hipLaunchKernelGGL(k_nn_level, dim3(blocks_for(level)), dim3(kThreads), 0, s, level, t->blo, t->bhi);
     if (level >= 1) hipLaunchKernelGGL(k_nn_inner, dim3(1), dim3(1024), 0, s, level, t->blo, t->bhi);
-    hipLaunchKernelGGL(k_nn_query, dim3((unsigned)(((int64_t)t->V * kNNLanes + kNNThreads - 1) / kNNThreads)),
+    hipLaunchKernelGGL(k_nn_query, dim3((unsigned)(((int64_t)t->V * 64 + kNNThreads - 1) / kNNThreads)),
                        dim3(kNNThreads), 0, s,
                        t->pos, t->V, t->st, n, P, t->blo, t->bhi, nearest ? nearest : t->nearest_tmp, dldp, t->d2);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, t->d2, t->V, t->loss);

@@ -135,6 +135,8 @@ typedef struct or_kdtree or_kdtree;
 or_kdtree* or_kd_build(const double* pts, int64_t n);
 void or_kd_free(or_kdtree* t);
 void or_kd_nearest(const or_kdtree* t, const double* q, int64_t m, int64_t* out, double* d2, int threads);
+/* FP32 distances ((dx*dx + dy*dy) + dz*dz, each step rounded), ties to the lower index */
+void or_kd_nearest_f32(const or_kdtree* t, const float* q, int64_t m, int64_t* out, float* d2, int threads);
 double or_loss_nearest_f64(const double* pos, int64_t V, const double* targets, const int64_t* nearest,
                            double* dldp);
 void or_fd_grad_f64(const double* dens, int dim, const uint32_t* edge, const uint8_t* boundary, int64_t V,

@@ -351,6 +351,62 @@ void or_kd_nearest(const or_kdtree* t, const double* q, int64_t m, int64_t* out,
     }
 }
 
+/* Nearest target in FP32 arithmetic: squared distance ((dx*dx + dy*dy) +
+ * dz*dz) with every operation rounded to FP32, ties to the lower target
+ * index.  The tree holds the FP32 coordinates exactly (as doubles); a box is
+ * pruned only when its FP64 distance, shrunk by 4e-6 relative (more than the
+ * four FP32 roundings of a distance can take off), is still above the best
+ * FP32 distance, so the result is that of a brute-force FP32 scan.          */
+static void kd_query_one_f32(const or_kdtree* t, const float* p, int64_t* best_id, float* best_d2)
+{
+    int64_t stack[256];
+    int sp = 0;
+    const double pd[3] = {p[0], p[1], p[2]};
+    float best = INFINITY;
+    int64_t bi = INT64_MAX;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const int64_t node = stack[--sp];
+        if (box_lb2(t->box + 6 * node, pd) * (1.0 - 4e-6) > (double)best + 1e-30) continue;
+        if (t->left[node] < 0) {
+            for (int64_t i = t->lo[node]; i < t->hi[node]; i++) {
+                const float dx = p[0] - (float)t->pts[3 * i], dy = p[1] - (float)t->pts[3 * i + 1],
+                            dz = p[2] - (float)t->pts[3 * i + 2];
+                const float d = (dx * dx + dy * dy) + dz * dz;
+                if (d < best || (d == best && t->id[i] < bi)) {
+                    best = d;
+                    bi = t->id[i];
+                }
+            }
+            continue;
+        }
+        const int64_t a = t->left[node], b = t->right[node];
+        const double la = box_lb2(t->box + 6 * a, pd), lb = box_lb2(t->box + 6 * b, pd);
+        if (la <= lb) {
+            stack[sp++] = b;
+            stack[sp++] = a;
+        } else {
+            stack[sp++] = a;
+            stack[sp++] = b;
+        }
+    }
+    *best_id = bi;
+    *best_d2 = best;
+}
+
+void or_kd_nearest_f32(const or_kdtree* t, const float* q, int64_t m, int64_t* out, float* d2, int threads)
+{
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 256)
+#endif
+    for (int64_t i = 0; i < m; i++) {
+        float dd;
+        kd_query_one_f32(t, q + 3 * i, &out[i], &dd);
+        if (d2) d2[i] = dd;
+    }
+}
+
 double or_loss_nearest_f64(const double* pos, int64_t V, const double* targets, const int64_t* nearest, double* dldp)
 {
     if (V == 0) return 0.0;

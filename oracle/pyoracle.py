@@ -126,6 +126,7 @@ def lib():
         L.or_kd_build.restype = C.c_void_p
         L.or_kd_free.argtypes = [C.c_void_p]
         L.or_kd_nearest.argtypes = [C.c_void_p, f64p, C.c_int64, i64p, f64p, C.c_int]
+        L.or_kd_nearest_f32.argtypes = [C.c_void_p, f32p, C.c_int64, i64p, f32p, C.c_int]
         L.or_loss_nearest_f64.argtypes = [f64p, C.c_int64, f64p, i64p, f64p]
         L.or_loss_nearest_f64.restype = C.c_double
         L.or_fd_grad_f64.argtypes = [f64p] + topo + [C.c_void_p, i64p, C.c_int, C.c_double, C.c_int, f64p, i32p]
@@ -336,6 +337,15 @@ class KDTree:
         idx = np.zeros(len(q), np.int64)
         d2 = np.zeros(len(q), np.float64)
         lib().or_kd_nearest(self.h, _ptr(q, C.c_double), len(q), _ptr(idx, C.c_int64), _ptr(d2, C.c_double), threads)
+        return idx, d2
+
+    def nearest_f32(self, q, threads=8):
+        """Nearest target with FP32 distances ((dx*dx + dy*dy) + dz*dz, every
+        step rounded to FP32; the targets must be FP32 values)."""
+        q = np.ascontiguousarray(q, np.float32)
+        idx = np.zeros(len(q), np.int64)
+        d2 = np.zeros(len(q), np.float32)
+        lib().or_kd_nearest_f32(self.h, _ptr(q, C.c_float), len(q), _ptr(idx, C.c_int64), _ptr(d2, C.c_float), threads)
         return idx, d2
 
     def close(self):

@@ -125,8 +125,11 @@ def test_loss_and_gradient_match_fp64_oracle(ctx, tape, oracle):
     # nearest target of the GPU's own FP32 positions, in FP64
     onear, _ = kd.nearest(pos32.astype(np.float64), threads=16)
     agree = np.mean(onear == near)
-    print(f"V={info.vertex_count} nearest agreement {agree:.6f}")
+    print(f"V={info.vertex_count} nearest agreement with FP64 distances {agree:.6f}")
     assert agree >= 0.999
+    # with the kernel's own FP32 distances the assignment is exact
+    near32, _ = kd.nearest_f32(pos32, threads=16)
+    assert np.array_equal(near, near32)
     L64, _ = oracle.loss_nearest_f64(pos32, targets, near)
     assert abs(L - L64) <= 1e-5 * L64
     # FP64 forward and adjoint, with the loss gradient taken at the GPU's
@@ -145,6 +148,29 @@ def test_loss_and_gradient_match_fp64_oracle(ctx, tape, oracle):
     off = np.ones(d64.size, bool)
     off[_crossing_samples(ref, dim)] = False
     assert np.all(grad.ravel()[off] == 0.0)
+
+
+def test_nearest_ties_go_to_lower_index(ctx, tape, oracle):
+    """Six targets at exactly 0.25 from every vertex (+-0.25 per axis), in a
+    shuffled order among random decoys: the search returns the lowest index
+    at the minimum FP32 distance, as a brute-force scan does."""
+    c = workloads.Chunk((0.0, 16.0, 0.0), 32.0, 33, 0.0, 2, 0)
+    info = tape.forward(capi.chunk_desc(c.pos, c.size, c.dim, c.overlap, c.iters, c.boundary_processing))
+    pos, _, _ = tape.mesh()
+    rng = np.random.default_rng(11)
+    offs = np.float32(0.25) * np.concatenate([np.eye(3), -np.eye(3)]).astype(np.float32)
+    t = (pos[:, None, :] + offs[None, :, :]).reshape(-1, 3)
+    t = np.concatenate([t, rng.uniform(-8, 40, (5000, 3)).astype(np.float32)])
+    t = t[rng.permutation(len(t))]
+    L, dldp, near = tape.loss_nearest(capi.DeviceArray.from_numpy(t))
+    kd = oracle.KDTree(t.astype(np.float64))
+    near32, d32 = kd.nearest_f32(pos, threads=16)
+    assert np.array_equal(near, near32)
+    # most vertices do have several targets at the exact minimum
+    dd = t[near] - pos
+    tied = np.sum(((dd * dd)[:, 0] + (dd * dd)[:, 1]) + (dd * dd)[:, 2] == np.float32(0.0625))
+    assert tied >= 0.9 * info.vertex_count
+    assert np.array_equal(dldp.numpy()[: info.vertex_count], (pos - t[near]) * (np.float32(2.0) / np.float32(info.vertex_count)))
 
 
 def test_gradient_matches_finite_differences(ctx, tape, oracle):

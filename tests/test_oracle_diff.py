@@ -97,6 +97,38 @@ def test_kdtree_matches_brute_force_with_ties(oracle):
     assert np.allclose(d2, D[np.arange(len(q)), ref], rtol=1e-12, atol=0)
 
 
+def _brute_f32(q, t):
+    """FP32 scan: ((dx*dx + dy*dy) + dz*dz), each operation rounded to FP32."""
+    dx = q[:, None, 0] - t[None, :, 0]
+    dy = q[:, None, 1] - t[None, :, 1]
+    dz = q[:, None, 2] - t[None, :, 2]
+    D = (dx * dx + dy * dy) + dz * dz
+    assert D.dtype == np.float32
+    return np.argmin(D, axis=1), D
+
+
+@pytest.mark.parametrize("case", ["lattice", "sphere"])
+def test_kdtree_f32_matches_brute_force_with_ties(oracle, case):
+    rng = np.random.default_rng(5)
+    if case == "lattice":
+        # integer lattice points: many exactly equal FP32 distances
+        t = rng.integers(0, 12, (2500, 3)).astype(np.float32)
+        q = np.concatenate([rng.integers(-3, 15, (400, 3)).astype(np.float32) + np.float32(0.5),
+                            rng.uniform(-3, 15, (300, 3)).astype(np.float32)])
+    else:
+        # targets on a sphere, queries far inside and outside it
+        t = workloads.sphere_targets(n=4000, seed=9, radius=10.0, centre=(0.0, 0.0, 0.0))
+        q = rng.uniform(-14, 14, (700, 3)).astype(np.float32)
+    kd = oracle.KDTree(t.astype(np.float64))
+    idx, d2 = kd.nearest_f32(q)
+    ref, D = _brute_f32(q, t)
+    assert np.array_equal(idx, ref)
+    assert np.array_equal(d2, D[np.arange(len(q)), ref])
+    if case == "lattice":
+        # the tie rule is exercised: some queries have several targets at the minimum
+        assert np.sum(np.sum(D == D.min(1, keepdims=True), 1) > 1) > 50
+
+
 def test_loss_gradient_closed_form(oracle):
     rng = np.random.default_rng(5)
     p = rng.uniform(0, 10, (200, 3))
