@@ -166,10 +166,13 @@ def test_nearest_ties_go_to_lower_index(ctx, tape, oracle):
     kd = oracle.KDTree(t.astype(np.float64))
     near32, d32 = kd.nearest_f32(pos, threads=16)
     assert np.array_equal(near, near32)
-    # most vertices do have several targets at the exact minimum
-    dd = t[near] - pos
-    tied = np.sum(((dd * dd)[:, 0] + (dd * dd)[:, 1]) + (dd * dd)[:, 2] == np.float32(0.0625))
-    assert tied >= 0.9 * info.vertex_count
+    # the tie rule is exercised: most vertices have two or more of their own
+    # six targets at the exact minimum distance
+    od = (pos[:, None, :] + offs[None, :, :]) - pos[:, None, :]
+    od2 = (od[..., 0] * od[..., 0] + od[..., 1] * od[..., 1]) + od[..., 2] * od[..., 2]
+    tied = np.sum(np.sum(od2 == d32[:, None], 1) >= 2)
+    print(f"{tied} of {info.vertex_count} vertices with tied nearest targets")
+    assert tied >= 0.5 * info.vertex_count
     assert np.array_equal(dldp.numpy()[: info.vertex_count], (pos - t[near]) * (np.float32(2.0) / np.float32(info.vertex_count)))
 
 
