@@ -67,7 +67,7 @@ class MeshOptions(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sample_tile", C.c_int32), ("sample_blocks", C.c_int32), ("row_blocks", C.c_int32),
-                ("smooth_blocks", C.c_int32)]
+                ("smooth_blocks", C.c_int32), ("row_split", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -275,8 +275,8 @@ class Context:
         check(lib().bmf_get_tuning(self.h, C.byref(t)), "bmf_get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, sample_tile=0, sample_blocks=0, row_blocks=0, smooth_blocks=0):
-        t = Tuning(sample_tile, sample_blocks, row_blocks, smooth_blocks)
+    def set_tuning(self, sample_tile=0, sample_blocks=0, row_blocks=0, smooth_blocks=0, row_split=0):
+        t = Tuning(sample_tile, sample_blocks, row_blocks, smooth_blocks, row_split)
         check(lib().bmf_set_tuning(self.h, C.byref(t)), "bmf_set_tuning")
 
     def autotune(self, chunks, reps=3):

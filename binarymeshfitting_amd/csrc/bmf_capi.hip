@@ -34,7 +34,7 @@ hipError_t dalloc(T** p, int64_t count) {
 
 void free_arenas(bmf_ctx* c) {
     Arenas& A = c->A;
-    void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plane_prefix, A.smooth_list, A.smooth_prefix,
+    void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plane_prefix, A.plane_chunk, A.smooth_list, A.smooth_prefix,
                     A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
                     A.pos,     A.pos4,      A.boundary, A.vedge,       A.idx,          A.ctr,
@@ -164,6 +164,7 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     c->tune.sample_blocks = c->persistent_blocks;
     c->tune.row_blocks = c->persistent_blocks;
     c->tune.smooth_blocks = 4 * c->persistent_blocks;
+    c->tune.row_split = -1;
     Arenas& A = c->A;
     const int64_t nc = lim->max_chunks;
     A.cap_chunks = (int32_t)nc;
@@ -172,13 +173,15 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     A.cap_vertices = lim->max_vertices;
     A.cap_indices = lim->max_indices;
     A.sample_tile = kSampleTileDefault;
+    A.row_split = -1;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && dalloc(&A.meta, nc) == hipSuccess && dalloc(&A.descs, nc) == hipSuccess &&
          dalloc(&A.results, nc) == hipSuccess && dalloc(&A.tile_prefix, nc + 1) == hipSuccess &&
-         dalloc(&A.plane_prefix, nc + 1) == hipSuccess && dalloc(&A.smooth_list, nc) == hipSuccess &&
+         dalloc(&A.plane_prefix, nc + 1) == hipSuccess &&
+         dalloc(&A.plane_chunk, nc * BMF_MAX_DIM) == hipSuccess && dalloc(&A.smooth_list, nc) == hipSuccess &&
          dalloc(&A.smooth_prefix, nc + 1) == hipSuccess && dalloc(&A.smooth_tprefix, nc + 1) == hipSuccess &&
          dalloc(&A.smooth_xcd_v, kXcds + 1) == hipSuccess && dalloc(&A.smooth_xcd_t, kXcds + 1) == hipSuccess &&
-         dalloc(&A.refine, A.cap_samples) == hipSuccess && dalloc(&A.plane_refine_n, nc * BMF_MAX_DIM) == hipSuccess &&
+         dalloc(&A.refine, A.cap_samples) == hipSuccess && dalloc(&A.plane_refine_n, nc * BMF_MAX_DIM * kMaxPlaneSplit) == hipSuccess &&
          dalloc(&A.cen4, A.cap_indices / 3 + 1) == hipSuccess &&
          dalloc(&A.dens, A.cap_samples) == hipSuccess && dalloc(&A.labels, A.cap_samples / 64 + nc) == hipSuccess &&
          dalloc(&A.exact, A.cap_samples / 64 + nc) == hipSuccess &&
@@ -341,7 +344,7 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
             const int64_t ns = dim * dim * dim;
             const int sb = (int)std::min<int64_t>(ceil_div(ns, A.sample_tile), pb);
             rc = enqueue_pipeline(c, Aj, 1, max_steps(c, &chunks[j], 1), row_word_class(chunks[j].dim), 1, j == 0, sb,
-                                  (int)dim, 1,
+                                  (int)dim * kMaxPlaneSplit, 1,
                                   std::min(pb, 1024));
             if (rc) return rc;
         }
@@ -676,10 +679,16 @@ int bmf_set_tuning(bmf_ctx* c, const bmf_tuning* t) {
     if (t->sample_blocks) n.sample_blocks = t->sample_blocks;
     if (t->row_blocks) n.row_blocks = t->row_blocks;
     if (t->smooth_blocks) n.smooth_blocks = (t->smooth_blocks + kXcds - 1) / kXcds * kXcds;
+    if (t->row_split) {
+        if (t->row_split != -1 && t->row_split != 1 && t->row_split != 2 && t->row_split != 4 && t->row_split != 8)
+            return BMF_EINVAL;
+        n.row_split = t->row_split;
+    }
     BMF_HIP(hipSetDevice(c->device));
     BMF_HIP(hipStreamSynchronize(c->stream));
     c->tune = n;
     c->A.sample_tile = n.sample_tile;
+    c->A.row_split = n.row_split;
     drop_graphs(c);
     return BMF_OK;
 }
@@ -702,6 +711,7 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
     auto run = [&](const bmf_tuning& t, float* out) -> int {
         Arenas A = c->A;
         A.sample_tile = t.sample_tile;
+        A.row_split = t.row_split;
         for (int i = 0; i < BMF_NUM_STAGES; i++) out[i] = 1e30f;
         for (int r = 0; r < reps; r++) {
             int e = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, t.sample_blocks, t.row_blocks,
@@ -739,22 +749,27 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
         cur.sample_tile = b.sample_tile;
         cur.sample_blocks = b.sample_blocks;
     }
-    // plane kernels: count (+ refine) and emit (stages 2 and 4)
+    // plane kernels: count (+ refine) and emit (stages 2 and 4), workgroups x row segments
     {
         float bt = 1e30f;
-        int bb = cur.row_blocks;
+        int bb = cur.row_blocks, bs = cur.row_split;
         const int mult[5] = {1, 2, 4, 8, 16};
-        for (int m : mult) {
-            bmf_tuning t = cur;
-            t.row_blocks = m * cu;
-            rc = run(t, st);
-            if (rc) return rc;
-            if (st[2] + st[4] < bt) {
-                bt = st[2] + st[4];
-                bb = t.row_blocks;
+        const int splits[4] = {1, 2, 4, 8};
+        for (int sp : splits)
+            for (int m : mult) {
+                bmf_tuning t = cur;
+                t.row_blocks = m * cu;
+                t.row_split = sp;
+                rc = run(t, st);
+                if (rc) return rc;
+                if (st[2] + st[4] < bt) {
+                    bt = st[2] + st[4];
+                    bb = t.row_blocks;
+                    bs = t.row_split;
+                }
             }
-        }
         cur.row_blocks = bb;
+        cur.row_split = bs;
     }
     // smoothing (stage 5)
     if (maxk > 0) {

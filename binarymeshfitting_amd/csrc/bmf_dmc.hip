@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_count(Arenas A, int n) {
     const int64_t total = A.ctr->total_planes;
     const uint32_t ppq = A.quad_tris ? 2u : 1u;
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const int j = find_chunk(A.plane_prefix, n, item);
+        const int j = A.plane_chunk[item];
         const ChunkMeta& M = A.meta[j];
         const int dim = M.dim;
         if (((dim + 63) >> 6) != W) continue;
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
     const uint64_t lt = lanemask_lt();
     const int64_t total = A.ctr->total_planes;
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const int j = find_chunk(A.plane_prefix, n, item);
+        const int j = A.plane_chunk[item];
         const ChunkMeta& M = A.meta[j];
         const int dim = M.dim;
         if (((dim + 63) >> 6) != W) continue;
@@ -464,7 +464,7 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
     const int64_t total = A.ctr->total_planes;
     const bool tris = A.quad_tris != 0;
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const int j = find_chunk(A.plane_prefix, n, item);
+        const int j = A.plane_chunk[item];
         const ChunkMeta& M = A.meta[j];
         const int dim = M.dim;
         if (((dim + 63) >> 6) != W) continue;

@@ -12,7 +12,9 @@ namespace bmf {
 // of 64 so that every wave owns whole label words.
 constexpr int kSampleThreads = 256;
 constexpr int kSampleTileDefault = 2048;
-constexpr int kRowThreads = 256;      // 4 waves; one x-plane per block, one label row per wave
+constexpr int kRowThreads = 256;      // 4 waves; one x-plane (or row segment of one) per block, one label row per wave
+constexpr int kMaxPlaneSplit = 8;     // row segments per x-plane when few planes are sampled
+constexpr int kSplitTarget = 512;     // row-kernel work items aimed at before splitting stops
 constexpr int kSmoothThreads = 256;   // one vertex per thread
 constexpr int kMaxRowWords = 4;       // ceil(BMF_MAX_DIM / 64)
 constexpr int kXcds = 8;              // MI355X: 8 XCDs, each with its own L2
@@ -83,6 +85,7 @@ struct Counters {
     int32_t n_smooth;
     int32_t overflow;
     int32_t error;
+    int32_t plane_split;  // row segments per x-plane in k_count / k_refine / k_emit
 };
 
 // Device pointers of one context.
@@ -92,13 +95,16 @@ struct Arenas {
     bmf_chunk_result* results; // [max_chunks]
     int64_t* tile_prefix;      // [max_chunks + 1] sampling tiles
     int64_t* plane_prefix;     // [max_chunks + 1] x-planes of the sampled chunks
+    int32_t* plane_chunk;      // [max_chunks * BMF_MAX_DIM] chunk of each x-plane item (written by k_sample)
     int32_t* smooth_list;      // [max_chunks] chunks that run the smoothing
     int64_t* smooth_prefix;    // [max_chunks + 1] vertex tiles of the listed chunks
     int64_t* smooth_tprefix;   // [max_chunks + 1] triangle tiles of the listed chunks
     int64_t* smooth_xcd_v;     // [kXcds + 1] vertex-tile range of each XCD group
     int64_t* smooth_xcd_t;     // [kXcds + 1] triangle-tile range of each XCD group
-    uint32_t* refine;          // [max_samples] per x-plane queue of samples (y*dim + z) to finish
-    uint32_t* plane_refine_n;  // [max_planes] queue length of each x-plane item
+    uint32_t* refine;          // [max_samples] per x-plane queue of samples (y*dim + z) to finish,
+                               // a region of rows [y0, y1) per row segment
+    int32_t row_split;         // row segments per x-plane item, or -1 for the automatic choice
+    uint32_t* plane_refine_n;  // [max_planes * kMaxPlaneSplit] queue length of each (x-plane, row segment) item
     float* dens;               // [max_samples]
     uint64_t* labels;          // [max_samples / 64 + max_chunks]
     uint64_t* exact;           // same layout as labels: density of the sample is final

@@ -144,6 +144,12 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         if (overflow) c->overflow = 1;
         c->total_tiles = overflow ? 0 : tot_tiles;
         c->total_planes = overflow ? 0 : tot_planes;
+        // few sampled planes (single large chunks): split each plane's rows
+        // into segments so that the row kernels still have kSplitTarget items
+        int split = A.row_split > 0 ? A.row_split : 1;
+        if (A.row_split <= 0)
+            while (split < kMaxPlaneSplit && tot_planes * split < kSplitTarget) split <<= 1;
+        c->plane_split = split;
         c->n_smooth = 0;
         c->total_samples = tot_samples;
     }
@@ -278,6 +284,13 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int 
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
     const bool adapt = adaptive && S->kind == BMF_SAMPLER_TERRAIN;
+    // the chunk of every x-plane item of the plane kernels that follow
+    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+        const ChunkMeta& M = A.meta[j];
+        if (M.state != kStateSampled) continue;
+        const int64_t pp = A.plane_prefix[j];
+        for (int x = threadIdx.x; x < M.dim; x += blockDim.x) A.plane_chunk[pp + x] = j;
+    }
     for (;;) {
         if (threadIdx.x == 0) {
             s_tile = (int64_t)atomicAdd(&A.ctr->sample_head, 1ull);
@@ -414,7 +427,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int 
 }
 
 // ------------------------------------------------------------------------
-// count: one block per (chunk, x-plane), one wave per label row ->
+// count: one block per (chunk, x-plane, row segment), one wave per label row ->
 // vertices and triangles of each row
 // ------------------------------------------------------------------------
 // Staged planes: s_lab[0..2] = labels of x-1, x, x+1; s_lab[3] = exactness
@@ -480,19 +493,23 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
 
 // ------------------------------------------------------------------------
 // refine: full density of the queued samples (endpoints of sign-changing
-// edges labelled early by k_sample); one block per x-plane queue.
+// edges labelled early by k_sample); one block per (x-plane, row segment) queue.
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A, int n) {
     const SamplerDev* __restrict__ S = A.sampler;
-    const int64_t total = A.ctr->total_planes;
-    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const uint32_t cnt = A.plane_refine_n[item];
+    const int split = A.ctr->plane_split;
+    const int64_t total = A.ctr->total_planes * split;
+    for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+        const uint32_t cnt = A.plane_refine_n[it];
         if (cnt == 0) continue;
-        const int j = find_chunk(A.plane_prefix, n, item);
+        const int64_t item = it / split;
+        const int seg = (int)(it - item * split);
+        const int j = A.plane_chunk[item];
         const ChunkMeta& M = A.meta[j];
         const uint32_t dim = (uint32_t)M.dim;
         const uint32_t x = (uint32_t)(item - A.plane_prefix[j]);
-        const uint32_t* __restrict__ q = A.refine + M.dens_off + (int64_t)x * dim * dim;
+        const uint32_t y0 = (uint32_t)((int64_t)seg * dim / split);
+        const uint32_t* __restrict__ q = A.refine + M.dens_off + (int64_t)x * dim * dim + (int64_t)y0 * dim;
         float* __restrict__ D = A.dens + M.dens_off + (int64_t)x * dim * dim;
         const float px = (float)x * M.delta + M.opos[0];
         for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) {
@@ -516,18 +533,22 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
     for (int t = threadIdx.x; t < 256; t += blockDim.x) s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
     const int wave = uniform_int(threadIdx.x >> 6);
     const int lane = lane_id();
-    const int64_t total = A.ctr->total_planes;
-    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const int j = find_chunk(A.plane_prefix, n, item);
+    const int split = A.ctr->plane_split;
+    const int64_t total = A.ctr->total_planes * split;
+    for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+        const int64_t item = it / split;
+        const int seg = (int)(it - item * split);
+        const int j = A.plane_chunk[item];
         const ChunkMeta& M = A.meta[j];
         const int dim = M.dim;
         if (((dim + 63) >> 6) != W) continue;
         const int x = (int)(item - A.plane_prefix[j]);
+        const int y0 = (int)((int64_t)seg * dim / split), y1 = (int)((int64_t)(seg + 1) * dim / split);
         uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
         uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
         if (!(M.any_air && M.any_solid)) {  // no surface in this chunk
-            for (int y = threadIdx.x; y < dim; y += blockDim.x) rv[y] = rt[y] = 0u;
-            if (threadIdx.x == 0) A.plane_refine_n[item] = 0;
+            for (int y = y0 + threadIdx.x; y < y1; y += blockDim.x) rv[y] = rt[y] = 0u;
+            if (threadIdx.x == 0) A.plane_refine_n[it] = 0;
             continue;
         }
         __syncthreads();  // previous item is done with the shared arrays
@@ -535,8 +556,9 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
         stage_planes(s_planes[0], s_planes[1], s_planes[2], A.labels + M.lab_off, M.lab_words, dim, x - 1);
         stage_plane(s_planes[3], A.exact + M.lab_off, M.lab_words, dim, x);
         __syncthreads();
-        uint32_t* q = A.refine + M.dens_off + (int64_t)x * dim * dim;
-        for (int y = wave; y < dim; y += kRowThreads / 64) {
+        // this segment's region of the plane's refine queue: its rows' samples
+        uint32_t* q = A.refine + M.dens_off + (int64_t)x * dim * dim + (int64_t)y0 * dim;
+        for (int y = y0 + wave; y < y1; y += kRowThreads / 64) {
             uint32_t nv = 0, nt = 0;
             count_row<W>(A, M, s_lab, s_ntri, x, y, q, &s_qn, nv, nt);
             if (lane == 0) {
@@ -545,7 +567,7 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) A.plane_refine_n[item] = s_qn;
+        if (threadIdx.x == 0) A.plane_refine_n[it] = s_qn;
     }
 }
 
@@ -555,8 +577,10 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
 constexpr int kScanThreads = 1024;
 
 __global__ void __launch_bounds__(kScanThreads) k_scan(Arenas A, int n) {
-    __shared__ uint32_t s_v[kScanThreads], s_t[kScanThreads];
-    const int tid = threadIdx.x;
+    constexpr int NW = kScanThreads / 64;
+    constexpr int kGroup = 8;  // tiles loaded together (their loads overlap)
+    __shared__ uint32_t s_wv[2][NW], s_wt[2][NW];  // wave totals, double-buffered by tile parity
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int j = blockIdx.x; j < n; j += gridDim.x) {
         ChunkMeta& M = A.meta[j];
         if (M.state != kStateSampled) continue;
@@ -564,33 +588,59 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(Arenas A, int n) {
         const int64_t R = (int64_t)dim * dim;
         uint32_t* rv = A.row_v + M.row_off;
         uint32_t* rt = A.row_t + M.row_off;
-        const int64_t per = (R + kScanThreads - 1) / kScanThreads;
-        const int64_t r0 = tid * per, r1 = min(R, r0 + per);
-        uint32_t sv = 0, st = 0;
-        for (int64_t r = r0; r < r1; r++) {
-            sv += rv[r];
-            st += rt[r];
-        }
-        s_v[tid] = sv;
-        s_t[tid] = st;
-        __syncthreads();
-        for (int off = 1; off < kScanThreads; off <<= 1) {
-            uint32_t av = tid >= off ? s_v[tid - off] : 0u, at = tid >= off ? s_t[tid - off] : 0u;
-            __syncthreads();
-            s_v[tid] += av;
-            s_t[tid] += at;
-            __syncthreads();
-        }
-        uint32_t bv = s_v[tid] - sv, bt = s_t[tid] - st;
-        for (int64_t r = r0; r < r1; r++) {
-            const uint32_t a = rv[r], b = rt[r];
-            rv[r] = bv;
-            rt[r] = bt;
-            bv += a;
-            bt += b;
+        // exclusive scan in tiles of kScanThreads consecutive rows (coalesced):
+        // wave scans, then the wave totals and the carry of the earlier tiles
+        const int ntiles = (int)((R + kScanThreads - 1) / kScanThreads);
+        uint32_t cv = 0, ct = 0;
+        int par = 0;
+        for (int g = 0; g < ntiles; g += kGroup) {
+            uint32_t av[kGroup], at[kGroup];
+#pragma unroll
+            for (int k = 0; k < kGroup; k++) {
+                const int64_t r = (int64_t)(g + k) * kScanThreads + tid;
+                av[k] = r < R ? rv[r] : 0u;
+                at[k] = r < R ? rt[r] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < kGroup; k++) {
+                if (g + k >= ntiles) break;  // uniform
+                uint32_t sv = av[k], sw = at[k];
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t uv = __shfl_up(sv, off), ut = __shfl_up(sw, off);
+                    if (lane >= off) {
+                        sv += uv;
+                        sw += ut;
+                    }
+                }
+                if (lane == 63) {
+                    s_wv[par][wave] = sv;
+                    s_wt[par][wave] = sw;
+                }
+                __syncthreads();
+                uint32_t bv = cv, bt = ct, tv = 0, tt = 0;
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    const uint32_t xv = s_wv[par][w], xt = s_wt[par][w];
+                    if (w < wave) {
+                        bv += xv;
+                        bt += xt;
+                    }
+                    tv += xv;
+                    tt += xt;
+                }
+                const int64_t r = (int64_t)(g + k) * kScanThreads + tid;
+                if (r < R) {
+                    rv[r] = bv + sv - av[k];
+                    rt[r] = bt + sw - at[k];
+                }
+                cv += tv;
+                ct += tt;
+                par ^= 1;
+            }
         }
         if (tid == 0) {
-            const uint32_t V = s_v[kScanThreads - 1], T = s_t[kScanThreads - 1];
+            const uint32_t V = cv, T = ct;
             M.V = (int32_t)V;
             M.T = (int32_t)T;
             M.contains_mesh = M.any_air && (M.any_solid || (dim % 32 != 0));
@@ -702,46 +752,57 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
 }
 
 // ------------------------------------------------------------------------
-// emit: one block per (chunk, x-plane), one wave per label row, lanes = z.
-// Vertices of the row's samples (ids by popcount prefix over the row's edge
-// masks), triangles of the row's interior cells (ids by a bit-sliced ballot
-// prefix of the per-cell triangle counts), and for chunks that smooth the
-// per-cell triangle base and mask.
+// emit: one block per (chunk, x-plane, row segment), lanes = z, in slabs of
+// at most kEmitRows label rows:
+//  pass 1, one wave per row of planes x and x+1 (rows y0 .. y0 + kEmitRows):
+//    the first vertex id of every sample (row base + popcount prefix over
+//    the row's x, y and z edge masks) with its x- and y-edge bits, into an
+//    LDS table; rows of plane x owned by the slab also write their vertices.
+//  pass 2, one wave per cell row: the triangles of the row's interior cells
+//    (ids by a bit-sliced ballot prefix of the per-cell triangle counts),
+//    their corner vertex ids looked up in the table, and for chunks that
+//    smooth the per-cell triangle base and mask.
 // ------------------------------------------------------------------------
+constexpr int kEmitRows = 16;
+
+// Local edge e of a cell (the 12-edge numbering of DMCChunk.cpp:543-565):
+// x-edges e = 2 dy + dz, y-edges e = 4 + 2 dx + dz, z-edges e = 8 + 2 dx + dy,
+// the edge of axis `ax` starting at corner (dx, dy, dz).
+__device__ __forceinline__ void edge_corner(int e, int& dx, int& dy, int& dz, int& ax) {
+    if (e < 4) {
+        ax = 0;
+        dx = 0;
+        dy = e >> 1;
+        dz = e & 1;
+    } else if (e < 8) {
+        ax = 1;
+        dx = (e - 4) >> 1;
+        dy = 0;
+        dz = e & 1;
+    } else {
+        ax = 2;
+        dx = (e - 8) >> 1;
+        dy = e & 1;
+        dz = 0;
+    }
+}
+
+// Pass 1 for label row (x + p, y): vertex ids into s_id, and for p == 0 with
+// `own` the vertices themselves (DMCChunk::calculate_isovertex, DMCChunk.cpp:
+// 657-674; vertices of a sample in x, y, z edge order).
 template <int W>
-__device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, const uint64_t* const* s_lab,
-                                         const uint8_t* s_ntri, const uint32_t (*s_base)[BMF_MAX_DIM + 1],
-                                         uint32_t (*s_vid)[64], int x, int y) {
+__device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M,
+                                             const uint64_t (*s_lab)[plane_words(W)], uint32_t base,
+                                             uint32_t* s_idrow, int x, int p, int y, bool own) {
     const int dim = M.dim;
     const int lane = lane_id();
-    const uint64_t lt = lanemask_lt();
-    const bool x1 = x + 1 < dim, x2 = x + 2 < dim, y1 = y + 1 < dim, y2 = y + 2 < dim;
-    const RowBits<W> R00 = lds_row<W>(s_lab[0], dim, y, true), R01 = lds_row<W>(s_lab[0], dim, y + 1, y1);
-    const RowBits<W> R10 = lds_row<W>(s_lab[1], dim, y, x1), R11 = lds_row<W>(s_lab[1], dim, y + 1, x1 && y1);
-    const bool tris = x < dim - 1 && y < dim - 1 && !quad_uniform<W>(R00, R01, R10, R11, dim);
-    const RowBits<W> EX00 = rxor<W>(R00, R10, x1), EY00 = rxor<W>(R00, R01, y1), EZ00 = zedges<W>(R00, dim);
-    const uint32_t nvr = total_pop<W>(EX00) + total_pop<W>(EY00) + total_pop<W>(EZ00);
-    if (nvr == 0 && !tris) return;
-    // edge masks of rows (x, y+1), (x+1, y), (x+1, y+1); only needed for triangles
-    RowBits<W> EX01 = zero_row<W>(), EY01 = zero_row<W>(), EZ01 = zero_row<W>();
-    RowBits<W> EX10 = zero_row<W>(), EY10 = zero_row<W>(), EZ10 = zero_row<W>();
-    RowBits<W> EX11 = zero_row<W>(), EY11 = zero_row<W>(), EZ11 = zero_row<W>();
-    if (tris) {  // here x + 1 < dim and y + 1 < dim
-        const RowBits<W> R02 = lds_row<W>(s_lab[0], dim, y + 2, y2), R12 = lds_row<W>(s_lab[1], dim, y + 2, y2);
-        const RowBits<W> R20 = lds_row<W>(s_lab[2], dim, y, x2), R21 = lds_row<W>(s_lab[2], dim, y + 1, x2);
-        EX01 = rxor<W>(R01, R11, true);
-        EY01 = rxor<W>(R01, R02, y2);
-        EZ01 = zedges<W>(R01, dim);
-        EX10 = rxor<W>(R10, R20, x2);
-        EY10 = rxor<W>(R10, R11, true);
-        EZ10 = zedges<W>(R10, dim);
-        EX11 = rxor<W>(R11, R21, x2);
-        EY11 = rxor<W>(R11, R12, y2);
-        EZ11 = zedges<W>(R11, dim);
-    }
-    const uint32_t vb00 = s_base[0][y], vb01 = s_base[0][y + 1], vb10 = s_base[1][y], vb11 = s_base[1][y + 1];
-    const int64_t vo = M.v_off, io = M.i_off;
-    const uint32_t Vc = (uint32_t)M.V, Tc = (uint32_t)M.T;
+    const int xp = x + p;
+    const RowBits<W> R0 = lds_row<W>(s_lab[p], dim, y, true);
+    const RowBits<W> EX = rxor<W>(R0, lds_row<W>(s_lab[p + 1], dim, y, xp + 1 < dim), xp + 1 < dim);
+    const RowBits<W> EY = rxor<W>(R0, lds_row<W>(s_lab[p], dim, y + 1, y + 1 < dim), y + 1 < dim);
+    const RowBits<W> EZ = zedges<W>(R0, dim);
+    const int64_t vo = M.v_off;
+    const uint32_t Vc = (uint32_t)M.V;
     float* __restrict__ P = A.pos + vo * 3;
     const bool cells = M.iters > 0 || A.normals;  // the chunk runs the smoothing kernels
     float4* __restrict__ P4 = cells ? A.pos4 + vo : nullptr;  // smoothing state
@@ -749,116 +810,112 @@ __device__ __forceinline__ void emit_row(const Arenas& A, const ChunkMeta& M, co
     uint8_t* __restrict__ B = A.boundary + vo;
     uint32_t* __restrict__ E = A.vedge + vo;
     const float* __restrict__ D = A.dens + M.dens_off;
-    uint32_t tcarry = s_base[2][y];
     for (int z0 = 0; z0 < dim; z0 += 64) {
         const int z = z0 + lane;
-        // ---- vertices owned by sample (x, y, z): x, y, z edges in order ----
-        if (z < dim) {
-            const uint32_t fx = bit_at<W>(EX00, z), fy = bit_at<W>(EY00, z), fz = bit_at<W>(EZ00, z);
-            if (fx | fy | fz) {
-                uint32_t id = vb00 + prefix_pop<W>(EX00, z) + prefix_pop<W>(EY00, z) + prefix_pop<W>(EZ00, z);
-                const float s0 = D[((int64_t)x * dim + y) * dim + z];
+        if (z >= dim) continue;
+        const uint32_t fx = bit_at<W>(EX, z), fy = bit_at<W>(EY, z), fz = bit_at<W>(EZ, z);
+        uint32_t id = base + prefix_pop<W>(EX, z) + prefix_pop<W>(EY, z) + prefix_pop<W>(EZ, z);
+        s_idrow[z] = id | (fx << 30) | (fy << 31);
+        if (!own || !(fx | fy | fz)) continue;
+        const float s0 = D[((int64_t)x * dim + y) * dim + z];
 #pragma unroll
-                for (int ax = 0; ax < 3; ax++) {
-                    const uint32_t f = ax == 0 ? fx : (ax == 1 ? fy : fz);
-                    if (!f) continue;
-                    if (id >= Vc) {  // counts and ids disagree: report, never write past the chunk
-                        A.ctr->error = 2;
+        for (int ax = 0; ax < 3; ax++) {
+            const uint32_t f = ax == 0 ? fx : (ax == 1 ? fy : fz);
+            if (!f) continue;
+            if (id >= Vc) {  // counts and ids disagree: report, never write past the chunk
+                A.ctr->error = 2;
+                break;
+            }
+            const int xe = x + (ax == 0), ye = y + (ax == 1), ze = z + (ax == 2);
+            const float s1 = D[((int64_t)xe * dim + ye) * dim + ze];
+            const float mu = (0.0f - s0) / (s1 - s0);
+            const float dx = (float)xe - (float)x, dy = (float)ye - (float)y, dz = (float)ze - (float)z;
+            const float qx = dx * mu + (float)x, qy = dy * mu + (float)y, qz = dz * mu + (float)z;
+            P[3 * (int64_t)id + 0] = qx;
+            P[3 * (int64_t)id + 1] = qy;
+            P[3 * (int64_t)id + 2] = qz;
+            if (P4) P4[id] = make_float4(qx, qy, qz, 0.0f);
+            if (N4) N4[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            B[id] = (uint8_t)(x == 0 || y == 0 || z == 0 || x == dim - 1 || y == dim - 1 || z == dim - 1 ||
+                              xe == dim - 1 || ye == dim - 1 || ze == dim - 1);
+            E[id] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | ((uint32_t)ax << 24);
+            id++;
+        }
+    }
+}
+
+// Pass 2 for cell row (x, y); s_id rows start at label row ys.
+template <int W>
+__device__ __forceinline__ void emit_tris_row(const Arenas& A, const ChunkMeta& M,
+                                              const uint64_t (*s_lab)[plane_words(W)], const uint8_t* s_ntri,
+                                              uint32_t tcarry,
+                                              const uint32_t (*s_id)[kEmitRows + 1][64 * W], int x, int y, int ys) {
+    const int dim = M.dim;
+    const int lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    const RowBits<W> R00 = lds_row<W>(s_lab[0], dim, y, true), R01 = lds_row<W>(s_lab[0], dim, y + 1, true);
+    const RowBits<W> R10 = lds_row<W>(s_lab[1], dim, y, true), R11 = lds_row<W>(s_lab[1], dim, y + 1, true);
+    const int64_t io = M.i_off;
+    const uint32_t Vc = (uint32_t)M.V, Tc = (uint32_t)M.T;
+    const bool cells = M.iters > 0 || A.normals;
+    for (int z0 = 0; z0 < dim - 1; z0 += 64) {
+        const int z = z0 + lane;
+        const uint32_t mask = z < dim - 1 ? cell_mask<W>(R00, R01, R10, R11, z) : 0u;
+        const uint32_t nt = s_ntri[mask];
+        const uint64_t b0 = __ballot(nt & 1u), b1 = __ballot(nt & 2u), b2 = __ballot(nt & 4u);
+        const uint32_t pre = (uint32_t)__popcll(b0 & lt) + 2u * (uint32_t)__popcll(b1 & lt) +
+                             4u * (uint32_t)__popcll(b2 & lt);
+        if (nt) {
+            const uint32_t tb = tcarry + pre;
+            const uint64_t row = c_tri_table[mask];
+            const int cnt = (int)(row >> 60);
+            if (tb + (uint32_t)(cnt / 3) > Tc) {
+                A.ctr->error = 3;
+            } else {
+                uint32_t* __restrict__ I = A.idx + io + 3ll * tb;
+                for (int s = 0; s < cnt; s++) {
+                    int dx, dy, dz, ax;
+                    edge_corner((int)((row >> (4 * s)) & 15ull), dx, dy, dz, ax);
+                    const uint32_t e = s_id[dx][y + dy - ys][z + dz];
+                    const uint32_t v = (e & 0x3FFFFFFFu) + (ax >= 1 ? (e >> 30) & 1u : 0u) + (ax == 2 ? e >> 31 : 0u);
+                    if (v >= Vc) {
+                        A.ctr->error = 4;
                         break;
                     }
-                    const int xe = x + (ax == 0), ye = y + (ax == 1), ze = z + (ax == 2);
-                    const float s1 = D[((int64_t)xe * dim + ye) * dim + ze];
-                    // DMCChunk::calculate_isovertex / _get_intersection (DMCChunk.cpp:657-674)
-                    const float mu = (0.0f - s0) / (s1 - s0);
-                    const float dx = (float)xe - (float)x, dy = (float)ye - (float)y, dz = (float)ze - (float)z;
-                    const float qx = dx * mu + (float)x, qy = dy * mu + (float)y, qz = dz * mu + (float)z;
-                    P[3 * (int64_t)id + 0] = qx;
-                    P[3 * (int64_t)id + 1] = qy;
-                    P[3 * (int64_t)id + 2] = qz;
-                    if (P4) P4[id] = make_float4(qx, qy, qz, 0.0f);
-                    if (N4) N4[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    B[id] = (uint8_t)(x == 0 || y == 0 || z == 0 || x == dim - 1 || y == dim - 1 || z == dim - 1 ||
-                                      xe == dim - 1 || ye == dim - 1 || ze == dim - 1);
-                    E[id] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | ((uint32_t)ax << 24);
-                    id++;
+                    I[s] = v;
+                }
+                if (cells) {
+                    const int64_t ci = M.dens_off + ((int64_t)x * dim + y) * dim + z;
+                    A.cell_tb[ci] = tb;
+                    A.cell_mask[ci] = (uint8_t)mask;
                 }
             }
         }
-        // ---- triangles of interior cell (x, y, z) ----
-        if (tris && z0 < dim - 1) {
-            const uint32_t mask = z < dim - 1 ? cell_mask<W>(R00, R01, R10, R11, z) : 0u;
-            const uint32_t nt = s_ntri[mask];
-            const uint64_t b0 = __ballot(nt & 1u), b1 = __ballot(nt & 2u), b2 = __ballot(nt & 4u);
-            const uint32_t pre = (uint32_t)__popcll(b0 & lt) + 2u * (uint32_t)__popcll(b1 & lt) +
-                                 4u * (uint32_t)__popcll(b2 & lt);
-            if (nt) {
-                // vertex ids of the 12 local edges (DMCChunk.cpp:543-565):
-                // x-edges e = 2*dy + dz on rows (x, y+dy); y-edges e = 4 + 2*dx + dz
-                // on rows (x+dx, y); z-edges e = 8 + 2*dx + dy on rows (x+dx, y+dy)
-#define BMF_VIDS(EX, EY, EZ, VB, DX, DY)                                                              \
-    {                                                                                                \
-        const uint32_t p0 = VB + prefix_pop<W>(EX, z) + prefix_pop<W>(EY, z) + prefix_pop<W>(EZ, z); \
-        const uint32_t ex0 = bit_at<W>(EX, z), ey0 = bit_at<W>(EY, z), ez0 = bit_at<W>(EZ, z);       \
-        const uint32_t p1 = p0 + ex0 + ey0 + ez0;                                                    \
-        if (DX == 0) {                                                                               \
-            s_vid[2 * DY + 0][lane] = p0;                                                            \
-            s_vid[2 * DY + 1][lane] = p1;                                                            \
-        }                                                                                            \
-        if (DY == 0) {                                                                               \
-            s_vid[4 + 2 * DX + 0][lane] = p0 + ex0;                                                  \
-            s_vid[4 + 2 * DX + 1][lane] = p1 + bit_at<W>(EX, z + 1);                                 \
-        }                                                                                            \
-        s_vid[8 + 2 * DX + DY][lane] = p0 + ex0 + ey0;                                               \
-    }
-                BMF_VIDS(EX00, EY00, EZ00, vb00, 0, 0)
-                BMF_VIDS(EX01, EY01, EZ01, vb01, 0, 1)
-                BMF_VIDS(EX10, EY10, EZ10, vb10, 1, 0)
-                BMF_VIDS(EX11, EY11, EZ11, vb11, 1, 1)
-#undef BMF_VIDS
-                const uint32_t tb = tcarry + pre;
-                const uint64_t row = c_tri_table[mask];
-                const int cnt = (int)(row >> 60);
-                if (tb + (uint32_t)(cnt / 3) > Tc) {
-                    A.ctr->error = 3;
-                } else {
-                    uint32_t* __restrict__ I = A.idx + io + 3ll * tb;
-                    for (int s = 0; s < cnt; s++) {
-                        const uint32_t v = s_vid[(row >> (4 * s)) & 15ull][lane];
-                        if (v >= Vc) {
-                            A.ctr->error = 4;
-                            break;
-                        }
-                        I[s] = v;
-                    }
-                    if (cells) {
-                        const int64_t ci = M.dens_off + ((int64_t)x * dim + y) * dim + z;
-                        A.cell_tb[ci] = tb;
-                        A.cell_mask[ci] = (uint8_t)mask;
-                    }
-                }
-            }
-            tcarry += (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
-        }
+        tcarry += (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
     }
 }
 
 template <int W>
 __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
+    constexpr int NW = kRowThreads / 64;
     __shared__ uint8_t s_ntri[256];
-    __shared__ uint64_t s_planes[3][plane_words(W)];
-    const uint64_t* const s_lab[3] = {s_planes[0], s_planes[1], s_planes[2]};
+    __shared__ uint64_t s_planes[3][plane_words(W)];  // labels of planes x, x+1, x+2
     __shared__ uint32_t s_base[3][BMF_MAX_DIM + 1];  // row vertex bases of planes x, x+1; triangle bases of plane x
-    __shared__ uint32_t s_vid[kRowThreads / 64][12][64];
+    __shared__ uint32_t s_id[2][kEmitRows + 1][64 * W];  // first vertex id | x-edge bit 30 | y-edge bit 31
     for (int t = threadIdx.x; t < 256; t += blockDim.x) s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
     const int wave = uniform_int(threadIdx.x >> 6);
-    const int64_t total = A.ctr->total_planes;
-    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const int j = find_chunk(A.plane_prefix, n, item);
+    const int split = A.ctr->plane_split;
+    const int64_t total = A.ctr->total_planes * split;
+    for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+        const int64_t item = it / split;
+        const int seg = (int)(it - item * split);
+        const int j = A.plane_chunk[item];
         const ChunkMeta& M = A.meta[j];
         const int dim = M.dim;
         if (((dim + 63) >> 6) != W) continue;
         if (M.status != BMF_OK || M.V == 0) continue;  // uniform across the block
         const int x = (int)(item - A.plane_prefix[j]);
+        const int y0 = (int)((int64_t)seg * dim / split), y1 = (int)((int64_t)(seg + 1) * dim / split);
         __syncthreads();  // previous item is done with the shared arrays
         stage_planes(s_planes[0], s_planes[1], s_planes[2], A.labels + M.lab_off, M.lab_words, dim, x);
         const uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
@@ -870,14 +927,29 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
         for (int y = threadIdx.x; y <= dim; y += blockDim.x) {
             const int64_t ra = r0 + y, rb = r0 + dim + y;
             s_base[0][y] = ra < R ? rv[y] : (uint32_t)M.V;
-            s_base[1][y] = y < dim ? (rb < R ? rv[dim + y] : (uint32_t)M.V) : 0u;
+            s_base[1][y] = rb < R ? rv[dim + y] : (uint32_t)M.V;
             s_base[2][y] = ra < R ? rt[y] : (uint32_t)M.T;
         }
-        __syncthreads();
-        for (int y = wave; y < dim; y += kRowThreads / 64) {
-            // rows that own no vertex and hold no triangle skip the label work
-            if (s_base[0][y + 1] == s_base[0][y] && s_base[2][y + 1] == s_base[2][y]) continue;
-            emit_row<W>(A, M, s_lab, s_ntri, s_base, s_vid[wave], x, y);
+        const bool cellsx = x < dim - 1;  // plane x has cells (and plane x+1 exists)
+        for (int ys = y0; ys < y1; ys += kEmitRows) {
+            const int ye = min(y1, ys + kEmitRows);         // cell / owned rows [ys, ye)
+            const int yl = min(dim - 1, ye);                // id rows [ys, yl]
+            __syncthreads();  // staging done; the previous slab is done with s_id
+            // pass 1: rows of plane x (and x+1 when it has cells); rows that own no
+            // vertex are never referenced by a triangle and skip the label work
+            const int nrow = yl - ys + 1;
+            for (int t = wave; t < (cellsx ? 2 : 1) * nrow; t += NW) {
+                const int p = t >= nrow ? 1 : 0, y = ys + (t - p * nrow);
+                if (s_base[p][y + 1] == s_base[p][y]) continue;
+                emit_ids_row<W>(A, M, s_planes, s_base[p][y], s_id[p][y - ys], x, p, y, p == 0 && y < ye);
+            }
+            if (!cellsx) continue;
+            __syncthreads();
+            // pass 2: cell rows with triangles
+            for (int y = ys + wave; y < ye && y < dim - 1; y += NW) {
+                if (s_base[2][y + 1] == s_base[2][y]) continue;
+                emit_tris_row<W>(A, M, s_planes, s_ntri, s_base[2][y], s_id, x, y, ys);
+            }
         }
     }
 }

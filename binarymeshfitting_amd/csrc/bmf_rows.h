@@ -218,6 +218,20 @@ __device__ __forceinline__ void stage_planes(uint64_t* p0, uint64_t* p1, uint64_
     stage_plane(p2, L, nwords, dim, x0 + 2);
 }
 
+// A wave-uniform row moved to scalar registers: the bit operations on it then
+// run on the scalar unit and it holds no vector registers.
+template <int W>
+__device__ __forceinline__ RowBits<W> uniform_row(const RowBits<W>& a) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a.w[k]);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a.w[k] >> 32));
+        r.w[k] = ((uint64_t)hi << 32) | lo;
+    }
+    return r;
+}
+
 // Row y of a staged plane.
 template <int W>
 __device__ __forceinline__ RowBits<W> lds_row(const uint64_t* P, int dim, int y, bool valid) {

@@ -227,6 +227,8 @@ typedef struct {
     int32_t sample_blocks;  /* persistent workgroups of the sampling kernel */
     int32_t row_blocks;     /* persistent workgroups of the plane kernels (count, refine, emit) */
     int32_t smooth_blocks;  /* workgroups of the smoothing kernels (rounded up to a multiple of 8) */
+    int32_t row_split;      /* row segments per x-plane item of the plane kernels: 1, 2, 4 or 8; -1
+                               doubles the split (up to 8) while sampled planes x split < 512 */
 } bmf_tuning;
 
 int bmf_get_tuning(bmf_ctx* ctx, bmf_tuning* tuning);

@@ -321,19 +321,21 @@ def test_tuning_shapes_give_identical_meshes(ctx):
 
     ref = run()
     try:
-        for tile, sb, rb, smb in [(1024, 512, 256, 256), (4096, 4096, 1024, 2048), (2048, 256, 4096, 8)]:
-            ctx.set_tuning(tile, sb, rb, smb)
+        for tile, sb, rb, smb, rs in [(1024, 512, 256, 256, 1), (4096, 4096, 1024, 2048, 8), (2048, 256, 4096, 8, 2),
+                                      (2048, 1024, 512, 1024, 4), (2048, 1024, 512, 1024, -1)]:
+            ctx.set_tuning(tile, sb, rb, smb, rs)
             got = run()
             assert len(got) == len(ref)
             for a, b in zip(ref, got):
                 for x, y in zip(a, b):
-                    assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (tile, sb, rb, smb)
+                    assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (tile, sb, rb, smb, rs)
     finally:
         ctx.set_tuning(**default)
 
 
 def test_set_tuning_rejects_bad_shapes(ctx):
-    for bad in [dict(sample_tile=1000), dict(sample_blocks=-1), dict(row_blocks=1 << 30)]:
+    for bad in [dict(sample_tile=1000), dict(sample_blocks=-1), dict(row_blocks=1 << 30), dict(row_split=3),
+                dict(row_split=16), dict(row_split=-2)]:
         with pytest.raises(capi.BmfError) as e:
             ctx.set_tuning(**bad)
         assert e.value.code == capi.EINVAL
