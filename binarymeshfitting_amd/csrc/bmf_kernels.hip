@@ -441,6 +441,7 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
     const int dim = M.dim;
     const int lane = lane_id();
     const bool x1 = x + 1 < dim, y1 = y + 1 < dim;
+    // wave-uniform rows (the same words in every lane); per lane only single bits
     const RowBits<W> r00 = lds_row<W>(s_lab[1], dim, y, true);
     const RowBits<W> r01 = lds_row<W>(s_lab[1], dim, y + 1, y1);
     const RowBits<W> r10 = lds_row<W>(s_lab[2], dim, y, x1);
@@ -459,9 +460,18 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
     const RowBits<W> ex = rxor<W>(r00, r10, x1), ey = rxor<W>(r00, r01, y1), ez = zedges<W>(r00, dim);
     nv = total_pop<W>(ex) + total_pop<W>(ey) + total_pop<W>(ez);
     if (x < dim - 1 && y < dim - 1 && !quad_uniform<W>(r00, r01, r10, r11, dim)) {
-        for (int z0 = 0; z0 < dim - 1; z0 += 64) {
-            const int z = z0 + lane;
-            const uint32_t c = z < dim - 1 ? s_ntri[cell_mask<W>(r00, r01, r10, r11, z)] : 0u;
+        const RowBits<W> s00 = rshr1<W>(r00), s01 = rshr1<W>(r01), s10 = rshr1<W>(r10), s11 = rshr1<W>(r11);
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (64 * k >= dim - 1) break;
+            const int z = 64 * k + lane;
+            // corner bit 4 dx + 2 dy + dz
+            const uint32_t c =
+                z < dim - 1 ? s_ntri[lane_bit(r00.w[k], lane) | (lane_bit(s00.w[k], lane) << 1) |
+                                     (lane_bit(r01.w[k], lane) << 2) | (lane_bit(s01.w[k], lane) << 3) |
+                                     (lane_bit(r10.w[k], lane) << 4) | (lane_bit(s10.w[k], lane) << 5) |
+                                     (lane_bit(r11.w[k], lane) << 6) | (lane_bit(s11.w[k], lane) << 7)]
+                            : 0u;
             nt += (uint32_t)__popcll(__ballot(c & 1u)) + 2u * (uint32_t)__popcll(__ballot(c & 2u)) +
                   4u * (uint32_t)__popcll(__ballot(c & 4u));
         }
@@ -476,18 +486,18 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
     for (int k = 0; k < W; k++) need.w[k] = ep.w[k] & ~done.w[k];
     if (total_pop<W>(need) == 0) return;
     // append them to this plane's region of the refine queue (an LDS counter
-    // per plane, no global atomics); k_refine evaluates them on dense waves
-    const uint64_t lt = lanemask_lt();
+    // per plane, no global atomics); k_refine evaluates them on dense waves.
+    // Bits past the row's last sample are clear, so word k of `need` is the
+    // set of lanes that append.
     const uint32_t rowbase = (uint32_t)(y * dim);
-    for (int z0 = 0; z0 < dim; z0 += 64) {
-        const int z = z0 + lane;
-        const bool want = z < dim && bit_at<W>(need, z);
-        const uint64_t b = __ballot(want);
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint64_t b = need.w[k];
         if (!b) continue;
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(s_qn, (uint32_t)__popcll(b));
         base = __shfl(base, 0);
-        if (want) q[base + __popcll(b & lt)] = rowbase + (uint32_t)z;
+        if (lane_bit(b, lane)) q[base + mbcnt64(b)] = rowbase + (uint32_t)(64 * k + lane);
     }
 }
 
@@ -767,29 +777,25 @@ constexpr int kEmitRows = 16;
 
 // Local edge e of a cell (the 12-edge numbering of DMCChunk.cpp:543-565):
 // x-edges e = 2 dy + dz, y-edges e = 4 + 2 dx + dz, z-edges e = 8 + 2 dx + dy,
-// the edge of axis `ax` starting at corner (dx, dy, dz).
-__device__ __forceinline__ void edge_corner(int e, int& dx, int& dy, int& dz, int& ax) {
-    if (e < 4) {
-        ax = 0;
-        dx = 0;
-        dy = e >> 1;
-        dz = e & 1;
-    } else if (e < 8) {
-        ax = 1;
-        dx = (e - 4) >> 1;
-        dy = 0;
-        dz = e & 1;
-    } else {
-        ax = 2;
-        dx = (e - 8) >> 1;
-        dy = e & 1;
-        dz = 0;
-    }
+// the edge of axis `ax` starting at corner (dx, dy, dz); 5 bits per edge,
+// dx | dy << 1 | dz << 2 | ax << 3.
+__host__ __device__ constexpr uint64_t edge_corner_code(int e) {
+    return e < 4 ? (uint64_t)(0 | ((e >> 1) << 1) | ((e & 1) << 2) | (0 << 3))
+                 : e < 8 ? (uint64_t)(((e - 4) >> 1) | (0 << 1) | ((e & 1) << 2) | (1 << 3))
+                         : (uint64_t)(((e - 8) >> 1) | ((e & 1) << 1) | (0 << 2) | (2 << 3));
 }
+__host__ __device__ constexpr uint64_t edge_corner_table() {
+    uint64_t t = 0;
+    for (int e = 0; e < 12; e++) t |= edge_corner_code(e) << (5 * e);
+    return t;
+}
+constexpr uint64_t kEdgeCorners = edge_corner_table();
 
 // Pass 1 for label row (x + p, y): vertex ids into s_id, and for p == 0 with
 // `own` the vertices themselves (DMCChunk::calculate_isovertex, DMCChunk.cpp:
-// 657-674; vertices of a sample in x, y, z edge order).
+// 657-674; vertices of a sample in x, y, z edge order).  The row masks are
+// wave-uniform (scalar registers); per lane only the bit and the count of
+// set bits below it are taken.
 template <int W>
 __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M,
                                              const uint64_t (*s_lab)[plane_words(W)], uint32_t base,
@@ -797,9 +803,10 @@ __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M
     const int dim = M.dim;
     const int lane = lane_id();
     const int xp = x + p;
-    const RowBits<W> R0 = lds_row<W>(s_lab[p], dim, y, true);
-    const RowBits<W> EX = rxor<W>(R0, lds_row<W>(s_lab[p + 1], dim, y, xp + 1 < dim), xp + 1 < dim);
-    const RowBits<W> EY = rxor<W>(R0, lds_row<W>(s_lab[p], dim, y + 1, y + 1 < dim), y + 1 < dim);
+    const bool vx = xp + 1 < dim, vy = y + 1 < dim;
+    const RowBits<W> R0 = uniform_row<W>(lds_row<W>(s_lab[p], dim, y, true));
+    const RowBits<W> EX = rxor<W>(R0, uniform_row<W>(lds_row<W>(s_lab[p + 1], dim, y, vx)), vx);
+    const RowBits<W> EY = rxor<W>(R0, uniform_row<W>(lds_row<W>(s_lab[p], dim, y + 1, vy)), vy);
     const RowBits<W> EZ = zedges<W>(R0, dim);
     const int64_t vo = M.v_off;
     const uint32_t Vc = (uint32_t)M.V;
@@ -810,37 +817,43 @@ __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M
     uint8_t* __restrict__ B = A.boundary + vo;
     uint32_t* __restrict__ E = A.vedge + vo;
     const float* __restrict__ D = A.dens + M.dens_off;
-    for (int z0 = 0; z0 < dim; z0 += 64) {
-        const int z = z0 + lane;
-        if (z >= dim) continue;
-        const uint32_t fx = bit_at<W>(EX, z), fy = bit_at<W>(EY, z), fz = bit_at<W>(EZ, z);
-        uint32_t id = base + prefix_pop<W>(EX, z) + prefix_pop<W>(EY, z) + prefix_pop<W>(EZ, z);
-        s_idrow[z] = id | (fx << 30) | (fy << 31);
-        if (!own || !(fx | fy | fz)) continue;
-        const float s0 = D[((int64_t)x * dim + y) * dim + z];
+    uint32_t kb = base;  // id of the first vertex of word k's samples
 #pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
-            const uint32_t f = ax == 0 ? fx : (ax == 1 ? fy : fz);
-            if (!f) continue;
-            if (id >= Vc) {  // counts and ids disagree: report, never write past the chunk
-                A.ctr->error = 2;
-                break;
+    for (int k = 0; k < W; k++) {
+        const uint64_t ex = EX.w[k], ey = EY.w[k], ez = EZ.w[k];
+        const int z = 64 * k + lane;
+        if (z < dim) {
+            const uint32_t fx = lane_bit(ex, lane), fy = lane_bit(ey, lane), fz = lane_bit(ez, lane);
+            uint32_t id = kb + mbcnt64(ex) + mbcnt64(ey) + mbcnt64(ez);
+            s_idrow[z] = id | (fx << 30) | (fy << 31);
+            if (own && (fx | fy | fz)) {
+                const float s0 = D[((int64_t)x * dim + y) * dim + z];
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++) {
+                    const uint32_t f = ax == 0 ? fx : (ax == 1 ? fy : fz);
+                    if (!f) continue;
+                    if (id >= Vc) {  // counts and ids disagree: report, never write past the chunk
+                        A.ctr->error = 2;
+                        break;
+                    }
+                    const int xe = x + (ax == 0), ye = y + (ax == 1), ze = z + (ax == 2);
+                    const float s1 = D[((int64_t)xe * dim + ye) * dim + ze];
+                    const float mu = (0.0f - s0) / (s1 - s0);
+                    const float dx = (float)xe - (float)x, dy = (float)ye - (float)y, dz = (float)ze - (float)z;
+                    const float qx = dx * mu + (float)x, qy = dy * mu + (float)y, qz = dz * mu + (float)z;
+                    P[3 * (int64_t)id + 0] = qx;
+                    P[3 * (int64_t)id + 1] = qy;
+                    P[3 * (int64_t)id + 2] = qz;
+                    if (P4) P4[id] = make_float4(qx, qy, qz, 0.0f);
+                    if (N4) N4[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    B[id] = (uint8_t)(x == 0 || y == 0 || z == 0 || x == dim - 1 || y == dim - 1 || z == dim - 1 ||
+                                      xe == dim - 1 || ye == dim - 1 || ze == dim - 1);
+                    E[id] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | ((uint32_t)ax << 24);
+                    id++;
+                }
             }
-            const int xe = x + (ax == 0), ye = y + (ax == 1), ze = z + (ax == 2);
-            const float s1 = D[((int64_t)xe * dim + ye) * dim + ze];
-            const float mu = (0.0f - s0) / (s1 - s0);
-            const float dx = (float)xe - (float)x, dy = (float)ye - (float)y, dz = (float)ze - (float)z;
-            const float qx = dx * mu + (float)x, qy = dy * mu + (float)y, qz = dz * mu + (float)z;
-            P[3 * (int64_t)id + 0] = qx;
-            P[3 * (int64_t)id + 1] = qy;
-            P[3 * (int64_t)id + 2] = qz;
-            if (P4) P4[id] = make_float4(qx, qy, qz, 0.0f);
-            if (N4) N4[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            B[id] = (uint8_t)(x == 0 || y == 0 || z == 0 || x == dim - 1 || y == dim - 1 || z == dim - 1 ||
-                              xe == dim - 1 || ye == dim - 1 || ze == dim - 1);
-            E[id] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | ((uint32_t)ax << 24);
-            id++;
         }
+        kb += (uint32_t)(__popcll(ex) + __popcll(ey) + __popcll(ez));
     }
 }
 
@@ -848,35 +861,44 @@ __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M
 template <int W>
 __device__ __forceinline__ void emit_tris_row(const Arenas& A, const ChunkMeta& M,
                                               const uint64_t (*s_lab)[plane_words(W)], const uint8_t* s_ntri,
-                                              uint32_t tcarry,
+                                              const uint64_t* s_tri, uint32_t tcarry,
                                               const uint32_t (*s_id)[kEmitRows + 1][64 * W], int x, int y, int ys) {
     const int dim = M.dim;
     const int lane = lane_id();
-    const uint64_t lt = lanemask_lt();
-    const RowBits<W> R00 = lds_row<W>(s_lab[0], dim, y, true), R01 = lds_row<W>(s_lab[0], dim, y + 1, true);
-    const RowBits<W> R10 = lds_row<W>(s_lab[1], dim, y, true), R11 = lds_row<W>(s_lab[1], dim, y + 1, true);
+    // corner rows of the cell row, and the same shifted by one sample (z + 1)
+    const RowBits<W> R00 = uniform_row<W>(lds_row<W>(s_lab[0], dim, y, true));
+    const RowBits<W> R01 = uniform_row<W>(lds_row<W>(s_lab[0], dim, y + 1, true));
+    const RowBits<W> R10 = uniform_row<W>(lds_row<W>(s_lab[1], dim, y, true));
+    const RowBits<W> R11 = uniform_row<W>(lds_row<W>(s_lab[1], dim, y + 1, true));
+    const RowBits<W> S00 = rshr1<W>(R00), S01 = rshr1<W>(R01), S10 = rshr1<W>(R10), S11 = rshr1<W>(R11);
     const int64_t io = M.i_off;
     const uint32_t Vc = (uint32_t)M.V, Tc = (uint32_t)M.T;
     const bool cells = M.iters > 0 || A.normals;
-    for (int z0 = 0; z0 < dim - 1; z0 += 64) {
-        const int z = z0 + lane;
-        const uint32_t mask = z < dim - 1 ? cell_mask<W>(R00, R01, R10, R11, z) : 0u;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        if (64 * k >= dim - 1) break;
+        const int z = 64 * k + lane;
+        // corner bit 4 dx + 2 dy + dz (DMCChunk.cpp:200-438)
+        const uint32_t mask =
+            z < dim - 1 ? lane_bit(R00.w[k], lane) | (lane_bit(S00.w[k], lane) << 1) | (lane_bit(R01.w[k], lane) << 2) |
+                              (lane_bit(S01.w[k], lane) << 3) | (lane_bit(R10.w[k], lane) << 4) |
+                              (lane_bit(S10.w[k], lane) << 5) | (lane_bit(R11.w[k], lane) << 6) |
+                              (lane_bit(S11.w[k], lane) << 7)
+                        : 0u;
         const uint32_t nt = s_ntri[mask];
         const uint64_t b0 = __ballot(nt & 1u), b1 = __ballot(nt & 2u), b2 = __ballot(nt & 4u);
-        const uint32_t pre = (uint32_t)__popcll(b0 & lt) + 2u * (uint32_t)__popcll(b1 & lt) +
-                             4u * (uint32_t)__popcll(b2 & lt);
         if (nt) {
-            const uint32_t tb = tcarry + pre;
-            const uint64_t row = c_tri_table[mask];
+            const uint32_t tb = tcarry + mbcnt64(b0) + 2u * mbcnt64(b1) + 4u * mbcnt64(b2);
+            const uint64_t row = s_tri[mask];
             const int cnt = (int)(row >> 60);
             if (tb + (uint32_t)(cnt / 3) > Tc) {
                 A.ctr->error = 3;
             } else {
                 uint32_t* __restrict__ I = A.idx + io + 3ll * tb;
                 for (int s = 0; s < cnt; s++) {
-                    int dx, dy, dz, ax;
-                    edge_corner((int)((row >> (4 * s)) & 15ull), dx, dy, dz, ax);
-                    const uint32_t e = s_id[dx][y + dy - ys][z + dz];
+                    const uint32_t g = (uint32_t)(kEdgeCorners >> (5 * ((row >> (4 * s)) & 15ull))) & 31u;
+                    const uint32_t e = s_id[g & 1u][y + (int)((g >> 1) & 1u) - ys][z + (int)((g >> 2) & 1u)];
+                    const uint32_t ax = g >> 3;
                     const uint32_t v = (e & 0x3FFFFFFFu) + (ax >= 1 ? (e >> 30) & 1u : 0u) + (ax == 2 ? e >> 31 : 0u);
                     if (v >= Vc) {
                         A.ctr->error = 4;
@@ -902,7 +924,11 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
     __shared__ uint64_t s_planes[3][plane_words(W)];  // labels of planes x, x+1, x+2
     __shared__ uint32_t s_base[3][BMF_MAX_DIM + 1];  // row vertex bases of planes x, x+1; triangle bases of plane x
     __shared__ uint32_t s_id[2][kEmitRows + 1][64 * W];  // first vertex id | x-edge bit 30 | y-edge bit 31
-    for (int t = threadIdx.x; t < 256; t += blockDim.x) s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
+    __shared__ uint64_t s_tri[256];                       // the triangle table
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
+        s_tri[t] = c_tri_table[t];
+        s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
+    }
     const int wave = uniform_int(threadIdx.x >> 6);
     const int split = A.ctr->plane_split;
     const int64_t total = A.ctr->total_planes * split;
@@ -948,7 +974,7 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
             // pass 2: cell rows with triangles
             for (int y = ys + wave; y < ye && y < dim - 1; y += NW) {
                 if (s_base[2][y + 1] == s_base[2][y]) continue;
-                emit_tris_row<W>(A, M, s_planes, s_ntri, s_base[2][y], s_id, x, y, ys);
+                emit_tris_row<W>(A, M, s_planes, s_ntri, s_tri, s_base[2][y], s_id, x, y, ys);
             }
         }
     }

@@ -260,6 +260,24 @@ __device__ __forceinline__ RowBits<W> rshl1(const RowBits<W>& a) {
     return r;
 }
 
+// bit z of the result = bit z + 1 of a (0 past the row)
+template <int W>
+__device__ __forceinline__ RowBits<W> rshr1(const RowBits<W>& a) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) r.w[k] = (a.w[k] >> 1) | (k + 1 < W ? (a.w[k + 1] << 63) : 0ull);
+    return r;
+}
+
+// Lane-wise views of a wave-uniform 64-bit word whose bit i belongs to lane
+// i (word k of a row, for samples z = 64 k + lane): the lane's bit, and the
+// number of set bits below it (v_mbcnt).
+__device__ __forceinline__ uint32_t lane_bit(uint64_t w, int lane) { return (uint32_t)(w >> lane) & 1u; }
+
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t w) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(w >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)w, 0u));
+}
+
 template <int W>
 __device__ __forceinline__ RowBits<W> ror(const RowBits<W>& a, const RowBits<W>& b) {
     RowBits<W> r;
