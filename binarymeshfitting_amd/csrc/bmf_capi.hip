@@ -56,7 +56,7 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
     BMF_MARK(0);
     BMF_HIP(launch_plan(A, n, allow_cull, reset, s));
     BMF_MARK(1);
-    BMF_HIP(launch_sample(A, n, sample_blocks, 1, s));
+    BMF_HIP(launch_sample(A, n, sample_blocks, 1, c->h_sampler.kind, s));
     BMF_MARK(2);
     const bool quads = A.mesh_mode == BMF_MESH_QUADS;
     BMF_HIP(launch_count(A, n, row_blocks, wclasses, s));
@@ -478,7 +478,7 @@ static int stage_one(bmf_ctx* c, const bmf_chunk_desc* d, bool scan) {
     const Arenas& A = c->A;
     BMF_HIP(launch_plan(A, 1, 0, 1, c->stream));
     BMF_HIP(launch_sample(A, 1, (int)std::min<int64_t>(ceil_div(ns, A.sample_tile), c->persistent_blocks), 0,
-                          c->stream));
+                          c->h_sampler.kind, c->stream));
     if (scan) {
         BMF_HIP(launch_count(A, 1, d->dim, row_word_class(d->dim), c->stream));
         BMF_HIP(launch_scan(A, 1, 1, c->stream));

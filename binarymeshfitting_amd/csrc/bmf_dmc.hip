@@ -156,6 +156,49 @@ __device__ void csg_gradient(const SamplerDev& S, const uint16_t* list, int ns, 
     }
 }
 
+// The same with the chunk's primitives staged in LDS (stage_csg layout).
+__device__ void csg_gradient_lds(const float4* s_csg, int ns, int nb, float x, float y, float z, float g[3]) {
+    float best = kCsgFloor;
+    g[0] = g[1] = g[2] = 0.0f;
+    for (int k = 0; k < ns; k++) {
+        const float4 sp = s_csg[2 * k];
+        const float dx = x - sp.x, dy = y - sp.y, dz = z - sp.z;
+        const float l = sqrtf((dx * dx + dy * dy) + dz * dz);
+        const float v = sp.w - l;
+        if (v > best) {
+            best = v;
+            const float inv = l > 0.0f ? 1.0f / l : 0.0f;
+            g[0] = -dx * inv;
+            g[1] = -dy * inv;
+            g[2] = -dz * inv;
+        }
+    }
+    for (int k = ns; k < ns + nb; k++) {
+        const float4 b0 = s_csg[2 * k], b1 = s_csg[2 * k + 1];
+        const float px = x - b0.x, py = y - b0.y, pz = z - b0.z;
+        const float qx = fabsf(px) - b0.w, qy = fabsf(py) - b1.x, qz = fabsf(pz) - b1.y;
+        const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+        const float outside = sqrtf((mx * mx + my * my) + mz * mz);
+        const float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+        const float v = -(outside + inside);
+        if (v > best) {
+            best = v;
+            const float sx = px < 0.0f ? -1.0f : 1.0f, sy = py < 0.0f ? -1.0f : 1.0f, sz = pz < 0.0f ? -1.0f : 1.0f;
+            if (outside > 0.0f) {
+                const float inv = 1.0f / outside;
+                g[0] = -mx * sx * inv;
+                g[1] = -my * sy * inv;
+                g[2] = -mz * sz * inv;
+            } else {
+                g[0] = g[1] = g[2] = 0.0f;
+                if (qx >= qy && qx >= qz) g[0] = -sx;
+                else if (qy >= qz) g[1] = -sy;
+                else g[2] = -sz;
+            }
+        }
+    }
+}
+
 // QEF of `count` (point, unit normal) pairs (qef_simd.h:392-579): normal
 // equations about the mass point, 5 Jacobi sweeps over the pairs (0,1),
 // (0,2), (1,2) with an exact 1/sqrt Givens cosine, 1/sigma kept when
@@ -312,8 +355,9 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_count(Arenas A, int n) {
 // range, so a dense pass of the whole block then places them (mass point or
 // QEF), free of the row pass's idle lanes.
 // ------------------------------------------------------------------------
-__device__ void place_vertex(const Arenas& A, const ChunkMeta& M, const SamplerDev& S, const uint16_t* csg, int j,
-                             int x, uint32_t code, uint32_t mask, bool qef, float out[3], float& err) {
+__device__ void place_vertex(const Arenas& A, const ChunkMeta& M, const SamplerDev& S, const uint16_t* csg,
+                             const float4* s_csg, int x, uint32_t code, uint32_t mask, bool qef, float out[3],
+                             float& err) {
     const int dim = M.dim;
     const int y = (int)((code >> 8) & 255u), z = (int)((code >> 16) & 255u), p = (int)(code >> 24);
     const float* __restrict__ D = A.dens + M.dens_off;
@@ -358,7 +402,8 @@ __device__ void place_vertex(const Arenas& A, const ChunkMeta& M, const SamplerD
             const float wx = q3[i][0] * M.delta + M.opos[0], wy = q3[i][1] * M.delta + M.opos[1],
                         wz = q3[i][2] * M.delta + M.opos[2];
             float g[3];
-            csg_gradient(S, csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
+            if (s_csg) csg_gradient_lds(s_csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
+            else csg_gradient(S, csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
             const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
             for (int c = 0; c < 3; c++) nrm[i][c] = len > 0.0f ? g[c] / len : 0.0f;
         }
@@ -375,6 +420,7 @@ template <int W>
 __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
     __shared__ DmcTables s_t;
     __shared__ uint64_t s_planes[2][plane_words(W)];
+    __shared__ float4 s_csg[2 * kCsgStage];  // the chunk's culled CSG primitives (QEF normals)
     load_dmc_tables(s_t);
     const SamplerDev* __restrict__ S = A.sampler;
     const int wave = uniform_int(threadIdx.x >> 6);
@@ -389,15 +435,18 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
         if (M.status != BMF_OK || M.V == 0) continue;
         const int x = (int)(item - A.plane_prefix[j]);
         if (x >= dim - 1) continue;
+        const bool qef = A.placement == BMF_PLACE_QEF && S->kind == BMF_SAMPLER_CSG;
+        const uint16_t* csg = A.csg_list ? A.csg_list + (int64_t)j * A.csg_stride : nullptr;
+        const bool staged = qef && csg && M.csg_ns + M.csg_nb <= kCsgStage;  // uniform across the block
         __syncthreads();
         stage_plane(s_planes[0], A.labels + M.lab_off, M.lab_words, dim, x);
         stage_plane(s_planes[1], A.labels + M.lab_off, M.lab_words, dim, x + 1);
+        if (staged) stage_csg(*S, csg, M.csg_ns, M.csg_nb, s_csg);
         __syncthreads();
         const uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
         const int64_t vo = M.v_off;
         const uint32_t Vc = (uint32_t)M.V;
         const bool smooth = M.iters > 0 && M.T > 0;
-        const bool qef = A.placement == BMF_PLACE_QEF && S->kind == BMF_SAMPLER_CSG;
         // ---- row pass ----
         for (int y = wave; y < dim - 1; y += kRowThreads / 64) {
             const RowBits<W> r00 = lds_row<W>(s_planes[0], dim, y, true);
@@ -434,13 +483,12 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
         __syncthreads();  // the plane's vertex codes are written
         // ---- dense placement of the plane's vertices [rv[0], next plane's first) ----
         const uint32_t v0 = rv[0], v1 = min(x + 1 < dim ? rv[dim] : Vc, Vc);
-        const uint16_t* csg = A.csg_list ? A.csg_list + (int64_t)j * A.csg_stride : nullptr;
         for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
             const uint32_t code = A.vedge[vo + v];
             const int y = (int)((code >> 8) & 255u), z = (int)((code >> 16) & 255u);
             const uint32_t mask = A.cell_mask[M.dens_off + ((int64_t)x * dim + y) * dim + z];
             float o[3], err;
-            place_vertex(A, M, *S, csg, j, x, code, mask, qef, o, err);
+            place_vertex(A, M, *S, csg, staged ? s_csg : nullptr, x, code, mask, qef, o, err);
             float* P = A.pos + (vo + v) * 3;
             P[0] = o[0];
             P[1] = o[1];

@@ -150,7 +150,8 @@ inline int row_word_class(int dim) { return 1 << (((dim + 63) >> 6) - 1); }
 // Launch wrappers (bmf_kernels.hip).
 hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, hipStream_t s);
 // adaptive: terrain samples stop adding octaves once their label is decided
-hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, hipStream_t s);
+// kind: the sampler the device struct holds for this launch (BMF_SAMPLER_*)
+hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int kind, hipStream_t s);
 // wclasses: bit W-1 set when the batch holds chunks with ceil(dim/64) == W.
 hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s);

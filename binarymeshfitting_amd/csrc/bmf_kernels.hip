@@ -269,13 +269,23 @@ __device__ __forceinline__ void sample_coords(const ChunkMeta& M, uint32_t i, ui
     pz = (float)z * M.delta + M.opos[2];
 }
 
-template <int ROUNDS>
-__global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int adaptive) {
+// Sampling modes, fixed per launch by the host: adaptive octaves for the
+// terrain sampler; the CSG union with the chunk's culled primitives staged in
+// LDS once per tile; every other case (external densities, implicit shapes,
+// the terrain in full) evaluated directly.
+constexpr int kSampleGeneric = 0, kSampleAdaptive = 1, kSampleCsg = 2;
+
+template <int ROUNDS, int MODE>
+__global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
     constexpr int kTile = ROUNDS * kSampleThreads;
+    constexpr bool adapt = MODE == kSampleAdaptive;
+    constexpr int kQ = adapt ? kTile : 1;                       // octave queues (adaptive mode only)
+    constexpr int kC = MODE == kSampleCsg ? 2 * kCsgStage : 1;  // staged CSG primitives
     __shared__ int64_t s_tile;
     __shared__ uint32_t s_qn[2];
-    __shared__ uint16_t s_qi[2][kTile];
-    __shared__ float s_qs[2][kTile];
+    __shared__ uint16_t s_qi[2][kQ];
+    __shared__ float s_qs[2][kQ];
+    __shared__ float4 s_csg[kC];
     __shared__ unsigned long long s_lab[kTile / 64];
     __shared__ unsigned long long s_exact[kTile / 64];
     __shared__ uint32_t s_flags;
@@ -283,7 +293,6 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int 
     const int64_t total = A.ctr->total_tiles;
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
-    const bool adapt = adaptive && S->kind == BMF_SAMPLER_TERRAIN;
     // the chunk of every x-plane item of the plane kernels that follow
     for (int j = blockIdx.x; j < n; j += gridDim.x) {
         const ChunkMeta& M = A.meta[j];
@@ -307,6 +316,14 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int 
         const float rd = 1.0f / (float)dim, rd2 = 1.0f / (float)dim2;
         float* __restrict__ dens = A.dens + M.dens_off;
         const uint32_t first = (uint32_t)(tile - A.tile_prefix[j]) * kTile;
+        bool staged = false;
+        if constexpr (MODE == kSampleCsg) {
+            staged = M.csg_ns + M.csg_nb <= kCsgStage;  // uniform across the block
+            if (staged) {
+                stage_csg(*S, A.csg_list + (int64_t)j * A.csg_stride, M.csg_ns, M.csg_nb, s_csg);
+                __syncthreads();
+            }
+        }
         // ---- phase 0: every sample of the tile ----
 #pragma unroll 1
         for (int r = 0; r < ROUNDS; r++) {
@@ -317,14 +334,15 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int 
             if (valid) {
                 float px, py, pz;
                 sample_coords(M, i, dim, dim2, rd, rd2, px, py, pz);
-                if (adapt) {
+                if constexpr (adapt) {
                     const float t2 = py - S->y0;
                     const int dec = decide_label(*S, 0, 0.0f, t2);
                     done = dec != 0;
                     air = dec == 1;
                 } else {
                     float d;
-                    if (S->kind == BMF_SAMPLER_EXTERNAL) d = S->ext[A.ext_base + M.dens_off + i];
+                    if (staged) d = csg_density_lds(s_csg, M.csg_ns, M.csg_nb, px, py, pz);
+                    else if (S->kind == BMF_SAMPLER_EXTERNAL) d = S->ext[A.ext_base + M.dens_off + i];
                     else if (S->kind == BMF_SAMPLER_CSG && A.csg_list)
                         d = csg_density_list(*S, A.csg_list + (int64_t)j * A.csg_stride, M.csg_ns, M.csg_nb, px, py,
                                              pz);
@@ -340,20 +358,22 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n, int 
                 s_lab[li >> 6] = bl;
                 s_exact[li >> 6] = adapt ? 0ull : vb;
             }
-            if (push) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&s_qn[0], (uint32_t)__popcll(push));
-                base = __shfl(base, 0);
-                if (valid && !done) {
-                    const uint32_t q = base + (uint32_t)__popcll(push & lt);
-                    s_qi[0][q] = (uint16_t)li;
-                    s_qs[0][q] = 0.0f;
+            if constexpr (adapt) {
+                if (push) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&s_qn[0], (uint32_t)__popcll(push));
+                    base = __shfl(base, 0);
+                    if (valid && !done) {
+                        const uint32_t q = base + (uint32_t)__popcll(push & lt);
+                        s_qi[0][q] = (uint16_t)li;
+                        s_qs[0][q] = 0.0f;
+                    }
                 }
             }
         }
         __syncthreads();
         // ---- octave rounds on the compacted queue ----
-        if (adapt) {
+        if constexpr (adapt) {
             const int O = S->octaves;
             for (int k = 0; k < O; k++) {
                 const int cur = k & 1, nxt = cur ^ 1;
@@ -1331,13 +1351,24 @@ hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, 
     if (A.csg_list) hipLaunchKernelGGL(k_csg_cull, dim3((unsigned)(n < 2048 ? n : 2048)), dim3(256), 0, s, A, n);
     return hipGetLastError();
 }
-hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, hipStream_t s) {
+hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int kind, hipStream_t s) {
+    const int mode = adaptive && kind == BMF_SAMPLER_TERRAIN ? kSampleAdaptive
+                     : kind == BMF_SAMPLER_CSG && A.csg_list  ? kSampleCsg
+                                                              : kSampleGeneric;
+    const dim3 g(blocks), b(kSampleThreads);
+#define BMF_SAMPLE(R)                                                                             \
+    switch (mode) {                                                                               \
+        case kSampleAdaptive: hipLaunchKernelGGL((k_sample<R, kSampleAdaptive>), g, b, 0, s, A, n); break; \
+        case kSampleCsg: hipLaunchKernelGGL((k_sample<R, kSampleCsg>), g, b, 0, s, A, n); break;           \
+        default: hipLaunchKernelGGL((k_sample<R, kSampleGeneric>), g, b, 0, s, A, n); break;               \
+    }
     switch (A.sample_tile) {
-        case 1024: hipLaunchKernelGGL(k_sample<4>, dim3(blocks), dim3(kSampleThreads), 0, s, A, n, adaptive); break;
-        case 2048: hipLaunchKernelGGL(k_sample<8>, dim3(blocks), dim3(kSampleThreads), 0, s, A, n, adaptive); break;
-        case 4096: hipLaunchKernelGGL(k_sample<16>, dim3(blocks), dim3(kSampleThreads), 0, s, A, n, adaptive); break;
+        case 1024: BMF_SAMPLE(4) break;
+        case 2048: BMF_SAMPLE(8) break;
+        case 4096: BMF_SAMPLE(16) break;
         default: return hipErrorInvalidValue;
     }
+#undef BMF_SAMPLE
     return hipGetLastError();
 }
 hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {

@@ -185,6 +185,46 @@ __device__ __forceinline__ float csg_density_list(const SamplerDev& S, const uin
     return best;
 }
 
+// Primitives of a chunk's culled list staged in LDS: two float4 per
+// primitive, (cx, cy, cz, r) and unused for a sphere, (cx, cy, cz, hx) and
+// (hy, hz, -, -) for a box; spheres first.  Same union, same operation order.
+constexpr int kCsgStage = 256;  // primitives per chunk staged in LDS (larger lists read global memory)
+
+__device__ __forceinline__ void stage_csg(const SamplerDev& S, const uint16_t* __restrict__ list, int ns, int nb,
+                                          float4* s_csg) {
+    for (int k = threadIdx.x; k < ns + nb; k += blockDim.x) {
+        if (k < ns) {
+            const float* sp = S.spheres + 4 * list[k];
+            s_csg[2 * k] = make_float4(sp[0], sp[1], sp[2], sp[3]);
+            s_csg[2 * k + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+            const float* b = S.boxes + 6 * list[k];
+            s_csg[2 * k] = make_float4(b[0], b[1], b[2], b[3]);
+            s_csg[2 * k + 1] = make_float4(b[4], b[5], 0.0f, 0.0f);
+        }
+    }
+}
+
+__device__ __forceinline__ float csg_density_lds(const float4* s_csg, int ns, int nb, float x, float y, float z) {
+    float best = kCsgFloor;
+    for (int k = 0; k < ns; k++) {
+        const float4 sp = s_csg[2 * k];
+        float dx = x - sp.x, dy = y - sp.y, dz = z - sp.z;
+        float l = sqrtf((dx * dx + dy * dy) + dz * dz);
+        best = fmaxf(best, sp.w - l);
+    }
+    for (int k = ns; k < ns + nb; k++) {
+        const float4 b0 = s_csg[2 * k], b1 = s_csg[2 * k + 1];
+        float px = x - b0.x, py = y - b0.y, pz = z - b0.z;
+        float qx = fabsf(px) - b0.w, qy = fabsf(py) - b1.x, qz = fabsf(pz) - b1.y;
+        float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+        float outside = sqrtf((mx * mx + my * my) + mz * mz);
+        float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+        best = fmaxf(best, -(outside + inside));
+    }
+    return best;
+}
+
 __device__ __forceinline__ float sample_density(const SamplerDev& S, float x, float y, float z) {
     if (S.kind == BMF_SAMPLER_TERRAIN) return terrain_density(S, x, y, z);
     if (S.kind == BMF_SAMPLER_CSG) return csg_density(S, x, y, z);
