@@ -203,27 +203,36 @@ __device__ void csg_gradient_lds(const float4* s_csg, int ns, int nb, float x, f
 // equations about the mass point, 5 Jacobi sweeps over the pairs (0,1),
 // (0,2), (1,2) with an exact 1/sqrt Givens cosine, 1/sigma kept when
 // min(|sigma|, |1/sigma|) >= tol, residual with the uncentred ATb.
-__device__ float qef_solve(const float (*p)[3], const float (*n)[3], int count, float tol, float out[3]) {
+// The sums are accumulated one pair at a time in pair order (qef_add), then
+// solved (qef_finish); no pair is kept.
+struct QefSums {
+    float ata[3][3];
+    float atb[3];
+    float pa[4];
+};
+
+__device__ __forceinline__ void qef_add(QefSums& q, float px, float py, float pz, const float nn[3]) {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) q.ata[r][c] = q.ata[r][c] + nn[r] * nn[c];
+    const float d = (px * nn[0] + py * nn[1]) + pz * nn[2];
+#pragma unroll
+    for (int r = 0; r < 3; r++) q.atb[r] = q.atb[r] + d * nn[r];
+    q.pa[0] = q.pa[0] + px;
+    q.pa[1] = q.pa[1] + py;
+    q.pa[2] = q.pa[2] + pz;
+    q.pa[3] = q.pa[3] + 1.0f;
+}
+
+__device__ float qef_finish(const QefSums& q, int count, float tol, float out[3]) {
     if (count < 2 || count > 12) {
         out[0] = out[1] = out[2] = 0.0f;
         return 0.0f;
     }
-    float ata[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, atb[3] = {0, 0, 0}, pa[4] = {0, 0, 0, 0};
-    for (int i = 0; i < count; i++) {
-        const float px = p[i][0], py = p[i][1], pz = p[i][2];
-        const float nn[3] = {n[i][0], n[i][1], n[i][2]};
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) ata[r][c] = ata[r][c] + nn[r] * nn[c];
-        const float d = (px * nn[0] + py * nn[1]) + pz * nn[2];
-#pragma unroll
-        for (int r = 0; r < 3; r++) atb[r] = atb[r] + d * nn[r];
-        pa[0] = pa[0] + px;
-        pa[1] = pa[1] + py;
-        pa[2] = pa[2] + pz;
-        pa[3] = pa[3] + 1.0f;
-    }
+    const float(&ata)[3][3] = q.ata;
+    const float(&atb)[3] = q.atb;
+    const float(&pa)[4] = q.pa;
     const float mp[3] = {pa[0] / pa[3], pa[1] / pa[3], pa[2] / pa[3]};
     float rhs[3];
 #pragma unroll
@@ -367,7 +376,7 @@ __device__ void place_vertex(const Arenas& A, const ChunkMeta& M, const SamplerD
     int s = 0;
     for (int q = 0; q < p && s < nn; s++)
         if (((row >> (4 * s)) & 15ull) == 15ull) q++;
-    float q3[7][3];
+    QefSums qs = {{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, {0, 0, 0}, {0, 0, 0, 0}};
     int k = 0;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     for (; s < nn; s++) {
@@ -381,12 +390,21 @@ __device__ void place_vertex(const Arenas& A, const ChunkMeta& M, const SamplerD
         const float mu = (0.0f - s0) / (s1 - s0);
         const float dx = (float)x1 - (float)x0, dy = (float)y1 - (float)y0, dz = (float)z1 - (float)z0;
         if (k < 7) {
-            q3[k][0] = dx * mu + (float)x0;
-            q3[k][1] = dy * mu + (float)y0;
-            q3[k][2] = dz * mu + (float)z0;
-            ax = ax + q3[k][0];
-            ay = ay + q3[k][1];
-            az = az + q3[k][2];
+            const float qx = dx * mu + (float)x0, qy = dy * mu + (float)y0, qz = dz * mu + (float)z0;
+            ax = ax + qx;
+            ay = ay + qy;
+            az = az + qz;
+            if (qef) {
+                // unit normal of the CSG field at the point (world coordinates)
+                const float wx = qx * M.delta + M.opos[0], wy = qy * M.delta + M.opos[1], wz = qz * M.delta + M.opos[2];
+                float g[3];
+                if (s_csg) csg_gradient_lds(s_csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
+                else csg_gradient(S, csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
+                const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+                float nrm[3];
+                for (int c = 0; c < 3; c++) nrm[c] = len > 0.0f ? g[c] / len : 0.0f;
+                qef_add(qs, qx, qy, qz, nrm);
+            }
         }
         k++;
     }
@@ -397,17 +415,7 @@ __device__ void place_vertex(const Arenas& A, const ChunkMeta& M, const SamplerD
         return;
     }
     if (qef) {
-        float nrm[7][3];
-        for (int i = 0; i < k; i++) {
-            const float wx = q3[i][0] * M.delta + M.opos[0], wy = q3[i][1] * M.delta + M.opos[1],
-                        wz = q3[i][2] * M.delta + M.opos[2];
-            float g[3];
-            if (s_csg) csg_gradient_lds(s_csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
-            else csg_gradient(S, csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
-            const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
-            for (int c = 0; c < 3; c++) nrm[i][c] = len > 0.0f ? g[c] / len : 0.0f;
-        }
-        err = qef_solve(q3, nrm, k, A.qef_tol, out);
+        err = qef_finish(qs, k, A.qef_tol, out);
     } else {
         const float fk = (float)k;
         out[0] = ax / fk;
