@@ -61,7 +61,6 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
     const bool quads = A.mesh_mode == BMF_MESH_QUADS;
     BMF_HIP(launch_count(A, n, row_blocks, wclasses, s));
     BMF_HIP(launch_refine(A, n, row_blocks, s));
-    if (quads) BMF_HIP(launch_dmc_count(A, n, row_blocks, wclasses, s));
     BMF_MARK(3);
     BMF_HIP(launch_scan(A, n, scan_blocks, s));
     BMF_MARK(4);

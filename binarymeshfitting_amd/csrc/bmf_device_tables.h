@@ -37,6 +37,21 @@ static __constant__ EdgeTriTable c_edge_tris = make_edge_tris();
 // 0xF between polygons.
 static __constant__ uint64_t c_dmc_table[256] = BMF_DMC_POLY_TABLE_INIT;
 
+// Polygons of one c_dmc_table row (edge lists separated by 15).
+__device__ __forceinline__ uint32_t dmc_polygons(uint64_t row) {
+    const int n = (int)(row >> 60);
+    uint32_t p = 0, any = 0;
+    for (int s = 0; s < n; s++) {
+        if (((row >> (4 * s)) & 15ull) == 15ull) {
+            p++;
+            any = 0;
+        } else {
+            any = 1;
+        }
+    }
+    return p + any;
+}
+
 // The four cells around a lattice edge along `axis`, in lexicographic cell
 // order, as (dx, dy, dz, local edge) offsets from the edge origin; local
 // edges numbered as in DMCChunk::polygonize_cell (DMCChunk.cpp:543-565).

@@ -4,10 +4,10 @@
 // sign-changing edges, and MeshProcessor<4>/<3> smoothing over per-vertex
 // rings.
 //
-// Per batch (after sample, count and refine of the triangle path, which
-// leave every edge endpoint's density exact):
-//   k_dmc_count    per label row: polygons of the row's cells, quads of the
-//                  row's interior sign-changing edges
+// Per batch (after sample, count and refine, which in this mode count per
+// label row the polygons of the row's cells and the quads of the row's
+// interior sign-changing edges, and leave every edge endpoint's density
+// exact):
 //   (k_scan)       row bases, arena offsets
 //   k_dmc_vertices per row: vertex positions; per surface cell its first
 //                  vertex id and mask (cell_tb / cell_mask arenas)
@@ -53,45 +53,6 @@ __device__ void load_dmc_tables(DmcTables& t) {
         t.np[m] = (uint8_t)(p + any);
         t.pe[m] = pe;
     }
-}
-
-// Samples z in [1, dim-2]: the z-range where both cells beside an x- or
-// y-edge lie in the chunk.
-template <int W>
-__device__ __forceinline__ RowBits<W> inner_z(int dim) {
-    RowBits<W> r;
-#pragma unroll
-    for (int k = 0; k < W; k++) {
-        uint64_t v = ~0ull;
-        const int lo = 64 * k;
-        if (lo == 0) v &= ~1ull;
-        const int hi = dim - 2 - lo;  // last valid bit in this word
-        if (hi < 0) v = 0ull;
-        else if (hi < 63) v &= (2ull << hi) - 1ull;
-        r.w[k] = v;
-    }
-    return r;
-}
-
-template <int W>
-__device__ __forceinline__ RowBits<W> rand_(const RowBits<W>& a, const RowBits<W>& b) {
-    RowBits<W> r;
-#pragma unroll
-    for (int k = 0; k < W; k++) r.w[k] = a.w[k] & b.w[k];
-    return r;
-}
-
-// Interior sign-changing edges owned by the samples of row (x, y): the four
-// cells around each lie in the chunk.
-template <int W>
-__device__ __forceinline__ void interior_edges(const RowBits<W>& r00, const RowBits<W>& r01, const RowBits<W>& r10,
-                                               int dim, int x, int y, RowBits<W>& ex, RowBits<W>& ey,
-                                               RowBits<W>& ez) {
-    const RowBits<W> iz = inner_z<W>(dim);
-    const bool xin = x >= 1 && x <= dim - 2, yin = y >= 1 && y <= dim - 2;
-    ex = (yin && x + 1 < dim) ? rand_<W>(rxor<W>(r00, r10, true), iz) : zero_row<W>();
-    ey = (xin && y + 1 < dim) ? rand_<W>(rxor<W>(r00, r01, true), iz) : zero_row<W>();
-    ez = (xin && yin) ? zedges<W>(r00, dim) : zero_row<W>();
 }
 
 // Sum over the wave of a per-lane count c < 8 (bit-sliced ballots), and the
@@ -302,60 +263,6 @@ __constant__ int8_t c_le[12][4] = {{0, 0, 0, 0}, {0, 0, 1, 0}, {0, 1, 0, 0}, {0,
                                    {0, 0, 0, 2}, {0, 1, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}};
 
 }  // namespace
-
-// ------------------------------------------------------------------------
-// count: polygons per row of cells, quads per row of owned edges
-// ------------------------------------------------------------------------
-template <int W>
-__global__ void __launch_bounds__(kRowThreads) k_dmc_count(Arenas A, int n) {
-    __shared__ DmcTables s_t;
-    __shared__ uint64_t s_planes[2][plane_words(W)];
-    load_dmc_tables(s_t);
-    const int wave = uniform_int(threadIdx.x >> 6);
-    const int lane = lane_id();
-    const uint64_t lt = lanemask_lt();
-    const int64_t total = A.ctr->total_planes;
-    const uint32_t ppq = A.quad_tris ? 2u : 1u;
-    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const int j = A.plane_chunk[item];
-        const ChunkMeta& M = A.meta[j];
-        const int dim = M.dim;
-        if (((dim + 63) >> 6) != W) continue;
-        const int x = (int)(item - A.plane_prefix[j]);
-        uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
-        uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
-        if (!(M.any_air && M.any_solid)) {
-            for (int y = threadIdx.x; y < dim; y += blockDim.x) rv[y] = rt[y] = 0u;
-            continue;
-        }
-        __syncthreads();
-        stage_plane(s_planes[0], A.labels + M.lab_off, M.lab_words, dim, x);
-        stage_plane(s_planes[1], A.labels + M.lab_off, M.lab_words, dim, x + 1);
-        __syncthreads();
-        for (int y = wave; y < dim; y += kRowThreads / 64) {
-            const bool x1 = x + 1 < dim, y1 = y + 1 < dim;
-            const RowBits<W> r00 = lds_row<W>(s_planes[0], dim, y, true);
-            const RowBits<W> r01 = lds_row<W>(s_planes[0], dim, y + 1, y1);
-            const RowBits<W> r10 = lds_row<W>(s_planes[1], dim, y, x1);
-            const RowBits<W> r11 = lds_row<W>(s_planes[1], dim, y + 1, x1 && y1);
-            uint32_t nv = 0;
-            if (x1 && y1 && !quad_uniform<W>(r00, r01, r10, r11, dim))
-                for (int z0 = 0; z0 < dim - 1; z0 += 64) {
-                    const int z = z0 + lane;
-                    const uint32_t c = z < dim - 1 ? s_t.np[cell_mask<W>(r00, r01, r10, r11, z)] : 0u;
-                    uint32_t pre;
-                    nv += wave_sum3(c, lt, pre);
-                }
-            RowBits<W> ex, ey, ez;
-            interior_edges<W>(r00, r01, r10, dim, x, y, ex, ey, ez);
-            const uint32_t nq = total_pop<W>(ex) + total_pop<W>(ey) + total_pop<W>(ez);
-            if (lane == 0) {
-                rv[y] = nv;
-                rt[y] = nq * ppq;
-            }
-        }
-    }
-}
 
 // ------------------------------------------------------------------------
 // vertices: one per polygon of each surface cell.  Row pass (wave per row,
@@ -719,13 +626,6 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_ring(Arenas A, int m)
 // ------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------
-hipError_t launch_dmc_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
-    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_count<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_count<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_count<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_count<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    return hipGetLastError();
-}
 
 hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
     if (wclasses & 1) hipLaunchKernelGGL(k_dmc_vertices<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);

@@ -160,7 +160,6 @@ hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 // Quad mode (bmf_dmc.hip): counts after launch_count/launch_refine, emission
 // after launch_scan, smoothing over the per-vertex rings.
-hipError_t launch_dmc_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
 hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
 hipError_t launch_dmc_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s);

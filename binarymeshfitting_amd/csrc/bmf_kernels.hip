@@ -451,9 +451,11 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
 // vertices and triangles of each row
 // ------------------------------------------------------------------------
 // Staged planes: s_lab[0..2] = labels of x-1, x, x+1; s_lab[3] = exactness
-// of x.  Counts the row's vertices and triangles, and queues for k_refine
-// the row's samples that are endpoints of sign-changing edges but were
-// labelled before their last octave (k_sample's adaptive evaluation).
+// of x.  Counts the row's vertices and triangles (in the quad mode its
+// cells' polygons and the quads of its interior sign-changing edges, with
+// s_ntri holding polygons per mask), and queues for k_refine the row's
+// samples that are endpoints of sign-changing edges but were labelled before
+// their last octave (k_sample's adaptive evaluation).
 template <int W>
 __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, const uint64_t* const* s_lab,
                                           const uint8_t* s_ntri, int x, int y, uint32_t* __restrict__ q,
@@ -478,7 +480,16 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
         (y < 1 || row_equal<W>(rym, r00)))
         return;
     const RowBits<W> ex = rxor<W>(r00, r10, x1), ey = rxor<W>(r00, r01, y1), ez = zedges<W>(r00, dim);
-    nv = total_pop<W>(ex) + total_pop<W>(ey) + total_pop<W>(ez);
+    const bool quads = A.mesh_mode == BMF_MESH_QUADS;
+    if (quads) {
+        RowBits<W> ix, iy, iz;
+        interior_edges<W>(r00, r01, r10, dim, x, y, ix, iy, iz);
+        nv = 0;
+        nt = (total_pop<W>(ix) + total_pop<W>(iy) + total_pop<W>(iz)) * (A.quad_tris ? 2u : 1u);
+    } else {
+        nv = total_pop<W>(ex) + total_pop<W>(ey) + total_pop<W>(ez);
+    }
+    uint32_t ncell = 0;  // triangles (polygons in the quad mode) of the row's cells
     if (x < dim - 1 && y < dim - 1 && !quad_uniform<W>(r00, r01, r10, r11, dim)) {
         const RowBits<W> s00 = rshr1<W>(r00), s01 = rshr1<W>(r01), s10 = rshr1<W>(r10), s11 = rshr1<W>(r11);
 #pragma unroll
@@ -492,10 +503,12 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
                                      (lane_bit(r10.w[k], lane) << 4) | (lane_bit(s10.w[k], lane) << 5) |
                                      (lane_bit(r11.w[k], lane) << 6) | (lane_bit(s11.w[k], lane) << 7)]
                             : 0u;
-            nt += (uint32_t)__popcll(__ballot(c & 1u)) + 2u * (uint32_t)__popcll(__ballot(c & 2u)) +
-                  4u * (uint32_t)__popcll(__ballot(c & 4u));
+            ncell += (uint32_t)__popcll(__ballot(c & 1u)) + 2u * (uint32_t)__popcll(__ballot(c & 2u)) +
+                     4u * (uint32_t)__popcll(__ballot(c & 4u));
         }
     }
+    if (quads) nv = ncell;
+    else nt = ncell;
     // endpoints of the sign-changing edges incident to the row's samples
     RowBits<W> ep = ror<W>(ror<W>(ex, ey), ror<W>(ez, rshl1<W>(ez)));
     ep = ror<W>(ep, ror<W>(rxor<W>(r00, rxm, x >= 1), rxor<W>(r00, rym, y >= 1)));
@@ -560,7 +573,9 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
     __shared__ uint64_t s_planes[4][plane_words(W)];
     const uint64_t* const s_lab[4] = {s_planes[0], s_planes[1], s_planes[2], s_planes[3]};
     __shared__ uint32_t s_qn;
-    for (int t = threadIdx.x; t < 256; t += blockDim.x) s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
+    const bool quads = A.mesh_mode == BMF_MESH_QUADS;
+    for (int t = threadIdx.x; t < 256; t += blockDim.x)
+        s_ntri[t] = (uint8_t)(quads ? dmc_polygons(c_dmc_table[t]) : (uint32_t)((c_tri_table[t] >> 60) / 3));
     const int wave = uniform_int(threadIdx.x >> 6);
     const int lane = lane_id();
     const int split = A.ctr->plane_split;

@@ -286,4 +286,44 @@ __device__ __forceinline__ RowBits<W> ror(const RowBits<W>& a, const RowBits<W>&
     return r;
 }
 
+// Samples z in [1, dim-2]: the z-range where both cells beside an x- or
+// y-edge lie in the chunk.
+template <int W>
+__device__ __forceinline__ RowBits<W> inner_z(int dim) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        uint64_t v = ~0ull;
+        const int lo = 64 * k;
+        if (lo == 0) v &= ~1ull;
+        const int hi = dim - 2 - lo;  // last valid bit in this word
+        if (hi < 0) v = 0ull;
+        else if (hi < 63) v &= (2ull << hi) - 1ull;
+        r.w[k] = v;
+    }
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ RowBits<W> rand_(const RowBits<W>& a, const RowBits<W>& b) {
+    RowBits<W> r;
+#pragma unroll
+    for (int k = 0; k < W; k++) r.w[k] = a.w[k] & b.w[k];
+    return r;
+}
+
+// Interior sign-changing edges owned by the samples of row (x, y): the four
+// cells around each lie in the chunk.
+template <int W>
+__device__ __forceinline__ void interior_edges(const RowBits<W>& r00, const RowBits<W>& r01, const RowBits<W>& r10,
+                                               int dim, int x, int y, RowBits<W>& ex, RowBits<W>& ey,
+                                               RowBits<W>& ez) {
+    const RowBits<W> iz = inner_z<W>(dim);
+    const bool xin = x >= 1 && x <= dim - 2, yin = y >= 1 && y <= dim - 2;
+    ex = (yin && x + 1 < dim) ? rand_<W>(rxor<W>(r00, r10, true), iz) : zero_row<W>();
+    ey = (xin && y + 1 < dim) ? rand_<W>(rxor<W>(r00, r01, true), iz) : zero_row<W>();
+    ez = (xin && yin) ? zedges<W>(r00, dim) : zero_row<W>();
+}
+
+
 }  // namespace bmf
