@@ -407,10 +407,26 @@ __device__ __forceinline__ float box_lb2(float px, float py, float pz, float4 l,
     return (dx * dx + dy * dy) + dz * dz;
 }
 
+// fmin with the value DPP control CTRL brings from another lane (+inf where
+// the source lane is outside the row or the row is not in ROW_MASK).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_fmin(float v) {
+    const int o = __builtin_amdgcn_update_dpp(__float_as_int(INFINITY), __float_as_int(v), CTRL, ROW_MASK, 0xf, false);
+    return fminf(v, __int_as_float(o));
+}
+
+// Minimum over the 64 lanes of a wave (all lanes active): prefix minima
+// within each row of 16 lanes (row_shr 1, 2, 4, 8), then row 0's last lane
+// into row 1 and row 2's into row 3 (row_bcast:15), then lane 31 into rows 2
+// and 3 (row_bcast:31); lane 63 then holds the minimum.
 __device__ __forceinline__ float wave_min(float x) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x = fminf(x, __shfl_xor(x, off));
-    return x;
+    x = dpp_fmin<0x111, 0xf>(x);
+    x = dpp_fmin<0x112, 0xf>(x);
+    x = dpp_fmin<0x114, 0xf>(x);
+    x = dpp_fmin<0x118, 0xf>(x);
+    x = dpp_fmin<0x142, 0xa>(x);
+    x = dpp_fmin<0x143, 0xc>(x);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
 // Lowest lane of the (non-empty) set m whose value x is smallest.
