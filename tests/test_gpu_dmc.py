@@ -64,6 +64,30 @@ def test_terrain_quads_match_oracle(ctx, oracle, mode, as_tris, K):
         ctx.set_mesh_options(mesh=capi.MESH_TRIANGLES)
 
 
+@pytest.mark.parametrize("split", [1, 2, 8])
+def test_quads_identical_across_row_splits(ctx, oracle, split):
+    """The count pass counts polygons and quads per row segment; every split
+    of the planes into row segments gives the oracle's mesh."""
+    ctx.set_sampler(**TERRAIN)
+    default = ctx.tuning()
+    ctx.set_mesh_options(mesh=capi.MESH_QUADS)
+    try:
+        ctx.set_tuning(row_split=split)
+        chunks = [workloads.Chunk((0.0, 0.0, 0.0), 64.0, 65, 0.0, 2, 0),
+                  workloads.Chunk((-20.0, 4.0, 7.0), 128.0, 129, 0.0, 0, 0)]
+        res = _run(ctx, chunks, capi.MODE_BATCHED)
+        spec = oracle.SamplerSpec(**TERRAIN)
+        for j, c in enumerate(chunks):
+            ref = _oracle_dmc(oracle, spec, c)
+            assert res[j].vertex_count == ref["V"] and res[j].index_count == ref["nI"], c
+            pos, idx, bnd = ctx.mesh(j)
+            assert np.array_equal(idx, ref["idx"]), c
+            assert np.array_equal(pos, ref["pos"]), c
+    finally:
+        ctx.set_mesh_options(mesh=capi.MESH_TRIANGLES)
+        ctx.set_tuning(**default)
+
+
 def test_sphere_quads_closed_and_oriented(ctx):
     ctx.set_sampler(kind=capi.SAMPLER_SPHERE, world_size=40.0)
     ctx.set_mesh_options(mesh=capi.MESH_QUADS)

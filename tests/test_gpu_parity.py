@@ -55,6 +55,34 @@ CHUNKS = [
 ]
 
 
+# Row-word class 4 (dim 193..256; 256 is the largest chunk the C ABI takes).
+LARGE = [
+    workloads.Chunk((0.0, 0.0, 0.0), 255.0, 256, 0.0, 2, 0),
+    workloads.Chunk((30.0, 10.0, -20.0), 96.0, 193, 0.0, 3, 1),   # spacing 0.5, boundary processing
+]
+
+
+def test_largest_dims_match_oracle(oracle):
+    """Both paths on the largest row-word class, against one oracle run."""
+    spec = _ospec(oracle, TERRAIN)
+    refs = [oracle.extract_chunk(spec, _ochunk(oracle, c)) for c in LARGE]
+    cd = workloads.to_capi(LARGE)
+    with capi.Context(max_chunks=4, max_samples=256 ** 3 + 193 ** 3, max_vertices=1 << 22,
+                      max_indices=1 << 24) as c:
+        c.set_sampler(**TERRAIN)
+        for mode in (capi.MODE_PLAIN, capi.MODE_BATCHED):
+            c.extract(cd, mode)
+            rc, res, _, _ = c.results(len(cd))
+            capi.check(rc, "bmf_result")
+            for j, (ch, ref) in enumerate(zip(LARGE, refs)):
+                pos, idx, bnd = c.mesh(j)
+                assert res[j].vertex_count == ref["V"] and res[j].index_count == ref["nI"], (mode, ch)
+                assert bool(res[j].contains_mesh) == ref["contains_mesh"]
+                assert np.array_equal(idx, ref["idx"]), (mode, ch)
+                assert np.array_equal(bnd, ref["boundary"]), (mode, ch)
+                assert np.array_equal(pos, ref["pos"]), (mode, ch)
+
+
 @pytest.mark.parametrize("c", CHUNKS[:6], ids=lambda c: f"dim{c.dim}")
 def test_sample_block_matches_oracle(ctx, oracle, c):
     ctx.set_sampler(**TERRAIN)
