@@ -35,7 +35,7 @@ hipError_t dalloc(T** p, int64_t count) {
 void free_arenas(bmf_ctx* c) {
     Arenas& A = c->A;
     void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plane_prefix, A.plane_chunk, A.smooth_list, A.smooth_prefix,
-                    A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.refine, A.plane_refine_n, A.cen4,
+                    A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.smooth_vmap, A.smooth_tmap, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
                     A.pos,     A.pos4,      A.boundary, A.vedge,       A.idx,          A.ctr,
                     A.sampler, c->d_spheres, c->d_boxes, A.ring, A.ring_n, A.qef_err, A.nrm4, A.fn4, A.nrm,
@@ -180,6 +180,8 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
          dalloc(&A.plane_chunk, nc * BMF_MAX_DIM) == hipSuccess && dalloc(&A.smooth_list, nc) == hipSuccess &&
          dalloc(&A.smooth_prefix, nc + 1) == hipSuccess && dalloc(&A.smooth_tprefix, nc + 1) == hipSuccess &&
          dalloc(&A.smooth_xcd_v, kXcds + 1) == hipSuccess && dalloc(&A.smooth_xcd_t, kXcds + 1) == hipSuccess &&
+         dalloc(&A.smooth_vmap, A.cap_vertices / kSmoothThreads + nc + 1) == hipSuccess &&
+         dalloc(&A.smooth_tmap, A.cap_indices / 3 / kSmoothThreads + nc + 1) == hipSuccess &&
          dalloc(&A.refine, A.cap_samples) == hipSuccess && dalloc(&A.plane_refine_n, nc * BMF_MAX_DIM * kMaxPlaneSplit) == hipSuccess &&
          dalloc(&A.cen4, A.cap_indices / 3 + 1) == hipSuccess &&
          dalloc(&A.dens, A.cap_samples) == hipSuccess && dalloc(&A.labels, A.cap_samples / 64 + nc) == hipSuccess &&

@@ -101,6 +101,8 @@ struct Arenas {
     int64_t* smooth_tprefix;   // [max_chunks + 1] triangle tiles of the listed chunks
     int64_t* smooth_xcd_v;     // [kXcds + 1] vertex-tile range of each XCD group
     int64_t* smooth_xcd_t;     // [kXcds + 1] triangle-tile range of each XCD group
+    int2* smooth_vmap;         // [vertex tiles] (chunk, tile within the chunk) of each smoothing vertex tile
+    int2* smooth_tmap;         // [triangle tiles] the same for the triangle tiles
     uint32_t* refine;          // [max_samples] per x-plane queue of samples (y*dim + z) to finish,
                                // a region of rows [y0, y1) per row segment
     int32_t row_split;         // row segments per x-plane item, or -1 for the automatic choice

@@ -1015,6 +1015,20 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
     }
 }
 
+// Tile -> (chunk, tile within the chunk) for the smoothing kernels, one
+// block per listed chunk, written once per frame before the first step.
+__global__ void __launch_bounds__(kSmoothThreads) k_smooth_map(Arenas A) {
+    const int ns = A.ctr->n_smooth;
+    for (int p = blockIdx.x; p < ns; p += gridDim.x) {
+        const int j = A.smooth_list[p];
+        const ChunkMeta& M = A.meta[j];
+        const int vt = (M.V + kSmoothThreads - 1) / kSmoothThreads, tt = (M.T + kSmoothThreads - 1) / kSmoothThreads;
+        const int64_t bv = A.smooth_prefix[p], bt = A.smooth_tprefix[p];
+        for (int r = threadIdx.x; r < vt; r += blockDim.x) A.smooth_vmap[bv + r] = make_int2(j, r);
+        for (int r = threadIdx.x; r < tt; r += blockDim.x) A.smooth_tmap[bt + r] = make_int2(j, r);
+    }
+}
+
 // ------------------------------------------------------------------------
 // smoothing, iteration m of MeshProcessor<3> (MeshProcessor.cpp:130-306 as
 // driven by ChunkGenerator.cpp:112-124), over the chunks with K > m:
@@ -1029,14 +1043,13 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
 // float3 output.
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int m) {
-    const int nsm = A.ctr->n_smooth;
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_t[xcd + 1];
     for (int64_t tile = A.smooth_xcd_t[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
-        const int k = find_chunk(A.smooth_tprefix, nsm, tile);
-        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int2 tm = A.smooth_tmap[tile];  // (chunk, tile within the chunk)
+        const ChunkMeta& M = A.meta[tm.x];
         if (M.iters <= m) continue;
-        const int64_t t = (tile - A.smooth_tprefix[k]) * kSmoothThreads + threadIdx.x;
+        const int64_t t = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
         if (t >= M.T) continue;
         const uint32_t* __restrict__ I = A.idx + M.i_off + 3 * t;
         const float4* __restrict__ P = A.pos4 + M.v_off;
@@ -1061,15 +1074,14 @@ __global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int m) {
 }
 
 __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
-    const int nsm = A.ctr->n_smooth;
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_v[xcd + 1];
     for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
-        const int k = find_chunk(A.smooth_prefix, nsm, tile);
-        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int2 tm = A.smooth_vmap[tile];  // (chunk, tile within the chunk)
+        const ChunkMeta& M = A.meta[tm.x];
         const int K = M.iters;
         if (K <= m) continue;
-        const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
+        const int64_t v = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
         if (v >= M.V) continue;
         const int64_t vo = M.v_off, T = M.T;
         if (!M.bp && A.boundary[vo + v]) continue;  // stays at its position in both buffers
@@ -1127,15 +1139,14 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
 // its y is not 0; positions move as in k_dual/k_primal (K > 0 only); the
 // last step writes the unit normal.
 __global__ void __launch_bounds__(kSmoothThreads) k_dual_nrm(Arenas A, int m) {
-    const int nsm = A.ctr->n_smooth;
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_t[xcd + 1];
     for (int64_t tile = A.smooth_xcd_t[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
-        const int k = find_chunk(A.smooth_tprefix, nsm, tile);
-        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int2 tm = A.smooth_tmap[tile];  // (chunk, tile within the chunk)
+        const ChunkMeta& M = A.meta[tm.x];
         const int K = M.iters;
         if (max(K, 1) <= m) continue;
-        const int64_t t = (tile - A.smooth_tprefix[k]) * kSmoothThreads + threadIdx.x;
+        const int64_t t = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
         if (t >= M.T) continue;
         const uint32_t* __restrict__ I = A.idx + M.i_off + 3 * t;
         const float4* __restrict__ P = A.pos4 + M.v_off;
@@ -1190,16 +1201,15 @@ __global__ void __launch_bounds__(kSmoothThreads) k_dual_nrm(Arenas A, int m) {
 }
 
 __global__ void __launch_bounds__(kSmoothThreads) k_primal_nrm(Arenas A, int m) {
-    const int nsm = A.ctr->n_smooth;
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_v[xcd + 1];
     for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
-        const int k = find_chunk(A.smooth_prefix, nsm, tile);
-        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int2 tm = A.smooth_vmap[tile];  // (chunk, tile within the chunk)
+        const ChunkMeta& M = A.meta[tm.x];
         const int K = M.iters;
         const int steps = max(K, 1);
         if (steps <= m) continue;
-        const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
+        const int64_t v = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
         if (v >= M.V) continue;
         const int64_t vo = M.v_off, T = M.T, g = vo + v;
         const int dim = M.dim;
@@ -1411,6 +1421,7 @@ hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s) {
 }
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
     blocks = blocks < kXcds ? kXcds : blocks - blocks % kXcds;  // whole XCD groups
+    if (m == 0) hipLaunchKernelGGL(k_smooth_map, dim3(std::min(blocks, 1024)), dim3(kSmoothThreads), 0, s, A);
     if (A.normals) {
         hipLaunchKernelGGL(k_dual_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
         hipLaunchKernelGGL(k_primal_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
