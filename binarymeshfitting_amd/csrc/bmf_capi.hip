@@ -34,7 +34,7 @@ hipError_t dalloc(T** p, int64_t count) {
 
 void free_arenas(bmf_ctx* c) {
     Arenas& A = c->A;
-    void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plane_prefix, A.plane_chunk, A.smooth_list, A.smooth_prefix,
+    void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plane_map, A.smooth_list, A.smooth_prefix,
                     A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.smooth_vmap, A.smooth_tmap, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
                     A.pos,     A.pos4,      A.boundary, A.vedge,       A.idx,          A.ctr,
@@ -44,6 +44,35 @@ void free_arenas(bmf_ctx* c) {
         if (p) (void)hipFree(p);
     A = Arenas{};
     c->d_spheres = c->d_boxes = nullptr;
+}
+
+// Streams of the row-word classes in `wclasses`: the first present class on
+// s, each further class on its own auxiliary stream, which waits for s here;
+// join_classes makes s wait for them again.  Under graph capture this is a
+// fork and a join of the captured graph.
+int fork_classes(bmf_ctx* c, int wclasses, hipStream_t s, hipStream_t cs[4], int& used) {
+    used = 0;
+    bool first = true;
+    for (int w = 0; w < 4; w++) {
+        cs[w] = s;
+        if (!(wclasses & (1 << w))) continue;
+        if (first) {
+            first = false;
+            continue;
+        }
+        if (used == 0) BMF_HIP(hipEventRecord(c->ev_fork, s));
+        BMF_HIP(hipStreamWaitEvent(c->aux[used], c->ev_fork, 0));
+        cs[w] = c->aux[used++];
+    }
+    return BMF_OK;
+}
+
+int join_classes(bmf_ctx* c, hipStream_t s, int used) {
+    for (int i = 0; i < used; i++) {
+        BMF_HIP(hipEventRecord(c->ev_join[i], c->aux[i]));
+        BMF_HIP(hipStreamWaitEvent(s, c->ev_join[i], 0));
+    }
+    return BMF_OK;
 }
 
 // The whole device pipeline for chunks [0, n) of the uploaded descriptors.
@@ -59,13 +88,23 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
     BMF_HIP(launch_sample(A, n, sample_blocks, 1, c->h_sampler.kind, s));
     BMF_MARK(2);
     const bool quads = A.mesh_mode == BMF_MESH_QUADS;
-    BMF_HIP(launch_count(A, n, row_blocks, wclasses, s));
+    hipStream_t cs[4];
+    int used = 0;
+    int rc = fork_classes(c, wclasses, s, cs, used);
+    if (rc) return rc;
+    BMF_HIP(launch_count(A, n, row_blocks, wclasses, cs));
+    rc = join_classes(c, s, used);
+    if (rc) return rc;
     BMF_HIP(launch_refine(A, n, row_blocks, s));
     BMF_MARK(3);
     BMF_HIP(launch_scan(A, n, scan_blocks, s));
     BMF_MARK(4);
-    if (quads) BMF_HIP(launch_dmc_emit(A, n, row_blocks, wclasses, s));
-    else BMF_HIP(launch_emit(A, n, row_blocks, wclasses, s));
+    rc = fork_classes(c, wclasses, s, cs, used);
+    if (rc) return rc;
+    if (quads) BMF_HIP(launch_dmc_emit(A, n, row_blocks, wclasses, cs));
+    else BMF_HIP(launch_emit(A, n, row_blocks, wclasses, cs));
+    rc = join_classes(c, s, used);
+    if (rc) return rc;
     BMF_MARK(5);
     for (int m = 0; m < maxk; m++) {
         if (quads) BMF_HIP(launch_dmc_smooth(A, m, smooth_blocks, s));
@@ -176,8 +215,7 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && dalloc(&A.meta, nc) == hipSuccess && dalloc(&A.descs, nc) == hipSuccess &&
          dalloc(&A.results, nc) == hipSuccess && dalloc(&A.tile_prefix, nc + 1) == hipSuccess &&
-         dalloc(&A.plane_prefix, nc + 1) == hipSuccess &&
-         dalloc(&A.plane_chunk, nc * BMF_MAX_DIM) == hipSuccess && dalloc(&A.smooth_list, nc) == hipSuccess &&
+         dalloc(&A.plane_map, nc * BMF_MAX_DIM) == hipSuccess && dalloc(&A.smooth_list, nc) == hipSuccess &&
          dalloc(&A.smooth_prefix, nc + 1) == hipSuccess && dalloc(&A.smooth_tprefix, nc + 1) == hipSuccess &&
          dalloc(&A.smooth_xcd_v, kXcds + 1) == hipSuccess && dalloc(&A.smooth_xcd_t, kXcds + 1) == hipSuccess &&
          dalloc(&A.smooth_vmap, A.cap_vertices / kSmoothThreads + nc + 1) == hipSuccess &&
@@ -196,6 +234,10 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
          hipHostMalloc((void**)&c->h_descs[1], sizeof(bmf_chunk_desc) * nc) == hipSuccess &&
          hipHostMalloc((void**)&c->h_results, sizeof(bmf_chunk_result) * nc) == hipSuccess &&
          hipHostMalloc((void**)&c->h_ctr, sizeof(Counters)) == hipSuccess;
+    for (int i = 0; ok && i < 3; i++)
+        ok = hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
          hipEventCreate(&c->ev_upload[0]) == hipSuccess && hipEventCreate(&c->ev_upload[1]) == hipSuccess;
     for (int i = 0; ok && i <= BMF_NUM_STAGES; i++) ok = hipEventCreate(&c->ev_stage[i]) == hipSuccess;
@@ -223,11 +265,14 @@ void bmf_destroy(bmf_ctx* c) {
         if (c->h_descs[i]) (void)hipHostFree(c->h_descs[i]);
     if (c->h_results) (void)hipHostFree(c->h_results);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
-    hipEvent_t evs[] = {c->ev0, c->ev1, c->ev_upload[0], c->ev_upload[1]};
+    hipEvent_t evs[] = {c->ev0, c->ev1, c->ev_upload[0], c->ev_upload[1], c->ev_fork, c->ev_join[0], c->ev_join[1],
+                        c->ev_join[2]};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
     for (int i = 0; i <= BMF_NUM_STAGES; i++)
         if (c->ev_stage[i]) (void)hipEventDestroy(c->ev_stage[i]);
+    for (int i = 0; i < 3; i++)
+        if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -755,7 +800,7 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
         float bt = 1e30f;
         int bb = cur.row_blocks, bs = cur.row_split;
         const int mult[5] = {1, 2, 4, 8, 16};
-        const int splits[4] = {1, 2, 4, 8};
+        const int splits[5] = {-1, 1, 2, 4, 8};  // -1: per row-word class from its plane count
         for (int sp : splits)
             for (int m : mult) {
                 bmf_tuning t = cur;

@@ -341,14 +341,14 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
     const int wave = uniform_int(threadIdx.x >> 6);
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
-    const int64_t total = A.ctr->total_planes;
+    const int64_t base = A.ctr->class_base[W - 1], total = A.ctr->class_planes[W - 1];
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const int j = A.plane_chunk[item];
+        const int2 pm = A.plane_map[base + item];
+        const int j = pm.x;
         const ChunkMeta& M = A.meta[j];
         const int dim = M.dim;
-        if (((dim + 63) >> 6) != W) continue;
         if (M.status != BMF_OK || M.V == 0) continue;
-        const int x = (int)(item - A.plane_prefix[j]);
+        const int x = pm.y;
         if (x >= dim - 1) continue;
         const bool qef = A.placement == BMF_PLACE_QEF && S->kind == BMF_SAMPLER_CSG;
         const uint16_t* csg = A.csg_list ? A.csg_list + (int64_t)j * A.csg_stride : nullptr;
@@ -424,15 +424,15 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
     load_dmc_tables(s_t);
     const int wave = uniform_int(threadIdx.x >> 6);
     const int lane = lane_id();
-    const int64_t total = A.ctr->total_planes;
+    const int64_t base = A.ctr->class_base[W - 1], total = A.ctr->class_planes[W - 1];
     const bool tris = A.quad_tris != 0;
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const int j = A.plane_chunk[item];
+        const int2 pm = A.plane_map[base + item];
+        const int j = pm.x;
         const ChunkMeta& M = A.meta[j];
         const int dim = M.dim;
-        if (((dim + 63) >> 6) != W) continue;
         if (M.status != BMF_OK || M.T == 0) continue;
-        const int x = (int)(item - A.plane_prefix[j]);
+        const int x = pm.y;
         __syncthreads();
         stage_plane(s_planes[0], A.labels + M.lab_off, M.lab_words, dim, x);
         stage_plane(s_planes[1], A.labels + M.lab_off, M.lab_words, dim, x + 1);
@@ -627,15 +627,16 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_ring(Arenas A, int m)
 // launch wrappers
 // ------------------------------------------------------------------------
 
-hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
-    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_vertices<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_vertices<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_vertices<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_vertices<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_quads<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_quads<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_quads<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_quads<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs) {
+    // per class: vertices, then the quads that index them
+    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_vertices<1>, dim3(blocks), dim3(kRowThreads), 0, cs[0], A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_vertices<2>, dim3(blocks), dim3(kRowThreads), 0, cs[1], A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_vertices<3>, dim3(blocks), dim3(kRowThreads), 0, cs[2], A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_vertices<4>, dim3(blocks), dim3(kRowThreads), 0, cs[3], A, n);
+    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_quads<1>, dim3(blocks), dim3(kRowThreads), 0, cs[0], A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_quads<2>, dim3(blocks), dim3(kRowThreads), 0, cs[1], A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_quads<3>, dim3(blocks), dim3(kRowThreads), 0, cs[2], A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_quads<4>, dim3(blocks), dim3(kRowThreads), 0, cs[3], A, n);
     return hipGetLastError();
 }
 

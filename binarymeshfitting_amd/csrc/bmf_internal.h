@@ -68,6 +68,7 @@ struct ChunkMeta {
     int32_t csg_nb;     //      listed in Arenas::csg_list (sphere ids, then box ids, in order)
     int64_t v_off;      // output offsets
     int64_t i_off;
+    int64_t plane_slot; // plane_map entry of this chunk's x-plane 0 (sampled chunks)
 };
 
 // Global counters, reset by the plan kernel.
@@ -78,14 +79,15 @@ struct Counters {
     unsigned long long need_v;
     unsigned long long need_i;
     int64_t total_tiles;
-    int64_t total_planes;
     int64_t total_samples;
     int64_t total_vtiles;
     int64_t total_ttiles;
     int32_t n_smooth;
     int32_t overflow;
     int32_t error;
-    int32_t plane_split;  // row segments per x-plane in k_count / k_refine / k_emit
+    int32_t plane_split[4];    // row segments per x-plane item of row-word class W (index W-1)
+    int64_t class_base[4];     // first plane_map entry of class W's planes
+    int64_t class_planes[4];   // sampled planes of class W
 };
 
 // Device pointers of one context.
@@ -94,8 +96,8 @@ struct Arenas {
     bmf_chunk_desc* descs;     // [max_chunks]
     bmf_chunk_result* results; // [max_chunks]
     int64_t* tile_prefix;      // [max_chunks + 1] sampling tiles
-    int64_t* plane_prefix;     // [max_chunks + 1] x-planes of the sampled chunks
-    int32_t* plane_chunk;      // [max_chunks * BMF_MAX_DIM] chunk of each x-plane item (written by k_sample)
+    int2* plane_map;           // [max_chunks * BMF_MAX_DIM] (chunk, x) of each sampled x-plane, grouped by
+                               // row-word class (written by k_sample's prologue)
     int32_t* smooth_list;      // [max_chunks] chunks that run the smoothing
     int64_t* smooth_prefix;    // [max_chunks + 1] vertex tiles of the listed chunks
     int64_t* smooth_tprefix;   // [max_chunks + 1] triangle tiles of the listed chunks
@@ -156,13 +158,18 @@ hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, 
 hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int kind, hipStream_t s);
 // wclasses: bit W-1 set when the batch holds chunks with ceil(dim/64) == W.
 hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
+// cs[W-1]: the stream of row-word class W
+hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs);
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
+// cs[W-1]: the stream of row-word class W
+hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs);
 hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 // Quad mode (bmf_dmc.hip): counts after launch_count/launch_refine, emission
 // after launch_scan, smoothing over the per-vertex rings.
-hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
+// cs[W-1]: the stream of row-word class W
+hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs);
 hipError_t launch_dmc_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s);
 hipError_t launch_ref_labels(const Arenas& A, uint32_t* words, hipStream_t s);

@@ -30,12 +30,14 @@ namespace bmf {
 constexpr int kPlanThreads = 1024;
 
 __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allow_cull, int reset_bumps) {
-    __shared__ int64_t s_scan[5][kPlanThreads];
+    __shared__ int64_t s_scan[4][kPlanThreads];
+    __shared__ int32_t s_cls[4][kPlanThreads];  // x-planes per row-word class
     const int tid = threadIdx.x;
     const SamplerDev* __restrict__ S = A.sampler;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
     const int j0 = tid * per, j1 = min(n, j0 + per);
-    int64_t sum[5] = {0, 0, 0, 0, 0};  // samples, label words, rows, tiles, planes
+    int64_t sum[4] = {0, 0, 0, 0};  // samples, label words, rows, tiles
+    int32_t cls[4] = {0, 0, 0, 0};
 
     for (int j = j0; j < j1; j++) {
         const bmf_chunk_desc d = A.descs[j];
@@ -81,44 +83,62 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         sum[1] += m.lab_words;
         sum[2] += sampled ? (int64_t)d.dim * d.dim : 0;
         sum[3] += sampled ? (ns + A.sample_tile - 1) / A.sample_tile : 0;
-        sum[4] += sampled ? d.dim : 0;
+        if (sampled) cls[((d.dim + 63) >> 6) - 1] += d.dim;
+        m.plane_slot = 0;
         A.meta[j] = m;
     }
 #pragma unroll
-    for (int q = 0; q < 5; q++) s_scan[q][tid] = sum[q];
+    for (int q = 0; q < 4; q++) s_scan[q][tid] = sum[q];
+#pragma unroll
+    for (int q = 0; q < 4; q++) s_cls[q][tid] = cls[q];
     __syncthreads();
     // Hillis-Steele inclusive scan across threads
     for (int off = 1; off < kPlanThreads; off <<= 1) {
-        int64_t v[5];
+        int64_t v[4];
+        int32_t u[4];
 #pragma unroll
-        for (int q = 0; q < 5; q++) v[q] = tid >= off ? s_scan[q][tid - off] : 0;
+        for (int q = 0; q < 4; q++) v[q] = tid >= off ? s_scan[q][tid - off] : 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) u[q] = tid >= off ? s_cls[q][tid - off] : 0;
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 5; q++) s_scan[q][tid] += v[q];
+        for (int q = 0; q < 4; q++) s_scan[q][tid] += v[q];
+#pragma unroll
+        for (int q = 0; q < 4; q++) s_cls[q][tid] += u[q];
         __syncthreads();
     }
-    int64_t base[5];
+    int64_t base[4];
 #pragma unroll
-    for (int q = 0; q < 5; q++) base[q] = s_scan[q][tid] - sum[q];
+    for (int q = 0; q < 4; q++) base[q] = s_scan[q][tid] - sum[q];
     const int64_t tot_samples = s_scan[0][kPlanThreads - 1];
     const int64_t tot_words = s_scan[1][kPlanThreads - 1];
     const int64_t tot_rows = s_scan[2][kPlanThreads - 1];
     const int64_t tot_tiles = s_scan[3][kPlanThreads - 1];
-    const int64_t tot_planes = s_scan[4][kPlanThreads - 1];
     const bool overflow = tot_samples > A.cap_samples || tot_words > A.cap_samples / 64 + A.cap_chunks ||
                           tot_rows > A.cap_rows;
-    int64_t tp = base[3], pp = base[4];
+    int64_t tp = base[3];
+    // plane_map slots: the classes one after another, chunk order inside a class
+    int64_t cslot[4];
+    {
+        int64_t cb = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            cslot[q] = cb + s_cls[q][tid] - cls[q];
+            cb += s_cls[q][kPlanThreads - 1];
+        }
+    }
     for (int j = j0; j < j1; j++) {
         ChunkMeta& m = A.meta[j];
         m.dens_off += base[0];
         m.lab_off += base[1];
         m.row_off += base[2];
         A.tile_prefix[j] = overflow ? 0 : tp;
-        A.plane_prefix[j] = overflow ? 0 : pp;
         if (m.state == kStateSampled) {
             const int64_t ns = (int64_t)m.dim * m.dim * m.dim;
             tp += (ns + A.sample_tile - 1) / A.sample_tile;
-            pp += m.dim;
+            const int q = ((m.dim + 63) >> 6) - 1;
+            m.plane_slot = cslot[q];
+            cslot[q] += m.dim;
             if (overflow) {
                 m.state = kStateSkipped;
                 m.status = BMF_ECAPACITY;
@@ -130,7 +150,6 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
     }
     if (tid == 0) {
         A.tile_prefix[n] = overflow ? 0 : tot_tiles;
-        A.plane_prefix[n] = overflow ? 0 : tot_planes;
         Counters* c = A.ctr;
         c->sample_head = 0;
         if (reset_bumps) {
@@ -143,13 +162,20 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         }
         if (overflow) c->overflow = 1;
         c->total_tiles = overflow ? 0 : tot_tiles;
-        c->total_planes = overflow ? 0 : tot_planes;
-        // few sampled planes (single large chunks): split each plane's rows
-        // into segments so that the row kernels still have kSplitTarget items
-        int split = A.row_split > 0 ? A.row_split : 1;
-        if (A.row_split <= 0)
-            while (split < kMaxPlaneSplit && tot_planes * split < kSplitTarget) split <<= 1;
-        c->plane_split = split;
+        // per row-word class: its planes in plane_map, and the row segments
+        // per plane (few sampled planes, e.g. single large chunks: split each
+        // plane's rows so that the class still has kSplitTarget items)
+        int64_t cb = 0;
+        for (int q = 0; q < 4; q++) {
+            const int64_t np = overflow ? 0 : (int64_t)s_cls[q][kPlanThreads - 1];
+            int split = A.row_split > 0 ? A.row_split : 1;
+            if (A.row_split <= 0)
+                while (split < kMaxPlaneSplit && np * split < kSplitTarget) split <<= 1;
+            c->plane_split[q] = split;
+            c->class_base[q] = cb;
+            c->class_planes[q] = np;
+            cb += np;
+        }
         c->n_smooth = 0;
         c->total_samples = tot_samples;
     }
@@ -293,12 +319,12 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
     const int64_t total = A.ctr->total_tiles;
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
-    // the chunk of every x-plane item of the plane kernels that follow
+    // (chunk, x) of every x-plane item of the plane kernels that follow
     for (int j = blockIdx.x; j < n; j += gridDim.x) {
         const ChunkMeta& M = A.meta[j];
         if (M.state != kStateSampled) continue;
-        const int64_t pp = A.plane_prefix[j];
-        for (int x = threadIdx.x; x < M.dim; x += blockDim.x) A.plane_chunk[pp + x] = j;
+        const int64_t ps = M.plane_slot;
+        for (int x = threadIdx.x; x < M.dim; x += blockDim.x) A.plane_map[ps + x] = make_int2(j, x);
     }
     for (;;) {
         if (threadIdx.x == 0) {
@@ -540,27 +566,30 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A, int n) {
     const SamplerDev* __restrict__ S = A.sampler;
-    const int split = A.ctr->plane_split;
-    const int64_t total = A.ctr->total_planes * split;
-    for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
-        const uint32_t cnt = A.plane_refine_n[it];
-        if (cnt == 0) continue;
-        const int64_t item = it / split;
-        const int seg = (int)(it - item * split);
-        const int j = A.plane_chunk[item];
-        const ChunkMeta& M = A.meta[j];
-        const uint32_t dim = (uint32_t)M.dim;
-        const uint32_t x = (uint32_t)(item - A.plane_prefix[j]);
-        const uint32_t y0 = (uint32_t)((int64_t)seg * dim / split);
-        const uint32_t* __restrict__ q = A.refine + M.dens_off + (int64_t)x * dim * dim + (int64_t)y0 * dim;
-        float* __restrict__ D = A.dens + M.dens_off + (int64_t)x * dim * dim;
-        const float px = (float)x * M.delta + M.opos[0];
-        for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) {
-            const uint32_t r = q[e];  // y * dim + z within the plane
-            const uint32_t y = r / dim, z = r - y * dim;
-            const float py = (float)y * M.delta + M.opos[1];
-            const float pz = (float)z * M.delta + M.opos[2];
-            D[r] = sample_density(*S, px, py, pz);
+    for (int cw = 0; cw < 4; cw++) {  // the row-word classes in turn
+        const int split = A.ctr->plane_split[cw];
+        const int64_t base = A.ctr->class_base[cw];
+        const int64_t total = A.ctr->class_planes[cw] * split;
+        for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
+            const int64_t item = it / split;
+            const int seg = (int)(it - item * split);
+            const uint32_t cnt = A.plane_refine_n[(base + item) * kMaxPlaneSplit + seg];
+            if (cnt == 0) continue;
+            const int2 pm = A.plane_map[base + item];
+            const ChunkMeta& M = A.meta[pm.x];
+            const uint32_t dim = (uint32_t)M.dim;
+            const uint32_t x = (uint32_t)pm.y;
+            const uint32_t y0 = (uint32_t)((int64_t)seg * dim / split);
+            const uint32_t* __restrict__ q = A.refine + M.dens_off + (int64_t)x * dim * dim + (int64_t)y0 * dim;
+            float* __restrict__ D = A.dens + M.dens_off + (int64_t)x * dim * dim;
+            const float px = (float)x * M.delta + M.opos[0];
+            for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) {
+                const uint32_t r = q[e];  // y * dim + z within the plane
+                const uint32_t y = r / dim, z = r - y * dim;
+                const float py = (float)y * M.delta + M.opos[1];
+                const float pz = (float)z * M.delta + M.opos[2];
+                D[r] = sample_density(*S, px, py, pz);
+            }
         }
     }
 }
@@ -578,22 +607,24 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
         s_ntri[t] = (uint8_t)(quads ? dmc_polygons(c_dmc_table[t]) : (uint32_t)((c_tri_table[t] >> 60) / 3));
     const int wave = uniform_int(threadIdx.x >> 6);
     const int lane = lane_id();
-    const int split = A.ctr->plane_split;
-    const int64_t total = A.ctr->total_planes * split;
+    const int split = A.ctr->plane_split[W - 1];
+    const int64_t base = A.ctr->class_base[W - 1];
+    const int64_t total = A.ctr->class_planes[W - 1] * split;
     for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
         const int64_t item = it / split;
         const int seg = (int)(it - item * split);
-        const int j = A.plane_chunk[item];
+        const int2 pm = A.plane_map[base + item];
+        const int j = pm.x;
         const ChunkMeta& M = A.meta[j];
         const int dim = M.dim;
-        if (((dim + 63) >> 6) != W) continue;
-        const int x = (int)(item - A.plane_prefix[j]);
+        const int x = pm.y;
+        const int64_t qslot = (base + item) * kMaxPlaneSplit + seg;  // this segment's refine-queue count
         const int y0 = (int)((int64_t)seg * dim / split), y1 = (int)((int64_t)(seg + 1) * dim / split);
         uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
         uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
         if (!(M.any_air && M.any_solid)) {  // no surface in this chunk
             for (int y = y0 + threadIdx.x; y < y1; y += blockDim.x) rv[y] = rt[y] = 0u;
-            if (threadIdx.x == 0) A.plane_refine_n[it] = 0;
+            if (threadIdx.x == 0) A.plane_refine_n[qslot] = 0;
             continue;
         }
         __syncthreads();  // previous item is done with the shared arrays
@@ -612,7 +643,7 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) A.plane_refine_n[it] = s_qn;
+        if (threadIdx.x == 0) A.plane_refine_n[qslot] = s_qn;
     }
 }
 
@@ -965,17 +996,18 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
         s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
     }
     const int wave = uniform_int(threadIdx.x >> 6);
-    const int split = A.ctr->plane_split;
-    const int64_t total = A.ctr->total_planes * split;
+    const int split = A.ctr->plane_split[W - 1];
+    const int64_t base = A.ctr->class_base[W - 1];
+    const int64_t total = A.ctr->class_planes[W - 1] * split;
     for (int64_t it = blockIdx.x; it < total; it += gridDim.x) {
         const int64_t item = it / split;
         const int seg = (int)(it - item * split);
-        const int j = A.plane_chunk[item];
+        const int2 pm = A.plane_map[base + item];
+        const int j = pm.x;
         const ChunkMeta& M = A.meta[j];
         const int dim = M.dim;
-        if (((dim + 63) >> 6) != W) continue;
         if (M.status != BMF_OK || M.V == 0) continue;  // uniform across the block
-        const int x = (int)(item - A.plane_prefix[j]);
+        const int x = pm.y;
         const int y0 = (int)((int64_t)seg * dim / split), y1 = (int)((int64_t)(seg + 1) * dim / split);
         __syncthreads();  // previous item is done with the shared arrays
         stage_planes(s_planes[0], s_planes[1], s_planes[2], A.labels + M.lab_off, M.lab_words, dim, x);
@@ -1396,24 +1428,32 @@ hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int k
 #undef BMF_SAMPLE
     return hipGetLastError();
 }
-hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
-    if (wclasses & 1) hipLaunchKernelGGL(k_count<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 2) hipLaunchKernelGGL(k_count<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 4) hipLaunchKernelGGL(k_count<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 8) hipLaunchKernelGGL(k_count<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs) {
+    if (wclasses & 1) hipLaunchKernelGGL(k_count<1>, dim3(blocks), dim3(kRowThreads), 0, cs[0], A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_count<2>, dim3(blocks), dim3(kRowThreads), 0, cs[1], A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_count<3>, dim3(blocks), dim3(kRowThreads), 0, cs[2], A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_count<4>, dim3(blocks), dim3(kRowThreads), 0, cs[3], A, n);
     return hipGetLastError();
+}
+hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
+    const hipStream_t cs[4] = {s, s, s, s};
+    return launch_count(A, n, blocks, wclasses, cs);
 }
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_scan, dim3(blocks), dim3(kScanThreads), 0, s, A, n);
     hipLaunchKernelGGL(k_plan_smooth, dim3(1), dim3(kPlanThreads), 0, s, A, n);
     return hipGetLastError();
 }
-hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
-    if (wclasses & 1) hipLaunchKernelGGL(k_emit<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 2) hipLaunchKernelGGL(k_emit<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 4) hipLaunchKernelGGL(k_emit<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
-    if (wclasses & 8) hipLaunchKernelGGL(k_emit<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs) {
+    if (wclasses & 1) hipLaunchKernelGGL(k_emit<1>, dim3(blocks), dim3(kRowThreads), 0, cs[0], A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_emit<2>, dim3(blocks), dim3(kRowThreads), 0, cs[1], A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_emit<3>, dim3(blocks), dim3(kRowThreads), 0, cs[2], A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_emit<4>, dim3(blocks), dim3(kRowThreads), 0, cs[3], A, n);
     return hipGetLastError();
+}
+hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
+    const hipStream_t cs[4] = {s, s, s, s};
+    return launch_emit(A, n, blocks, wclasses, cs);
 }
 hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kSampleThreads), 0, s, A, n);
