@@ -30,8 +30,7 @@ namespace bmf {
 constexpr int kPlanThreads = 1024;
 
 __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allow_cull, int reset_bumps) {
-    __shared__ int64_t s_scan[4][kPlanThreads];
-    __shared__ int32_t s_cls[4][kPlanThreads];  // x-planes per row-word class
+    __shared__ int64_t s_w[8][kPlanThreads / 64 + 1];
     const int tid = threadIdx.x;
     const SamplerDev* __restrict__ S = A.sampler;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
@@ -87,33 +86,22 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         m.plane_slot = 0;
         A.meta[j] = m;
     }
+    // exclusive prefixes and totals over the block: the four arena sizes,
+    // then the x-planes of each row-word class
+    int64_t vals[8], excl[8], tot[8];
 #pragma unroll
-    for (int q = 0; q < 4; q++) s_scan[q][tid] = sum[q];
-#pragma unroll
-    for (int q = 0; q < 4; q++) s_cls[q][tid] = cls[q];
-    __syncthreads();
-    // Hillis-Steele inclusive scan across threads
-    for (int off = 1; off < kPlanThreads; off <<= 1) {
-        int64_t v[4];
-        int32_t u[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) v[q] = tid >= off ? s_scan[q][tid - off] : 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) u[q] = tid >= off ? s_cls[q][tid - off] : 0;
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; q++) s_scan[q][tid] += v[q];
-#pragma unroll
-        for (int q = 0; q < 4; q++) s_cls[q][tid] += u[q];
-        __syncthreads();
+    for (int q = 0; q < 4; q++) {
+        vals[q] = sum[q];
+        vals[4 + q] = cls[q];
     }
+    block_scan<8, kPlanThreads>(vals, excl, tot, s_w);
     int64_t base[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) base[q] = s_scan[q][tid] - sum[q];
-    const int64_t tot_samples = s_scan[0][kPlanThreads - 1];
-    const int64_t tot_words = s_scan[1][kPlanThreads - 1];
-    const int64_t tot_rows = s_scan[2][kPlanThreads - 1];
-    const int64_t tot_tiles = s_scan[3][kPlanThreads - 1];
+    for (int q = 0; q < 4; q++) base[q] = excl[q];
+    const int64_t tot_samples = tot[0];
+    const int64_t tot_words = tot[1];
+    const int64_t tot_rows = tot[2];
+    const int64_t tot_tiles = tot[3];
     const bool overflow = tot_samples > A.cap_samples || tot_words > A.cap_samples / 64 + A.cap_chunks ||
                           tot_rows > A.cap_rows;
     int64_t tp = base[3];
@@ -123,8 +111,8 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         int64_t cb = 0;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            cslot[q] = cb + s_cls[q][tid] - cls[q];
-            cb += s_cls[q][kPlanThreads - 1];
+            cslot[q] = cb + excl[4 + q];
+            cb += tot[4 + q];
         }
     }
     for (int j = j0; j < j1; j++) {
@@ -167,7 +155,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         // plane's rows so that the class still has kSplitTarget items)
         int64_t cb = 0;
         for (int q = 0; q < 4; q++) {
-            const int64_t np = overflow ? 0 : (int64_t)s_cls[q][kPlanThreads - 1];
+            const int64_t np = overflow ? 0 : tot[4 + q];
             int split = A.row_split > 0 ? A.row_split : 1;
             if (A.row_split <= 0)
                 while (split < kMaxPlaneSplit && np * split < kSplitTarget) split <<= 1;
@@ -753,23 +741,17 @@ __device__ __forceinline__ bool smooths(const ChunkMeta& M, int normals) {
 // round-robin by block index; a different dispatch order only costs
 // locality, not correctness).
 __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
-    __shared__ int64_t s_scan[3][kPlanThreads];
+    __shared__ int64_t s_w[2][kPlanThreads / 64 + 1];
     const int tid = threadIdx.x;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
     const int j0 = tid * per, j1 = min(n, j0 + per);
     // ---- rank of each smoothing chunk (ordered compaction) ----
     int64_t cnt = 0;
     for (int j = j0; j < j1; j++) cnt += smooths(A.meta[j], A.normals);
-    s_scan[0][tid] = cnt;
-    __syncthreads();
-    for (int off = 1; off < kPlanThreads; off <<= 1) {
-        const int64_t a = tid >= off ? s_scan[0][tid - off] : 0;
-        __syncthreads();
-        s_scan[0][tid] += a;
-        __syncthreads();
-    }
-    const int ns = (int)s_scan[0][kPlanThreads - 1];
-    int k = (int)(s_scan[0][tid] - cnt);
+    int64_t c1[1] = {cnt}, e1[1], t1[1];
+    block_scan<1, kPlanThreads>(c1, e1, t1, s_w);
+    const int ns = (int)t1[0];
+    int k = (int)e1[0];
     for (int j = j0; j < j1; j++) {
         if (!smooths(A.meta[j], A.normals)) continue;
         // grouped position: groups g = k % 8 in order, rank k / 8 inside
@@ -789,18 +771,9 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
         tv += (M.V + kSmoothThreads - 1) / kSmoothThreads;
         tt += (M.T + kSmoothThreads - 1) / kSmoothThreads;
     }
-    __syncthreads();
-    s_scan[1][tid] = tv;
-    s_scan[2][tid] = tt;
-    __syncthreads();
-    for (int off = 1; off < kPlanThreads; off <<= 1) {
-        const int64_t a = tid >= off ? s_scan[1][tid - off] : 0, b = tid >= off ? s_scan[2][tid - off] : 0;
-        __syncthreads();
-        s_scan[1][tid] += a;
-        s_scan[2][tid] += b;
-        __syncthreads();
-    }
-    int64_t bv = s_scan[1][tid] - tv, bt = s_scan[2][tid] - tt;
+    int64_t c2[2] = {tv, tt}, e2[2], t2[2];
+    block_scan<2, kPlanThreads>(c2, e2, t2, s_w);
+    int64_t bv = e2[0], bt = e2[1];
     for (int p = p0; p < p1; p++) {
         const ChunkMeta& M = A.meta[A.smooth_list[p]];
         A.smooth_prefix[p] = bv;
@@ -812,10 +785,10 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
     __syncthreads();
     if (tid == kPlanThreads - 1) {
         A.ctr->n_smooth = ns;
-        A.smooth_prefix[ns] = s_scan[1][tid];
-        A.smooth_tprefix[ns] = s_scan[2][tid];
-        A.ctr->total_vtiles = s_scan[1][tid];
-        A.ctr->total_ttiles = s_scan[2][tid];
+        A.smooth_prefix[ns] = t2[0];
+        A.smooth_tprefix[ns] = t2[1];
+        A.ctr->total_vtiles = t2[0];
+        A.ctr->total_ttiles = t2[1];
     }
     __syncthreads();
     // ---- tile range of each XCD group ----

@@ -25,6 +25,48 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// Exclusive prefix and total over a block of NT threads (NT / 64 <= 64
+// waves) of N values per thread, one value at a time: lane shuffles inside
+// each wave, then wave 0 scans the wave totals.  s_w: N x (NT / 64 + 1)
+// scratch in LDS (the last column holds the totals).
+template <int N, int NT>
+__device__ __forceinline__ void block_scan(const int64_t (&v)[N], int64_t (&excl)[N], int64_t (&tot)[N],
+                                           int64_t (*s_w)[NT / 64 + 1]) {
+    constexpr int NW = NT / 64;
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    int64_t inc[N];
+    for (int q = 0; q < N; q++) {
+        int64_t x = v[q];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int64_t t = __shfl_up(x, off);
+            if (lane >= off) x += t;
+        }
+        inc[q] = x;
+        if (lane == 63) s_w[q][w] = x;
+    }
+    __syncthreads();
+    if (w == 0) {
+        for (int q = 0; q < N; q++) {
+            const int64_t x = lane < NW ? s_w[q][lane] : 0;
+            int64_t y = x;
+#pragma unroll
+            for (int off = 1; off < NW; off <<= 1) {
+                const int64_t t = __shfl_up(y, off);
+                if (lane >= off) y += t;
+            }
+            if (lane < NW) s_w[q][lane] = y - x;
+            if (lane == NW - 1) s_w[q][NW] = y;
+        }
+    }
+    __syncthreads();
+    for (int q = 0; q < N; q++) {
+        excl[q] = s_w[q][w] + inc[q] - v[q];
+        tot[q] = s_w[q][NW];
+    }
+    __syncthreads();  // the scratch may be reused
+}
+
 // Chunk j whose half-open range [prefix[j], prefix[j+1]) holds g.
 __device__ __forceinline__ int find_chunk(const int64_t* __restrict__ prefix, int n, int64_t g) {
     int lo = 0, hi = n;
