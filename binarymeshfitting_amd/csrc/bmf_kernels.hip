@@ -472,11 +472,21 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
 // their last octave (k_sample's adaptive evaluation).
 template <int W>
 __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, const uint64_t* const* s_lab,
-                                          const uint8_t* s_ntri, int x, int y, uint32_t* __restrict__ q,
-                                          uint32_t* s_qn, uint32_t& nv, uint32_t& nt) {
+                                          const uint8_t* s_ntri, const uint8_t (*s_cls)[BMF_MAX_DIM], int x, int y,
+                                          uint32_t* __restrict__ q, uint32_t* s_qn, uint32_t& nv, uint32_t& nt) {
     const int dim = M.dim;
     const int lane = lane_id();
     const bool x1 = x + 1 < dim, y1 = y + 1 < dim;
+    nv = 0;
+    nt = 0;
+    // a row inside one uniform region (the row, and its neighbours at x-1,
+    // x+1, y-1, y+1, x+1/y+1, all one constant label) has nothing; decided
+    // from the staged rows' classes (0: every label 0, 1: every label 1,
+    // 2: mixed) before any row is read
+    const uint32_t c00 = s_cls[1][y];
+    if (c00 != 2u && (!y1 || s_cls[1][y + 1] == c00) && (!x1 || s_cls[2][y] == c00) &&
+        (!(x1 && y1) || s_cls[2][y + 1] == c00) && (x < 1 || s_cls[0][y] == c00) && (y < 1 || s_cls[1][y - 1] == c00))
+        return;
     // wave-uniform rows (the same words in every lane); per lane only single bits
     const RowBits<W> r00 = lds_row<W>(s_lab[1], dim, y, true);
     const RowBits<W> r01 = lds_row<W>(s_lab[1], dim, y + 1, y1);
@@ -484,15 +494,6 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
     const RowBits<W> r11 = lds_row<W>(s_lab[2], dim, y + 1, x1 && y1);
     const RowBits<W> rxm = lds_row<W>(s_lab[0], dim, y, x >= 1);
     const RowBits<W> rym = lds_row<W>(s_lab[1], dim, y - 1, y >= 1);
-    nv = 0;
-    nt = 0;
-    // a row inside one uniform region (the row, and its neighbours at
-    // x-1, x+1, y-1, y+1, x+1/y+1, all one constant label) has nothing
-    if ((row_equal<W>(r00, zero_row<W>()) || row_equal<W>(r00, full_row<W>(dim))) &&
-        (!y1 || row_equal<W>(r01, r00)) && (!x1 || row_equal<W>(r10, r00)) &&
-        (!(x1 && y1) || row_equal<W>(r11, r00)) && (x < 1 || row_equal<W>(rxm, r00)) &&
-        (y < 1 || row_equal<W>(rym, r00)))
-        return;
     const RowBits<W> ex = rxor<W>(r00, r10, x1), ey = rxor<W>(r00, r01, y1), ez = zedges<W>(r00, dim);
     const bool quads = A.mesh_mode == BMF_MESH_QUADS;
     if (quads) {
@@ -589,6 +590,7 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
     __shared__ uint8_t s_ntri[256];
     __shared__ uint64_t s_planes[4][plane_words(W)];
     const uint64_t* const s_lab[4] = {s_planes[0], s_planes[1], s_planes[2], s_planes[3]};
+    __shared__ uint8_t s_cls[3][BMF_MAX_DIM];  // rows of the staged label planes: 0 / 1 every label 0 / 1, 2 mixed
     __shared__ uint32_t s_qn;
     const bool quads = A.mesh_mode == BMF_MESH_QUADS;
     for (int t = threadIdx.x; t < 256; t += blockDim.x)
@@ -620,11 +622,30 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
         stage_planes(s_planes[0], s_planes[1], s_planes[2], A.labels + M.lab_off, M.lab_words, dim, x - 1);
         stage_plane(s_planes[3], A.exact + M.lab_off, M.lab_words, dim, x);
         __syncthreads();
+        // class of every staged label row, one thread per row
+        for (int t = threadIdx.x; t < 3 * dim; t += blockDim.x) {
+            const int p = t / dim, y = t - p * dim;
+            const uint64_t* P = s_planes[p];
+            bool any1 = false, any0 = false;
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                const int bit = y * dim + 64 * k, w = bit >> 6, sh = bit & 63;
+                uint64_t v = P[w];
+                if (sh) v = (v >> sh) | (P[w + 1] << (64 - sh));
+                const int nb = dim - 64 * k;
+                const uint64_t m = nb >= 64 ? ~0ull : ((1ull << nb) - 1ull);
+                v &= m;
+                any1 = any1 || v != 0ull;
+                any0 = any0 || v != m;
+            }
+            s_cls[p][y] = (uint8_t)(any1 ? (any0 ? 2 : 1) : 0);
+        }
+        __syncthreads();
         // this segment's region of the plane's refine queue: its rows' samples
         uint32_t* q = A.refine + M.dens_off + (int64_t)x * dim * dim + (int64_t)y0 * dim;
         for (int y = y0 + wave; y < y1; y += kRowThreads / 64) {
             uint32_t nv = 0, nt = 0;
-            count_row<W>(A, M, s_lab, s_ntri, x, y, q, &s_qn, nv, nt);
+            count_row<W>(A, M, s_lab, s_ntri, s_cls, x, y, q, &s_qn, nv, nt);
             if (lane == 0) {
                 rv[y] = nv;
                 rt[y] = nt;
