@@ -364,6 +364,7 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
         const bool smooth = M.iters > 0 && M.T > 0;
         // ---- row pass ----
         for (int y = wave; y < dim - 1; y += kRowThreads / 64) {
+            if (rv[y + 1] == rv[y]) continue;  // the row's cells have no polygon
             const RowBits<W> r00 = lds_row<W>(s_planes[0], dim, y, true);
             const RowBits<W> r01 = lds_row<W>(s_planes[0], dim, y + 1, true);
             const RowBits<W> r10 = lds_row<W>(s_planes[1], dim, y, true);
@@ -443,6 +444,10 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
         uint32_t* __restrict__ I = A.idx + M.i_off;
         for (int y = wave; y < dim; y += kRowThreads / 64) {
             const bool x1 = x + 1 < dim, y1 = y + 1 < dim;
+            // rows whose samples own no quad (the next row's base, or the next
+            // plane's, or the chunk's total after the last row) skip the label work
+            const uint32_t next = y1 ? rt[y + 1] : (x1 ? rt[dim] : (uint32_t)M.T);
+            if (next == rt[y]) continue;
             const RowBits<W> r00 = lds_row<W>(s_planes[0], dim, y, true);
             const RowBits<W> r01 = lds_row<W>(s_planes[0], dim, y + 1, y1);
             const RowBits<W> r10 = lds_row<W>(s_planes[1], dim, y, x1);
