@@ -29,8 +29,9 @@ constexpr float kCsgFloor = -16.0f;
 
 __device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) { return __umul24(a, b); }
 
-__device__ __forceinline__ uint32_t grad_index(int32_t i, int32_t j, int32_t k, uint32_t seed) {
-    uint32_t h = mul24((uint32_t)i, kP1) ^ mul24((uint32_t)j, kP2) ^ mul24((uint32_t)k, kP3) ^ seed;
+// Gradient index of a lattice point from its combined coordinate hash
+// h = mul24(i, P1) ^ mul24(j, P2) ^ mul24(k, P3) ^ seed.
+__device__ __forceinline__ uint32_t grad_mix(uint32_t h) {
     h ^= h >> 13;
     h = mul24(h ^ (h >> 24), kM1);
     h ^= h >> 11;
@@ -71,10 +72,15 @@ __device__ __forceinline__ float snoise3(float x, float y, float z, uint32_t see
     float x1 = (x0 - (float)i1) + G3, y1 = (y0 - (float)j1) + G3, z1 = (z0 - (float)k1) + G3;
     float x2 = (x0 - (float)i2) + G3x2, y2 = (y0 - (float)j2) + G3x2, z2 = (z0 - (float)k2) + G3x2;
     float x3 = (x0 - 1.0f) + 0.5f, y3 = (y0 - 1.0f) + 0.5f, z3 = (z0 - 1.0f) + 0.5f;
-    float n0 = corner(x0, y0, z0, grad_index(i, j, k, seed));
-    float n1 = corner(x1, y1, z1, grad_index(i + i1, j + j1, k + k1, seed));
-    float n2 = corner(x2, y2, z2, grad_index(i + i2, j + j2, k + k2, seed));
-    float n3 = corner(x3, y3, z3, grad_index(i + 1, j + 1, k + 1, seed));
+    // the corners' lattice coordinates are i or i + 1 per axis: six products
+    // (the seed folded into the z terms) serve the four corner hashes
+    const uint32_t a0 = mul24((uint32_t)i, kP1), a1 = mul24((uint32_t)(i + 1), kP1);
+    const uint32_t b0 = mul24((uint32_t)j, kP2), b1 = mul24((uint32_t)(j + 1), kP2);
+    const uint32_t c0 = mul24((uint32_t)k, kP3) ^ seed, c1 = mul24((uint32_t)(k + 1), kP3) ^ seed;
+    float n0 = corner(x0, y0, z0, grad_mix(a0 ^ b0 ^ c0));
+    float n1 = corner(x1, y1, z1, grad_mix((i1 ? a1 : a0) ^ (j1 ? b1 : b0) ^ (k1 ? c1 : c0)));
+    float n2 = corner(x2, y2, z2, grad_mix((i2 ? a1 : a0) ^ (j2 ? b1 : b0) ^ (k2 ? c1 : c0)));
+    float n3 = corner(x3, y3, z3, grad_mix(a1 ^ b1 ^ c1));
     float n = ((n0 + n1) + n2) + n3;
     float r = kNoiseScale * n;
     return fminf(fmaxf(r, -1.0f), 1.0f);
@@ -89,10 +95,11 @@ __device__ __forceinline__ float snoise3(float x, float y, float z, uint32_t see
 //   per corner: 3 fma (6) + max + 2 mul + final mul + 4 selects + add = 16,
 //   x 4 = 64;  sum 3 + scale 1 + clamp 2 = 6.   Total 117.
 constexpr int kSimplexFlops = 117;
-// Integer ops of the lattice hash per simplex evaluation: 4 corners x
-// (3 mul24 + 3 xor for the combine, 2 mul24 + 6 xor/shift for the mix,
-// 2 shift/and for the index) plus 9 adds for the corner lattice points.
-constexpr int kHashIntOps = 73;
+// Integer ops of the lattice hash per simplex evaluation: 6 mul24 + 2 xor
+// for the per-axis terms, 12 xor + 6 selects to combine them per corner,
+// and per corner 2 mul24 + 6 xor/shift for the mix and 2 shift/and for the
+// index (4 x 10), plus 3 adds for the i + 1 lattice points.
+constexpr int kHashIntOps = 69;
 // Per octave outside snoise3: 3 coordinate multiplies + the fma accumulate.
 constexpr int kOctaveFlops = 5;
 // Per terrain sample outside the octave loop: 3 x (cvt + mul + add) for the
