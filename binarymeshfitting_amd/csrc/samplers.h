@@ -69,9 +69,12 @@ __device__ __forceinline__ float snoise3(float x, float y, float z, uint32_t see
     bool gxy = x0 >= y0, gyz = y0 >= z0, gxz = x0 >= z0;
     int i1 = gxy & gxz, j1 = (!gxy) & gyz, k1 = (!gxz) & (!gyz);
     int i2 = gxy | gxz, j2 = (!gxy) | gyz, k2 = (!gxz) | (!gyz);
-    float x1 = (x0 - (float)i1) + G3, y1 = (y0 - (float)j1) + G3, z1 = (z0 - (float)k1) + G3;
-    float x2 = (x0 - (float)i2) + G3x2, y2 = (y0 - (float)j2) + G3x2, z2 = (z0 - (float)k2) + G3x2;
-    float x3 = (x0 - 1.0f) + 0.5f, y3 = (y0 - 1.0f) + 0.5f, z3 = (z0 - 1.0f) + 0.5f;
+    // x0 - (float)i1 is x0 - 1 or exactly x0 (i1 is 0 or 1), so the middle
+    // corners select between x0 and the last corner's x0 - 1
+    const float xm = x0 - 1.0f, ym = y0 - 1.0f, zm = z0 - 1.0f;
+    float x1 = (i1 ? xm : x0) + G3, y1 = (j1 ? ym : y0) + G3, z1 = (k1 ? zm : z0) + G3;
+    float x2 = (i2 ? xm : x0) + G3x2, y2 = (j2 ? ym : y0) + G3x2, z2 = (k2 ? zm : z0) + G3x2;
+    float x3 = xm + 0.5f, y3 = ym + 0.5f, z3 = zm + 0.5f;
     // the corners' lattice coordinates are i or i + 1 per axis: six products
     // (the seed folded into the z terms) serve the four corner hashes
     const uint32_t a0 = mul24((uint32_t)i, kP1), a1 = mul24((uint32_t)(i + 1), kP1);
@@ -86,20 +89,22 @@ __device__ __forceinline__ float snoise3(float x, float y, float z, uint32_t see
     return fminf(fmaxf(r, -1.0f), 1.0f);
 }
 
-// FP32 operation count of snoise3 as written, one per add/sub/mul/min/max/
-// floor/convert/compare/select on float data, two per fma; the integer hash
-// work is excluded (reported separately as kHashIntOps):
+// FP32 operation count of one simplex evaluation as defined (the direct
+// sequence; snoise3 shares a few of these between corners, with identical
+// results), one per add/sub/mul/min/max/floor/convert/compare/select on
+// float data, two per fma; the integer hash work is excluded (reported
+// separately as kHashIntOps):
 //   skew: 2 add + 1 mul = 3;  x+s: 3 add + 3 floor + 3 cvt = 9;
 //   t: 1 cvt + 1 mul = 2;  x0: 6 sub;  rank compares: 3;
 //   corners 1 and 2: 2 x (3 cvt + 3 sub + 3 add) = 18;  corner 3: 6;
 //   per corner: 3 fma (6) + max + 2 mul + final mul + 4 selects + add = 16,
 //   x 4 = 64;  sum 3 + scale 1 + clamp 2 = 6.   Total 117.
 constexpr int kSimplexFlops = 117;
-// Integer ops of the lattice hash per simplex evaluation: 6 mul24 + 2 xor
-// for the per-axis terms, 12 xor + 6 selects to combine them per corner,
-// and per corner 2 mul24 + 6 xor/shift for the mix and 2 shift/and for the
-// index (4 x 10), plus 3 adds for the i + 1 lattice points.
-constexpr int kHashIntOps = 69;
+// Integer ops of the lattice hash per simplex evaluation as defined: 4
+// corners x (3 mul24 + 3 xor for the combine, 2 mul24 + 6 xor/shift for the
+// mix, 2 shift/and for the index) plus 9 adds for the corner lattice points
+// (snoise3 shares the per-axis products between corners).
+constexpr int kHashIntOps = 73;
 // Per octave outside snoise3: 3 coordinate multiplies + the fma accumulate.
 constexpr int kOctaveFlops = 5;
 // Per terrain sample outside the octave loop: 3 x (cvt + mul + add) for the
