@@ -117,10 +117,12 @@ __device__ void csg_gradient(const SamplerDev& S, const uint16_t* list, int ns, 
     }
 }
 
-// The same with the chunk's primitives staged in LDS (stage_csg layout).
+// The same with the chunk's primitives staged in LDS (stage_csg layout):
+// the loop finds the winning primitive (first index on ties), whose
+// gradient is then formed once, with the same operations as above.
 __device__ void csg_gradient_lds(const float4* s_csg, int ns, int nb, float x, float y, float z, float g[3]) {
     float best = kCsgFloor;
-    g[0] = g[1] = g[2] = 0.0f;
+    int win = -1;
     for (int k = 0; k < ns; k++) {
         const float4 sp = s_csg[2 * k];
         const float dx = x - sp.x, dy = y - sp.y, dz = z - sp.z;
@@ -128,10 +130,7 @@ __device__ void csg_gradient_lds(const float4* s_csg, int ns, int nb, float x, f
         const float v = sp.w - l;
         if (v > best) {
             best = v;
-            const float inv = l > 0.0f ? 1.0f / l : 0.0f;
-            g[0] = -dx * inv;
-            g[1] = -dy * inv;
-            g[2] = -dz * inv;
+            win = k;
         }
     }
     for (int k = ns; k < ns + nb; k++) {
@@ -144,19 +143,38 @@ __device__ void csg_gradient_lds(const float4* s_csg, int ns, int nb, float x, f
         const float v = -(outside + inside);
         if (v > best) {
             best = v;
-            const float sx = px < 0.0f ? -1.0f : 1.0f, sy = py < 0.0f ? -1.0f : 1.0f, sz = pz < 0.0f ? -1.0f : 1.0f;
-            if (outside > 0.0f) {
-                const float inv = 1.0f / outside;
-                g[0] = -mx * sx * inv;
-                g[1] = -my * sy * inv;
-                g[2] = -mz * sz * inv;
-            } else {
-                g[0] = g[1] = g[2] = 0.0f;
-                if (qx >= qy && qx >= qz) g[0] = -sx;
-                else if (qy >= qz) g[1] = -sy;
-                else g[2] = -sz;
-            }
+            win = k;
         }
+    }
+    g[0] = g[1] = g[2] = 0.0f;
+    if (win < 0) return;  // the floor
+    if (win < ns) {
+        const float4 sp = s_csg[2 * win];
+        const float dx = x - sp.x, dy = y - sp.y, dz = z - sp.z;
+        const float l = sqrtf((dx * dx + dy * dy) + dz * dz);
+        const float inv = l > 0.0f ? 1.0f / l : 0.0f;
+        g[0] = -dx * inv;
+        g[1] = -dy * inv;
+        g[2] = -dz * inv;
+        return;
+    }
+    const float4 b0 = s_csg[2 * win], b1 = s_csg[2 * win + 1];
+    const float px = x - b0.x, py = y - b0.y, pz = z - b0.z;
+    const float qx = fabsf(px) - b0.w, qy = fabsf(py) - b1.x, qz = fabsf(pz) - b1.y;
+    const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+    const float outside = sqrtf((mx * mx + my * my) + mz * mz);
+    const float sx = px < 0.0f ? -1.0f : 1.0f, sy = py < 0.0f ? -1.0f : 1.0f, sz = pz < 0.0f ? -1.0f : 1.0f;
+    if (outside > 0.0f) {
+        const float inv = 1.0f / outside;
+        g[0] = -mx * sx * inv;
+        g[1] = -my * sy * inv;
+        g[2] = -mz * sz * inv;
+    } else if (qx >= qy && qx >= qz) {
+        g[0] = -sx;
+    } else if (qy >= qz) {
+        g[1] = -sy;
+    } else {
+        g[2] = -sz;
     }
 }
 
