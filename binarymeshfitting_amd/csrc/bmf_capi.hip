@@ -37,7 +37,7 @@ void free_arenas(bmf_ctx* c) {
     void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plane_map, A.smooth_list, A.smooth_prefix,
                     A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.smooth_vmap, A.smooth_tmap, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
-                    A.pos,     A.pos4,      A.boundary, A.vedge,       A.idx,          A.ctr,
+                    A.pos,     A.pos4,      A.boundary, A.vedge,       A.vring,        A.idx,          A.ctr,
                     A.sampler, c->d_spheres, c->d_boxes, A.ring, A.ring_n, A.qef_err, A.nrm4, A.fn4, A.nrm,
                     A.csg_list};
     for (void* p : ptrs)
@@ -228,6 +228,7 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
          dalloc(&A.cell_tb, A.cap_samples) == hipSuccess && dalloc(&A.cell_mask, A.cap_samples) == hipSuccess &&
          dalloc(&A.pos, 3 * A.cap_vertices) == hipSuccess && dalloc(&A.pos4, A.cap_vertices) == hipSuccess &&
          dalloc(&A.boundary, A.cap_vertices) == hipSuccess && dalloc(&A.vedge, A.cap_vertices) == hipSuccess &&
+         dalloc(&A.vring, A.cap_vertices) == hipSuccess &&
          dalloc(&A.idx, A.cap_indices) == hipSuccess && dalloc(&A.ctr, 1) == hipSuccess &&
          dalloc(&A.sampler, 1) == hipSuccess;
     ok = ok && hipHostMalloc((void**)&c->h_descs[0], sizeof(bmf_chunk_desc) * nc) == hipSuccess &&

@@ -121,6 +121,7 @@ struct Arenas {
     float4* cen4;              // [max_indices / 3] triangle centroids
     uint8_t* boundary;         // [max_vertices]
     uint32_t* vedge;           // [max_vertices] lattice edge of each vertex: x | y<<8 | z<<16 | axis<<24
+    uint4* vring;              // [max_vertices] ring cells of each vertex, packed by the first smoothing step
     uint32_t* idx;             // [max_indices]
     Counters* ctr;
     SamplerDev* sampler;

@@ -1099,6 +1099,93 @@ __global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int m) {
     }
 }
 
+// A vertex's ring, packed: word c for ring cell c (the cells around the
+// vertex's lattice edge in c_ring order) = the cell's first triangle | the
+// bits of its triangles that use the edge << 27.  A chunk holds at most
+// 5 * 255^3 < 2^27 triangles.  The first smoothing step packs it (four
+// independent cell reads), later steps load it as one 16-byte word.
+constexpr uint32_t kRingFirst = (1u << 27) - 1u;
+
+// Ring cell c around an edge along `ax` is the edge origin offset by
+// (c >> 1) - 1 and (c & 1) - 1 along the other two axes in x, y, z order,
+// with local edge 4 ax + 3 - c: the rows of c_ring.
+__device__ __forceinline__ uint4 pack_ring(const uint32_t* __restrict__ CT, const uint8_t* __restrict__ CM,
+                                           uint32_t code, int dim, uint32_t T, bool& bad) {
+    const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
+    const int ax = (int)(code >> 24);
+    int64_t ci[4];
+    bool in[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int u = (c >> 1) - 1, w = (c & 1) - 1;
+        const int cx = ex + (ax == 0 ? 0 : u), cy = ey + (ax == 0 ? u : (ax == 1 ? 0 : w)), cz = ez + (ax == 2 ? 0 : w);
+        in[c] = cx >= 0 && cy >= 0 && cz >= 0 && cx <= dim - 2 && cy <= dim - 2 && cz <= dim - 2;
+        ci[c] = in[c] ? ((int64_t)cx * dim + cy) * dim + cz : 0;
+    }
+    uint32_t tb[4], mk[4], r[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        tb[c] = CT[ci[c]];
+        mk[c] = CM[ci[c]];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) mk[c] = c_edge_tris.m[mk[c]][4 * ax + 3 - c];  // any mask byte is a valid row
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        uint32_t tm = in[c] ? mk[c] : 0u;
+        if (tm && tb[c] + (uint32_t)(31 - __clz(tm)) >= T) {  // past the chunk's triangles
+            bad = true;
+            tm = 0u;
+        }
+        r[c] = (tm ? tb[c] & kRingFirst : 0u) | (tm << 27);
+    }
+    return make_uint4(r[0], r[1], r[2], r[3]);
+}
+
+// Sums of the ring's centroids (and face normals) from +0 in ascending
+// triangle id, the order of MeshProcessor<3>'s adjacency lists; returns the
+// ring size.  Four centroids are loaded per round, a lane's unused slots
+// reading its round's first triangle and adding nothing.
+template <bool NRM>
+__device__ __forceinline__ uint32_t ring_sums(uint4 r, const float4* __restrict__ C, const float4* __restrict__ F,
+                                              float (&s)[6]) {
+    uint32_t bits = (r.x >> 27) | ((r.y >> 27) << 5) | ((r.z >> 27) << 10) | ((r.w >> 27) << 15);
+    const uint32_t cnt = (uint32_t)__popc(bits);
+    // triangle of bit b = 5 c + slot: first(c) + slot = (first(c) - 5 c) + b, modulo 2^32
+    const uint32_t f0 = r.x & kRingFirst, f1 = (r.y & kRingFirst) - 5u, f2 = (r.z & kRingFirst) - 10u,
+                   f3 = (r.w & kRingFirst) - 15u;
+    while (bits) {
+        uint32_t t[4];
+        bool ok[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            ok[k] = bits != 0u;
+            const uint32_t b = ok[k] ? (uint32_t)(__ffs(bits) - 1) : 0u;
+            const uint32_t f = b < 5u ? f0 : (b < 10u ? f1 : (b < 15u ? f2 : f3));
+            t[k] = ok[k] ? f + b : (k == 0 ? 0u : t[0]);  // slot 0 is always used
+            bits &= bits - 1u;
+        }
+        float4 q[4], n[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            q[k] = C[t[k]];
+            if (NRM) n[k] = F[t[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            s[0] = ok[k] ? s[0] + q[k].x : s[0];
+            s[1] = ok[k] ? s[1] + q[k].y : s[1];
+            s[2] = ok[k] ? s[2] + q[k].z : s[2];
+            if (NRM) {
+                s[3] = ok[k] ? s[3] + n[k].x : s[3];
+                s[4] = ok[k] ? s[4] + n[k].y : s[4];
+                s[5] = ok[k] ? s[5] + n[k].z : s[5];
+            }
+        }
+    }
+    return cnt;
+}
+
 __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_v[xcd + 1];
@@ -1109,44 +1196,29 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
         if (K <= m) continue;
         const int64_t v = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
         if (v >= M.V) continue;
-        const int64_t vo = M.v_off, T = M.T;
-        if (!M.bp && A.boundary[vo + v]) continue;  // stays at its position in both buffers
-        const int dim = M.dim;
-        const float4* __restrict__ C = A.cen4 + M.i_off / 3;
-        const uint32_t* __restrict__ CT = A.cell_tb + M.dens_off;
-        const uint8_t* __restrict__ CM = A.cell_mask + M.dens_off;
-        const uint32_t code = A.vedge[vo + v];
-        const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
-        const int ax = (int)(code >> 24);
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        uint32_t cnt = 0;
-        for (int c = 0; c < 4; c++) {
-            const int cx = ex + c_ring[ax][c][0], cy = ey + c_ring[ax][c][1], cz = ez + c_ring[ax][c][2];
-            if (cx < 0 || cy < 0 || cz < 0 || cx > dim - 2 || cy > dim - 2 || cz > dim - 2) continue;
-            const int64_t ci = ((int64_t)cx * dim + cy) * dim + cz;
-            const uint32_t le = (uint32_t)c_ring[ax][c][3];
-            const uint32_t tb = CT[ci];
-            uint32_t tm = c_edge_tris.m[CM[ci]][le];
-            while (tm) {
-                const int64_t t = (int64_t)tb + (__ffs(tm) - 1);
-                tm &= tm - 1u;
-                if (t >= T) {
-                    A.ctr->error = 7;
-                    break;
-                }
-                const float4 q = C[t];
-                sx = sx + q.x;
-                sy = sy + q.y;
-                sz = sz + q.z;
-                cnt++;
-            }
+        // the chunk's fields are read before the stores below (which the
+        // compiler cannot tell apart from the chunk table)
+        const int64_t g = M.v_off + v, dens_off = M.dens_off, t_off = M.i_off / 3;
+        const int dim = M.dim, bp = M.bp;
+        const uint32_t T = (uint32_t)M.T;
+        if (!bp && A.boundary[g]) continue;  // stays at its position in both buffers
+        uint4 r;
+        if (m == 0) {
+            bool bad = false;
+            r = pack_ring(A.cell_tb + dens_off, A.cell_mask + dens_off, A.vedge[g], dim, T, bad);
+            if (bad) A.ctr->error = 7;
+        } else {
+            r = A.vring[g];
         }
+        float s[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        const uint32_t cnt = ring_sums<false>(r, A.cen4 + t_off, nullptr, s);
+        if (m == 0 && K > 1) A.vring[g] = r;  // stored after the loads: a store here is waited for with them
         if (cnt == 0) continue;  // MeshProcessor.cpp:247: no adjacency, unchanged
         const float fc = (float)cnt;
-        const float px = sx / fc, py = sy / fc, pz = sz / fc;
-        A.pos4[vo + v] = make_float4(px, py, pz, 0.0f);
+        const float px = s[0] / fc, py = s[1] / fc, pz = s[2] / fc;
+        A.pos4[g] = make_float4(px, py, pz, 0.0f);
         if (m == K - 1) {
-            float* __restrict__ P = A.pos + (vo + v) * 3;
+            float* __restrict__ P = A.pos + g * 3;
             P[0] = px;
             P[1] = py;
             P[2] = pz;
@@ -1237,41 +1309,21 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_nrm(Arenas A, int m) 
         if (steps <= m) continue;
         const int64_t v = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
         if (v >= M.V) continue;
-        const int64_t vo = M.v_off, T = M.T, g = vo + v;
-        const int dim = M.dim;
-        const float4* __restrict__ C = A.cen4 + M.i_off / 3;
-        const float4* __restrict__ F = A.fn4 + M.i_off / 3;
-        const uint32_t* __restrict__ CT = A.cell_tb + M.dens_off;
-        const uint8_t* __restrict__ CM = A.cell_mask + M.dens_off;
-        const uint32_t code = A.vedge[g];
-        const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
-        const int ax = (int)(code >> 24);
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
-        uint32_t cnt = 0;
-        for (int c = 0; c < 4; c++) {
-            const int cx = ex + c_ring[ax][c][0], cy = ey + c_ring[ax][c][1], cz = ez + c_ring[ax][c][2];
-            if (cx < 0 || cy < 0 || cz < 0 || cx > dim - 2 || cy > dim - 2 || cz > dim - 2) continue;
-            const int64_t ci = ((int64_t)cx * dim + cy) * dim + cz;
-            const uint32_t le = (uint32_t)c_ring[ax][c][3];
-            const uint32_t tb = CT[ci];
-            uint32_t tm = c_edge_tris.m[CM[ci]][le];
-            while (tm) {
-                const int64_t t = (int64_t)tb + (__ffs(tm) - 1);
-                tm &= tm - 1u;
-                if (t >= T) {
-                    A.ctr->error = 7;
-                    break;
-                }
-                const float4 q = C[t], f = F[t];
-                sx = sx + q.x;
-                sy = sy + q.y;
-                sz = sz + q.z;
-                nx = nx + f.x;
-                ny = ny + f.y;
-                nz = nz + f.z;
-                cnt++;
-            }
+        const int64_t g = M.v_off + v, dens_off = M.dens_off, t_off = M.i_off / 3;
+        const int dim = M.dim, bp = M.bp;
+        const uint32_t T = (uint32_t)M.T;
+        uint4 r;
+        if (m == 0) {
+            bool bad = false;
+            r = pack_ring(A.cell_tb + dens_off, A.cell_mask + dens_off, A.vedge[g], dim, T, bad);
+            if (bad) A.ctr->error = 7;
+        } else {
+            r = A.vring[g];
         }
+        float s[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        const uint32_t cnt = ring_sums<true>(r, A.cen4 + t_off, A.fn4 + t_off, s);
+        if (m == 0 && steps > 1) A.vring[g] = r;
+        const float sx = s[0], sy = s[1], sz = s[2], nx = s[3], ny = s[4], nz = s[5];
         float4 n = A.nrm4[g];
         if (cnt > 0) {
             const float fc = (float)cnt;
@@ -1287,7 +1339,7 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_nrm(Arenas A, int m) 
                 n = make_float4(ax2, ay2, az2, 0.0f);
                 A.nrm4[g] = n;
             }
-            if (K > 0 && (M.bp || !A.boundary[g])) {
+            if (K > 0 && (bp || !A.boundary[g])) {
                 const float px = sx / fc, py = sy / fc, pz = sz / fc;
                 A.pos4[g] = make_float4(px, py, pz, 0.0f);
                 if (m == K - 1) {
