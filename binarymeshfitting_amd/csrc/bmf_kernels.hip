@@ -1068,19 +1068,39 @@ __global__ void __launch_bounds__(kSmoothThreads) k_smooth_map(Arenas A) {
 // by k_primal: k_dual has consumed them); the last iteration also writes the
 // float3 output.
 // ------------------------------------------------------------------------
+// The chunk fields a smoothing tile reads, loaded together and kept in
+// scalar registers: a tile, so its chunk, is the whole block's.
+struct SmoothChunk {
+    int iters, dim, bp, V, T;
+    int64_t v_off, i_off, dens_off;
+};
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+    const uint64_t u = (uint64_t)v;
+    return (int64_t)(((uint64_t)(uint32_t)uniform_int((int)(u >> 32)) << 32) | (uint32_t)uniform_int((int)(uint32_t)u));
+}
+__device__ __forceinline__ SmoothChunk smooth_chunk(const ChunkMeta& M) {
+    const int iters = M.iters, dim = M.dim, bp = M.bp, V = M.V, T = M.T;
+    const int64_t v_off = M.v_off, i_off = M.i_off, dens_off = M.dens_off;
+    return SmoothChunk{uniform_int(iters), uniform_int(dim), uniform_int(bp), uniform_int(V), uniform_int(T),
+                       uniform_i64(v_off), uniform_i64(i_off), uniform_i64(dens_off)};
+}
+
 __global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int m) {
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_t[xcd + 1];
     for (int64_t tile = A.smooth_xcd_t[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
         const int2 tm = A.smooth_tmap[tile];  // (chunk, tile within the chunk)
-        const ChunkMeta& M = A.meta[tm.x];
+        const int chunk = uniform_int(tm.x), ty = uniform_int(tm.y);
+        const SmoothChunk M = smooth_chunk(A.meta[chunk]);
+        const int64_t T = M.T, i_off = M.i_off, v_off = M.v_off;
+        const uint32_t V = (uint32_t)M.V;
         if (M.iters <= m) continue;
-        const int64_t t = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
-        if (t >= M.T) continue;
-        const uint32_t* __restrict__ I = A.idx + M.i_off + 3 * t;
-        const float4* __restrict__ P = A.pos4 + M.v_off;
+        const int64_t t = (int64_t)ty * kSmoothThreads + threadIdx.x;
+        if (t >= T) continue;
+        const uint32_t* __restrict__ I = A.idx + i_off + 3 * t;
+        const float4* __restrict__ P = A.pos4 + v_off;
         const uint32_t i0 = I[0], i1 = I[1], i2 = I[2];
-        if (i0 >= (uint32_t)M.V || i1 >= (uint32_t)M.V || i2 >= (uint32_t)M.V) {
+        if (i0 >= V || i1 >= V || i2 >= V) {
             A.ctr->error = 5;
             continue;
         }
@@ -1095,7 +1115,7 @@ __global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int m) {
         cx = cx + c.x;
         cy = cy + c.y;
         cz = cz + c.z;
-        A.cen4[M.i_off / 3 + t] = make_float4(cx / 3.0f, cy / 3.0f, cz / 3.0f, 0.0f);
+        A.cen4[i_off / 3 + t] = make_float4(cx / 3.0f, cy / 3.0f, cz / 3.0f, 0.0f);
     }
 }
 
@@ -1191,24 +1211,24 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
     const int64_t end = A.smooth_xcd_v[xcd + 1];
     for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
         const int2 tm = A.smooth_vmap[tile];  // (chunk, tile within the chunk)
-        const ChunkMeta& M = A.meta[tm.x];
-        const int K = M.iters;
-        if (K <= m) continue;
-        const int64_t v = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
-        if (v >= M.V) continue;
-        // the chunk's fields are read before the stores below (which the
-        // compiler cannot tell apart from the chunk table)
-        const int64_t g = M.v_off + v, dens_off = M.dens_off, t_off = M.i_off / 3;
-        const int dim = M.dim, bp = M.bp;
+        const int chunk = uniform_int(tm.x), ty = uniform_int(tm.y);
+        const SmoothChunk M = smooth_chunk(A.meta[chunk]);
+        const int K = M.iters, dim = M.dim, bp = M.bp;
+        const int64_t V = M.V, v_off = M.v_off, dens_off = M.dens_off, t_off = M.i_off / 3;
         const uint32_t T = (uint32_t)M.T;
-        if (!bp && A.boundary[g]) continue;  // stays at its position in both buffers
-        uint4 r;
+        if (K <= m) continue;
+        const int64_t v = (int64_t)ty * kSmoothThreads + threadIdx.x;
+        if (v >= V) continue;
+        const int64_t g = v_off + v;
+        // the boundary flag, the lattice edge and the packed ring load together
+        const uint8_t bnd = A.boundary[g];
+        const uint32_t code = m == 0 ? A.vedge[g] : 0u;
+        uint4 r = m == 0 ? make_uint4(0u, 0u, 0u, 0u) : A.vring[g];
+        if (!bp && bnd) continue;  // stays at its position in both buffers
         if (m == 0) {
             bool bad = false;
-            r = pack_ring(A.cell_tb + dens_off, A.cell_mask + dens_off, A.vedge[g], dim, T, bad);
+            r = pack_ring(A.cell_tb + dens_off, A.cell_mask + dens_off, code, dim, T, bad);
             if (bad) A.ctr->error = 7;
-        } else {
-            r = A.vring[g];
         }
         float s[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         const uint32_t cnt = ring_sums<false>(r, A.cen4 + t_off, nullptr, s);
@@ -1241,10 +1261,11 @@ __global__ void __launch_bounds__(kSmoothThreads) k_dual_nrm(Arenas A, int m) {
     const int64_t end = A.smooth_xcd_t[xcd + 1];
     for (int64_t tile = A.smooth_xcd_t[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
         const int2 tm = A.smooth_tmap[tile];  // (chunk, tile within the chunk)
-        const ChunkMeta& M = A.meta[tm.x];
+        const int chunk = uniform_int(tm.x), ty = uniform_int(tm.y);
+        const SmoothChunk M = smooth_chunk(A.meta[chunk]);
         const int K = M.iters;
         if (max(K, 1) <= m) continue;
-        const int64_t t = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
+        const int64_t t = (int64_t)ty * kSmoothThreads + threadIdx.x;
         if (t >= M.T) continue;
         const uint32_t* __restrict__ I = A.idx + M.i_off + 3 * t;
         const float4* __restrict__ P = A.pos4 + M.v_off;
@@ -1303,11 +1324,12 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_nrm(Arenas A, int m) 
     const int64_t end = A.smooth_xcd_v[xcd + 1];
     for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
         const int2 tm = A.smooth_vmap[tile];  // (chunk, tile within the chunk)
-        const ChunkMeta& M = A.meta[tm.x];
+        const int chunk = uniform_int(tm.x), ty = uniform_int(tm.y);
+        const SmoothChunk M = smooth_chunk(A.meta[chunk]);
         const int K = M.iters;
         const int steps = max(K, 1);
         if (steps <= m) continue;
-        const int64_t v = (int64_t)tm.y * kSmoothThreads + threadIdx.x;
+        const int64_t v = (int64_t)ty * kSmoothThreads + threadIdx.x;
         if (v >= M.V) continue;
         const int64_t g = M.v_off + v, dens_off = M.dens_off, t_off = M.i_off / 3;
         const int dim = M.dim, bp = M.bp;
