@@ -362,18 +362,22 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
     const int64_t base = A.ctr->class_base[W - 1], total = A.ctr->class_planes[W - 1];
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
         const int2 pm = A.plane_map[base + item];
-        const int j = pm.x;
-        const ChunkMeta& M = A.meta[j];
+        const int j = uniform_int(pm.x);
+        const ChunkMeta M = read_chunk(A.meta, j);
         const int dim = M.dim;
         if (M.status != BMF_OK || M.V == 0) continue;
-        const int x = pm.y;
+        const int x = uniform_int(pm.y);
         if (x >= dim - 1) continue;
         const bool qef = A.placement == BMF_PLACE_QEF && S->kind == BMF_SAMPLER_CSG;
         const uint16_t* csg = A.csg_list ? A.csg_list + (int64_t)j * A.csg_stride : nullptr;
         const bool staged = qef && csg && M.csg_ns + M.csg_nb <= kCsgStage;  // uniform across the block
         __syncthreads();
-        stage_plane(s_planes[0], A.labels + M.lab_off, M.lab_words, dim, x);
-        stage_plane(s_planes[1], A.labels + M.lab_off, M.lab_words, dim, x + 1);
+        {
+            uint64_t* const dst[2] = {s_planes[0], s_planes[1]};
+            const uint64_t* const src[2] = {A.labels + M.lab_off, A.labels + M.lab_off};
+            const int xp[2] = {x, x + 1};
+            stage_plane_group<2>(dst, src, M.lab_words, dim, xp);
+        }
         if (staged) stage_csg(*S, csg, M.csg_ns, M.csg_nb, s_csg);
         __syncthreads();
         const uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
@@ -447,14 +451,17 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
     const bool tris = A.quad_tris != 0;
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
         const int2 pm = A.plane_map[base + item];
-        const int j = pm.x;
-        const ChunkMeta& M = A.meta[j];
+        const ChunkMeta M = read_chunk(A.meta, pm.x);
         const int dim = M.dim;
         if (M.status != BMF_OK || M.T == 0) continue;
-        const int x = pm.y;
+        const int x = uniform_int(pm.y);
         __syncthreads();
-        stage_plane(s_planes[0], A.labels + M.lab_off, M.lab_words, dim, x);
-        stage_plane(s_planes[1], A.labels + M.lab_off, M.lab_words, dim, x + 1);
+        {
+            uint64_t* const dst[2] = {s_planes[0], s_planes[1]};
+            const uint64_t* const src[2] = {A.labels + M.lab_off, A.labels + M.lab_off};
+            const int xp[2] = {x, x + 1};
+            stage_plane_group<2>(dst, src, M.lab_words, dim, xp);
+        }
         __syncthreads();
         const uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
         const bool smooth = M.iters > 0;

@@ -565,9 +565,9 @@ __global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A, int n) {
             const uint32_t cnt = A.plane_refine_n[(base + item) * kMaxPlaneSplit + seg];
             if (cnt == 0) continue;
             const int2 pm = A.plane_map[base + item];
-            const ChunkMeta& M = A.meta[pm.x];
+            const ChunkMeta M = read_chunk(A.meta, pm.x);
             const uint32_t dim = (uint32_t)M.dim;
-            const uint32_t x = (uint32_t)pm.y;
+            const uint32_t x = (uint32_t)uniform_int(pm.y);
             const uint32_t y0 = (uint32_t)((int64_t)seg * dim / split);
             const uint32_t* __restrict__ q = A.refine + M.dens_off + (int64_t)x * dim * dim + (int64_t)y0 * dim;
             float* __restrict__ D = A.dens + M.dens_off + (int64_t)x * dim * dim;
@@ -604,10 +604,9 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
         const int64_t item = it / split;
         const int seg = (int)(it - item * split);
         const int2 pm = A.plane_map[base + item];
-        const int j = pm.x;
-        const ChunkMeta& M = A.meta[j];
+        const ChunkMeta M = read_chunk(A.meta, pm.x);
         const int dim = M.dim;
-        const int x = pm.y;
+        const int x = uniform_int(pm.y);
         const int64_t qslot = (base + item) * kMaxPlaneSplit + seg;  // this segment's refine-queue count
         const int y0 = (int)((int64_t)seg * dim / split), y1 = (int)((int64_t)(seg + 1) * dim / split);
         uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
@@ -619,8 +618,14 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
         }
         __syncthreads();  // previous item is done with the shared arrays
         if (threadIdx.x == 0) s_qn = 0;
-        stage_planes(s_planes[0], s_planes[1], s_planes[2], A.labels + M.lab_off, M.lab_words, dim, x - 1);
-        stage_plane(s_planes[3], A.exact + M.lab_off, M.lab_words, dim, x);
+        {
+            // label planes x-1, x, x+1 and the exact-sample plane x
+            uint64_t* const dst[4] = {s_planes[0], s_planes[1], s_planes[2], s_planes[3]};
+            const uint64_t* const src[4] = {A.labels + M.lab_off, A.labels + M.lab_off, A.labels + M.lab_off,
+                                            A.exact + M.lab_off};
+            const int xp[4] = {x - 1, x, x + 1, x};
+            stage_plane_group<4>(dst, src, M.lab_words, dim, xp);
+        }
         __syncthreads();
         // class of every staged label row, one thread per row
         for (int t = threadIdx.x; t < 3 * dim; t += blockDim.x) {
@@ -887,17 +892,19 @@ __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M
             uint32_t id = kb + mbcnt64(ex) + mbcnt64(ey) + mbcnt64(ez);
             s_idrow[z] = id | (fx << 30) | (fy << 31);
             if (own && (fx | fy | fz)) {
-                const float s0 = D[((int64_t)x * dim + y) * dim + z];
+                // the sample and the far ends of its edges load together (an
+                // edge that is not set reads the sample again)
+                const int64_t b = ((int64_t)x * dim + y) * dim + z;
+                const float s0 = D[b];
+                const float e1[3] = {D[fx ? b + (int64_t)dim * dim : b], D[fy ? b + dim : b], D[fz ? b + 1 : b]};
+                const bool fits = id + fx + fy + fz <= Vc;
+                if (!fits) A.ctr->error = 2;  // counts and ids disagree: report, never write past the chunk
 #pragma unroll
                 for (int ax = 0; ax < 3; ax++) {
                     const uint32_t f = ax == 0 ? fx : (ax == 1 ? fy : fz);
-                    if (!f) continue;
-                    if (id >= Vc) {  // counts and ids disagree: report, never write past the chunk
-                        A.ctr->error = 2;
-                        break;
-                    }
+                    if (!f || !fits) continue;
                     const int xe = x + (ax == 0), ye = y + (ax == 1), ze = z + (ax == 2);
-                    const float s1 = D[((int64_t)xe * dim + ye) * dim + ze];
+                    const float s1 = e1[ax];
                     const float mu = (0.0f - s0) / (s1 - s0);
                     const float dx = (float)xe - (float)x, dy = (float)ye - (float)y, dz = (float)ze - (float)z;
                     const float qx = dx * mu + (float)x, qy = dy * mu + (float)y, qz = dz * mu + (float)z;
@@ -997,11 +1004,10 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
         const int64_t item = it / split;
         const int seg = (int)(it - item * split);
         const int2 pm = A.plane_map[base + item];
-        const int j = pm.x;
-        const ChunkMeta& M = A.meta[j];
+        const ChunkMeta M = read_chunk(A.meta, pm.x);
         const int dim = M.dim;
         if (M.status != BMF_OK || M.V == 0) continue;  // uniform across the block
-        const int x = pm.y;
+        const int x = uniform_int(pm.y);
         const int y0 = (int)((int64_t)seg * dim / split), y1 = (int)((int64_t)(seg + 1) * dim / split);
         __syncthreads();  // previous item is done with the shared arrays
         stage_planes(s_planes[0], s_planes[1], s_planes[2], A.labels + M.lab_off, M.lab_words, dim, x);
@@ -1013,9 +1019,10 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
         const int64_t R = (int64_t)dim * dim, r0 = (int64_t)x * dim;
         for (int y = threadIdx.x; y <= dim; y += blockDim.x) {
             const int64_t ra = r0 + y, rb = r0 + dim + y;
-            s_base[0][y] = ra < R ? rv[y] : (uint32_t)M.V;
-            s_base[1][y] = rb < R ? rv[dim + y] : (uint32_t)M.V;
-            s_base[2][y] = ra < R ? rt[y] : (uint32_t)M.T;
+            const uint32_t va = rv[ra < R ? y : 0], vb = rv[rb < R ? dim + y : 0], ta = rt[ra < R ? y : 0];
+            s_base[0][y] = ra < R ? va : (uint32_t)M.V;
+            s_base[1][y] = rb < R ? vb : (uint32_t)M.V;
+            s_base[2][y] = ra < R ? ta : (uint32_t)M.T;
         }
         const bool cellsx = x < dim - 1;  // plane x has cells (and plane x+1 exists)
         for (int ys = y0; ys < y1; ys += kEmitRows) {

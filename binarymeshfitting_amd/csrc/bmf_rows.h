@@ -20,6 +20,24 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Chunk j of the chunk table (j wave-uniform), for kernels that never write
+// the table: read through the constant address space, its fields load with
+// scalar loads into scalar registers and are not reloaded after the
+// kernel's own stores (which the compiler cannot otherwise tell apart from
+// the table).
+__device__ __forceinline__ ChunkMeta read_chunk(const ChunkMeta* table, int j) {
+    static_assert(sizeof(ChunkMeta) % 4 == 0, "ChunkMeta is copied in 32-bit words");
+    constexpr int kWords = (int)(sizeof(ChunkMeta) / 4);
+    using ConstWords = const __attribute__((address_space(4))) uint32_t*;
+    const ConstWords src = (ConstWords)(table + uniform_int(j));
+    uint32_t w[kWords];
+#pragma unroll
+    for (int i = 0; i < kWords; i++) w[i] = src[i];
+    ChunkMeta m;
+    __builtin_memcpy(&m, w, sizeof(ChunkMeta));
+    return m;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int l = lane_id();
     return l == 0 ? 0ull : (~0ull >> (64 - l));
@@ -231,33 +249,45 @@ __device__ __forceinline__ uint32_t cell_mask(const RowBits<W>& r00, const RowBi
 // to u64 words in LDS (coalesced funnel-shifted loads); rows are then read
 // from LDS.  Word pw of each plane is a zero pad for the funnel reads.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ void stage_plane(uint64_t* s_p, const uint64_t* __restrict__ L, int64_t nwords, int dim,
-                                            int xp) {
+// NP planes staged together: plane q is bitmap plane xp[q] of src[q] (all
+// zeros outside 0 .. dim-1) into dst[q], a zero word after each plane's
+// last; every load is issued before the first LDS write.
+template <int NP>
+__device__ __forceinline__ void stage_plane_group(uint64_t* const (&dst)[NP], const uint64_t* const (&src)[NP],
+                                                  int64_t nwords, int dim, const int (&xp)[NP]) {
     const int64_t pbits = (int64_t)dim * dim;
     const int pw = (int)((pbits + 63) >> 6);
     for (int k = threadIdx.x; k <= pw; k += blockDim.x) {
-        uint64_t v = 0ull;
-        if (xp >= 0 && xp < dim && k < pw) {
-            const int64_t bit = (int64_t)xp * pbits + 64ll * k;
+        uint64_t lo[NP], hi[NP];
+        int sh[NP];
+        bool ok[NP], has_hi[NP];
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            ok[q] = xp[q] >= 0 && xp[q] < dim && k < pw;
+            const int64_t bit = ok[q] ? (int64_t)xp[q] * pbits + 64ll * k : 0;
             const int64_t w = bit >> 6;
-            const int sh = (int)(bit & 63);
-            v = L[w];
-            if (sh) {
-                const uint64_t hi = (w + 1 < nwords) ? L[w + 1] : 0ull;
-                v = (v >> sh) | (hi << (64 - sh));
-            }
-            const int64_t rem = pbits - 64ll * k;
-            if (rem < 64) v &= (1ull << rem) - 1ull;
+            sh[q] = (int)(bit & 63);
+            has_hi[q] = w + 1 < nwords;
+            lo[q] = src[q][w];
+            hi[q] = src[q][has_hi[q] ? w + 1 : w];
         }
-        s_p[k] = v;
+        const int64_t rem = pbits - 64ll * k;  // bits of the plane from word k on
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            uint64_t v = lo[q];
+            if (sh[q]) v = (v >> sh[q]) | ((has_hi[q] ? hi[q] : 0ull) << (64 - sh[q]));
+            if (rem < 64) v &= rem > 0 ? (1ull << rem) - 1ull : 0ull;
+            dst[q][k] = ok[q] ? v : 0ull;
+        }
     }
 }
 
 __device__ __forceinline__ void stage_planes(uint64_t* p0, uint64_t* p1, uint64_t* p2, const uint64_t* __restrict__ L,
                                              int64_t nwords, int dim, int x0) {
-    stage_plane(p0, L, nwords, dim, x0);
-    stage_plane(p1, L, nwords, dim, x0 + 1);
-    stage_plane(p2, L, nwords, dim, x0 + 2);
+    uint64_t* const dst[3] = {p0, p1, p2};
+    const uint64_t* const src[3] = {L, L, L};
+    const int xp[3] = {x0, x0 + 1, x0 + 2};
+    stage_plane_group<3>(dst, src, nwords, dim, xp);
 }
 
 // A wave-uniform row moved to scalar registers: the bit operations on it then
