@@ -305,21 +305,23 @@ __device__ __forceinline__ RowBits<W> uniform_row(const RowBits<W>& a) {
 }
 
 // Row y of a staged plane.
+// Both words of every funnel read load unconditionally (the staged plane's
+// zero pad word keeps w + 1 in bounds) and an invalid row reads row 0, so
+// the reads carry no branches and issue together.
 template <int W>
 __device__ __forceinline__ RowBits<W> lds_row(const uint64_t* P, int dim, int y, bool valid) {
-    RowBits<W> r = zero_row<W>();
-    if (!valid) return r;
-    const int base = y * dim;
+    RowBits<W> r;
+    const int base = (valid ? y : 0) * dim;
 #pragma unroll
     for (int k = 0; k < W; k++) {
         const int bit = base + 64 * k;
         const int w = bit >> 6;
         const int sh = bit & 63;
-        uint64_t v = P[w];
-        if (sh) v = (v >> sh) | (P[w + 1] << (64 - sh));
+        const uint64_t lo = P[w], hi = P[w + 1];
+        uint64_t v = (lo >> sh) | ((hi << 1) << (63 - sh));  // the second term is 0 for sh == 0
         const int nb = dim - 64 * k;
-        if (nb < 64) v &= (1ull << nb) - 1ull;
-        r.w[k] = v;
+        if (nb < 64) v &= nb > 0 ? (1ull << nb) - 1ull : 0ull;
+        r.w[k] = valid ? v : 0ull;
     }
     return r;
 }
