@@ -200,7 +200,9 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     c->persistent_blocks = c->cu_count * 8;
     c->tune.sample_tile = kSampleTileDefault;
     c->tune.sample_blocks = c->persistent_blocks;
-    c->tune.row_blocks = c->persistent_blocks;
+    // two full rounds of the widest common plane kernel (k_emit<2>: five
+    // resident workgroups per CU, bound by its LDS)
+    c->tune.row_blocks = c->cu_count * 10;
     c->tune.smooth_blocks = 4 * c->persistent_blocks;
     c->tune.row_split = -1;
     Arenas& A = c->A;
@@ -800,7 +802,7 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
     {
         float bt = 1e30f;
         int bb = cur.row_blocks, bs = cur.row_split;
-        const int mult[5] = {1, 2, 4, 8, 16};
+        const int mult[7] = {1, 2, 4, 8, 10, 12, 16};
         const int splits[5] = {-1, 1, 2, 4, 8};  // -1: per row-word class from its plane count
         for (int sp : splits)
             for (int m : mult) {
