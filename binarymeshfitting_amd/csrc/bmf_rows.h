@@ -96,12 +96,19 @@ __device__ __forceinline__ int find_chunk(const int64_t* __restrict__ prefix, in
     return lo;
 }
 
-// q = n / d for n < 2^24 (float estimate + exact correction).
+// q = n / d for n <= 2^24, 0 < d < 2^24 and a quotient below 2^22, with
+// rcp = 1/d correctly rounded.  (float)n is exact and fl(n * rcp) =
+// (n/d)(1 + e) with |e| <= 2^-23 + 2^-48, so the estimate is within 0.5 of
+// n/d and its truncation is floor(n/d) - 1, floor(n/d) or floor(n/d) + 1:
+// one correction step each way, as selects, makes it exact.  (The sample
+// coordinates divide by dim and dim^2 with quotients below 256.)
 __device__ __forceinline__ uint32_t udiv24(uint32_t n, uint32_t d, float rcp) {
     int32_t q = (int32_t)((float)n * rcp);
     int32_t r = (int32_t)n - q * (int32_t)d;
-    while (r < 0) { q--; r += (int32_t)d; }
-    while (r >= (int32_t)d) { q++; r -= (int32_t)d; }
+    const bool lo = r < 0;
+    q = lo ? q - 1 : q;
+    r = lo ? r + (int32_t)d : r;
+    q = r >= (int32_t)d ? q + 1 : q;
     return (uint32_t)q;
 }
 
