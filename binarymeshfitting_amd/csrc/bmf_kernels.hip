@@ -957,22 +957,30 @@ __device__ __forceinline__ void emit_tris_row(const Arenas& A, const ChunkMeta& 
         if (nt) {
             const uint32_t tb = tcarry + mbcnt64(b0) + 2u * mbcnt64(b1) + 4u * mbcnt64(b2);
             const uint64_t row = s_tri[mask];
-            const int cnt = (int)(row >> 60);
-            if (tb + (uint32_t)(cnt / 3) > Tc) {
+            if (tb + nt > Tc) {
                 A.ctr->error = 3;
             } else {
+                // per triangle: the three corner ids read together, written
+                // together; an id past the chunk's vertices (counts and ids
+                // disagree) is reported, its slot is inside the chunk's range
                 uint32_t* __restrict__ I = A.idx + io + 3ll * tb;
-                for (int s = 0; s < cnt; s++) {
-                    const uint32_t g = (uint32_t)(kEdgeCorners >> (5 * ((row >> (4 * s)) & 15ull))) & 31u;
-                    const uint32_t e = s_id[g & 1u][y + (int)((g >> 1) & 1u) - ys][z + (int)((g >> 2) & 1u)];
-                    const uint32_t ax = g >> 3;
-                    const uint32_t v = (e & 0x3FFFFFFFu) + (ax >= 1 ? (e >> 30) & 1u : 0u) + (ax == 2 ? e >> 31 : 0u);
-                    if (v >= Vc) {
-                        A.ctr->error = 4;
-                        break;
+                bool bad = false;
+                for (int q = 0; q < (int)nt; q++) {
+                    uint32_t v[3];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        const int sl = 3 * q + c;
+                        const uint32_t g = (uint32_t)(kEdgeCorners >> (5 * ((row >> (4 * sl)) & 15ull))) & 31u;
+                        const uint32_t e = s_id[g & 1u][y + (int)((g >> 1) & 1u) - ys][z + (int)((g >> 2) & 1u)];
+                        const uint32_t ax = g >> 3;
+                        v[c] = (e & 0x3FFFFFFFu) + (ax >= 1 ? (e >> 30) & 1u : 0u) + (ax == 2 ? e >> 31 : 0u);
+                        bad = bad || v[c] >= Vc;
                     }
-                    I[s] = v;
+                    I[3 * q] = v[0];
+                    I[3 * q + 1] = v[1];
+                    I[3 * q + 2] = v[2];
                 }
+                if (bad) A.ctr->error = 4;
                 if (cells) {
                     const int64_t ci = M.dens_off + ((int64_t)x * dim + y) * dim + z;
                     A.cell_tb[ci] = tb;
