@@ -409,7 +409,7 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
         hipGraph_t graph = nullptr;
         BMF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, c->tune.sample_blocks, c->tune.row_blocks,
-                              std::min(n, c->cu_count), c->tune.smooth_blocks);
+                              n, c->tune.smooth_blocks);
         hipGraph_t captured = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &captured);
         if (rc) {
@@ -604,7 +604,7 @@ int bmf_profile_stages(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, float* s
     for (int i = 0; i < n; i++) wclasses |= row_word_class(chunks[i].dim);
     BMF_HIP(hipEventRecord(c->ev0, c->stream));
     rc = enqueue_pipeline(c, c->A, n, maxk, wclasses, 1, 1, c->tune.sample_blocks, c->tune.row_blocks,
-                          std::min(n, c->cu_count), c->tune.smooth_blocks, c->ev_stage);
+                          n, c->tune.smooth_blocks, c->ev_stage);
     if (rc) return rc;
     BMF_HIP(hipEventRecord(c->ev1, c->stream));
     BMF_HIP(hipStreamSynchronize(c->stream));
@@ -764,7 +764,7 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
         for (int i = 0; i < BMF_NUM_STAGES; i++) out[i] = 1e30f;
         for (int r = 0; r < reps; r++) {
             int e = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, t.sample_blocks, t.row_blocks,
-                                     std::min(n, c->cu_count), t.smooth_blocks, c->ev_stage);
+                                     n, t.smooth_blocks, c->ev_stage);
             if (e) return e;
             BMF_HIP(hipStreamSynchronize(c->stream));
             for (int i = 0; i < BMF_NUM_STAGES; i++) {
