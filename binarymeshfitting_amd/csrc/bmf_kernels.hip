@@ -253,34 +253,47 @@ __global__ void __launch_bounds__(256) k_csg_cull(Arenas A, int n) {
 // adaptation every
 // sample is evaluated fully (bit-identical values either way).
 // ------------------------------------------------------------------------
-__device__ __forceinline__ float density_from_sum(const SamplerDev& S, float sum, float t2) {
+// The terrain terms of the sampler and a chunk's sample geometry as values:
+// read once per tile, so the loops below never reload them after their
+// own stores.
+struct TerrainTerms {
+    float amp, y0, inv_norm;
+};
+struct ChunkGeom {
+    float ox, oy, oz, delta;
+};
+
+__device__ __forceinline__ float density_from_sum(const TerrainTerms& S, float sum, float t2) {
     float fb = sum * S.inv_norm;
     fb = fminf(fmaxf(fb, -1.0f), 1.0f);
     return S.amp * fb - t2;
+}
+__device__ __forceinline__ float density_from_sum(const SamplerDev& S, float sum, float t2) {
+    return density_from_sum(TerrainTerms{S.amp, S.y0, S.inv_norm}, sum, t2);
 }
 
 // 0: undecided, 1: certainly air (d < 0), 2: certainly solid.  The final
 // FP32 sum lies in [sum - reach[k], sum + reach[k]] (see SamplerDev::reach:
 // the interval the remaining fma steps can reach, widened by their rounding
 // and that of the two operations here, so both ends are safe bounds).
-__device__ __forceinline__ int decide_label(const SamplerDev& S, int k, float sum, float t2) {
-    const float lo = sum - S.reach[k], hi = sum + S.reach[k];
+__device__ __forceinline__ int decide_label(const TerrainTerms& S, float reach, float sum, float t2) {
+    const float lo = sum - reach, hi = sum + reach;
     const float d0 = density_from_sum(S, lo, t2), d1 = density_from_sum(S, hi, t2);
     if (fmaxf(d0, d1) < 0.0f) return 1;
     if (!(fminf(d0, d1) < 0.0f)) return 2;
     return 0;
 }
 
-__device__ __forceinline__ void sample_coords(const ChunkMeta& M, uint32_t i, uint32_t dim, uint32_t dim2, float rd,
+__device__ __forceinline__ void sample_coords(const ChunkGeom& G, uint32_t i, uint32_t dim, uint32_t dim2, float rd,
                                               float rd2, float& px, float& py, float& pz) {
     const uint32_t x = udiv24(i, dim2, rd2);
     const uint32_t rem = i - x * dim2;
     const uint32_t y = udiv24(rem, dim, rd);
     const uint32_t z = rem - y * dim;
     // NoiseSampler.cpp:21-27: (float)i * scale + p, two roundings
-    px = (float)x * M.delta + M.opos[0];
-    py = (float)y * M.delta + M.opos[1];
-    pz = (float)z * M.delta + M.opos[2];
+    px = (float)x * G.delta + G.ox;
+    py = (float)y * G.delta + G.oy;
+    pz = (float)z * G.delta + G.oz;
 }
 
 // Sampling modes, fixed per launch by the host: adaptive octaves for the
@@ -304,6 +317,8 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
     __shared__ unsigned long long s_exact[kTile / 64];
     __shared__ uint32_t s_flags;
     const SamplerDev* __restrict__ S = A.sampler;
+    const TerrainTerms TT{S->amp, S->y0, S->inv_norm};
+    const float reach0 = S->reach[0];
     const int64_t total = A.ctr->total_tiles;
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
@@ -326,6 +341,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
         if (tile >= total) break;
         const int j = find_chunk(A.tile_prefix, n, tile);
         const ChunkMeta& M = A.meta[j];
+        const ChunkGeom G{M.opos[0], M.opos[1], M.opos[2], M.delta};
         const uint32_t dim = (uint32_t)M.dim, dim2 = dim * dim, ns = dim2 * dim;
         const float rd = 1.0f / (float)dim, rd2 = 1.0f / (float)dim2;
         float* __restrict__ dens = A.dens + M.dens_off;
@@ -347,10 +363,10 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
             bool air = false, done = true;
             if (valid) {
                 float px, py, pz;
-                sample_coords(M, i, dim, dim2, rd, rd2, px, py, pz);
+                sample_coords(G, i, dim, dim2, rd, rd2, px, py, pz);
                 if constexpr (adapt) {
-                    const float t2 = py - S->y0;
-                    const int dec = decide_label(*S, 0, 0.0f, t2);
+                    const float t2 = py - TT.y0;
+                    const int dec = decide_label(TT, reach0, 0.0f, t2);
                     done = dec != 0;
                     air = dec == 1;
                 } else {
@@ -393,7 +409,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                 const int cur = k & 1, nxt = cur ^ 1;
                 const uint32_t nq = s_qn[cur];
                 if (nq == 0) break;
-                const float f = S->fo[k], a = S->ao[k];
+                const float f = S->fo[k], a = S->ao[k], reach1 = S->reach[k + 1];
                 const uint32_t seed = S->seed + (uint32_t)k;
                 for (uint32_t e0 = 0; e0 < nq; e0 += kSampleThreads) {
                     const uint32_t e = e0 + threadIdx.x;
@@ -405,17 +421,17 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                         li = s_qi[cur][e];
                         sum = s_qs[cur][e];
                         float px, py, pz;
-                        sample_coords(M, first + li, dim, dim2, rd, rd2, px, py, pz);
-                        t2 = py - S->y0;
+                        sample_coords(G, first + li, dim, dim2, rd, rd2, px, py, pz);
+                        t2 = py - TT.y0;
                         const float nz = snoise3(px * f, py * f, pz * f, seed);
                         sum = __builtin_fmaf(a, nz, sum);
                         if (k == O - 1) {
-                            const float d = density_from_sum(*S, sum, t2);
+                            const float d = density_from_sum(TT, sum, t2);
                             dens[first + li] = d;
                             dec = d < 0.0f ? 1 : 2;
                             atomicOr(&s_exact[li >> 6], 1ull << (li & 63));
                         } else {
-                            dec = decide_label(*S, k + 1, sum, t2);
+                            dec = decide_label(TT, reach1, sum, t2);
                         }
                         if (dec == 1) atomicOr(&s_lab[li >> 6], 1ull << (li & 63));
                     }
