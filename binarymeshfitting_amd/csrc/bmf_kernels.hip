@@ -301,6 +301,11 @@ __device__ __forceinline__ void sample_coords(const ChunkGeom& G, uint32_t i, ui
 // LDS once per tile; every other case (external densities, implicit shapes,
 // the terrain in full) evaluated directly.
 constexpr int kSampleGeneric = 0, kSampleAdaptive = 1, kSampleCsg = 2;
+// Adaptive mode: undecided samples of a wave (of 64) from which phase 0
+// evaluates octave 0 in place rather than queueing them: queueing costs a
+// push, a pop and the sample coordinates again, in-place evaluation the
+// idle lanes of one octave.
+constexpr int kInPlaceOctave = 52;
 
 template <int ROUNDS, int MODE>
 __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
@@ -318,7 +323,10 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
     __shared__ uint32_t s_flags;
     const SamplerDev* __restrict__ S = A.sampler;
     const TerrainTerms TT{S->amp, S->y0, S->inv_norm};
-    const float reach0 = S->reach[0];
+    const float reach0 = S->reach[0], reach1_0 = S->reach[1];
+    const int O = S->octaves;  // adaptive mode: octave 0's terms for phase 0
+    const float f0 = S->fo[0], a0 = S->ao[0];
+    const uint32_t seed0 = S->seed;
     const int64_t total = A.ctr->total_tiles;
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
@@ -360,43 +368,66 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
             const uint32_t li = r * kSampleThreads + threadIdx.x;  // waves cover 64 consecutive samples
             const uint32_t i = first + li;
             const bool valid = i < ns;
-            bool air = false, done = true;
-            if (valid) {
+            bool air = false, done = true, fin = false;
+            int qw = 0;        // adaptive: the queue the wave's undecided samples join
+            float sum0 = 0.0f;  // and their partial sum
+            if constexpr (adapt) {
+                float px = 0.0f, py = 0.0f, pz = 0.0f, t2 = 0.0f;
+                int dec = 2;
+                if (valid) {
+                    sample_coords(G, i, dim, dim2, rd, rd2, px, py, pz);
+                    t2 = py - TT.y0;
+                    dec = decide_label(TT, reach0, 0.0f, t2);
+                }
+                // a wave whose samples are nearly all undecided evaluates octave
+                // 0 here, as the octave rounds would (same operations), and its
+                // survivors join the octave-1 queue; a sparse wave queues them
+                // for octave 0 instead, so that the rounds run on dense waves
+                if (__popcll(__ballot(valid && dec == 0)) >= kInPlaceOctave) {
+                    qw = 1;
+                    if (valid && dec == 0) {
+                        sum0 = __builtin_fmaf(a0, snoise3(px * f0, py * f0, pz * f0, seed0), 0.0f);
+                        if (O == 1) {
+                            const float d = density_from_sum(TT, sum0, t2);
+                            dens[i] = d;
+                            dec = d < 0.0f ? 1 : 2;
+                            fin = true;
+                        } else {
+                            dec = decide_label(TT, reach1_0, sum0, t2);
+                        }
+                    }
+                }
+                done = dec != 0;
+                air = dec == 1;
+            } else if (valid) {
                 float px, py, pz;
                 sample_coords(G, i, dim, dim2, rd, rd2, px, py, pz);
-                if constexpr (adapt) {
-                    const float t2 = py - TT.y0;
-                    const int dec = decide_label(TT, reach0, 0.0f, t2);
-                    done = dec != 0;
-                    air = dec == 1;
-                } else {
-                    float d;
-                    if (staged) d = csg_density_lds(s_csg, M.csg_ns, M.csg_nb, px, py, pz);
-                    else if (S->kind == BMF_SAMPLER_EXTERNAL) d = S->ext[A.ext_base + M.dens_off + i];
-                    else if (S->kind == BMF_SAMPLER_CSG && A.csg_list)
-                        d = csg_density_list(*S, A.csg_list + (int64_t)j * A.csg_stride, M.csg_ns, M.csg_nb, px, py,
-                                             pz);
-                    else d = sample_density(*S, px, py, pz);
-                    dens[i] = d;
-                    air = d < 0.0f;
-                }
+                float d;
+                if (staged) d = csg_density_lds(s_csg, M.csg_ns, M.csg_nb, px, py, pz);
+                else if (S->kind == BMF_SAMPLER_EXTERNAL) d = S->ext[A.ext_base + M.dens_off + i];
+                else if (S->kind == BMF_SAMPLER_CSG && A.csg_list)
+                    d = csg_density_list(*S, A.csg_list + (int64_t)j * A.csg_stride, M.csg_ns, M.csg_nb, px, py, pz);
+                else d = sample_density(*S, px, py, pz);
+                dens[i] = d;
+                air = d < 0.0f;
             }
             const uint64_t bl = __ballot(valid && done && air);
             const uint64_t push = __ballot(valid && !done);
             const uint64_t vb = __ballot(valid);
+            const uint64_t fb = __ballot(fin);
             if (lane == 0) {
                 s_lab[li >> 6] = bl;
-                s_exact[li >> 6] = adapt ? 0ull : vb;
+                s_exact[li >> 6] = adapt ? fb : vb;
             }
             if constexpr (adapt) {
                 if (push) {
                     uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&s_qn[0], (uint32_t)__popcll(push));
+                    if (lane == 0) base = atomicAdd(&s_qn[qw], (uint32_t)__popcll(push));
                     base = __shfl(base, 0);
                     if (valid && !done) {
                         const uint32_t q = base + (uint32_t)__popcll(push & lt);
-                        s_qi[0][q] = (uint16_t)li;
-                        s_qs[0][q] = 0.0f;
+                        s_qi[qw][q] = (uint16_t)li;
+                        s_qs[qw][q] = sum0;
                     }
                 }
             }
@@ -404,11 +435,10 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
         __syncthreads();
         // ---- octave rounds on the compacted queue ----
         if constexpr (adapt) {
-            const int O = S->octaves;
             for (int k = 0; k < O; k++) {
                 const int cur = k & 1, nxt = cur ^ 1;
                 const uint32_t nq = s_qn[cur];
-                if (nq == 0) break;
+                if (nq == 0 && s_qn[nxt] == 0) break;  // (octave 1 may be queued by phase 0)
                 const float f = S->fo[k], a = S->ao[k], reach1 = S->reach[k + 1];
                 const uint32_t seed = S->seed + (uint32_t)k;
                 for (uint32_t e0 = 0; e0 < nq; e0 += kSampleThreads) {
