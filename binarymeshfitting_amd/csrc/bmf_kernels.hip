@@ -306,6 +306,12 @@ constexpr int kSampleGeneric = 0, kSampleAdaptive = 1, kSampleCsg = 2;
 // push, a pop and the sample coordinates again, in-place evaluation the
 // idle lanes of one octave.
 constexpr int kInPlaceOctave = 52;
+// Octave queue entries: sample index within the tile (kTile <= 4096) in the
+// low 12 bits, the sample's next octave (< BMF_MAX_OCTAVES = 16) above.
+constexpr int kQueueOctaveShift = 12;
+constexpr uint32_t kQueueSampleMask = (1u << kQueueOctaveShift) - 1u;
+static_assert(16 * kSampleThreads <= (1 << kQueueOctaveShift), "tile index fits the queue entry");
+static_assert(BMF_MAX_OCTAVES <= 16, "octave index fits the queue entry");
 
 template <int ROUNDS, int MODE>
 __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
@@ -323,10 +329,19 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
     __shared__ uint32_t s_flags;
     const SamplerDev* __restrict__ S = A.sampler;
     const TerrainTerms TT{S->amp, S->y0, S->inv_norm};
-    const float reach0 = S->reach[0], reach1_0 = S->reach[1];
-    const int O = S->octaves;  // adaptive mode: octave 0's terms for phase 0
-    const float f0 = S->fo[0], a0 = S->ao[0];
-    const uint32_t seed0 = S->seed;
+    const float reach0 = S->reach[0];
+    const int O = S->octaves;
+    const uint32_t seed0 = S->seed;  // octave k uses seed0 + k
+    // adaptive mode: per-octave frequency, amplitude and reach (visible to
+    // every thread after the first barrier of the tile loop)
+    __shared__ float s_oct[3][BMF_MAX_OCTAVES + 1];
+    if constexpr (adapt) {
+        for (int t = threadIdx.x; t <= O; t += blockDim.x) {
+            s_oct[0][t] = t < O ? S->fo[t] : 0.0f;
+            s_oct[1][t] = t < O ? S->ao[t] : 0.0f;
+            s_oct[2][t] = S->reach[t];
+        }
+    }
     const int64_t total = A.ctr->total_tiles;
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
@@ -369,7 +384,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
             const uint32_t i = first + li;
             const bool valid = i < ns;
             bool air = false, done = true, fin = false;
-            int qw = 0;        // adaptive: the queue the wave's undecided samples join
+            int kq = 0;         // adaptive: the next octave of the wave's undecided samples
             float sum0 = 0.0f;  // and their partial sum
             if constexpr (adapt) {
                 float px = 0.0f, py = 0.0f, pz = 0.0f, t2 = 0.0f;
@@ -379,23 +394,24 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                     t2 = py - TT.y0;
                     dec = decide_label(TT, reach0, 0.0f, t2);
                 }
-                // a wave whose samples are nearly all undecided evaluates octave
-                // 0 here, as the octave rounds would (same operations), and its
-                // survivors join the octave-1 queue; a sparse wave queues them
-                // for octave 0 instead, so that the rounds run on dense waves
-                if (__popcll(__ballot(valid && dec == 0)) >= kInPlaceOctave) {
-                    qw = 1;
+                // while nearly all of the wave's samples are undecided it
+                // evaluates their octaves here, as the rounds would (same
+                // operations); the survivors of a sparse wave are queued with
+                // their next octave, so that the rounds run on dense waves
+                while (kq < O && __popcll(__ballot(valid && dec == 0)) >= kInPlaceOctave) {
                     if (valid && dec == 0) {
-                        sum0 = __builtin_fmaf(a0, snoise3(px * f0, py * f0, pz * f0, seed0), 0.0f);
-                        if (O == 1) {
+                        const float f = s_oct[0][kq];
+                        sum0 = __builtin_fmaf(s_oct[1][kq], snoise3(px * f, py * f, pz * f, seed0 + (uint32_t)kq), sum0);
+                        if (kq == O - 1) {
                             const float d = density_from_sum(TT, sum0, t2);
                             dens[i] = d;
                             dec = d < 0.0f ? 1 : 2;
                             fin = true;
                         } else {
-                            dec = decide_label(TT, reach1_0, sum0, t2);
+                            dec = decide_label(TT, s_oct[2][kq + 1], sum0, t2);
                         }
                     }
+                    kq++;
                 }
                 done = dec != 0;
                 air = dec == 1;
@@ -422,46 +438,48 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
             if constexpr (adapt) {
                 if (push) {
                     uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&s_qn[qw], (uint32_t)__popcll(push));
+                    if (lane == 0) base = atomicAdd(&s_qn[0], (uint32_t)__popcll(push));
                     base = __shfl(base, 0);
                     if (valid && !done) {
                         const uint32_t q = base + (uint32_t)__popcll(push & lt);
-                        s_qi[qw][q] = (uint16_t)li;
-                        s_qs[qw][q] = sum0;
+                        s_qi[0][q] = (uint16_t)(li | ((uint32_t)kq << kQueueOctaveShift));
+                        s_qs[0][q] = sum0;
                     }
                 }
             }
         }
         __syncthreads();
-        // ---- octave rounds on the compacted queue ----
+        // ---- rounds on the compacted queue: each entry (sample, next octave,
+        // partial sum) advances one octave per round, so O rounds finish all ----
         if constexpr (adapt) {
-            for (int k = 0; k < O; k++) {
-                const int cur = k & 1, nxt = cur ^ 1;
+            for (int rnd = 0; rnd < O; rnd++) {
+                const int cur = rnd & 1, nxt = cur ^ 1;
                 const uint32_t nq = s_qn[cur];
-                if (nq == 0 && s_qn[nxt] == 0) break;  // (octave 1 may be queued by phase 0)
-                const float f = S->fo[k], a = S->ao[k], reach1 = S->reach[k + 1];
-                const uint32_t seed = S->seed + (uint32_t)k;
+                if (nq == 0) break;
                 for (uint32_t e0 = 0; e0 < nq; e0 += kSampleThreads) {
                     const uint32_t e = e0 + threadIdx.x;
                     const bool active = e < nq;
-                    uint32_t li = 0;
-                    float sum = 0.0f, t2 = 0.0f;
+                    uint32_t li = 0, k = 0;
+                    float sum = 0.0f;
                     int dec = 2;
                     if (active) {
-                        li = s_qi[cur][e];
+                        const uint32_t qv = s_qi[cur][e];
+                        li = qv & kQueueSampleMask;
+                        k = qv >> kQueueOctaveShift;
                         sum = s_qs[cur][e];
                         float px, py, pz;
                         sample_coords(G, first + li, dim, dim2, rd, rd2, px, py, pz);
-                        t2 = py - TT.y0;
-                        const float nz = snoise3(px * f, py * f, pz * f, seed);
-                        sum = __builtin_fmaf(a, nz, sum);
-                        if (k == O - 1) {
+                        const float t2 = py - TT.y0;
+                        const float f = s_oct[0][k];
+                        const float nz = snoise3(px * f, py * f, pz * f, seed0 + k);
+                        sum = __builtin_fmaf(s_oct[1][k], nz, sum);
+                        if ((int)k == O - 1) {
                             const float d = density_from_sum(TT, sum, t2);
                             dens[first + li] = d;
                             dec = d < 0.0f ? 1 : 2;
                             atomicOr(&s_exact[li >> 6], 1ull << (li & 63));
                         } else {
-                            dec = decide_label(TT, reach1, sum, t2);
+                            dec = decide_label(TT, s_oct[2][k + 1], sum, t2);
                         }
                         if (dec == 1) atomicOr(&s_lab[li >> 6], 1ull << (li & 63));
                     }
@@ -472,7 +490,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                         base = __shfl(base, 0);
                         if (active && dec == 0) {
                             const uint32_t q = base + (uint32_t)__popcll(push & lt);
-                            s_qi[nxt][q] = (uint16_t)li;
+                            s_qi[nxt][q] = (uint16_t)(li | ((k + 1) << kQueueOctaveShift));
                             s_qs[nxt][q] = sum;
                         }
                     }
