@@ -429,27 +429,18 @@ __device__ __forceinline__ float wave_min(float x) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
+// A wave-uniform 64-bit value held in every lane, moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Lowest lane of the (non-empty) set m whose value x is smallest.
 __device__ __forceinline__ int wave_argmin(float x, uint64_t m, int lane) {
     const float y = (m >> lane) & 1 ? x : INFINITY;
     const float mn = wave_min(y);
     return __ffsll((unsigned long long)(__ballot(y == mn) & m)) - 1;
-}
-
-// Position of the k-th (from 0) set bit of m, or -1 when m has k bits or fewer.
-__device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
-    if (k >= __popcll(m)) return -1;
-    int pos = 0;
-#pragma unroll
-    for (int w = 32; w >= 1; w >>= 1) {
-        const int c = __popcll(m & ((1ull << w) - 1));
-        if (k >= c) {
-            k -= c;
-            m >>= w;
-            pos += w;
-        }
-    }
-    return pos;
 }
 
 // Nearest target of each vertex, one wave per vertex.  The hierarchy (heap
@@ -508,8 +499,9 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
         const int k = depth - 1;
         const int lk = k == 0 ? 0 : f0 + kNNFan * (k - 1);
         const float lb = s_lb[w][k][lane];
-        const int64_t b0 = s_base[w][k];
-        unsigned long long m = s_mk[w][k] & __ballot(lb <= U);
+        // the frame's base and live set are the wave's (moved to scalar registers)
+        const int64_t b0 = uniform_u64((uint64_t)s_base[w][k]);
+        unsigned long long m = uniform_u64(s_mk[w][k]) & __ballot(lb <= U);
         if (lk == D) {
             if (U == INFINITY && m) {
                 const int j = wave_argmin(lb, m, lane);
@@ -519,10 +511,17 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
                 m &= __ballot(lb <= U);
             }
             while (m) {
-                const int j = kth_set_bit(m, lane / kLeaf);
-                if (j >= 0) eval(b0 + j);
+                // lane group g (kLeaf lanes) takes the g-th lowest live leaf:
+                // the positions come off the wave-uniform mask on the scalar
+                // unit, each group selects its own
+                int j = -1;
 #pragma unroll
-                for (int t = 0; t < 64 / kLeaf; t++) m &= m - 1;
+                for (int t = 0; t < 64 / kLeaf; t++) {
+                    const int pos = m ? __ffsll((unsigned long long)m) - 1 : -1;
+                    j = lane / kLeaf == t ? pos : j;
+                    m &= m - 1;
+                }
+                if (j >= 0) eval(b0 + j);
                 U = wave_min(bd);
                 m &= __ballot(lb <= U);
             }
