@@ -470,21 +470,22 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
     const int f0 = D == 0 ? 0 : D - kNNFan * ((D - 1) / kNNFan);
     float bd = INFINITY;  // this lane's best (distance, original index, sorted position)
     int bi = 0x7FFFFFFF;
-    int64_t bs = -1;
+    int bs = -1;  // sorted position (n < 2^31, heap indices < 2^30)
     float U = INFINITY;  // best distance over the wave
     // the frames of this wave: per-lane box distances, the live set, heap index of descendant 0
     __shared__ float s_lb[kNNThreads / 64][kNNFrames][64];
     __shared__ unsigned long long s_mk[kNNThreads / 64][kNNFrames];
-    __shared__ int64_t s_base[kNNThreads / 64][kNNFrames];
+    __shared__ int s_base[kNNThreads / 64][kNNFrames];
     const int w = (int)(threadIdx.x >> 6);
     s_lb[w][0][lane] = lane == 0 ? box_lb2(px, py, pz, lo[1], hi[1]) : INFINITY;
     s_mk[w][0] = 1;
     s_base[w][0] = 1;
     int depth = 1;
-    auto eval = [&](int64_t leaf) {
+    const int Pi = (int)P, ni = (int)n;
+    auto eval = [&](int leaf) {
         // this lane's target of `leaf` (heap index), if any
-        const int64_t i = (leaf - P) * kLeaf + (lane & (kLeaf - 1));
-        if (i < n) {
+        const int i = (leaf - Pi) * kLeaf + (lane & (kLeaf - 1));
+        if (i < ni) {
             const float4 q = st[i];
             const float d = dist2(px, py, pz, q);
             const int id = __float_as_int(q.w);
@@ -500,7 +501,7 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
         const int lk = k == 0 ? 0 : f0 + kNNFan * (k - 1);
         const float lb = s_lb[w][k][lane];
         // the frame's base and live set are the wave's (moved to scalar registers)
-        const int64_t b0 = uniform_u64((uint64_t)s_base[w][k]);
+        const int b0 = __builtin_amdgcn_readfirstlane(s_base[w][k]);
         unsigned long long m = uniform_u64(s_mk[w][k]) & __ballot(lb <= U);
         if (lk == D) {
             if (U == INFINITY && m) {
@@ -535,7 +536,7 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
         const int j = wave_argmin(lb, m, lane);
         s_mk[w][k] = m & ~(1ull << j);
         const int f = k == 0 ? f0 : kNNFan;
-        const int64_t c0 = (b0 + j) << f;
+        const int c0 = (b0 + j) << f;
         const float lc = lane < (1 << f) ? box_lb2(px, py, pz, lo[c0 + lane], hi[c0 + lane]) : INFINITY;
         s_lb[w][k + 1][lane] = lc;
         s_mk[w][k + 1] = __ballot(lane < (1 << f) && lc <= U);
@@ -547,7 +548,7 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
     for (int off = 32; off >= 1; off >>= 1) {
         const float od = __shfl_xor(bd, off);
         const int oi = __shfl_xor(bi, off);
-        const int64_t os = __shfl_xor(bs, off);
+        const int os = __shfl_xor(bs, off);
         if (od < bd || (od == bd && oi < bi)) {
             bd = od;
             bi = oi;
