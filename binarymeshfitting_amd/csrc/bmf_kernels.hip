@@ -699,8 +699,7 @@ __global__ void __launch_bounds__(kRowThreads) k_count(Arenas A, int n) {
 #pragma unroll
             for (int k = 0; k < W; k++) {
                 const int bit = y * dim + 64 * k, w = bit >> 6, sh = bit & 63;
-                uint64_t v = P[w];
-                if (sh) v = (v >> sh) | (P[w + 1] << (64 - sh));
+                uint64_t v = (P[w] >> sh) | ((P[w + 1] << 1) << (63 - sh));  // the pad word keeps w + 1 in bounds
                 const int nb = dim - 64 * k;
                 const uint64_t m = nb >= 64 ? ~0ull : ((1ull << nb) - 1ull);
                 v &= m;
