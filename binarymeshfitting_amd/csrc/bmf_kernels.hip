@@ -1203,6 +1203,7 @@ __global__ void __launch_bounds__(kSmoothThreads) k_dual(Arenas A, int m) {
 // 5 * 255^3 < 2^27 triangles.  The first smoothing step packs it (four
 // independent cell reads), later steps load it as one 16-byte word.
 constexpr uint32_t kRingFirst = (1u << 27) - 1u;
+constexpr int kRingRound = 8;  // centroids loaded per round (a ring has about six triangles)
 
 // Ring cell c around an edge along `ax` is the edge origin offset by
 // (c >> 1) - 1 and (c & 1) - 1 along the other two axes in x, y, z order,
@@ -1242,7 +1243,7 @@ __device__ __forceinline__ uint4 pack_ring(const uint32_t* __restrict__ CT, cons
 
 // Sums of the ring's centroids (and face normals) from +0 in ascending
 // triangle id, the order of MeshProcessor<3>'s adjacency lists; returns the
-// ring size.  Four centroids are loaded per round, a lane's unused slots
+// ring size.  kRingRound centroids are loaded per round, a lane's unused slots
 // reading its round's first triangle and adding nothing.
 template <bool NRM>
 __device__ __forceinline__ uint32_t ring_sums(uint4 r, const float4* __restrict__ C, const float4* __restrict__ F,
@@ -1253,24 +1254,24 @@ __device__ __forceinline__ uint32_t ring_sums(uint4 r, const float4* __restrict_
     const uint32_t f0 = r.x & kRingFirst, f1 = (r.y & kRingFirst) - 5u, f2 = (r.z & kRingFirst) - 10u,
                    f3 = (r.w & kRingFirst) - 15u;
     while (bits) {
-        uint32_t t[4];
-        bool ok[4];
+        uint32_t t[kRingRound];
+        bool ok[kRingRound];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < kRingRound; k++) {
             ok[k] = bits != 0u;
             const uint32_t b = ok[k] ? (uint32_t)(__ffs(bits) - 1) : 0u;
             const uint32_t f = b < 5u ? f0 : (b < 10u ? f1 : (b < 15u ? f2 : f3));
             t[k] = ok[k] ? f + b : (k == 0 ? 0u : t[0]);  // slot 0 is always used
             bits &= bits - 1u;
         }
-        float4 q[4], n[4];
+        float4 q[kRingRound], n[kRingRound];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < kRingRound; k++) {
             q[k] = C[t[k]];
             if (NRM) n[k] = F[t[k]];
         }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < kRingRound; k++) {
             s[0] = ok[k] ? s[0] + q[k].x : s[0];
             s[1] = ok[k] ? s[1] + q[k].y : s[1];
             s[2] = ok[k] ? s[2] + q[k].z : s[2];
