@@ -284,16 +284,27 @@ __device__ __forceinline__ int decide_label(const TerrainTerms& S, float reach, 
     return 0;
 }
 
-__device__ __forceinline__ void sample_coords(const ChunkGeom& G, uint32_t i, uint32_t dim, uint32_t dim2, float rd,
-                                              float rd2, float& px, float& py, float& pz) {
-    const uint32_t x = udiv24(i, dim2, rd2);
+__device__ __forceinline__ void sample_index(uint32_t i, uint32_t dim, uint32_t dim2, float rd, float rd2, uint32_t& x,
+                                             uint32_t& y, uint32_t& z) {
+    x = udiv24(i, dim2, rd2);
     const uint32_t rem = i - x * dim2;
-    const uint32_t y = udiv24(rem, dim, rd);
-    const uint32_t z = rem - y * dim;
-    // NoiseSampler.cpp:21-27: (float)i * scale + p, two roundings
+    y = udiv24(rem, dim, rd);
+    z = rem - y * dim;
+}
+
+// NoiseSampler.cpp:21-27: (float)i * scale + p, two roundings
+__device__ __forceinline__ void sample_position(const ChunkGeom& G, uint32_t x, uint32_t y, uint32_t z, float& px,
+                                                float& py, float& pz) {
     px = (float)x * G.delta + G.ox;
     py = (float)y * G.delta + G.oy;
     pz = (float)z * G.delta + G.oz;
+}
+
+__device__ __forceinline__ void sample_coords(const ChunkGeom& G, uint32_t i, uint32_t dim, uint32_t dim2, float rd,
+                                              float rd2, float& px, float& py, float& pz) {
+    uint32_t x, y, z;
+    sample_index(i, dim, dim2, rd, rd2, x, y, z);
+    sample_position(G, x, y, z, px, py, pz);
 }
 
 // Sampling modes, fixed per launch by the host: adaptive octaves for the
@@ -378,11 +389,34 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
             }
         }
         // ---- phase 0: every sample of the tile ----
+        // A thread's sample advances by kSampleThreads per round: its (x, y, z)
+        // advances by (0, q, r) with q, r = kSampleThreads / dim, % dim and at
+        // most one carry per axis while q + 1 < dim; smaller chunks divide.
+        const uint32_t step_y = kSampleThreads / dim, step_z = kSampleThreads - step_y * dim;
+        const bool stepping = step_y + 1 < dim;
+        uint32_t cx, cy, cz;
+        sample_index(first + threadIdx.x, dim, dim2, rd, rd2, cx, cy, cz);
 #pragma unroll 1
         for (int r = 0; r < ROUNDS; r++) {
             const uint32_t li = r * kSampleThreads + threadIdx.x;  // waves cover 64 consecutive samples
             const uint32_t i = first + li;
             const bool valid = i < ns;
+            if (r > 0) {
+                if (stepping) {
+                    cz += step_z;
+                    cy += step_y;
+                    if (cz >= dim) {
+                        cz -= dim;
+                        cy++;
+                    }
+                    if (cy >= dim) {
+                        cy -= dim;
+                        cx++;
+                    }
+                } else {
+                    sample_index(i, dim, dim2, rd, rd2, cx, cy, cz);
+                }
+            }
             bool air = false, done = true, fin = false;
             int kq = 0;         // adaptive: the next octave of the wave's undecided samples
             float sum0 = 0.0f;  // and their partial sum
@@ -390,7 +424,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                 float px = 0.0f, py = 0.0f, pz = 0.0f, t2 = 0.0f;
                 int dec = 2;
                 if (valid) {
-                    sample_coords(G, i, dim, dim2, rd, rd2, px, py, pz);
+                    sample_position(G, cx, cy, cz, px, py, pz);
                     t2 = py - TT.y0;
                     dec = decide_label(TT, reach0, 0.0f, t2);
                 }
@@ -417,7 +451,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                 air = dec == 1;
             } else if (valid) {
                 float px, py, pz;
-                sample_coords(G, i, dim, dim2, rd, rd2, px, py, pz);
+                sample_position(G, cx, cy, cz, px, py, pz);
                 float d;
                 if (staged) d = csg_density_lds(s_csg, M.csg_ns, M.csg_nb, px, py, pz);
                 else if (S->kind == BMF_SAMPLER_EXTERNAL) d = S->ext[A.ext_base + M.dens_off + i];
