@@ -111,7 +111,6 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
         else BMF_HIP(launch_smooth(A, m, smooth_blocks, s));
     }
     BMF_MARK(6);
-    BMF_HIP(launch_finalize(A, n, s));
     BMF_MARK(7);
 #undef BMF_MARK
     return BMF_OK;

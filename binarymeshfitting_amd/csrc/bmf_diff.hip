@@ -791,7 +791,6 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
     if (e == hipSuccess) e = launch_refine(A, 1, dim * kMaxPlaneSplit, s);
     if (e == hipSuccess) e = launch_scan(A, 1, 1, s);
     if (e == hipSuccess) e = launch_emit(A, 1, dim * kMaxPlaneSplit, wcls, s);
-    if (e == hipSuccess) e = launch_finalize(A, 1, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->h_results, A.results, sizeof(bmf_chunk_result), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -172,7 +172,6 @@ hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 // cs[W-1]: the stream of row-word class W
 hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs);
 hipError_t launch_dmc_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
-hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s);
 hipError_t launch_ref_labels(const Arenas& A, uint32_t* words, hipStream_t s);
 hipError_t launch_ref_masks(const Arenas& A, uint64_t* masks, hipStream_t s);
 

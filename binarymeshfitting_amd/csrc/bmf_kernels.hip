@@ -29,6 +29,27 @@ namespace bmf {
 // ------------------------------------------------------------------------
 constexpr int kPlanThreads = 1024;
 
+// The host-visible result of chunk j from its final record: written by
+// k_plan for the chunks it culls or skips and by k_scan for the sampled ones
+// (every field is final there: later kernels report errors through the
+// counters, not the chunk records).
+__device__ __forceinline__ void write_result(const Arenas& A, int j, const ChunkMeta& M) {
+    bmf_chunk_result r;
+    const bool ok = M.status == BMF_OK;
+    r.vertex_offset = ok ? M.v_off : 0;
+    r.index_offset = ok ? M.i_off : 0;
+    r.vertex_count = ok ? M.V : 0;
+    r.index_count = ok ? M.ipp * M.T : 0;
+    r.contains_mesh = M.contains_mesh;
+    r.status = M.status;
+    r.sampled = M.state == kStateSampled;
+    r.delta = M.delta;
+    r.overlap_pos[0] = M.opos[0];
+    r.overlap_pos[1] = M.opos[1];
+    r.overlap_pos[2] = M.opos[2];
+    A.results[j] = r;
+}
+
 __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allow_cull, int reset_bumps) {
     __shared__ int64_t s_w[8][kPlanThreads / 64 + 1];
     const int tid = threadIdx.x;
@@ -135,6 +156,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
             // culled chunks: DMCChunk.cpp:159-162 on uniform labels
             m.contains_mesh = (m.state == kStateAir) && (m.dim % 32 != 0);
         }
+        if (m.state != kStateSampled) write_result(A, j, m);  // final: k_scan skips it
     }
     if (tid == 0) {
         A.tile_prefix[n] = overflow ? 0 : tot_tiles;
@@ -847,6 +869,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(Arenas A, int n) {
                 M.i_off = io;
             }
             (void)ok;
+            write_result(A, j, M);
         }
         __syncthreads();
     }
@@ -1495,33 +1518,6 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_nrm(Arenas A, int m) 
     }
 }
 
-// ------------------------------------------------------------------------
-// finalize: per-chunk results for the host
-// ------------------------------------------------------------------------
-__global__ void k_finalize(Arenas A, int n) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const ChunkMeta& M = A.meta[j];
-    bmf_chunk_result r;
-    const bool ok = M.status == BMF_OK;
-    r.vertex_offset = ok ? M.v_off : 0;
-    r.index_offset = ok ? M.i_off : 0;
-    r.vertex_count = ok ? M.V : 0;
-    r.index_count = ok ? M.ipp * M.T : 0;
-    r.contains_mesh = M.contains_mesh;
-    r.status = M.status;
-    r.sampled = M.state == kStateSampled;
-    r.delta = M.delta;
-    r.overlap_pos[0] = M.opos[0];
-    r.overlap_pos[1] = M.opos[1];
-    r.overlap_pos[2] = M.opos[2];
-    A.results[j] = r;
-}
-
-// ------------------------------------------------------------------------
-// stage inspection in the reference's layouts (chunk 0 of the last plan)
-// ------------------------------------------------------------------------
-// DMCChunk::label_grid words [x][y][ceil(dim/32)], bit z%32 (DMCChunk.cpp:143)
 __global__ void k_ref_labels(Arenas A, uint32_t* words) {
     const ChunkMeta& M = A.meta[0];
     const int dim = M.dim, zc = (dim + 31) / 32;
@@ -1650,10 +1646,6 @@ hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
         hipLaunchKernelGGL(k_dual, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
         hipLaunchKernelGGL(k_primal, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
     }
-    return hipGetLastError();
-}
-hipError_t launch_finalize(const Arenas& A, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_finalize, dim3((n + 255) / 256), dim3(256), 0, s, A, n);
     return hipGetLastError();
 }
 hipError_t launch_ref_labels(const Arenas& A, uint32_t* words, hipStream_t s) {
