@@ -170,7 +170,9 @@ int bmf_last_timing(bmf_ctx* ctx, float* total_ms);
 /* Runs the BATCHED path once without a graph, with HIP events between the
  * stages on the context stream, and returns the stage times in ms:
  * [0] plan, [1] sample, [2] count, [3] scan, [4] emit, [5] smoothing (all
- * iterations), [6] finalize.  The per-stage counterpart of the reference's
+ * iterations), [6] finalize (no kernel since the chunk results are written
+ * by the plan and scan stages: the event pair's own overhead, a few us).
+ * The per-stage counterpart of the reference's
  * stage timers (DMCChunk.cpp:785-840, ChunkGenerator.cpp:153-193). */
 #define BMF_NUM_STAGES 7
 int bmf_profile_stages(bmf_ctx* ctx, const bmf_chunk_desc* chunks, int n, float* stage_ms);
