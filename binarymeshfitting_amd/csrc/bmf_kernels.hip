@@ -290,9 +290,6 @@ __device__ __forceinline__ float density_from_sum(const TerrainTerms& S, float s
     fb = fminf(fmaxf(fb, -1.0f), 1.0f);
     return S.amp * fb - t2;
 }
-__device__ __forceinline__ float density_from_sum(const SamplerDev& S, float sum, float t2) {
-    return density_from_sum(TerrainTerms{S.amp, S.y0, S.inv_norm}, sum, t2);
-}
 
 // 0: undecided, 1: certainly air (d < 0), 2: certainly solid.  The final
 // FP32 sum lies in [sum - reach[k], sum + reach[k]] (see SamplerDev::reach:
