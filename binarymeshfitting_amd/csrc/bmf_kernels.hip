@@ -956,6 +956,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
 //    smooth the per-cell triangle base and mask.
 // ------------------------------------------------------------------------
 constexpr int kEmitRows = 16;
+static_assert((kEmitRows + 1) * 3 * BMF_MAX_DIM < (1 << 14), "a slab's vertex ids fit the 14-bit offsets of the id table");
 
 // Local edge e of a cell (the 12-edge numbering of DMCChunk.cpp:543-565):
 // x-edges e = 2 dy + dz, y-edges e = 4 + 2 dx + dz, z-edges e = 8 + 2 dx + dy,
@@ -981,7 +982,7 @@ constexpr uint64_t kEdgeCorners = edge_corner_table();
 template <int W>
 __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M,
                                              const uint64_t (*s_lab)[plane_words(W)], uint32_t base,
-                                             uint32_t* s_idrow, int x, int p, int y, bool own) {
+                                             uint16_t* s_idrow, uint32_t slab_base, int x, int p, int y, bool own) {
     const int dim = M.dim;
     const int lane = lane_id();
     const int xp = x + p;
@@ -1007,7 +1008,7 @@ __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M
         if (z < dim) {
             const uint32_t fx = lane_bit(ex, lane), fy = lane_bit(ey, lane), fz = lane_bit(ez, lane);
             uint32_t id = kb + mbcnt64(ex) + mbcnt64(ey) + mbcnt64(ez);
-            s_idrow[z] = id | (fx << 30) | (fy << 31);
+            s_idrow[z] = (uint16_t)((id - slab_base) | (fx << 14) | (fy << 15));
             if (own && (fx | fy | fz)) {
                 // the sample and the far ends of its edges load together (an
                 // edge that is not set reads the sample again)
@@ -1046,7 +1047,8 @@ template <int W>
 __device__ __forceinline__ void emit_tris_row(const Arenas& A, const ChunkMeta& M,
                                               const uint64_t (*s_lab)[plane_words(W)], const uint8_t* s_ntri,
                                               const uint64_t* s_tri, uint32_t tcarry,
-                                              const uint32_t (*s_id)[kEmitRows + 1][64 * W], int x, int y, int ys) {
+                                              const uint16_t (*s_id)[kEmitRows + 1][64 * W], uint32_t slab0,
+                                              uint32_t slab1, int x, int y, int ys) {
     const int dim = M.dim;
     const int lane = lane_id();
     // corner rows of the cell row, and the same shifted by one sample (z + 1)
@@ -1089,8 +1091,9 @@ __device__ __forceinline__ void emit_tris_row(const Arenas& A, const ChunkMeta& 
                         const int sl = 3 * q + c;
                         const uint32_t g = (uint32_t)(kEdgeCorners >> (5 * ((row >> (4 * sl)) & 15ull))) & 31u;
                         const uint32_t e = s_id[g & 1u][y + (int)((g >> 1) & 1u) - ys][z + (int)((g >> 2) & 1u)];
+                        const uint32_t sb = (g & 1u) ? slab1 : slab0;
                         const uint32_t ax = g >> 3;
-                        v[c] = (e & 0x3FFFFFFFu) + (ax >= 1 ? (e >> 30) & 1u : 0u) + (ax == 2 ? e >> 31 : 0u);
+                        v[c] = sb + (e & 0x3FFFu) + (ax >= 1 ? (e >> 14) & 1u : 0u) + (ax == 2 ? e >> 15 : 0u);
                         bad = bad || v[c] >= Vc;
                     }
                     I[3 * q] = v[0];
@@ -1115,7 +1118,10 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
     __shared__ uint8_t s_ntri[256];
     __shared__ uint64_t s_planes[3][plane_words(W)];  // labels of planes x, x+1, x+2
     __shared__ uint32_t s_base[3][BMF_MAX_DIM + 1];  // row vertex bases of planes x, x+1; triangle bases of plane x
-    __shared__ uint32_t s_id[2][kEmitRows + 1][64 * W];  // first vertex id | x-edge bit 30 | y-edge bit 31
+    // first vertex id of each sample of the slab's rows, less the slab's first
+    // row base of its plane (a slab's rows hold at most (kEmitRows + 1) *
+    // 3 * 256 < 2^14 vertices), | x-edge bit 14 | y-edge bit 15
+    __shared__ uint16_t s_id[2][kEmitRows + 1][64 * W];
     __shared__ uint64_t s_tri[256];                       // the triangle table
     for (int t = threadIdx.x; t < 256; t += blockDim.x) {
         s_tri[t] = c_tri_table[t];
@@ -1154,20 +1160,21 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
             const int ye = min(y1, ys + kEmitRows);         // cell / owned rows [ys, ye)
             const int yl = min(dim - 1, ye);                // id rows [ys, yl]
             __syncthreads();  // staging done; the previous slab is done with s_id
+            const uint32_t slab0 = s_base[0][ys], slab1 = s_base[1][ys];  // the id table's offsets
             // pass 1: rows of plane x (and x+1 when it has cells); rows that own no
             // vertex are never referenced by a triangle and skip the label work
             const int nrow = yl - ys + 1;
             for (int t = wave; t < (cellsx ? 2 : 1) * nrow; t += NW) {
                 const int p = t >= nrow ? 1 : 0, y = ys + (t - p * nrow);
                 if (s_base[p][y + 1] == s_base[p][y]) continue;
-                emit_ids_row<W>(A, M, s_planes, s_base[p][y], s_id[p][y - ys], x, p, y, p == 0 && y < ye);
+                emit_ids_row<W>(A, M, s_planes, s_base[p][y], s_id[p][y - ys], p ? slab1 : slab0, x, p, y, p == 0 && y < ye);
             }
             if (!cellsx) continue;
             __syncthreads();
             // pass 2: cell rows with triangles
             for (int y = ys + wave; y < ye && y < dim - 1; y += NW) {
                 if (s_base[2][y + 1] == s_base[2][y]) continue;
-                emit_tris_row<W>(A, M, s_planes, s_ntri, s_tri, s_base[2][y], s_id, x, y, ys);
+                emit_tris_row<W>(A, M, s_planes, s_ntri, s_tri, s_base[2][y], s_id, slab0, slab1, x, y, ys);
             }
         }
     }
