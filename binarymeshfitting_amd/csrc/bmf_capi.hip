@@ -199,8 +199,9 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     c->persistent_blocks = c->cu_count * 8;
     c->tune.sample_tile = kSampleTileDefault;
     c->tune.sample_blocks = c->persistent_blocks;
-    // two full rounds of the widest common plane kernel (k_emit<2>: five
-    // resident workgroups per CU, bound by its LDS)
+    // ten per CU: the best of 4..20 per CU for count + emit on a batch of
+    // 512 64^3-cell chunks (fewer leave workgroups with a third plane item,
+    // more only add empty workgroups); the autotuner tries the others
     c->tune.row_blocks = c->cu_count * 10;
     c->tune.smooth_blocks = 4 * c->persistent_blocks;
     c->tune.row_split = -1;
