@@ -302,36 +302,56 @@ __device__ void place_vertex(const Arenas& A, const ChunkMeta& M, const SamplerD
     for (int q = 0; q < p && s < nn; s++)
         if (((row >> (4 * s)) & 15ull) == 15ull) q++;
     QefSums qs = {{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, {0, 0, 0}, {0, 0, 0, 0}};
-    int k = 0;
+    // the polygon's edges (at most 7; an eighth is an error), their endpoint
+    // densities loaded together before any is used; local edge e runs along
+    // axis e >> 2 from the corner whose other two offsets are bits 1 and 0
+    // of e (the rows of c_le)
+    int ee[7], k = 0;
+    float d0[7], d1[7];
+    const int64_t own = ((int64_t)x * dim + y) * dim + z;
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+        const int e = s + j < nn ? (int)((row >> (4 * (s + j))) & 15ull) : 15;
+        const bool valid = e != 15 && k == j;  // the polygon ends at its first separator
+        ee[j] = valid ? e : 0;
+        const int a = ee[j] >> 2, b1 = (ee[j] >> 1) & 1, b0 = ee[j] & 1;
+        const int ox = a == 0 ? 0 : b1, oy = a == 0 ? b1 : (a == 2 ? b0 : 0), oz = a == 2 ? 0 : b0;
+        const int64_t i0 = (((int64_t)(x + ox) * dim) + (y + oy)) * dim + (z + oz);
+        const int64_t i1 = i0 + (a == 0 ? (int64_t)dim * dim : (a == 1 ? (int64_t)dim : 1));
+        d0[j] = D[valid ? i0 : own];
+        d1[j] = D[valid ? i1 : own];
+        k += valid ? 1 : 0;
+    }
+    {
+        const int e8 = s + 7 < nn ? (int)((row >> (4 * (s + 7))) & 15ull) : 15;
+        if (k == 7 && e8 != 15) k = 8;  // more than 7 edges: reported below
+    }
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
-    for (; s < nn; s++) {
-        const int e = (int)((row >> (4 * s)) & 15ull);
-        if (e == 15) break;
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+        if (j >= k) break;
+        const int e = ee[j];
         const int x0 = x + c_le[e][0], y0 = y + c_le[e][1], z0 = z + c_le[e][2], a = c_le[e][3];
         const int x1 = x0 + (a == 0), y1 = y0 + (a == 1), z1 = z0 + (a == 2);
-        const float s0 = D[((int64_t)x0 * dim + y0) * dim + z0];
-        const float s1 = D[((int64_t)x1 * dim + y1) * dim + z1];
+        const float s0 = d0[j], s1 = d1[j];
         // DMCChunk::calculate_isovertex (DMCChunk.cpp:657-674)
         const float mu = (0.0f - s0) / (s1 - s0);
         const float dx = (float)x1 - (float)x0, dy = (float)y1 - (float)y0, dz = (float)z1 - (float)z0;
-        if (k < 7) {
-            const float qx = dx * mu + (float)x0, qy = dy * mu + (float)y0, qz = dz * mu + (float)z0;
-            ax = ax + qx;
-            ay = ay + qy;
-            az = az + qz;
-            if (qef) {
-                // unit normal of the CSG field at the point (world coordinates)
-                const float wx = qx * M.delta + M.opos[0], wy = qy * M.delta + M.opos[1], wz = qz * M.delta + M.opos[2];
-                float g[3];
-                if (s_csg) csg_gradient_lds(s_csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
-                else csg_gradient(S, csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
-                const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
-                float nrm[3];
-                for (int c = 0; c < 3; c++) nrm[c] = len > 0.0f ? g[c] / len : 0.0f;
-                qef_add(qs, qx, qy, qz, nrm);
-            }
+        const float qx = dx * mu + (float)x0, qy = dy * mu + (float)y0, qz = dz * mu + (float)z0;
+        ax = ax + qx;
+        ay = ay + qy;
+        az = az + qz;
+        if (qef) {
+            // unit normal of the CSG field at the point (world coordinates)
+            const float wx = qx * M.delta + M.opos[0], wy = qy * M.delta + M.opos[1], wz = qz * M.delta + M.opos[2];
+            float g[3];
+            if (s_csg) csg_gradient_lds(s_csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
+            else csg_gradient(S, csg, M.csg_ns, M.csg_nb, wx, wy, wz, g);
+            const float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+            float nrm[3];
+            for (int c = 0; c < 3; c++) nrm[c] = len > 0.0f ? g[c] / len : 0.0f;
+            qef_add(qs, qx, qy, qz, nrm);
         }
-        k++;
     }
     err = 0.0f;
     if (k == 0 || k > 7) {
