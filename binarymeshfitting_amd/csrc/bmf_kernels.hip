@@ -1,6 +1,8 @@
 // HIP kernels of the chunk-extraction path (gfx950, wave64).
 //
-// Per batch:  plan -> sample -> count -> scan -> emit -> smooth -> finalize
+// Per batch:  plan -> sample -> count -> scan -> emit -> smooth
+// (the per-chunk results are written by plan for culled or skipped chunks
+// and by scan for sampled ones, so no launch follows the smoothing)
 //
 // Data layout in HBM (per sampled chunk, offsets from the plan kernel):
 //   densities   FP32 [x][y][z]                    (DMCChunk density_block)
