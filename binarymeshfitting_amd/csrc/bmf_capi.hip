@@ -151,6 +151,63 @@ int check_desc(const bmf_chunk_desc& d) {
     return BMF_OK;
 }
 
+int build_sampler(const bmf_sampler_desc* s, SamplerDev* out) {
+    if (!s || !out) return BMF_EINVAL;
+    if (s->kind < BMF_SAMPLER_TERRAIN || s->kind > BMF_SAMPLER_EXTERNAL) return BMF_EINVAL;
+    if (s->kind == BMF_SAMPLER_EXTERNAL && !s->densities) return BMF_EINVAL;
+    if (s->kind == BMF_SAMPLER_CSG &&
+        (s->n_spheres < 0 || s->n_boxes < 0 || (s->n_spheres && !s->spheres) || (s->n_boxes && !s->boxes)))
+        return BMF_EINVAL;
+    SamplerDev d{};
+    d.kind = s->kind;
+    d.world_size = s->world_size;
+    d.amp = s->amp;
+    d.y0 = s->y0;
+    d.seed = s->seed;
+    if (s->kind == BMF_SAMPLER_TERRAIN) {
+        if (s->octaves < 1 || s->octaves > BMF_MAX_OCTAVES) return BMF_EINVAL;
+        if (!finite(s->amp) || !finite(s->y0) || !finite(s->freq) || !finite(s->lacunarity) || !finite(s->gain))
+            return BMF_EINVAL;
+        d.octaves = s->octaves;
+        // per-octave constants: FP32 product chains, normaliser = sum of amplitudes
+        float f = s->freq, a = 1.0f, norm = 0.0f;
+        for (int o = 0; o < s->octaves; o++) {
+            d.fo[o] = f;
+            d.ao[o] = a;
+            norm = norm + a;
+            f = f * s->lacunarity;
+            a = a * s->gain;
+        }
+        if (!(norm > 0.0f) || !finite(norm)) return BMF_EINVAL;
+        d.inv_norm = 1.0f / norm;
+        // adaptive-octave bounds (k_sample decide_label)
+        double total = 0.0;
+        for (int o = 0; o < s->octaves; o++) total += std::fabs((double)d.ao[o]);
+        const double ulp_total = total * std::ldexp(1.0, -23);
+        for (int k = 0; k <= s->octaves; k++) {
+            double r = 0.0;
+            for (int o = k; o < s->octaves; o++) r += std::fabs((double)d.ao[o]);
+            r += (double)(s->octaves - k + 2) * ulp_total;
+            float rf = (float)r;
+            if ((double)rf < r) rf = std::nextafter(rf, INFINITY);
+            d.reach[k] = rf;
+        }
+    }
+    if (s->kind == BMF_SAMPLER_CSG) {
+        d.n_spheres = s->n_spheres;
+        d.n_boxes = s->n_boxes;
+    }
+    if (s->kind == BMF_SAMPLER_EXTERNAL) d.ext = s->densities;
+    *out = d;
+    return BMF_OK;
+}
+
+void drop_graphs(bmf_ctx* c) {
+    for (auto& g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
 }  // namespace bmf
 
 extern "C" {
@@ -282,79 +339,62 @@ void bmf_destroy(bmf_ctx* c) {
 
 int bmf_set_sampler(bmf_ctx* c, const bmf_sampler_desc* s) {
     if (!c || !s) return BMF_EINVAL;
-    if (s->kind < BMF_SAMPLER_TERRAIN || s->kind > BMF_SAMPLER_EXTERNAL) return BMF_EINVAL;
-    if (s->kind == BMF_SAMPLER_EXTERNAL && !s->densities) return BMF_EINVAL;
     SamplerDev d{};
-    d.kind = s->kind;
-    d.world_size = s->world_size;
-    d.amp = s->amp;
-    d.y0 = s->y0;
-    d.seed = s->seed;
-    if (s->kind == BMF_SAMPLER_TERRAIN) {
-        if (s->octaves < 1 || s->octaves > BMF_MAX_OCTAVES) return BMF_EINVAL;
-        if (!finite(s->amp) || !finite(s->y0) || !finite(s->freq) || !finite(s->lacunarity) || !finite(s->gain))
-            return BMF_EINVAL;
-        d.octaves = s->octaves;
-        // per-octave constants: FP32 product chains, normaliser = sum of amplitudes
-        float f = s->freq, a = 1.0f, norm = 0.0f;
-        for (int o = 0; o < s->octaves; o++) {
-            d.fo[o] = f;
-            d.ao[o] = a;
-            norm = norm + a;
-            f = f * s->lacunarity;
-            a = a * s->gain;
-        }
-        if (!(norm > 0.0f) || !finite(norm)) return BMF_EINVAL;
-        d.inv_norm = 1.0f / norm;
-        // adaptive-octave bounds (k_sample decide_label)
-        double total = 0.0;
-        for (int o = 0; o < s->octaves; o++) total += std::fabs((double)d.ao[o]);
-        const double ulp_total = total * std::ldexp(1.0, -23);
-        for (int k = 0; k <= s->octaves; k++) {
-            double r = 0.0;
-            for (int o = k; o < s->octaves; o++) r += std::fabs((double)d.ao[o]);
-            r += (double)(s->octaves - k + 2) * ulp_total;
-            float rf = (float)r;
-            if ((double)rf < r) rf = std::nextafter(rf, INFINITY);
-            d.reach[k] = rf;
-        }
-    }
+    int rc = build_sampler(s, &d);
+    if (rc) return rc;
     BMF_HIP(hipSetDevice(c->device));
-    BMF_HIP(hipStreamSynchronize(c->stream));
-    if (c->d_spheres) (void)hipFree(c->d_spheres);
-    if (c->d_boxes) (void)hipFree(c->d_boxes);
-    if (c->A.csg_list) (void)hipFree(c->A.csg_list);
-    c->d_spheres = c->d_boxes = nullptr;
-    c->A.csg_list = nullptr;
-    c->A.csg_stride = 0;
+    // the new CSG buffers are allocated and filled before anything of the
+    // current sampler is released: a call that fails leaves it in place
+    float* spheres = nullptr;
+    float* boxes = nullptr;
+    uint16_t* list = nullptr;
+    int32_t stride = 0;
+    auto release_new = [&]() {
+        if (spheres) (void)hipFree(spheres);
+        if (boxes) (void)hipFree(boxes);
+        if (list) (void)hipFree(list);
+    };
     if (s->kind == BMF_SAMPLER_CSG) {
-        if (s->n_spheres < 0 || s->n_boxes < 0 || (s->n_spheres && !s->spheres) || (s->n_boxes && !s->boxes))
-            return BMF_EINVAL;
-        d.n_spheres = s->n_spheres;
-        d.n_boxes = s->n_boxes;
-        BMF_HIP(dalloc(&c->d_spheres, 4 * (int64_t)s->n_spheres));
-        BMF_HIP(dalloc(&c->d_boxes, 6 * (int64_t)s->n_boxes));
-        if (s->n_spheres)
-            BMF_HIP(hipMemcpy(c->d_spheres, s->spheres, sizeof(float) * 4 * s->n_spheres, hipMemcpyHostToDevice));
-        if (s->n_boxes)
-            BMF_HIP(hipMemcpy(c->d_boxes, s->boxes, sizeof(float) * 6 * s->n_boxes, hipMemcpyHostToDevice));
+        bool ok = dalloc(&spheres, 4 * (int64_t)s->n_spheres) == hipSuccess &&
+                  dalloc(&boxes, 6 * (int64_t)s->n_boxes) == hipSuccess;
+        if (ok && s->n_spheres)
+            ok = hipMemcpy(spheres, s->spheres, sizeof(float) * 4 * s->n_spheres, hipMemcpyHostToDevice) == hipSuccess;
+        if (ok && s->n_boxes)
+            ok = hipMemcpy(boxes, s->boxes, sizeof(float) * 6 * s->n_boxes, hipMemcpyHostToDevice) == hipSuccess;
         // per-chunk culled primitive lists (u16 ids; without them every
         // sample evaluates every primitive)
         const int64_t np = (int64_t)s->n_spheres + s->n_boxes;
-        if (np > 0 && s->n_spheres < 65536 && s->n_boxes < 65536) {
-            BMF_HIP(dalloc(&c->A.csg_list, np * c->A.cap_chunks));
-            c->A.csg_stride = (int32_t)np;
+        if (ok && np > 0 && s->n_spheres < 65536 && s->n_boxes < 65536) {
+            ok = dalloc(&list, np * c->A.cap_chunks) == hipSuccess;
+            stride = (int32_t)np;
         }
-        d.spheres = c->d_spheres;
-        d.boxes = c->d_boxes;
+        if (!ok) {
+            release_new();
+            return BMF_EHIP;
+        }
+        d.spheres = spheres;
+        d.boxes = boxes;
     }
-    if (s->kind == BMF_SAMPLER_EXTERNAL) d.ext = s->densities;
+    // queued work may still read the current sampler's buffers
+    if (hipStreamSynchronize(c->stream) != hipSuccess ||
+        hipMemcpy(c->A.sampler, &d, sizeof(SamplerDev), hipMemcpyHostToDevice) != hipSuccess) {
+        // the device copy of the sampler is now unknown: no extraction until
+        // a later call succeeds
+        release_new();
+        c->sampler_set = false;
+        drop_graphs(c);
+        return BMF_EHIP;
+    }
+    if (c->d_spheres) (void)hipFree(c->d_spheres);
+    if (c->d_boxes) (void)hipFree(c->d_boxes);
+    if (c->A.csg_list) (void)hipFree(c->A.csg_list);
+    c->d_spheres = spheres;
+    c->d_boxes = boxes;
+    c->A.csg_list = list;
+    c->A.csg_stride = stride;
     c->h_sampler = d;
-    BMF_HIP(hipMemcpy(c->A.sampler, &d, sizeof(SamplerDev), hipMemcpyHostToDevice));
     // captured frames hold the arena pointers (the CSG lists among them)
-    for (auto& g : c->graphs)
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    c->graphs.clear();
+    drop_graphs(c);
     c->sampler_set = true;
     return BMF_OK;
 }
@@ -652,9 +692,7 @@ int bmf_set_mesh_options(bmf_ctx* c, const bmf_mesh_options* o) {
     A.placement = o->placement;
     A.qef_tol = o->qef_tolerance;
     A.quad_tris = o->quads_as_triangles;
-    for (auto& g : c->graphs)
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    c->graphs.clear();
+    drop_graphs(c);
     c->have_extract = false;
     c->results_valid = false;
     return BMF_OK;
@@ -707,12 +745,6 @@ int bmf_get_tuning(bmf_ctx* c, bmf_tuning* t) {
     if (!c || !t) return BMF_EINVAL;
     *t = c->tune;
     return BMF_OK;
-}
-
-static void drop_graphs(bmf_ctx* c) {
-    for (auto& g : c->graphs)
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    c->graphs.clear();
 }
 
 int bmf_set_tuning(bmf_ctx* c, const bmf_tuning* t) {

@@ -58,5 +58,10 @@ int max_steps(const bmf_ctx* c, const bmf_chunk_desc* chunks, int n);
 int upload_descs(bmf_ctx* c, const bmf_chunk_desc* chunks, int n);
 // BMF_OK or BMF_EINVAL for one descriptor (host only).
 int check_desc(const bmf_chunk_desc& d);
+// Validates a sampler descriptor and fills its device view (host only; the
+// CSG primitive pointers are left for the caller to set).
+int build_sampler(const bmf_sampler_desc* s, SamplerDev* out);
+// Destroys the context's captured frames (they hold arena pointers).
+void drop_graphs(bmf_ctx* c);
 
 }  // namespace bmf

@@ -766,11 +766,21 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
     t->valid = false;
     c->have_extract = false;
     c->results_valid = false;
-    // the sampler of this call: the context's, or the caller's densities
+    // the sampler of this call: the context's, or the caller's densities; the
+    // context's own is put back on every way out of this call, so later
+    // extractions never read the caller's buffer
+    struct RestoreSampler {
+        bmf_ctx* c;
+        bool active = false;
+        ~RestoreSampler() {
+            if (active) (void)hipMemcpy(c->A.sampler, &c->h_sampler, sizeof(SamplerDev), hipMemcpyHostToDevice);
+        }
+    } restore{c};
     if (densities) {
         SamplerDev ext{};
         ext.kind = BMF_SAMPLER_EXTERNAL;
         ext.ext = densities;
+        restore.active = true;
         BMF_HIP(hipMemcpy(c->A.sampler, &ext, sizeof(SamplerDev), hipMemcpyHostToDevice));
     }
     rc = upload_descs(c, d, 1);
@@ -794,7 +804,6 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
     if (e == hipSuccess) e = hipMemcpyAsync(c->h_results, A.results, sizeof(bmf_chunk_result), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (densities) (void)hipMemcpy(c->A.sampler, &c->h_sampler, sizeof(SamplerDev), hipMemcpyHostToDevice);
     if (e != hipSuccess) return BMF_EHIP;
     const bmf_chunk_result r = c->h_results[0];
     if (c->h_ctr->error) return BMF_EHIP;
@@ -809,6 +818,7 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
     const int64_t slots = t->K / t->ckpt_every + 1;
     if (slots > t->ckpt_slots) {
         dfree(t->ckpt);
+        t->ckpt_slots = 0;  // a failed reallocation leaves no slots, so the next forward retries it
         BMF_HIP(dalloc(&t->ckpt, slots * t->cap_v));
         t->ckpt_slots = slots;
     }

@@ -260,6 +260,28 @@ def test_invalid_descriptors_rejected(ctx):
     assert ctx.results(1)[0] == capi.OK
 
 
+def test_rejected_sampler_keeps_the_previous_one(ctx, oracle):
+    """A CSG sampler that fails validation leaves the context's current
+    sampler (and its device buffers) in place: later extractions in both
+    modes still give the previous sampler's meshes."""
+    import ctypes as C
+
+    wl = workloads.csg_scene()
+    ctx.set_sampler(**wl.sampler)
+    spec = _ospec(oracle, wl.sampler)
+    c = wl.chunks[21]
+    L = capi.lib()
+    for n_s, n_b in [(5, 0), (-1, 0), (0, 3)]:  # counts without arrays, a negative count
+        bad = capi.SamplerDesc(capi.SAMPLER_CSG, 256.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0, None, n_s, None, n_b, None)
+        assert L.bmf_set_sampler(ctx.h, C.byref(bad)) == capi.EINVAL
+    for mode in (capi.MODE_BATCHED, capi.MODE_PLAIN):
+        ctx.extract(workloads.to_capi([c]), mode)
+        rc, res, _, _ = ctx.results(1)
+        assert rc == capi.OK
+        pos, idx, bnd = ctx.mesh(0)
+        _check_mesh(oracle, spec, c, pos, idx, bnd, res[0], tol=0.0)
+
+
 @pytest.mark.parametrize("params", [
     dict(octaves=1),
     dict(octaves=7, freq=0.005),
