@@ -43,10 +43,16 @@ def parse():
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed passes of the CPU baseline (median)")
     ap.add_argument("--lod-steps", type=int, default=10,
                     help="timed batched steps of the uneven-LOD leg (1,536 chunks); 0 skips it")
+    ap.add_argument("--lod-class", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="restrict the uneven-LOD leg to the chunks whose label rows take W u64 words "
+                         "(W = ceil(dim / 64): 1 for 33^3, 2 for 65^3, 3 for 129^3 samples), so that the frame "
+                         "forks no row-word classes onto auxiliary streams; 0 = all chunks")
     ap.add_argument("--hermite-steps", type=int, default=5,
                     help="timed batched steps of the CSG quad/QEF leg (64 chunks of 64^3 cells); 0 skips it")
     ap.add_argument("--single-steps", type=int, default=20,
                     help="timed plain extractions of the single 64^3-cell chunk; 0 skips it")
+    ap.add_argument("--quads-steps", type=int, default=10,
+                    help="timed batched steps of the uniform batch in DMC quad mode (512 chunks); 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=5,
                     help="timed steps of the forward+loss+backward leg (one 128^3-cell chunk, K=8); 0 skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -90,11 +96,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def lod_leg(capi, workloads, steps):
+def lod_leg(capi, roofline, workloads, steps, only_class=0):
     """Uneven-LOD batch: 1,536 chunks (922 / 461 / 153 of 32^3 / 64^3 / 128^3
     cells at spacing 1 / 2 / 4, origins from seed 7 in a 4096^3 world), 7-octave
-    field, K ~ U{0..8}; shapes from the autotuner, then batched and plain."""
+    field, K ~ U{0..8}; shapes from the autotuner, then batched and plain.
+    only_class W > 0 keeps the chunks with ceil(dim / 64) == W."""
     wl = workloads.mixed_lod()
+    if only_class:
+        wl = workloads.Workload(f"{wl.name}-class{only_class}", wl.sampler,
+                                [c for c in wl.chunks if (c.dim + 63) // 64 == only_class])
     cd = workloads.to_capi(wl.chunks)
     n = len(cd)
     arr = capi.chunk_array(cd)
@@ -115,6 +125,11 @@ def lod_leg(capi, workloads, steps):
         capi.check(rc, "bmf_result")
         sampled = sum(r.sampled for r in res)
         nonempty = sum(1 for r in res if r.vertex_count)
+        meshes = {j: ctx.mesh(j)[0] for j in range(n) if res[j].vertex_count}
+        flops = sum(c.dim ** 3 * roofline.terrain_flops_per_sample(wl.sampler["octaves"])
+                    + roofline.smoothing_flops(r.vertex_count, r.index_count // 3, c.iters)
+                    for r, c in zip(res, wl.chunks) if r.sampled)
+        nbytes = sum(roofline.mesh_bytes(r.vertex_count, r.index_count // 3) for r in res)
         capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
         ctx.sync()
         t0 = time.perf_counter()
@@ -123,14 +138,21 @@ def lod_leg(capi, workloads, steps):
         plain = time.perf_counter() - t0
     log(f"[bench] uneven LOD: {n / wall:.1f} chunks/s batched ({1e3 * wall:.3f} ms/step), plain {n / plain:.1f} chunks/s,"
         f" sampled {sampled}/{n}, non-empty {nonempty}/{n}, tuned {tuned} in {t_tune:.2f} s")
-    return {"workload": "uneven-lod-1536: 922/461/153 chunks of 32^3/64^3/128^3 cells, spacing 1/2/4, 7 octaves, K~U{0..8}",
+    name = "uneven-lod-1536: 922/461/153 chunks of 32^3/64^3/128^3 cells, spacing 1/2/4, 7 octaves, K~U{0..8}"
+    if only_class:
+        name += f"; only the {n} chunks with ceil(dim/64) = {only_class}"
+    return {"workload": name,
             "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4),
             "cells_per_s": round(wl.cells / wall, 1), "plain_chunks_per_s": round(n / plain, 2),
             "batched_over_plain": round(plain / wall, 2), "sampled_chunks": sampled, "nonempty_chunks": nonempty,
-            "nonempty_fraction": round(nonempty / n, 4), "tuning": tuned, "autotune_s": round(t_tune, 3)}
+            "nonempty_fraction": round(nonempty / n, 4), "surface_chunks_per_s": round(sampled / wall, 2),
+            "tuning": tuned, "autotune_s": round(t_tune, 3),
+            "roofline": roofline.frame_fractions(flops, nbytes, wall),
+            "plain_roofline": roofline.frame_fractions(flops, nbytes, plain),
+            "_check": {"workload": wl, "meshes": meshes}}
 
 
-def hermite_leg(capi, workloads, steps):
+def hermite_leg(capi, roofline, workloads, steps):
     """64 chunks of 64^3 cells tiled 4x4x4 over [0,256)^3 of the CSG union of
     200 spheres and 100 boxes (seed 42); dual-marching-cubes quads with FP32
     QEF placement (tolerance 0.1), batched."""
@@ -154,13 +176,24 @@ def hermite_leg(capi, workloads, steps):
         V = sum(r.vertex_count for r in res)
         Q = sum(r.index_count for r in res) // 4
         err = np.concatenate([ctx.qef_errors(j) for j in range(n) if res[j].vertex_count] or [np.zeros(1)])
+        meshes = {j: ctx.mesh(j)[0] for j in range(n) if res[j].vertex_count}
         stages = ctx.profile_stages(cd)
+        prims = len(wl.sampler["spheres"]) * roofline.CSG_SPHERE_FLOPS + \
+            len(wl.sampler["boxes"]) * roofline.CSG_BOX_FLOPS
+        flops = sum(c.dim ** 3 * prims for r, c in zip(res, wl.chunks) if r.sampled)
+        nbytes = sum(roofline.mesh_bytes(r.vertex_count, r.index_count // 4 * 2) for r in res)
     log(f"[bench] hermite/QEF: {n / wall:.1f} chunks/s ({1e3 * wall:.3f} ms/step), V={V} quads={Q}, "
         f"stages {', '.join(f'{k}={v:.3f}' for k, v in stages.items())}")
     return {"workload": "csg-64x64: 4x4x4 chunks of 64^3 cells, 200 spheres + 100 boxes (seed 42), DMC quads, QEF tol 0.1",
             "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4), "vertices": V, "quads": Q,
-            "qef_residual_median": float(np.median(err)), "qef_residual_max": float(err.max()),
-            "stage_ms": {k: round(v, 4) for k, v in stages.items()}}
+            "qef_residual": {"definition": "uncentred |ATb - ATA x|^2 of each vertex's QEF (qef_simd.h:434-440), "
+                                           "not an error against FP64",
+                             "median": float(np.median(err)), "max": float(err.max())},
+            "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+            "roofline": roofline.frame_fractions(flops, nbytes, wall,
+                                                 note="FLOP: every sample against all 200 spheres and 100 boxes "
+                                                      "(the uncull union), counted from csrc/samplers.h"),
+            "_check": {"workload": wl, "meshes": meshes}}
 
 
 def single_leg(capi, workloads, steps, with_cpu):
@@ -179,7 +212,15 @@ def single_leg(capi, workloads, steps, with_cpu):
             capi.check(ctx.extract_raw(arr, 1, capi.MODE_PLAIN), "bmf_extract")
             ctx.sync()
         gpu_ms = 1e3 * (time.perf_counter() - t0) / steps
-    out = {"workload": "one 64^3-cell fbm terrain chunk, K=4, plain path", "gpu_ms": round(gpu_ms, 4)}
+        pos = ctx.mesh(0)[0]
+        # the same chunk with vertex normals (the reference's smooth_normals branch)
+        ctx.set_mesh_options(normals=True)
+        capi.check(ctx.extract_raw(arr, 1, capi.MODE_PLAIN), "bmf_extract normals")
+        capi.check(ctx.results(1)[0], "bmf_result")
+        npos, nrm = ctx.mesh(0)[0], ctx.normals(0)
+        ctx.set_mesh_options(normals=False)
+    out = {"workload": "one 64^3-cell fbm terrain chunk, K=4, plain path", "gpu_ms": round(gpu_ms, 4),
+           "_check": {"workload": wl, "pos": pos, "normals_pos": npos, "normals": nrm}}
     if with_cpu:
         from oracle import pyoracle
 
@@ -197,6 +238,56 @@ def single_leg(capi, workloads, steps, with_cpu):
     return out
 
 
+def quads_leg(capi, roofline, workloads, steps):
+    """The uniform 512-chunk batch in dual-marching-cubes quad mode: one
+    vertex per EdgeTable polygon at the mass point of its edge points, one
+    quad per sign-changing edge (Tables.hpp:76-354), then K = 4 smoothing
+    steps as MeshProcessor<4> (MeshProcessor.cpp:130-306), batched."""
+    wl = workloads.uniform_grid(n_side=8, c=64, iters=4)
+    cd = workloads.to_capi(wl.chunks)
+    n = len(cd)
+    arr = capi.chunk_array(cd)
+    with capi.Context(max_chunks=n, max_samples=n * 65 ** 3, max_vertices=1 << 22, max_indices=1 << 24) as ctx:
+        ctx.set_sampler(**wl.sampler)
+        ctx.set_mesh_options(mesh=capi.MESH_QUADS)
+        for _ in range(2):
+            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        ctx.sync()
+        wall = (time.perf_counter() - t0) / steps
+        rc, res, _, _ = ctx.results(n)
+        capi.check(rc, "bmf_result")
+        sampled = sum(r.sampled for r in res)
+        V = sum(r.vertex_count for r in res)
+        Q = sum(r.index_count for r in res) // 4
+        meshes = {j: ctx.mesh(j) for j in range(n) if res[j].vertex_count}
+        counts = [(r.vertex_count, r.index_count, r.contains_mesh) for r in res]
+        stages = ctx.profile_stages(cd)
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
+        ctx.sync()
+        t0 = time.perf_counter()
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
+        ctx.sync()
+        plain = time.perf_counter() - t0
+    flops = sampled * 65 ** 3 * roofline.terrain_flops_per_sample(wl.sampler["octaves"]) + \
+        roofline.smoothing_flops(V, 2 * Q, 4)
+    nbytes = roofline.mesh_bytes(V, 2 * Q)
+    log(f"[bench] quads: {n / wall:.1f} chunks/s ({1e3 * wall:.3f} ms/step), plain {n / plain:.1f} chunks/s, "
+        f"V={V} quads={Q}, stages {', '.join(f'{k}={v:.3f}' for k, v in stages.items())}")
+    return {"workload": "uniform-512x64-quads: 8x8x8 chunks of 64^3 cells, fbm terrain seed 1337, "
+                        "DMC quads (mass point), K=4, batched",
+            "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4),
+            "cells_per_s": round(n * 64 ** 3 / wall, 1), "surface_chunks_per_s": round(sampled / wall, 2),
+            "plain_chunks_per_s": round(n / plain, 2), "batched_over_plain": round(plain / wall, 2),
+            "sampled_chunks": sampled, "vertices": V, "quads": Q,
+            "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+            "roofline": roofline.frame_fractions(flops, nbytes, wall),
+            "_check": {"workload": wl, "meshes": meshes, "counts": counts}}
+
+
 def diff_leg(capi, roofline, workloads, steps):
     """Differentiable path on one 128^3-cell terrain chunk (129^3 samples),
     8 smoothing iterations, checkpoints every 2, loss = mean squared distance
@@ -206,9 +297,14 @@ def diff_leg(capi, roofline, workloads, steps):
         ctx.set_sampler(**workloads.terrain_sampler())
         with capi.Tape(ctx, max_dim=dim, max_vertices=1 << 20, max_indices=1 << 22, checkpoint_every=2) as tape:
             cd = capi.chunk_desc((0.0, 0.0, 0.0), float(dim - 1), dim, 0.0, K, 0)
-            targets = capi.DeviceArray.from_numpy(workloads.sphere_targets())
+            targets_np = workloads.sphere_targets()
+            targets = capi.DeviceArray.from_numpy(targets_np)
             info = tape.forward(cd)
-            L, g, _ = tape.loss_nearest(targets)
+            # the targets are fixed: their search hierarchy is built once
+            t0 = time.perf_counter()
+            tape.set_targets(targets)
+            t_build = time.perf_counter() - t0
+            L, g, near = tape.loss_nearest_set()
             grad = capi.DeviceArray((dim, dim, dim), np.float32)
             capi.check(capi.lib().bmf_backward_diff(tape.h, g.ptr, grad.ptr), "bmf_backward_diff")
             t_f = t_l = t_b = 0.0
@@ -216,12 +312,14 @@ def diff_leg(capi, roofline, workloads, steps):
                 t0 = time.perf_counter()
                 capi.check(capi.lib().bmf_forward_diff(ctx.h, capi.C.byref(cd), None, tape.h), "bmf_forward_diff")
                 t1 = time.perf_counter()
-                capi.check(capi.lib().bmf_loss_nearest(tape.h, targets.ptr, targets.shape[0], g.ptr, None, None),
-                           "bmf_loss_nearest")
+                capi.check(capi.lib().bmf_tape_loss_nearest(tape.h, g.ptr, None, None), "bmf_tape_loss_nearest")
                 t2 = time.perf_counter()
                 capi.check(capi.lib().bmf_backward_diff(tape.h, g.ptr, grad.ptr), "bmf_backward_diff")
                 t3 = time.perf_counter()
                 t_f, t_l, t_b = t_f + t1 - t0, t_l + t2 - t1, t_b + t3 - t2
+            check = {"chunk": workloads.Chunk((0.0, 0.0, 0.0), float(dim - 1), dim, 0.0, K, 0),
+                     "densities": tape.densities(), "targets": targets_np, "nearest": near,
+                     "gradient": grad.numpy()}
             V = info.vertex_count
             T = info.index_count // 3
             n3 = dim ** 3
@@ -237,7 +335,10 @@ def diff_leg(capi, roofline, workloads, steps):
                     "backward_ms": round(1e3 * t_b / steps, 4), "vertices": V, "triangles": T, "loss": L,
                     "forward_fp32_frac": round(fwd_flops / (t_f / steps) / 1e12 / roofline.PEAK_FP32_TFLOPS, 4),
                     "backward_hbm_frac": round(bwd_bytes / (t_b / steps) / 1e9 / roofline.PEAK_HBM_GBS, 5),
-                    "timing": "host wall clock per call; each call ends with a stream synchronisation"}
+                    "targets_build_ms": round(1e3 * t_build, 4),
+                    "timing": "host wall clock per call; each call ends with a stream synchronisation; the "
+                              "target hierarchy is built once (targets_build_ms) and each step's loss queries it",
+                    "_check": check}
 
 
 def main():
@@ -332,6 +433,7 @@ def main():
 
     # ---- plain path ---------------------------------------------------------------
     plain = None
+    plain_roof = None
     if not args.no_plain and args.plain_steps > 0:
         capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
         ctx.sync()
@@ -342,16 +444,22 @@ def main():
         ctx.sync()
         plain = n * args.plain_steps / (time.perf_counter() - t0)
         log(f"[bench] plain: {plain:.1f} chunks/s")
+        plain_roof = roofline.frame_fractions(path_flops, path_bytes, n / plain)
 
     # ---- uneven LOD: 1,536 chunks of 32/64/128 cells, autotuned shapes ------------------
     lod = None
     if args.lod_steps > 0:
-        lod = lod_leg(capi, workloads, args.lod_steps)
+        lod = lod_leg(capi, roofline, workloads, args.lod_steps, args.lod_class)
 
     # ---- Hermite / QEF: 64 CSG chunks, dual quads placed by FP32 QEF ---------------------
     hermite = None
     if args.hermite_steps > 0:
-        hermite = hermite_leg(capi, workloads, args.hermite_steps)
+        hermite = hermite_leg(capi, roofline, workloads, args.hermite_steps)
+
+    # ---- config 2 as worded: the same batch in dual-marching-cubes quad mode --------------
+    quads = None
+    if args.quads_steps > 0:
+        quads = quads_leg(capi, roofline, workloads, args.quads_steps)
 
     # ---- single chunk, plain path: latency against one CPU thread ----------------------
     single = None
@@ -366,6 +474,7 @@ def main():
     # ---- CPU baseline + parity -----------------------------------------------------
     cpu = None
     parity = None
+    fp64 = None
     if not args.no_cpu:
         from oracle import pyoracle
 
@@ -402,6 +511,74 @@ def main():
                     bad += 1
         parity = {"chunks": n, "mismatched": bad, "check": "bit-exact positions and triangle lists vs CPU"}
         log(f"[bench] parity: {bad} of {n} chunks differ")
+        # ---- FP32-vs-FP64 error of every leg's results (the oracle's FP64 twins) ---------
+        from oracle import fp64_errors as fe
+
+        t0 = time.perf_counter()
+        js = [j for j in range(n) if res[j].vertex_count]
+        gpos = {j: ctx.mesh(j)[0] for j in js}
+        e = fe.merge(fe.over_chunks(lambda j: fe.triangle_positions(spec, wl.chunks[j], gpos[j]), js, threads))
+        fp64 = {"positions": fe.stats(e, dim - 1)}
+        if lod:
+            ck = lod["_check"]
+            lw = ck["workload"]
+            parts = fe.over_chunks(lambda j: (fe.triangle_positions(pyoracle.SamplerSpec(**lw.sampler), lw.chunks[j],
+                                                                    ck["meshes"][j]) / (lw.chunks[j].dim - 1)),
+                                   sorted(ck["meshes"]), threads)
+            lod["fp64_error"] = {"positions_over_extent": fe.stats(fe.merge(parts))}
+        if hermite:
+            ck = hermite["_check"]
+            hw = ck["workload"]
+            hspec = pyoracle.SamplerSpec(**hw.sampler)
+            parts = fe.over_chunks(lambda j: fe.quad_positions(hspec, hw.chunks[j], ck["meshes"][j],
+                                                               placement=pyoracle.PLACE_QEF, tol=0.1),
+                                   sorted(ck["meshes"]), threads)
+            hermite["fp64_error"] = {"qef_positions": fe.stats(fe.merge(parts), 64)}
+        if quads:
+            ck = quads["_check"]
+            qw = ck["workload"]
+            qspec = pyoracle.SamplerSpec(**qw.sampler)
+
+            def quad_ref(j):
+                c = qw.chunks[j]
+                oc = pyoracle.make_chunk(c.pos, c.size, c.dim, c.overlap, c.iters, c.boundary_processing)
+                delta, opos = pyoracle.geometry(oc)
+                m = pyoracle.dmc_mesh(pyoracle.sample_block(qspec, c.dim, opos, delta), qspec, opos, delta)
+                if c.iters > 0 and m["V"] and m["nI"]:
+                    m["pos"] = pyoracle.smooth_n(m["pos"], m["boundary"], m["idx"], 4, c.iters, c.boundary_processing)
+                return m
+
+            qbad = 0
+            for j, m in enumerate(fe.over_chunks(quad_ref, range(n), threads)):
+                V_, nI_, _ = ck["counts"][j]
+                if V_ != m["V"] or nI_ != m["nI"]:
+                    qbad += 1
+                elif V_:
+                    qp, qi, qb = ck["meshes"][j]
+                    if not (np.array_equal(qp, m["pos"]) and np.array_equal(qi, m["idx"])
+                            and np.array_equal(qb, m["boundary"])):
+                        qbad += 1
+            quads["parity"] = {"chunks": n, "mismatched": qbad,
+                               "check": "bit-exact positions, quad lists and boundary flags vs CPU"}
+            parts = fe.over_chunks(lambda j: fe.quad_positions(qspec, qw.chunks[j], ck["meshes"][j][0]),
+                                   sorted(ck["meshes"]), threads)
+            quads["fp64_error"] = {"positions": fe.stats(fe.merge(parts), 64)}
+            log(f"[bench] quads parity: {qbad} of {n} chunks differ")
+        if single:
+            ck = single["_check"]
+            c0 = ck["workload"].chunks[0]
+            ep = fe.triangle_positions(spec, c0, ck["pos"])
+            epn, en = fe.triangle_normals(spec, c0, ck["normals_pos"], ck["normals"])
+            single["fp64_error"] = {"positions": fe.stats(ep, c0.dim - 1), "normals": fe.stats(en),
+                                    "positions_with_normals": fe.stats(epn, c0.dim - 1)}
+        if diff:
+            ck = diff["_check"]
+            dc = ck["chunk"]
+            dmesh = pyoracle.extract_chunk(spec, pyoracle.make_chunk(dc.pos, dc.size, dc.dim, dc.overlap, dc.iters,
+                                                                      dc.boundary_processing))
+            diff["fp64_error"] = fe.gradient(ck["densities"], dmesh, dc.iters, dc.boundary_processing,
+                                             ck["targets"], ck["nearest"], ck["gradient"], threads=threads)
+        log(f"[bench] FP64 error statements in {time.perf_counter() - t0:.1f} s")
 
     ms_per_step = 1e3 * wall / args.steps
     line = {
@@ -421,18 +598,25 @@ def main():
                    "chunks": n, "cells_per_chunk": (dim - 1) ** 3, "smoothing_iters": 4, "octaves": octaves,
                    "sampled_chunks": sampled, "vertices": V, "triangles": T},
         "cells_per_s": round(value * (dim - 1) ** 3, 1),
+        "surface_chunks_per_s": round(sampled * args.steps / wall, 2),
         "plain_chunks_per_s": round(plain, 2) if plain else None,
+        "plain_roofline": plain_roof,
         "batched_over_plain": round(value / plain, 2) if plain else None,
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roof,
         "path_roofline": path,
+        "fp64_error": fp64,
         "uneven_lod": lod,
         "hermite_qef": hermite,
+        "dmc_quads": quads,
         "single_chunk": single,
         "fwd_bwd": diff,
         "cpu_baseline": cpu,
         "parity": parity,
     }
+    for leg in (lod, hermite, quads, single, diff):
+        if leg:
+            leg.pop("_check", None)
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0

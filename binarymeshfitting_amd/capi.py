@@ -22,8 +22,11 @@ EXPORTS = [
     "bmf_label_block", "bmf_cell_masks", "bmf_last_timing", "bmf_profile_stages", "bmf_tape_create",
     "bmf_tape_destroy", "bmf_forward_diff", "bmf_tape_get", "bmf_tape_positions", "bmf_loss_nearest",
     "bmf_backward_diff", "bmf_get_tuning", "bmf_set_tuning", "bmf_autotune", "bmf_set_mesh_options",
-    "bmf_get_mesh_options", "bmf_qef_error_to_host", "bmf_normals_to_host",
+    "bmf_get_mesh_options", "bmf_qef_error_to_host", "bmf_normals_to_host", "bmf_tape_set_targets",
+    "bmf_tape_loss_nearest", "bmf_format", "bmf_lod_queue", "bmf_extract_queued",
 ]
+SPACE_GRID, SPACE_WORLD = 0, 1
+COLOR_NONE, COLOR_WHITE, COLOR_NOISE = 0, 1, 2
 MESH_TRIANGLES, MESH_QUADS = 0, 1
 PLACE_MASS_POINT, PLACE_QEF = 0, 1
 NUM_STAGES = 7
@@ -80,6 +83,28 @@ class TapeInfo(C.Structure):
                 ("densities", C.c_void_p)]
 
 
+class LodParams(C.Structure):
+    _fields_ = [("focus", C.c_float * 3), ("split_multiplier", C.c_float), ("group_multiplier", C.c_float),
+                ("size_modifier", C.c_float), ("max_level", C.c_int32), ("min_level", C.c_int32),
+                ("max_gen", C.c_int32), ("chunk_resolution", C.c_int32), ("process_iters", C.c_int32),
+                ("boundary_processing", C.c_int32), ("overlap", C.c_float)]
+
+    @classmethod
+    def of(cls, p):
+        """From a workloads.LodParams."""
+        return cls((C.c_float * 3)(*p.focus), p.split_multiplier, p.group_multiplier, p.size_modifier, p.max_level,
+                   p.min_level, p.max_gen, p.chunk_resolution, p.process_iters, p.boundary_processing, p.overlap)
+
+
+class FormatOptions(C.Structure):
+    _fields_ = [("space", C.c_int32), ("colors", C.c_int32), ("global_indices", C.c_int32)]
+
+
+class Vbo(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("colors", C.c_void_p), ("normals", C.c_void_p), ("indices", C.c_void_p),
+                ("cap_vertices", C.c_int64), ("cap_indices", C.c_int64)]
+
+
 _lib = None
 
 
@@ -126,6 +151,12 @@ def lib():
         L.bmf_tape_positions.argtypes = [tp, C.c_int32, C.c_void_p]
         L.bmf_loss_nearest.argtypes = [tp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, P(C.c_float)]
         L.bmf_backward_diff.argtypes = [tp, C.c_void_p, C.c_void_p]
+        L.bmf_tape_set_targets.argtypes = [tp, C.c_void_p, C.c_int64]
+        L.bmf_tape_loss_nearest.argtypes = [tp, C.c_void_p, C.c_void_p, P(C.c_float)]
+        L.bmf_format.argtypes = [ctxp, P(FormatOptions), P(Vbo), P(C.c_int64), P(C.c_int64)]
+        L.bmf_lod_queue.argtypes = [ctxp, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, P(LodParams), C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32)]
+        L.bmf_extract_queued.argtypes = [ctxp, C.c_int]
         _lib = L
     return _lib
 
@@ -262,6 +293,48 @@ class Context:
         out = np.zeros((V, 3), np.float32)
         check(lib().bmf_normals_to_host(self.h, chunk, _ptr(out, C.c_float), V), "bmf_normals_to_host")
         return out
+
+    def format(self, space=SPACE_GRID, colors=COLOR_WHITE, global_indices=True, normals=False, cap_vertices=None,
+               cap_indices=None):
+        """bmf_format into fresh device arrays; returns (rc, positions, colors, normals, indices) as numpy
+        (None for arrays not asked for), cut to the formatted totals."""
+        view = MeshView()
+        check(lib().bmf_mesh_view_get(self.h, C.byref(view)), "bmf_mesh_view_get")
+        cv = view.total_vertices if cap_vertices is None else cap_vertices
+        ci = view.total_indices if cap_indices is None else cap_indices
+        pos = DeviceArray((max(cv, 1), 3), np.float32)
+        col = DeviceArray((max(cv, 1), 3), np.float32) if colors != COLOR_NONE else None
+        nrm = DeviceArray((max(cv, 1), 3), np.float32) if normals else None
+        idx = DeviceArray((max(ci, 1),), np.uint32)
+        o = FormatOptions(space, colors, int(global_indices))
+        vb = Vbo(pos.ptr, col.ptr if col else None, nrm.ptr if nrm else None, idx.ptr, cv, ci)
+        nv, ni = C.c_int64(), C.c_int64()
+        rc = lib().bmf_format(self.h, C.byref(o), C.byref(vb), C.byref(nv), C.byref(ni))
+        V, I = nv.value, ni.value
+        if rc != OK:
+            return rc, V, I, None, None, None, None
+        return (rc, V, I, pos.numpy()[:V], col.numpy()[:V] if col else None, nrm.numpy()[:V] if nrm else None,
+                idx.numpy()[:I])
+
+    def lod_queue(self, nodes, renderables, params, cap_items=None):
+        """bmf_lod_queue on host arrays (workloads.WORLD_NODE nodes, int32 list, workloads.LodParams).
+        Returns (rc, flags, dirty, items, n_chunks); the descriptors stay in the context for extract_queued."""
+        cap = self.limits.max_chunks if cap_items is None else cap_items
+        dn = DeviceArray.from_numpy(np.ascontiguousarray(nodes))
+        dr = DeviceArray.from_numpy(np.ascontiguousarray(renderables, np.int32))
+        fl = DeviceArray((max(len(nodes), 1),), np.uint32)
+        di = DeviceArray((max(len(renderables), 1),), np.int32)
+        it = DeviceArray((max(cap, 1),), np.dtype([("node", "<i4"), ("child", "<i4"), ("level", "<i4"),
+                                                    ("action", "<i4"), ("morton", "<u8")], align=True))
+        nd, nc = C.c_int32(), C.c_int32()
+        rc = lib().bmf_lod_queue(self.h, dn.ptr, len(nodes), dr.ptr, len(renderables), C.byref(LodParams.of(params)),
+                                 fl.ptr, di.ptr, it.ptr, cap, C.byref(nd), C.byref(nc))
+        if rc != OK:
+            return rc, None, None, None, nc.value
+        return rc, fl.numpy()[: len(nodes)], di.numpy()[: nd.value], it.numpy()[: nc.value], nc.value
+
+    def extract_queued(self, mode=MODE_BATCHED):
+        check(lib().bmf_extract_queued(self.h, mode), "bmf_extract_queued")
 
     def qef_errors(self, chunk):
         rc, res, _, _ = self.results(chunk + 1)
@@ -416,6 +489,19 @@ class Tape:
         L = C.c_float()
         check(lib().bmf_loss_nearest(self.h, targets.ptr, targets.shape[0], g.ptr, near.ptr, C.byref(L)),
               "bmf_loss_nearest")
+        return L.value, g, near.numpy()[: i.vertex_count]
+
+    def set_targets(self, targets):
+        """Builds the search hierarchy of a fixed target set (DeviceArray [n][3] FP32) once."""
+        check(lib().bmf_tape_set_targets(self.h, targets.ptr, targets.shape[0]), "bmf_tape_set_targets")
+
+    def loss_nearest_set(self):
+        """The loss against the set of set_targets: (loss, dL/dp DeviceArray, nearest numpy)."""
+        i = self.info()
+        g = DeviceArray((max(i.vertex_count, 1), 3), np.float32)
+        near = DeviceArray((max(i.vertex_count, 1),), np.int32)
+        L = C.c_float()
+        check(lib().bmf_tape_loss_nearest(self.h, g.ptr, near.ptr, C.byref(L)), "bmf_tape_loss_nearest")
         return L.value, g, near.numpy()[: i.vertex_count]
 
     def backward(self, dldp):

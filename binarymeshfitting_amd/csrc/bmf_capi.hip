@@ -124,6 +124,7 @@ namespace bmf {
 // buffer is refilled only after its previous copy has executed, so frames
 // can be enqueued back to back without a host synchronisation.
 int upload_descs(bmf_ctx* c, const bmf_chunk_desc* chunks, int n) {
+    c->queued_n = -1;  // the arena no longer holds bmf_lod_queue's descriptors
     const int slot = c->upload_slot;
     c->upload_slot ^= 1;
     BMF_HIP(hipEventSynchronize(c->ev_upload[slot]));
@@ -325,6 +326,12 @@ void bmf_destroy(bmf_ctx* c) {
         if (c->h_descs[i]) (void)hipHostFree(c->h_descs[i]);
     if (c->h_results) (void)hipHostFree(c->h_results);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+    if (c->d_fmt) (void)hipFree(c->d_fmt);
+    if (c->h_fmt) (void)hipHostFree(c->h_fmt);
+    if (c->d_lod_first) (void)hipFree(c->d_lod_first);
+    if (c->d_lod_code) (void)hipFree(c->d_lod_code);
+    if (c->d_lod_counts) (void)hipFree(c->d_lod_counts);
+    if (c->h_lod_counts) (void)hipHostFree(c->h_lod_counts);
     hipEvent_t evs[] = {c->ev0, c->ev1, c->ev_upload[0], c->ev_upload[1], c->ev_fork, c->ev_join[0], c->ev_join[1],
                         c->ev_join[2]};
     for (hipEvent_t e : evs)
@@ -399,21 +406,22 @@ int bmf_set_sampler(bmf_ctx* c, const bmf_sampler_desc* s) {
     return BMF_OK;
 }
 
-int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
-    if (!c) return BMF_EINVAL;
-    if (!c->sampler_set) return BMF_ESTATE;
-    if (mode != BMF_MODE_PLAIN && mode != BMF_MODE_BATCHED) return BMF_EINVAL;
-    int rc = bmf_validate_chunks(chunks, n, &c->lim);
-    if (rc) return rc;
+// The extraction of n descriptors: `chunks` (host) uploaded first, or, when
+// null, the n descriptors already in the arena, all with uni's dim and iters.
+static int extract_impl(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode, const bmf_chunk_desc* uni) {
     BMF_HIP(hipSetDevice(c->device));
     c->results_valid = false;
     c->last_n = n;
     c->have_extract = true;
     if (n == 0) return BMF_OK;
-    rc = upload_descs(c, chunks, n);
-    if (rc) return rc;
-    int maxk = max_steps(c, chunks, n), wclasses = 0;
-    for (int i = 0; i < n; i++) wclasses |= row_word_class(chunks[i].dim);
+    int rc = BMF_OK;
+    if (chunks) {
+        rc = upload_descs(c, chunks, n);
+        if (rc) return rc;
+    }
+    auto desc = [&](int j) -> const bmf_chunk_desc& { return chunks ? chunks[j] : *uni; };
+    int maxk = chunks ? max_steps(c, chunks, n) : max_steps(c, uni, 1), wclasses = 0;
+    for (int i = 0; i < (chunks ? n : 1); i++) wclasses |= row_word_class(desc(i).dim);
     const Arenas& A = c->A;
     const int pb = c->persistent_blocks;
     if (mode == BMF_MODE_PLAIN) {
@@ -423,16 +431,17 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
         BMF_HIP(hipEventRecord(c->ev0, c->stream));
         int64_t ext_base = 0;
         for (int j = 0; j < n; j++) {
+            const bmf_chunk_desc& dj = desc(j);
             Arenas Aj = A;
             Aj.ext_base = ext_base;
-            ext_base += (int64_t)chunks[j].dim * chunks[j].dim * chunks[j].dim;
+            ext_base += (int64_t)dj.dim * dj.dim * dj.dim;
             Aj.descs = A.descs + j;
             Aj.meta = A.meta + j;
             Aj.results = A.results + j;
-            const int64_t dim = chunks[j].dim;
+            const int64_t dim = dj.dim;
             const int64_t ns = dim * dim * dim;
             const int sb = (int)std::min<int64_t>(ceil_div(ns, A.sample_tile), pb);
-            rc = enqueue_pipeline(c, Aj, 1, max_steps(c, &chunks[j], 1), row_word_class(chunks[j].dim), 1, j == 0, sb,
+            rc = enqueue_pipeline(c, Aj, 1, max_steps(c, &dj, 1), row_word_class(dj.dim), 1, j == 0, sb,
                                   (int)dim * kMaxPlaneSplit, 1,
                                   std::min(pb, 1024));
             if (rc) return rc;
@@ -472,6 +481,25 @@ int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
     BMF_HIP(hipGraphLaunch(ge->exec, c->stream));
     BMF_HIP(hipEventRecord(c->ev1, c->stream));
     return BMF_OK;
+}
+
+int bmf_extract(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mode) {
+    if (!c) return BMF_EINVAL;
+    if (!c->sampler_set) return BMF_ESTATE;
+    if (mode != BMF_MODE_PLAIN && mode != BMF_MODE_BATCHED) return BMF_EINVAL;
+    int rc = bmf_validate_chunks(chunks, n, &c->lim);
+    if (rc) return rc;
+    return extract_impl(c, chunks, n, mode, nullptr);
+}
+
+int bmf_extract_queued(bmf_ctx* c, int mode) {
+    if (!c) return BMF_EINVAL;
+    if (!c->sampler_set || c->queued_n < 0) return BMF_ESTATE;
+    if (mode != BMF_MODE_PLAIN && mode != BMF_MODE_BATCHED) return BMF_EINVAL;
+    bmf_chunk_desc uni{};
+    uni.dim = c->queued_dim;
+    uni.iters = c->queued_iters;
+    return extract_impl(c, nullptr, c->queued_n, mode, &uni);
 }
 
 int bmf_sync(bmf_ctx* c) {
@@ -660,7 +688,6 @@ int bmf_set_mesh_options(bmf_ctx* c, const bmf_mesh_options* o) {
     if (!finite(o->qef_tolerance) || o->qef_tolerance < 0.0f) return BMF_EINVAL;
     if (o->quads_as_triangles != 0 && o->quads_as_triangles != 1) return BMF_EINVAL;
     if (o->normals != 0 && o->normals != 1) return BMF_EINVAL;
-    if (o->normals && o->mesh != BMF_MESH_TRIANGLES) return BMF_EINVAL;
     BMF_HIP(hipSetDevice(c->device));
     BMF_HIP(hipStreamSynchronize(c->stream));
     Arenas& A = c->A;

@@ -48,6 +48,16 @@ struct bmf_ctx {
     int persistent_blocks = 2048;  // default workgroups of the persistent kernels
     bmf_tuning tune{};             // BATCHED launch shapes (bmf_set_tuning / bmf_autotune)
     bmf_mesh_options mesh{0, 0, 0.1f, 0, 0};  // bmf_set_mesh_options
+    int64_t* d_fmt = nullptr;  // bmf_format: per-chunk offsets and tile counts [4][max_chunks + 1]
+    int64_t* h_fmt = nullptr;  // pinned: formatted vertex and index totals
+    // bmf_lod_queue: scratch, and the descriptors it left in the descriptor
+    // arena for bmf_extract_queued (queued_n < 0: none)
+    int* d_lod_first = nullptr;    // [lod_cap_nodes] lowest list position of a group candidate child
+    uint8_t* d_lod_code = nullptr; // [lod_cap_list] pass-A code of each list position
+    int lod_cap_nodes = 0, lod_cap_list = 0;
+    int* d_lod_counts = nullptr;   // dirty nodes, chunks
+    int* h_lod_counts = nullptr;   // pinned
+    int queued_n = -1, queued_dim = 0, queued_iters = 0;
 };
 
 namespace bmf {

@@ -606,8 +606,10 @@ struct bmf_tape {
     int* err = nullptr;
     float4* ckpt = nullptr;
     int64_t ckpt_slots = 0;
-    // nearest-target search
+    // nearest-target search: hierarchy buffers for up to nn_cap targets and
+    // the target set last built into them (nn_n targets, 0 = none)
     int64_t nn_cap = 0;
+    int64_t nn_n = 0;
     uint32_t* keys[2] = {nullptr, nullptr};
     int32_t* vals[2] = {nullptr, nullptr};
     float4* st = nullptr;
@@ -643,6 +645,7 @@ void free_nn(bmf_tape* t) {
     t->cub_tmp = nullptr;
     t->cub_bytes = 0;
     t->nn_cap = 0;
+    t->nn_n = 0;
 }
 
 int reserve_nn(bmf_tape* t, int64_t n) {
@@ -667,6 +670,61 @@ int reserve_nn(bmf_tape* t, int64_t n) {
         return BMF_EHIP;
     }
     t->nn_cap = n;
+    return BMF_OK;
+}
+
+// Buffers of the nearest-target hierarchy of n targets: Morton keys and
+// target ids (double-buffered for the radix sort), the targets in Morton
+// order (x, y, z, id), and the boxes of the heap-numbered nodes 1 .. 2P-1.
+struct NNBuffers {
+    uint32_t* keys[2];
+    int32_t* vals[2];
+    float4* st;
+    float4* blo;
+    float4* bhi;
+    void* cub_tmp;
+    size_t cub_bytes;
+};
+
+NNBuffers tape_nn(bmf_tape* t) {
+    return NNBuffers{{t->keys[0], t->keys[1]}, {t->vals[0], t->vals[1]}, t->st, t->blo, t->bhi, t->cub_tmp,
+                     t->cub_bytes};
+}
+
+// Enqueues the hierarchy build of the n targets [n][3] into b: bounding box,
+// Morton keys, radix sort of (key, id), sorted targets, leaf and inner boxes.
+int nn_build(const float* targets, int64_t n, int* bb, const NNBuffers& b, hipStream_t s) {
+    const int64_t P = leaf_pow2(n);
+    hipLaunchKernelGGL(k_nn_bbox_init, dim3(1), dim3(64), 0, s, bb);
+    hipLaunchKernelGGL(k_nn_bbox, dim3((unsigned)std::min<int64_t>(256, blocks_for(n))), dim3(kThreads), 0, s, targets,
+                       n, bb);
+    hipLaunchKernelGGL(k_nn_keys, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, n, bb, b.keys[0], b.vals[0]);
+    BMF_HIP(hipGetLastError());
+    hipcub::DoubleBuffer<uint32_t> kb(b.keys[0], b.keys[1]);
+    hipcub::DoubleBuffer<int32_t> vb(b.vals[0], b.vals[1]);
+    size_t bytes = b.cub_bytes;
+    BMF_HIP(hipcub::DeviceRadixSort::SortPairs(b.cub_tmp, bytes, kb, vb, (int)n, 0, 30, s));
+    hipLaunchKernelGGL(k_nn_gather, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, vb.Current(), n, b.st);
+    hipLaunchKernelGGL(k_nn_leaves, dim3(blocks_for(P)), dim3(kThreads), 0, s, b.st, n, P, b.blo, b.bhi);
+    int64_t level = P >> 1;
+    for (; level >= 2048; level >>= 1)
+        hipLaunchKernelGGL(k_nn_level, dim3(blocks_for(level)), dim3(kThreads), 0, s, level, b.blo, b.bhi);
+    if (level >= 1) hipLaunchKernelGGL(k_nn_inner, dim3(1), dim3(1024), 0, s, level, b.blo, b.bhi);
+    BMF_HIP(hipGetLastError());
+    return BMF_OK;
+}
+
+// Enqueues the nearest-target query of the tape's V final positions against
+// a hierarchy of n targets, dL/dp, and the loss (FP64, fixed order) into
+// loss_out (device).
+int nn_query(bmf_tape* t, const NNBuffers& b, int64_t n, float* dldp, int32_t* nearest, double* loss_out,
+             hipStream_t s) {
+    const int64_t P = leaf_pow2(n);
+    hipLaunchKernelGGL(k_nn_query, dim3((unsigned)(((int64_t)t->V * 64 + kNNThreads - 1) / kNNThreads)),
+                       dim3(kNNThreads), 0, s, t->pos, t->V, b.st, n, P, b.blo, b.bhi,
+                       nearest ? nearest : t->nearest_tmp, dldp, t->d2);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, t->d2, t->V, loss_out);
+    BMF_HIP(hipGetLastError());
     return BMF_OK;
 }
 
@@ -899,33 +957,51 @@ int bmf_loss_nearest(bmf_tape* t, const float* targets, int64_t n, float* dldp, 
     if (!t->valid) return BMF_ESTATE;
     BMF_HIP(hipSetDevice(t->ctx->device));
     hipStream_t s = t->ctx->stream;
+    int rc = reserve_nn(t, n);
+    if (rc) return rc;
+    t->nn_n = 0;
+    rc = nn_build(targets, n, t->bb, tape_nn(t), s);
+    if (rc) return rc;
+    t->nn_n = n;
+    if (t->V == 0) {
+        BMF_HIP(hipStreamSynchronize(s));
+        if (loss) *loss = 0.0f;
+        return BMF_OK;
+    }
+    rc = nn_query(t, tape_nn(t), n, dldp, nearest, t->loss, s);
+    if (rc) return rc;
+    double l = 0.0;
+    BMF_HIP(hipMemcpyAsync(&l, t->loss, sizeof(double), hipMemcpyDeviceToHost, s));
+    BMF_HIP(hipStreamSynchronize(s));
+    if (loss) *loss = (float)l;
+    return BMF_OK;
+}
+
+int bmf_tape_set_targets(bmf_tape* t, const float* targets, int64_t n) {
+    if (!t || !targets || n < 1 || n > (int64_t)0x7FFFFFFF) return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(t->ctx->device));
+    hipStream_t s = t->ctx->stream;
+    int rc = reserve_nn(t, n);
+    if (rc) return rc;
+    t->nn_n = 0;
+    rc = nn_build(targets, n, t->bb, tape_nn(t), s);
+    if (rc) return rc;
+    BMF_HIP(hipStreamSynchronize(s));
+    t->nn_n = n;
+    return BMF_OK;
+}
+
+int bmf_tape_loss_nearest(bmf_tape* t, float* dldp, int32_t* nearest, float* loss) {
+    if (!t || !dldp) return BMF_EINVAL;
+    if (!t->valid || t->nn_n < 1) return BMF_ESTATE;
+    BMF_HIP(hipSetDevice(t->ctx->device));
+    hipStream_t s = t->ctx->stream;
     if (t->V == 0) {
         if (loss) *loss = 0.0f;
         return BMF_OK;
     }
-    int rc = reserve_nn(t, n);
+    int rc = nn_query(t, tape_nn(t), t->nn_n, dldp, nearest, t->loss, s);
     if (rc) return rc;
-    const int64_t P = leaf_pow2(n);
-    hipLaunchKernelGGL(k_nn_bbox_init, dim3(1), dim3(64), 0, s, t->bb);
-    hipLaunchKernelGGL(k_nn_bbox, dim3((unsigned)std::min<int64_t>(256, blocks_for(n))), dim3(kThreads), 0, s, targets,
-                       n, t->bb);
-    hipLaunchKernelGGL(k_nn_keys, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, n, t->bb, t->keys[0], t->vals[0]);
-    BMF_HIP(hipGetLastError());
-    hipcub::DoubleBuffer<uint32_t> kb(t->keys[0], t->keys[1]);
-    hipcub::DoubleBuffer<int32_t> vb(t->vals[0], t->vals[1]);
-    size_t bytes = t->cub_bytes;
-    BMF_HIP(hipcub::DeviceRadixSort::SortPairs(t->cub_tmp, bytes, kb, vb, (int)n, 0, 30, s));
-    hipLaunchKernelGGL(k_nn_gather, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, vb.Current(), n, t->st);
-    hipLaunchKernelGGL(k_nn_leaves, dim3(blocks_for(P)), dim3(kThreads), 0, s, t->st, n, P, t->blo, t->bhi);
-    int64_t level = P >> 1;
-    for (; level >= 2048; level >>= 1)
-        hipLaunchKernelGGL(k_nn_level, dim3(blocks_for(level)), dim3(kThreads), 0, s, level, t->blo, t->bhi);
-    if (level >= 1) hipLaunchKernelGGL(k_nn_inner, dim3(1), dim3(1024), 0, s, level, t->blo, t->bhi);
-    hipLaunchKernelGGL(k_nn_query, dim3((unsigned)(((int64_t)t->V * 64 + kNNThreads - 1) / kNNThreads)),
-                       dim3(kNNThreads), 0, s,
-                       t->pos, t->V, t->st, n, P, t->blo, t->bhi, nearest ? nearest : t->nearest_tmp, dldp, t->d2);
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, t->d2, t->V, t->loss);
-    BMF_HIP(hipGetLastError());
     double l = 0.0;
     BMF_HIP(hipMemcpyAsync(&l, t->loss, sizeof(double), hipMemcpyDeviceToHost, s));
     BMF_HIP(hipStreamSynchronize(s));

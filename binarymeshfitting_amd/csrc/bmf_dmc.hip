@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
         const uint32_t* rv = A.row_v + M.row_off + (int64_t)x * dim;
         const int64_t vo = M.v_off;
         const uint32_t Vc = (uint32_t)M.V;
-        const bool smooth = M.iters > 0 && M.T > 0;
+        const bool smooth = (M.iters > 0 || A.normals) && M.T > 0;  // the chunk runs the smoothing kernels
         // ---- row pass ----
         for (int y = wave; y < dim - 1; y += kRowThreads / 64) {
             if (rv[y + 1] == rv[y]) continue;  // the row's cells have no polygon
@@ -453,6 +453,13 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
             P[2] = o[2];
             if (smooth) A.pos4[vo + v] = make_float4(o[0], o[1], o[2], 0.0f);
             if (A.qef_err) A.qef_err[vo + v] = err;
+            if (A.normals) {  // a vertex with no polygon keeps the zero normal
+                A.nrm4[vo + v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                float* O = A.nrm + (vo + v) * 3;
+                O[0] = 0.0f;
+                O[1] = 0.0f;
+                O[2] = 0.0f;
+            }
         }
     }
 }
@@ -484,7 +491,7 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
         }
         __syncthreads();
         const uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
-        const bool smooth = M.iters > 0;
+        const bool smooth = M.iters > 0 || A.normals;  // rings are needed
         const uint32_t Vc = (uint32_t)M.V, Tc = (uint32_t)M.T;
         uint32_t* __restrict__ I = A.idx + M.i_off;
         for (int y = wave; y < dim; y += kRowThreads / 64) {
@@ -673,6 +680,168 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_ring(Arenas A, int m)
     }
 }
 
+// With vertex normals (the smooth_normals branch of MeshProcessor<N>,
+// MeshProcessor.cpp:159-216, 271-287): every chunk of the list runs max(K, 1)
+// steps.  Step m's face normal while m == 0, m < max_norms (min(K/2 - 3, 10))
+// or m < 3 is, for a triangle, -cross(normalize(p0 - p1), normalize(p0 - p2));
+// for a quad, n1 = cross(normalize(p0 - p1), normalize(p0 - p2)) and
+// n2 = cross(normalize(p2 - p3), normalize(p2 - p0)) with the reference's NaN
+// fallbacks (a NaN n1.x takes n2, or (0, 1, 0) when n2 is NaN too; a NaN
+// n2.x then takes n1), and -normalize((n1 + n2) * 0.5) (:187-205); later
+// steps take the sum of the corners' normals.  normalize(v) is
+// v / sqrt((x*x + y*y) + z*z).
+__device__ __forceinline__ float3 unit_cross(float ux, float uy, float uz, float wx, float wy, float wz) {
+    const float lu = sqrtf((ux * ux + uy * uy) + uz * uz), lw = sqrtf((wx * wx + wy * wy) + wz * wz);
+    ux = ux / lu;
+    uy = uy / lu;
+    uz = uz / lu;
+    wx = wx / lw;
+    wy = wy / lw;
+    wz = wz / lw;
+    return make_float3(uy * wz - wy * uz, uz * wx - wz * ux, ux * wy - wx * uy);
+}
+
+__global__ void __launch_bounds__(kSmoothThreads) k_dual_n_nrm(Arenas A, int m) {
+    const int nsm = A.ctr->n_smooth;
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t end = A.smooth_xcd_t[xcd + 1];
+    for (int64_t tile = A.smooth_xcd_t[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
+        const int k = find_chunk(A.smooth_tprefix, nsm, tile);
+        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int K = M.iters;
+        if (max(K, 1) <= m) continue;
+        const int64_t t = (tile - A.smooth_tprefix[k]) * kSmoothThreads + threadIdx.x;
+        if (t >= M.T) continue;
+        const int N = M.ipp;
+        const uint32_t* __restrict__ I = A.idx + M.i_off + (int64_t)N * t;
+        const float4* __restrict__ P = A.pos4 + M.v_off;
+        const float4* __restrict__ NV = A.nrm4 + M.v_off;
+        uint32_t u[4] = {0u, 0u, 0u, 0u};
+        bool ok = true;
+        for (int c = 0; c < N; c++) {
+            u[c] = I[c];
+            if (u[c] >= (uint32_t)M.V) ok = false;
+        }
+        if (!ok) {
+            A.ctr->error = 11;
+            continue;
+        }
+        float4 q[4];
+        for (int c = 0; c < N; c++) q[c] = P[u[c]];
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        for (int c = 0; c < N; c++) {
+            cx = cx + q[c].x;
+            cy = cy + q[c].y;
+            cz = cz + q[c].z;
+        }
+        const float fN = (float)N;
+        const int64_t tt = M.i_off / N + t;
+        A.cen4[tt] = make_float4(cx / fN, cy / fN, cz / fN, 0.0f);
+        const int max_norms = min(K / 2 - 3, 10);
+        float4 fn;
+        if (m == 0 || m < max_norms || m < 3) {
+            if (N == 3) {
+                const float3 f = unit_cross(q[0].x - q[1].x, q[0].y - q[1].y, q[0].z - q[1].z, q[0].x - q[2].x,
+                                            q[0].y - q[2].y, q[0].z - q[2].z);
+                fn = make_float4(-f.x, -f.y, -f.z, 0.0f);
+            } else {
+                float3 n1 = unit_cross(q[0].x - q[1].x, q[0].y - q[1].y, q[0].z - q[1].z, q[0].x - q[2].x,
+                                       q[0].y - q[2].y, q[0].z - q[2].z);
+                float3 n2 = unit_cross(q[2].x - q[3].x, q[2].y - q[3].y, q[2].z - q[3].z, q[2].x - q[0].x,
+                                       q[2].y - q[0].y, q[2].z - q[0].z);
+                if (__builtin_isnan(n1.x)) {
+                    n1 = n2;
+                    if (__builtin_isnan(n1.x)) n1 = make_float3(0.0f, 1.0f, 0.0f);
+                }
+                if (__builtin_isnan(n2.x)) n2 = n1;
+                const float hx = (n1.x + n2.x) * 0.5f, hy = (n1.y + n2.y) * 0.5f, hz = (n1.z + n2.z) * 0.5f;
+                const float lh = sqrtf((hx * hx + hy * hy) + hz * hz);
+                fn = make_float4(-(hx / lh), -(hy / lh), -(hz / lh), 0.0f);
+            }
+        } else {
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            for (int c = 0; c < N; c++) {
+                const float4 nc = NV[u[c]];
+                sx = sx + nc.x;
+                sy = sy + nc.y;
+                sz = sz + nc.z;
+            }
+            fn = make_float4(sx, sy, sz, 0.0f);
+        }
+        A.fn4[tt] = fn;
+    }
+}
+
+// Primal step with normals over the sorted ring: the ring's centroids and
+// face normals summed from +0 in ascending primitive order and divided by
+// the ring size; the normal is normalised when m == 3 or (m == 0 and K <= 3)
+// and kept when its y is not 0, for every vertex with a ring; positions of
+// boundary vertices stay unless boundary processing (K > 0 only); the last
+// step writes the unit normal (0 for a zero normal).
+__global__ void __launch_bounds__(kSmoothThreads) k_primal_ring_nrm(Arenas A, int m) {
+    const int nsm = A.ctr->n_smooth;
+    const int xcd = blockIdx.x % kXcds;
+    const int64_t end = A.smooth_xcd_v[xcd + 1];
+    for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
+        const int k = find_chunk(A.smooth_prefix, nsm, tile);
+        const ChunkMeta& M = A.meta[A.smooth_list[k]];
+        const int K = M.iters;
+        const int steps = max(K, 1);
+        if (steps <= m) continue;
+        const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
+        if (v >= M.V) continue;
+        const int64_t g = M.v_off + v;
+        const int n = (int)min(A.ring_n[g], (uint32_t)kRingSlots);
+        const int64_t t_off = M.i_off / M.ipp;
+        const float4* __restrict__ C = A.cen4 + t_off;
+        const float4* __restrict__ F = A.fn4 + t_off;
+        float4 nv = A.nrm4[g];
+        if (n > 0) {
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+            for (int i = 0; i < n; i++) {
+                const uint32_t pid = A.ring[(int64_t)i * A.cap_vertices + g];
+                const float4 q = C[pid], f = F[pid];
+                sx = sx + q.x;
+                sy = sy + q.y;
+                sz = sz + q.z;
+                nx = nx + f.x;
+                ny = ny + f.y;
+                nz = nz + f.z;
+            }
+            const float fc = (float)n;
+            float ax = nx / fc, ay = ny / fc, az = nz / fc;
+            const bool unit = m != steps - 1 && (m == 3 || (m == 0 && K <= 3));
+            if (unit) {
+                const float l = sqrtf((ax * ax + ay * ay) + az * az);
+                ax = ax / l;
+                ay = ay / l;
+                az = az / l;
+            }
+            if (ay != 0.0f) {
+                nv = make_float4(ax, ay, az, 0.0f);
+                A.nrm4[g] = nv;
+            }
+            if (K > 0 && (M.bp || !A.boundary[g])) {
+                const float px = sx / fc, py = sy / fc, pz = sz / fc;
+                A.pos4[g] = make_float4(px, py, pz, 0.0f);
+                if (m == K - 1) {
+                    float* __restrict__ P = A.pos + g * 3;
+                    P[0] = px;
+                    P[1] = py;
+                    P[2] = pz;
+                }
+            }
+        }
+        if (m == steps - 1) {
+            const float l = sqrtf((nv.x * nv.x + nv.y * nv.y) + nv.z * nv.z);
+            float* __restrict__ O = A.nrm + g * 3;
+            O[0] = l > 0.0f ? nv.x / l : 0.0f;
+            O[1] = l > 0.0f ? nv.y / l : 0.0f;
+            O[2] = l > 0.0f ? nv.z / l : 0.0f;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------
@@ -693,8 +862,13 @@ hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, con
 hipError_t launch_dmc_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
     blocks = blocks < kXcds ? kXcds : blocks - blocks % kXcds;  // whole XCD groups
     if (m == 0) hipLaunchKernelGGL(k_ring_sort, dim3(blocks), dim3(kSmoothThreads), 0, s, A);
-    hipLaunchKernelGGL(k_dual_n, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
-    hipLaunchKernelGGL(k_primal_ring, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    if (A.normals) {
+        hipLaunchKernelGGL(k_dual_n_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+        hipLaunchKernelGGL(k_primal_ring_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    } else {
+        hipLaunchKernelGGL(k_dual_n, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+        hipLaunchKernelGGL(k_primal_ring, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
+    }
     return hipGetLastError();
 }
 

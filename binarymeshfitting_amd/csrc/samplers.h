@@ -111,15 +111,22 @@ constexpr int kOctaveFlops = 5;
 // coordinate, normalise, 2 clamp, amp multiply, y - y0, final subtract.
 constexpr int kTerrainFlops = 15;
 
-__device__ __forceinline__ float terrain_density(const SamplerDev& S, float x, float y, float z) {
+// FBM of 3D simplex noise: sum of a_o * snoise3(p * f_o) (fused
+// accumulate), normalised by the sum of the amplitudes, clamped to [-1, 1].
+__device__ __forceinline__ float fbm_value(const float* fo, const float* ao, float inv_norm, int octaves,
+                                           uint32_t seed, float x, float y, float z) {
     float acc = 0.0f;
-    for (int o = 0; o < S.octaves; o++) {
-        const float f = S.fo[o];
-        float n = snoise3(x * f, y * f, z * f, S.seed + (uint32_t)o);
-        acc = __builtin_fmaf(S.ao[o], n, acc);
+    for (int o = 0; o < octaves; o++) {
+        const float f = fo[o];
+        float n = snoise3(x * f, y * f, z * f, seed + (uint32_t)o);
+        acc = __builtin_fmaf(ao[o], n, acc);
     }
-    float fb = acc * S.inv_norm;
-    fb = fminf(fmaxf(fb, -1.0f), 1.0f);
+    float fb = acc * inv_norm;
+    return fminf(fmaxf(fb, -1.0f), 1.0f);
+}
+
+__device__ __forceinline__ float terrain_density(const SamplerDev& S, float x, float y, float z) {
+    const float fb = fbm_value(S.fo, S.ao, S.inv_norm, S.octaves, S.seed, x, y, z);
     float t1 = S.amp * fb;
     float t2 = y - S.y0;
     return t1 - t2;
@@ -151,6 +158,13 @@ __device__ __forceinline__ float implicit_density(int kind, float res, float x, 
     }
     return 0.0f;
 }
+
+// FP32 operations per primitive of the CSG union below (add, sub, mul, sqrt,
+// abs, min/max each 1): sphere 3 sub + 3 mul + 2 add + sqrt + sub + max;
+// box 3 sub + 3 abs + 3 sub + 3 max + 3 mul + 2 add + sqrt + 3 min/max + add
+// + neg + max.
+constexpr int kCsgSphereFlops = 11;
+constexpr int kCsgBoxFlops = 24;
 
 // CSG union of spheres r - |p - c| and exact boxes -sdf, floored.
 __device__ __forceinline__ float csg_density(const SamplerDev& S, float x, float y, float z) {

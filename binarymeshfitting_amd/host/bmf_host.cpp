@@ -156,4 +156,24 @@ int ChunkGenerator::process_queue(std::vector<WorldOctreeNode*>& batch) {
     return BMF_OK;
 }
 
+int ChunkGenerator::format_batch(const bmf_format_options& options, const bmf_vbo& out, int64_t* vertices,
+                                 int64_t* indices) {
+    if (!ctx_) return BMF_ESTATE;
+    return bmf_format(ctx_, &options, &out, vertices, indices);
+}
+
+int ChunkGenerator::process_lod(const bmf_world_node* nodes, int32_t n_nodes, const int32_t* renderables,
+                                int32_t n_renderables, const bmf_lod_params& params, uint32_t* flags_out,
+                                int32_t* dirty_out, bmf_lod_item* items_out, int32_t cap_items, int32_t* n_dirty,
+                                int32_t* n_chunks) {
+    if (!ctx_) return BMF_ESTATE;
+    int rc = bmf_lod_queue(ctx_, nodes, n_nodes, renderables, n_renderables, &params, flags_out, dirty_out, items_out,
+                           cap_items, n_dirty, n_chunks);
+    if (rc) return rc;
+    rc = bmf_extract_queued(ctx_, mode);
+    if (rc) return rc;
+    results_.resize((size_t)*n_chunks);
+    return bmf_result(ctx_, results_.data(), *n_chunks, nullptr, nullptr);
+}
+
 }  // namespace bmf_host

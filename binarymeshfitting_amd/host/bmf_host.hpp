@@ -137,6 +137,17 @@ public:
     // status; on BMF_ECAPACITY the arenas are grown to the reported needs and
     // the batch is run again once.
     int process_queue(std::vector<WorldOctreeNode*>& batch);
+    // WorldOctreeNode::format -> GLChunk::format_data over the last batch
+    // (WorldOctreeNode.cpp:68-86, GLChunk.cpp:278-297, DMCChunk.cpp:578-591),
+    // on the device into `out`: bmf_format.  Returns a BMF_* status.
+    int format_batch(const bmf_format_options& options, const bmf_vbo& out, int64_t* vertices, int64_t* indices);
+    // The watcher's LOD pass over a node table and renderables list in device
+    // memory (WorldWatcher::check_leaves / process_batch), then the extraction
+    // of the generate batch it queued on the device: bmf_lod_queue +
+    // bmf_extract_queued.  Returns a BMF_* status.
+    int process_lod(const bmf_world_node* nodes, int32_t n_nodes, const int32_t* renderables, int32_t n_renderables,
+                    const bmf_lod_params& params, uint32_t* flags_out, int32_t* dirty_out, bmf_lod_item* items_out,
+                    int32_t cap_items, int32_t* n_dirty, int32_t* n_chunks);
     int mode = BMF_MODE_BATCHED;
     WorldProperties properties;
 

@@ -4,7 +4,9 @@
 // (sum over the u32 stream of position bit patterns then indices of
 // word_k * (2k + 1), times 0x9E3779B97F4A7C15, modulo 2^64).
 // With smooth_normals = 1 it also prints the same checksum over the normals.
-//   bmf_host_demo <n_side> <resolution> <iters> <mode: 0 plain, 1 batched> [smooth_normals]
+// Every node sits at `level` (default max_level = 7): below max_level the
+// overlap rule of ChunkGenerator.cpp:98 gives 0.035 + 0.005 * iters.
+//   bmf_host_demo <n_side> <resolution> <iters> <mode: 0 plain, 1 batched> [smooth_normals] [level]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -21,6 +23,7 @@ int main(int argc, char** argv) {
     const int mode = argc > 4 ? std::atoi(argv[4]) : BMF_MODE_BATCHED;
     const bool normals = argc > 5 && std::atoi(argv[5]) != 0;
     WorldProperties props;
+    const int level = argc > 6 ? std::atoi(argv[6]) : props.max_level;
     props.smooth_normals = normals;
     props.chunk_resolution = res;
     props.process_iters = iters;
@@ -41,11 +44,11 @@ int main(int argc, char** argv) {
         for (int j = 0; j < n_side; j++)
             for (int k = 0; k < n_side; k++) {
                 chunks.emplace_back(new DMCChunk());
-                chunks.back()->init(vec3(size * i, size * j, size * k), size, props.max_level, sampler);
+                chunks.back()->init(vec3(size * i, size * j, size * k), size, level, sampler);
                 chunks.back()->dim = (uint32_t)res;
                 nodes.emplace_back(new WorldOctreeNode());
                 nodes.back()->chunk = chunks.back().get();
-                nodes.back()->level = props.max_level;
+                nodes.back()->level = level;
                 batch.push_back(nodes.back().get());
             }
     rc = gen.process_queue(batch);

@@ -18,6 +18,8 @@ SIMPLEX_FLOPS = _const("kSimplexFlops")
 OCTAVE_FLOPS = _const("kOctaveFlops")
 TERRAIN_FLOPS = _const("kTerrainFlops")
 HASH_INT_OPS = _const("kHashIntOps")
+CSG_SPHERE_FLOPS = _const("kCsgSphereFlops")
+CSG_BOX_FLOPS = _const("kCsgBoxFlops")
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, packed (78.6 unpacked)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6
@@ -46,3 +48,17 @@ def mesh_bytes(V, T):
     """Compulsory output bytes of a mesh: 12 B positions + 1 B boundary per
     vertex, 12 B indices per triangle, 16 B of counts."""
     return 13 * V + 12 * T + 16
+
+
+def frame_fractions(flops, nbytes, seconds, note=None):
+    """Whole-frame work over its time against the FP32-VALU and HBM peaks:
+    algorithmic FLOP (full-octave sampling and the smoothing) and compulsory
+    mesh bytes per frame."""
+    out = {"flops_per_step": int(flops), "bytes_per_step": int(nbytes),
+           "tflops": round(flops / seconds / 1e12, 4),
+           "fp32_frac": round(flops / seconds / 1e12 / PEAK_FP32_TFLOPS, 5),
+           "hbm_gbs": round(nbytes / seconds / 1e9, 3),
+           "hbm_frac": round(nbytes / seconds / 1e9 / PEAK_HBM_GBS, 6)}
+    if note:
+        out["note"] = note
+    return out

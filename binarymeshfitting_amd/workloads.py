@@ -154,3 +154,98 @@ def to_capi(chunks):
     from . import capi
 
     return [capi.chunk_desc(c.pos, c.size, c.dim, c.overlap, c.iters, c.boundary_processing) for c in chunks]
+
+
+# ---- world octree for the LOD pass (bmf_lod_queue) ----------------------------------------
+# numpy layouts of bmf_world_node / bmf_lod_params / bmf_lod_item (include/bmf.h)
+WORLD_NODE = np.dtype([("pos", "<f4", 3), ("size", "<f4"), ("level", "<i4"), ("flags", "<u4"), ("world_leaf", "<i4"),
+                       ("parent", "<i4"), ("children", "<i4", 8), ("morton", "<u8")], align=True)
+LOD_ITEM = np.dtype([("node", "<i4"), ("child", "<i4"), ("level", "<i4"), ("action", "<i4"), ("morton", "<u8")],
+                    align=True)
+NODE_DRAW, NODE_DESTROY, NODE_SPLIT, NODE_GROUP, NODE_DIRTY, NODE_DRAW_CHILDREN, NODE_SUPERCEDED = \
+    1, 2, 4, 8, 32, 64, 256
+_MCD = ((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1))  # Tables.hpp:7-9
+
+
+@dataclass
+class LodParams:
+    """WorldProperties defaults (WorldOctree.cpp:20-33) and the watcher's max_gen (WorldWatcher.cpp:39)."""
+    focus: tuple = (0.0, 0.0, 0.0)
+    split_multiplier: float = 1.0
+    group_multiplier: float = 2.0
+    size_modifier: float = 0.0
+    max_level: int = 7
+    min_level: int = 1
+    max_gen: int = 400
+    chunk_resolution: int = 33
+    process_iters: int = 0
+    boundary_processing: int = 0
+    overlap: float = 0.035
+
+
+def _needs_split(pos, size, level, p):
+    """WorldOctree::node_needs_split, for building the input tree (float32)."""
+    if level >= p.max_level:
+        return False
+    if level < p.min_level:
+        return True
+    f = np.float32
+    h = f(size) * f(0.5)
+    d = [f(p.focus[a]) - (f(pos[a]) + h) for a in range(3)]
+    dist = np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+    return dist < (f(size) * f(p.split_multiplier) + f(p.size_modifier)) + f(size) * f(0.5)
+
+
+def lod_world(world_size=4096.0, build_focus=(0.0, 0.0, 0.0), params=None, seed=0, perturb=0.0):
+    """A world octree as WorldOctree::init + split_leaves leave it
+    (WorldOctree.cpp:115-173): the root of side 2 * world_size centred on the
+    origin, split while node_needs_split(build_focus); leaves are drawn world
+    leaves, inner nodes superseded.  Returns (nodes, renderables) with the
+    leaves in depth-first order as the list.  perturb > 0 flips that fraction
+    of flags at random (DESTROY, SPLIT, DRAW_CHILDREN, GROUP, SUPERCEDED,
+    world_leaf) to exercise every branch of the pass."""
+    p = params or LodParams(focus=build_focus)
+    if params is not None:
+        p = LodParams(**{**p.__dict__, "focus": tuple(build_focus)})
+    rows = []
+    size = np.float32(2.0 * world_size)
+    rows.append(dict(pos=(np.float32(-0.5) * size,) * 3, size=size, level=0, parent=-1, morton=1))
+    leaves = []
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        r = rows[i]
+        if not _needs_split(r["pos"], r["size"], r["level"], p):
+            leaves.append(i)
+            r["children"] = [-1] * 8
+            continue
+        cs = np.float32(r["size"]) * np.float32(0.5)
+        kids = []
+        for k, (dx, dy, dz) in enumerate(_MCD):
+            cp = tuple(np.float32(r["pos"][a]) + np.float32(d) * cs for a, d in enumerate((dx, dy, dz)))
+            kids.append(len(rows))
+            rows.append(dict(pos=cp, size=cs, level=r["level"] + 1, parent=i,
+                             morton=(r["morton"] << 3) | (dx | (dy << 1) | (dz << 2))))
+        r["children"] = kids
+        stack.extend(kids)
+    nodes = np.zeros(len(rows), WORLD_NODE)
+    for i, r in enumerate(rows):
+        leaf = r["children"][0] < 0
+        nodes[i]["pos"] = r["pos"]
+        nodes[i]["size"] = r["size"]
+        nodes[i]["level"] = r["level"]
+        nodes[i]["flags"] = NODE_DRAW if leaf else NODE_SUPERCEDED
+        nodes[i]["world_leaf"] = int(leaf)
+        nodes[i]["parent"] = r["parent"]
+        nodes[i]["children"] = r["children"]
+        nodes[i]["morton"] = r["morton"]
+    if perturb > 0:
+        rng = SplitMix64(seed)
+        u = rng.uniforms(3 * len(nodes)).reshape(len(nodes), 3)
+        bits = (NODE_DESTROY, NODE_SPLIT, NODE_DRAW_CHILDREN, NODE_GROUP, NODE_SUPERCEDED, NODE_DRAW)
+        for i in range(len(nodes)):
+            if u[i, 0] < perturb:
+                nodes[i]["flags"] ^= bits[int(u[i, 1] * len(bits)) % len(bits)]
+            if u[i, 2] < perturb / 4:
+                nodes[i]["world_leaf"] ^= 1
+    return nodes, np.array(leaves, np.int32)

@@ -199,19 +199,23 @@ typedef struct {
     int32_t placement;           /* BMF_PLACE_* (QUADS only) */
     float qef_tolerance;         /* truncation threshold of the QEF pseudo-inverse */
     int32_t quads_as_triangles;  /* QUADS: write (a,b,c), (c,d,a) per quad */
-    int32_t normals;             /* TRIANGLES: also compute unit vertex normals (below) */
+    int32_t normals;             /* also compute unit vertex normals (below), either mesh */
 } bmf_mesh_options;
 
-/* Vertex normals (TRIANGLES).  The reference computes them only with
- * smooth_normals (MeshProcessor.cpp:159-216, 271-287), off on its live path
- * (ChunkGenerator.cpp:116); here they follow that branch for N = 3: per
- * smoothing step m the face normal -cross(normalize(p0-p1), normalize(p0-p2))
- * while m == 0, m < min(K/2-3, 10) or m < 3, else the sum of the corners'
- * normals; per primal step the ring average, normalised when m == 3 or
- * (m == 0 and K <= 3), kept when its y is not 0, for every vertex with a ring
- * (boundary vertices included; their positions stay pinned); K = 0 runs one
- * such step without moving positions; the output is the final normal divided
- * by its length (0 for a zero normal). */
+/* Vertex normals.  The reference computes them only with smooth_normals
+ * (MeshProcessor.cpp:159-216, 271-287), off on its live path
+ * (ChunkGenerator.cpp:116); here they follow that branch of MeshProcessor<N>:
+ * per smoothing step m the face normal while m == 0, m < min(K/2-3, 10) or
+ * m < 3 -- for a triangle -cross(normalize(p0-p1), normalize(p0-p2)), for a
+ * quad -normalize((n1 + n2) * 0.5) with n1 = cross(normalize(p0-p1),
+ * normalize(p0-p2)), n2 = cross(normalize(p2-p3), normalize(p2-p0)) and the
+ * reference's NaN fallbacks (MeshProcessor.cpp:187-205) -- else the sum of
+ * the corners' normals; per primal step the ring average, normalised when
+ * m == 3 or (m == 0 and K <= 3), kept when its y is not 0, for every vertex
+ * with a ring (boundary vertices included; their positions stay pinned);
+ * K = 0 runs one such step without moving positions; the output is the final
+ * normal divided by its length (0 for a zero normal or a vertex in no
+ * polygon).  QUADS written as triangles use the triangle face normal. */
 int bmf_normals_to_host(bmf_ctx* ctx, int chunk, float* normals, int64_t cap_vertices);
 
 int bmf_set_mesh_options(bmf_ctx* ctx, const bmf_mesh_options* options);
@@ -220,6 +224,120 @@ int bmf_get_mesh_options(bmf_ctx* ctx, bmf_mesh_options* options);
 /* Per-vertex QEF residual |ATb - ATA x|^2 (qef_simd.h:434-440) of one
  * chunk's vertices (QUADS with BMF_PLACE_QEF; zeros otherwise). */
 int bmf_qef_error_to_host(bmf_ctx* ctx, int chunk, float* errors, int64_t cap_vertices);
+
+/* ---- output formatting ------------------------------------------------- */
+/* The meshes of the last bmf_extract as VBO-ready arrays in device memory:
+ * WorldOctreeNode::format -> GLChunk::format_data (WorldOctreeNode.cpp:68-86,
+ * GLChunk.cpp:278-297) for every chunk of the batch, concatenated in batch
+ * order as DMCChunk::copy_verts_and_inds does (DMCChunk.cpp:578-591).  A
+ * chunk is included when it contains a mesh (contains_mesh, the condition of
+ * both) and its status is BMF_OK; its first vertex / index in the arrays is
+ * the sum of the vertex / index counts of the included chunks before it
+ * (bmf_result gives the counts). */
+enum { BMF_SPACE_GRID = 0,   /* positions in grid units: the reference's VBO (its vertex
+                                shader applies p * scale + overlap_pos, shaders/main_vs.glsl:34) */
+       BMF_SPACE_WORLD = 1   /* positions p * scale + overlap_pos (one FP32 multiply, then an add) */
+};
+enum { BMF_COLOR_NONE = 0,   /* no colours written */
+       BMF_COLOR_WHITE = 1,  /* DualVertex::color = (1, 1, 1) (DMCChunk.cpp:681) */
+       BMF_COLOR_NOISE = 2   /* ColorMapper::generate_colors (ColorMapper.cpp:15-62): n = 4 * FBM at
+                                the vertex's grid position (the project's simplex FBM, frequency 0.01,
+                                lacunarity 2, gain 0.5, 4 octaves, seed 1337; the reference's
+                                FastNoiseSIMD is not part of its tree), colour
+                                hsl_to_rgb((n + 1) * 0.5 * 360, 0.72, 1) (:60-119, an HSV map) */
+};
+typedef struct {
+    int32_t space;           /* BMF_SPACE_* */
+    int32_t colors;          /* BMF_COLOR_* */
+    int32_t global_indices;  /* 1: indices plus the chunk's first vertex in the arrays; 0: chunk-local */
+} bmf_format_options;
+
+typedef struct {
+    float* positions;        /* device [cap_vertices][3] */
+    float* colors;           /* device [cap_vertices][3]; may be NULL with BMF_COLOR_NONE */
+    float* normals;          /* device [cap_vertices][3] unit normals, or NULL (needs mesh option normals) */
+    uint32_t* indices;       /* device [cap_indices] */
+    int64_t cap_vertices;
+    int64_t cap_indices;
+} bmf_vbo;
+
+/* Formats the last extraction into `out` on the context stream, waits, and
+ * writes the totals to vertices / indices (may be NULL).  BMF_ECAPACITY (and
+ * nothing written) when they exceed the capacities; BMF_ESTATE before any
+ * extraction or when normals are asked for without the mesh option. */
+int bmf_format(bmf_ctx* ctx, const bmf_format_options* options, const bmf_vbo* out, int64_t* vertices,
+               int64_t* indices);
+
+/* ---- device work-queue producer ---------------------------------------- */
+/* The watcher thread's LOD pass on the GPU: WorldWatcher::check_leaves
+ * (WorldWatcher.cpp:155-212) over the renderables list with
+ * WorldOctree::node_needs_split / node_needs_group (WorldOctree.cpp:239-261),
+ * then process_batch (WorldWatcher.cpp:214-238, split_node / group_node_1)
+ * and ChunkGenerator::process_queue's chunk set-up (create_chunk,
+ * WorldOctree.cpp:263-269; the overlap rule, ChunkGenerator.cpp:98).  The
+ * descriptors of the resulting generate batch are written straight into the
+ * context's descriptor arena; bmf_extract_queued then extracts them with no
+ * host upload. */
+#define BMF_NODE_DRAW 1u             /* NODE_FLAGS (WorldOctreeNode.hpp:15-27) */
+#define BMF_NODE_DESTROY 2u
+#define BMF_NODE_SPLIT 4u
+#define BMF_NODE_GROUP 8u
+#define BMF_NODE_DIRTY 32u
+#define BMF_NODE_DRAW_CHILDREN 64u
+#define BMF_NODE_SUPERCEDED 256u
+
+/* The fields of WorldOctreeNode (WorldOctreeNode.hpp) the pass reads; one
+ * entry per node of the world octree, referenced by index. */
+typedef struct {
+    float pos[3];            /* OctreeNode::pos */
+    float size;              /* OctreeNode::size */
+    int32_t level;           /* OctreeNode::level */
+    uint32_t flags;          /* BMF_NODE_* */
+    int32_t world_leaf;      /* world_leaf_flag */
+    int32_t parent;          /* node index, -1 for the root */
+    int32_t children[8];     /* node indices, -1 when absent */
+    uint64_t morton;         /* morton_code */
+} bmf_world_node;
+
+typedef struct {
+    float focus[3];          /* WorldWatcher::focus_pos */
+    float split_multiplier;  /* WorldProperties (WorldOctree.cpp:20-33) */
+    float group_multiplier;
+    float size_modifier;
+    int32_t max_level;
+    int32_t min_level;
+    int32_t max_gen;         /* the cap of check_leaves (400 in WorldWatcher::update) */
+    int32_t chunk_resolution;/* dim of every chunk */
+    int32_t process_iters;
+    int32_t boundary_processing;
+    float overlap;           /* base overlap of the rule */
+} bmf_lod_params;
+
+enum { BMF_LOD_SPLIT = 1, BMF_LOD_GROUP = 2 };
+
+/* One chunk of the generate batch, in batch order. */
+typedef struct {
+    int32_t node;            /* the split node (the chunk is its child `child`) or the grouped node */
+    int32_t child;           /* 0..7 (the child's corner, Tables.hpp:7-9), -1 for a grouped node */
+    int32_t level;           /* the chunk's level */
+    int32_t action;          /* BMF_LOD_SPLIT or BMF_LOD_GROUP */
+    uint64_t morton;         /* the chunk node's morton code ((parent << 3) | corner for a child) */
+} bmf_lod_item;
+
+/* nodes [n_nodes], renderables [n_renderables] (node indices in list order),
+ * flags_out [n_nodes] (every node's flags after the pass: SPLIT / GROUP set on
+ * the pushed nodes), dirty_out [n_renderables] (the pushed nodes in push
+ * order) and items_out [cap_items] are device memory.  Waits, then writes the
+ * counts; BMF_ECAPACITY when the generate batch exceeds cap_items or the
+ * context's max_chunks (n_chunks then holds the size needed). */
+int bmf_lod_queue(bmf_ctx* ctx, const bmf_world_node* nodes, int32_t n_nodes, const int32_t* renderables,
+                  int32_t n_renderables, const bmf_lod_params* params, uint32_t* flags_out, int32_t* dirty_out,
+                  bmf_lod_item* items_out, int32_t cap_items, int32_t* n_dirty, int32_t* n_chunks);
+
+/* bmf_extract of the descriptors the last bmf_lod_queue left in the arena
+ * (BMF_ESTATE when there are none: a later bmf_extract or forward pass
+ * replaced them). */
+int bmf_extract_queued(bmf_ctx* ctx, int mode);
 
 /* ---- launch shapes ------------------------------------------------------ */
 /* Tile and workgroup shapes of the BATCHED path (the PLAIN path sizes its
@@ -295,6 +413,19 @@ int bmf_tape_positions(bmf_tape* tape, int32_t iteration, float* positions);
  * and L to host memory (may be NULL). */
 int bmf_loss_nearest(bmf_tape* tape, const float* targets, int64_t n_targets, float* dloss_dpos, int32_t* nearest,
                      float* loss);
+
+/* The same loss split in two for a fixed target set (an optimisation loop
+ * over the densities with the targets unchanged): bmf_tape_set_targets
+ * builds the search hierarchy of the n device targets [n][3] once (the tape
+ * keeps its own sorted copy; the caller's buffer may be reused after the
+ * call returns), and bmf_tape_loss_nearest then evaluates the loss of the
+ * tape's current forward against that set, with the outputs of
+ * bmf_loss_nearest.  The set stays until the next bmf_tape_set_targets or
+ * bmf_loss_nearest on the tape (which builds its own); forward passes keep
+ * it.  bmf_tape_loss_nearest returns BMF_ESTATE when no set was built. */
+int bmf_tape_set_targets(bmf_tape* tape, const float* targets, int64_t n_targets);
+int bmf_tape_loss_nearest(bmf_tape* tape, float* dloss_dpos, int32_t* nearest, float* loss);
+
 
 /* Backward pass: dL/d(density) [dim^3] (device) from dL/d(final positions)
  * [V][3] (device).  Samples that are not an endpoint of a sign-changing

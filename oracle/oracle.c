@@ -157,8 +157,8 @@ void or_fbm_params(const or_sampler_t* s, float* fo, float* ao, float* inv_norm)
     *inv_norm = 1.0f / norm;
 }
 
-static inline float terrain_f32(const or_sampler_t* s, const float* fo, const float* ao, float inv_norm,
-                                float x, float y, float z)
+/* The FBM of the terrain, normalised and clamped to [-1, 1]. */
+float or_fbm_f32(const or_sampler_t* s, const float* fo, const float* ao, float inv_norm, float x, float y, float z)
 {
     float S = 0.0f;
     for (int o = 0; o < s->octaves; o++) {
@@ -166,7 +166,13 @@ static inline float terrain_f32(const or_sampler_t* s, const float* fo, const fl
         S = fmaf(ao[o], n, S);
     }
     float f = S * inv_norm;
-    f = fminf(fmaxf(f, -1.0f), 1.0f);
+    return fminf(fmaxf(f, -1.0f), 1.0f);
+}
+
+static inline float terrain_f32(const or_sampler_t* s, const float* fo, const float* ao, float inv_norm,
+                                float x, float y, float z)
+{
+    const float f = or_fbm_f32(s, fo, ao, inv_norm, x, y, z);
     float t1 = s->amp * f;
     float t2 = y - s->y0;
     return t1 - t2;

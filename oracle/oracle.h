@@ -77,6 +77,8 @@ double or_snoise3_f64(double x, double y, double z, uint32_t seed);
 uint32_t or_grad_index(int32_t i, int32_t j, int32_t k, uint32_t seed);
 void or_fbm_params(const or_sampler_t* s, float* fo, float* ao, float* inv_norm);
 float or_terrain_value_f32(const or_sampler_t* s, float x, float y, float z);
+/* The terrain's FBM alone (normalised, clamped), with or_fbm_params' tables. */
+float or_fbm_f32(const or_sampler_t* s, const float* fo, const float* ao, float inv_norm, float x, float y, float z);
 double or_terrain_value_f64(const or_sampler_t* s, double x, double y, double z);
 void or_sample_block_f32(const or_sampler_t* s, int dim, const float opos[3], float delta, float* out);
 void or_sample_block_f64(const or_sampler_t* s, int dim, const double opos[3], double delta, double* out);
@@ -166,8 +168,54 @@ int or_smooth_normals_f32(float* pos, const uint8_t* boundary, int64_t V, const 
                           int boundary_processing, float* normals);
 int or_smooth_normals_f64(double* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int iters,
                           int boundary_processing, double* normals);
+/* N-gon (N = 3 or 4) smoothing with vertex normals: MeshProcessor<N>'s
+ * smooth_normals branch, quad face normals of MeshProcessor.cpp:187-205
+ * (oracle_dmc.c). */
+int or_smooth_normals_n_f32(float* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int N,
+                            int iters, int boundary_processing, float* normals);
+int or_smooth_normals_n_f64(double* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int N,
+                            int iters, int boundary_processing, double* normals);
 void or_dmc_positions_f64(const or_sampler_t* s, const double* dens, int dim, const double opos[3], double delta,
                           int placement, double tol, const uint32_t* code, int64_t V, double* pos, double* qerr);
+
+
+/* ---- output formatting and vertex colours (oracle_format.c) ---- */
+void or_hsv_to_rgb_f32(float h, float s, float v, float out[3]);
+float or_color_noise_f32(float x, float y, float z);
+void or_vertex_color_f32(const float p[3], float rgb[3]);
+void or_format_batch(int n, const int64_t* V, const int64_t* I, const int32_t* contains_mesh, const float* delta,
+                     const float* opos, const int64_t* voff, const int64_t* ioff, const float* pos,
+                     const float* nrm, const uint32_t* idx, int space, int colors, int global, float* out_pos,
+                     float* out_col, float* out_nrm, uint32_t* out_idx, int64_t* total_v, int64_t* total_i);
+
+
+/* ---- the watcher's LOD pass (oracle_lod.c); layouts of bmf_world_node,
+ * bmf_lod_params and bmf_lod_item ---- */
+typedef struct {
+    float pos[3];
+    float size;
+    int32_t level;
+    uint32_t flags;
+    int32_t world_leaf;
+    int32_t parent;
+    int32_t children[8];
+    uint64_t morton;
+} or_world_node_t;
+typedef struct {
+    float focus[3];
+    float split_multiplier, group_multiplier, size_modifier;
+    int32_t max_level, min_level, max_gen, chunk_resolution, process_iters, boundary_processing;
+    float overlap;
+} or_lod_params_t;
+typedef struct {
+    int32_t node, child, level, action;
+    uint64_t morton;
+} or_lod_item_t;
+int or_node_needs_split(const or_world_node_t* n, const or_lod_params_t* P);
+int or_node_needs_group(const or_world_node_t* n, const or_lod_params_t* P);
+int or_lod_queue(const or_world_node_t* nodes, int n_nodes, const int32_t* rlist, int n_r, const or_lod_params_t* P,
+                 uint32_t* flags, int32_t* dirty, or_lod_item_t* items, or_chunk_t* descs, int cap, int* n_dirty,
+                 int* n_chunks);
 
 #ifdef __cplusplus
 }

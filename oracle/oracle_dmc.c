@@ -411,3 +411,134 @@ int or_smooth_normals_f64(double* pos, const uint8_t* boundary, int64_t V, const
 {
     NORMALS_BODY(double, sqrt)
 }
+
+/* ------------------------------------------------------------------------ */
+/* Vertex normals of an N-gon mesh (N = 3 or 4): the smooth_normals branch  */
+/* of MeshProcessor<N> (MeshProcessor.cpp:159-216, 271-287).  Same schedule */
+/* as the triangle path above; the face normal of step m while              */
+/* m == 0 || m < max_norms || m < 3 is                                       */
+/*   N = 3: -cross(normalize(p0-p1), normalize(p0-p2));                     */
+/*   N = 4: n1 = cross(normalize(p0-p1), normalize(p0-p2)),                 */
+/*          n2 = cross(normalize(p2-p3), normalize(p2-p0)); a NaN n1.x takes */
+/*          n2 (and (0,1,0) when that is NaN too), then a NaN n2.x takes n1; */
+/*          the face normal is -normalize((n1 + n2) * 0.5)                  */
+/* (MeshProcessor.cpp:187-205); later steps take the sum of the N corner    */
+/* normals in corner order.  Centroids ((0 + p0) + ...)/N, rings in        */
+/* ascending primitive order.                                               */
+/* ------------------------------------------------------------------------ */
+#define CROSS_NORM(T, SQRT, out, u0, u1, u2, w0, w1, w2)                                        \
+    do {                                                                                        \
+        const T lu_ = SQRT((u0 * u0 + u1 * u1) + u2 * u2);                                      \
+        const T lw_ = SQRT((w0 * w0 + w1 * w1) + w2 * w2);                                      \
+        const T a0_ = u0 / lu_, a1_ = u1 / lu_, a2_ = u2 / lu_;                                  \
+        const T b0_ = w0 / lw_, b1_ = w1 / lw_, b2_ = w2 / lw_;                                  \
+        out[0] = a1_ * b2_ - b1_ * a2_;                                                         \
+        out[1] = a2_ * b0_ - b2_ * a0_;                                                         \
+        out[2] = a0_ * b1_ - b0_ * a1_;                                                         \
+    } while (0)
+
+#define NORMALS_N_BODY(T, SQRT)                                                                 \
+    if (V == 0 || nI < N || (N != 3 && N != 4)) return 0;                                       \
+    const int64_t nP = nI / N;                                                                  \
+    const int K = iters;                                                                        \
+    const int steps = K > 0 ? K : 1;                                                            \
+    const int max_norms = (K / 2 - 3 < 10) ? K / 2 - 3 : 10;                                    \
+    uint32_t* val = (uint32_t*)calloc((size_t)V + 1, sizeof(uint32_t));                         \
+    uint32_t* off = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(V + 1));                      \
+    uint32_t* nxt = (uint32_t*)calloc((size_t)V + 1, sizeof(uint32_t));                         \
+    uint32_t* adj = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(N * nP + 1));                 \
+    T* cen = (T*)malloc(sizeof(T) * (size_t)(3 * nP + 3));                                      \
+    T* fn = (T*)malloc(sizeof(T) * (size_t)(3 * nP + 3));                                       \
+    T* vn = (T*)calloc((size_t)(3 * V + 3), sizeof(T));                                         \
+    for (int64_t i = 0; i < N * nP; i++) val[idx[i]]++;                                         \
+    off[0] = 0;                                                                                 \
+    for (int64_t v = 0; v < V; v++) off[v + 1] = off[v] + val[v];                               \
+    for (int64_t t = 0; t < nP; t++)                                                            \
+        for (int k = 0; k < N; k++) {                                                           \
+            const uint32_t v = idx[N * t + k];                                                  \
+            adj[off[v] + nxt[v]++] = (uint32_t)t;                                               \
+        }                                                                                       \
+    for (int m = 0; m < steps; m++) {                                                           \
+        for (int64_t t = 0; t < nP; t++) {                                                      \
+            const T* p[4];                                                                      \
+            for (int c = 0; c < N; c++) p[c] = pos + 3 * idx[N * t + c];                        \
+            for (int k = 0; k < 3; k++) {                                                       \
+                T acc = (T)0;                                                                   \
+                for (int c = 0; c < N; c++) acc = acc + p[c][k];                                \
+                cen[3 * t + k] = acc / (T)N;                                                    \
+            }                                                                                   \
+            if (m == 0 || m < max_norms || m < 3) {                                             \
+                if (N == 3) {                                                                   \
+                    T f[3];                                                                     \
+                    CROSS_NORM(T, SQRT, f, (p[0][0] - p[1][0]), (p[0][1] - p[1][1]), (p[0][2] - p[1][2]), \
+                               (p[0][0] - p[2][0]), (p[0][1] - p[2][1]), (p[0][2] - p[2][2]));  \
+                    for (int k = 0; k < 3; k++) fn[3 * t + k] = -f[k];                          \
+                } else {                                                                        \
+                    T n1[3], n2[3];                                                             \
+                    CROSS_NORM(T, SQRT, n1, (p[0][0] - p[1][0]), (p[0][1] - p[1][1]), (p[0][2] - p[1][2]), \
+                               (p[0][0] - p[2][0]), (p[0][1] - p[2][1]), (p[0][2] - p[2][2]));  \
+                    CROSS_NORM(T, SQRT, n2, (p[2][0] - p[3][0]), (p[2][1] - p[3][1]), (p[2][2] - p[3][2]), \
+                               (p[2][0] - p[0][0]), (p[2][1] - p[0][1]), (p[2][2] - p[0][2]));  \
+                    if (isnan(n1[0])) {                                                         \
+                        for (int k = 0; k < 3; k++) n1[k] = n2[k];                              \
+                        if (isnan(n1[0])) {                                                     \
+                            n1[0] = (T)0;                                                       \
+                            n1[1] = (T)1;                                                       \
+                            n1[2] = (T)0;                                                       \
+                        }                                                                       \
+                    }                                                                           \
+                    if (isnan(n2[0]))                                                           \
+                        for (int k = 0; k < 3; k++) n2[k] = n1[k];                              \
+                    T h[3];                                                                     \
+                    for (int k = 0; k < 3; k++) h[k] = (n1[k] + n2[k]) * (T)0.5;               \
+                    const T lh = SQRT((h[0] * h[0] + h[1] * h[1]) + h[2] * h[2]);               \
+                    for (int k = 0; k < 3; k++) fn[3 * t + k] = -(h[k] / lh);                   \
+                }                                                                               \
+            } else {                                                                            \
+                for (int k = 0; k < 3; k++) {                                                   \
+                    T s = (T)0;                                                                 \
+                    for (int c = 0; c < N; c++) s = s + vn[3 * idx[N * t + c] + k];             \
+                    fn[3 * t + k] = s;                                                          \
+                }                                                                               \
+            }                                                                                   \
+        }                                                                                       \
+        const int last = m == steps - 1;                                                        \
+        const int unit = !last && (m == 3 || (m == 0 && K <= 3));                               \
+        for (int64_t v = 0; v < V; v++) {                                                       \
+            if (nxt[v] == 0) continue;                                                          \
+            T ap[3] = {(T)0, (T)0, (T)0}, an[3] = {(T)0, (T)0, (T)0};                           \
+            for (uint32_t q = off[v]; q < off[v] + nxt[v]; q++)                                 \
+                for (int k = 0; k < 3; k++) {                                                   \
+                    ap[k] = ap[k] + cen[3 * adj[q] + k];                                        \
+                    an[k] = an[k] + fn[3 * adj[q] + k];                                         \
+                }                                                                               \
+            for (int k = 0; k < 3; k++) an[k] = an[k] / (T)nxt[v];                              \
+            if (unit) {                                                                         \
+                const T l = SQRT((an[0] * an[0] + an[1] * an[1]) + an[2] * an[2]);              \
+                for (int k = 0; k < 3; k++) an[k] = an[k] / l;                                  \
+            }                                                                                   \
+            if (an[1] != (T)0)                                                                  \
+                for (int k = 0; k < 3; k++) vn[3 * v + k] = an[k];                              \
+            if (K > 0 && (boundary_processing || !boundary[v]))                                 \
+                for (int k = 0; k < 3; k++) pos[3 * v + k] = ap[k] / (T)nxt[v];                 \
+        }                                                                                       \
+    }                                                                                           \
+    for (int64_t v = 0; v < V; v++) {                                                           \
+        const T* n = vn + 3 * v;                                                                \
+        const T l = SQRT((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);                            \
+        for (int k = 0; k < 3; k++) normals[3 * v + k] = l > (T)0 ? n[k] / l : (T)0;            \
+    }                                                                                           \
+    free(val); free(off); free(nxt); free(adj); free(cen); free(fn); free(vn);                  \
+    return 0;
+
+int or_smooth_normals_n_f32(float* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int N,
+                            int iters, int boundary_processing, float* normals)
+{
+    NORMALS_N_BODY(float, sqrtf)
+}
+
+int or_smooth_normals_n_f64(double* pos, const uint8_t* boundary, int64_t V, const uint32_t* idx, int64_t nI, int N,
+                            int iters, int boundary_processing, double* normals)
+{
+    NORMALS_N_BODY(double, sqrt)
+}

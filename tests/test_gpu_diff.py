@@ -227,3 +227,25 @@ def test_tape_error_paths(ctx, tape):
         with pytest.raises(capi.BmfError) as e:   # mesh larger than the tape's capacity
             t2.forward(capi.chunk_desc((0.0, 16.0, 0.0), 32.0, 33, 0.0, 2, 0))
         assert e.value.code == capi.ECAPACITY
+
+
+def test_cached_target_set_matches_per_call_build(ctx, oracle):
+    """bmf_tape_set_targets + bmf_tape_loss_nearest give bmf_loss_nearest's
+    outputs bit for bit, and the set survives later forward passes."""
+    with capi.Tape(ctx, max_dim=65, max_vertices=1 << 18, max_indices=1 << 20, checkpoint_every=2) as t:
+        c = workloads.Chunk((0.0, 0.0, 0.0), 64.0, 65, 0.0, 4, 0)
+        t.forward(capi.chunk_desc(c.pos, c.size, c.dim, c.overlap, c.iters, c.boundary_processing))
+        with pytest.raises(capi.BmfError) as e:
+            t.loss_nearest_set()
+        assert e.value.code == capi.ESTATE
+        tgt = capi.DeviceArray.from_numpy(workloads.sphere_targets(n=50_000, seed=3, radius=20.0,
+                                                                   centre=(32.0, 32.0, 32.0)))
+        t.set_targets(tgt)
+        for K in (4, 2):
+            t.forward(capi.chunk_desc(c.pos, c.size, c.dim, c.overlap, K, c.boundary_processing))
+            L1, g1, n1 = t.loss_nearest_set()
+            L0, g0, n0 = t.loss_nearest(tgt)
+            V = t.info().vertex_count
+            assert L1 == L0
+            assert np.array_equal(n1, n0)
+            assert np.array_equal(g1.numpy()[:V], g0.numpy()[:V])

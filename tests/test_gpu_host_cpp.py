@@ -56,3 +56,31 @@ def test_chunk_generator_smooth_normals(oracle):
                 p, n = oracle.smooth_normals(base["pos"], base["boundary"], base["idx"], iters)
                 assert r["checksum"] == oracle.checksum_mesh(p, base["idx"])
                 assert r["normals"] == oracle.checksum_mesh(n, np.zeros(0, np.uint32))
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["plain", "batched"])
+def test_overlap_rule_below_max_level(oracle, mode):
+    """Nodes one level above max_level take overlap 0.035 + 0.005 K
+    (ChunkGenerator.cpp:98); the chunks' meshes equal the oracle's with that
+    overlap (sample grid shifted and stretched, DMCChunk.cpp:94-101)."""
+    n_side, res, iters, level = 2, 33, 3, 6
+    out = subprocess.run([str(build.HOST_DEMO), str(n_side), str(res), str(iters), str(mode), "0", str(level)],
+                         check=True, capture_output=True, text=True, timeout=120).stdout
+    rows = [json.loads(line) for line in out.splitlines() if line.startswith("{")]
+    spec = oracle.SamplerSpec(**workloads.terrain_sampler())
+    size = float(res - 1)
+    overlap = float(np.float32(0.035) + np.float32(0.005) * np.float32(iters))
+    c = 0
+    nonempty = 0
+    for i in range(n_side):
+        for j in range(n_side):
+            for k in range(n_side):
+                ch = oracle.make_chunk((size * i, size * j, size * k), size, res, overlap, iters, 0)
+                ref = oracle.extract_chunk(spec, ch)
+                r = rows[c]
+                c += 1
+                assert r["contains_mesh"] == int(ref["contains_mesh"])
+                assert r["V"] == ref["V"] and r["I"] == ref["nI"]
+                assert r["checksum"] == oracle.checksum_mesh(ref["pos"], ref["idx"])
+                nonempty += r["V"] > 0
+    assert nonempty > 0

@@ -1,8 +1,9 @@
 """Vertex normals of the triangle path on the GPU against the CPU oracle.
 
 Bar: normals bit-exact with the FP32 oracle (same operation sequence),
-positions identical to the run without normals; the FP32 normals' error
-against the FP64 twin is reported.
+positions identical to the run without normals, in triangle and quad mode;
+the FP32 normals' error against the FP64 pipeline is reported (and bounded
+loosely in quad mode).
 """
 import numpy as np
 import pytest
@@ -52,7 +53,64 @@ def test_normals_match_oracle(ctx, oracle, mode):
         ctx.set_mesh_options(normals=False)
 
 
-def test_normals_rejected_in_quad_mode(ctx):
-    with pytest.raises(capi.BmfError) as e:
-        ctx.set_mesh_options(mesh=capi.MESH_QUADS, normals=True)
-    assert e.value.code == capi.EINVAL
+@pytest.mark.parametrize("mode", [capi.MODE_PLAIN, capi.MODE_BATCHED], ids=["plain", "batched"])
+@pytest.mark.parametrize("as_tris", [False, True], ids=["quads", "tris"])
+def test_quad_normals_match_oracle(ctx, oracle, mode, as_tris):
+    """Quad mode with normals (MeshProcessor<4>'s smooth_normals branch, or
+    <3> on flush_to_tris triangles): positions and normals bit-exact with the
+    FP32 oracle, positions identical to the run without normals; the FP32
+    normals' error against the full FP64 pipeline is printed."""
+    from oracle import fp64_errors as fe
+
+    chunks = [workloads.Chunk((0.0, 0.0, 0.0), 64.0, 65, 0.0, K, bp) for K, bp in ((0, 0), (3, 0), (4, 1), (8, 0))]
+    chunks.append(workloads.Chunk((64.0, 0.0, 0.0), 32.0, 33, 0.0, 2, 0))
+    cd = workloads.to_capi(chunks)
+    ctx.set_mesh_options(mesh=capi.MESH_QUADS, quads_as_triangles=as_tris)
+    try:
+        ctx.extract(cd, mode)
+        capi.check(ctx.results(len(cd))[0], "bmf_result")
+        plain_pos = [ctx.mesh(j)[0] for j in range(len(cd))]
+        ctx.set_mesh_options(mesh=capi.MESH_QUADS, quads_as_triangles=as_tris, normals=True)
+        ctx.extract(cd, mode)
+        rc, res, _, _ = ctx.results(len(cd))
+        capi.check(rc, "bmf_result")
+        spec = oracle.SamplerSpec(**TERRAIN)
+        N = 3 if as_tris else 4
+        worst64 = 0.0
+        for j, c in enumerate(chunks):
+            pos, idx, bnd = ctx.mesh(j)
+            nrm = ctx.normals(j)
+            assert np.array_equal(pos, plain_pos[j])
+            oc = oracle.make_chunk(c.pos, c.size, c.dim, c.overlap, 0, c.boundary_processing)
+            delta, opos = oracle.geometry(oc)
+            m = oracle.dmc_mesh(oracle.sample_block(spec, c.dim, opos, delta), spec, opos, delta,
+                                as_triangles=as_tris)
+            assert np.array_equal(idx, m["idx"])
+            p32, n32 = oracle.smooth_normals_n(m["pos"], m["boundary"], m["idx"], N, c.iters, c.boundary_processing)
+            assert np.array_equal(pos, p32), (c, np.abs(pos - p32).max())
+            assert np.array_equal(nrm, n32), (c, np.abs(nrm - n32).max())
+            _, en = fe.quad_positions(spec, c, pos, as_triangles=as_tris, want_normals=True, gpu_nrm=nrm)
+            worst64 = max(worst64, float(np.quantile(en, 0.99)))
+        print(f"quad normals (N={N}): p99 |FP32 - FP64| over chunks (worst) {worst64:.3g}")
+        assert worst64 < 1e-3
+    finally:
+        ctx.set_mesh_options()
+
+
+def test_quad_normals_sphere_radial(ctx):
+    """A sphere in quad mode: every vertex with a quad gets a unit normal
+    pointing away from the centre (toward the air side)."""
+    with capi.Context(max_chunks=2, max_samples=2 * 25 ** 3, max_vertices=1 << 16, max_indices=1 << 18) as c2:
+        c2.set_sampler(kind=capi.SAMPLER_SPHERE, world_size=80.0)
+        c2.set_mesh_options(mesh=capi.MESH_QUADS, normals=True)
+        c2.extract(workloads.to_capi([workloads.Chunk((-12.3, -12.3, -12.3), 24.0, 25, 0.0, 4, 0)]), capi.MODE_BATCHED)
+        capi.check(c2.results(1)[0], "bmf_result")
+        pos, idx, _ = c2.mesh(0)
+        nrm = c2.normals(0)
+        ring = np.zeros(len(pos), bool)
+        ring[idx] = True
+        r = pos[ring].astype(np.float64) - 12.3
+        r /= np.linalg.norm(r, axis=1, keepdims=True)
+        cosang = np.sum(r * nrm[ring], 1)
+        assert np.quantile(cosang, 0.01) > 0.95 and cosang.min() > 0.8
+        assert np.all(nrm[~ring] == 0.0)

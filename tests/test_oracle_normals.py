@@ -94,3 +94,130 @@ def test_normal_schedule_matches_numpy(oracle, K):
     _, n64 = oracle.smooth_normals(m["pos"], m["boundary"], m["idx"], K, f64=True)
     ref = _numpy_normals(m["pos"], m["boundary"], m["idx"], K, 0)
     assert np.abs(n64 - ref).max() < 1e-9
+
+
+# ---- quad mode (MeshProcessor<4>: face normal of MeshProcessor.cpp:187-205) -----------------
+def _dmc(oracle, spec, pos, size, dim, as_triangles=False):
+    oc = oracle.make_chunk(pos, size, dim, 0.0, 0, 0)
+    delta, opos = oracle.geometry(oc)
+    return oracle.dmc_mesh(oracle.sample_block(spec, dim, opos, delta), spec, opos, delta,
+                           as_triangles=as_triangles)
+
+
+@pytest.mark.parametrize("K", [0, 1, 4, 9])
+def test_quad_plane_normals_are_exact(oracle, K):
+    spec = oracle.SamplerSpec(kind=oracle.SAMPLER_PLANE_Y, world_size=64.0)
+    m = _dmc(oracle, spec, (-8.0, -8.3, -8.0), 16.0, 17)
+    assert m["V"] > 0 and m["nI"] > 0
+    p, n = oracle.smooth_normals_n(m["pos"], m["boundary"], m["idx"], 4, K)
+    ring = np.zeros(m["V"], bool)
+    ring[m["idx"]] = True
+    assert np.array_equal(n[ring], np.tile(np.float32([0.0, 1.0, 0.0]), (ring.sum(), 1)))
+    assert np.all(n[~ring] == 0.0)
+
+
+@pytest.mark.parametrize("K", [0, 2, 4, 8])
+def test_quad_sphere_normals_are_radial(oracle, K):
+    spec = oracle.SamplerSpec(kind=oracle.SAMPLER_SPHERE, world_size=80.0)
+    m = _dmc(oracle, spec, (-12.3, -12.3, -12.3), 24.0, 25)
+    p, n = oracle.smooth_normals_n(m["pos"], m["boundary"], m["idx"], 4, K)
+    ring = np.zeros(m["V"], bool)
+    ring[m["idx"]] = True
+    r = p[ring].astype(np.float64) - 12.3
+    r /= np.linalg.norm(r, axis=1, keepdims=True)
+    cosang = np.sum(r * n[ring], 1)
+    assert np.all(np.abs(np.linalg.norm(n[ring], axis=1) - 1.0) < 1e-6)
+    assert np.quantile(cosang, 0.01) > 0.95 and cosang.min() > 0.8
+
+
+@pytest.mark.parametrize("K,bp", [(0, 0), (3, 0), (4, 1), (8, 0)])
+def test_n3_normals_are_the_triangle_paths(oracle, K, bp):
+    """N = 3 of the N-gon routine is the triangle path's routine, bit for bit."""
+    spec = oracle.SamplerSpec(**workloads.terrain_sampler())
+    m = _mc(oracle, spec, (0.0, 16.0, 0.0), 32.0, 33, bp=bp)
+    pa, na = oracle.smooth_normals(m["pos"], m["boundary"], m["idx"], K, bp)
+    pb, nb = oracle.smooth_normals_n(m["pos"], m["boundary"], m["idx"], 3, K, bp)
+    assert np.array_equal(pa, pb) and np.array_equal(na, nb)
+
+
+@pytest.mark.parametrize("quad,want", [
+    # a unit square in y = 0, wound so that both corner normals point down: the face normal is +y
+    ([(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)], (0.0, 1.0, 0.0)),
+    # p0 == p1: n1 is NaN and takes n2 = cross((0,0,-1), (1,0,0)) = (0,-1,0)
+    ([(0, 0, 0), (0, 0, 0), (1, 0, 0), (1, 0, 1)], (0.0, 1.0, 0.0)),
+    # all corners equal: n1 and n2 are NaN, n1 falls back to (0,1,0) and n2 to n1
+    ([(2, 3, 4)] * 4, (0.0, -1.0, 0.0)),
+    # p2 == p3: n2 is NaN and takes n1 = cross((-1,0,0), (-1,0,-1)/sqrt2) = (0,-1/sqrt2,0)
+    ([(0, 0, 0), (1, 0, 0), (1, 0, 1), (1, 0, 1)], (0.0, 1.0, 0.0)),
+])
+def test_quad_face_normal_known_answers(oracle, quad, want):
+    pos = np.array(quad, np.float32)
+    idx = np.arange(4, dtype=np.uint32)
+    bnd = np.zeros(4, np.uint8)
+    for f64 in (False, True):
+        p, n = oracle.smooth_normals_n(pos, bnd, idx, 4, 0, f64=f64)
+        assert np.array_equal(p, pos)  # K = 0 moves nothing
+        assert np.array_equal(n, np.tile(np.array(want, n.dtype), (4, 1))), (f64, n)
+
+
+def _numpy_quad_normals(pos, bnd, idx, K, bp):
+    """Direct restatement in float64 of the quad normal schedule."""
+    p = pos.astype(np.float64).copy()
+    q = idx.reshape(-1, 4)
+    adj = [[] for _ in range(len(p))]
+    for i, quad in enumerate(q):
+        for v in quad:
+            adj[v].append(i)
+    vn = np.zeros_like(p)
+    steps = max(K, 1)
+    max_norms = min(K // 2 - 3, 10)
+
+    def unit(v):
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    for m in range(steps):
+        c = [p[q[:, k]] for k in range(4)]
+        cen = (c[0] + c[1] + c[2] + c[3]) / 4.0
+        if m == 0 or m < max_norms or m < 3:
+            n1 = np.cross(unit(c[0] - c[1]), unit(c[0] - c[2]))
+            n2 = np.cross(unit(c[2] - c[3]), unit(c[2] - c[0]))
+            fn = -unit((n1 + n2) * 0.5)
+        else:
+            fn = vn[q[:, 0]] + vn[q[:, 1]] + vn[q[:, 2]] + vn[q[:, 3]]
+        last = m == steps - 1
+        unit_step = (not last) and (m == 3 or (m == 0 and K <= 3))
+        newp = p.copy()
+        for v in range(len(p)):
+            if not adj[v]:
+                continue
+            an = fn[adj[v]].sum(0) / len(adj[v])
+            if unit_step:
+                an = an / np.linalg.norm(an)
+            if an[1] != 0:
+                vn[v] = an
+            if K > 0 and (bp or not bnd[v]):
+                newp[v] = cen[adj[v]].sum(0) / len(adj[v])
+        p = newp
+    l = np.linalg.norm(vn, axis=1, keepdims=True)
+    return np.divide(vn, l, out=np.zeros_like(vn), where=l > 0)
+
+
+@pytest.mark.parametrize("K", [1, 3, 4, 10])
+def test_quad_normal_schedule_matches_numpy(oracle, K):
+    spec = oracle.SamplerSpec(**workloads.terrain_sampler())
+    m = _dmc(oracle, spec, (0.0, 24.0, 0.0), 12.0, 13)
+    _, n64 = oracle.smooth_normals_n(m["pos"], m["boundary"], m["idx"], 4, K, f64=True)
+    ref = _numpy_quad_normals(m["pos"], m["boundary"], m["idx"], K, 0)
+    assert np.abs(n64 - ref).max() < 1e-9
+
+
+@pytest.mark.parametrize("K,bp", [(0, 0), (4, 0), (4, 1), (8, 0)])
+def test_quad_normals_fp32_vs_fp64(oracle, K, bp):
+    spec = oracle.SamplerSpec(**workloads.terrain_sampler())
+    m = _dmc(oracle, spec, (0.0, 16.0, 0.0), 32.0, 33)
+    p32, n32 = oracle.smooth_normals_n(m["pos"], m["boundary"], m["idx"], 4, K, bp)
+    p64, n64 = oracle.smooth_normals_n(m["pos"], m["boundary"], m["idx"], 4, K, bp, f64=True)
+    assert np.array_equal(p32, oracle.smooth_n(m["pos"], m["boundary"], m["idx"], 4, K, bp) if K else m["pos"])
+    err = np.abs(n32 - n64).max(1)
+    print(f"K={K} bp={bp}: quad |n32 - n64| max {err.max():.3g}, p99 {np.quantile(err, 0.99):.3g}")
+    assert np.quantile(err, 0.99) < 1e-5 and err.max() < 1e-2
