@@ -23,10 +23,11 @@ EXPORTS = [
     "bmf_tape_destroy", "bmf_forward_diff", "bmf_tape_get", "bmf_tape_positions", "bmf_loss_nearest",
     "bmf_backward_diff", "bmf_get_tuning", "bmf_set_tuning", "bmf_autotune", "bmf_set_mesh_options",
     "bmf_get_mesh_options", "bmf_qef_error_to_host", "bmf_normals_to_host", "bmf_tape_set_targets",
-    "bmf_tape_loss_nearest", "bmf_format", "bmf_lod_queue", "bmf_extract_queued",
+    "bmf_tape_loss_nearest", "bmf_loss_nearest_noalloc", "bmf_format", "bmf_lod_queue", "bmf_extract_queued",
 ]
 SPACE_GRID, SPACE_WORLD = 0, 1
 COLOR_NONE, COLOR_WHITE, COLOR_NOISE = 0, 1, 2
+NOALLOC_MAX_TARGETS = 262144
 MESH_TRIANGLES, MESH_QUADS = 0, 1
 PLACE_MASS_POINT, PLACE_QEF = 0, 1
 NUM_STAGES = 7
@@ -153,6 +154,7 @@ def lib():
         L.bmf_backward_diff.argtypes = [tp, C.c_void_p, C.c_void_p]
         L.bmf_tape_set_targets.argtypes = [tp, C.c_void_p, C.c_int64]
         L.bmf_tape_loss_nearest.argtypes = [tp, C.c_void_p, C.c_void_p, P(C.c_float)]
+        L.bmf_loss_nearest_noalloc.argtypes = [tp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
         L.bmf_format.argtypes = [ctxp, P(FormatOptions), P(Vbo), P(C.c_int64), P(C.c_int64)]
         L.bmf_lod_queue.argtypes = [ctxp, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, P(LodParams), C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32)]

@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "../../include/bmf.h"
@@ -744,6 +745,70 @@ int smooth_steps(bmf_tape* t, float4* state, int k, hipStream_t s, int ckpt_from
     return BMF_OK;
 }
 
+// ---- scratch of bmf_loss_nearest_noalloc ---------------------------------
+// The hierarchy of up to BMF_NOALLOC_MAX_TARGETS targets lives in one
+// file-scope device array, so that entry point allocates nothing: Morton
+// keys and target ids (2 x 2 x 1 MiB), the sorted targets (4 MiB), the boxes
+// of 2 x 32768 heap nodes (2 x 1 MiB) and the radix sort's temporary
+// storage (2 MiB), 12 MiB per GPU for the life of the process.  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in,
+// not a device address): each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr int64_t kNAMax = BMF_NOALLOC_MAX_TARGETS;
+constexpr int64_t kNALeaves = kNAMax / kLeaf;
+static_assert(kNAMax % kLeaf == 0 && (kNALeaves & (kNALeaves - 1)) == 0, "leaves of the maximum set: a power of two");
+constexpr size_t kNAKeyBytes = sizeof(uint32_t) * kNAMax;
+constexpr size_t kNATgtBytes = sizeof(float4) * kNAMax;
+constexpr size_t kNABoxBytes = sizeof(float4) * 2 * kNALeaves;
+constexpr size_t kNACubBytes = 2u << 20;
+constexpr size_t kScratchBytes = 4 * kNAKeyBytes + kNATgtBytes + 2 * kNABoxBytes + kNACubBytes;
+static_assert(kNAKeyBytes % 256 == 0 && kNATgtBytes % 256 == 0 && kNABoxBytes % 256 == 0, "256-byte aligned parts");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// on the first call for that device.
+int scratch_acquire(int device) {
+    if (device < 0 || device >= kMaxDevices) return BMF_EINVAL;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return BMF_OK;
+    void* ptr = nullptr;
+    BMF_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return BMF_EHIP;
+    g_scratch_dev[device] = ptr;
+    return BMF_OK;
+}
+
+void* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+
+NNBuffers scratch_nn(void* base) {
+    unsigned char* p = static_cast<unsigned char*>(base);
+    NNBuffers b;
+    b.keys[0] = reinterpret_cast<uint32_t*>(p);
+    b.keys[1] = reinterpret_cast<uint32_t*>(p + kNAKeyBytes);
+    b.vals[0] = reinterpret_cast<int32_t*>(p + 2 * kNAKeyBytes);
+    b.vals[1] = reinterpret_cast<int32_t*>(p + 3 * kNAKeyBytes);
+    p += 4 * kNAKeyBytes;
+    b.st = reinterpret_cast<float4*>(p);
+    p += kNATgtBytes;
+    b.blo = reinterpret_cast<float4*>(p);
+    b.bhi = reinterpret_cast<float4*>(p + kNABoxBytes);
+    p += 2 * kNABoxBytes;
+    b.cub_tmp = p;
+    b.cub_bytes = kNACubBytes;
+    return b;
+}
+
+__global__ void k_store_f64(double* __restrict__ out, double v) { out[0] = v; }
+
 }  // namespace
 
 extern "C" {
@@ -778,6 +843,9 @@ int bmf_tape_create(bmf_tape** out, bmf_ctx* ctx, int32_t max_dim, int64_t max_v
         bmf_tape_destroy(t);
         return BMF_EHIP;
     }
+    // set-up of bmf_loss_nearest_noalloc: the scratch address on this device
+    // (the device is current); a failure here only disables that entry point
+    (void)scratch_acquire(ctx->device);
     *out = t;
     return BMF_OK;
 }
@@ -1007,6 +1075,35 @@ int bmf_tape_loss_nearest(bmf_tape* t, float* dldp, int32_t* nearest, float* los
     BMF_HIP(hipStreamSynchronize(s));
     if (loss) *loss = (float)l;
     return BMF_OK;
+}
+
+int bmf_loss_nearest_noalloc(bmf_tape* t, const float* targets, int64_t n, float* dldp, int32_t* nearest,
+                             double* loss) {
+    if (!t || !targets || n < 1 || !dldp) return BMF_EINVAL;
+    if (n > BMF_NOALLOC_MAX_TARGETS) return BMF_ECAPACITY;
+    if (!t->valid) return BMF_ESTATE;
+    void* base = scratch_on(t->ctx->device);
+    if (!base) return BMF_EHIP;  // the address lookup at bmf_tape_create failed
+    BMF_HIP(hipSetDevice(t->ctx->device));
+    hipStream_t s = t->ctx->stream;
+    const NNBuffers b = scratch_nn(base);
+    // the radix sort's temporary storage for n pairs must fit its share (a
+    // size query: nothing is launched or allocated)
+    size_t need = 0;
+    {
+        hipcub::DoubleBuffer<uint32_t> kb(b.keys[0], b.keys[1]);
+        hipcub::DoubleBuffer<int32_t> vb(b.vals[0], b.vals[1]);
+        BMF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, kb, vb, (int)n, 0, 30, s));
+    }
+    if (need > b.cub_bytes) return BMF_ECAPACITY;
+    int rc = nn_build(targets, n, t->bb, b, s);
+    if (rc) return rc;
+    if (t->V == 0) {
+        if (loss) hipLaunchKernelGGL(k_store_f64, dim3(1), dim3(1), 0, s, loss, 0.0);
+        BMF_HIP(hipGetLastError());
+        return BMF_OK;
+    }
+    return nn_query(t, b, n, dldp, nearest, loss ? loss : t->loss, s);
 }
 
 int bmf_backward_diff(bmf_tape* t, const float* dldp, float* dldd) {

@@ -426,6 +426,22 @@ int bmf_loss_nearest(bmf_tape* tape, const float* targets, int64_t n_targets, fl
 int bmf_tape_set_targets(bmf_tape* tape, const float* targets, int64_t n_targets);
 int bmf_tape_loss_nearest(bmf_tape* tape, float* dloss_dpos, int32_t* nearest, float* loss);
 
+/* bmf_loss_nearest for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight optimisation loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The search hierarchy is
+ * built in a static device array of the library (12 MiB per GPU, held until
+ * the process ends; its address is looked up once per device by
+ * bmf_tape_create) instead of the tape's growable buffers, so the call
+ * allocates nothing; n_targets above BMF_NOALLOC_MAX_TARGETS returns
+ * BMF_ECAPACITY.  Same outputs as bmf_loss_nearest bit for bit, except that
+ * the FP64 loss goes to DEVICE memory `loss` (may be NULL) and nothing is
+ * synchronised: the work is enqueued on the context stream, and the targets
+ * must stay valid until it completes.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (tapes of contexts with
+ * different streams, without ordering between them) race. */
+#define BMF_NOALLOC_MAX_TARGETS 262144
+int bmf_loss_nearest_noalloc(bmf_tape* tape, const float* targets, int64_t n_targets, float* dloss_dpos,
+                             int32_t* nearest, double* loss);
 
 /* Backward pass: dL/d(density) [dim^3] (device) from dL/d(final positions)
  * [V][3] (device).  Samples that are not an endpoint of a sign-changing
