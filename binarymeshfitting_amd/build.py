@@ -12,7 +12,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libbmf.so"
-SOURCES = ["bmf_kernels.hip", "bmf_capi.hip", "bmf_diff.hip", "bmf_dmc.hip", "bmf_format.hip", "bmf_lod.hip"]
+SOURCES = ["bmf_kernels.hip", "bmf_capi.hip", "bmf_diff.hip", "bmf_dmc.hip", "bmf_format.hip", "bmf_lod.hip", "bmf_stitch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

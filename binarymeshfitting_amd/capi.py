@@ -23,7 +23,7 @@ EXPORTS = [
     "bmf_tape_destroy", "bmf_forward_diff", "bmf_tape_get", "bmf_tape_positions", "bmf_loss_nearest",
     "bmf_backward_diff", "bmf_get_tuning", "bmf_set_tuning", "bmf_autotune", "bmf_set_mesh_options",
     "bmf_get_mesh_options", "bmf_qef_error_to_host", "bmf_normals_to_host", "bmf_tape_set_targets",
-    "bmf_tape_loss_nearest", "bmf_loss_nearest_noalloc", "bmf_format", "bmf_lod_queue", "bmf_extract_queued",
+    "bmf_tape_loss_nearest", "bmf_loss_nearest_noalloc", "bmf_format", "bmf_lod_queue", "bmf_extract_queued", "bmf_stitch",
 ]
 SPACE_GRID, SPACE_WORLD = 0, 1
 COLOR_NONE, COLOR_WHITE, COLOR_NOISE = 0, 1, 2
@@ -159,6 +159,8 @@ def lib():
         L.bmf_lod_queue.argtypes = [ctxp, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, P(LodParams), C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32)]
         L.bmf_extract_queued.argtypes = [ctxp, C.c_int]
+        L.bmf_stitch.argtypes = [ctxp, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, P(C.c_int32), C.c_void_p,
+                                 C.c_int64, P(C.c_int64)]
         _lib = L
     return _lib
 
@@ -334,6 +336,20 @@ class Context:
         if rc != OK:
             return rc, None, None, None, nc.value
         return rc, fl.numpy()[: len(nodes)], di.numpy()[: nd.value], it.numpy()[: nc.value], nc.value
+
+    def stitch(self, nodes, leaves, chunk_resolution, cap_triangles=1 << 20):
+        """bmf_stitch on host arrays (workloads.WORLD_NODE nodes, int32 leaf list).
+        Returns (rc, triangles [T][3][3] numpy or None, T)."""
+        dn = DeviceArray.from_numpy(np.ascontiguousarray(nodes))
+        dl = DeviceArray.from_numpy(np.ascontiguousarray(leaves, np.int32))
+        out = DeviceArray((max(cap_triangles, 1), 3, 3), np.float32)
+        nt = C.c_int64()
+        res = C.c_int32(chunk_resolution)
+        rc = lib().bmf_stitch(self.h, dn.ptr, len(nodes), dl.ptr, len(leaves), C.byref(res), out.ptr, cap_triangles,
+                              C.byref(nt))
+        if rc != OK:
+            return rc, None, nt.value
+        return rc, out.numpy()[: nt.value], nt.value
 
     def extract_queued(self, mode=MODE_BATCHED):
         check(lib().bmf_extract_queued(self.h, mode), "bmf_extract_queued")

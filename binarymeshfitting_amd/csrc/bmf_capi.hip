@@ -332,6 +332,10 @@ void bmf_destroy(bmf_ctx* c) {
     if (c->d_lod_code) (void)hipFree(c->d_lod_code);
     if (c->d_lod_counts) (void)hipFree(c->d_lod_counts);
     if (c->h_lod_counts) (void)hipHostFree(c->h_lod_counts);
+    if (c->d_stitch_counts) (void)hipFree(c->d_stitch_counts);
+    if (c->d_stitch_tiles) (void)hipFree(c->d_stitch_tiles);
+    if (c->d_stitch_misc) (void)hipFree(c->d_stitch_misc);
+    if (c->h_stitch_total) (void)hipHostFree(c->h_stitch_total);
     hipEvent_t evs[] = {c->ev0, c->ev1, c->ev_upload[0], c->ev_upload[1], c->ev_fork, c->ev_join[0], c->ev_join[1],
                         c->ev_join[2]};
     for (hipEvent_t e : evs)

@@ -58,6 +58,13 @@ struct bmf_ctx {
     int* d_lod_counts = nullptr;   // dirty nodes, chunks
     int* h_lod_counts = nullptr;   // pinned
     int queued_n = -1, queued_dim = 0, queued_iters = 0;
+    // bmf_stitch scratch: triangles per candidate point, tile totals / offsets,
+    // (root, deepest level) and the total
+    uint8_t* d_stitch_counts = nullptr;
+    int64_t* d_stitch_tiles = nullptr;
+    int64_t* d_stitch_misc = nullptr;
+    int64_t* h_stitch_total = nullptr;  // pinned
+    int64_t stitch_cap_cand = 0;
 };
 
 namespace bmf {

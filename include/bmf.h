@@ -339,6 +339,38 @@ int bmf_lod_queue(bmf_ctx* ctx, const bmf_world_node* nodes, int32_t n_nodes, co
  * replaced them). */
 int bmf_extract_queued(bmf_ctx* ctx, int mode);
 
+/* ---- seams between chunks ---------------------------------------------- */
+/* The stitcher: marching-cubes triangles of the dual cells that straddle the
+ * boundaries between the chunks of a world octree, the output of
+ * WorldStitcher::stitch_all (WorldStitcher.cpp:26-49, stitch_cell :184-239,
+ * the face / edge recursions :241-460, stitch_indexes :491-572) over the
+ * chunks' boundary-shell octrees (DMCChunk::generate_octree,
+ * DMCChunk.cpp:700-778).  Every world leaf is a chunk of
+ * chunk_resolution^3 unit nodes (a power of two) placed as the shell octree
+ * places them (pos + MCD * size/2 + MCD * size/4 + ..., FP32 in that order);
+ * a node's value is the context's sampler at that position (the reference's
+ * read of the chunk's densities is commented out there, DMCChunk.cpp:755,
+ * which leaves every value 0 and the seams empty).  The dual cells are those
+ * of the reference's recursion (around every point where the octant
+ * neighbourhoods fall in more than one chunk), enumerated point by point;
+ * triangles are written in a fixed order: the chunks of `leaves` in order,
+ * then the lattice points of each chunk's closed cube on its boundary in
+ * x-major order (each point by the finest of its chunks, the lowest node
+ * index among equals), then the triangle table's order.  Chunk meshes
+ * extracted on the same lattice (bmf_chunk_desc.size = node size *
+ * (dim - 1) / dim) close against the seams.  `nodes` / `leaves` (every world
+ * leaf, in the output order) / `triangles` ([cap_triangles][3][3] FP32, a
+ * triangle soup) are device memory; nodes need correct level and morton
+ * fields.  Waits; BMF_ECAPACITY (nothing written) when the count exceeds
+ * cap_triangles.  The EXTERNAL sampler has no values off the chunks' sample
+ * grids and is rejected. */
+typedef struct {
+    int32_t chunk_resolution;  /* unit nodes per chunk edge: a power of two, 2..256 */
+} bmf_stitch_params;
+
+int bmf_stitch(bmf_ctx* ctx, const bmf_world_node* nodes, int32_t n_nodes, const int32_t* leaves, int32_t n_leaves,
+               const bmf_stitch_params* params, float* triangles, int64_t cap_triangles, int64_t* n_triangles);
+
 /* ---- launch shapes ------------------------------------------------------ */
 /* Tile and workgroup shapes of the BATCHED path (the PLAIN path sizes its
  * launches per chunk).  A field left 0 keeps its current value. */

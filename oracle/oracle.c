@@ -807,3 +807,11 @@ double or_qef_solve_f64(const double* p, const double* n, int count, double tol,
 {
     QEF_BODY(double, sqrt)
 }
+
+/* The sampler's FP32 value at one point (every octave of the terrain). */
+float or_sample_value_f32(const or_sampler_t* s, float x, float y, float z)
+{
+    if (s->kind == OR_SAMPLER_TERRAIN) return or_terrain_value_f32(s, x, y, z);
+    if (s->kind == OR_SAMPLER_CSG) return or_csg_value_f32(s, x, y, z, NULL);
+    return implicit_f32(s->kind, s->world_size, x, y, z);
+}

@@ -77,6 +77,8 @@ double or_snoise3_f64(double x, double y, double z, uint32_t seed);
 uint32_t or_grad_index(int32_t i, int32_t j, int32_t k, uint32_t seed);
 void or_fbm_params(const or_sampler_t* s, float* fo, float* ao, float* inv_norm);
 float or_terrain_value_f32(const or_sampler_t* s, float x, float y, float z);
+/* The sampler's FP32 value at one point (every octave of the terrain). */
+float or_sample_value_f32(const or_sampler_t* s, float x, float y, float z);
 /* The terrain's FBM alone (normalised, clamped), with or_fbm_params' tables. */
 float or_fbm_f32(const or_sampler_t* s, const float* fo, const float* ao, float inv_norm, float x, float y, float z);
 double or_terrain_value_f64(const or_sampler_t* s, double x, double y, double z);
@@ -216,6 +218,13 @@ int or_node_needs_group(const or_world_node_t* n, const or_lod_params_t* P);
 int or_lod_queue(const or_world_node_t* nodes, int n_nodes, const int32_t* rlist, int n_r, const or_lod_params_t* P,
                  uint32_t* flags, int32_t* dirty, or_lod_item_t* items, or_chunk_t* descs, int cap, int* n_dirty,
                  int* n_chunks);
+
+
+/* ---- seams between the chunks of a world octree (oracle_seams.c) ---- */
+int64_t or_stitch_recursive(const or_sampler_t* s, const or_world_node_t* nodes, int n_nodes, int dim, float* out,
+                            int64_t cap);
+int64_t or_stitch_points(const or_sampler_t* s, const or_world_node_t* nodes, int n_nodes, const int32_t* leaves,
+                         int n_leaves, int dim, float* out, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -126,6 +126,11 @@ def lib():
                                       C.c_int, C.c_int, C.c_int, f32p, f32p, f32p, u32p, i64p, i64p]
         L.or_lod_queue.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_int, P(C.c_int), P(C.c_int)]
+        L.or_stitch_recursive.argtypes = [P(Sampler), C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64]
+        L.or_stitch_recursive.restype = C.c_int64
+        L.or_stitch_points.argtypes = [P(Sampler), C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                       C.c_int64]
+        L.or_stitch_points.restype = C.c_int64
         L.or_node_needs_split.argtypes = [C.c_void_p, C.c_void_p]
         L.or_node_needs_group.argtypes = [C.c_void_p, C.c_void_p]
         L.or_smooth_normals_n_f32.argtypes = [f32p, u8p, C.c_int64, u32p, C.c_int64, C.c_int, C.c_int, C.c_int,
@@ -567,3 +572,24 @@ def node_needs_split(node, params):
 def node_needs_group(node, params):
     n = np.ascontiguousarray(np.atleast_1d(node))
     return bool(lib().or_node_needs_group(n.ctypes.data, lod_params(params).ctypes.data))
+
+
+# ---- seams (oracle_seams.c) ----------------------------------------------------------------
+def stitch_recursive(spec, nodes, dim, cap=1 << 22):
+    """Triangles [T][3][3] of the reference's stitcher recursion, in its order."""
+    nodes = np.ascontiguousarray(nodes)
+    out = np.zeros((cap, 3, 3), np.float32)
+    n = lib().or_stitch_recursive(spec.ref, nodes.ctypes.data, len(nodes), dim, out.ctypes.data, cap)
+    assert n <= cap
+    return out[:n].copy()
+
+
+def stitch_points(spec, nodes, leaves, dim, cap=1 << 22):
+    """Triangles [T][3][3] of the point-by-point formulation, in the GPU's order."""
+    nodes = np.ascontiguousarray(nodes)
+    lv = np.ascontiguousarray(leaves, np.int32)
+    out = np.zeros((cap, 3, 3), np.float32)
+    n = lib().or_stitch_points(spec.ref, nodes.ctypes.data, len(nodes), lv.ctypes.data, len(lv), dim,
+                               out.ctypes.data, cap)
+    assert n <= cap
+    return out[:n].copy()
