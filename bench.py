@@ -308,6 +308,7 @@ def diff_leg(capi, roofline, workloads, steps):
             grad = capi.DeviceArray((dim, dim, dim), np.float32)
             capi.check(capi.lib().bmf_backward_diff(tape.h, g.ptr, grad.ptr), "bmf_backward_diff")
             t_f = t_l = t_b = 0.0
+            dev = [0.0, 0.0, 0.0]
             for _ in range(steps):
                 t0 = time.perf_counter()
                 capi.check(capi.lib().bmf_forward_diff(ctx.h, capi.C.byref(cd), None, tape.h), "bmf_forward_diff")
@@ -317,6 +318,8 @@ def diff_leg(capi, roofline, workloads, steps):
                 capi.check(capi.lib().bmf_backward_diff(tape.h, g.ptr, grad.ptr), "bmf_backward_diff")
                 t3 = time.perf_counter()
                 t_f, t_l, t_b = t_f + t1 - t0, t_l + t2 - t1, t_b + t3 - t2
+                dev = [a + b for a, b in zip(dev, tape.timing())]
+            dev = [d / steps for d in dev]
             check = {"chunk": workloads.Chunk((0.0, 0.0, 0.0), float(dim - 1), dim, 0.0, K, 0),
                      "densities": tape.densities(), "targets": targets_np, "nearest": near,
                      "gradient": grad.numpy()}
@@ -328,16 +331,20 @@ def diff_leg(capi, roofline, workloads, steps):
             bwd_bytes = 12 * V + 4 * n3
             fwd_flops = n3 * roofline.terrain_flops_per_sample(5) + roofline.smoothing_flops(V, T, K)
             log(f"[bench] fwd+bwd: {1e3 * per:.3f} ms/step (forward {1e3 * t_f / steps:.3f}, loss {1e3 * t_l / steps:.3f},"
-                f" backward {1e3 * t_b / steps:.3f}), V={V}, loss {L:.4f}")
+                f" backward {1e3 * t_b / steps:.3f}; device {dev[0]:.3f} / {dev[1]:.3f} / {dev[2]:.3f}), V={V}, "
+                f"loss {L:.4f}")
             return {"workload": "one 128^3-cell fbm terrain chunk, K=8, 200k sphere targets",
                     "steps": steps, "ms_per_step": round(1e3 * per, 4), "steps_per_s": round(1.0 / per, 2),
                     "forward_ms": round(1e3 * t_f / steps, 4), "loss_ms": round(1e3 * t_l / steps, 4),
                     "backward_ms": round(1e3 * t_b / steps, 4), "vertices": V, "triangles": T, "loss": L,
-                    "forward_fp32_frac": round(fwd_flops / (t_f / steps) / 1e12 / roofline.PEAK_FP32_TFLOPS, 4),
-                    "backward_hbm_frac": round(bwd_bytes / (t_b / steps) / 1e9 / roofline.PEAK_HBM_GBS, 5),
+                    "device_ms": {"forward": round(dev[0], 4), "loss": round(dev[1], 4), "backward": round(dev[2], 4)},
+                    "forward_fp32_frac": round(fwd_flops / (dev[0] * 1e-3) / 1e12 / roofline.PEAK_FP32_TFLOPS, 4),
+                    "backward_hbm_frac": round(bwd_bytes / (dev[2] * 1e-3) / 1e9 / roofline.PEAK_HBM_GBS, 5),
                     "targets_build_ms": round(1e3 * t_build, 4),
-                    "timing": "host wall clock per call; each call ends with a stream synchronisation; the "
-                              "target hierarchy is built once (targets_build_ms) and each step's loss queries it",
+                    "timing": "ms_per_step and *_ms: host wall clock per call (each call ends with a stream "
+                              "synchronisation); device_ms and the roofline fractions: HIP events on the library's "
+                              "stream around each call's GPU work; the target hierarchy is built once "
+                              "(targets_build_ms) and each step's loss queries it",
                     "_check": check}
 
 

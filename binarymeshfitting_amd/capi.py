@@ -24,6 +24,7 @@ EXPORTS = [
     "bmf_backward_diff", "bmf_get_tuning", "bmf_set_tuning", "bmf_autotune", "bmf_set_mesh_options",
     "bmf_get_mesh_options", "bmf_qef_error_to_host", "bmf_normals_to_host", "bmf_tape_set_targets",
     "bmf_tape_loss_nearest", "bmf_loss_nearest_noalloc", "bmf_format", "bmf_lod_queue", "bmf_extract_queued", "bmf_stitch",
+    "bmf_tape_timing",
 ]
 SPACE_GRID, SPACE_WORLD = 0, 1
 COLOR_NONE, COLOR_WHITE, COLOR_NOISE = 0, 1, 2
@@ -152,6 +153,7 @@ def lib():
         L.bmf_tape_positions.argtypes = [tp, C.c_int32, C.c_void_p]
         L.bmf_loss_nearest.argtypes = [tp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, P(C.c_float)]
         L.bmf_backward_diff.argtypes = [tp, C.c_void_p, C.c_void_p]
+        L.bmf_tape_timing.argtypes = [tp, P(C.c_float)]
         L.bmf_tape_set_targets.argtypes = [tp, C.c_void_p, C.c_int64]
         L.bmf_tape_loss_nearest.argtypes = [tp, C.c_void_p, C.c_void_p, P(C.c_float)]
         L.bmf_loss_nearest_noalloc.argtypes = [tp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -508,6 +510,12 @@ class Tape:
         check(lib().bmf_loss_nearest(self.h, targets.ptr, targets.shape[0], g.ptr, near.ptr, C.byref(L)),
               "bmf_loss_nearest")
         return L.value, g, near.numpy()[: i.vertex_count]
+
+    def timing(self):
+        """Device ms of the last forward, loss and backward (-1 where not run)."""
+        ms = (C.c_float * 3)()
+        check(lib().bmf_tape_timing(self.h, ms), "bmf_tape_timing")
+        return list(ms)
 
     def set_targets(self, targets):
         """Builds the search hierarchy of a fixed target set (DeviceArray [n][3] FP32) once."""

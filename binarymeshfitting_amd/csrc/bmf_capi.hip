@@ -271,6 +271,7 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     A.cap_vertices = lim->max_vertices;
     A.cap_indices = lim->max_indices;
     A.sample_tile = kSampleTileDefault;
+    A.smoothing_follows = 1;
     A.row_split = -1;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && dalloc(&A.meta, nc) == hipSuccess && dalloc(&A.descs, nc) == hipSuccess &&

@@ -21,7 +21,7 @@
 //                           wave's loads of one slot are contiguous
 //   deg u8 [V], mv u8 [V]   ring size; whether the primal step moves the
 //                           vertex (MeshProcessor.cpp:247)
-//   p4 float4 [V]           working positions; ckpt float4 [slots][V]
+//   p4, q4 float4 [V]       working positions (ping-pong); ckpt float4 [slots][V]
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -124,42 +124,42 @@ __global__ void k_tape_vid0(const uint32_t* __restrict__ vedge, int V, int dim, 
     vid0[((int64_t)x * dim + y) * dim + z] = (uint32_t)v;
 }
 
-// Dual step (MeshProcessor.cpp:130-236): centroid ((0 + p0) + p1) + p2, / 3.
-__global__ void k_tape_dual(const uint32_t* __restrict__ idx, const float4* __restrict__ p4, int T,
-                            float4* __restrict__ cen) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    const float4 a = p4[idx[3 * t]], b = p4[idx[3 * t + 1]], c = p4[idx[3 * t + 2]];
-    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    cx = cx + a.x;
-    cy = cy + a.y;
-    cz = cz + a.z;
-    cx = cx + b.x;
-    cy = cy + b.y;
-    cz = cz + b.z;
-    cx = cx + c.x;
-    cy = cy + c.y;
-    cz = cz + c.z;
-    cen[t] = make_float4(cx / 3.0f, cy / 3.0f, cz / 3.0f, 0.0f);
-}
-
-// Primal step (MeshProcessor.cpp:238-306): ring sum in ascending triangle
-// order from +0, divided by the ring size.
-__global__ void k_tape_primal(const uint32_t* __restrict__ adj, int stride, const uint8_t* __restrict__ deg,
-                              const uint8_t* __restrict__ mv, const float4* __restrict__ cen, int V,
-                              float4* __restrict__ p4) {
+// One smoothing iteration (MeshProcessor.cpp:130-306) in one launch: the new
+// position of a moved vertex is its ring's centroids summed from +0 in
+// ascending triangle order and divided by the ring size, each centroid
+// ((0 + p0) + p1) + p2, / 3 recomputed from the corners by every vertex of
+// the triangle (the same FP32 value each time, so no separate dual pass);
+// other vertices keep theirs.  Reads `in`, writes `out` (ping-pong buffers).
+__global__ void k_tape_step(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ adj, int stride,
+                            const uint8_t* __restrict__ deg, const uint8_t* __restrict__ mv,
+                            const float4* __restrict__ in, int V, float4* __restrict__ out) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V || !mv[v]) return;
+    if (v >= V) return;
+    if (!mv[v]) {
+        out[v] = in[v];
+        return;
+    }
     const int n = deg[v];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     for (int k = 0; k < n; k++) {
-        const float4 q = cen[adj[(int64_t)k * stride + v]];
-        sx = sx + q.x;
-        sy = sy + q.y;
-        sz = sz + q.z;
+        const uint32_t t = adj[(int64_t)k * stride + v];
+        const float4 a = in[idx[3 * t]], b = in[idx[3 * t + 1]], c = in[idx[3 * t + 2]];
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        cx = cx + a.x;
+        cy = cy + a.y;
+        cz = cz + a.z;
+        cx = cx + b.x;
+        cy = cy + b.y;
+        cz = cz + b.z;
+        cx = cx + c.x;
+        cy = cy + c.y;
+        cz = cz + c.z;
+        sx = sx + cx / 3.0f;
+        sy = sy + cy / 3.0f;
+        sz = sz + cz / 3.0f;
     }
     const float fc = (float)n;
-    p4[v] = make_float4(sx / fc, sy / fc, sz / fc, 0.0f);
+    out[v] = make_float4(sx / fc, sy / fc, sz / fc, 0.0f);
 }
 
 __global__ void k_tape_out(const float4* __restrict__ p4, int V, float* __restrict__ pos) {
@@ -180,43 +180,37 @@ __global__ void k_bwd_init(const float* __restrict__ g3, int V, float4* __restri
     g[v] = make_float4(g3[3 * v], g3[3 * v + 1], g3[3 * v + 2], 0.0f);
 }
 
-// dL/dc_t = sum over the corners u of t that the primal step moves of
-// dL/dp'_u / |ring(u)|.
-__global__ void k_bwd_tri(const uint32_t* __restrict__ idx, const float4* __restrict__ g,
-                          const uint8_t* __restrict__ deg, const uint8_t* __restrict__ mv, int T,
-                          float4* __restrict__ gc) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const uint32_t u = idx[3 * t + k];
-        if (!mv[u]) continue;
-        const float4 q = g[u];
-        const float n = (float)deg[u];
-        ax = ax + q.x / n;
-        ay = ay + q.y / n;
-        az = az + q.z / n;
-    }
-    gc[t] = make_float4(ax, ay, az, 0.0f);
-}
-
-// dL/dp_v = [v not moved] dL/dp'_v + (sum over ring(v) of dL/dc_t) / 3.
-__global__ void k_bwd_vert(const uint32_t* __restrict__ adj, int stride, const uint8_t* __restrict__ deg,
-                           const uint8_t* __restrict__ mv, const float4* __restrict__ gc, int V,
-                           float4* __restrict__ g) {
+// One transposed smoothing iteration in one launch: dL/dp_v = [v not moved]
+// dL/dp'_v + (sum over ring(v) of dL/dc_t) / 3, where dL/dc_t = the sum over
+// the corners u of t that the primal step moves of dL/dp'_u / |ring(u)|,
+// from +0 in corner order, recomputed by every vertex of the triangle (the
+// same FP32 value each time).  Reads `in`, writes `out` (ping-pong buffers).
+__global__ void k_bwd_step(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ adj, int stride,
+                           const uint8_t* __restrict__ deg, const uint8_t* __restrict__ mv,
+                           const float4* __restrict__ in, int V, float4* __restrict__ out) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const int n = deg[v];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     for (int k = 0; k < n; k++) {
-        const float4 q = gc[adj[(int64_t)k * stride + v]];
-        sx = sx + q.x;
-        sy = sy + q.y;
-        sz = sz + q.z;
+        const uint32_t t = adj[(int64_t)k * stride + v];
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const uint32_t u = idx[3 * t + c];
+            if (!mv[u]) continue;
+            const float4 q = in[u];
+            const float nu = (float)deg[u];
+            ax = ax + q.x / nu;
+            ay = ay + q.y / nu;
+            az = az + q.z / nu;
+        }
+        sx = sx + ax;
+        sy = sy + ay;
+        sz = sz + az;
     }
-    const float4 own = mv[v] ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : g[v];
-    g[v] = make_float4(own.x + sx / 3.0f, own.y + sy / 3.0f, own.z + sz / 3.0f, 0.0f);
+    const float4 own = mv[v] ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : in[v];
+    out[v] = make_float4(own.x + sx / 3.0f, own.y + sy / 3.0f, own.z + sz / 3.0f, 0.0f);
 }
 
 // dL/d(density) per sample: the sample is s0 of its own sign-changing edges
@@ -601,10 +595,13 @@ struct bmf_tape {
     uint8_t* mv = nullptr;
     float4* p4 = nullptr;
     float4* q4 = nullptr;
-    float4* cen = nullptr;
     float* pos = nullptr;
     float4* g = nullptr;
+    float4* g2 = nullptr;   // ping-pong partner of g in the backward
+    float4* r4 = nullptr;   // ping-pong partner of q4 in bmf_tape_positions
     int* err = nullptr;
+    hipEvent_t ev[6] = {};  // start / end of the last forward, loss and backward
+    bool timed[3] = {false, false, false};
     float4* ckpt = nullptr;
     int64_t ckpt_slots = 0;
     // nearest-target search: hierarchy buffers for up to nn_cap targets and
@@ -729,19 +726,25 @@ int nn_query(bmf_tape* t, const NNBuffers& b, int64_t n, float* dldp, int32_t* n
     return BMF_OK;
 }
 
-// Runs k smoothing iterations on `state` (float4 [V]) with the tape's ring.
-int smooth_steps(bmf_tape* t, float4* state, int k, hipStream_t s, int ckpt_from = -1) {
+// Runs k smoothing iterations starting from `a` (float4 [V]), ping-ponging
+// between a and b, one launch per iteration; the state after iteration m is
+// kept in checkpoint slot (from + m + 1) / every when that divides (from >= 0).
+// Returns the buffer that holds the final state in *final.
+int smooth_steps(bmf_tape* t, float4* a, float4* b, int k, hipStream_t s, float4** final, int ckpt_from = -1) {
+    float4* cur = a;
+    float4* nxt = b;
     for (int m = 0; m < k; m++) {
-        hipLaunchKernelGGL(k_tape_dual, dim3(blocks_for(t->T)), dim3(kThreads), 0, s, t->idx, state, t->T, t->cen);
-        hipLaunchKernelGGL(k_tape_primal, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->adj, (int)t->cap_v, t->deg,
-                           t->mv, t->cen, t->V, state);
+        hipLaunchKernelGGL(k_tape_step, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->idx, t->adj, (int)t->cap_v,
+                           t->deg, t->mv, cur, t->V, nxt);
         BMF_HIP(hipGetLastError());
+        std::swap(cur, nxt);
         if (ckpt_from >= 0 && (ckpt_from + m + 1) % t->ckpt_every == 0) {
             const int64_t slot = (ckpt_from + m + 1) / t->ckpt_every;
-            BMF_HIP(hipMemcpyAsync(t->ckpt + slot * t->cap_v, state, sizeof(float4) * (size_t)t->V,
+            BMF_HIP(hipMemcpyAsync(t->ckpt + slot * t->cap_v, cur, sizeof(float4) * (size_t)t->V,
                                    hipMemcpyDeviceToDevice, s));
         }
     }
+    *final = cur;
     return BMF_OK;
 }
 
@@ -828,18 +831,20 @@ int bmf_tape_create(bmf_tape** out, bmf_ctx* ctx, int32_t max_dim, int64_t max_v
     t->cap_i = max_indices;
     t->ckpt_every = checkpoint_every;
     const int64_t n3 = (int64_t)max_dim * max_dim * max_dim;
-    const int64_t nt = max_indices / 3 + 1;
     const bool ok = dalloc(&t->dens, n3) == hipSuccess && dalloc(&t->vid0, n3) == hipSuccess &&
                     dalloc(&t->vedge, max_vertices) == hipSuccess && dalloc(&t->boundary, max_vertices) == hipSuccess &&
                     dalloc(&t->idx, max_indices) == hipSuccess &&
                     dalloc(&t->adj, (int64_t)kMaxRing * max_vertices) == hipSuccess &&
                     dalloc(&t->deg, max_vertices) == hipSuccess && dalloc(&t->mv, max_vertices) == hipSuccess &&
                     dalloc(&t->p4, max_vertices) == hipSuccess && dalloc(&t->q4, max_vertices) == hipSuccess &&
-                    dalloc(&t->cen, nt) == hipSuccess && dalloc(&t->pos, 3 * max_vertices) == hipSuccess &&
-                    dalloc(&t->g, max_vertices) == hipSuccess && dalloc(&t->err, 1) == hipSuccess &&
+                    dalloc(&t->pos, 3 * max_vertices) == hipSuccess &&
+                    dalloc(&t->g, max_vertices) == hipSuccess && dalloc(&t->g2, max_vertices) == hipSuccess &&
+                    dalloc(&t->r4, max_vertices) == hipSuccess && dalloc(&t->err, 1) == hipSuccess &&
                     dalloc(&t->bb, 8) == hipSuccess && dalloc(&t->d2, max_vertices) == hipSuccess &&
                     dalloc(&t->loss, 1) == hipSuccess && dalloc(&t->nearest_tmp, max_vertices) == hipSuccess;
-    if (!ok) {
+    bool evs = ok;
+    for (int i = 0; i < 6 && evs; i++) evs = hipEventCreate(&t->ev[i]) == hipSuccess;
+    if (!ok || !evs) {
         bmf_tape_destroy(t);
         return BMF_EHIP;
     }
@@ -864,9 +869,12 @@ void bmf_tape_destroy(bmf_tape* t) {
     dfree(t->mv);
     dfree(t->p4);
     dfree(t->q4);
-    dfree(t->cen);
     dfree(t->pos);
     dfree(t->g);
+    dfree(t->g2);
+    dfree(t->r4);
+    for (int i = 0; i < 6; i++)
+        if (t->ev[i]) (void)hipEventDestroy(t->ev[i]);
     dfree(t->err);
     dfree(t->ckpt);
     dfree(t->bb);
@@ -890,6 +898,8 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
     hipStream_t s = c->stream;
     BMF_HIP(hipStreamSynchronize(s));
     t->valid = false;
+    t->timed[0] = false;
+    BMF_HIP(hipEventRecord(t->ev[0], s));
     c->have_extract = false;
     c->results_valid = false;
     // the sampler of this call: the context's, or the caller's densities; the
@@ -915,6 +925,7 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
     A.ext_base = 0;
     A.mesh_mode = BMF_MESH_TRIANGLES;  // the adjoint is defined on the triangle path
     A.quad_tris = 0;
+    A.smoothing_follows = 0;  // the tape smooths its own copy: every output position is written by emit
     A.normals = 0;
     const int wcls = row_word_class(dim);
     // extraction with every octave evaluated (densities exact everywhere),
@@ -968,21 +979,24 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
         }
         BMF_HIP(hipGetLastError());
         BMF_HIP(hipMemcpyAsync(t->ckpt, t->p4, sizeof(float4) * V, hipMemcpyDeviceToDevice, s));
+        float4* fin = t->p4;
         if (t->K > 0 && T > 0) {
-            rc = smooth_steps(t, t->p4, t->K, s, 0);
+            rc = smooth_steps(t, t->p4, t->q4, t->K, s, &fin, 0);
             if (rc) return rc;
         } else {
             for (int64_t q = 1; q < slots; q++)
                 BMF_HIP(hipMemcpyAsync(t->ckpt + q * t->cap_v, t->p4, sizeof(float4) * V, hipMemcpyDeviceToDevice, s));
         }
-        hipLaunchKernelGGL(k_tape_out, dim3(blocks_for(V)), dim3(kThreads), 0, s, t->p4, V, t->pos);
+        hipLaunchKernelGGL(k_tape_out, dim3(blocks_for(V)), dim3(kThreads), 0, s, fin, V, t->pos);
         BMF_HIP(hipGetLastError());
     }
+    BMF_HIP(hipEventRecord(t->ev[1], s));
     int herr = 0;
     BMF_HIP(hipMemcpyAsync(&herr, t->err, sizeof(int), hipMemcpyDeviceToHost, s));
     BMF_HIP(hipStreamSynchronize(s));
     if (V > 0 && herr) return BMF_EHIP;
     t->valid = true;
+    t->timed[0] = true;
     return BMF_OK;
 }
 
@@ -1012,9 +1026,10 @@ int bmf_tape_positions(bmf_tape* t, int32_t iteration, float* out) {
     const int64_t slot = iteration / t->ckpt_every;
     BMF_HIP(hipMemcpyAsync(t->q4, t->ckpt + slot * t->cap_v, sizeof(float4) * t->V, hipMemcpyDeviceToDevice, s));
     const int rest = t->T > 0 ? iteration - (int)(slot * t->ckpt_every) : 0;
-    int rc = smooth_steps(t, t->q4, rest, s);
+    float4* fin = t->q4;
+    int rc = smooth_steps(t, t->q4, t->r4, rest, s, &fin);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_tape_out, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->q4, t->V, out);
+    hipLaunchKernelGGL(k_tape_out, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, fin, t->V, out);
     BMF_HIP(hipGetLastError());
     BMF_HIP(hipStreamSynchronize(s));
     return BMF_OK;
@@ -1028,6 +1043,8 @@ int bmf_loss_nearest(bmf_tape* t, const float* targets, int64_t n, float* dldp, 
     int rc = reserve_nn(t, n);
     if (rc) return rc;
     t->nn_n = 0;
+    t->timed[1] = false;
+    BMF_HIP(hipEventRecord(t->ev[2], s));
     rc = nn_build(targets, n, t->bb, tape_nn(t), s);
     if (rc) return rc;
     t->nn_n = n;
@@ -1038,10 +1055,12 @@ int bmf_loss_nearest(bmf_tape* t, const float* targets, int64_t n, float* dldp, 
     }
     rc = nn_query(t, tape_nn(t), n, dldp, nearest, t->loss, s);
     if (rc) return rc;
+    BMF_HIP(hipEventRecord(t->ev[3], s));
     double l = 0.0;
     BMF_HIP(hipMemcpyAsync(&l, t->loss, sizeof(double), hipMemcpyDeviceToHost, s));
     BMF_HIP(hipStreamSynchronize(s));
     if (loss) *loss = (float)l;
+    t->timed[1] = true;
     return BMF_OK;
 }
 
@@ -1068,12 +1087,16 @@ int bmf_tape_loss_nearest(bmf_tape* t, float* dldp, int32_t* nearest, float* los
         if (loss) *loss = 0.0f;
         return BMF_OK;
     }
+    t->timed[1] = false;
+    BMF_HIP(hipEventRecord(t->ev[2], s));
     int rc = nn_query(t, tape_nn(t), t->nn_n, dldp, nearest, t->loss, s);
     if (rc) return rc;
+    BMF_HIP(hipEventRecord(t->ev[3], s));
     double l = 0.0;
     BMF_HIP(hipMemcpyAsync(&l, t->loss, sizeof(double), hipMemcpyDeviceToHost, s));
     BMF_HIP(hipStreamSynchronize(s));
     if (loss) *loss = (float)l;
+    t->timed[1] = true;
     return BMF_OK;
 }
 
@@ -1117,17 +1140,32 @@ int bmf_backward_diff(bmf_tape* t, const float* dldp, float* dldd) {
         BMF_HIP(hipStreamSynchronize(s));
         return BMF_OK;
     }
+    t->timed[2] = false;
+    BMF_HIP(hipEventRecord(t->ev[4], s));
     hipLaunchKernelGGL(k_bwd_init, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, dldp, t->V, t->g);
+    float4* cur = t->g;
+    float4* nxt = t->g2;
     if (t->T > 0)
         for (int m = t->K - 1; m >= 0; m--) {
-            hipLaunchKernelGGL(k_bwd_tri, dim3(blocks_for(t->T)), dim3(kThreads), 0, s, t->idx, t->g, t->deg, t->mv,
-                               t->T, t->cen);
-            hipLaunchKernelGGL(k_bwd_vert, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->adj, (int)t->cap_v, t->deg,
-                               t->mv, t->cen, t->V, t->g);
+            hipLaunchKernelGGL(k_bwd_step, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->idx, t->adj,
+                               (int)t->cap_v, t->deg, t->mv, cur, t->V, nxt);
+            std::swap(cur, nxt);
         }
-    hipLaunchKernelGGL(k_bwd_place, dim3(blocks_for(n3)), dim3(kThreads), 0, s, t->dens, t->vid0, t->g, t->dim, dldd);
+    hipLaunchKernelGGL(k_bwd_place, dim3(blocks_for(n3)), dim3(kThreads), 0, s, t->dens, t->vid0, cur, t->dim, dldd);
     BMF_HIP(hipGetLastError());
+    BMF_HIP(hipEventRecord(t->ev[5], s));
     BMF_HIP(hipStreamSynchronize(s));
+    t->timed[2] = true;
+    return BMF_OK;
+}
+
+int bmf_tape_timing(bmf_tape* t, float* ms) {
+    if (!t || !ms) return BMF_EINVAL;
+    BMF_HIP(hipSetDevice(t->ctx->device));
+    for (int k = 0; k < 3; k++) {
+        ms[k] = -1.0f;
+        if (t->timed[k]) BMF_HIP(hipEventElapsedTime(&ms[k], t->ev[2 * k], t->ev[2 * k + 1]));
+    }
     return BMF_OK;
 }
 

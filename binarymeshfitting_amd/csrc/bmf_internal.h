@@ -129,6 +129,8 @@ struct Arenas {
     int32_t cap_chunks;
     int64_t ext_base;          // EXTERNAL: offset of this launch's first chunk in the caller's densities
     int32_t sample_tile;       // samples per sampling work-queue tile: 1024, 2048 or 4096
+    int32_t smoothing_follows; // the launch runs the smoothing kernels after the emit kernel (which then
+                               // leaves the output positions of the vertices they move to them)
     uint16_t* csg_list;        // CSG: [chunk][csg_stride] primitives per chunk (k_plan), or null
     int32_t csg_stride;        // n_spheres + n_boxes
     // mesh options (BMF_MESH_*, BMF_PLACE_*): the quad mode's extra arenas

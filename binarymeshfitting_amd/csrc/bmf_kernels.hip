@@ -997,6 +997,11 @@ __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M
     const uint32_t Vc = (uint32_t)M.V;
     float* __restrict__ P = A.pos + vo * 3;
     const bool cells = M.iters > 0 || A.normals;  // the chunk runs the smoothing kernels
+    // with K > 0 the last primal step writes the output position of every
+    // vertex it moves (every vertex has a ring; boundary vertices move only
+    // with boundary processing), so only the others are written here
+    const bool moves = M.iters > 0 && A.smoothing_follows;
+    const bool moves_boundary = moves && M.bp;
     float4* __restrict__ P4 = cells ? A.pos4 + vo : nullptr;  // smoothing state
     float4* __restrict__ N4 = cells && A.normals ? A.nrm4 + vo : nullptr;
     uint8_t* __restrict__ B = A.boundary + vo;
@@ -1028,13 +1033,16 @@ __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M
                     const float mu = (0.0f - s0) / (s1 - s0);
                     const float dx = (float)xe - (float)x, dy = (float)ye - (float)y, dz = (float)ze - (float)z;
                     const float qx = dx * mu + (float)x, qy = dy * mu + (float)y, qz = dz * mu + (float)z;
-                    P[3 * (int64_t)id + 0] = qx;
-                    P[3 * (int64_t)id + 1] = qy;
-                    P[3 * (int64_t)id + 2] = qz;
+                    const bool bnd = x == 0 || y == 0 || z == 0 || x == dim - 1 || y == dim - 1 || z == dim - 1 ||
+                                     xe == dim - 1 || ye == dim - 1 || ze == dim - 1;
+                    if (!(bnd ? moves_boundary : moves)) {
+                        P[3 * (int64_t)id + 0] = qx;
+                        P[3 * (int64_t)id + 1] = qy;
+                        P[3 * (int64_t)id + 2] = qz;
+                    }
                     if (P4) P4[id] = make_float4(qx, qy, qz, 0.0f);
                     if (N4) N4[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    B[id] = (uint8_t)(x == 0 || y == 0 || z == 0 || x == dim - 1 || y == dim - 1 || z == dim - 1 ||
-                                      xe == dim - 1 || ye == dim - 1 || ze == dim - 1);
+                    B[id] = (uint8_t)bnd;
                     E[id] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | ((uint32_t)ax << 24);
                     id++;
                 }
@@ -1375,7 +1383,16 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal(Arenas A, int m) {
         float s[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         const uint32_t cnt = ring_sums<false>(r, A.cen4 + t_off, nullptr, s);
         if (m == 0 && K > 1) A.vring[g] = r;  // stored after the loads: a store here is waited for with them
-        if (cnt == 0) continue;  // MeshProcessor.cpp:247: no adjacency, unchanged
+        if (cnt == 0) {  // MeshProcessor.cpp:247: no adjacency, unchanged (the emit kernel left its output to here)
+            if (m == K - 1) {
+                const float4 q = A.pos4[g];
+                float* __restrict__ P = A.pos + g * 3;
+                P[0] = q.x;
+                P[1] = q.y;
+                P[2] = q.z;
+            }
+            continue;
+        }
         const float fc = (float)cnt;
         const float px = s[0] / fc, py = s[1] / fc, pz = s[2] / fc;
         A.pos4[g] = make_float4(px, py, pz, 0.0f);
@@ -1513,6 +1530,13 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_nrm(Arenas A, int m) 
                     P[2] = pz;
                 }
             }
+        } else if (K > 0 && m == K - 1 && (bp || !A.boundary[g])) {
+            // no ring: unchanged, and the emit kernel left its output to here
+            const float4 q = A.pos4[g];
+            float* __restrict__ P = A.pos + g * 3;
+            P[0] = q.x;
+            P[1] = q.y;
+            P[2] = q.z;
         }
         if (m == steps - 1) {
             const float l = sqrtf((n.x * n.x + n.y * n.y) + n.z * n.z);

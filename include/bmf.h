@@ -475,6 +475,11 @@ int bmf_tape_loss_nearest(bmf_tape* tape, float* dloss_dpos, int32_t* nearest, f
 int bmf_loss_nearest_noalloc(bmf_tape* tape, const float* targets, int64_t n_targets, float* dloss_dpos,
                              int32_t* nearest, double* loss);
 
+/* Device time in ms (HIP events on the context stream) of the tape's last
+ * forward pass [0], loss [1] (bmf_loss_nearest or bmf_tape_loss_nearest) and
+ * backward pass [2]; -1 for a call that has not completed. */
+int bmf_tape_timing(bmf_tape* tape, float* ms);
+
 /* Backward pass: dL/d(density) [dim^3] (device) from dL/d(final positions)
  * [V][3] (device).  Samples that are not an endpoint of a sign-changing
  * edge get 0. */

@@ -249,3 +249,16 @@ def test_cached_target_set_matches_per_call_build(ctx, oracle):
             assert L1 == L0
             assert np.array_equal(n1, n0)
             assert np.array_equal(g1.numpy()[:V], g0.numpy()[:V])
+
+
+def test_tape_timing_reports_each_call(ctx):
+    with capi.Tape(ctx, max_dim=65, max_vertices=1 << 18, max_indices=1 << 20, checkpoint_every=2) as t:
+        assert t.timing() == [-1.0, -1.0, -1.0]
+        t.forward(capi.chunk_desc((0.0, 0.0, 0.0), 64.0, 65, 0.0, 4, 0))
+        t.set_targets(capi.DeviceArray.from_numpy(workloads.sphere_targets(n=10_000, radius=20.0,
+                                                                           centre=(32.0, 32.0, 32.0))))
+        _, g, _ = t.loss_nearest_set()
+        t.backward(g)
+        ms = t.timing()
+        print("device ms (forward, loss, backward):", ms)
+        assert all(0.0 < x < 1000.0 for x in ms)
