@@ -15,12 +15,19 @@
 //   vid0  u32 [dim^3]       first vertex id owned by each sample (valid
 //                           where the sample owns a sign-changing edge)
 //   vedge u32 [V], boundary u8 [V], idx u32 [3T]   copies of the mesh
-//   adj   u32 [kMaxRing][V] ring of each vertex: its triangles in ascending
-//                           id (the adjacency of MeshProcessor::set_adj,
-//                           MeshProcessor.cpp:98-128), slot-major so that a
+//   ring  u32x2 [kMaxRing][V] ring of each vertex: its triangles in
+//                           ascending id (the adjacency of
+//                           MeshProcessor::set_adj, MeshProcessor.cpp:98-128),
+//                           each as its two other corners and the vertex's
+//                           place among the three; slot-major so that a
 //                           wave's loads of one slot are contiguous
 //   deg u8 [V], mv u8 [V]   ring size; whether the primal step moves the
 //                           vertex (MeshProcessor.cpp:247)
+//   nbr   u32 [kMaxNbr][V], nnz u8 [V], bscale f32 [V]   the transposed
+//                           smoothing step (ELL rows, slot-major): each
+//                           vertex's distinct ring neighbours with the number
+//                           of its ring triangles they share (id | m << 27),
+//                           and 1 / (3 |ring|) for moved vertices (else 1)
 //   p4, q4 float4 [V]       working positions (ping-pong); ckpt float4 [slots][V]
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -40,10 +47,18 @@ using namespace bmf;
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kStepThreads = 64;  // smoothing steps: one wave per workgroup spreads a small mesh over more CUs
 // A vertex's lattice edge borders 4 cells and one cell's triangulation uses
 // one edge in at most 5 table slots (MCTable.h:27-283), so a ring holds at
 // most 20 triangles.
 constexpr int kMaxRing = 20;
+// A ring of n triangles touches at most 2n other vertices.
+constexpr int kMaxNbr = 2 * kMaxRing;
+// Vertex ids share a word with a multiplicity (5 bits) in the neighbour
+// records and with a corner place (2 bits) in the ring records.
+constexpr int kVidBits = 27;
+constexpr uint32_t kVidMask = (1u << kVidBits) - 1u;
+constexpr int64_t kMaxTapeVertices = int64_t(1) << kVidBits;
 constexpr int kLeaf = 8;       // targets per leaf box of the nearest-target hierarchy
 constexpr int kNNThreads = 256;  // four vertices per workgroup, one wave each
 constexpr int kNNFan = 6;        // a wave tests 2^kNNFan = 64 boxes of one subtree level at once
@@ -67,24 +82,70 @@ void dfree(T*& p) {
 }
 
 unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kThreads - 1) / kThreads); }
+unsigned step_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kStepThreads - 1) / kStepThreads); }
 
 // ------------------------------------------------------------------------
 // forward
 // ------------------------------------------------------------------------
-__global__ void k_tape_init(const float* __restrict__ pos, int V, float4* __restrict__ p4) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    p4[v] = make_float4(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 0.0f);
+// Takes the chunk's extraction into the tape in one launch: densities, edge
+// codes, boundary flags and triangles copied out of the context's arenas,
+// the positions as float4 (working copy and checkpoint 0, and as float3
+// when no smoothing follows), and each sample's first vertex id (vertices
+// are numbered in sample order, x-, y-, z-edge within a sample).
+__global__ void k_tape_take(const float* __restrict__ dens_in, const uint32_t* __restrict__ vedge_in,
+                            const uint8_t* __restrict__ bnd_in, const uint32_t* __restrict__ idx_in,
+                            const float* __restrict__ pos_in, int64_t n3, int V, int64_t n_idx, int dim,
+                            float* __restrict__ dens, uint32_t* __restrict__ vedge, uint8_t* __restrict__ bnd,
+                            uint32_t* __restrict__ idx, float4* __restrict__ p4, float4* __restrict__ ck0,
+                            float* __restrict__ pos3, uint32_t* __restrict__ vid0) {
+    const int64_t total = std::max<int64_t>(n3, std::max<int64_t>(n_idx, V));
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < n3) dens[i] = dens_in[i];
+        if (i < n_idx) idx[i] = idx_in[i];
+        if (i < V) {
+            const uint32_t code = vedge_in[i];
+            vedge[i] = code;
+            bnd[i] = bnd_in[i];
+            const float x = pos_in[3 * i], y = pos_in[3 * i + 1], z = pos_in[3 * i + 2];
+            p4[i] = make_float4(x, y, z, 0.0f);
+            ck0[i] = make_float4(x, y, z, 0.0f);
+            if (pos3) {
+                pos3[3 * i] = x;
+                pos3[3 * i + 1] = y;
+                pos3[3 * i + 2] = z;
+            }
+            const uint32_t smp = code & 0xFFFFFFu;
+            if (i == 0 || (vedge_in[i - 1] & 0xFFFFFFu) != smp) {
+                const int sx = (int)(smp & 255u), sy = (int)((smp >> 8) & 255u), sz = (int)(smp >> 16);
+                vid0[((int64_t)sx * dim + sy) * dim + sz] = (uint32_t)i;
+            }
+        }
+    }
 }
 
 // Ring of each vertex from the per-cell triangle bases and masks written by
 // the emit kernel: the 4 cells around its edge in lexicographic order and,
-// in each, the table slots on its local edge in slot order.
-__global__ void k_tape_ring(const uint32_t* __restrict__ vedge, const uint8_t* __restrict__ boundary,
-                            const uint32_t* __restrict__ cell_tb, const uint8_t* __restrict__ cell_mask, int dim,
-                            int V, int T, int bp, int stride, uint32_t* __restrict__ adj, uint8_t* __restrict__ deg,
-                            uint8_t* __restrict__ mv, int* __restrict__ err) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+// in each, the table slots on its local edge in slot order.  Each triangle
+// is stored as its other two corners in corner order, the first with the
+// vertex's own place (0, 1, 2) in bits 30-31.
+//
+// The same launch builds the transposed smoothing step.  The forward step is
+// p'_u = [u not moved] p_u + [u moved] / (3 |ring(u)|) * (sum over the
+// triangles of ring(u) of their corners), so
+//   dL/dp_v = [v not moved] dL/dp'_v + sum over moved u of m_uv / (3 |ring(u)|) dL/dp'_u,
+// m_uv = the number of triangles holding both u and v (m_vv = |ring(v)|).
+// Row v lists its ring neighbours u != v, each once in order of first
+// appearance, with m_uv; the step reads [u moved] and the scaled gradient
+// dL/dp'_u / (3 |ring(u)|) from u's own record, so no row needs another
+// vertex's ring.  The other corners are staged in LDS, one column per lane.
+__global__ void __launch_bounds__(kStepThreads) k_tape_adj(
+    const uint32_t* __restrict__ vedge, const uint8_t* __restrict__ boundary, const uint32_t* __restrict__ cell_tb,
+    const uint8_t* __restrict__ cell_mask, const uint32_t* __restrict__ idx, int dim, int V, int T, int bp,
+    int stride, uint2* __restrict__ ring, uint8_t* __restrict__ deg, uint8_t* __restrict__ mv,
+    uint32_t* __restrict__ nbr, uint8_t* __restrict__ nnz, float* __restrict__ bscale, int* __restrict__ err) {
+    __shared__ uint32_t others[kMaxNbr][kStepThreads];
+    const int tid = threadIdx.x;
+    const int v = blockIdx.x * blockDim.x + tid;
     if (v >= V) return;
     const uint32_t code = vedge[v];
     const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
@@ -105,23 +166,77 @@ __global__ void k_tape_ring(const uint32_t* __restrict__ vedge, const uint8_t* _
                 *err = 1;
                 continue;
             }
-            adj[(int64_t)n * stride + v] = t;
+            const uint32_t a = idx[3 * t], b = idx[3 * t + 1], d = idx[3 * t + 2];
+            const uint32_t place = a == (uint32_t)v ? 0u : (b == (uint32_t)v ? 1u : 2u);
+            if (place == 2u && d != (uint32_t)v) *err = 1;
+            const uint32_t o1 = place == 0u ? b : a, o2 = place == 2u ? b : d;
+            ring[(int64_t)n * stride + v] = make_uint2(o1 | (place << 30), o2);
+            others[2 * n][tid] = o1;
+            others[2 * n + 1][tid] = o2;
             n++;
         }
     }
+    const bool moved = (bp || !boundary[v]) && n > 0;
     deg[v] = (uint8_t)n;
-    mv[v] = (uint8_t)((bp || !boundary[v]) && n > 0);
+    mv[v] = (uint8_t)moved;
+    bscale[v] = moved ? 1.0f / (3.0f * (float)n) : 1.0f;
+    int out = 0;
+    for (int i = 0; i < 2 * n; i++) {
+        const uint32_t u = others[i][tid];
+        bool seen = false;
+        for (int j = 0; j < i && !seen; j++) seen = others[j][tid] == u;
+        if (seen) continue;
+        uint32_t m = 1;
+        for (int j = i + 1; j < 2 * n; j++) m += others[j][tid] == u;
+        nbr[(int64_t)out * stride + v] = u | (m << kVidBits);
+        out++;
+    }
+    nnz[v] = (uint8_t)out;
 }
 
-// vid0[sample] = first vertex owned by the sample (vertices are numbered in
-// sample order, x-, y-, z-edge within a sample).
-__global__ void k_tape_vid0(const uint32_t* __restrict__ vedge, int V, int dim, uint32_t* __restrict__ vid0) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    const uint32_t s = vedge[v] & 0xFFFFFFu;
-    if (v > 0 && (vedge[v - 1] & 0xFFFFFFu) == s) return;
-    const int x = (int)(s & 255u), y = (int)((s >> 8) & 255u), z = (int)(s >> 16);
-    vid0[((int64_t)x * dim + y) * dim + z] = (uint32_t)v;
+// x / 3 rounded to nearest in three FP32 operations: q0 = x RN(1/3), the
+// exact remainder r = x - 3 q0 (one fma), then q0 + r RN(1/3) (one fma) --
+// the correction step of Markstein's division (IA-64 and Elementary
+// Functions, 2000, ch. 8); zeros and infinities take q0.  Equal to the IEEE
+// quotient for every float (tests/test_exact_division.py runs the same
+// operations on the host over all 2^32 of them).
+__device__ __forceinline__ float div3(float x) {
+    const float third = 1.0f / 3.0f;
+    const float q0 = x * third;
+    const float q = __builtin_fmaf(__builtin_fmaf(-q0, 3.0f, x), third, q0);
+    // v_cmp_class mask: -inf (bit 2), -0 (5), +0 (6), +inf (9)
+    return __builtin_amdgcn_classf(x, (1 << 2) | (1 << 5) | (1 << 6) | (1 << 9)) ? q0 : q;
+}
+
+// The new position of one moved vertex: its ring's centroids summed from +0
+// in ascending triangle order, divided by the ring size.
+__device__ __forceinline__ float4 smooth_vertex(const uint2* __restrict__ ring, int stride, int n,
+                                               const float4* __restrict__ in, float4 pv, int v) {
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll 4
+    for (int k = 0; k < n; k++) {
+        const uint2 e = ring[(int64_t)k * stride + v];
+        const uint32_t place = e.x >> 30;
+        const float4 a = in[e.x & 0x3FFFFFFFu], b = in[e.y];
+        const float4 c0 = place == 0u ? pv : a;
+        const float4 c1 = place == 0u ? a : (place == 1u ? pv : b);
+        const float4 c2 = place == 2u ? pv : b;
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        cx = cx + c0.x;
+        cy = cy + c0.y;
+        cz = cz + c0.z;
+        cx = cx + c1.x;
+        cy = cy + c1.y;
+        cz = cz + c1.z;
+        cx = cx + c2.x;
+        cy = cy + c2.y;
+        cz = cz + c2.z;
+        sx = sx + div3(cx);
+        sy = sy + div3(cy);
+        sz = sz + div3(cz);
+    }
+    const float fc = (float)n;
+    return make_float4(sx / fc, sy / fc, sz / fc, 0.0f);
 }
 
 // One smoothing iteration (MeshProcessor.cpp:130-306) in one launch: the new
@@ -129,37 +244,26 @@ __global__ void k_tape_vid0(const uint32_t* __restrict__ vedge, int V, int dim, 
 // ascending triangle order and divided by the ring size, each centroid
 // ((0 + p0) + p1) + p2, / 3 recomputed from the corners by every vertex of
 // the triangle (the same FP32 value each time, so no separate dual pass);
-// other vertices keep theirs.  Reads `in`, writes `out` (ping-pong buffers).
-__global__ void k_tape_step(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ adj, int stride,
-                            const uint8_t* __restrict__ deg, const uint8_t* __restrict__ mv,
-                            const float4* __restrict__ in, int V, float4* __restrict__ out) {
+// other vertices keep theirs.  Reads `in`, writes `out` (ping-pong buffers)
+// and, when given, a checkpoint copy `ck` and float3 positions `pos3`.
+__global__ void __launch_bounds__(kStepThreads) k_tape_step(const uint2* __restrict__ ring, int stride,
+                                                            const uint8_t* __restrict__ deg,
+                                                            const uint8_t* __restrict__ mv,
+                                                            const float4* __restrict__ in, int V,
+                                                            float4* __restrict__ out, float4* __restrict__ ck,
+                                                            float* __restrict__ pos3) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    if (!mv[v]) {
-        out[v] = in[v];
-        return;
+    const float4 pv = in[v];
+    float4 r = pv;
+    if (mv[v]) r = smooth_vertex(ring, stride, deg[v], in, pv, v);
+    out[v] = r;
+    if (ck) ck[v] = r;
+    if (pos3) {
+        pos3[3 * v] = r.x;
+        pos3[3 * v + 1] = r.y;
+        pos3[3 * v + 2] = r.z;
     }
-    const int n = deg[v];
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for (int k = 0; k < n; k++) {
-        const uint32_t t = adj[(int64_t)k * stride + v];
-        const float4 a = in[idx[3 * t]], b = in[idx[3 * t + 1]], c = in[idx[3 * t + 2]];
-        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        cx = cx + a.x;
-        cy = cy + a.y;
-        cz = cz + a.z;
-        cx = cx + b.x;
-        cy = cy + b.y;
-        cz = cz + b.z;
-        cx = cx + c.x;
-        cy = cy + c.y;
-        cz = cz + c.z;
-        sx = sx + cx / 3.0f;
-        sy = sy + cy / 3.0f;
-        sz = sz + cz / 3.0f;
-    }
-    const float fc = (float)n;
-    out[v] = make_float4(sx / fc, sy / fc, sz / fc, 0.0f);
 }
 
 __global__ void k_tape_out(const float4* __restrict__ p4, int V, float* __restrict__ pos) {
@@ -174,43 +278,48 @@ __global__ void k_tape_out(const float4* __restrict__ p4, int V, float* __restri
 // ------------------------------------------------------------------------
 // backward
 // ------------------------------------------------------------------------
-__global__ void k_bwd_init(const float* __restrict__ g3, int V, float4* __restrict__ g) {
+// The backward's working record of vertex v: .w = [v moved] and .xyz =
+// dL/dp_v scaled by bscale_v = 1 / (3 |ring(v)|) for moved vertices, so that
+// a row of the transposed step needs nothing of its neighbours but their
+// records.  With no smoothing (`scaled` = 0) the record is dL/dp_v itself.
+__global__ void k_bwd_init(const float* __restrict__ g3, int V, const uint8_t* __restrict__ mv,
+                           const float* __restrict__ bscale, int scaled, float4* __restrict__ g) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    g[v] = make_float4(g3[3 * v], g3[3 * v + 1], g3[3 * v + 2], 0.0f);
+    const float sc = scaled ? bscale[v] : 1.0f;
+    g[v] = make_float4(g3[3 * v] * sc, g3[3 * v + 1] * sc, g3[3 * v + 2] * sc, scaled ? (float)mv[v] : 0.0f);
 }
 
-// One transposed smoothing iteration in one launch: dL/dp_v = [v not moved]
-// dL/dp'_v + (sum over ring(v) of dL/dc_t) / 3, where dL/dc_t = the sum over
-// the corners u of t that the primal step moves of dL/dp'_u / |ring(u)|,
-// from +0 in corner order, recomputed by every vertex of the triangle (the
-// same FP32 value each time).  Reads `in`, writes `out` (ping-pong buffers).
-__global__ void k_bwd_step(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ adj, int stride,
-                           const uint8_t* __restrict__ deg, const uint8_t* __restrict__ mv,
-                           const float4* __restrict__ in, int V, float4* __restrict__ out) {
+// One transposed smoothing iteration (k_tape_adj states the sum): from the
+// records of the step after, dL/dp_v = m_self x_v + sum over row v of
+// m_uv [u moved] x_u, m_self = |ring(v)| for a moved v (its own record is
+// scaled) and 1 otherwise.  Writes v's record for the step before, or
+// dL/dp_v itself after the first iteration (`last`).
+__global__ void __launch_bounds__(kStepThreads) k_bwd_step(const uint32_t* __restrict__ nbr, int stride,
+                                                           const uint8_t* __restrict__ nnz,
+                                                           const uint8_t* __restrict__ deg,
+                                                           const uint8_t* __restrict__ mv,
+                                                           const float* __restrict__ bscale,
+                                                           const float4* __restrict__ in, int V, int last,
+                                                           float4* __restrict__ out) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    const int n = deg[v];
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    const int n = nnz[v];
+    const bool moved = mv[v] != 0;
+    const float4 own = in[v];
+    const float ms = moved ? (float)deg[v] : 1.0f;
+    float sx = ms * own.x, sy = ms * own.y, sz = ms * own.z;
+#pragma unroll 4
     for (int k = 0; k < n; k++) {
-        const uint32_t t = adj[(int64_t)k * stride + v];
-        float ax = 0.0f, ay = 0.0f, az = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const uint32_t u = idx[3 * t + c];
-            if (!mv[u]) continue;
-            const float4 q = in[u];
-            const float nu = (float)deg[u];
-            ax = ax + q.x / nu;
-            ay = ay + q.y / nu;
-            az = az + q.z / nu;
-        }
-        sx = sx + ax;
-        sy = sy + ay;
-        sz = sz + az;
+        const uint32_t e = nbr[(int64_t)k * stride + v];
+        const float4 q = in[e & kVidMask];
+        const float w = (float)(e >> kVidBits) * q.w;
+        sx = __builtin_fmaf(w, q.x, sx);
+        sy = __builtin_fmaf(w, q.y, sy);
+        sz = __builtin_fmaf(w, q.z, sz);
     }
-    const float4 own = mv[v] ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : in[v];
-    out[v] = make_float4(own.x + sx / 3.0f, own.y + sy / 3.0f, own.z + sz / 3.0f, 0.0f);
+    const float sc = last ? 1.0f : bscale[v];
+    out[v] = make_float4(sx * sc, sy * sc, sz * sc, moved ? 1.0f : 0.0f);
 }
 
 // dL/d(density) per sample: the sample is s0 of its own sign-changing edges
@@ -220,10 +329,12 @@ __global__ void k_bwd_step(const uint32_t* __restrict__ idx, const uint32_t* __r
 // moves along its edge axis by mu (grid units).
 __global__ void k_bwd_place(const float* __restrict__ dens, const uint32_t* __restrict__ vid0,
                             const float4* __restrict__ g, int dim, float* __restrict__ out) {
-    const int64_t n3 = (int64_t)dim * dim * dim;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t udim = (uint32_t)dim;
+    const uint32_t n3 = udim * udim * udim;  // dim <= 256
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n3) return;
-    const int z = (int)(i % dim), y = (int)((i / dim) % dim), x = (int)(i / ((int64_t)dim * dim));
+    const uint32_t row = i / udim;
+    const int z = (int)(i - row * udim), y = (int)(row % udim), x = (int)(row / udim);
     const int64_t st[3] = {(int64_t)dim * dim, (int64_t)dim, 1};
     const int co[3] = {x, y, z};
     const float di = dens[i];
@@ -566,7 +677,17 @@ __global__ void __launch_bounds__(1024) k_sum(const double* __restrict__ x, int 
     __shared__ double s[1024];
     const int tid = threadIdx.x;
     double a = 0.0;
-    for (int i = tid; i < n; i += 1024) a += x[i];
+    int i = tid;
+    // the same order as one add per stride step; the loads of 8 steps are
+    // issued together
+    for (; i + 7 * 1024 < n; i += 8 * 1024) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = x[i + j * 1024];
+#pragma unroll
+        for (int j = 0; j < 8; j++) a += v[j];
+    }
+    for (; i < n; i += 1024) a += x[i];
     s[tid] = a;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1) {
@@ -590,7 +711,10 @@ struct bmf_tape {
     uint32_t* vedge = nullptr;
     uint8_t* boundary = nullptr;
     uint32_t* idx = nullptr;
-    uint32_t* adj = nullptr;
+    uint2* ring = nullptr;
+    uint32_t* nbr = nullptr;
+    uint8_t* nnz = nullptr;
+    float* bscale = nullptr;
     uint8_t* deg = nullptr;
     uint8_t* mv = nullptr;
     float4* p4 = nullptr;
@@ -728,23 +852,25 @@ int nn_query(bmf_tape* t, const NNBuffers& b, int64_t n, float* dldp, int32_t* n
 
 // Runs k smoothing iterations starting from `a` (float4 [V]), ping-ponging
 // between a and b, one launch per iteration; the state after iteration m is
-// kept in checkpoint slot (from + m + 1) / every when that divides (from >= 0).
-// Returns the buffer that holds the final state in *final.
-int smooth_steps(bmf_tape* t, float4* a, float4* b, int k, hipStream_t s, float4** final, int ckpt_from = -1) {
+// kept in checkpoint slot (from + m + 1) / every when that divides (from >= 0),
+// and the final state is written as float3 to pos3.
+int smooth_steps(bmf_tape* t, float4* a, float4* b, int k, hipStream_t s, float* pos3, int ckpt_from = -1) {
     float4* cur = a;
     float4* nxt = b;
+    if (k == 0) {
+        hipLaunchKernelGGL(k_tape_out, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, a, t->V, pos3);
+        BMF_HIP(hipGetLastError());
+        return BMF_OK;
+    }
     for (int m = 0; m < k; m++) {
-        hipLaunchKernelGGL(k_tape_step, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->idx, t->adj, (int)t->cap_v,
-                           t->deg, t->mv, cur, t->V, nxt);
+        float4* ck = nullptr;
+        if (ckpt_from >= 0 && (ckpt_from + m + 1) % t->ckpt_every == 0)
+            ck = t->ckpt + (int64_t)((ckpt_from + m + 1) / t->ckpt_every) * t->cap_v;
+        hipLaunchKernelGGL(k_tape_step, dim3(step_blocks(t->V)), dim3(kStepThreads), 0, s, t->ring, (int)t->cap_v,
+                           t->deg, t->mv, cur, t->V, nxt, ck, m == k - 1 ? pos3 : nullptr);
         BMF_HIP(hipGetLastError());
         std::swap(cur, nxt);
-        if (ckpt_from >= 0 && (ckpt_from + m + 1) % t->ckpt_every == 0) {
-            const int64_t slot = (ckpt_from + m + 1) / t->ckpt_every;
-            BMF_HIP(hipMemcpyAsync(t->ckpt + slot * t->cap_v, cur, sizeof(float4) * (size_t)t->V,
-                                   hipMemcpyDeviceToDevice, s));
-        }
     }
-    *final = cur;
     return BMF_OK;
 }
 
@@ -820,7 +946,8 @@ int bmf_tape_create(bmf_tape** out, bmf_ctx* ctx, int32_t max_dim, int64_t max_v
                     int32_t checkpoint_every) {
     if (!out || !ctx) return BMF_EINVAL;
     *out = nullptr;
-    if (max_dim < 2 || max_dim > BMF_MAX_DIM || max_vertices < 1 || max_indices < 3 || checkpoint_every < 1)
+    if (max_dim < 2 || max_dim > BMF_MAX_DIM || max_vertices < 1 || max_vertices > kMaxTapeVertices ||
+        max_indices < 3 || checkpoint_every < 1)
         return BMF_EINVAL;
     BMF_HIP(hipSetDevice(ctx->device));
     bmf_tape* t = new (std::nothrow) bmf_tape();
@@ -834,7 +961,9 @@ int bmf_tape_create(bmf_tape** out, bmf_ctx* ctx, int32_t max_dim, int64_t max_v
     const bool ok = dalloc(&t->dens, n3) == hipSuccess && dalloc(&t->vid0, n3) == hipSuccess &&
                     dalloc(&t->vedge, max_vertices) == hipSuccess && dalloc(&t->boundary, max_vertices) == hipSuccess &&
                     dalloc(&t->idx, max_indices) == hipSuccess &&
-                    dalloc(&t->adj, (int64_t)kMaxRing * max_vertices) == hipSuccess &&
+                    dalloc(&t->ring, (int64_t)kMaxRing * max_vertices) == hipSuccess &&
+                    dalloc(&t->nbr, (int64_t)kMaxNbr * max_vertices) == hipSuccess &&
+                    dalloc(&t->nnz, max_vertices) == hipSuccess && dalloc(&t->bscale, max_vertices) == hipSuccess &&
                     dalloc(&t->deg, max_vertices) == hipSuccess && dalloc(&t->mv, max_vertices) == hipSuccess &&
                     dalloc(&t->p4, max_vertices) == hipSuccess && dalloc(&t->q4, max_vertices) == hipSuccess &&
                     dalloc(&t->pos, 3 * max_vertices) == hipSuccess &&
@@ -864,7 +993,10 @@ void bmf_tape_destroy(bmf_tape* t) {
     dfree(t->vedge);
     dfree(t->boundary);
     dfree(t->idx);
-    dfree(t->adj);
+    dfree(t->ring);
+    dfree(t->nbr);
+    dfree(t->nnz);
+    dfree(t->bscale);
     dfree(t->deg);
     dfree(t->mv);
     dfree(t->p4);
@@ -960,35 +1092,31 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
         t->ckpt_slots = slots;
     }
     const int V = t->V, T = t->T;
-    BMF_HIP(hipMemcpyAsync(t->dens, A.dens, sizeof(float) * n3, hipMemcpyDeviceToDevice, s));
+    BMF_HIP(hipMemsetAsync(t->err, 0, sizeof(int), s));
+    const bool smooth = t->K > 0 && T > 0;  // per-cell triangle bases and masks exist for chunks with K > 0
+    {
+        const int64_t most = std::max<int64_t>(n3, std::max<int64_t>(3 * (int64_t)T, V));
+        const unsigned blocks = (unsigned)std::min<int64_t>((most + kThreads - 1) / kThreads, 4096);
+        hipLaunchKernelGGL(k_tape_take, dim3(blocks), dim3(kThreads), 0, s, A.dens, A.vedge + r.vertex_offset,
+                           A.boundary + r.vertex_offset, A.idx + r.index_offset, A.pos + 3 * r.vertex_offset, n3, V,
+                           3 * (int64_t)T, dim, t->dens, t->vedge, t->boundary, t->idx, t->p4, t->ckpt,
+                           smooth ? nullptr : t->pos, t->vid0);
+        BMF_HIP(hipGetLastError());
+    }
     if (V > 0) {
-        BMF_HIP(hipMemcpyAsync(t->vedge, A.vedge + r.vertex_offset, sizeof(uint32_t) * V, hipMemcpyDeviceToDevice, s));
-        BMF_HIP(hipMemcpyAsync(t->boundary, A.boundary + r.vertex_offset, V, hipMemcpyDeviceToDevice, s));
-        BMF_HIP(hipMemcpyAsync(t->idx, A.idx + r.index_offset, sizeof(uint32_t) * 3 * T, hipMemcpyDeviceToDevice, s));
-        hipLaunchKernelGGL(k_tape_init, dim3(blocks_for(V)), dim3(kThreads), 0, s, A.pos + 3 * r.vertex_offset, V,
-                           t->p4);
-        hipLaunchKernelGGL(k_tape_vid0, dim3(blocks_for(V)), dim3(kThreads), 0, s, t->vedge, V, dim, t->vid0);
-        BMF_HIP(hipMemsetAsync(t->err, 0, sizeof(int), s));
-        if (t->K > 0 && T > 0) {
-            // per-cell triangle bases and masks exist for chunks with K > 0
-            hipLaunchKernelGGL(k_tape_ring, dim3(blocks_for(V)), dim3(kThreads), 0, s, t->vedge, t->boundary, A.cell_tb,
-                               A.cell_mask, dim, V, T, t->bp, (int)t->cap_v, t->adj, t->deg, t->mv, t->err);
+        if (smooth) {
+            hipLaunchKernelGGL(k_tape_adj, dim3(step_blocks(V)), dim3(kStepThreads), 0, s, t->vedge, t->boundary,
+                               A.cell_tb, A.cell_mask, t->idx, dim, V, T, t->bp, (int)t->cap_v, t->ring, t->deg, t->mv,
+                               t->nbr, t->nnz, t->bscale, t->err);
+            BMF_HIP(hipGetLastError());
+            rc = smooth_steps(t, t->p4, t->q4, t->K, s, t->pos, 0);
+            if (rc) return rc;
         } else {
             BMF_HIP(hipMemsetAsync(t->deg, 0, V, s));
             BMF_HIP(hipMemsetAsync(t->mv, 0, V, s));
-        }
-        BMF_HIP(hipGetLastError());
-        BMF_HIP(hipMemcpyAsync(t->ckpt, t->p4, sizeof(float4) * V, hipMemcpyDeviceToDevice, s));
-        float4* fin = t->p4;
-        if (t->K > 0 && T > 0) {
-            rc = smooth_steps(t, t->p4, t->q4, t->K, s, &fin, 0);
-            if (rc) return rc;
-        } else {
             for (int64_t q = 1; q < slots; q++)
                 BMF_HIP(hipMemcpyAsync(t->ckpt + q * t->cap_v, t->p4, sizeof(float4) * V, hipMemcpyDeviceToDevice, s));
         }
-        hipLaunchKernelGGL(k_tape_out, dim3(blocks_for(V)), dim3(kThreads), 0, s, fin, V, t->pos);
-        BMF_HIP(hipGetLastError());
     }
     BMF_HIP(hipEventRecord(t->ev[1], s));
     int herr = 0;
@@ -1026,11 +1154,8 @@ int bmf_tape_positions(bmf_tape* t, int32_t iteration, float* out) {
     const int64_t slot = iteration / t->ckpt_every;
     BMF_HIP(hipMemcpyAsync(t->q4, t->ckpt + slot * t->cap_v, sizeof(float4) * t->V, hipMemcpyDeviceToDevice, s));
     const int rest = t->T > 0 ? iteration - (int)(slot * t->ckpt_every) : 0;
-    float4* fin = t->q4;
-    int rc = smooth_steps(t, t->q4, t->r4, rest, s, &fin);
+    int rc = smooth_steps(t, t->q4, t->r4, rest, s, out);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_tape_out, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, fin, t->V, out);
-    BMF_HIP(hipGetLastError());
     BMF_HIP(hipStreamSynchronize(s));
     return BMF_OK;
 }
@@ -1142,13 +1267,15 @@ int bmf_backward_diff(bmf_tape* t, const float* dldp, float* dldd) {
     }
     t->timed[2] = false;
     BMF_HIP(hipEventRecord(t->ev[4], s));
-    hipLaunchKernelGGL(k_bwd_init, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, dldp, t->V, t->g);
+    const bool smooth = t->K > 0 && t->T > 0;
+    hipLaunchKernelGGL(k_bwd_init, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, dldp, t->V, t->mv, t->bscale,
+                       (int)smooth, t->g);
     float4* cur = t->g;
     float4* nxt = t->g2;
-    if (t->T > 0)
+    if (smooth)
         for (int m = t->K - 1; m >= 0; m--) {
-            hipLaunchKernelGGL(k_bwd_step, dim3(blocks_for(t->V)), dim3(kThreads), 0, s, t->idx, t->adj,
-                               (int)t->cap_v, t->deg, t->mv, cur, t->V, nxt);
+            hipLaunchKernelGGL(k_bwd_step, dim3(step_blocks(t->V)), dim3(kStepThreads), 0, s, t->nbr, (int)t->cap_v,
+                               t->nnz, t->deg, t->mv, t->bscale, cur, t->V, (int)(m == 0), nxt);
             std::swap(cur, nxt);
         }
     hipLaunchKernelGGL(k_bwd_place, dim3(blocks_for(n3)), dim3(kThreads), 0, s, t->dens, t->vid0, cur, t->dim, dldd);
