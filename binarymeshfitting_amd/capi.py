@@ -13,6 +13,14 @@ LIB_PATH = Path(__file__).resolve().parent / "libbmf.so"
 
 OK, EINVAL, ECAPACITY, EHIP, ENODEV, ESTATE = range(6)
 SAMPLER_TERRAIN, SAMPLER_SPHERE, SAMPLER_CUBOID, SAMPLER_TORUS_Z, SAMPLER_PLANE_Y, SAMPLER_CSG, SAMPLER_EXTERNAL = range(7)
+SAMPLER_NOISE = 7
+NOISE_VALUE, NOISE_SIMPLEX = 0, 1
+FRACTAL_FBM, FRACTAL_RIGID = 0, 1
+PERTURB_NONE, PERTURB_GRADIENT, PERTURB_OFFSET_XZ = 0, 1, 2
+MAX_PERTURB_OCTAVES = 24
+PRESET_TERRAIN2D, PRESET_TERRAIN2D_PERT, PRESET_TERRAIN3D, PRESET_TERRAIN3D_PERT, PRESET_WINDY3D, PRESET_WORLD = range(6)
+NOISE_FIELDS = ("noise_dims", "noise_basis", "noise_fractal", "g_scale", "y_scale", "height", "perturb",
+                "perturb_octaves", "perturb_amp", "perturb_freq", "perturb_gain")
 MODE_PLAIN, MODE_BATCHED = 0, 1
 MAX_DIM = 256
 
@@ -24,7 +32,7 @@ EXPORTS = [
     "bmf_backward_diff", "bmf_get_tuning", "bmf_set_tuning", "bmf_autotune", "bmf_set_mesh_options",
     "bmf_get_mesh_options", "bmf_qef_error_to_host", "bmf_normals_to_host", "bmf_tape_set_targets",
     "bmf_tape_loss_nearest", "bmf_loss_nearest_noalloc", "bmf_format", "bmf_lod_queue", "bmf_extract_queued", "bmf_stitch",
-    "bmf_tape_timing",
+    "bmf_tape_timing", "bmf_noise_preset",
 ]
 SPACE_GRID, SPACE_WORLD = 0, 1
 COLOR_NONE, COLOR_WHITE, COLOR_NOISE = 0, 1, 2
@@ -46,7 +54,20 @@ class SamplerDesc(C.Structure):
         ("freq", C.c_float), ("lacunarity", C.c_float), ("gain", C.c_float), ("octaves", C.c_int32),
         ("seed", C.c_uint32), ("spheres", C.POINTER(C.c_float)), ("n_spheres", C.c_int32),
         ("boxes", C.POINTER(C.c_float)), ("n_boxes", C.c_int32), ("densities", C.c_void_p),
+        ("noise_dims", C.c_int32), ("noise_basis", C.c_int32), ("noise_fractal", C.c_int32),
+        ("g_scale", C.c_float), ("y_scale", C.c_float), ("height", C.c_float), ("perturb", C.c_int32),
+        ("perturb_octaves", C.c_int32), ("perturb_amp", C.c_float), ("perturb_freq", C.c_float),
+        ("perturb_gain", C.c_float),
     ]
+
+
+def noise_preset(which, world_size=256.0, seed=1337):
+    """bmf_noise_preset as a dict of set_sampler keyword arguments (the
+    reference's block sampler `which` stands for, with the project's noise)."""
+    d = SamplerDesc()
+    check(lib().bmf_noise_preset(which, world_size, seed & 0xFFFFFFFF, C.byref(d)), "bmf_noise_preset")
+    keys = ("kind", "world_size", "freq", "lacunarity", "gain", "octaves", "seed") + NOISE_FIELDS
+    return {k: getattr(d, k) for k in keys}
 
 
 class ChunkDesc(C.Structure):
@@ -154,6 +175,7 @@ def lib():
         L.bmf_loss_nearest.argtypes = [tp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, P(C.c_float)]
         L.bmf_backward_diff.argtypes = [tp, C.c_void_p, C.c_void_p]
         L.bmf_tape_timing.argtypes = [tp, P(C.c_float)]
+        L.bmf_noise_preset.argtypes = [C.c_int32, C.c_float, C.c_uint32, P(SamplerDesc)]
         L.bmf_tape_set_targets.argtypes = [tp, C.c_void_p, C.c_int64]
         L.bmf_tape_loss_nearest.argtypes = [tp, C.c_void_p, C.c_void_p, P(C.c_float)]
         L.bmf_loss_nearest_noalloc.argtypes = [tp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -219,13 +241,19 @@ class Context:
             pass
 
     def set_sampler(self, kind=SAMPLER_TERRAIN, world_size=256.0, amp=24.0, y0=32.0, freq=0.02, lacunarity=2.0,
-                    gain=0.5, octaves=5, seed=1337, spheres=None, boxes=None, densities=None):
-        """densities: a DeviceArray of packed dim^3 blocks for SAMPLER_EXTERNAL."""
+                    gain=0.5, octaves=5, seed=1337, spheres=None, boxes=None, densities=None, **noise):
+        """densities: a DeviceArray of packed dim^3 blocks for SAMPLER_EXTERNAL;
+        noise: the NOISE sampler's fields (NOISE_FIELDS, e.g. from noise_preset)."""
         sp = np.ascontiguousarray(spheres if spheres is not None else np.zeros((0, 4)), np.float32)
         bx = np.ascontiguousarray(boxes if boxes is not None else np.zeros((0, 6)), np.float32)
+        unknown = set(noise) - set(NOISE_FIELDS)
+        if unknown:
+            raise TypeError(f"unknown sampler fields {sorted(unknown)}")
         d = SamplerDesc(kind, world_size, amp, y0, freq, lacunarity, gain, octaves, seed & 0xFFFFFFFF,
                         _ptr(sp, C.c_float), len(sp), _ptr(bx, C.c_float), len(bx),
                         densities.ptr if densities is not None else None)
+        for k, v in noise.items():
+            setattr(d, k, v)
         self._sampler_keep = densities
         check(lib().bmf_set_sampler(self.h, C.byref(d)), "bmf_set_sampler")
 

@@ -85,7 +85,7 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
     BMF_MARK(0);
     BMF_HIP(launch_plan(A, n, allow_cull, reset, s));
     BMF_MARK(1);
-    BMF_HIP(launch_sample(A, n, sample_blocks, 1, c->h_sampler.kind, s));
+    BMF_HIP(launch_sample(A, n, sample_blocks, 1, c->h_sampler.kind, c->h_sampler.noise_dims == 2, s));
     BMF_MARK(2);
     const bool quads = A.mesh_mode == BMF_MESH_QUADS;
     hipStream_t cs[4];
@@ -154,7 +154,7 @@ int check_desc(const bmf_chunk_desc& d) {
 
 int build_sampler(const bmf_sampler_desc* s, SamplerDev* out) {
     if (!s || !out) return BMF_EINVAL;
-    if (s->kind < BMF_SAMPLER_TERRAIN || s->kind > BMF_SAMPLER_EXTERNAL) return BMF_EINVAL;
+    if (s->kind < BMF_SAMPLER_TERRAIN || s->kind > BMF_SAMPLER_NOISE) return BMF_EINVAL;
     if (s->kind == BMF_SAMPLER_EXTERNAL && !s->densities) return BMF_EINVAL;
     if (s->kind == BMF_SAMPLER_CSG &&
         (s->n_spheres < 0 || s->n_boxes < 0 || (s->n_spheres && !s->spheres) || (s->n_boxes && !s->boxes)))
@@ -165,7 +165,7 @@ int build_sampler(const bmf_sampler_desc* s, SamplerDev* out) {
     d.amp = s->amp;
     d.y0 = s->y0;
     d.seed = s->seed;
-    if (s->kind == BMF_SAMPLER_TERRAIN) {
+    if (s->kind == BMF_SAMPLER_TERRAIN || s->kind == BMF_SAMPLER_NOISE) {
         if (s->octaves < 1 || s->octaves > BMF_MAX_OCTAVES) return BMF_EINVAL;
         if (!finite(s->amp) || !finite(s->y0) || !finite(s->freq) || !finite(s->lacunarity) || !finite(s->gain))
             return BMF_EINVAL;
@@ -192,6 +192,55 @@ int build_sampler(const bmf_sampler_desc* s, SamplerDev* out) {
             float rf = (float)r;
             if ((double)rf < r) rf = std::nextafter(rf, INFINITY);
             d.reach[k] = rf;
+        }
+    }
+    if (s->kind == BMF_SAMPLER_NOISE) {
+        const bool dims_ok = s->noise_dims == 2 || s->noise_dims == 3;
+        const bool kinds_ok = (s->noise_basis == BMF_NOISE_VALUE || s->noise_basis == BMF_NOISE_SIMPLEX) &&
+                              (s->noise_fractal == BMF_FRACTAL_FBM || s->noise_fractal == BMF_FRACTAL_RIGID) &&
+                              s->perturb >= BMF_PERTURB_NONE && s->perturb <= BMF_PERTURB_OFFSET_XZ;
+        if (!dims_ok || !kinds_ok) return BMF_EINVAL;
+        if (s->perturb == BMF_PERTURB_OFFSET_XZ && s->noise_dims != 3) return BMF_EINVAL;
+        const int po = s->perturb == BMF_PERTURB_NONE ? 0 : s->perturb_octaves;
+        if (po < 0 || po > BMF_MAX_PERTURB_OCTAVES) return BMF_EINVAL;
+        if (!finite(s->g_scale) || !finite(s->y_scale) || !finite(s->height) || !finite(s->perturb_amp) ||
+            !finite(s->perturb_freq) || !finite(s->perturb_gain))
+            return BMF_EINVAL;
+        d.noise_dims = s->noise_dims;
+        d.noise_basis = s->noise_basis;
+        d.noise_fractal = s->noise_fractal;
+        d.g_scale = s->g_scale;
+        d.y_scale = s->y_scale;
+        d.height = s->height;
+        d.perturb = s->perturb;
+        d.perturb_octaves = po;
+        d.perturb_amp = s->perturb_amp;
+        // perturbation constants: FP32 chains (frequency doubles per octave)
+        float f = s->perturb_freq, a = s->perturb == BMF_PERTURB_GRADIENT ? s->perturb_amp : 1.0f, norm = 0.0f;
+        for (int k = 0; k < po; k++) {
+            d.pf[k] = f;
+            d.pa[k] = a;
+            norm = norm + a;
+            f = f * 2.0f;
+            a = a * s->perturb_gain;
+        }
+        d.pinv = norm > 0.0f ? 1.0f / norm : 0.0f;
+        if (s->perturb == BMF_PERTURB_OFFSET_XZ && po > 0 && !(norm > 0.0f && finite(norm))) return BMF_EINVAL;
+        // bounds of n for the chunk culling: FBM is clamped to [-1, 1]; the
+        // rigid sum t_0 - sum a_o t_o (t in [0, 1]) is smallest, step by step
+        // in FP32 (every fma monotone), at t_0 = 0 and t_o = 1 when a_o >= 0
+        d.cull = 1;
+        if (s->noise_fractal == BMF_FRACTAL_FBM) {
+            d.nlo = -1.0f;
+            d.nhi = 1.0f;
+        } else {
+            float r = 0.0f;
+            for (int o = 1; o < s->octaves; o++) {
+                if (!(d.ao[o] >= 0.0f)) d.cull = 0;
+                r = std::fmaf(-1.0f, d.ao[o], r);
+            }
+            d.nlo = r;
+            d.nhi = 1.0f;
         }
     }
     if (s->kind == BMF_SAMPLER_CSG) {
@@ -347,6 +396,81 @@ void bmf_destroy(bmf_ctx* c) {
         if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+int bmf_noise_preset(int32_t which, float world_size, uint32_t seed, bmf_sampler_desc* out) {
+    if (!out || which < BMF_PRESET_TERRAIN2D || which > BMF_PRESET_WORLD) return BMF_EINVAL;
+    bmf_sampler_desc d{};
+    d.kind = BMF_SAMPLER_NOISE;
+    d.world_size = world_size;
+    d.seed = seed;
+    // FastNoiseSIMD's defaults, which the block functions keep unless they set them
+    const float freq = 0.01f;
+    d.freq = freq;
+    d.lacunarity = 2.0f;
+    d.gain = 0.5f;
+    d.octaves = 3;
+    d.noise_dims = 3;
+    d.noise_basis = BMF_NOISE_VALUE;
+    d.noise_fractal = BMF_FRACTAL_FBM;
+    d.g_scale = 1.0f;
+    d.y_scale = 1.0f;
+    d.height = 1.0f;
+    d.perturb = BMF_PERTURB_NONE;
+    d.perturb_gain = 0.5f;
+    auto gradient = [&](int octaves, float amp, float f, float gain) {
+        d.perturb = BMF_PERTURB_GRADIENT;
+        d.perturb_octaves = octaves;
+        d.perturb_amp = amp / freq;
+        d.perturb_freq = f * freq;
+        d.perturb_gain = gain;
+    };
+    switch (which) {
+        case BMF_PRESET_TERRAIN2D:  // NoiseSampler.cpp:115-125, 139
+            d.noise_dims = 2;
+            d.octaves = 12;
+            d.height = 64.0f;
+            break;
+        case BMF_PRESET_TERRAIN2D_PERT:  // :150-166 without properties
+            d.noise_dims = 2;
+            d.g_scale = 0.25f;
+            d.height = 64.0f;
+            gradient(20, 1.0f, 0.5f, 0.5f);
+            break;
+        case BMF_PRESET_WORLD:  // WorldOctree.cpp:47-54 into terrain2d_pert_block
+            d.noise_dims = 2;
+            d.g_scale = 0.25f;
+            d.height = 75.0f;
+            gradient(13, 0.87f, 0.585f, 0.488f);
+            break;
+        case BMF_PRESET_TERRAIN3D:  // :198-207, 219: rigid value noise, dy scaled by ym
+            d.octaves = 4;
+            d.noise_fractal = BMF_FRACTAL_RIGID;
+            d.g_scale = 0.15f;
+            d.y_scale = 0.5f;
+            break;
+        case BMF_PRESET_TERRAIN3D_PERT:  // :231-243, 255
+            d.octaves = 8;
+            d.noise_basis = BMF_NOISE_SIMPLEX;
+            d.noise_fractal = BMF_FRACTAL_RIGID;
+            d.g_scale = 0.15f;
+            d.height = 48.0f;
+            gradient(3, 1.0f, 0.05f, 0.5f);
+            break;
+        case BMF_PRESET_WINDY3D:  // :267-305, 313: offsets from a 1-octave field at p * 0.01
+            d.octaves = 4;
+            d.noise_basis = BMF_NOISE_SIMPLEX;
+            d.g_scale = 0.25f;
+            d.y_scale = 0.5f;
+            d.height = world_size * 0.5f;
+            d.perturb = BMF_PERTURB_OFFSET_XZ;
+            d.perturb_octaves = 1;
+            d.perturb_amp = 1.0f;
+            d.perturb_freq = 0.01f * freq;
+            break;
+    }
+    *out = d;
+    return BMF_OK;
 }
 
 int bmf_set_sampler(bmf_ctx* c, const bmf_sampler_desc* s) {
@@ -600,7 +724,7 @@ static int stage_one(bmf_ctx* c, const bmf_chunk_desc* d, bool scan) {
     const Arenas& A = c->A;
     BMF_HIP(launch_plan(A, 1, 0, 1, c->stream));
     BMF_HIP(launch_sample(A, 1, (int)std::min<int64_t>(ceil_div(ns, A.sample_tile), c->persistent_blocks), 0,
-                          c->h_sampler.kind, c->stream));
+                          c->h_sampler.kind, c->h_sampler.noise_dims == 2, c->stream));
     if (scan) {
         BMF_HIP(launch_count(A, 1, d->dim, row_word_class(d->dim), c->stream));
         BMF_HIP(launch_scan(A, 1, 1, c->stream));

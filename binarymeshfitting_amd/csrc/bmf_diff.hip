@@ -1065,7 +1065,8 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
     hipError_t e = launch_plan(A, 1, 0, 1, s);
     if (e == hipSuccess)
         e = launch_sample(A, 1, (int)std::min<int64_t>((n3 + A.sample_tile - 1) / A.sample_tile, c->persistent_blocks),
-                          0, densities ? (int)BMF_SAMPLER_EXTERNAL : (int)c->h_sampler.kind, s);
+                          0, densities ? (int)BMF_SAMPLER_EXTERNAL : (int)c->h_sampler.kind,
+                          !densities && c->h_sampler.noise_dims == 2, s);
     if (e == hipSuccess) e = launch_count(A, 1, dim * kMaxPlaneSplit, wcls, s);
     if (e == hipSuccess) e = launch_refine(A, 1, dim * kMaxPlaneSplit, s);
     if (e == hipSuccess) e = launch_scan(A, 1, 1, s);

@@ -43,6 +43,17 @@ struct SamplerDev {
     const float* spheres;       // device, n_spheres x 4
     const float* boxes;         // device, n_boxes x 6
     const float* ext;           // EXTERNAL: caller's densities, chunks packed in descriptor order
+    // NOISE: basis, fractal, density map and perturbation (bmf_sampler_desc)
+    int32_t noise_dims, noise_basis, noise_fractal;
+    float g_scale, y_scale, height;
+    int32_t perturb, perturb_octaves;
+    float perturb_amp;
+    float pf[BMF_MAX_PERTURB_OCTAVES];  // perturbation frequency f * 2^k (FP32 chain)
+    float pa[BMF_MAX_PERTURB_OCTAVES];  // GRADIENT: amp * gain^k; OFFSET_XZ: gain^k
+    float pinv;                         // OFFSET_XZ: 1 / sum of pa
+    // k_plan's chunk culling: n lies in [nlo, nhi] for every sample (cull = 0: no bound)
+    float nlo, nhi;
+    int32_t cull;
 };
 
 // Per-chunk state in device memory.
@@ -158,7 +169,8 @@ inline int row_word_class(int dim) { return 1 << (((dim + 63) >> 6) - 1); }
 hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, hipStream_t s);
 // adaptive: terrain samples stop adding octaves once their label is decided
 // kind: the sampler the device struct holds for this launch (BMF_SAMPLER_*)
-hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int kind, hipStream_t s);
+// slab: the sampler is a 2D noise field (its columns are evaluated once per tile)
+hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int kind, int slab, hipStream_t s);
 // wclasses: bit W-1 set when the batch holds chunks with ceil(dim/64) == W.
 hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
 // cs[W-1]: the stream of row-word class W

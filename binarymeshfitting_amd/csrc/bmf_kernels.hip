@@ -84,6 +84,18 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
             const float yhi = (float)(d.dim - 1) * m.delta + m.opos[1];
             if (ylo - S->y0 > a) m.state = kStateAir;
             else if (yhi - S->y0 < -a) m.state = kStateSolid;
+        } else if (allow_cull && S->kind == BMF_SAMPLER_NOISE && S->cull) {
+            // d = (-dy) - n h with dy = (y g) ym monotone in y and n in
+            // [nlo, nhi]: every FP32 step is monotone, so the extremes of d
+            // over the chunk come from the end values
+            const float ylo = m.opos[1];
+            const float yhi = (float)(d.dim - 1) * m.delta + m.opos[1];
+            const float dy0 = (ylo * S->g_scale) * S->y_scale, dy1 = (yhi * S->g_scale) * S->y_scale;
+            const float p0 = S->nlo * S->height, p1 = S->nhi * S->height;
+            const float dmax = (-fminf(dy0, dy1)) - fminf(p0, p1);
+            const float dmin = (-fmaxf(dy0, dy1)) - fmaxf(p0, p1);
+            if (dmax < 0.0f) m.state = kStateAir;
+            else if (dmin >= 0.0f) m.state = kStateSolid;
         }
         m.csg_ns = m.csg_nb = 0;  // k_csg_cull fills them
         const int64_t ns = (int64_t)d.dim * d.dim * d.dim;
@@ -330,9 +342,11 @@ __device__ __forceinline__ void sample_coords(const ChunkGeom& G, uint32_t i, ui
 
 // Sampling modes, fixed per launch by the host: adaptive octaves for the
 // terrain sampler; the CSG union with the chunk's culled primitives staged in
-// LDS once per tile; every other case (external densities, implicit shapes,
-// the terrain in full) evaluated directly.
-constexpr int kSampleGeneric = 0, kSampleAdaptive = 1, kSampleCsg = 2;
+// LDS once per tile; a 2D noise field with the noise of the tile's (x, z)
+// columns evaluated once into LDS and broadcast along y; every other case
+// (external densities, implicit shapes, 3D noise fields, the terrain in full)
+// evaluated directly.
+constexpr int kSampleGeneric = 0, kSampleAdaptive = 1, kSampleCsg = 2, kSampleSlab = 3;
 // Adaptive mode: undecided samples of a wave (of 64) from which phase 0
 // evaluates octave 0 in place rather than queueing them: queueing costs a
 // push, a pop and the sample coordinates again, in-place evaluation the
@@ -351,6 +365,10 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
     constexpr bool adapt = MODE == kSampleAdaptive;
     constexpr int kQ = adapt ? kTile : 1;                       // octave queues (adaptive mode only)
     constexpr int kC = MODE == kSampleCsg ? 2 * kCsgStage : 1;  // staged CSG primitives
+    // 2D noise columns of a tile: its samples span at most kTile / dim^2 + 2
+    // x-planes of dim columns, i.e. at most kTile / dim + 2 dim <= kTile / 2 + 2 BMF_MAX_DIM
+    constexpr int kSlab = MODE == kSampleSlab ? kTile / 2 + 2 * BMF_MAX_DIM : 1;
+    __shared__ float s_slab[kSlab];
     __shared__ int64_t s_tile;
     __shared__ uint32_t s_qn[2];
     __shared__ uint16_t s_qi[2][kQ];
@@ -408,6 +426,23 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                 stage_csg(*S, A.csg_list + (int64_t)j * A.csg_stride, M.csg_ns, M.csg_nb, s_csg);
                 __syncthreads();
             }
+        }
+        uint32_t slab_x0 = 0;
+        if constexpr (MODE == kSampleSlab) {
+            const uint32_t last = min(first + (uint32_t)kTile, ns) - 1u;
+            slab_x0 = first / dim2;
+            const uint32_t ncol = (last / dim2 - slab_x0 + 1u) * dim;
+            if (ncol > (uint32_t)kSlab) {
+                if (threadIdx.x == 0) A.ctr->error = 12;  // unreachable by the bound above
+            } else {
+                for (uint32_t q = threadIdx.x; q < ncol; q += blockDim.x) {
+                    const uint32_t qx = q / dim, qz = q - qx * dim;
+                    float px, py, pz;
+                    sample_position(G, slab_x0 + qx, 0u, qz, px, py, pz);
+                    s_slab[q] = noise_value(*S, px, 0.0f, pz);
+                }
+            }
+            __syncthreads();
         }
         // ---- phase 0: every sample of the tile ----
         // A thread's sample advances by kSampleThreads per round: its (x, y, z)
@@ -474,7 +509,10 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                 float px, py, pz;
                 sample_position(G, cx, cy, cz, px, py, pz);
                 float d;
-                if (staged) d = csg_density_lds(s_csg, M.csg_ns, M.csg_nb, px, py, pz);
+                if constexpr (MODE == kSampleSlab) {
+                    const uint32_t q = (cx - slab_x0) * dim + cz;
+                    d = noise_density_from(*S, q < (uint32_t)kSlab ? s_slab[q] : 0.0f, py);
+                } else if (staged) d = csg_density_lds(s_csg, M.csg_ns, M.csg_nb, px, py, pz);
                 else if (S->kind == BMF_SAMPLER_EXTERNAL) d = S->ext[A.ext_base + M.dens_off + i];
                 else if (S->kind == BMF_SAMPLER_CSG && A.csg_list)
                     d = csg_density_list(*S, A.csg_list + (int64_t)j * A.csg_stride, M.csg_ns, M.csg_nb, px, py, pz);
@@ -1615,15 +1653,17 @@ hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, 
     if (A.csg_list) hipLaunchKernelGGL(k_csg_cull, dim3((unsigned)(n < 2048 ? n : 2048)), dim3(256), 0, s, A, n);
     return hipGetLastError();
 }
-hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int kind, hipStream_t s) {
+hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int kind, int slab, hipStream_t s) {
     const int mode = adaptive && kind == BMF_SAMPLER_TERRAIN ? kSampleAdaptive
                      : kind == BMF_SAMPLER_CSG && A.csg_list  ? kSampleCsg
+                     : kind == BMF_SAMPLER_NOISE && slab      ? kSampleSlab
                                                               : kSampleGeneric;
     const dim3 g(blocks), b(kSampleThreads);
 #define BMF_SAMPLE(R)                                                                             \
     switch (mode) {                                                                               \
         case kSampleAdaptive: hipLaunchKernelGGL((k_sample<R, kSampleAdaptive>), g, b, 0, s, A, n); break; \
         case kSampleCsg: hipLaunchKernelGGL((k_sample<R, kSampleCsg>), g, b, 0, s, A, n); break;           \
+        case kSampleSlab: hipLaunchKernelGGL((k_sample<R, kSampleSlab>), g, b, 0, s, A, n); break;         \
         default: hipLaunchKernelGGL((k_sample<R, kSampleGeneric>), g, b, 0, s, A, n); break;               \
     }
     switch (A.sample_tile) {

@@ -29,13 +29,23 @@ constexpr float kCsgFloor = -16.0f;
 
 __device__ __forceinline__ uint32_t mul24(uint32_t a, uint32_t b) { return __umul24(a, b); }
 
+// mul24 by a constant as the full-rate v_mul_u32_u24 itself: where only the
+// low bits of the product are used, the compiler otherwise drops the operand
+// mask and emits the quarter-rate 32-bit v_mul_lo_u32 (same low 24 bits).
+template <uint32_t M>
+__device__ __forceinline__ uint32_t mul24_const(uint32_t a) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(M), "v"(a));
+    return r;
+}
+
 // Gradient index of a lattice point from its combined coordinate hash
 // h = mul24(i, P1) ^ mul24(j, P2) ^ mul24(k, P3) ^ seed.
 __device__ __forceinline__ uint32_t grad_mix(uint32_t h) {
     h ^= h >> 13;
-    h = mul24(h ^ (h >> 24), kM1);
+    h = mul24_const<kM1>(h ^ (h >> 24));
     h ^= h >> 11;
-    h = mul24(h ^ (h >> 24), kM2);
+    h = mul24_const<kM2>(h ^ (h >> 24));
     h ^= h >> 16;
     return (h >> 4) & 15u;
 }
@@ -251,9 +261,135 @@ __device__ __forceinline__ float csg_density_lds(const float4* s_csg, int ns, in
     return best;
 }
 
+// ---- noise fields: the NoiseSamplers block shapes (NoiseSampler.cpp:113-318)
+// with the project's own noise.  At sample position p: c = g_scale p,
+// perturbed; n = the fractal of the basis at c; d = -((p.y g_scale) y_scale)
+// - height n.  A 2D field varies over (x, z) only.
+
+// Lattice value in [-1, 1) of the corner whose per-axis hash products are
+// a, b, c (c with the seed folded in): the simplex lattice hash mixed to 24
+// bits k, then k / 2^23 - 1 (exact).
+__device__ __forceinline__ float lattice_value(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t h = a ^ b ^ c;
+    h ^= h >> 13;
+    h = mul24_const<kM1>(h ^ (h >> 24));
+    h ^= h >> 11;
+    h = mul24_const<kM2>(h ^ (h >> 24));
+    h ^= h >> 16;
+    return (float)(h & 0xFFFFFFu) * 0x1p-23f - 1.0f;
+}
+
+// Quintic fade 6t^5 - 15t^4 + 10t^3 and the blend a + u (b - a).
+__device__ __forceinline__ float fade5(float t) {
+    const float s = __builtin_fmaf(__builtin_fmaf(t, 6.0f, -15.0f), t, 10.0f);
+    return ((t * t) * t) * s;
+}
+__device__ __forceinline__ float blend(float a, float b, float u) { return __builtin_fmaf(u, b - a, a); }
+
+// Value noise: the lattice values blended along x, then y, then z.
+__device__ __forceinline__ float value3(float x, float y, float z, uint32_t seed) {
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    const int32_t i = (int32_t)fx, j = (int32_t)fy, k = (int32_t)fz;
+    const float ux = fade5(x - fx), uy = fade5(y - fy), uz = fade5(z - fz);
+    const uint32_t a0 = mul24((uint32_t)i, kP1), a1 = mul24((uint32_t)(i + 1), kP1);
+    const uint32_t b0 = mul24((uint32_t)j, kP2), b1 = mul24((uint32_t)(j + 1), kP2);
+    const uint32_t c0 = mul24((uint32_t)k, kP3) ^ seed, c1 = mul24((uint32_t)(k + 1), kP3) ^ seed;
+    const float x00 = blend(lattice_value(a0, b0, c0), lattice_value(a1, b0, c0), ux);
+    const float x10 = blend(lattice_value(a0, b1, c0), lattice_value(a1, b1, c0), ux);
+    const float x01 = blend(lattice_value(a0, b0, c1), lattice_value(a1, b0, c1), ux);
+    const float x11 = blend(lattice_value(a0, b1, c1), lattice_value(a1, b1, c1), ux);
+    return blend(blend(x00, x10, uy), blend(x01, x11, uy), uz);
+}
+__device__ __forceinline__ float value2(float x, float z, uint32_t seed) {
+    const float fx = floorf(x), fz = floorf(z);
+    const int32_t i = (int32_t)fx, k = (int32_t)fz;
+    const float ux = fade5(x - fx), uz = fade5(z - fz);
+    const uint32_t a0 = mul24((uint32_t)i, kP1), a1 = mul24((uint32_t)(i + 1), kP1);
+    const uint32_t c0 = mul24((uint32_t)k, kP3) ^ seed, c1 = mul24((uint32_t)(k + 1), kP3) ^ seed;
+    const float x0 = blend(lattice_value(a0, 0u, c0), lattice_value(a1, 0u, c0), ux);
+    const float x1 = blend(lattice_value(a0, 0u, c1), lattice_value(a1, 0u, c1), ux);
+    return blend(x0, x1, uz);
+}
+
+__device__ __forceinline__ float noise_basis(const SamplerDev& S, float x, float y, float z, uint32_t seed) {
+    if (S.noise_dims == 2) return S.noise_basis == BMF_NOISE_VALUE ? value2(x, z, seed) : snoise3(x, 0.0f, z, seed);
+    return S.noise_basis == BMF_NOISE_VALUE ? value3(x, y, z, seed) : snoise3(x, y, z, seed);
+}
+
+// The fractal at c: FBM (normalised, clamped) or rigid t_0 - sum_{o>=1} a_o t_o,
+// t_o = 1 - |b(c f_o)|.
+__device__ __forceinline__ float noise_fractal(const SamplerDev& S, float cx, float cy, float cz) {
+    if (S.noise_fractal == BMF_FRACTAL_RIGID) {
+        float r = 0.0f;
+        for (int o = 0; o < S.octaves; o++) {
+            const float f = S.fo[o];
+            const float t = 1.0f - fabsf(noise_basis(S, cx * f, cy * f, cz * f, S.seed + (uint32_t)o));
+            r = o == 0 ? t : __builtin_fmaf(-t, S.ao[o], r);
+        }
+        return r;
+    }
+    float acc = 0.0f;
+    for (int o = 0; o < S.octaves; o++) {
+        const float f = S.fo[o];
+        acc = __builtin_fmaf(S.ao[o], noise_basis(S, cx * f, cy * f, cz * f, S.seed + (uint32_t)o), acc);
+    }
+    return fminf(fmaxf(acc * S.inv_norm, -1.0f), 1.0f);
+}
+
+// c = g_scale p and its perturbation: GRADIENT adds, per octave k,
+// pa_k (s(c pf_k, seed + 1000 + 3k + a))_a on the field's axes a;
+// OFFSET_XZ adds perturb_amp w to x and z, w = the normalised FBM of s at
+// p pf_k (one field for both axes, as windy3d_block's two offset sets).
+__device__ __forceinline__ void noise_warp(const SamplerDev& S, float px, float py, float pz, float& cx, float& cy,
+                                           float& cz) {
+    cx = px * S.g_scale;
+    cy = py * S.g_scale;
+    cz = pz * S.g_scale;
+    if (S.perturb == BMF_PERTURB_GRADIENT) {
+        for (int k = 0; k < S.perturb_octaves; k++) {
+            const float f = S.pf[k], a = S.pa[k];
+            const uint32_t sk = S.seed + 1000u + 3u * (uint32_t)k;
+            const float qx = cx * f, qy = cy * f, qz = cz * f;
+            if (S.noise_dims == 2) {
+                const float wx = snoise3(qx, 0.0f, qz, sk), wz = snoise3(qx, 0.0f, qz, sk + 2u);
+                cx = __builtin_fmaf(a, wx, cx);
+                cz = __builtin_fmaf(a, wz, cz);
+            } else {
+                const float wx = snoise3(qx, qy, qz, sk), wy = snoise3(qx, qy, qz, sk + 1u);
+                const float wz = snoise3(qx, qy, qz, sk + 2u);
+                cx = __builtin_fmaf(a, wx, cx);
+                cy = __builtin_fmaf(a, wy, cy);
+                cz = __builtin_fmaf(a, wz, cz);
+            }
+        }
+    } else if (S.perturb == BMF_PERTURB_OFFSET_XZ) {
+        float acc = 0.0f;
+        for (int k = 0; k < S.perturb_octaves; k++) {
+            const float f = S.pf[k];
+            acc = __builtin_fmaf(S.pa[k], snoise3(px * f, py * f, pz * f, S.seed + 2000u + (uint32_t)k), acc);
+        }
+        const float w = fminf(fmaxf(acc * S.pinv, -1.0f), 1.0f);
+        cx = __builtin_fmaf(S.perturb_amp, w, cx);
+        cz = __builtin_fmaf(S.perturb_amp, w, cz);
+    }
+}
+
+// n at a sample (a 2D field's n depends on x and z only).
+__device__ __forceinline__ float noise_value(const SamplerDev& S, float px, float py, float pz) {
+    float cx, cy, cz;
+    noise_warp(S, px, py, pz, cx, cy, cz);
+    return noise_fractal(S, cx, cy, cz);
+}
+
+__device__ __forceinline__ float noise_density_from(const SamplerDev& S, float n, float py) {
+    const float dy = (py * S.g_scale) * S.y_scale;
+    return (-dy) - n * S.height;
+}
+
 __device__ __forceinline__ float sample_density(const SamplerDev& S, float x, float y, float z) {
     if (S.kind == BMF_SAMPLER_TERRAIN) return terrain_density(S, x, y, z);
     if (S.kind == BMF_SAMPLER_CSG) return csg_density(S, x, y, z);
+    if (S.kind == BMF_SAMPLER_NOISE) return noise_density_from(S, noise_value(S, x, y, z), y);
     return implicit_density(S.kind, S.world_size, x, y, z);
 }
 

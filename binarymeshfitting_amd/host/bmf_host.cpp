@@ -19,6 +19,35 @@ void NoiseSamplers::create_sampler_terrain_fbm_3d(Sampler* s, const TerrainPrope
     s->desc.seed = p.seed;
 }
 
+namespace {
+void noise_preset(Sampler* s, int32_t which, uint32_t seed) {
+    bmf_sampler_desc d{};
+    if (bmf_noise_preset(which, s->world_size, seed, &d) == BMF_OK) s->desc = d;
+}
+}  // namespace
+
+void NoiseSamplers::create_sampler_terrain_2d(Sampler* s, uint32_t seed) { noise_preset(s, BMF_PRESET_TERRAIN2D, seed); }
+void NoiseSamplers::create_sampler_terrain_pert_2d(Sampler* s, uint32_t seed) {
+    noise_preset(s, BMF_PRESET_TERRAIN2D_PERT, seed);
+}
+void NoiseSamplers::create_sampler_terrain_3d(Sampler* s, uint32_t seed) { noise_preset(s, BMF_PRESET_TERRAIN3D, seed); }
+void NoiseSamplers::create_sampler_terrain_pert_3d(Sampler* s, uint32_t seed) {
+    noise_preset(s, BMF_PRESET_TERRAIN3D_PERT, seed);
+}
+void NoiseSamplers::create_sampler_windy_3d(Sampler* s, uint32_t seed) { noise_preset(s, BMF_PRESET_WINDY3D, seed); }
+
+void NoiseSamplers::apply_properties(Sampler* s, const NoiseSamplerProperties& p) {
+    // the perturbation acts on c = g_scale p here, before the noise frequency
+    const float f = s->desc.freq;
+    s->desc.g_scale = p.g_scale;
+    s->desc.height = p.height;
+    s->desc.perturb = BMF_PERTURB_GRADIENT;
+    s->desc.perturb_octaves = p.octaves;
+    s->desc.perturb_amp = p.amp / f;
+    s->desc.perturb_freq = p.frequency * f;
+    s->desc.perturb_gain = p.gain;
+}
+
 Sampler ImplicitFunctions::create_sampler(int kind, float world_size) {
     Sampler s;
     s.world_size = world_size;

@@ -72,6 +72,31 @@ struct TerrainProperties {
     uint32_t seed = 1337;
 };
 void create_sampler_terrain_fbm_3d(Sampler* s, const TerrainProperties& p);
+
+// The reference's noise sampler factories (NoiseSampler.hpp:91-149) as NOISE
+// samplers with the project's own noise (bmf_noise_preset): the block shapes
+// of terrain2d_block, terrain2d_pert_block, terrain3d_block,
+// terrain3d_pert_block and windy3d_block.
+void create_sampler_terrain_2d(Sampler* s, uint32_t seed = 1337);
+void create_sampler_terrain_pert_2d(Sampler* s, uint32_t seed = 1337);
+void create_sampler_terrain_3d(Sampler* s, uint32_t seed = 1337);
+void create_sampler_terrain_pert_3d(Sampler* s, uint32_t seed = 1337);
+void create_sampler_windy_3d(Sampler* s, uint32_t seed = 1337);
+
+// NoiseSamplerProperties (NoiseSampler.hpp:7-22): what terrain2d_pert_block
+// reads from the world (g_scale, height, and the perturbation's octaves,
+// amplitude, frequency and gain, in FastNoiseSIMD's frequency-scaled units);
+// the defaults are the world's (WorldOctree.cpp:49-54).
+struct NoiseSamplerProperties {
+    float g_scale = 0.25f;
+    float height = 75.0f;
+    int octaves = 13;
+    float amp = 0.87f;
+    float frequency = 0.585f;
+    float gain = 0.488f;
+};
+// terrain2d_pert_block called with `properties` (NoiseSampler.cpp:150-158).
+void apply_properties(Sampler* s, const NoiseSamplerProperties& p);
 }  // namespace NoiseSamplers
 
 namespace ImplicitFunctions {

@@ -6,7 +6,11 @@
 // With smooth_normals = 1 it also prints the same checksum over the normals.
 // Every node sits at `level` (default max_level = 7): below max_level the
 // overlap rule of ChunkGenerator.cpp:98 gives 0.035 + 0.005 * iters.
-//   bmf_host_demo <n_side> <resolution> <iters> <mode: 0 plain, 1 batched> [smooth_normals] [level]
+// The sampler is the FBM terrain, or `world` (the reference world's
+// terrain2d_pert sampler with its noise_properties, WorldOctree.cpp:40-54)
+// or `pert3d` (create_sampler_terrain_pert_3d, DebugScene.cpp:242), whose
+// chunk rows start n_side / 2 chunks below y = 0.
+//   bmf_host_demo <n_side> <resolution> <iters> <mode: 0 plain, 1 batched> [smooth_normals] [level] [sampler]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,8 +31,19 @@ int main(int argc, char** argv) {
     props.smooth_normals = normals;
     props.chunk_resolution = res;
     props.process_iters = iters;
+    const char* kind = argc > 7 ? argv[7] : "fbm";
     Sampler sampler;
-    NoiseSamplers::create_sampler_terrain_fbm_3d(&sampler, NoiseSamplers::TerrainProperties{});
+    float y_base = 0.0f;
+    if (!std::strcmp(kind, "world")) {
+        NoiseSamplers::create_sampler_terrain_pert_2d(&sampler);
+        NoiseSamplers::apply_properties(&sampler, NoiseSamplers::NoiseSamplerProperties{});
+        y_base = -(float)(n_side / 2);
+    } else if (!std::strcmp(kind, "pert3d")) {
+        NoiseSamplers::create_sampler_terrain_pert_3d(&sampler);
+        y_base = -(float)(n_side / 2);
+    } else {
+        NoiseSamplers::create_sampler_terrain_fbm_3d(&sampler, NoiseSamplers::TerrainProperties{});
+    }
     ChunkGenerator gen;
     gen.mode = mode;
     int rc = gen.init(props, sampler);
@@ -44,7 +59,7 @@ int main(int argc, char** argv) {
         for (int j = 0; j < n_side; j++)
             for (int k = 0; k < n_side; k++) {
                 chunks.emplace_back(new DMCChunk());
-                chunks.back()->init(vec3(size * i, size * j, size * k), size, level, sampler);
+                chunks.back()->init(vec3(size * i, size * ((float)j + y_base), size * k), size, level, sampler);
                 chunks.back()->dim = (uint32_t)res;
                 nodes.emplace_back(new WorldOctreeNode());
                 nodes.back()->chunk = chunks.back().get();

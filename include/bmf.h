@@ -61,9 +61,18 @@ enum {
     BMF_SAMPLER_TORUS_Z = 3, /* ImplicitFunctions::torus_z, ImplicitSampler.cpp:27-34 */
     BMF_SAMPLER_PLANE_Y = 4, /* ImplicitFunctions::plane_y, ImplicitSampler.cpp:62-65 */
     BMF_SAMPLER_CSG = 5,     /* max-union of spheres and boxes, floored at -16 */
-    BMF_SAMPLER_EXTERNAL = 6 /* densities supplied by the caller in device memory (any
-                                host sampler's Sampler::block output, Sampler.hpp:20-34) */
+    BMF_SAMPLER_EXTERNAL = 6, /* densities supplied by the caller in device memory (any
+                                 host sampler's Sampler::block output, Sampler.hpp:20-34) */
+    BMF_SAMPLER_NOISE = 7     /* the NoiseSamplers block shapes (NoiseSampler.cpp:113-318)
+                                 with the project's own noise: see the noise_* fields and
+                                 bmf_noise_preset */
 };
+
+/* NOISE sampler: basis, fractal and coordinate perturbation. */
+enum { BMF_NOISE_VALUE = 0, BMF_NOISE_SIMPLEX = 1 };
+enum { BMF_FRACTAL_FBM = 0, BMF_FRACTAL_RIGID = 1 };
+enum { BMF_PERTURB_NONE = 0, BMF_PERTURB_GRADIENT = 1, BMF_PERTURB_OFFSET_XZ = 2 };
+#define BMF_MAX_PERTURB_OCTAVES 24
 
 typedef struct {
     int32_t kind;
@@ -79,7 +88,37 @@ typedef struct {
     const float* densities;           /* EXTERNAL, device memory: the dim^3 FP32 [x][y][z] blocks
                                          of the chunks of each later bmf_extract, packed in
                                          descriptor order; must stay valid while work is queued */
+    /* NOISE: at sample position p, c = g_scale p, warped by the perturbation; n = the
+     * fractal (octaves, freq, lacunarity, gain, seed above) of the basis at c; density
+     * d = -((p.y g_scale) y_scale) - height n.  noise_dims 2 evaluates the noise over
+     * (x, z) only (a height field broadcast along y, as terrain2d_block's one-row
+     * NOISE_BLOCK), 3 over the volume. */
+    int32_t noise_dims;               /* 2 or 3 */
+    int32_t noise_basis;              /* BMF_NOISE_* */
+    int32_t noise_fractal;            /* BMF_FRACTAL_* */
+    float g_scale, y_scale, height;
+    int32_t perturb;                  /* BMF_PERTURB_*; OFFSET_XZ needs noise_dims 3 */
+    int32_t perturb_octaves;          /* 0..BMF_MAX_PERTURB_OCTAVES */
+    float perturb_amp, perturb_freq, perturb_gain;
 } bmf_sampler_desc;
+
+/* Fills `out` (every field) with the NOISE sampler that stands for one of the
+ * reference's block samplers, with the settings those functions pass to
+ * FastNoiseSIMD carried over (its defaults where they set none: frequency
+ * 0.01, 3 fractal octaves, gain 0.5, lacunarity 2, 3 perturb octaves).  A
+ * perturbation given there in the noise's frequency-scaled coordinates
+ * (frequency F, amplitude A) acts here on c, so it is carried as F * 0.01 and
+ * A / 0.01.  world_size is Sampler::world_size (windy3d's `resolution`). */
+enum {
+    BMF_PRESET_TERRAIN2D = 0,      /* NoiseSamplers::terrain2d_block, NoiseSampler.cpp:113-146 */
+    BMF_PRESET_TERRAIN2D_PERT = 1, /* terrain2d_pert_block, :148-194 (WorldOctree.cpp:40's default) */
+    BMF_PRESET_TERRAIN3D = 2,      /* terrain3d_block, :196-227 */
+    BMF_PRESET_TERRAIN3D_PERT = 3, /* terrain3d_pert_block, :229-263 */
+    BMF_PRESET_WINDY3D = 4,        /* windy3d_block, :265-318 */
+    BMF_PRESET_WORLD = 5           /* terrain2d_pert_block with the world's noise_properties,
+                                      WorldOctree.cpp:40-54 */
+};
+int bmf_noise_preset(int32_t which, float world_size, uint32_t seed, bmf_sampler_desc* out);
 
 /* One chunk: DMCChunk fields (DMCChunk.hpp:24-60) plus the per-chunk
  * WorldProperties the generator applies (ChunkGenerator.cpp:85-98). */

@@ -342,6 +342,10 @@ double or_csg_value_f64(const or_sampler_t* s, double x, double y, double z, dou
  * Coordinate = (float)i * delta + origin (two roundings, no FMA). */
 void or_sample_block_f32(const or_sampler_t* s, int dim, const float opos[3], float delta, float* out)
 {
+    if (s->kind == OR_SAMPLER_NOISE) {
+        or_noise_block_f32(s, dim, opos, delta, out);
+        return;
+    }
     float fo[32], ao[32], inv = 1.0f;
     if (s->kind == OR_SAMPLER_TERRAIN) or_fbm_params(s, fo, ao, &inv);
     for (int ix = 0; ix < dim; ix++) {
@@ -365,6 +369,7 @@ static double sample_f64(const or_sampler_t* s, double x, double y, double z)
 {
     if (s->kind == OR_SAMPLER_TERRAIN) return or_terrain_value_f64(s, x, y, z);
     if (s->kind == OR_SAMPLER_CSG) return or_csg_value_f64(s, x, y, z, 0);
+    if (s->kind == OR_SAMPLER_NOISE) return or_noise_field_f64(s, x, y, z);
     return implicit_f64(s->kind, s->world_size, x, y, z);
 }
 
@@ -813,5 +818,6 @@ float or_sample_value_f32(const or_sampler_t* s, float x, float y, float z)
 {
     if (s->kind == OR_SAMPLER_TERRAIN) return or_terrain_value_f32(s, x, y, z);
     if (s->kind == OR_SAMPLER_CSG) return or_csg_value_f32(s, x, y, z, NULL);
+    if (s->kind == OR_SAMPLER_NOISE) return or_noise_field_f32(s, x, y, z);
     return implicit_f32(s->kind, s->world_size, x, y, z);
 }

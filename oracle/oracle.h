@@ -32,8 +32,12 @@ enum {
     OR_SAMPLER_CUBOID = 2,  /* ImplicitSampler.cpp:52-60 */
     OR_SAMPLER_TORUS_Z = 3, /* ImplicitSampler.cpp:27-34 */
     OR_SAMPLER_PLANE_Y = 4, /* ImplicitSampler.cpp:62-65 */
-    OR_SAMPLER_CSG = 5      /* union of spheres and boxes, positive inside */
+    OR_SAMPLER_CSG = 5,     /* union of spheres and boxes, positive inside */
+    OR_SAMPLER_NOISE = 7    /* noise fields: NoiseSampler.cpp:113-318 shapes, own noise */
 };
+enum { OR_NOISE_VALUE = 0, OR_NOISE_SIMPLEX = 1 };
+enum { OR_FRACTAL_FBM = 0, OR_FRACTAL_RIGID = 1 };
+enum { OR_PERTURB_NONE = 0, OR_PERTURB_GRADIENT = 1, OR_PERTURB_OFFSET_XZ = 2 };
 
 typedef struct {
     int32_t kind;
@@ -46,6 +50,11 @@ typedef struct {
     int32_t n_spheres;
     const float* boxes;   /* CSG: n_boxes x (cx, cy, cz, hx, hy, hz) */
     int32_t n_boxes;
+    /* NOISE (oracle_noise.c) */
+    int32_t noise_dims, noise_basis, noise_fractal;
+    float g_scale, y_scale, height;
+    int32_t perturb, perturb_octaves;
+    float perturb_amp, perturb_freq, perturb_gain;
 } or_sampler_t;
 
 typedef struct {
@@ -76,6 +85,17 @@ float or_snoise3_f32(float x, float y, float z, uint32_t seed);
 double or_snoise3_f64(double x, double y, double z, uint32_t seed);
 uint32_t or_grad_index(int32_t i, int32_t j, int32_t k, uint32_t seed);
 void or_fbm_params(const or_sampler_t* s, float* fo, float* ao, float* inv_norm);
+/* Noise fields (oracle_noise.c). */
+float or_value3_f32(float x, float y, float z, uint32_t seed);
+float or_value2_f32(float x, float z, uint32_t seed);
+double or_value3_f64(double x, double y, double z, uint32_t seed);
+double or_value2_f64(double x, double z, uint32_t seed);
+float or_lattice_value(int32_t i, int32_t j, int32_t k, uint32_t seed);
+float or_noise_field_f32(const or_sampler_t* s, float x, float y, float z);
+double or_noise_field_f64(const or_sampler_t* s, double x, double y, double z);
+/* n (the fractal after the perturbation) at the column / point, FP32. */
+float or_noise_fractal_f32(const or_sampler_t* s, float x, float y, float z);
+void or_noise_block_f32(const or_sampler_t* s, int dim, const float opos[3], float delta, float* out);
 float or_terrain_value_f32(const or_sampler_t* s, float x, float y, float z);
 /* The sampler's FP32 value at one point (every octave of the terrain). */
 float or_sample_value_f32(const or_sampler_t* s, float x, float y, float z);

@@ -13,6 +13,9 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 SAMPLER_TERRAIN, SAMPLER_SPHERE, SAMPLER_CUBOID, SAMPLER_TORUS_Z, SAMPLER_PLANE_Y, SAMPLER_CSG = range(6)
+SAMPLER_NOISE = 7
+NOISE_FIELDS = ("noise_dims", "noise_basis", "noise_fractal", "g_scale", "y_scale", "height", "perturb",
+                "perturb_octaves", "perturb_amp", "perturb_freq", "perturb_gain")
 
 
 class Sampler(C.Structure):
@@ -30,6 +33,17 @@ class Sampler(C.Structure):
         ("n_spheres", C.c_int32),
         ("boxes", C.POINTER(C.c_float)),
         ("n_boxes", C.c_int32),
+        ("noise_dims", C.c_int32),
+        ("noise_basis", C.c_int32),
+        ("noise_fractal", C.c_int32),
+        ("g_scale", C.c_float),
+        ("y_scale", C.c_float),
+        ("height", C.c_float),
+        ("perturb", C.c_int32),
+        ("perturb_octaves", C.c_int32),
+        ("perturb_amp", C.c_float),
+        ("perturb_freq", C.c_float),
+        ("perturb_gain", C.c_float),
     ]
 
 
@@ -76,6 +90,24 @@ def lib():
         L.or_snoise3_f64.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32]
         L.or_snoise3_f64.restype = C.c_double
         L.or_grad_index.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
+        L.or_lattice_value.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
+        L.or_lattice_value.restype = C.c_float
+        L.or_value3_f32.argtypes = [C.c_float, C.c_float, C.c_float, C.c_uint32]
+        L.or_value3_f32.restype = C.c_float
+        L.or_value2_f32.argtypes = [C.c_float, C.c_float, C.c_uint32]
+        L.or_value2_f32.restype = C.c_float
+        L.or_value3_f64.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32]
+        L.or_value3_f64.restype = C.c_double
+        L.or_value2_f64.argtypes = [C.c_double, C.c_double, C.c_uint32]
+        L.or_value2_f64.restype = C.c_double
+        L.or_noise_field_f32.argtypes = [P(Sampler), C.c_float, C.c_float, C.c_float]
+        L.or_noise_field_f32.restype = C.c_float
+        L.or_noise_field_f64.argtypes = [P(Sampler), C.c_double, C.c_double, C.c_double]
+        L.or_noise_field_f64.restype = C.c_double
+        L.or_noise_fractal_f32.argtypes = [P(Sampler), C.c_float, C.c_float, C.c_float]
+        L.or_noise_fractal_f32.restype = C.c_float
+        L.or_sample_value_f32.argtypes = [P(Sampler), C.c_float, C.c_float, C.c_float]
+        L.or_sample_value_f32.restype = C.c_float
         L.or_grad_index.restype = C.c_uint32
         L.or_fbm_params.argtypes = [P(Sampler), f32p, f32p, f32p]
         L.or_terrain_value_f32.argtypes = [P(Sampler), C.c_float, C.c_float, C.c_float]
@@ -162,12 +194,17 @@ class SamplerSpec:
     """Keeps the numpy arrays a Sampler struct points into alive."""
 
     def __init__(self, kind=SAMPLER_TERRAIN, world_size=256.0, amp=24.0, y0=32.0, freq=0.02, lacunarity=2.0,
-                 gain=0.5, octaves=5, seed=1337, spheres=None, boxes=None):
+                 gain=0.5, octaves=5, seed=1337, spheres=None, boxes=None, **noise):
+        unknown = set(noise) - set(NOISE_FIELDS)
+        if unknown:
+            raise TypeError(f"unknown sampler fields {sorted(unknown)}")
         self.spheres = np.ascontiguousarray(spheres if spheres is not None else np.zeros((0, 4)), np.float32)
         self.boxes = np.ascontiguousarray(boxes if boxes is not None else np.zeros((0, 6)), np.float32)
         self.s = Sampler(kind, world_size, amp, y0, freq, lacunarity, gain, octaves, seed & 0xFFFFFFFF,
                          _ptr(self.spheres, C.c_float), len(self.spheres), _ptr(self.boxes, C.c_float),
                          len(self.boxes))
+        for k, v in noise.items():
+            setattr(self.s, k, v)
 
     @property
     def ref(self):

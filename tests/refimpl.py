@@ -171,3 +171,38 @@ def snoise_lattice_sum(x, y, z, seed):
                 g = GRADS[grad_index(i, j, k, seed)]
                 total += w ** 4 * (g[0] * dx + g[1] * dy + g[2] * dz)
     return float(np.clip(76.8 * total, -1.0, 1.0))
+
+
+# ------------------------------------------------------------ noise fields --
+def lattice_value(i, j, k, seed):
+    """The value-noise lattice value: the simplex lattice hash (as in
+    grad_index) mixed to 24 bits k, returned as k / 2^23 - 1."""
+    h = (_mul24(i & 0xFFFFFFFF, P1) ^ _mul24(j & 0xFFFFFFFF, P2) ^ _mul24(k & 0xFFFFFFFF, P3) ^ seed) & 0xFFFFFFFF
+    h ^= h >> 13
+    h = _mul24(h ^ (h >> 24), M1)
+    h ^= h >> 11
+    h = _mul24(h ^ (h >> 24), M2)
+    h ^= h >> 16
+    return (h & 0xFFFFFF) / 2.0 ** 23 - 1.0
+
+
+def quintic(t):
+    return t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
+
+
+def value_noise(p, seed):
+    """Value noise in 2 or 3 dimensions (p = (x, z) or (x, y, z)) as the
+    tensor-product sum over the cell's corners of the lattice value times
+    the product of the per-axis quintic weights (not the nested blends)."""
+    cells = [int(np.floor(c)) for c in p]
+    w = [quintic(c - f) for c, f in zip(p, cells)]
+    total = 0.0
+    for corner in range(1 << len(p)):
+        bits = [(corner >> a) & 1 for a in range(len(p))]
+        weight = 1.0
+        for a, b in enumerate(bits):
+            weight *= w[a] if b else 1.0 - w[a]
+        idx = [cells[a] + bits[a] for a in range(len(p))]
+        ijk = (idx[0], 0, idx[1]) if len(p) == 2 else tuple(idx)
+        total += weight * lattice_value(*ijk, seed)
+    return total
