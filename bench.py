@@ -47,6 +47,8 @@ def parse():
                     help="restrict the uneven-LOD leg to the chunks whose label rows take W u64 words "
                          "(W = ceil(dim / 64): 1 for 33^3, 2 for 65^3, 3 for 129^3 samples), so that the frame "
                          "forks no row-word classes onto auxiliary streams; 0 = all chunks")
+    ap.add_argument("--world-steps", type=int, default=10,
+                    help="timed batched steps of the default-world leg (0 skips it)")
     ap.add_argument("--hermite-steps", type=int, default=5,
                     help="timed batched steps of the CSG quad/QEF leg (64 chunks of 64^3 cells); 0 skips it")
     ap.add_argument("--single-steps", type=int, default=20,
@@ -150,6 +152,61 @@ def lod_leg(capi, roofline, workloads, steps, only_class=0):
             "roofline": roofline.frame_fractions(flops, nbytes, wall),
             "plain_roofline": roofline.frame_fractions(flops, nbytes, plain),
             "_check": {"workload": wl, "meshes": meshes}}
+
+
+def world_leg(capi, roofline, workloads, steps):
+    """The reference's default world: WorldOctree's sampler (terrain2d_pert_block with the
+    world's noise_properties, WorldOctree.cpp:40-54) over the renderable leaves the LOD pass
+    leaves around a focus on the surface of a 4096-unit world (max level 7), chunk_resolution
+    32, K = 0 (WorldProperties' defaults), overlap 0.035 below the max level
+    (ChunkGenerator.cpp:98), as one batch."""
+    p = workloads.LodParams(max_level=7, min_level=1)
+    focus = (0.0, -140.0, 0.0)
+    nodes, leaves = workloads.lod_world(world_size=4096.0, build_focus=focus, params=p)
+    chunks = [workloads.Chunk(tuple(float(v) for v in nodes[L]["pos"]), float(nodes[L]["size"]), 32,
+                              0.035 if int(nodes[L]["level"]) < p.max_level else 0.0, 0, 0) for L in leaves]
+    sd = capi.noise_preset(capi.PRESET_WORLD)
+    wl = workloads.Workload("world-default", sd, chunks)
+    cd = workloads.to_capi(chunks)
+    n = len(cd)
+    arr = capi.chunk_array(cd)
+    with capi.Context(max_chunks=1024, max_samples=n * 32 ** 3, max_vertices=1 << 22, max_indices=1 << 24) as ctx:
+        ctx.set_sampler(**sd)
+        for _ in range(2):
+            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        ctx.sync()
+        wall = (time.perf_counter() - t0) / steps
+        stages = ctx.profile_stages(cd)
+        rc, res, _, _ = ctx.results(n)
+        capi.check(rc, "bmf_result")
+        sampled = sum(r.sampled for r in res)
+        counts = [(r.vertex_count, r.index_count, r.contains_mesh) for r in res]
+        meshes = {j: ctx.mesh(j) for j in range(n) if res[j].vertex_count}
+        nbytes = sum(roofline.mesh_bytes(r.vertex_count, r.index_count // 3) for r in res)
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
+        ctx.sync()
+        t0 = time.perf_counter()
+        capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
+        ctx.sync()
+        plain = time.perf_counter() - t0
+    V = sum(c[0] for c in counts)
+    log(f"[bench] default world: {n / wall:.1f} chunks/s batched ({1e3 * wall:.3f} ms/step), plain {n / plain:.1f}"
+        f" chunks/s, sampled {sampled}/{n}, non-empty {len(meshes)}/{n}, V={V}, stages "
+        + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
+    return {"workload": f"world-default-{n}: the reference world's terrain2d_pert sampler (own noise; 3 value-noise "
+                        f"octaves, 13 gradient-perturbation octaves) on the {n} LOD leaves around {focus} of a "
+                        f"4096-unit world, 32^3 samples per chunk, K=0",
+            "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4),
+            "plain_chunks_per_s": round(n / plain, 2), "batched_over_plain": round(plain / wall, 2),
+            "sampled_chunks": sampled, "nonempty_chunks": len(meshes), "vertices": V,
+            "surface_chunks_per_s": round(sampled / wall, 2),
+            "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+            "hbm_frac": round(nbytes / wall / (roofline.PEAK_HBM_GBS * 1e9), 6),
+            "_check": {"workload": wl, "meshes": meshes, "counts": counts}}
 
 
 def hermite_leg(capi, roofline, workloads, steps):
@@ -458,6 +515,11 @@ def main():
     if args.lod_steps > 0:
         lod = lod_leg(capi, roofline, workloads, args.lod_steps, args.lod_class)
 
+    # ---- the reference's default world sampler over its LOD leaves ----------------------
+    world = None
+    if args.world_steps > 0:
+        world = world_leg(capi, roofline, workloads, args.world_steps)
+
     # ---- Hermite / QEF: 64 CSG chunks, dual quads placed by FP32 QEF ---------------------
     hermite = None
     if args.hermite_steps > 0:
@@ -533,6 +595,32 @@ def main():
                                                                     ck["meshes"][j]) / (lw.chunks[j].dim - 1)),
                                    sorted(ck["meshes"]), threads)
             lod["fp64_error"] = {"positions_over_extent": fe.stats(fe.merge(parts))}
+        if world:
+            ck = world["_check"]
+            ww = ck["workload"]
+            wspec = pyoracle.SamplerSpec(**ww.sampler)
+
+            def world_ref(j):
+                c = ww.chunks[j]
+                return pyoracle.extract_chunk(wspec, pyoracle.make_chunk(c.pos, c.size, c.dim, c.overlap, c.iters,
+                                                                         c.boundary_processing))
+
+            wbad = 0
+            for j, m in enumerate(fe.over_chunks(world_ref, range(len(ww.chunks)), threads)):
+                V_, nI_, cm_ = ck["counts"][j]
+                if V_ != m["V"] or nI_ != m["nI"] or bool(cm_) != bool(m["contains_mesh"]):
+                    wbad += 1
+                elif V_:
+                    wp, wi, wb = ck["meshes"][j]
+                    if not (np.array_equal(wp, m["pos"]) and np.array_equal(wi, m["idx"])
+                            and np.array_equal(wb, m["boundary"])):
+                        wbad += 1
+            world["parity"] = {"chunks": len(ww.chunks), "mismatched": wbad,
+                               "check": "bit-exact positions, triangle lists and boundary flags vs CPU"}
+            parts = fe.over_chunks(lambda j: (fe.triangle_positions(wspec, ww.chunks[j], ck["meshes"][j][0])
+                                              / (ww.chunks[j].dim - 1)), sorted(ck["meshes"]), threads)
+            world["fp64_error"] = {"positions_over_extent": fe.stats(fe.merge(parts))}
+            log(f"[bench] default world parity: {wbad} of {len(ww.chunks)} chunks differ")
         if hermite:
             ck = hermite["_check"]
             hw = ck["workload"]
@@ -614,6 +702,7 @@ def main():
         "path_roofline": path,
         "fp64_error": fp64,
         "uneven_lod": lod,
+        "default_world": world,
         "hermite_qef": hermite,
         "dmc_quads": quads,
         "single_chunk": single,
@@ -621,7 +710,7 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
     }
-    for leg in (lod, hermite, quads, single, diff):
+    for leg in (lod, world, hermite, quads, single, diff):
         if leg:
             leg.pop("_check", None)
     print(json.dumps(line), flush=True)
