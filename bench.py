@@ -209,6 +209,26 @@ def world_leg(capi, roofline, workloads, steps):
             "_check": {"workload": wl, "meshes": meshes, "counts": counts}}
 
 
+def plan_scaling(capi, workloads, sizes=(512, 4096, 16384, 65536)):
+    """Time of the batch prologue (the plan stage: geometry, culling, arena
+    offsets) and of the whole frame for growing batches of small chunks
+    (9^3 samples, spread over and off the terrain band, K = 1): one block up
+    to 4,096 descriptors, three passes over many blocks beyond."""
+    rng = np.random.default_rng(3)
+    out = []
+    with capi.Context(max_chunks=max(sizes), max_samples=max(sizes) * 9 ** 3, max_vertices=1 << 22,
+                      max_indices=1 << 24) as ctx:
+        ctx.set_sampler(**workloads.terrain_sampler())
+        for n in sizes:
+            cs = [workloads.Chunk((float(rng.uniform(-4000, 4000)), float(rng.uniform(-40, 100)),
+                                   float(rng.uniform(-4000, 4000))), 8.0, 9, 0.0, 1, 0) for _ in range(n)]
+            st = ctx.profile_stages(workloads.to_capi(cs))
+            out.append({"chunks": n, "plan_ms": round(st["plan"], 4), "frame_ms": round(sum(st.values()), 4)})
+    log("[bench] plan scaling: " + ", ".join(f"{r['chunks']}: plan {r['plan_ms']:.3f} / frame {r['frame_ms']:.3f} ms"
+                                              for r in out))
+    return out
+
+
 def hermite_leg(capi, roofline, workloads, steps):
     """64 chunks of 64^3 cells tiled 4x4x4 over [0,256)^3 of the CSG union of
     200 spheres and 100 boxes (seed 42); dual-marching-cubes quads with FP32
@@ -515,6 +535,9 @@ def main():
     if args.lod_steps > 0:
         lod = lod_leg(capi, roofline, workloads, args.lod_steps, args.lod_class)
 
+    # ---- the batch prologue at growing batch sizes ---------------------------------------
+    scaling = plan_scaling(capi, workloads) if args.lod_steps > 0 else None
+
     # ---- the reference's default world sampler over its LOD leaves ----------------------
     world = None
     if args.world_steps > 0:
@@ -703,6 +726,7 @@ def main():
         "fp64_error": fp64,
         "uneven_lod": lod,
         "default_world": world,
+        "plan_scaling": scaling,
         "hermite_qef": hermite,
         "dmc_quads": quads,
         "single_chunk": single,

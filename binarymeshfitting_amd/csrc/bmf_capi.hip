@@ -34,7 +34,7 @@ hipError_t dalloc(T** p, int64_t count) {
 
 void free_arenas(bmf_ctx* c) {
     Arenas& A = c->A;
-    void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plane_map, A.smooth_list, A.smooth_prefix,
+    void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plan_part, A.plan_tmp, A.plane_map, A.smooth_list, A.smooth_prefix,
                     A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.smooth_vmap, A.smooth_tmap, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
                     A.pos,     A.pos4,      A.boundary, A.vedge,       A.vring,        A.idx,          A.ctr,
@@ -325,6 +325,8 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && dalloc(&A.meta, nc) == hipSuccess && dalloc(&A.descs, nc) == hipSuccess &&
          dalloc(&A.results, nc) == hipSuccess && dalloc(&A.tile_prefix, nc + 1) == hipSuccess &&
+         dalloc(&A.plan_part, 40 * ((nc + 1023) / 1024) + 64) == hipSuccess &&
+         dalloc(&A.plan_tmp, 2 * nc) == hipSuccess &&
          dalloc(&A.plane_map, nc * BMF_MAX_DIM) == hipSuccess && dalloc(&A.smooth_list, nc) == hipSuccess &&
          dalloc(&A.smooth_prefix, nc + 1) == hipSuccess && dalloc(&A.smooth_tprefix, nc + 1) == hipSuccess &&
          dalloc(&A.smooth_xcd_v, kXcds + 1) == hipSuccess && dalloc(&A.smooth_xcd_t, kXcds + 1) == hipSuccess &&

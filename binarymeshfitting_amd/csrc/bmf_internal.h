@@ -107,6 +107,9 @@ struct Arenas {
     bmf_chunk_desc* descs;     // [max_chunks]
     bmf_chunk_result* results; // [max_chunks]
     int64_t* tile_prefix;      // [max_chunks + 1] sampling tiles
+    int64_t* plan_part;        // [40 nb + 64] multi-block plan passes (nb = max_chunks / 1024 rounded up):
+                               // block totals and bases, flags, class and group bases
+    int64_t* plan_tmp;         // [2 max_chunks] multi-block smoothing plan: per-chunk exclusive tile sums
     int2* plane_map;           // [max_chunks * BMF_MAX_DIM] (chunk, x) of each sampled x-plane, grouped by
                                // row-word class (written by k_sample's prologue)
     int32_t* smooth_list;      // [max_chunks] chunks that run the smoothing

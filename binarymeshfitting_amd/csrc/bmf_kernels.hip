@@ -27,9 +27,14 @@
 namespace bmf {
 
 // ------------------------------------------------------------------------
-// plan: geometry, culling, arena offsets, counters (one block)
+// plan: geometry, culling, arena offsets, counters (one block; three passes
+// over many blocks for batches beyond one block)
 // ------------------------------------------------------------------------
 constexpr int kPlanThreads = 1024;
+// Up to this many descriptors the plans run in one block (a few per thread,
+// one launch each); larger batches take the multi-block passes, whose extra
+// launches cost more than they save below it.
+constexpr int kPlanOneBlock = 4 * kPlanThreads;
 
 // The host-visible result of chunk j from its final record: written by
 // k_plan for the chunks it culls or skips and by k_scan for the sampled ones
@@ -52,157 +57,233 @@ __device__ __forceinline__ void write_result(const Arenas& A, int j, const Chunk
     A.results[j] = r;
 }
 
+// Plan record of one descriptor: DMCChunk::label_grid geometry
+// (DMCChunk.cpp:94-101), analytic culling, sizes; `add` gets its
+// contributions to the four arena sums (samples, label words, rows, sampling
+// tiles) and to the x-planes of its row-word class (add[4 + class]).
+__device__ __forceinline__ ChunkMeta plan_record(const Arenas& A, const SamplerDev* __restrict__ S,
+                                                 const bmf_chunk_desc& d, int allow_cull, int64_t (&add)[8]) {
+    ChunkMeta m;
+    float t = d.overlap * 2.0f;
+    t = 1.0f + t;
+    t = d.size * t;
+    m.delta = t / (float)(d.dim - 1);
+    const float so = d.size * d.overlap;
+    m.opos[0] = d.pos[0] - so;
+    m.opos[1] = d.pos[1] - so;
+    m.opos[2] = d.pos[2] - so;
+    m.dim = d.dim;
+    m.iters = d.iters;
+    m.bp = d.boundary_processing;
+    m.state = kStateSampled;
+    if (allow_cull && S->kind == BMF_SAMPLER_TERRAIN) {
+        // |fbm| <= 1 exactly, so |amp*fbm| <= |amp|; sample y is monotone in j.
+        const float a = fabsf(S->amp);
+        const float ylo = m.opos[1];
+        const float yhi = (float)(d.dim - 1) * m.delta + m.opos[1];
+        if (ylo - S->y0 > a) m.state = kStateAir;
+        else if (yhi - S->y0 < -a) m.state = kStateSolid;
+    } else if (allow_cull && S->kind == BMF_SAMPLER_NOISE && S->cull) {
+        // d = (-dy) - n h with dy = (y g) ym monotone in y and n in
+        // [nlo, nhi]: every FP32 step is monotone, so the extremes of d
+        // over the chunk come from the end values
+        const float ylo = m.opos[1];
+        const float yhi = (float)(d.dim - 1) * m.delta + m.opos[1];
+        const float dy0 = (ylo * S->g_scale) * S->y_scale, dy1 = (yhi * S->g_scale) * S->y_scale;
+        const float p0 = S->nlo * S->height, p1 = S->nhi * S->height;
+        const float dmax = (-fminf(dy0, dy1)) - fminf(p0, p1);
+        const float dmin = (-fmaxf(dy0, dy1)) - fmaxf(p0, p1);
+        if (dmax < 0.0f) m.state = kStateAir;
+        else if (dmin >= 0.0f) m.state = kStateSolid;
+    }
+    m.csg_ns = m.csg_nb = 0;  // k_csg_cull fills them
+    const int64_t ns = (int64_t)d.dim * d.dim * d.dim;
+    const bool sampled = m.state == kStateSampled;
+    m.lab_words = sampled ? (ns + 63) / 64 : 0;
+    m.any_air = m.state == kStateAir;
+    m.any_solid = m.state == kStateSolid;
+    m.contains_mesh = 0;
+    m.status = BMF_OK;
+    m.V = 0;
+    m.T = 0;
+    m.ipp = (A.mesh_mode == BMF_MESH_QUADS && !A.quad_tris) ? 4 : 3;
+    m.v_off = 0;
+    m.i_off = 0;
+    m.dens_off = 0;
+    m.lab_off = 0;
+    m.row_off = 0;
+    m.plane_slot = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) add[q] = 0;
+    add[0] = sampled ? ns : 0;
+    add[1] = m.lab_words;
+    add[2] = sampled ? (int64_t)d.dim * d.dim : 0;
+    add[3] = sampled ? (ns + A.sample_tile - 1) / A.sample_tile : 0;
+    if (sampled) add[4 + ((d.dim + 63) >> 6) - 1] = d.dim;
+    return m;
+}
+
+__device__ __forceinline__ bool plan_overflow(const Arenas& A, const int64_t (&tot)[8]) {
+    return tot[0] > A.cap_samples || tot[1] > A.cap_samples / 64 + A.cap_chunks || tot[2] > A.cap_rows;
+}
+
+// A record's final state once its offsets are known (base: where its block's
+// arena ranges start; cbase: its row-word class's first plane slot).
+__device__ __forceinline__ void plan_finish(const Arenas& A, int j, ChunkMeta& m, const int64_t (&base)[8],
+                                            const int64_t (&cbase)[4], bool overflow) {
+    m.dens_off += base[0];
+    m.lab_off += base[1];
+    m.row_off += base[2];
+    if (m.state == kStateSampled) {
+        const int q = ((m.dim + 63) >> 6) - 1;
+        m.plane_slot += cbase[q] + base[4 + q];
+        if (overflow) {
+            m.state = kStateSkipped;
+            m.status = BMF_ECAPACITY;
+        }
+    } else {
+        // culled chunks: DMCChunk.cpp:159-162 on uniform labels
+        m.contains_mesh = (m.state == kStateAir) && (m.dim % 32 != 0);
+    }
+    A.meta[j] = m;
+    if (m.state != kStateSampled) write_result(A, j, m);  // final: k_scan skips it
+}
+
+// The batch counters from the totals: sampling tiles, overflow, and per
+// row-word class its planes in plane_map and the row segments per plane
+// (few sampled planes, e.g. single large chunks: split each plane's rows so
+// that the class still has kSplitTarget items).
+__device__ __forceinline__ void plan_counters(const Arenas& A, int n, const int64_t (&tot)[8], bool overflow,
+                                              int reset_bumps) {
+    A.tile_prefix[n] = overflow ? 0 : tot[3];
+    Counters* c = A.ctr;
+    c->sample_head = 0;
+    if (reset_bumps) {
+        c->v_bump = 0;
+        c->i_bump = 0;
+        c->need_v = 0;
+        c->need_i = 0;
+        c->overflow = 0;
+        c->error = 0;
+    }
+    if (overflow) c->overflow = 1;
+    c->total_tiles = overflow ? 0 : tot[3];
+    int64_t cb = 0;
+    for (int q = 0; q < 4; q++) {
+        const int64_t np = overflow ? 0 : tot[4 + q];
+        int split = A.row_split > 0 ? A.row_split : 1;
+        if (A.row_split <= 0)
+            while (split < kMaxPlaneSplit && np * split < kSplitTarget) split <<= 1;
+        c->plane_split[q] = split;
+        c->class_base[q] = cb;
+        c->class_planes[q] = np;
+        cb += np;
+    }
+    c->n_smooth = 0;
+    c->total_samples = tot[0];
+}
+
+// Batches of up to kPlanOneBlock descriptors: one block, each thread a
+// segment of consecutive descriptors, one launch.
 __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allow_cull, int reset_bumps) {
     __shared__ int64_t s_w[8][kPlanThreads / 64 + 1];
-    const int tid = threadIdx.x;
-    const SamplerDev* __restrict__ S = A.sampler;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
-    const int j0 = tid * per, j1 = min(n, j0 + per);
-    int64_t sum[4] = {0, 0, 0, 0};  // samples, label words, rows, tiles
-    int32_t cls[4] = {0, 0, 0, 0};
-
+    const int j0 = threadIdx.x * per, j1 = min(n, j0 + per);
+    int64_t sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = j0; j < j1; j++) {
-        const bmf_chunk_desc d = A.descs[j];
-        ChunkMeta m;
-        // DMCChunk::label_grid geometry (DMCChunk.cpp:94-101)
-        float t = d.overlap * 2.0f;
-        t = 1.0f + t;
-        t = d.size * t;
-        m.delta = t / (float)(d.dim - 1);
-        const float so = d.size * d.overlap;
-        m.opos[0] = d.pos[0] - so;
-        m.opos[1] = d.pos[1] - so;
-        m.opos[2] = d.pos[2] - so;
-        m.dim = d.dim;
-        m.iters = d.iters;
-        m.bp = d.boundary_processing;
-        m.state = kStateSampled;
-        if (allow_cull && S->kind == BMF_SAMPLER_TERRAIN) {
-            // |fbm| <= 1 exactly, so |amp*fbm| <= |amp|; sample y is monotone in j.
-            const float a = fabsf(S->amp);
-            const float ylo = m.opos[1];
-            const float yhi = (float)(d.dim - 1) * m.delta + m.opos[1];
-            if (ylo - S->y0 > a) m.state = kStateAir;
-            else if (yhi - S->y0 < -a) m.state = kStateSolid;
-        } else if (allow_cull && S->kind == BMF_SAMPLER_NOISE && S->cull) {
-            // d = (-dy) - n h with dy = (y g) ym monotone in y and n in
-            // [nlo, nhi]: every FP32 step is monotone, so the extremes of d
-            // over the chunk come from the end values
-            const float ylo = m.opos[1];
-            const float yhi = (float)(d.dim - 1) * m.delta + m.opos[1];
-            const float dy0 = (ylo * S->g_scale) * S->y_scale, dy1 = (yhi * S->g_scale) * S->y_scale;
-            const float p0 = S->nlo * S->height, p1 = S->nhi * S->height;
-            const float dmax = (-fminf(dy0, dy1)) - fminf(p0, p1);
-            const float dmin = (-fmaxf(dy0, dy1)) - fmaxf(p0, p1);
-            if (dmax < 0.0f) m.state = kStateAir;
-            else if (dmin >= 0.0f) m.state = kStateSolid;
-        }
-        m.csg_ns = m.csg_nb = 0;  // k_csg_cull fills them
-        const int64_t ns = (int64_t)d.dim * d.dim * d.dim;
-        const bool sampled = m.state == kStateSampled;
-        m.lab_words = sampled ? (ns + 63) / 64 : 0;
-        m.any_air = m.state == kStateAir;
-        m.any_solid = m.state == kStateSolid;
-        m.contains_mesh = 0;
-        m.status = BMF_OK;
-        m.V = 0;
-        m.T = 0;
-        m.ipp = (A.mesh_mode == BMF_MESH_QUADS && !A.quad_tris) ? 4 : 3;
-        m.v_off = 0;
-        m.i_off = 0;
+        int64_t add[8];
+        ChunkMeta m = plan_record(A, A.sampler, A.descs[j], allow_cull, add);
+        // offsets within the thread's segment; tile_prefix likewise
         m.dens_off = sum[0];
         m.lab_off = sum[1];
         m.row_off = sum[2];
-        sum[0] += sampled ? ns : 0;
-        sum[1] += m.lab_words;
-        sum[2] += sampled ? (int64_t)d.dim * d.dim : 0;
-        sum[3] += sampled ? (ns + A.sample_tile - 1) / A.sample_tile : 0;
-        if (sampled) cls[((d.dim + 63) >> 6) - 1] += d.dim;
-        m.plane_slot = 0;
+        A.tile_prefix[j] = sum[3];
+        if (m.state == kStateSampled) m.plane_slot = sum[4 + ((m.dim + 63) >> 6) - 1];
         A.meta[j] = m;
+#pragma unroll
+        for (int q = 0; q < 8; q++) sum[q] += add[q];
     }
-    // exclusive prefixes and totals over the block: the four arena sizes,
-    // then the x-planes of each row-word class
-    int64_t vals[8], excl[8], tot[8];
+    int64_t excl[8], tot[8];
+    block_scan<8, kPlanThreads>(sum, excl, tot, s_w);
+    const bool overflow = plan_overflow(A, tot);
+    // plane_map slots: the classes one after another, chunk order inside a class
+    int64_t cbase[4], cb = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        vals[q] = sum[q];
-        vals[4 + q] = cls[q];
-    }
-    block_scan<8, kPlanThreads>(vals, excl, tot, s_w);
-    int64_t base[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) base[q] = excl[q];
-    const int64_t tot_samples = tot[0];
-    const int64_t tot_words = tot[1];
-    const int64_t tot_rows = tot[2];
-    const int64_t tot_tiles = tot[3];
-    const bool overflow = tot_samples > A.cap_samples || tot_words > A.cap_samples / 64 + A.cap_chunks ||
-                          tot_rows > A.cap_rows;
-    int64_t tp = base[3];
-    // plane_map slots: the classes one after another, chunk order inside a class
-    int64_t cslot[4];
-    {
-        int64_t cb = 0;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            cslot[q] = cb + excl[4 + q];
-            cb += tot[4 + q];
-        }
+        cbase[q] = cb;
+        cb += tot[4 + q];
     }
     for (int j = j0; j < j1; j++) {
-        ChunkMeta& m = A.meta[j];
-        m.dens_off += base[0];
-        m.lab_off += base[1];
-        m.row_off += base[2];
-        A.tile_prefix[j] = overflow ? 0 : tp;
-        if (m.state == kStateSampled) {
-            const int64_t ns = (int64_t)m.dim * m.dim * m.dim;
-            tp += (ns + A.sample_tile - 1) / A.sample_tile;
-            const int q = ((m.dim + 63) >> 6) - 1;
-            m.plane_slot = cslot[q];
-            cslot[q] += m.dim;
-            if (overflow) {
-                m.state = kStateSkipped;
-                m.status = BMF_ECAPACITY;
-            }
-        } else {
-            // culled chunks: DMCChunk.cpp:159-162 on uniform labels
-            m.contains_mesh = (m.state == kStateAir) && (m.dim % 32 != 0);
-        }
-        if (m.state != kStateSampled) write_result(A, j, m);  // final: k_scan skips it
+        A.tile_prefix[j] = overflow ? 0 : A.tile_prefix[j] + excl[3];
+        ChunkMeta m = A.meta[j];
+        plan_finish(A, j, m, excl, cbase, overflow);
     }
-    if (tid == 0) {
-        A.tile_prefix[n] = overflow ? 0 : tot_tiles;
-        Counters* c = A.ctr;
-        c->sample_head = 0;
-        if (reset_bumps) {
-            c->v_bump = 0;
-            c->i_bump = 0;
-            c->need_v = 0;
-            c->need_i = 0;
-            c->overflow = 0;
-            c->error = 0;
-        }
-        if (overflow) c->overflow = 1;
-        c->total_tiles = overflow ? 0 : tot_tiles;
-        // per row-word class: its planes in plane_map, and the row segments
-        // per plane (few sampled planes, e.g. single large chunks: split each
-        // plane's rows so that the class still has kSplitTarget items)
+    if (threadIdx.x == 0) plan_counters(A, n, tot, overflow, reset_bumps);
+}
+
+// Larger batches, pass 1 of 3 (one descriptor per thread over many blocks):
+// records with offsets relative to their block, and each block's totals.
+__global__ void __launch_bounds__(kPlanThreads) k_plan_part(Arenas A, int n, int allow_cull) {
+    __shared__ int64_t s_w[8][kPlanThreads / 64 + 1];
+    const int j = blockIdx.x * kPlanThreads + threadIdx.x;
+    int64_t add[8];
+    ChunkMeta m{};
+    if (j < n) m = plan_record(A, A.sampler, A.descs[j], allow_cull, add);
+    else
+        for (int q = 0; q < 8; q++) add[q] = 0;
+    int64_t excl[8], tot[8];
+    block_scan<8, kPlanThreads>(add, excl, tot, s_w);
+    if (j < n) {
+        A.tile_prefix[j] = excl[3];
+        m.dens_off = excl[0];
+        m.lab_off = excl[1];
+        m.row_off = excl[2];
+        if (m.state == kStateSampled) m.plane_slot = excl[4 + ((m.dim + 63) >> 6) - 1];
+        A.meta[j] = m;
+    }
+    if (threadIdx.x < 8) A.plan_part[8 * blockIdx.x + threadIdx.x] = tot[threadIdx.x];
+}
+
+// Pass 2 (one block, one thread per pass-1 block, at most kPlanThreads):
+// block bases, batch totals, overflow flag, counters.  Bases go to
+// plan_part[8 (nb + b) ...], the overflow flag after them.
+__global__ void __launch_bounds__(kPlanThreads) k_plan_total(Arenas A, int n, int nb, int reset_bumps) {
+    __shared__ int64_t s_w[8][kPlanThreads / 64 + 1];
+    const int b = threadIdx.x;
+    int64_t v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) v[q] = b < nb ? A.plan_part[8 * b + q] : 0;
+    int64_t excl[8], tot[8];
+    block_scan<8, kPlanThreads>(v, excl, tot, s_w);
+    const bool overflow = plan_overflow(A, tot);
+    if (b < nb)
+#pragma unroll
+        for (int q = 0; q < 8; q++) A.plan_part[8 * (nb + b) + q] = excl[q];
+    if (b == 0) {
+        A.plan_part[16 * nb] = overflow;
         int64_t cb = 0;
         for (int q = 0; q < 4; q++) {
-            const int64_t np = overflow ? 0 : tot[4 + q];
-            int split = A.row_split > 0 ? A.row_split : 1;
-            if (A.row_split <= 0)
-                while (split < kMaxPlaneSplit && np * split < kSplitTarget) split <<= 1;
-            c->plane_split[q] = split;
-            c->class_base[q] = cb;
-            c->class_planes[q] = np;
-            cb += np;
+            A.plan_part[16 * nb + 1 + q] = cb;
+            cb += tot[4 + q];
         }
-        c->n_smooth = 0;
-        c->total_samples = tot_samples;
+        plan_counters(A, n, tot, overflow, reset_bumps);
     }
+}
+
+// Pass 3: final offsets, sampling-tile prefixes, plane slots, the results
+// of culled chunks.
+__global__ void __launch_bounds__(kPlanThreads) k_plan_fix(Arenas A, int n, int nb) {
+    const int j = blockIdx.x * kPlanThreads + threadIdx.x;
+    if (j >= n) return;
+    int64_t base[8], cbase[4];
+#pragma unroll
+    for (int q = 0; q < 8; q++) base[q] = A.plan_part[8 * (nb + blockIdx.x) + q];
+    const bool overflow = A.plan_part[16 * nb] != 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) cbase[q] = A.plan_part[16 * nb + 1 + q];
+    A.tile_prefix[j] = overflow ? 0 : A.tile_prefix[j] + base[3];
+    ChunkMeta m = A.meta[j];
+    plan_finish(A, j, m, base, cbase, overflow);
 }
 
 // ------------------------------------------------------------------------
@@ -983,6 +1064,120 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_smooth(Arenas A, int n) {
     }
 }
 
+// Larger batches (more than kPlanOneBlock chunks): the same list and
+// prefixes in five passes over many blocks.  The grouped order puts rank k
+// (k-th smoothing chunk in chunk order) in group g = k % kXcds at place
+// k / kXcds, so the prefix at a position is the totals of the groups before
+// its own plus the sum over the earlier ranks of its residue class: block
+// scans carry one running sum per residue class.
+__device__ __forceinline__ int grouped_pos(int k, int ns) {
+    const int g = k & (kXcds - 1);
+    int pos = k / kXcds;
+    for (int h = 0; h < g; h++) pos += (ns - h + kXcds - 1) / kXcds;
+    return pos;
+}
+
+// Pass 1: smoothing chunks per block.
+__global__ void __launch_bounds__(kPlanThreads) k_smooth_count(Arenas A, int n) {
+    __shared__ int64_t s_w[1][kPlanThreads / 64 + 1];
+    const int j = blockIdx.x * kPlanThreads + threadIdx.x;
+    int64_t f[1] = {j < n && smooths(A.meta[j], A.normals)}, e[1], t[1];
+    block_scan<1, kPlanThreads>(f, e, t, s_w);
+    if (threadIdx.x == 0) A.plan_part[blockIdx.x] = t[0];
+}
+
+// Pass 2 (one block): rank base of each block, and the count.
+__global__ void __launch_bounds__(kPlanThreads) k_smooth_ranks(Arenas A, int nb) {
+    __shared__ int64_t s_w[1][kPlanThreads / 64 + 1];
+    const int b = threadIdx.x;
+    int64_t v[1] = {b < nb ? A.plan_part[b] : 0}, e[1], t[1];
+    block_scan<1, kPlanThreads>(v, e, t, s_w);
+    if (b < nb) A.plan_part[nb + b] = e[0];
+    if (b == 0) {
+        A.plan_part[2 * nb] = t[0];
+        A.ctr->n_smooth = (int)t[0];
+    }
+}
+
+// Pass 3: each smoothing chunk's rank and place in the list, and the block's
+// sums of vertex / triangle tiles per residue class (exclusive sums of the
+// chunk kept in plan_tmp).
+__global__ void __launch_bounds__(kPlanThreads) k_smooth_place(Arenas A, int n, int nb) {
+    __shared__ int64_t s_r[1][kPlanThreads / 64 + 1];
+    __shared__ int64_t s_w[2 * kXcds][kPlanThreads / 64 + 1];
+    const int j = blockIdx.x * kPlanThreads + threadIdx.x;
+    const bool on = j < n && smooths(A.meta[j], A.normals);
+    int64_t r1[1] = {on}, re[1], rt[1];
+    block_scan<1, kPlanThreads>(r1, re, rt, s_r);
+    const int k = (int)(A.plan_part[nb + blockIdx.x] + re[0]);
+    const int g = k & (kXcds - 1);
+    int64_t f[2 * kXcds], e[2 * kXcds], t[2 * kXcds];
+    for (int q = 0; q < 2 * kXcds; q++) f[q] = 0;
+    if (on) {
+        const ChunkMeta& M = A.meta[j];
+        f[g] = (M.V + kSmoothThreads - 1) / kSmoothThreads;
+        f[kXcds + g] = (M.T + kSmoothThreads - 1) / kSmoothThreads;
+        A.smooth_list[grouped_pos(k, (int)A.plan_part[2 * nb])] = j;
+    }
+    block_scan<2 * kXcds, kPlanThreads>(f, e, t, s_w);
+    if (on) {
+        A.plan_tmp[2 * (int64_t)j] = e[g];
+        A.plan_tmp[2 * (int64_t)j + 1] = e[kXcds + g];
+    }
+    if (threadIdx.x < 2 * kXcds) A.plan_part[2 * nb + 1 + 2 * kXcds * blockIdx.x + threadIdx.x] = t[threadIdx.x];
+}
+
+// Pass 4 (one block): per block and class the sums of the blocks before it,
+// the group totals, their bases (= each XCD group's first tile) and totals.
+__global__ void __launch_bounds__(kPlanThreads) k_smooth_bases(Arenas A, int nb) {
+    __shared__ int64_t s_w[2 * kXcds][kPlanThreads / 64 + 1];
+    const int b = threadIdx.x;
+    int64_t* part = A.plan_part + 2 * nb + 1;
+    int64_t* base = part + 2 * kXcds * nb;
+    int64_t v[2 * kXcds], e[2 * kXcds], t[2 * kXcds];
+    for (int q = 0; q < 2 * kXcds; q++) v[q] = b < nb ? part[2 * kXcds * b + q] : 0;
+    block_scan<2 * kXcds, kPlanThreads>(v, e, t, s_w);
+    if (b < nb)
+        for (int q = 0; q < 2 * kXcds; q++) base[2 * kXcds * b + q] = e[q];
+    if (b == 0) {
+        int64_t gv = 0, gt = 0;
+        int64_t* gbase = base + 2 * kXcds * nb;
+        for (int g = 0; g < kXcds; g++) {
+            gbase[g] = gv;
+            gbase[kXcds + g] = gt;
+            A.smooth_xcd_v[g] = gv;
+            A.smooth_xcd_t[g] = gt;
+            gv += t[g];
+            gt += t[kXcds + g];
+        }
+        A.smooth_xcd_v[kXcds] = gv;
+        A.smooth_xcd_t[kXcds] = gt;
+        const int ns = (int)A.plan_part[2 * nb];
+        A.smooth_prefix[ns] = gv;
+        A.smooth_tprefix[ns] = gt;
+        A.ctr->total_vtiles = gv;
+        A.ctr->total_ttiles = gt;
+    }
+}
+
+// Pass 5: the tile prefixes at each list position.
+__global__ void __launch_bounds__(kPlanThreads) k_smooth_prefix(Arenas A, int n, int nb) {
+    __shared__ int64_t s_r[1][kPlanThreads / 64 + 1];
+    const int j = blockIdx.x * kPlanThreads + threadIdx.x;
+    const bool on = j < n && smooths(A.meta[j], A.normals);
+    int64_t r1[1] = {on}, re[1], rt[1];
+    block_scan<1, kPlanThreads>(r1, re, rt, s_r);
+    if (!on) return;
+    const int k = (int)(A.plan_part[nb + blockIdx.x] + re[0]);
+    const int ns = (int)A.plan_part[2 * nb];
+    const int g = k & (kXcds - 1);
+    const int64_t* base = A.plan_part + 2 * nb + 1 + 2 * kXcds * nb;
+    const int64_t* gbase = base + 2 * kXcds * nb;
+    const int pos = grouped_pos(k, ns);
+    A.smooth_prefix[pos] = gbase[g] + base[2 * kXcds * blockIdx.x + g] + A.plan_tmp[2 * (int64_t)j];
+    A.smooth_tprefix[pos] = gbase[kXcds + g] + base[2 * kXcds * blockIdx.x + kXcds + g] + A.plan_tmp[2 * (int64_t)j + 1];
+}
+
 // ------------------------------------------------------------------------
 // emit: one block per (chunk, x-plane, row segment), lanes = z, in slabs of
 // at most kEmitRows label rows:
@@ -1649,7 +1844,15 @@ __global__ void k_ref_masks(Arenas A, uint64_t* masks) {
 // launch wrappers
 // ------------------------------------------------------------------------
 hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, hipStream_t s) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, s, A, n, allow_cull, reset_bumps);
+    if (n <= kPlanOneBlock) {
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, s, A, n, allow_cull, reset_bumps);
+    } else {
+        const int nb = (n + kPlanThreads - 1) / kPlanThreads;
+        if (nb > kPlanThreads) return hipErrorInvalidValue;  // beyond BMF's 2^20 chunks
+        hipLaunchKernelGGL(k_plan_part, dim3(nb), dim3(kPlanThreads), 0, s, A, n, allow_cull);
+        hipLaunchKernelGGL(k_plan_total, dim3(1), dim3(kPlanThreads), 0, s, A, n, nb, reset_bumps);
+        hipLaunchKernelGGL(k_plan_fix, dim3(nb), dim3(kPlanThreads), 0, s, A, n, nb);
+    }
     if (A.csg_list) hipLaunchKernelGGL(k_csg_cull, dim3((unsigned)(n < 2048 ? n : 2048)), dim3(256), 0, s, A, n);
     return hipGetLastError();
 }
@@ -1688,7 +1891,16 @@ hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStr
 }
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_scan, dim3(blocks), dim3(kScanThreads), 0, s, A, n);
-    hipLaunchKernelGGL(k_plan_smooth, dim3(1), dim3(kPlanThreads), 0, s, A, n);
+    if (n <= kPlanOneBlock) {
+        hipLaunchKernelGGL(k_plan_smooth, dim3(1), dim3(kPlanThreads), 0, s, A, n);
+    } else {
+        const int nb = (n + kPlanThreads - 1) / kPlanThreads;
+        hipLaunchKernelGGL(k_smooth_count, dim3(nb), dim3(kPlanThreads), 0, s, A, n);
+        hipLaunchKernelGGL(k_smooth_ranks, dim3(1), dim3(kPlanThreads), 0, s, A, nb);
+        hipLaunchKernelGGL(k_smooth_place, dim3(nb), dim3(kPlanThreads), 0, s, A, n, nb);
+        hipLaunchKernelGGL(k_smooth_bases, dim3(1), dim3(kPlanThreads), 0, s, A, nb);
+        hipLaunchKernelGGL(k_smooth_prefix, dim3(nb), dim3(kPlanThreads), 0, s, A, n, nb);
+    }
     return hipGetLastError();
 }
 hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs) {
