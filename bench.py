@@ -461,6 +461,7 @@ def main():
     value = n * args.steps / wall
     rc, res, _, _ = ctx.results(n)
     capi.check(rc, "bmf_result")
+    work = ctx.frame_stats()
     sampled = sum(r.sampled for r in res)
     V = sum(r.vertex_count for r in res)
     T = sum(r.index_count for r in res) // 3
@@ -479,6 +480,8 @@ def main():
     flops = samples * roofline.terrain_flops_per_sample(octaves)
     sample_s = stages["sample"] * 1e-3
     achieved_tflops = flops / sample_s / 1e12
+    exec_flops = (work["octave_evals"] * (roofline.SIMPLEX_FLOPS + roofline.OCTAVE_FLOPS)
+                  + work["samples"] * roofline.TERRAIN_FLOPS)
     traffic = None
     tfile = ROOT / "profiles" / "sample_kernel_uniform512x64_traffic.json"
     if tfile.exists():
@@ -498,6 +501,14 @@ def main():
         "algorithmic_flops_per_launch": flops,
         "algorithmic_bytes_per_launch": int(samples * roofline.LABEL_BYTES),
         "avg_launch_ms": round(stages["sample"], 4),
+        # what the kernel executed: the adaptive evaluation stops a sample's octaves once its
+        # sign is decided (octaves counted by the kernel itself), against the FP32 peak
+        # without packed math (the kernels are built without v_pk_* FP32 instructions)
+        "executed": {"octave_evals": work["octave_evals"], "samples": work["samples"],
+                     "octaves_per_sample": round(work["octave_evals"] / max(1, work["samples"]), 4),
+                     "flops_per_launch": exec_flops, "tflops": round(exec_flops / sample_s / 1e12, 3),
+                     "frac_of_unpacked_peak": round(exec_flops / sample_s / 1e12
+                                                    / roofline.PEAK_FP32_UNPACKED_TFLOPS, 4)},
         "hbm_frac": round((traffic or samples * roofline.LABEL_BYTES) / sample_s / 1e9 / roofline.PEAK_HBM_GBS, 5),
     }
     # whole frame: full-octave sampling + smoothing FLOPs and the compulsory
@@ -583,11 +594,26 @@ def main():
             pyoracle.extract_batch(spec, ochunks, threads)
             times.append(time.perf_counter() - t0)
         cpu_s = float(np.median(times))
+        # like for like: the oracle on the chunks the GPU samples only (the others are classified
+        # by the plan's |fbm| <= 1 bound with no sampling, which a CPU loop can apply as cheaply)
+        rc0, res0, _, _ = ctx.results(n)
+        sub = [oc for oc, r in zip(ochunks, res0) if r.sampled]
+        sub_times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            pyoracle.extract_batch(spec, sub, threads)
+            sub_times.append(time.perf_counter() - t0)
+        sub_s = float(np.median(sub_times))
         cpu = {"value": round(n / cpu_s, 3), "unit": "chunks/s", "cores": threads, "kind": "port",
                "sample": f"all {n} chunks of the workload; median of {runs} passes after a warm-up "
                          f"({cpu_s:.2f} s wall per pass)",
+               "culled_value": round(n / sub_s, 3),
+               "culled_sample": f"the same batch with the GPU's analytic culling applied: the oracle on the "
+                                f"{len(sub)} chunks the GPU samples, the other {n - len(sub)} taken as classified "
+                                f"({sub_s:.3f} s wall per pass, median of {runs}); every octave is still evaluated "
+                                f"(the GPU stops early where the sign is decided)",
                "host": host_cpu()}
-        log(f"[bench] CPU: {n / cpu_s:.2f} chunks/s")
+        log(f"[bench] CPU: {n / cpu_s:.2f} chunks/s; with the same culling {n / sub_s:.2f} chunks/s")
         # the batched path's meshes vs the CPU run
         capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
         rc, res, _, _ = ctx.results(n)

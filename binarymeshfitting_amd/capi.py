@@ -32,7 +32,7 @@ EXPORTS = [
     "bmf_backward_diff", "bmf_get_tuning", "bmf_set_tuning", "bmf_autotune", "bmf_set_mesh_options",
     "bmf_get_mesh_options", "bmf_qef_error_to_host", "bmf_normals_to_host", "bmf_tape_set_targets",
     "bmf_tape_loss_nearest", "bmf_loss_nearest_noalloc", "bmf_format", "bmf_lod_queue", "bmf_extract_queued", "bmf_stitch",
-    "bmf_tape_timing", "bmf_noise_preset",
+    "bmf_tape_timing", "bmf_noise_preset", "bmf_get_frame_stats",
 ]
 SPACE_GRID, SPACE_WORLD = 0, 1
 COLOR_NONE, COLOR_WHITE, COLOR_NOISE = 0, 1, 2
@@ -176,6 +176,7 @@ def lib():
         L.bmf_backward_diff.argtypes = [tp, C.c_void_p, C.c_void_p]
         L.bmf_tape_timing.argtypes = [tp, P(C.c_float)]
         L.bmf_noise_preset.argtypes = [C.c_int32, C.c_float, C.c_uint32, P(SamplerDesc)]
+        L.bmf_get_frame_stats.argtypes = [ctxp, P(C.c_int64 * 3)]
         L.bmf_tape_set_targets.argtypes = [tp, C.c_void_p, C.c_int64]
         L.bmf_tape_loss_nearest.argtypes = [tp, C.c_void_p, C.c_void_p, P(C.c_float)]
         L.bmf_loss_nearest_noalloc.argtypes = [tp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -405,6 +406,12 @@ class Context:
         t = Tuning()
         check(lib().bmf_autotune(self.h, arr, len(chunks), reps, C.byref(t)), "bmf_autotune")
         return t.as_dict()
+
+    def frame_stats(self):
+        """bmf_get_frame_stats of the last extraction (after results())."""
+        out = (C.c_int64 * 3)()
+        check(lib().bmf_get_frame_stats(self.h, C.byref(out)), "bmf_get_frame_stats")
+        return {"samples": out[0], "octave_evals": out[1], "refine_evals": out[2]}
 
     def profile_stages(self, chunks):
         arr = chunk_array(chunks)

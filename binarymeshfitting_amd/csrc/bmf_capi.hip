@@ -787,6 +787,15 @@ int bmf_last_timing(bmf_ctx* c, float* total_ms) {
     return BMF_OK;
 }
 
+int bmf_get_frame_stats(bmf_ctx* c, bmf_frame_stats* out) {
+    if (!c || !out) return BMF_EINVAL;
+    if (!c->results_valid) return BMF_ESTATE;
+    out->samples = c->h_ctr->total_samples;
+    out->octave_evals = (int64_t)c->h_ctr->octave_evals;
+    out->refine_evals = (int64_t)c->h_ctr->refine_evals;
+    return BMF_OK;
+}
+
 int bmf_profile_stages(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, float* stage_ms) {
     if (!c || !stage_ms) return BMF_EINVAL;
     if (!c->sampler_set) return BMF_ESTATE;

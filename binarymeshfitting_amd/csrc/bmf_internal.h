@@ -99,6 +99,8 @@ struct Counters {
     int32_t plane_split[4];    // row segments per x-plane item of row-word class W (index W-1)
     int64_t class_base[4];     // first plane_map entry of class W's planes
     int64_t class_planes[4];   // sampled planes of class W
+    unsigned long long octave_evals;  // terrain noise octaves evaluated by the sampling kernel
+    unsigned long long refine_evals;  // and by the refinement kernel
 };
 
 // Device pointers of one context.

@@ -158,6 +158,8 @@ __device__ __forceinline__ void plan_counters(const Arenas& A, int n, const int6
     A.tile_prefix[n] = overflow ? 0 : tot[3];
     Counters* c = A.ctr;
     c->sample_head = 0;
+    c->octave_evals = 0;
+    c->refine_evals = 0;
     if (reset_bumps) {
         c->v_bump = 0;
         c->i_bump = 0;
@@ -476,6 +478,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
     const int64_t total = A.ctr->total_tiles;
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
+    unsigned long long evals = 0;  // adaptive mode: octaves this wave evaluated (wave-uniform)
     // (chunk, x) of every x-plane item of the plane kernels that follow
     for (int j = blockIdx.x; j < n; j += gridDim.x) {
         const ChunkMeta& M = A.meta[j];
@@ -569,7 +572,10 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                 // evaluates their octaves here, as the rounds would (same
                 // operations); the survivors of a sparse wave are queued with
                 // their next octave, so that the rounds run on dense waves
-                while (kq < O && __popcll(__ballot(valid && dec == 0)) >= kInPlaceOctave) {
+                for (;;) {
+                    const uint32_t live = (uint32_t)__popcll(__ballot(valid && dec == 0));
+                    if (kq >= O || live < (uint32_t)kInPlaceOctave) break;
+                    evals += live;
                     if (valid && dec == 0) {
                         const float f = s_oct[0][kq];
                         sum0 = __builtin_fmaf(s_oct[1][kq], snoise3(px * f, py * f, pz * f, seed0 + (uint32_t)kq), sum0);
@@ -657,6 +663,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
                         }
                         if (dec == 1) atomicOr(&s_lab[li >> 6], 1ull << (li & 63));
                     }
+                    evals += (unsigned long long)__popcll(__ballot(active));
                     const uint64_t push = __ballot(active && dec == 0);
                     if (push) {
                         uint32_t base = 0;
@@ -696,6 +703,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
         }
         __syncthreads();
     }
+    if (adapt && lane == 0 && evals) atomicAdd(&A.ctr->octave_evals, evals);
 }
 
 // ------------------------------------------------------------------------
@@ -810,6 +818,8 @@ __global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A, int n) {
             const uint32_t* __restrict__ q = A.refine + M.dens_off + (int64_t)x * dim * dim + (int64_t)y0 * dim;
             float* __restrict__ D = A.dens + M.dens_off + (int64_t)x * dim * dim;
             const float px = (float)x * M.delta + M.opos[0];
+            if (threadIdx.x == 0 && S->kind == BMF_SAMPLER_TERRAIN)
+                atomicAdd(&A.ctr->refine_evals, (unsigned long long)cnt * (unsigned long long)S->octaves);
             for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) {
                 const uint32_t r = q[e];  // y * dim + z within the plane
                 const uint32_t y = r / dim, z = r - y * dim;

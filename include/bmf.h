@@ -206,6 +206,20 @@ int bmf_cell_masks(bmf_ctx* ctx, const bmf_chunk_desc* chunk, uint64_t* masks);
  * ms).  Replaces the reference's clock() batch timer (WorldWatcher.cpp:69-73). */
 int bmf_last_timing(bmf_ctx* ctx, float* total_ms);
 
+/* Work done by the last extraction: the samples of its sampled chunks and,
+ * for the terrain sampler, the noise octaves that its adaptive sampling
+ * evaluated, summed over samples (every octave of every sample, as the
+ * reference's block sampler computes, would be samples * octaves), and those
+ * the refinement of early-labelled edge endpoints evaluated (0 for the other
+ * samplers).  Valid after bmf_result.  Counterpart of the reference's
+ * per-stage counters in its timing printout (DMCChunk.cpp:785-840). */
+typedef struct {
+    int64_t samples;
+    int64_t octave_evals;
+    int64_t refine_evals;
+} bmf_frame_stats;
+int bmf_get_frame_stats(bmf_ctx* ctx, bmf_frame_stats* out);
+
 /* Runs the BATCHED path once without a graph, with HIP events between the
  * stages on the context stream, and returns the stage times in ms:
  * [0] plan, [1] sample, [2] count, [3] scan, [4] emit, [5] smoothing (all

@@ -410,3 +410,27 @@ def test_csg_culled_sampling_matches_full_union(ctx, oracle):
     for j, c in enumerate(picks):
         pos, idx, bnd = ctx.mesh(j)
         _check_mesh(oracle, spec, c, pos, idx, bnd, res[j], tol=0.0)
+
+
+def test_frame_stats_count_the_adaptive_work(ctx):
+    """bmf_get_frame_stats: the samples of the sampled chunks, and octave
+    counts between the decided-at-once floor and every octave of every
+    sample; the refinement re-evaluates whole samples."""
+    ctx.set_sampler(**TERRAIN)
+    wl = workloads.uniform_grid(n_side=4, c=64, iters=2)
+    ctx.extract(workloads.to_capi(wl.chunks), capi.MODE_BATCHED)
+    rc, res, _, _ = ctx.results(len(wl.chunks))
+    capi.check(rc, "bmf_result")
+    st = ctx.frame_stats()
+    samples = sum(c.dim ** 3 for c, r in zip(wl.chunks, res) if r.sampled)
+    O = TERRAIN["octaves"]
+    assert st["samples"] == samples
+    assert 0 < st["octave_evals"] < samples * O
+    assert st["refine_evals"] % O == 0
+    print(st, f"{st['octave_evals'] / samples:.3f} octaves per sample")
+    ctx.set_sampler(kind=capi.SAMPLER_SPHERE, world_size=48.0)
+    ctx.extract(workloads.to_capi([workloads.Chunk((-20.0, -19.0, -21.5), 40.0, 41)]), capi.MODE_BATCHED)
+    rc, _, _, _ = ctx.results(1)
+    capi.check(rc, "bmf_result")
+    st = ctx.frame_stats()
+    assert st["samples"] == 41 ** 3 and st["octave_evals"] == 0 and st["refine_evals"] == 0
