@@ -34,7 +34,7 @@ hipError_t dalloc(T** p, int64_t count) {
 
 void free_arenas(bmf_ctx* c) {
     Arenas& A = c->A;
-    void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plan_part, A.plan_tmp, A.plane_map, A.smooth_list, A.smooth_prefix,
+    void* ptrs[] = {A.meta,    A.descs,     A.results,  A.tile_prefix, A.plan_part, A.plan_tmp, A.stat_blocks, A.plane_map, A.smooth_list, A.smooth_prefix,
                     A.smooth_tprefix, A.smooth_xcd_v, A.smooth_xcd_t, A.smooth_vmap, A.smooth_tmap, A.refine, A.plane_refine_n, A.cen4,
                     A.dens,    A.labels,    A.exact,    A.row_v,       A.row_t,        A.cell_tb,   A.cell_mask,
                     A.pos,     A.pos4,      A.boundary, A.vedge,       A.vring,        A.idx,          A.ctr,
@@ -86,6 +86,8 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
     BMF_HIP(launch_plan(A, n, allow_cull, reset, s));
     BMF_MARK(1);
     BMF_HIP(launch_sample(A, n, sample_blocks, 1, c->h_sampler.kind, c->h_sampler.noise_dims == 2, s));
+    c->stat_nsample = sample_blocks;
+    c->stat_nrefine = row_blocks;
     BMF_MARK(2);
     const bool quads = A.mesh_mode == BMF_MESH_QUADS;
     hipStream_t cs[4];
@@ -321,12 +323,14 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
     A.cap_indices = lim->max_indices;
     A.sample_tile = kSampleTileDefault;
     A.smoothing_follows = 1;
+    A.stat_cap = 64 * c->cu_count;
     A.row_split = -1;
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && dalloc(&A.meta, nc) == hipSuccess && dalloc(&A.descs, nc) == hipSuccess &&
          dalloc(&A.results, nc) == hipSuccess && dalloc(&A.tile_prefix, nc + 1) == hipSuccess &&
          dalloc(&A.plan_part, 40 * ((nc + 1023) / 1024) + 64) == hipSuccess &&
          dalloc(&A.plan_tmp, 2 * nc) == hipSuccess &&
+         dalloc(&A.stat_blocks, 2 * (int64_t)(64 * c->cu_count)) == hipSuccess &&
          dalloc(&A.plane_map, nc * BMF_MAX_DIM) == hipSuccess && dalloc(&A.smooth_list, nc) == hipSuccess &&
          dalloc(&A.smooth_prefix, nc + 1) == hipSuccess && dalloc(&A.smooth_tprefix, nc + 1) == hipSuccess &&
          dalloc(&A.smooth_xcd_v, kXcds + 1) == hipSuccess && dalloc(&A.smooth_xcd_t, kXcds + 1) == hipSuccess &&
@@ -791,8 +795,21 @@ int bmf_get_frame_stats(bmf_ctx* c, bmf_frame_stats* out) {
     if (!c || !out) return BMF_EINVAL;
     if (!c->results_valid) return BMF_ESTATE;
     out->samples = c->h_ctr->total_samples;
-    out->octave_evals = (int64_t)c->h_ctr->octave_evals;
-    out->refine_evals = (int64_t)c->h_ctr->refine_evals;
+    // per-workgroup counts of the last pipeline's two kernels (plain stores there)
+    const int ns = std::min(c->stat_nsample, c->A.stat_cap), nr = std::min(c->stat_nrefine, c->A.stat_cap);
+    std::vector<unsigned long long> h((size_t)std::max(ns, nr));
+    BMF_HIP(hipSetDevice(c->device));
+    BMF_HIP(hipStreamSynchronize(c->stream));
+    int64_t sum[2] = {0, 0};
+    for (int k = 0; k < 2; k++) {
+        const int m = k == 0 ? ns : nr;
+        if (m <= 0) continue;
+        BMF_HIP(hipMemcpy(h.data(), c->A.stat_blocks + (size_t)k * c->A.stat_cap, sizeof(unsigned long long) * m,
+                          hipMemcpyDeviceToHost));
+        for (int b = 0; b < m; b++) sum[k] += (int64_t)h[b];
+    }
+    out->octave_evals = sum[0];
+    out->refine_evals = sum[1];
     return BMF_OK;
 }
 

@@ -99,8 +99,6 @@ struct Counters {
     int32_t plane_split[4];    // row segments per x-plane item of row-word class W (index W-1)
     int64_t class_base[4];     // first plane_map entry of class W's planes
     int64_t class_planes[4];   // sampled planes of class W
-    unsigned long long octave_evals;  // terrain noise octaves evaluated by the sampling kernel
-    unsigned long long refine_evals;  // and by the refinement kernel
 };
 
 // Device pointers of one context.
@@ -112,6 +110,9 @@ struct Arenas {
     int64_t* plan_part;        // [40 nb + 64] multi-block plan passes (nb = max_chunks / 1024 rounded up):
                                // block totals and bases, flags, class and group bases
     int64_t* plan_tmp;         // [2 max_chunks] multi-block smoothing plan: per-chunk exclusive tile sums
+    unsigned long long* stat_blocks;  // [2][stat_cap] terrain octaves evaluated per workgroup of
+                                      // the sampling and the refinement kernels (plain stores, no atomics)
+    int32_t stat_cap;
     int2* plane_map;           // [max_chunks * BMF_MAX_DIM] (chunk, x) of each sampled x-plane, grouped by
                                // row-word class (written by k_sample's prologue)
     int32_t* smooth_list;      // [max_chunks] chunks that run the smoothing

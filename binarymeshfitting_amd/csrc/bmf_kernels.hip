@@ -34,7 +34,7 @@ constexpr int kPlanThreads = 1024;
 // Up to this many descriptors the plans run in one block (a few per thread,
 // one launch each); larger batches take the multi-block passes, whose extra
 // launches cost more than they save below it.
-constexpr int kPlanOneBlock = 4 * kPlanThreads;
+constexpr int kPlanOneBlock = 2 * kPlanThreads;
 
 // The host-visible result of chunk j from its final record: written by
 // k_plan for the chunks it culls or skips and by k_scan for the sampled ones
@@ -55,6 +55,16 @@ __device__ __forceinline__ void write_result(const Arenas& A, int j, const Chunk
     r.overlap_pos[1] = M.opos[1];
     r.overlap_pos[2] = M.opos[2];
     A.results[j] = r;
+}
+
+// a[i] of a register array by unrolled selects (a runtime index into a
+// register array would place it in scratch memory)
+template <int N>
+__device__ __forceinline__ int64_t pick(const int64_t (&a)[N], int i) {
+    int64_t r = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) r = k == i ? a[k] : r;
+    return r;
 }
 
 // Plan record of one descriptor: DMCChunk::label_grid geometry
@@ -119,7 +129,9 @@ __device__ __forceinline__ ChunkMeta plan_record(const Arenas& A, const SamplerD
     add[1] = m.lab_words;
     add[2] = sampled ? (int64_t)d.dim * d.dim : 0;
     add[3] = sampled ? (ns + A.sample_tile - 1) / A.sample_tile : 0;
-    if (sampled) add[4 + ((d.dim + 63) >> 6) - 1] = d.dim;
+    const int cls = ((d.dim + 63) >> 6) - 1;
+#pragma unroll
+    for (int q = 0; q < 4; q++) add[4 + q] = sampled && q == cls ? d.dim : 0;
     return m;
 }
 
@@ -136,7 +148,13 @@ __device__ __forceinline__ void plan_finish(const Arenas& A, int j, ChunkMeta& m
     m.row_off += base[2];
     if (m.state == kStateSampled) {
         const int q = ((m.dim + 63) >> 6) - 1;
-        m.plane_slot += cbase[q] + base[4 + q];
+        int64_t cb = 0, bb = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            cb = k == q ? cbase[k] : cb;
+            bb = k == q ? base[4 + k] : bb;
+        }
+        m.plane_slot += cb + bb;
         if (overflow) {
             m.state = kStateSkipped;
             m.status = BMF_ECAPACITY;
@@ -158,8 +176,6 @@ __device__ __forceinline__ void plan_counters(const Arenas& A, int n, const int6
     A.tile_prefix[n] = overflow ? 0 : tot[3];
     Counters* c = A.ctr;
     c->sample_head = 0;
-    c->octave_evals = 0;
-    c->refine_evals = 0;
     if (reset_bumps) {
         c->v_bump = 0;
         c->i_bump = 0;
@@ -200,7 +216,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan(Arenas A, int n, int allo
         m.lab_off = sum[1];
         m.row_off = sum[2];
         A.tile_prefix[j] = sum[3];
-        if (m.state == kStateSampled) m.plane_slot = sum[4 + ((m.dim + 63) >> 6) - 1];
+        if (m.state == kStateSampled) m.plane_slot = pick<8>(sum, 4 + ((m.dim + 63) >> 6) - 1);
         A.meta[j] = m;
 #pragma unroll
         for (int q = 0; q < 8; q++) sum[q] += add[q];
@@ -240,7 +256,7 @@ __global__ void __launch_bounds__(kPlanThreads) k_plan_part(Arenas A, int n, int
         m.dens_off = excl[0];
         m.lab_off = excl[1];
         m.row_off = excl[2];
-        if (m.state == kStateSampled) m.plane_slot = excl[4 + ((m.dim + 63) >> 6) - 1];
+        if (m.state == kStateSampled) m.plane_slot = pick<8>(excl, 4 + ((m.dim + 63) >> 6) - 1);
         A.meta[j] = m;
     }
     if (threadIdx.x < 8) A.plan_part[8 * blockIdx.x + threadIdx.x] = tot[threadIdx.x];
@@ -479,6 +495,8 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt();
     unsigned long long evals = 0;  // adaptive mode: octaves this wave evaluated (wave-uniform)
+    __shared__ unsigned long long s_evals;
+    if (threadIdx.x == 0) s_evals = 0;  // ordered before its use by the tile loop's barriers
     // (chunk, x) of every x-plane item of the plane kernels that follow
     for (int j = blockIdx.x; j < n; j += gridDim.x) {
         const ChunkMeta& M = A.meta[j];
@@ -703,7 +721,10 @@ __global__ void __launch_bounds__(kSampleThreads) k_sample(Arenas A, int n) {
         }
         __syncthreads();
     }
-    if (adapt && lane == 0 && evals) atomicAdd(&A.ctr->octave_evals, evals);
+    // the workgroup's count: one LDS add per wave, one plain store per workgroup
+    if (adapt && lane == 0 && evals) atomicAdd(&s_evals, evals);
+    __syncthreads();
+    if (threadIdx.x == 0 && (int)blockIdx.x < A.stat_cap) A.stat_blocks[blockIdx.x] = s_evals;
 }
 
 // ------------------------------------------------------------------------
@@ -801,6 +822,7 @@ __device__ __forceinline__ void count_row(const Arenas& A, const ChunkMeta& M, c
 // ------------------------------------------------------------------------
 __global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A, int n) {
     const SamplerDev* __restrict__ S = A.sampler;
+    unsigned long long evals = 0;  // terrain octaves this workgroup evaluated (uniform)
     for (int cw = 0; cw < 4; cw++) {  // the row-word classes in turn
         const int split = A.ctr->plane_split[cw];
         const int64_t base = A.ctr->class_base[cw];
@@ -818,8 +840,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A, int n) {
             const uint32_t* __restrict__ q = A.refine + M.dens_off + (int64_t)x * dim * dim + (int64_t)y0 * dim;
             float* __restrict__ D = A.dens + M.dens_off + (int64_t)x * dim * dim;
             const float px = (float)x * M.delta + M.opos[0];
-            if (threadIdx.x == 0 && S->kind == BMF_SAMPLER_TERRAIN)
-                atomicAdd(&A.ctr->refine_evals, (unsigned long long)cnt * (unsigned long long)S->octaves);
+            if (S->kind == BMF_SAMPLER_TERRAIN) evals += (unsigned long long)cnt * (unsigned long long)S->octaves;
             for (uint32_t e = threadIdx.x; e < cnt; e += blockDim.x) {
                 const uint32_t r = q[e];  // y * dim + z within the plane
                 const uint32_t y = r / dim, z = r - y * dim;
@@ -829,6 +850,7 @@ __global__ void __launch_bounds__(kSampleThreads) k_refine(Arenas A, int n) {
             }
         }
     }
+    if (threadIdx.x == 0 && (int)blockIdx.x < A.stat_cap) A.stat_blocks[A.stat_cap + blockIdx.x] = evals;
 }
 
 // Instantiated per row-word class W = ceil(dim/64); blocks of other classes
@@ -1122,19 +1144,25 @@ __global__ void __launch_bounds__(kPlanThreads) k_smooth_place(Arenas A, int n, 
     const int k = (int)(A.plan_part[nb + blockIdx.x] + re[0]);
     const int g = k & (kXcds - 1);
     int64_t f[2 * kXcds], e[2 * kXcds], t[2 * kXcds];
-    for (int q = 0; q < 2 * kXcds; q++) f[q] = 0;
+    int64_t vt = 0, tt = 0;
     if (on) {
         const ChunkMeta& M = A.meta[j];
-        f[g] = (M.V + kSmoothThreads - 1) / kSmoothThreads;
-        f[kXcds + g] = (M.T + kSmoothThreads - 1) / kSmoothThreads;
+        vt = (M.V + kSmoothThreads - 1) / kSmoothThreads;
+        tt = (M.T + kSmoothThreads - 1) / kSmoothThreads;
         A.smooth_list[grouped_pos(k, (int)A.plan_part[2 * nb])] = j;
+    }
+#pragma unroll
+    for (int q = 0; q < kXcds; q++) {
+        f[q] = q == g ? vt : 0;
+        f[kXcds + q] = q == g ? tt : 0;
     }
     block_scan<2 * kXcds, kPlanThreads>(f, e, t, s_w);
     if (on) {
-        A.plan_tmp[2 * (int64_t)j] = e[g];
-        A.plan_tmp[2 * (int64_t)j + 1] = e[kXcds + g];
+        A.plan_tmp[2 * (int64_t)j] = pick<2 * kXcds>(e, g);
+        A.plan_tmp[2 * (int64_t)j + 1] = pick<2 * kXcds>(e, kXcds + g);
     }
-    if (threadIdx.x < 2 * kXcds) A.plan_part[2 * nb + 1 + 2 * kXcds * blockIdx.x + threadIdx.x] = t[threadIdx.x];
+    if (threadIdx.x < 2 * kXcds)
+        A.plan_part[2 * nb + 1 + 2 * kXcds * blockIdx.x + threadIdx.x] = pick<2 * kXcds>(t, (int)threadIdx.x);
 }
 
 // Pass 4 (one block): per block and class the sums of the blocks before it,
@@ -1145,13 +1173,16 @@ __global__ void __launch_bounds__(kPlanThreads) k_smooth_bases(Arenas A, int nb)
     int64_t* part = A.plan_part + 2 * nb + 1;
     int64_t* base = part + 2 * kXcds * nb;
     int64_t v[2 * kXcds], e[2 * kXcds], t[2 * kXcds];
+#pragma unroll
     for (int q = 0; q < 2 * kXcds; q++) v[q] = b < nb ? part[2 * kXcds * b + q] : 0;
     block_scan<2 * kXcds, kPlanThreads>(v, e, t, s_w);
     if (b < nb)
+#pragma unroll
         for (int q = 0; q < 2 * kXcds; q++) base[2 * kXcds * b + q] = e[q];
     if (b == 0) {
         int64_t gv = 0, gt = 0;
         int64_t* gbase = base + 2 * kXcds * nb;
+#pragma unroll
         for (int g = 0; g < kXcds; g++) {
             gbase[g] = gv;
             gbase[kXcds + g] = gt;
