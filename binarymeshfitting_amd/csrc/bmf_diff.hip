@@ -23,11 +23,8 @@
 //                           wave's loads of one slot are contiguous
 //   deg u8 [V], mv u8 [V]   ring size; whether the primal step moves the
 //                           vertex (MeshProcessor.cpp:247)
-//   nbr   u32 [kMaxNbr][V], nnz u8 [V], bscale f32 [V]   the transposed
-//                           smoothing step (ELL rows, slot-major): each
-//                           vertex's distinct ring neighbours with the number
-//                           of its ring triangles they share (id | m << 27),
-//                           and 1 / (3 |ring|) for moved vertices (else 1)
+//   bscale f32 [V]          1 / (3 |ring|) for moved vertices, else 1 (the
+//                           scale of the backward's gradient records)
 //   p4, q4 float4 [V]       working positions (ping-pong); ckpt float4 [slots][V]
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -52,13 +49,9 @@ constexpr int kStepThreads = 64;  // smoothing steps: one wave per workgroup spr
 // one edge in at most 5 table slots (MCTable.h:27-283), so a ring holds at
 // most 20 triangles.
 constexpr int kMaxRing = 20;
-// A ring of n triangles touches at most 2n other vertices.
-constexpr int kMaxNbr = 2 * kMaxRing;
-// Vertex ids share a word with a multiplicity (5 bits) in the neighbour
-// records and with a corner place (2 bits) in the ring records.
-constexpr int kVidBits = 27;
-constexpr uint32_t kVidMask = (1u << kVidBits) - 1u;
-constexpr int64_t kMaxTapeVertices = int64_t(1) << kVidBits;
+// Vertex ids share a word with a corner place (2 bits) in the ring records.
+constexpr uint32_t kVidMask = 0x3FFFFFFFu;
+constexpr int64_t kMaxTapeVertices = int64_t(1) << 30;
 constexpr int kLeaf = 8;       // targets per leaf box of the nearest-target hierarchy
 constexpr int kNNThreads = 256;  // four vertices per workgroup, one wave each
 constexpr int kNNFan = 6;        // a wave tests 2^kNNFan = 64 boxes of one subtree level at once
@@ -127,25 +120,14 @@ __global__ void k_tape_take(const float* __restrict__ dens_in, const uint32_t* _
 // the emit kernel: the 4 cells around its edge in lexicographic order and,
 // in each, the table slots on its local edge in slot order.  Each triangle
 // is stored as its other two corners in corner order, the first with the
-// vertex's own place (0, 1, 2) in bits 30-31.
-//
-// The same launch builds the transposed smoothing step.  The forward step is
-// p'_u = [u not moved] p_u + [u moved] / (3 |ring(u)|) * (sum over the
-// triangles of ring(u) of their corners), so
-//   dL/dp_v = [v not moved] dL/dp'_v + sum over moved u of m_uv / (3 |ring(u)|) dL/dp'_u,
-// m_uv = the number of triangles holding both u and v (m_vv = |ring(v)|).
-// Row v lists its ring neighbours u != v, each once in order of first
-// appearance, with m_uv; the step reads [u moved] and the scaled gradient
-// dL/dp'_u / (3 |ring(u)|) from u's own record, so no row needs another
-// vertex's ring.  The other corners are staged in LDS, one column per lane.
+// vertex's own place (0, 1, 2) in bits 30-31.  Also the vertex's backward
+// scale: 1 / (3 |ring|) if the primal step moves it, else 1.
 __global__ void __launch_bounds__(kStepThreads) k_tape_adj(
     const uint32_t* __restrict__ vedge, const uint8_t* __restrict__ boundary, const uint32_t* __restrict__ cell_tb,
     const uint8_t* __restrict__ cell_mask, const uint32_t* __restrict__ idx, int dim, int V, int T, int bp,
     int stride, uint2* __restrict__ ring, uint8_t* __restrict__ deg, uint8_t* __restrict__ mv,
-    uint32_t* __restrict__ nbr, uint8_t* __restrict__ nnz, float* __restrict__ bscale, int* __restrict__ err) {
-    __shared__ uint32_t others[kMaxNbr][kStepThreads];
-    const int tid = threadIdx.x;
-    const int v = blockIdx.x * blockDim.x + tid;
+    float* __restrict__ bscale, int* __restrict__ err) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const uint32_t code = vedge[v];
     const int ex = (int)(code & 255u), ey = (int)((code >> 8) & 255u), ez = (int)((code >> 16) & 255u);
@@ -171,8 +153,6 @@ __global__ void __launch_bounds__(kStepThreads) k_tape_adj(
             if (place == 2u && d != (uint32_t)v) *err = 1;
             const uint32_t o1 = place == 0u ? b : a, o2 = place == 2u ? b : d;
             ring[(int64_t)n * stride + v] = make_uint2(o1 | (place << 30), o2);
-            others[2 * n][tid] = o1;
-            others[2 * n + 1][tid] = o2;
             n++;
         }
     }
@@ -180,18 +160,6 @@ __global__ void __launch_bounds__(kStepThreads) k_tape_adj(
     deg[v] = (uint8_t)n;
     mv[v] = (uint8_t)moved;
     bscale[v] = moved ? 1.0f / (3.0f * (float)n) : 1.0f;
-    int out = 0;
-    for (int i = 0; i < 2 * n; i++) {
-        const uint32_t u = others[i][tid];
-        bool seen = false;
-        for (int j = 0; j < i && !seen; j++) seen = others[j][tid] == u;
-        if (seen) continue;
-        uint32_t m = 1;
-        for (int j = i + 1; j < 2 * n; j++) m += others[j][tid] == u;
-        nbr[(int64_t)out * stride + v] = u | (m << kVidBits);
-        out++;
-    }
-    nnz[v] = (uint8_t)out;
 }
 
 // x / 3 rounded to nearest in three FP32 operations: q0 = x RN(1/3), the
@@ -217,7 +185,7 @@ __device__ __forceinline__ float4 smooth_vertex(const uint2* __restrict__ ring, 
     for (int k = 0; k < n; k++) {
         const uint2 e = ring[(int64_t)k * stride + v];
         const uint32_t place = e.x >> 30;
-        const float4 a = in[e.x & 0x3FFFFFFFu], b = in[e.y];
+        const float4 a = in[e.x & kVidMask], b = in[e.y];
         const float4 c0 = place == 0u ? pv : a;
         const float4 c1 = place == 0u ? a : (place == 1u ? pv : b);
         const float4 c2 = place == 2u ? pv : b;
@@ -290,13 +258,18 @@ __global__ void k_bwd_init(const float* __restrict__ g3, int V, const uint8_t* _
     g[v] = make_float4(g3[3 * v] * sc, g3[3 * v + 1] * sc, g3[3 * v + 2] * sc, scaled ? (float)mv[v] : 0.0f);
 }
 
-// One transposed smoothing iteration (k_tape_adj states the sum): from the
-// records of the step after, dL/dp_v = m_self x_v + sum over row v of
-// m_uv [u moved] x_u, m_self = |ring(v)| for a moved v (its own record is
-// scaled) and 1 otherwise.  Writes v's record for the step before, or
-// dL/dp_v itself after the first iteration (`last`).
-__global__ void __launch_bounds__(kStepThreads) k_bwd_step(const uint32_t* __restrict__ nbr, int stride,
-                                                           const uint8_t* __restrict__ nnz,
+// One transposed smoothing iteration.  The forward step is
+// p'_u = [u not moved] p_u + [u moved] / (3 |ring(u)|) * (sum over the
+// triangles of ring(u) of their corners), so
+//   dL/dp_v = [v not moved] dL/dp'_v
+//             + sum over the triangles t of ring(v), over the corners u of t,
+//               of [u moved] dL/dp'_u / (3 |ring(u)|);
+// with the records x_u = [u moved] dL/dp'_u / (3 |ring(u)|) (w = [u moved])
+// that is m_self x_v plus, per ring triangle, its two other corners' records
+// times their w (m_self = |ring(v)| for a moved v, whose own record is
+// scaled; 1 otherwise).  Writes v's record for the step before, or dL/dp_v
+// itself after the first iteration (`last`).
+__global__ void __launch_bounds__(kStepThreads) k_bwd_step(const uint2* __restrict__ ring, int stride,
                                                            const uint8_t* __restrict__ deg,
                                                            const uint8_t* __restrict__ mv,
                                                            const float* __restrict__ bscale,
@@ -304,19 +277,21 @@ __global__ void __launch_bounds__(kStepThreads) k_bwd_step(const uint32_t* __res
                                                            float4* __restrict__ out) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    const int n = nnz[v];
+    const int n = deg[v];
     const bool moved = mv[v] != 0;
     const float4 own = in[v];
-    const float ms = moved ? (float)deg[v] : 1.0f;
+    const float ms = moved ? (float)n : 1.0f;
     float sx = ms * own.x, sy = ms * own.y, sz = ms * own.z;
 #pragma unroll 4
     for (int k = 0; k < n; k++) {
-        const uint32_t e = nbr[(int64_t)k * stride + v];
-        const float4 q = in[e & kVidMask];
-        const float w = (float)(e >> kVidBits) * q.w;
-        sx = __builtin_fmaf(w, q.x, sx);
-        sy = __builtin_fmaf(w, q.y, sy);
-        sz = __builtin_fmaf(w, q.z, sz);
+        const uint2 e = ring[(int64_t)k * stride + v];
+        const float4 a = in[e.x & kVidMask], b = in[e.y];
+        sx = __builtin_fmaf(a.w, a.x, sx);
+        sy = __builtin_fmaf(a.w, a.y, sy);
+        sz = __builtin_fmaf(a.w, a.z, sz);
+        sx = __builtin_fmaf(b.w, b.x, sx);
+        sy = __builtin_fmaf(b.w, b.y, sy);
+        sz = __builtin_fmaf(b.w, b.z, sz);
     }
     const float sc = last ? 1.0f : bscale[v];
     out[v] = make_float4(sx * sc, sy * sc, sz * sc, moved ? 1.0f : 0.0f);
@@ -712,8 +687,6 @@ struct bmf_tape {
     uint8_t* boundary = nullptr;
     uint32_t* idx = nullptr;
     uint2* ring = nullptr;
-    uint32_t* nbr = nullptr;
-    uint8_t* nnz = nullptr;
     float* bscale = nullptr;
     uint8_t* deg = nullptr;
     uint8_t* mv = nullptr;
@@ -962,8 +935,7 @@ int bmf_tape_create(bmf_tape** out, bmf_ctx* ctx, int32_t max_dim, int64_t max_v
                     dalloc(&t->vedge, max_vertices) == hipSuccess && dalloc(&t->boundary, max_vertices) == hipSuccess &&
                     dalloc(&t->idx, max_indices) == hipSuccess &&
                     dalloc(&t->ring, (int64_t)kMaxRing * max_vertices) == hipSuccess &&
-                    dalloc(&t->nbr, (int64_t)kMaxNbr * max_vertices) == hipSuccess &&
-                    dalloc(&t->nnz, max_vertices) == hipSuccess && dalloc(&t->bscale, max_vertices) == hipSuccess &&
+                    dalloc(&t->bscale, max_vertices) == hipSuccess &&
                     dalloc(&t->deg, max_vertices) == hipSuccess && dalloc(&t->mv, max_vertices) == hipSuccess &&
                     dalloc(&t->p4, max_vertices) == hipSuccess && dalloc(&t->q4, max_vertices) == hipSuccess &&
                     dalloc(&t->pos, 3 * max_vertices) == hipSuccess &&
@@ -994,8 +966,6 @@ void bmf_tape_destroy(bmf_tape* t) {
     dfree(t->boundary);
     dfree(t->idx);
     dfree(t->ring);
-    dfree(t->nbr);
-    dfree(t->nnz);
     dfree(t->bscale);
     dfree(t->deg);
     dfree(t->mv);
@@ -1108,7 +1078,7 @@ int bmf_forward_diff(bmf_ctx* c, const bmf_chunk_desc* d, const float* densities
         if (smooth) {
             hipLaunchKernelGGL(k_tape_adj, dim3(step_blocks(V)), dim3(kStepThreads), 0, s, t->vedge, t->boundary,
                                A.cell_tb, A.cell_mask, t->idx, dim, V, T, t->bp, (int)t->cap_v, t->ring, t->deg, t->mv,
-                               t->nbr, t->nnz, t->bscale, t->err);
+                               t->bscale, t->err);
             BMF_HIP(hipGetLastError());
             rc = smooth_steps(t, t->p4, t->q4, t->K, s, t->pos, 0);
             if (rc) return rc;
@@ -1275,8 +1245,8 @@ int bmf_backward_diff(bmf_tape* t, const float* dldp, float* dldd) {
     float4* nxt = t->g2;
     if (smooth)
         for (int m = t->K - 1; m >= 0; m--) {
-            hipLaunchKernelGGL(k_bwd_step, dim3(step_blocks(t->V)), dim3(kStepThreads), 0, s, t->nbr, (int)t->cap_v,
-                               t->nnz, t->deg, t->mv, t->bscale, cur, t->V, (int)(m == 0), nxt);
+            hipLaunchKernelGGL(k_bwd_step, dim3(step_blocks(t->V)), dim3(kStepThreads), 0, s, t->ring, (int)t->cap_v,
+                               t->deg, t->mv, t->bscale, cur, t->V, (int)(m == 0), nxt);
             std::swap(cur, nxt);
         }
     hipLaunchKernelGGL(k_bwd_place, dim3(blocks_for(n3)), dim3(kThreads), 0, s, t->dens, t->vid0, cur, t->dim, dldd);

@@ -937,8 +937,7 @@ constexpr int kScanThreads = 1024;
 
 __global__ void __launch_bounds__(kScanThreads) k_scan(Arenas A, int n) {
     constexpr int NW = kScanThreads / 64;
-    constexpr int kGroup = 8;  // tiles loaded together (their loads overlap)
-    __shared__ uint32_t s_wv[2][NW], s_wt[2][NW];  // wave totals, double-buffered by tile parity
+    __shared__ uint32_t s_wv[NW], s_wt[NW];  // wave totals
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int j = blockIdx.x; j < n; j += gridDim.x) {
         ChunkMeta& M = A.meta[j];
@@ -947,22 +946,25 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(Arenas A, int n) {
         const int64_t R = (int64_t)dim * dim;
         uint32_t* rv = A.row_v + M.row_off;
         uint32_t* rt = A.row_t + M.row_off;
-        // exclusive scan in tiles of kScanThreads consecutive rows (coalesced):
-        // wave scans, then the wave totals and the carry of the earlier tiles
-        const int ntiles = (int)((R + kScanThreads - 1) / kScanThreads);
-        uint32_t cv = 0, ct = 0;
-        int par = 0;
-        for (int g = 0; g < ntiles; g += kGroup) {
-            uint32_t av[kGroup], at[kGroup];
+        // exclusive scan of the rows: each wave scans a contiguous run of
+        // 64-row tiles (coalesced, wave shuffles, a running carry; four tiles'
+        // loads issued together), then one exchange of the wave totals and a
+        // second pass that adds each wave's base
+        const int64_t ntile = (R + 63) / 64;
+        const int64_t tpw = (ntile + NW - 1) / NW;
+        const int64_t t0 = (int64_t)wave * tpw, t1 = t0 + tpw < ntile ? t0 + tpw : ntile;
+        uint32_t wv = 0, wt = 0;  // running totals of this wave's run
+        for (int64_t tg = t0; tg < t1; tg += 4) {
+            uint32_t av[4], at[4];
 #pragma unroll
-            for (int k = 0; k < kGroup; k++) {
-                const int64_t r = (int64_t)(g + k) * kScanThreads + tid;
-                av[k] = r < R ? rv[r] : 0u;
-                at[k] = r < R ? rt[r] : 0u;
+            for (int k = 0; k < 4; k++) {
+                const int64_t r = (tg + k) * 64 + lane;
+                const bool in = tg + k < t1 && r < R;
+                av[k] = in ? rv[r] : 0u;
+                at[k] = in ? rt[r] : 0u;
             }
 #pragma unroll
-            for (int k = 0; k < kGroup; k++) {
-                if (g + k >= ntiles) break;  // uniform
+            for (int k = 0; k < 4; k++) {
                 uint32_t sv = av[k], sw = at[k];
 #pragma unroll
                 for (int off = 1; off < 64; off <<= 1) {
@@ -972,32 +974,39 @@ __global__ void __launch_bounds__(kScanThreads) k_scan(Arenas A, int n) {
                         sw += ut;
                     }
                 }
-                if (lane == 63) {
-                    s_wv[par][wave] = sv;
-                    s_wt[par][wave] = sw;
+                const int64_t r = (tg + k) * 64 + lane;
+                if (tg + k < t1 && r < R) {
+                    rv[r] = wv + sv - av[k];
+                    rt[r] = wt + sw - at[k];
                 }
-                __syncthreads();
-                uint32_t bv = cv, bt = ct, tv = 0, tt = 0;
-#pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    const uint32_t xv = s_wv[par][w], xt = s_wt[par][w];
-                    if (w < wave) {
-                        bv += xv;
-                        bt += xt;
-                    }
-                    tv += xv;
-                    tt += xt;
-                }
-                const int64_t r = (int64_t)(g + k) * kScanThreads + tid;
-                if (r < R) {
-                    rv[r] = bv + sv - av[k];
-                    rt[r] = bt + sw - at[k];
-                }
-                cv += tv;
-                ct += tt;
-                par ^= 1;
+                wv += (uint32_t)__shfl(sv, 63);
+                wt += (uint32_t)__shfl(sw, 63);
             }
         }
+        if (lane == 0) {
+            s_wv[wave] = wv;
+            s_wt[wave] = wt;
+        }
+        __syncthreads();
+        uint32_t bv = 0, bt = 0, cv = 0, ct = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const uint32_t xv = s_wv[w], xt = s_wt[w];
+            if (w < wave) {
+                bv += xv;
+                bt += xt;
+            }
+            cv += xv;
+            ct += xt;
+        }
+        if (bv | bt)
+            for (int64_t t = t0; t < t1; t++) {
+                const int64_t r = t * 64 + lane;
+                if (r < R) {
+                    rv[r] += bv;
+                    rt[r] += bt;
+                }
+            }
         if (tid == 0) {
             const uint32_t V = cv, T = ct;
             M.V = (int32_t)V;

@@ -473,11 +473,11 @@ typedef struct {
     const float* densities;   /* device [dim^3]: the densities the forward used */
 } bmf_tape_info;
 
-/* max_dim bounds the chunk's samples per axis; max_vertices (at most 2^27)
+/* max_dim bounds the chunk's samples per axis; max_vertices (at most 2^30)
  * / max_indices bound its mesh; checkpoint_every >= 1.  The tape holds
- * about 440 bytes per vertex of max_vertices (ring and transposed-step
- * records, positions, gradients), 4 per index, 8 per sample of max_dim^3,
- * and 16 per vertex per kept smoothing state. */
+ * about 280 bytes per vertex of max_vertices (ring records, positions,
+ * gradients), 4 per index, 8 per sample of max_dim^3, and 16 per vertex
+ * per kept smoothing state. */
 int bmf_tape_create(bmf_tape** out, bmf_ctx* ctx, int32_t max_dim, int64_t max_vertices, int64_t max_indices,
                     int32_t checkpoint_every);
 void bmf_tape_destroy(bmf_tape* tape);
