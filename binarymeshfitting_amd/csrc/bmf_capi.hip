@@ -548,6 +548,7 @@ static int extract_impl(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mod
     c->results_valid = false;
     c->last_n = n;
     c->have_extract = true;
+    c->stats_batch = mode == BMF_MODE_BATCHED;  // PLAIN frames run one pipeline per chunk
     if (n == 0) return BMF_OK;
     int rc = BMF_OK;
     if (chunks) {
@@ -793,7 +794,7 @@ int bmf_last_timing(bmf_ctx* c, float* total_ms) {
 
 int bmf_get_frame_stats(bmf_ctx* c, bmf_frame_stats* out) {
     if (!c || !out) return BMF_EINVAL;
-    if (!c->results_valid) return BMF_ESTATE;
+    if (!c->results_valid || !c->stats_batch) return BMF_ESTATE;
     out->samples = c->h_ctr->total_samples;
     // per-workgroup counts of the last pipeline's two kernels (plain stores there)
     const int ns = std::min(c->stat_nsample, c->A.stat_cap), nr = std::min(c->stat_nrefine, c->A.stat_cap);
@@ -831,6 +832,7 @@ int bmf_profile_stages(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, float* s
     rc = enqueue_pipeline(c, c->A, n, maxk, wclasses, 1, 1, c->tune.sample_blocks, c->tune.row_blocks,
                           n, c->tune.smooth_blocks, c->ev_stage);
     if (rc) return rc;
+    c->stats_batch = true;
     BMF_HIP(hipEventRecord(c->ev1, c->stream));
     BMF_HIP(hipStreamSynchronize(c->stream));
     for (int i = 0; i < BMF_NUM_STAGES; i++)

@@ -47,6 +47,7 @@ struct bmf_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int persistent_blocks = 2048;  // default workgroups of the persistent kernels
     int stat_nsample = 0, stat_nrefine = 0;  // workgroups of the last pipeline's sampling / refinement
+    bool stats_batch = false;                // that pipeline ran the whole batch (BATCHED or profiled)
     bmf_tuning tune{};             // BATCHED launch shapes (bmf_set_tuning / bmf_autotune)
     bmf_mesh_options mesh{0, 0, 0.1f, 0, 0};  // bmf_set_mesh_options
     int64_t* d_fmt = nullptr;  // bmf_format: per-chunk offsets and tile counts [4][max_chunks + 1]

@@ -211,8 +211,10 @@ int bmf_last_timing(bmf_ctx* ctx, float* total_ms);
  * evaluated, summed over samples (every octave of every sample, as the
  * reference's block sampler computes, would be samples * octaves), and those
  * the refinement of early-labelled edge endpoints evaluated (0 for the other
- * samplers).  Valid after bmf_result.  Counterpart of the reference's
- * per-stage counters in its timing printout (DMCChunk.cpp:785-840). */
+ * samplers).  Valid after bmf_result of a BATCHED extraction (or after
+ * bmf_profile_stages); BMF_ESTATE after a PLAIN one, whose chunks run one
+ * pipeline each.  Counterpart of the reference's per-stage counters in its
+ * timing printout (DMCChunk.cpp:785-840). */
 typedef struct {
     int64_t samples;
     int64_t octave_evals;

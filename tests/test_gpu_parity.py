@@ -434,3 +434,9 @@ def test_frame_stats_count_the_adaptive_work(ctx):
     capi.check(rc, "bmf_result")
     st = ctx.frame_stats()
     assert st["samples"] == 41 ** 3 and st["octave_evals"] == 0 and st["refine_evals"] == 0
+    ctx.extract(workloads.to_capi([workloads.Chunk((-20.0, -19.0, -21.5), 40.0, 41)]), capi.MODE_PLAIN)
+    rc, _, _, _ = ctx.results(1)
+    capi.check(rc, "bmf_result")
+    with pytest.raises(capi.BmfError) as e:
+        ctx.frame_stats()  # PLAIN frames keep no batch counts
+    assert e.value.code == capi.ESTATE
