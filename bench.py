@@ -57,6 +57,8 @@ def parse():
                     help="timed batched steps of the uniform batch in DMC quad mode (512 chunks); 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=5,
                     help="timed steps of the forward+loss+backward leg (one 128^3-cell chunk, K=8); 0 skips it")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the library's default launch shapes for the headline batch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 launches) and exit; its FETCH_SIZE and "
                          "WRITE_SIZE readings scale the counters of the extraction kernels to HBM bytes")
@@ -443,6 +445,10 @@ def main():
     arr = capi.chunk_array(cd)
     ctx = capi.Context(max_chunks=n, max_samples=n * dim ** 3, max_vertices=1 << 22, max_indices=1 << 24)
     ctx.set_sampler(**wl.sampler)
+    # launch shapes: the autotuner's pick on this batch (tile size, workgroups per kernel, row split)
+    t0 = time.perf_counter()
+    tuned = ctx.autotune(cd, reps=3) if args.autotune else ctx.tuning()
+    log(f"[bench] tuning {tuned} ({'autotuned' if args.autotune else 'defaults'} in {time.perf_counter() - t0:.2f} s)")
 
     # ---- batched path: warmup, then K timed steps ----------------------------
     log(f"[bench] warmup {args.warmup} batched steps of {n} chunks")
@@ -740,7 +746,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "uniform-512x64: 8x8x8 chunks of 64^3 cells, fbm terrain seed 1337, K=4, batched",
                    "chunks": n, "cells_per_chunk": (dim - 1) ** 3, "smoothing_iters": 4, "octaves": octaves,
-                   "sampled_chunks": sampled, "vertices": V, "triangles": T},
+                   "sampled_chunks": sampled, "vertices": V, "triangles": T, "tuning": tuned},
         "cells_per_s": round(value * (dim - 1) ** 3, 1),
         "surface_chunks_per_s": round(sampled * args.steps / wall, 2),
         "plain_chunks_per_s": round(plain, 2) if plain else None,

@@ -34,14 +34,18 @@ def _world(W, focus, maxl):
 
 
 @pytest.mark.parametrize("world", WORLDS, ids=["uniform", "two-lod", "three-lod"])
-@pytest.mark.parametrize("sampler", ["sphere", "terrain"])
+@pytest.mark.parametrize("sampler", ["sphere", "terrain", "noise2d", "noise3d"])
 def test_stitch_matches_oracle(ctx, oracle, world, sampler):
     W, focus, maxl, dim = world
     nodes, leaves = _world(W, focus, maxl)
     if sampler == "sphere":
         sd = dict(kind=capi.SAMPLER_SPHERE, world_size=2 * W)
-    else:
+    elif sampler == "terrain":
         sd = {**wl.terrain_sampler(), "y0": 0.0, "amp": W / 4, "freq": 0.05}
+    elif sampler == "noise2d":  # the reference world's sampler, scaled to the small worlds
+        sd = {**capi.noise_preset(capi.PRESET_WORLD), "g_scale": 2.0, "height": W / 4}
+    else:
+        sd = {**capi.noise_preset(capi.PRESET_TERRAIN3D_PERT), "g_scale": 1.0, "height": W / 4}
     ctx.set_sampler(**sd)
     rc, tris, n = ctx.stitch(nodes, leaves, dim)
     capi.check(rc, "bmf_stitch")
