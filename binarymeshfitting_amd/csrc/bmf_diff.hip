@@ -52,7 +52,7 @@ constexpr int kMaxRing = 20;
 // Vertex ids share a word with a corner place (2 bits) in the ring records.
 constexpr uint32_t kVidMask = 0x3FFFFFFFu;
 constexpr int64_t kMaxTapeVertices = int64_t(1) << 30;
-constexpr int kLeaf = 8;       // targets per leaf box of the nearest-target hierarchy
+constexpr int kLeaf = 64;      // targets per leaf box of the nearest-target hierarchy
 constexpr int kNNThreads = 256;  // four vertices per workgroup, one wave each
 constexpr int kNNFan = 6;        // a wave tests 2^kNNFan = 64 boxes of one subtree level at once
 constexpr int kNNFrames = 6;     // frames of the wave's walk: root + ceil(28 / kNNFan) levels for n < 2^31
@@ -531,8 +531,9 @@ __device__ __forceinline__ int wave_argmin(float x, uint64_t m, int lane) {
 // takes D mod kNNFan levels so that the leaf frames are full).  Lane j
 // tests descendant j; the frame keeps the set whose box is no farther than
 // the best target distance U found so far, and pops its nearest box first.
-// A frame of leaves is evaluated kLeaf lanes per leaf, 64 / kLeaf leaves per
-// step, re-pruned against U between steps; the very first leaf frame starts
+// A frame of leaves is evaluated min(kLeaf, 64) lanes per leaf (each lane
+// taking kLeaf / 64 targets of a larger leaf), 64 / kLeaf leaves per step
+// when leaves are smaller than a wave, re-pruned against U between steps; the very first leaf frame starts
 // with its nearest leaf alone so that U is finite early.  Pruning drops a
 // box only when it is strictly farther than U, an actual target distance,
 // so every target at the minimum distance is evaluated and the result is
@@ -563,17 +564,22 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
     s_base[w][0] = 1;
     int depth = 1;
     const int Pi = (int)P, ni = (int)n;
+    // a leaf's targets: kLane lanes each take kSub of them (lane, lane + kLane, ...)
+    constexpr int kLane = kLeaf < 64 ? kLeaf : 64, kSub = kLeaf / kLane;
     auto eval = [&](int leaf) {
-        // this lane's target of `leaf` (heap index), if any
-        const int i = (leaf - Pi) * kLeaf + (lane & (kLeaf - 1));
-        if (i < ni) {
-            const float4 q = st[i];
-            const float d = dist2(px, py, pz, q);
-            const int id = __float_as_int(q.w);
-            if (d < bd || (d == bd && id < bi)) {
-                bd = d;
-                bi = id;
-                bs = i;
+        // this lane's targets of `leaf` (heap index), if any
+#pragma unroll
+        for (int u = 0; u < kSub; u++) {
+            const int i = (leaf - Pi) * kLeaf + u * kLane + (lane & (kLane - 1));
+            if (i < ni) {
+                const float4 q = st[i];
+                const float d = dist2(px, py, pz, q);
+                const int id = __float_as_int(q.w);
+                if (d < bd || (d == bd && id < bi)) {
+                    bd = d;
+                    bi = id;
+                    bs = i;
+                }
             }
         }
     };
@@ -587,20 +593,20 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
         if (lk == D) {
             if (U == INFINITY && m) {
                 const int j = wave_argmin(lb, m, lane);
-                if (lane < kLeaf) eval(b0 + j);
+                if (lane < kLane) eval(b0 + j);
                 m &= ~(1ull << j);
                 U = wave_min(bd);
                 m &= __ballot(lb <= U);
             }
             while (m) {
-                // lane group g (kLeaf lanes) takes the g-th lowest live leaf:
+                // lane group g (kLane lanes) takes the g-th lowest live leaf:
                 // the positions come off the wave-uniform mask on the scalar
                 // unit, each group selects its own
                 int j = -1;
 #pragma unroll
-                for (int t = 0; t < 64 / kLeaf; t++) {
+                for (int t = 0; t < 64 / kLane; t++) {
                     const int pos = m ? __ffsll((unsigned long long)m) - 1 : -1;
-                    j = lane / kLeaf == t ? pos : j;
+                    j = lane / kLane == t ? pos : j;
                     m &= m - 1;
                 }
                 if (j >= 0) eval(b0 + j);
@@ -851,8 +857,9 @@ int smooth_steps(bmf_tape* t, float4* a, float4* b, int k, hipStream_t s, float*
 // The hierarchy of up to BMF_NOALLOC_MAX_TARGETS targets lives in one
 // file-scope device array, so that entry point allocates nothing: Morton
 // keys and target ids (2 x 2 x 1 MiB), the sorted targets (4 MiB), the boxes
-// of 2 x 32768 heap nodes (2 x 1 MiB) and the radix sort's temporary
-// storage (2 MiB), 12 MiB per GPU for the life of the process.  Host code
+// of 2 x 4096 heap nodes of 64-target leaves (2 x 128 KiB) and the radix
+// sort's temporary storage (2 MiB), 10.25 MiB per GPU for the life of the
+// process.  Host code
 // never passes the array's name to a kernel (on the host it is a stand-in,
 // not a device address): each device's address comes from
 // hipGetSymbolAddress, taken once per device with that device current.
@@ -864,6 +871,7 @@ constexpr size_t kNATgtBytes = sizeof(float4) * kNAMax;
 constexpr size_t kNABoxBytes = sizeof(float4) * 2 * kNALeaves;
 constexpr size_t kNACubBytes = 2u << 20;
 constexpr size_t kScratchBytes = 4 * kNAKeyBytes + kNATgtBytes + 2 * kNABoxBytes + kNACubBytes;
+static_assert(kScratchBytes == (41u << 18), "10.25 MiB of scratch with 64-target leaves");
 static_assert(kNAKeyBytes % 256 == 0 && kNATgtBytes % 256 == 0 && kNABoxBytes % 256 == 0, "256-byte aligned parts");
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
