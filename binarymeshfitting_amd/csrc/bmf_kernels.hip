@@ -1259,6 +1259,20 @@ __host__ __device__ constexpr uint64_t edge_corner_table() {
 }
 constexpr uint64_t kEdgeCorners = edge_corner_table();
 
+// The same per edge, as emit's pass 2 uses it: the edge's id-table entry
+// relative to the cell's own (plane dx, row + dy, column + dz of the
+// [2][kEmitRows + 1][64 W] table) in bits 16-31, the entry's x- and y-edge
+// bits that precede the edge's vertex among its sample's (bits 14 and 15:
+// none for an x-edge, the x-edge for a y-edge, both for a z-edge), the
+// plane in bit 0.
+template <int W>
+__host__ __device__ constexpr uint32_t edge_decode(int e) {
+    const uint32_t g = (uint32_t)(kEdgeCorners >> (5 * e)) & 31u;
+    const uint32_t dx = g & 1u, dy = (g >> 1) & 1u, dz = (g >> 2) & 1u, ax = g >> 3;
+    const uint32_t off = dx * (uint32_t)((kEmitRows + 1) * 64 * W) + dy * (uint32_t)(64 * W) + dz;
+    return (off << 16) | (ax >= 1 ? 0x4000u : 0u) | (ax == 2 ? 0x8000u : 0u) | dx;
+}
+
 // Pass 1 for label row (x + p, y): vertex ids into s_id, and for p == 0 with
 // `own` the vertices themselves (DMCChunk::calculate_isovertex, DMCChunk.cpp:
 // 657-674; vertices of a sample in x, y, z edge order).  The row masks are
@@ -1339,7 +1353,7 @@ __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M
 template <int W>
 __device__ __forceinline__ void emit_tris_row(const Arenas& A, const ChunkMeta& M,
                                               const uint64_t (*s_lab)[plane_words(W)], const uint8_t* s_ntri,
-                                              const uint64_t* s_tri, uint32_t tcarry,
+                                              const uint64_t* s_tri, const uint32_t* s_edge, uint32_t tcarry,
                                               const uint16_t (*s_id)[kEmitRows + 1][64 * W], uint32_t slab0,
                                               uint32_t slab1, int x, int y, int ys) {
     const int dim = M.dim;
@@ -1377,16 +1391,22 @@ __device__ __forceinline__ void emit_tris_row(const Arenas& A, const ChunkMeta& 
                 // disagree) is reported, its slot is inside the chunk's range
                 uint32_t* __restrict__ I = A.idx + io + 3ll * tb;
                 bool bad = false;
-                for (int q = 0; q < (int)nt; q++) {
+                // the cell's own entry of the id table; a corner's entry is
+                // its edge's offset from it (edge_decode)
+                const uint16_t* __restrict__ cid = &s_id[0][0][0] + (y - ys) * (64 * W) + z;
+                const uint32_t sdiff = slab1 - slab0;
+                // at most 5 triangles per cell: unrolled, so each slot's table
+                // bits are a constant field of the row
+#pragma unroll
+                for (int q = 0; q < 5; q++) {
+                    if (q >= (int)nt) break;
                     uint32_t v[3];
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
                         const int sl = 3 * q + c;
-                        const uint32_t g = (uint32_t)(kEdgeCorners >> (5 * ((row >> (4 * sl)) & 15ull))) & 31u;
-                        const uint32_t e = s_id[g & 1u][y + (int)((g >> 1) & 1u) - ys][z + (int)((g >> 2) & 1u)];
-                        const uint32_t sb = (g & 1u) ? slab1 : slab0;
-                        const uint32_t ax = g >> 3;
-                        v[c] = sb + (e & 0x3FFFu) + (ax >= 1 ? (e >> 14) & 1u : 0u) + (ax == 2 ? e >> 15 : 0u);
+                        const uint32_t d = s_edge[(uint32_t)(row >> (4 * sl)) & 15u];
+                        const uint32_t e = cid[d >> 16];
+                        v[c] = slab0 + (d & 1u) * sdiff + (e & 0x3FFFu) + (uint32_t)__popc(e & d & 0xC000u);
                         bad = bad || v[c] >= Vc;
                     }
                     I[3 * q] = v[0];
@@ -1416,6 +1436,8 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
     // 3 * 256 < 2^14 vertices), | x-edge bit 14 | y-edge bit 15
     __shared__ uint16_t s_id[2][kEmitRows + 1][64 * W];
     __shared__ uint64_t s_tri[256];                       // the triangle table
+    __shared__ uint32_t s_edge[16];                       // edge_decode of edges 0..11
+    if (threadIdx.x < 16) s_edge[threadIdx.x] = threadIdx.x < 12 ? edge_decode<W>(threadIdx.x) : 0u;
     for (int t = threadIdx.x; t < 256; t += blockDim.x) {
         s_tri[t] = c_tri_table[t];
         s_ntri[t] = (uint8_t)((c_tri_table[t] >> 60) / 3);
@@ -1467,7 +1489,7 @@ __global__ void __launch_bounds__(kRowThreads) k_emit(Arenas A, int n) {
             // pass 2: cell rows with triangles
             for (int y = ys + wave; y < ye && y < dim - 1; y += NW) {
                 if (s_base[2][y + 1] == s_base[2][y]) continue;
-                emit_tris_row<W>(A, M, s_planes, s_ntri, s_tri, s_base[2][y], s_id, slab0, slab1, x, y, ys);
+                emit_tris_row<W>(A, M, s_planes, s_ntri, s_tri, s_edge, s_base[2][y], s_id, slab0, slab1, x, y, ys);
             }
         }
     }
