@@ -10,12 +10,12 @@
 // exact):
 //   (k_scan)       row bases, arena offsets
 //   k_dmc_vertices per row: vertex positions; per surface cell its first
-//                  vertex id and mask (cell_tb / cell_mask arenas)
-//   k_dmc_quads    per row: quad (or two-triangle) indices; rings of the
-//                  chunks that smooth (slot per corner by atomic counter,
-//                  then sorted, so the order is the reference's ascending
-//                  primitive id)
-//   k_ring_sort, then K x (k_dual_n, k_primal_ring)
+//                  vertex id and mask (cell_tb / cell_mask arenas); for the
+//                  chunks that smooth, each vertex's ring (the primitives
+//                  around it in ascending id, the reference's order), from
+//                  the quad ids of its polygon's edges
+//   k_dmc_quads    per row: quad (or two-triangle) indices
+//   K x (k_dual_n, k_primal_ring)
 #include <hip/hip_runtime.h>
 
 #include "bmf_device_tables.h"
@@ -283,6 +283,89 @@ __constant__ int8_t c_le[12][4] = {{0, 0, 0, 0}, {0, 0, 1, 0}, {0, 1, 0, 0}, {0,
 }  // namespace
 
 // ------------------------------------------------------------------------
+// Rings.  Quad ids follow the (x, y, z, axis) order of their lattice edges:
+// a row's base (row_t after the scan, per quad) plus the row's interior
+// sign-changing edges before the edge's sample (all axes) and before its axis
+// at that sample.  A vertex (a polygon of a cell) lies on the quads of its
+// polygon's interior edges, so visiting the cell's local edges in the order of
+// their (dx, dy, dz, axis) offsets lists its quads in ascending id.  With
+// quads written as triangles (a, b, c), (c, d, a) (MeshProcessor.cpp:74-91) a
+// quad's first and last ring cells (corners a and c) are on both triangles,
+// the others on one, chosen by the quad's winding.
+// ------------------------------------------------------------------------
+// For label row d = 2 dx + dy of cell (x, y, z), i.e. row (x + dx, y + dy):
+// the row's first quad id plus its interior sign-changing edges before
+// sample z (all axes), and flag byte d: the interior edge bits of axes 0-2 at
+// z (bits 0-2) and at z + 1 (bits 3-5), the label at z and z + 1 (6, 7).
+// The rows come from the staged planes x, x+1, x+2.
+template <int W>
+__device__ __forceinline__ void quad_row_at(const uint64_t (*s_planes)[plane_words(W)], const uint32_t* rtc,
+                                            uint32_t div, int dim, int x, int y, int z, int d, uint32_t& pre,
+                                            uint32_t& fl) {
+    const int dx = d >> 1, dy = d & 1, xx = x + dx, yy = y + dy;
+    const RowBits<W> ra = lds_row<W>(s_planes[dx], dim, yy, true);
+    const RowBits<W> rb = lds_row<W>(s_planes[dx], dim, yy + 1, yy + 1 < dim);
+    const RowBits<W> rc = lds_row<W>(s_planes[dx + 1], dim, yy, xx + 1 < dim);
+    RowBits<W> e0, e1, e2;
+    interior_edges<W>(ra, rb, rc, dim, xx, yy, e0, e1, e2);
+    pre = rtc[(int64_t)xx * dim + yy] / div + prefix_pop<W>(e0, z) + prefix_pop<W>(e1, z) + prefix_pop<W>(e2, z);
+    const uint32_t f = bit_at<W>(e0, z) | (bit_at<W>(e1, z) << 1) | (bit_at<W>(e2, z) << 2) |
+                       (bit_at<W>(e0, z + 1) << 3) | (bit_at<W>(e1, z + 1) << 4) | (bit_at<W>(e2, z + 1) << 5) |
+                       (bit_at<W>(ra, z) << 6) | (bit_at<W>(ra, z + 1) << 7);
+    fl |= f << (8 * d);
+}
+
+// Writes vertex g's ring (polygon p of a cell with polygon-of-edge nibbles
+// pe) from the four rows' quad prefixes and flags; returns its size.
+__device__ __forceinline__ uint32_t dmc_ring(const Arenas& A, const uint32_t (&pre)[4], uint32_t fl, uint64_t pe,
+                                             uint32_t p, int64_t g, bool tris) {
+    uint32_t n = 0;
+    bool over = false;
+    const int64_t cap = A.cap_vertices;
+    auto put = [&](uint32_t pid) {
+        if (n < (uint32_t)kRingSlots) A.ring[(int64_t)n * cap + g] = pid;
+        else over = true;
+        n++;
+    };
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        // local edges in (dx, dy, dz, axis) order: 0, 4, 8, 1, 5, 2, 9, 3, 6, 10, 7, 11
+        constexpr int kOrder[12] = {0, 4, 8, 1, 5, 2, 9, 3, 6, 10, 7, 11};
+        const int le = kOrder[i];
+        if ((uint32_t)((pe >> (4 * le)) & 15ull) != p) continue;
+        // c_le[le]: the origin offset and axis (a = le >> 2, the other two
+        // offsets bits 1 and 0 of le)
+        const int a = le >> 2, b1 = (le >> 1) & 1, b0 = le & 1;
+        const int dx = a == 0 ? 0 : b1, dy = a == 0 ? b1 : (a == 2 ? b0 : 0), dz = a == 2 ? 0 : b0;
+        const int d = 2 * dx + dy;
+        const uint32_t f = fl >> (8 * d);
+        const uint32_t cb = dz ? f >> 3 : f;  // interior edge bits of the edge's sample
+        if (!((cb >> a) & 1u)) continue;      // on the chunk's boundary: no quad
+        const uint32_t qid = pre[d] + (dz ? (f & 1u) + ((f >> 1) & 1u) + ((f >> 2) & 1u) : 0u) +
+                             (a >= 1 ? cb & 1u : 0u) + (a == 2 ? (cb >> 1) & 1u : 0u);
+        if (!tris) {
+            put(qid);
+            continue;
+        }
+        // ring cell r = 3 - (le & 3) of the edge (c_ring order); the quad is
+        // (v0, v1, v3, v2), with v1 and v2 swapped when the solid end of the
+        // edge is not on the side its axis winds toward
+        const int r = 3 - (le & 3);
+        const uint32_t prim = 2u * qid;
+        if (r == 0 || r == 3) {
+            put(prim);
+            put(prim + 1u);
+        } else {
+            const bool solid_origin = ((f >> (6 + dz)) & 1u) == 0u;
+            const bool flip = solid_origin != (a != 1);
+            put(prim + ((r == 1) == flip ? 1u : 0u));
+        }
+    }
+    if (over) A.ctr->error = 10;
+    return n;
+}
+
+// ------------------------------------------------------------------------
 // vertices: one per polygon of each surface cell.  Row pass (wave per row,
 // lane = z): per-cell first vertex id and mask, per vertex its cell and
 // polygon code and boundary flag.  The plane's vertices are a contiguous id
@@ -372,7 +455,7 @@ __device__ void place_vertex(const Arenas& A, const ChunkMeta& M, const SamplerD
 template <int W>
 __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
     __shared__ DmcTables s_t;
-    __shared__ uint64_t s_planes[2][plane_words(W)];
+    __shared__ uint64_t s_planes[3][plane_words(W)];  // labels of planes x, x+1, x+2
     __shared__ float4 s_csg[2 * kCsgStage];  // the chunk's culled CSG primitives (QEF normals)
     load_dmc_tables(s_t);
     const SamplerDev* __restrict__ S = A.sampler;
@@ -393,10 +476,10 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
         const bool staged = qef && csg && M.csg_ns + M.csg_nb <= kCsgStage;  // uniform across the block
         __syncthreads();
         {
-            uint64_t* const dst[2] = {s_planes[0], s_planes[1]};
-            const uint64_t* const src[2] = {A.labels + M.lab_off, A.labels + M.lab_off};
-            const int xp[2] = {x, x + 1};
-            stage_plane_group<2>(dst, src, M.lab_words, dim, xp);
+            uint64_t* const dst[3] = {s_planes[0], s_planes[1], s_planes[2]};
+            const uint64_t* const src[3] = {A.labels + M.lab_off, A.labels + M.lab_off, A.labels + M.lab_off};
+            const int xp[3] = {x, x + 1, x + 2};
+            stage_plane_group<3>(dst, src, M.lab_words, dim, xp);
         }
         if (staged) stage_csg(*S, csg, M.csg_ns, M.csg_nb, s_csg);
         __syncthreads();
@@ -404,6 +487,8 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
         const int64_t vo = M.v_off;
         const uint32_t Vc = (uint32_t)M.V;
         const bool smooth = (M.iters > 0 || A.normals) && M.T > 0;  // the chunk runs the smoothing kernels
+        const bool tris = A.quad_tris != 0;
+        const uint32_t* rtc = A.row_t + M.row_off;
         // ---- row pass ----
         for (int y = wave; y < dim - 1; y += kRowThreads / 64) {
             if (rv[y + 1] == rv[y]) continue;  // the row's cells have no polygon
@@ -434,7 +519,6 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
                     A.boundary[vo + v] =
                         (uint8_t)(x == 0 || y == 0 || z == 0 || x + 1 == dim - 1 || y + 1 == dim - 1 || z + 1 == dim - 1);
                     A.vedge[vo + v] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16) | (p << 24);
-                    if (smooth) A.ring_n[vo + v] = 0u;
                 }
             }
         }
@@ -460,6 +544,23 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_vertices(Arenas A, int n) {
                 O[1] = 0.0f;
                 O[2] = 0.0f;
             }
+        }
+        if (!smooth) continue;
+        // ---- dense pass: the rings of the plane's vertices ----
+        for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+            const uint32_t code = A.vedge[vo + v];
+            const int y = (int)((code >> 8) & 255u), z = (int)((code >> 16) & 255u);
+            const uint32_t mask = A.cell_mask[M.dens_off + ((int64_t)x * dim + y) * dim + z];
+            uint32_t pre[4] = {0u, 0u, 0u, 0u}, fl = 0u;
+            // one row at a time (few live registers), its prefix kept by selects
+#pragma unroll 1
+            for (int d = 0; d < 4; d++) {
+                uint32_t pd;
+                quad_row_at<W>(s_planes, rtc, tris ? 2u : 1u, dim, x, y, z, d, pd, fl);
+#pragma unroll
+                for (int q = 0; q < 4; q++) pre[q] = q == d ? pd : pre[q];
+            }
+            A.ring_n[vo + v] = dmc_ring(A, pre, fl, s_t.pe[mask], code >> 24, vo + v, tris);
         }
     }
 }
@@ -491,7 +592,6 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
         }
         __syncthreads();
         const uint32_t* rt = A.row_t + M.row_off + (int64_t)x * dim;
-        const bool smooth = M.iters > 0 || A.normals;  // rings are needed
         const uint32_t Vc = (uint32_t)M.V, Tc = (uint32_t)M.T;
         uint32_t* __restrict__ I = A.idx + M.i_off;
         for (int y = wave; y < dim; y += kRowThreads / 64) {
@@ -514,6 +614,7 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
                 if (!(f[0] | f[1] | f[2])) continue;
                 uint32_t qid = qbase + prefix_pop<W>(ex, z) + prefix_pop<W>(ey, z) + prefix_pop<W>(ez, z);
                 const bool solid_origin = bit_at<W>(r00, z) == 0u;
+#pragma unroll
                 for (int a = 0; a < 3; a++) {
                     if (!f[a]) continue;
                     uint32_t v[4];
@@ -555,22 +656,6 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
                         t[2] = o[2];
                         t[3] = o[3];
                     }
-                    if (smooth) {
-                        const int64_t vo = M.v_off;
-                        for (int r = 0; r < 4; r++) {
-                            const uint32_t u = o[r];
-                            // corner r of the quad: triangles (0,1,2) and (2,3,0)
-                            const int reps = tris ? ((r == 0 || r == 2) ? 2 : 1) : 1;
-                            for (int h = 0; h < reps; h++) {
-                                const uint32_t pid = tris ? (r == 1 ? prim : (r == 3 ? prim + 1 : prim + (uint32_t)h)) : prim;
-                                const uint32_t slot = atomicAdd(&A.ring_n[vo + u], 1u);
-                                if (slot < (uint32_t)kRingSlots)
-                                    A.ring[(int64_t)slot * A.cap_vertices + vo + u] = pid;
-                                else
-                                    A.ctr->error = 10;
-                            }
-                        }
-                    }
                     qid++;
                 }
             }
@@ -582,32 +667,6 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
 // smoothing over the rings (chunks of the smoothing list, XCD-grouped tiles
 // as in the triangle path)
 // ------------------------------------------------------------------------
-__global__ void __launch_bounds__(kSmoothThreads) k_ring_sort(Arenas A) {
-    const int nsm = A.ctr->n_smooth;
-    const int xcd = blockIdx.x % kXcds;
-    const int64_t end = A.smooth_xcd_v[xcd + 1];
-    for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
-        const int k = find_chunk(A.smooth_prefix, nsm, tile);
-        const ChunkMeta& M = A.meta[A.smooth_list[k]];
-        const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
-        if (v >= M.V) continue;
-        const int64_t g = M.v_off + v;
-        const int n = (int)min(A.ring_n[g], (uint32_t)kRingSlots);
-        uint32_t r[kRingSlots];
-        for (int i = 0; i < n; i++) r[i] = A.ring[(int64_t)i * A.cap_vertices + g];
-        for (int i = 1; i < n; i++) {  // insertion sort: ascending primitive id
-            const uint32_t t = r[i];
-            int p = i - 1;
-            while (p >= 0 && r[p] > t) {
-                r[p + 1] = r[p];
-                p--;
-            }
-            r[p + 1] = t;
-        }
-        for (int i = 0; i < n; i++) A.ring[(int64_t)i * A.cap_vertices + g] = r[i];
-    }
-}
-
 // Dual step (MeshProcessor.cpp:130-236) for N-gons: ((0 + p0) + p1 ...) / N.
 __global__ void __launch_bounds__(kSmoothThreads) k_dual_n(Arenas A, int m) {
     const int nsm = A.ctr->n_smooth;
@@ -644,7 +703,7 @@ __global__ void __launch_bounds__(kSmoothThreads) k_dual_n(Arenas A, int m) {
     }
 }
 
-// Primal step (MeshProcessor.cpp:238-306) over the sorted ring.
+// Primal step (MeshProcessor.cpp:238-306) over the ring (ascending primitive id).
 __global__ void __launch_bounds__(kSmoothThreads) k_primal_ring(Arenas A, int m) {
     const int nsm = A.ctr->n_smooth;
     const int xcd = blockIdx.x % kXcds;
@@ -861,7 +920,6 @@ hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, con
 
 hipError_t launch_dmc_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
     blocks = blocks < kXcds ? kXcds : blocks - blocks % kXcds;  // whole XCD groups
-    if (m == 0) hipLaunchKernelGGL(k_ring_sort, dim3(blocks), dim3(kSmoothThreads), 0, s, A);
     if (A.normals) {
         hipLaunchKernelGGL(k_dual_n_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
         hipLaunchKernelGGL(k_primal_ring_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
