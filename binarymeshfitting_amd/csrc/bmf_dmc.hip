@@ -667,65 +667,85 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
 // smoothing over the rings (chunks of the smoothing list, XCD-grouped tiles
 // as in the triangle path)
 // ------------------------------------------------------------------------
-// Dual step (MeshProcessor.cpp:130-236) for N-gons: ((0 + p0) + p1 ...) / N.
+// Dual step (MeshProcessor.cpp:130-236) for N-gons (N = 3 or 4, one per
+// chunk): ((0 + p0) + p1 ...) / N, the corner ids and then the corners
+// loaded together.  Tiles come from the per-frame tile map.
 __global__ void __launch_bounds__(kSmoothThreads) k_dual_n(Arenas A, int m) {
-    const int nsm = A.ctr->n_smooth;
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_t[xcd + 1];
     for (int64_t tile = A.smooth_xcd_t[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
-        const int k = find_chunk(A.smooth_tprefix, nsm, tile);
-        const ChunkMeta& M = A.meta[A.smooth_list[k]];
-        if (M.iters <= m) continue;
-        const int64_t t = (tile - A.smooth_tprefix[k]) * kSmoothThreads + threadIdx.x;
-        if (t >= M.T) continue;
-        const int N = M.ipp;
-        const uint32_t* __restrict__ I = A.idx + M.i_off + (int64_t)N * t;
-        const float4* __restrict__ P = A.pos4 + M.v_off;
-        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        bool ok = true;
-        for (int c = 0; c < N; c++) {
-            const uint32_t u = I[c];
-            if (u >= (uint32_t)M.V) {
-                ok = false;
-                break;
-            }
-            const float4 q = P[u];
-            cx = cx + q.x;
-            cy = cy + q.y;
-            cz = cz + q.z;
-        }
-        if (!ok) {
+        const int2 tm = A.smooth_tmap[tile];  // (chunk, tile within the chunk)
+        const ChunkMeta& M = A.meta[uniform_int(tm.x)];
+        const int iters = uniform_int(M.iters), N = uniform_int(M.ipp), T = uniform_int(M.T);
+        const uint32_t V = (uint32_t)uniform_int(M.V);
+        const int64_t i_off = M.i_off, v_off = M.v_off;
+        if (iters <= m) continue;
+        const int64_t t = (int64_t)uniform_int(tm.y) * kSmoothThreads + threadIdx.x;
+        if (t >= T) continue;
+        const uint32_t* __restrict__ I = A.idx + i_off + (int64_t)N * t;
+        const float4* __restrict__ P = A.pos4 + v_off;
+        const uint32_t u0 = I[0], u1 = I[1], u2 = I[2], u3 = N == 4 ? I[3] : u0;
+        if (u0 >= V || u1 >= V || u2 >= V || u3 >= V) {
             A.ctr->error = 11;
             continue;
         }
+        const float4 a = P[u0], b = P[u1], c = P[u2], d = P[u3];
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        cx = cx + a.x;
+        cy = cy + a.y;
+        cz = cz + a.z;
+        cx = cx + b.x;
+        cy = cy + b.y;
+        cz = cz + b.z;
+        cx = cx + c.x;
+        cy = cy + c.y;
+        cz = cz + c.z;
+        if (N == 4) {
+            cx = cx + d.x;
+            cy = cy + d.y;
+            cz = cz + d.z;
+        }
         const float fn = (float)N;
-        A.cen4[M.i_off / N + t] = make_float4(cx / fn, cy / fn, cz / fn, 0.0f);
+        A.cen4[i_off / N + t] = make_float4(cx / fn, cy / fn, cz / fn, 0.0f);
     }
 }
 
-// Primal step (MeshProcessor.cpp:238-306) over the ring (ascending primitive id).
+// Primal step (MeshProcessor.cpp:238-306) over the ring (ascending primitive
+// id): the centroids summed from +0 in ring order, eight ring slots and then
+// their centroids loaded per round.
+constexpr int kRingRoundN = 8;
 __global__ void __launch_bounds__(kSmoothThreads) k_primal_ring(Arenas A, int m) {
-    const int nsm = A.ctr->n_smooth;
     const int xcd = blockIdx.x % kXcds;
     const int64_t end = A.smooth_xcd_v[xcd + 1];
+    const int64_t cap = A.cap_vertices;
     for (int64_t tile = A.smooth_xcd_v[xcd] + blockIdx.x / kXcds; tile < end; tile += gridDim.x / kXcds) {
-        const int k = find_chunk(A.smooth_prefix, nsm, tile);
-        const ChunkMeta& M = A.meta[A.smooth_list[k]];
-        const int K = M.iters;
+        const int2 tm = A.smooth_vmap[tile];  // (chunk, tile within the chunk)
+        const ChunkMeta& M = A.meta[uniform_int(tm.x)];
+        const int K = uniform_int(M.iters), N = uniform_int(M.ipp), V = uniform_int(M.V), bp = uniform_int(M.bp);
+        const int64_t i_off = M.i_off, v_off = M.v_off;
         if (K <= m) continue;
-        const int64_t v = (tile - A.smooth_prefix[k]) * kSmoothThreads + threadIdx.x;
-        if (v >= M.V) continue;
-        const int64_t g = M.v_off + v;
-        if (!M.bp && A.boundary[g]) continue;
+        const int64_t v = (int64_t)uniform_int(tm.y) * kSmoothThreads + threadIdx.x;
+        if (v >= V) continue;
+        const int64_t g = v_off + v;
+        const uint8_t bnd = A.boundary[g];
         const int n = (int)min(A.ring_n[g], (uint32_t)kRingSlots);
-        if (n == 0) continue;
-        const float4* __restrict__ C = A.cen4 + M.i_off / M.ipp;
+        if ((!bp && bnd) || n == 0) continue;
+        const float4* __restrict__ C = A.cen4 + i_off / N;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        for (int i = 0; i < n; i++) {
-            const float4 q = C[A.ring[(int64_t)i * A.cap_vertices + g]];
-            sx = sx + q.x;
-            sy = sy + q.y;
-            sz = sz + q.z;
+        for (int i0 = 0; i0 < n; i0 += kRingRoundN) {
+            uint32_t id[kRingRoundN];
+#pragma unroll
+            for (int k = 0; k < kRingRoundN; k++) id[k] = A.ring[(int64_t)(i0 + k < n ? i0 + k : i0) * cap + g];
+            float4 q[kRingRoundN];
+#pragma unroll
+            for (int k = 0; k < kRingRoundN; k++) q[k] = C[id[k]];
+#pragma unroll
+            for (int k = 0; k < kRingRoundN; k++) {
+                const bool ok = i0 + k < n;
+                sx = ok ? sx + q[k].x : sx;
+                sy = ok ? sy + q[k].y : sy;
+                sz = ok ? sz + q[k].z : sz;
+            }
         }
         const float fc = (float)n;
         const float px = sx / fc, py = sy / fc, pz = sz / fc;
@@ -920,6 +940,10 @@ hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, con
 
 hipError_t launch_dmc_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
     blocks = blocks < kXcds ? kXcds : blocks - blocks % kXcds;  // whole XCD groups
+    if (m == 0 && !A.normals) {
+        const hipError_t e = launch_smooth_map(A, blocks, s);
+        if (e != hipSuccess) return e;
+    }
     if (A.normals) {
         hipLaunchKernelGGL(k_dual_n_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);
         hipLaunchKernelGGL(k_primal_ring_nrm, dim3(blocks), dim3(kSmoothThreads), 0, s, A, m);

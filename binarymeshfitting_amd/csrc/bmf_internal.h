@@ -187,6 +187,8 @@ hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStre
 hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs);
 hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
+// (chunk, tile) of every smoothing vertex and primitive tile, once per frame before the first step
+hipError_t launch_smooth_map(const Arenas& A, int blocks, hipStream_t s);
 // Quad mode (bmf_dmc.hip): counts after launch_count/launch_refine, emission
 // after launch_scan, smoothing over the per-vertex rings.
 // cs[W-1]: the stream of row-word class W

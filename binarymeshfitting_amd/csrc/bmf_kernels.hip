@@ -1990,6 +1990,10 @@ hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kSampleThreads), 0, s, A, n);
     return hipGetLastError();
 }
+hipError_t launch_smooth_map(const Arenas& A, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_smooth_map, dim3(std::min(blocks, 1024)), dim3(kSmoothThreads), 0, s, A);
+    return hipGetLastError();
+}
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s) {
     blocks = blocks < kXcds ? kXcds : blocks - blocks % kXcds;  // whole XCD groups
     if (m == 0) hipLaunchKernelGGL(k_smooth_map, dim3(std::min(blocks, 1024)), dim3(kSmoothThreads), 0, s, A);
