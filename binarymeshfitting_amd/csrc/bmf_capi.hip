@@ -77,6 +77,12 @@ int join_classes(bmf_ctx* c, hipStream_t s, int used) {
 
 // The whole device pipeline for chunks [0, n) of the uploaded descriptors.
 // With `ev` (BMF_NUM_STAGES + 1 events) the stage boundaries are recorded.
+// Workgroups of the batched row scan: one per chunk up to two per CU; the
+// kernel strides over the batch and skips the chunks that were not sampled,
+// so a large batch with few sampled chunks does not dispatch a workgroup of
+// 1024 threads per chunk.
+int scan_blocks(const bmf_ctx* c, int n) { return std::min(n, 2 * c->cu_count); }
+
 int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses, int allow_cull, int reset,
                      int sample_blocks, int row_blocks, int scan_blocks, int smooth_blocks, hipEvent_t* ev = nullptr) {
     hipStream_t s = c->stream;
@@ -594,7 +600,7 @@ static int extract_impl(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int mod
         hipGraph_t graph = nullptr;
         BMF_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, c->tune.sample_blocks, c->tune.row_blocks,
-                              n, c->tune.smooth_blocks);
+                              scan_blocks(c, n), c->tune.smooth_blocks);
         hipGraph_t captured = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &captured);
         if (rc) {
@@ -830,7 +836,7 @@ int bmf_profile_stages(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, float* s
     for (int i = 0; i < n; i++) wclasses |= row_word_class(chunks[i].dim);
     BMF_HIP(hipEventRecord(c->ev0, c->stream));
     rc = enqueue_pipeline(c, c->A, n, maxk, wclasses, 1, 1, c->tune.sample_blocks, c->tune.row_blocks,
-                          n, c->tune.smooth_blocks, c->ev_stage);
+                          scan_blocks(c, n), c->tune.smooth_blocks, c->ev_stage);
     if (rc) return rc;
     c->stats_batch = true;
     BMF_HIP(hipEventRecord(c->ev1, c->stream));
@@ -982,7 +988,7 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
         for (int i = 0; i < BMF_NUM_STAGES; i++) out[i] = 1e30f;
         for (int r = 0; r < reps; r++) {
             int e = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, t.sample_blocks, t.row_blocks,
-                                     n, t.smooth_blocks, c->ev_stage);
+                                     scan_blocks(c, n), t.smooth_blocks, c->ev_stage);
             if (e) return e;
             BMF_HIP(hipStreamSynchronize(c->stream));
             for (int i = 0; i < BMF_NUM_STAGES; i++) {
@@ -1042,10 +1048,11 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
     if (maxk > 0) {
         float bt = 1e30f;
         int bb = cur.smooth_blocks;
-        const int mult[5] = {2, 4, 8, 16, 32};
-        for (int m : mult) {
+        // from 64 workgroups (a few small meshes) to 32 per CU
+        const int cand[7] = {8 * kXcds, cu, 2 * cu, 4 * cu, 8 * cu, 16 * cu, 32 * cu};
+        for (int b : cand) {
             bmf_tuning t = cur;
-            t.smooth_blocks = (m * cu + kXcds - 1) / kXcds * kXcds;
+            t.smooth_blocks = (b + kXcds - 1) / kXcds * kXcds;
             rc = run(t, st);
             if (rc) return rc;
             if (st[5] < bt) {
