@@ -57,6 +57,10 @@ def parse():
                     help="timed batched steps of the uniform batch in DMC quad mode (512 chunks); 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=5,
                     help="timed steps of the forward+loss+backward leg (one 128^3-cell chunk, K=8); 0 skips it")
+    ap.add_argument("--contexts", type=int, default=2,
+                    help="contexts the headline's timed batches alternate over (each with its own arenas and "
+                         "stream, so one batch's sampling overlaps the previous batch's row and smoothing "
+                         "kernels); 1 = one context, batches back to back on one stream")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the library's default launch shapes for the headline batch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -451,28 +455,65 @@ def main():
     log(f"[bench] tuning {tuned} ({'autotuned' if args.autotune else 'defaults'} in {time.perf_counter() - t0:.2f} s)")
 
     # ---- batched path: warmup, then K timed steps ----------------------------
-    log(f"[bench] warmup {args.warmup} batched steps of {n} chunks")
-    for _ in range(max(1, args.warmup)):
-        capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-    rc, res, _, _ = ctx.results(n)
-    capi.check(rc, "bmf_result")
-    ctx.sync()
-    log(f"[bench] timing {args.steps} batched steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-    ctx.sync()
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    # A stream of batches alternates over `contexts` contexts (each its own
+    # arenas, stream and captured frame, the tuned shapes of the first): the
+    # batches overlap on the GPU. One context is timed first on its own.
+    nctx = max(1, args.contexts)
+    ctxs = [ctx]
+    for _ in range(nctx - 1):
+        cx = capi.Context(max_chunks=n, max_samples=n * dim ** 3, max_vertices=1 << 22, max_indices=1 << 24)
+        cx.set_sampler(**wl.sampler)
+        cx.set_tuning(**tuned)
+        ctxs.append(cx)
+    log(f"[bench] warmup {args.warmup} batched steps of {n} chunks on each of {nctx} context(s)")
+    for cx in ctxs:
+        for _ in range(max(1, args.warmup)):
+            capi.check(cx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        rc, res, _, _ = cx.results(n)
+        capi.check(rc, "bmf_result")
+        cx.sync()
+
+    def timed(cs):
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            capi.check(cs[i % len(cs)].extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        for cx in cs:
+            cx.sync()
+        return time.perf_counter() - t0
+
+    log(f"[bench] timing {args.steps} batched steps on one context")
+    wall1 = timed(ctxs[:1])
+    one_ctx = {"chunks_per_s": round(n * args.steps / wall1, 2), "ms_per_step": round(1e3 * wall1 / args.steps, 4)}
+    log(f"[bench] one context: {one_ctx['chunks_per_s']:.1f} chunks/s ({one_ctx['ms_per_step']:.3f} ms/step)")
+    if nctx > 1:
+        log(f"[bench] timing {args.steps} batched steps alternating over {nctx} contexts")
+        wall = timed(ctxs)
+    else:
+        wall = wall1
     value = n * args.steps / wall
     rc, res, _, _ = ctx.results(n)
     capi.check(rc, "bmf_result")
+    # every context's meshes are the first context's, bit for bit
+    ctx_identical = True
+    if nctx > 1:
+        for cx in ctxs[1:]:
+            rc2, res2, _, _ = cx.results(n)
+            capi.check(rc2, "bmf_result")
+            for j in range(n):
+                if res[j].vertex_count != res2[j].vertex_count or res[j].index_count != res2[j].index_count:
+                    ctx_identical = False
+                elif res[j].vertex_count:
+                    a, b = ctx.mesh(j), cx.mesh(j)
+                    if not all(np.array_equal(u, w) for u, w in zip(a, b)):
+                        ctx_identical = False
+        for cx in ctxs[1:]:
+            cx.close()
     work = ctx.frame_stats()
     sampled = sum(r.sampled for r in res)
     V = sum(r.vertex_count for r in res)
     T = sum(r.index_count for r in res) // 3
-    log(f"[bench] batched: {value:.1f} chunks/s ({1e3 * wall / args.steps:.3f} ms/step), "
-        f"{sampled}/{n} chunks sampled, V={V} T={T}")
+    log(f"[bench] batched: {value:.1f} chunks/s ({1e3 * wall / args.steps:.3f} ms/step over {nctx} context(s), "
+        f"meshes identical across contexts: {ctx_identical}), {sampled}/{n} chunks sampled, V={V} T={T}")
 
     # ---- per-stage device times (HIP events on the library stream) -------------
     stages = None
@@ -746,7 +787,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": "uniform-512x64: 8x8x8 chunks of 64^3 cells, fbm terrain seed 1337, K=4, batched",
                    "chunks": n, "cells_per_chunk": (dim - 1) ** 3, "smoothing_iters": 4, "octaves": octaves,
-                   "sampled_chunks": sampled, "vertices": V, "triangles": T, "tuning": tuned},
+                   "sampled_chunks": sampled, "vertices": V, "triangles": T, "tuning": tuned,
+                   "pipeline": {"contexts": nctx, "note": "the timed batches alternate over this many contexts "
+                                "(own arenas, stream and captured frame each; the same tuned shapes), so "
+                                "consecutive batches overlap on the GPU; ms_per_step is the time per batch "
+                                "of the stream", "meshes_identical_across_contexts": ctx_identical}},
+        "one_context": one_ctx,
         "cells_per_s": round(value * (dim - 1) ** 3, 1),
         "surface_chunks_per_s": round(sampled * args.steps / wall, 2),
         "plain_chunks_per_s": round(plain, 2) if plain else None,
