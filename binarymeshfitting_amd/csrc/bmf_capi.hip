@@ -46,35 +46,6 @@ void free_arenas(bmf_ctx* c) {
     c->d_spheres = c->d_boxes = nullptr;
 }
 
-// Streams of the row-word classes in `wclasses`: the first present class on
-// s, each further class on its own auxiliary stream, which waits for s here;
-// join_classes makes s wait for them again.  Under graph capture this is a
-// fork and a join of the captured graph.
-int fork_classes(bmf_ctx* c, int wclasses, hipStream_t s, hipStream_t cs[4], int& used) {
-    used = 0;
-    bool first = true;
-    for (int w = 0; w < 4; w++) {
-        cs[w] = s;
-        if (!(wclasses & (1 << w))) continue;
-        if (first) {
-            first = false;
-            continue;
-        }
-        if (used == 0) BMF_HIP(hipEventRecord(c->ev_fork, s));
-        BMF_HIP(hipStreamWaitEvent(c->aux[used], c->ev_fork, 0));
-        cs[w] = c->aux[used++];
-    }
-    return BMF_OK;
-}
-
-int join_classes(bmf_ctx* c, hipStream_t s, int used) {
-    for (int i = 0; i < used; i++) {
-        BMF_HIP(hipEventRecord(c->ev_join[i], c->aux[i]));
-        BMF_HIP(hipStreamWaitEvent(s, c->ev_join[i], 0));
-    }
-    return BMF_OK;
-}
-
 // The whole device pipeline for chunks [0, n) of the uploaded descriptors.
 // With `ev` (BMF_NUM_STAGES + 1 events) the stage boundaries are recorded.
 // Workgroups of the batched row scan: one per chunk up to two per CU; the
@@ -96,23 +67,16 @@ int enqueue_pipeline(bmf_ctx* c, const Arenas& A, int n, int maxk, int wclasses,
     c->stat_nrefine = row_blocks;
     BMF_MARK(2);
     const bool quads = A.mesh_mode == BMF_MESH_QUADS;
-    hipStream_t cs[4];
-    int used = 0;
-    int rc = fork_classes(c, wclasses, s, cs, used);
-    if (rc) return rc;
-    BMF_HIP(launch_count(A, n, row_blocks, wclasses, cs));
-    rc = join_classes(c, s, used);
-    if (rc) return rc;
+    // the row-word classes of a mixed batch one after another on the one
+    // stream: a context is one stream (one hardware queue), so several
+    // contexts on a GPU overlap their batches
+    BMF_HIP(launch_count(A, n, row_blocks, wclasses, s));
     BMF_HIP(launch_refine(A, n, row_blocks, s));
     BMF_MARK(3);
     BMF_HIP(launch_scan(A, n, scan_blocks, s));
     BMF_MARK(4);
-    rc = fork_classes(c, wclasses, s, cs, used);
-    if (rc) return rc;
-    if (quads) BMF_HIP(launch_dmc_emit(A, n, row_blocks, wclasses, cs));
-    else BMF_HIP(launch_emit(A, n, row_blocks, wclasses, cs));
-    rc = join_classes(c, s, used);
-    if (rc) return rc;
+    if (quads) BMF_HIP(launch_dmc_emit(A, n, row_blocks, wclasses, s));
+    else BMF_HIP(launch_emit(A, n, row_blocks, wclasses, s));
     BMF_MARK(5);
     for (int m = 0; m < maxk; m++) {
         if (quads) BMF_HIP(launch_dmc_smooth(A, m, smooth_blocks, s));
@@ -357,10 +321,6 @@ int bmf_create(bmf_ctx** out, int device, const bmf_limits* lim) {
          hipHostMalloc((void**)&c->h_descs[1], sizeof(bmf_chunk_desc) * nc) == hipSuccess &&
          hipHostMalloc((void**)&c->h_results, sizeof(bmf_chunk_result) * nc) == hipSuccess &&
          hipHostMalloc((void**)&c->h_ctr, sizeof(Counters)) == hipSuccess;
-    for (int i = 0; ok && i < 3; i++)
-        ok = hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming) == hipSuccess;
-    ok = ok && hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
          hipEventCreate(&c->ev_upload[0]) == hipSuccess && hipEventCreate(&c->ev_upload[1]) == hipSuccess;
     for (int i = 0; ok && i <= BMF_NUM_STAGES; i++) ok = hipEventCreate(&c->ev_stage[i]) == hipSuccess;
@@ -398,14 +358,11 @@ void bmf_destroy(bmf_ctx* c) {
     if (c->d_stitch_tiles) (void)hipFree(c->d_stitch_tiles);
     if (c->d_stitch_misc) (void)hipFree(c->d_stitch_misc);
     if (c->h_stitch_total) (void)hipHostFree(c->h_stitch_total);
-    hipEvent_t evs[] = {c->ev0, c->ev1, c->ev_upload[0], c->ev_upload[1], c->ev_fork, c->ev_join[0], c->ev_join[1],
-                        c->ev_join[2]};
+    hipEvent_t evs[] = {c->ev0, c->ev1, c->ev_upload[0], c->ev_upload[1]};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
     for (int i = 0; i <= BMF_NUM_STAGES; i++)
         if (c->ev_stage[i]) (void)hipEventDestroy(c->ev_stage[i]);
-    for (int i = 0; i < 3; i++)
-        if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -23,11 +23,6 @@ struct bmf_ctx {
     int device = 0;
     int cu_count = 256;
     hipStream_t stream = nullptr;
-    // the row-word classes of one batch run concurrently: the first class on
-    // `stream`, each further one on an auxiliary stream (fork / join events)
-    hipStream_t aux[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t ev_fork = nullptr;
-    hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
     bmf_limits lim{};
     bmf::Arenas A{};
     bmf_chunk_desc* h_descs[2] = {nullptr, nullptr};  // pinned, double-buffered uploads

@@ -925,16 +925,16 @@ __global__ void __launch_bounds__(kSmoothThreads) k_primal_ring_nrm(Arenas A, in
 // launch wrappers
 // ------------------------------------------------------------------------
 
-hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs) {
+hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
     // per class: vertices, then the quads that index them
-    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_vertices<1>, dim3(blocks), dim3(kRowThreads), 0, cs[0], A, n);
-    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_vertices<2>, dim3(blocks), dim3(kRowThreads), 0, cs[1], A, n);
-    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_vertices<3>, dim3(blocks), dim3(kRowThreads), 0, cs[2], A, n);
-    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_vertices<4>, dim3(blocks), dim3(kRowThreads), 0, cs[3], A, n);
-    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_quads<1>, dim3(blocks), dim3(kRowThreads), 0, cs[0], A, n);
-    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_quads<2>, dim3(blocks), dim3(kRowThreads), 0, cs[1], A, n);
-    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_quads<3>, dim3(blocks), dim3(kRowThreads), 0, cs[2], A, n);
-    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_quads<4>, dim3(blocks), dim3(kRowThreads), 0, cs[3], A, n);
+    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_vertices<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_vertices<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_vertices<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_vertices<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 1) hipLaunchKernelGGL(k_dmc_quads<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_dmc_quads<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_dmc_quads<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_dmc_quads<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
     return hipGetLastError();
 }
 

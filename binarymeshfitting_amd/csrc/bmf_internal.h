@@ -177,22 +177,18 @@ hipError_t launch_plan(const Arenas& A, int n, int allow_cull, int reset_bumps, 
 // kind: the sampler the device struct holds for this launch (BMF_SAMPLER_*)
 // slab: the sampler is a 2D noise field (its columns are evaluated once per tile)
 hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int kind, int slab, hipStream_t s);
-// wclasses: bit W-1 set when the batch holds chunks with ceil(dim/64) == W.
+// wclasses: bit W-1 set when the batch holds chunks with ceil(dim/64) == W
+// (the classes' kernels one after another on s).
 hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
-// cs[W-1]: the stream of row-word class W
-hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs);
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
-// cs[W-1]: the stream of row-word class W
-hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs);
 hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s);
 hipError_t launch_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 // (chunk, tile) of every smoothing vertex and primitive tile, once per frame before the first step
 hipError_t launch_smooth_map(const Arenas& A, int blocks, hipStream_t s);
 // Quad mode (bmf_dmc.hip): counts after launch_count/launch_refine, emission
 // after launch_scan, smoothing over the per-vertex rings.
-// cs[W-1]: the stream of row-word class W
-hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs);
+hipError_t launch_dmc_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s);
 hipError_t launch_dmc_smooth(const Arenas& A, int m, int blocks, hipStream_t s);
 hipError_t launch_ref_labels(const Arenas& A, uint32_t* words, hipStream_t s);
 hipError_t launch_ref_masks(const Arenas& A, uint64_t* masks, hipStream_t s);

@@ -1950,16 +1950,12 @@ hipError_t launch_sample(const Arenas& A, int n, int blocks, int adaptive, int k
 #undef BMF_SAMPLE
     return hipGetLastError();
 }
-hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs) {
-    if (wclasses & 1) hipLaunchKernelGGL(k_count<1>, dim3(blocks), dim3(kRowThreads), 0, cs[0], A, n);
-    if (wclasses & 2) hipLaunchKernelGGL(k_count<2>, dim3(blocks), dim3(kRowThreads), 0, cs[1], A, n);
-    if (wclasses & 4) hipLaunchKernelGGL(k_count<3>, dim3(blocks), dim3(kRowThreads), 0, cs[2], A, n);
-    if (wclasses & 8) hipLaunchKernelGGL(k_count<4>, dim3(blocks), dim3(kRowThreads), 0, cs[3], A, n);
-    return hipGetLastError();
-}
 hipError_t launch_count(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
-    const hipStream_t cs[4] = {s, s, s, s};
-    return launch_count(A, n, blocks, wclasses, cs);
+    if (wclasses & 1) hipLaunchKernelGGL(k_count<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_count<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_count<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_count<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    return hipGetLastError();
 }
 hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_scan, dim3(blocks), dim3(kScanThreads), 0, s, A, n);
@@ -1975,16 +1971,12 @@ hipError_t launch_scan(const Arenas& A, int n, int blocks, hipStream_t s) {
     }
     return hipGetLastError();
 }
-hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, const hipStream_t* cs) {
-    if (wclasses & 1) hipLaunchKernelGGL(k_emit<1>, dim3(blocks), dim3(kRowThreads), 0, cs[0], A, n);
-    if (wclasses & 2) hipLaunchKernelGGL(k_emit<2>, dim3(blocks), dim3(kRowThreads), 0, cs[1], A, n);
-    if (wclasses & 4) hipLaunchKernelGGL(k_emit<3>, dim3(blocks), dim3(kRowThreads), 0, cs[2], A, n);
-    if (wclasses & 8) hipLaunchKernelGGL(k_emit<4>, dim3(blocks), dim3(kRowThreads), 0, cs[3], A, n);
-    return hipGetLastError();
-}
 hipError_t launch_emit(const Arenas& A, int n, int blocks, int wclasses, hipStream_t s) {
-    const hipStream_t cs[4] = {s, s, s, s};
-    return launch_emit(A, n, blocks, wclasses, cs);
+    if (wclasses & 1) hipLaunchKernelGGL(k_emit<1>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 2) hipLaunchKernelGGL(k_emit<2>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 4) hipLaunchKernelGGL(k_emit<3>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    if (wclasses & 8) hipLaunchKernelGGL(k_emit<4>, dim3(blocks), dim3(kRowThreads), 0, s, A, n);
+    return hipGetLastError();
 }
 hipError_t launch_refine(const Arenas& A, int n, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_refine, dim3(blocks), dim3(kSampleThreads), 0, s, A, n);
