@@ -104,7 +104,57 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def lod_leg(capi, roofline, workloads, steps, only_class=0):
+def stream_times(capi, ctx, make_ctx, arr, n, steps, nctx, warmup=2):
+    """A stream of `steps` batched extractions of the same batch: first on
+    `ctx` alone (back to back on its stream), then alternating over `ctx` and
+    nctx - 1 further contexts from make_ctx() (own arenas, stream and captured
+    frame; the same sampler, mesh options and launch shapes), so consecutive
+    batches overlap on the GPU.  Returns the seconds per batch of each, the
+    context count of the faster (a per-workload choice, like the launch
+    shapes: batches whose sampling fills the CUs overlap little) and whether
+    every context's meshes equal the first's bit for bit."""
+    extra = []
+    try:
+        for _ in range(nctx - 1):
+            cx = make_ctx()
+            cx.set_tuning(**ctx.tuning())
+            extra.append(cx)
+        cs = [ctx] + extra
+        for cx in cs:
+            for _ in range(max(1, warmup)):
+                capi.check(cx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+            cx.sync()
+
+        def run(group):
+            t0 = time.perf_counter()
+            for i in range(steps):
+                capi.check(group[i % len(group)].extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+            for cx in group:
+                cx.sync()
+            return (time.perf_counter() - t0) / steps
+
+        one = run([ctx])
+        multi = run(cs) if extra else one
+        same = True
+        if extra:
+            rc, res, _, _ = ctx.results(n)
+            capi.check(rc, "bmf_result")
+            for cx in extra:
+                rc2, res2, _, _ = cx.results(n)
+                capi.check(rc2, "bmf_result")
+                for j in range(n):
+                    if (res[j].vertex_count, res[j].index_count) != (res2[j].vertex_count, res2[j].index_count):
+                        same = False
+                    elif res[j].vertex_count and not all(
+                            np.array_equal(u, w) for u, w in zip(ctx.mesh(j), cx.mesh(j))):
+                        same = False
+    finally:
+        for cx in extra:
+            cx.close()
+    return one, multi, (nctx if multi < one else 1), same
+
+
+def lod_leg(capi, roofline, workloads, steps, only_class=0, nctx=2):
     """Uneven-LOD batch: 1,536 chunks (922 / 461 / 153 of 32^3 / 64^3 / 128^3
     cells at spacing 1 / 2 / 4, origins from seed 7 in a 4096^3 world), 7-octave
     field, K ~ U{0..8}; shapes from the autotuner, then batched and plain.
@@ -121,14 +171,14 @@ def lod_leg(capi, roofline, workloads, steps, only_class=0):
         t0 = time.perf_counter()
         tuned = ctx.autotune(cd, reps=3)
         t_tune = time.perf_counter() - t0
-        for _ in range(2):
-            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        ctx.sync()
-        wall = (time.perf_counter() - t0) / steps
+
+        def make():
+            cx = capi.Context(max_chunks=2048, max_samples=96 * 129 ** 3, max_vertices=1 << 23, max_indices=1 << 25)
+            cx.set_sampler(**wl.sampler)
+            return cx
+
+        wall1, wallm, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
+        wall = min(wall1, wallm)
         rc, res, _, _ = ctx.results(n)
         capi.check(rc, "bmf_result")
         sampled = sum(r.sampled for r in res)
@@ -144,8 +194,9 @@ def lod_leg(capi, roofline, workloads, steps, only_class=0):
         capi.check(ctx.extract_raw(arr, n, capi.MODE_PLAIN), "bmf_extract plain")
         ctx.sync()
         plain = time.perf_counter() - t0
-    log(f"[bench] uneven LOD: {n / wall:.1f} chunks/s batched ({1e3 * wall:.3f} ms/step), plain {n / plain:.1f} chunks/s,"
-        f" sampled {sampled}/{n}, non-empty {nonempty}/{n}, tuned {tuned} in {t_tune:.2f} s")
+    log(f"[bench] uneven LOD: {n / wall:.1f} chunks/s batched ({1e3 * wall:.3f} ms/step over {used} context(s); one "
+        f"context {n / wall1:.1f}), plain {n / plain:.1f} chunks/s, sampled {sampled}/{n}, non-empty {nonempty}/{n}, "
+        f"tuned {tuned} in {t_tune:.2f} s")
     name = "uneven-lod-1536: 922/461/153 chunks of 32^3/64^3/128^3 cells, spacing 1/2/4, 7 octaves, K~U{0..8}"
     if only_class:
         name += f"; only the {n} chunks with ceil(dim/64) = {only_class}"
@@ -155,12 +206,16 @@ def lod_leg(capi, roofline, workloads, steps, only_class=0):
             "batched_over_plain": round(plain / wall, 2), "sampled_chunks": sampled, "nonempty_chunks": nonempty,
             "nonempty_fraction": round(nonempty / n, 4), "surface_chunks_per_s": round(sampled / wall, 2),
             "tuning": tuned, "autotune_s": round(t_tune, 3),
+            "contexts": used, "meshes_identical_across_contexts": same,
+            "one_context": {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)},
+            "stream_over_contexts": {"contexts": nctx, "chunks_per_s": round(n / wallm, 2),
+                                     "ms_per_step": round(1e3 * wallm, 4)},
             "roofline": roofline.frame_fractions(flops, nbytes, wall),
             "plain_roofline": roofline.frame_fractions(flops, nbytes, plain),
             "_check": {"workload": wl, "meshes": meshes}}
 
 
-def world_leg(capi, roofline, workloads, steps):
+def world_leg(capi, roofline, workloads, steps, nctx=2):
     """The reference's default world: WorldOctree's sampler (terrain2d_pert_block with the
     world's noise_properties, WorldOctree.cpp:40-54) over the renderable leaves the LOD pass
     leaves around a focus on the surface of a 4096-unit world (max level 7), chunk_resolution
@@ -176,16 +231,14 @@ def world_leg(capi, roofline, workloads, steps):
     cd = workloads.to_capi(chunks)
     n = len(cd)
     arr = capi.chunk_array(cd)
-    with capi.Context(max_chunks=1024, max_samples=n * 32 ** 3, max_vertices=1 << 22, max_indices=1 << 24) as ctx:
-        ctx.set_sampler(**sd)
-        for _ in range(2):
-            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        ctx.sync()
-        wall = (time.perf_counter() - t0) / steps
+    def make():
+        cx = capi.Context(max_chunks=1024, max_samples=n * 32 ** 3, max_vertices=1 << 22, max_indices=1 << 24)
+        cx.set_sampler(**sd)
+        return cx
+
+    with make() as ctx:
+        wall1, wallm, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
+        wall = min(wall1, wallm)
         stages = ctx.profile_stages(cd)
         rc, res, _, _ = ctx.results(n)
         capi.check(rc, "bmf_result")
@@ -200,13 +253,18 @@ def world_leg(capi, roofline, workloads, steps):
         ctx.sync()
         plain = time.perf_counter() - t0
     V = sum(c[0] for c in counts)
-    log(f"[bench] default world: {n / wall:.1f} chunks/s batched ({1e3 * wall:.3f} ms/step), plain {n / plain:.1f}"
+    log(f"[bench] default world: {n / wall:.1f} chunks/s batched ({1e3 * wall:.3f} ms/step over {used} context(s); "
+        f"one context {n / wall1:.1f}), plain {n / plain:.1f}"
         f" chunks/s, sampled {sampled}/{n}, non-empty {len(meshes)}/{n}, V={V}, stages "
         + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
     return {"workload": f"world-default-{n}: the reference world's terrain2d_pert sampler (own noise; 3 value-noise "
                         f"octaves, 13 gradient-perturbation octaves) on the {n} LOD leaves around {focus} of a "
                         f"4096-unit world, 32^3 samples per chunk, K=0",
             "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4),
+            "contexts": used, "meshes_identical_across_contexts": same,
+            "one_context": {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)},
+            "stream_over_contexts": {"contexts": nctx, "chunks_per_s": round(n / wallm, 2),
+                                     "ms_per_step": round(1e3 * wallm, 4)},
             "plain_chunks_per_s": round(n / plain, 2), "batched_over_plain": round(plain / wall, 2),
             "sampled_chunks": sampled, "nonempty_chunks": len(meshes), "vertices": V,
             "surface_chunks_per_s": round(sampled / wall, 2),
@@ -235,7 +293,7 @@ def plan_scaling(capi, workloads, sizes=(512, 4096, 16384, 65536)):
     return out
 
 
-def hermite_leg(capi, roofline, workloads, steps):
+def hermite_leg(capi, roofline, workloads, steps, nctx=2):
     """64 chunks of 64^3 cells tiled 4x4x4 over [0,256)^3 of the CSG union of
     200 spheres and 100 boxes (seed 42); dual-marching-cubes quads with FP32
     QEF placement (tolerance 0.1), batched."""
@@ -243,17 +301,15 @@ def hermite_leg(capi, roofline, workloads, steps):
     cd = workloads.to_capi(wl.chunks)
     n = len(cd)
     arr = capi.chunk_array(cd)
-    with capi.Context(max_chunks=n, max_samples=n * 65 ** 3, max_vertices=1 << 22, max_indices=1 << 24) as ctx:
-        ctx.set_sampler(**wl.sampler)
-        ctx.set_mesh_options(mesh=capi.MESH_QUADS, placement=capi.PLACE_QEF, qef_tolerance=0.1)
-        for _ in range(2):
-            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        ctx.sync()
-        wall = (time.perf_counter() - t0) / steps
+    def make():
+        cx = capi.Context(max_chunks=n, max_samples=n * 65 ** 3, max_vertices=1 << 22, max_indices=1 << 24)
+        cx.set_sampler(**wl.sampler)
+        cx.set_mesh_options(mesh=capi.MESH_QUADS, placement=capi.PLACE_QEF, qef_tolerance=0.1)
+        return cx
+
+    with make() as ctx:
+        wall1, wallm, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
+        wall = min(wall1, wallm)
         rc, res, _, _ = ctx.results(n)
         capi.check(rc, "bmf_result")
         V = sum(r.vertex_count for r in res)
@@ -265,10 +321,15 @@ def hermite_leg(capi, roofline, workloads, steps):
             len(wl.sampler["boxes"]) * roofline.CSG_BOX_FLOPS
         flops = sum(c.dim ** 3 * prims for r, c in zip(res, wl.chunks) if r.sampled)
         nbytes = sum(roofline.mesh_bytes(r.vertex_count, r.index_count // 4 * 2) for r in res)
-    log(f"[bench] hermite/QEF: {n / wall:.1f} chunks/s ({1e3 * wall:.3f} ms/step), V={V} quads={Q}, "
+    log(f"[bench] hermite/QEF: {n / wall:.1f} chunks/s ({1e3 * wall:.3f} ms/step over {used} context(s); one context "
+        f"{n / wall1:.1f}), V={V} quads={Q}, "
         f"stages {', '.join(f'{k}={v:.3f}' for k, v in stages.items())}")
     return {"workload": "csg-64x64: 4x4x4 chunks of 64^3 cells, 200 spheres + 100 boxes (seed 42), DMC quads, QEF tol 0.1",
             "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4), "vertices": V, "quads": Q,
+            "contexts": used, "meshes_identical_across_contexts": same,
+            "one_context": {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)},
+            "stream_over_contexts": {"contexts": nctx, "chunks_per_s": round(n / wallm, 2),
+                                     "ms_per_step": round(1e3 * wallm, 4)},
             "qef_residual": {"definition": "uncentred |ATb - ATA x|^2 of each vertex's QEF (qef_simd.h:434-440), "
                                            "not an error against FP64",
                              "median": float(np.median(err)), "max": float(err.max())},
@@ -321,7 +382,7 @@ def single_leg(capi, workloads, steps, with_cpu):
     return out
 
 
-def quads_leg(capi, roofline, workloads, steps):
+def quads_leg(capi, roofline, workloads, steps, nctx=2):
     """The uniform 512-chunk batch in dual-marching-cubes quad mode: one
     vertex per EdgeTable polygon at the mass point of its edge points, one
     quad per sign-changing edge (Tables.hpp:76-354), then K = 4 smoothing
@@ -330,17 +391,15 @@ def quads_leg(capi, roofline, workloads, steps):
     cd = workloads.to_capi(wl.chunks)
     n = len(cd)
     arr = capi.chunk_array(cd)
-    with capi.Context(max_chunks=n, max_samples=n * 65 ** 3, max_vertices=1 << 22, max_indices=1 << 24) as ctx:
-        ctx.set_sampler(**wl.sampler)
-        ctx.set_mesh_options(mesh=capi.MESH_QUADS)
-        for _ in range(2):
-            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            capi.check(ctx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        ctx.sync()
-        wall = (time.perf_counter() - t0) / steps
+    def make():
+        cx = capi.Context(max_chunks=n, max_samples=n * 65 ** 3, max_vertices=1 << 22, max_indices=1 << 24)
+        cx.set_sampler(**wl.sampler)
+        cx.set_mesh_options(mesh=capi.MESH_QUADS)
+        return cx
+
+    with make() as ctx:
+        wall1, wallm, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
+        wall = min(wall1, wallm)
         rc, res, _, _ = ctx.results(n)
         capi.check(rc, "bmf_result")
         sampled = sum(r.sampled for r in res)
@@ -358,11 +417,16 @@ def quads_leg(capi, roofline, workloads, steps):
     flops = sampled * 65 ** 3 * roofline.terrain_flops_per_sample(wl.sampler["octaves"]) + \
         roofline.smoothing_flops(V, 2 * Q, 4)
     nbytes = roofline.mesh_bytes(V, 2 * Q)
-    log(f"[bench] quads: {n / wall:.1f} chunks/s ({1e3 * wall:.3f} ms/step), plain {n / plain:.1f} chunks/s, "
+    log(f"[bench] quads: {n / wall:.1f} chunks/s ({1e3 * wall:.3f} ms/step over {used} context(s); one context "
+        f"{n / wall1:.1f}), plain {n / plain:.1f} chunks/s, "
         f"V={V} quads={Q}, stages {', '.join(f'{k}={v:.3f}' for k, v in stages.items())}")
     return {"workload": "uniform-512x64-quads: 8x8x8 chunks of 64^3 cells, fbm terrain seed 1337, "
                         "DMC quads (mass point), K=4, batched",
             "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4),
+            "contexts": used, "meshes_identical_across_contexts": same,
+            "one_context": {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)},
+            "stream_over_contexts": {"contexts": nctx, "chunks_per_s": round(n / wallm, 2),
+                                     "ms_per_step": round(1e3 * wallm, 4)},
             "cells_per_s": round(n * 64 ** 3 / wall, 1), "surface_chunks_per_s": round(sampled / wall, 2),
             "plain_chunks_per_s": round(n / plain, 2), "batched_over_plain": round(plain / wall, 2),
             "sampled_chunks": sampled, "vertices": V, "quads": Q,
@@ -447,72 +511,37 @@ def main():
     dim = wl.chunks[0].dim
     cd = workloads.to_capi(wl.chunks)
     arr = capi.chunk_array(cd)
-    ctx = capi.Context(max_chunks=n, max_samples=n * dim ** 3, max_vertices=1 << 22, max_indices=1 << 24)
-    ctx.set_sampler(**wl.sampler)
+    def make():
+        cx = capi.Context(max_chunks=n, max_samples=n * dim ** 3, max_vertices=1 << 22, max_indices=1 << 24)
+        cx.set_sampler(**wl.sampler)
+        return cx
+
+    ctx = make()
     # launch shapes: the autotuner's pick on this batch (tile size, workgroups per kernel, row split)
     t0 = time.perf_counter()
     tuned = ctx.autotune(cd, reps=3) if args.autotune else ctx.tuning()
     log(f"[bench] tuning {tuned} ({'autotuned' if args.autotune else 'defaults'} in {time.perf_counter() - t0:.2f} s)")
 
     # ---- batched path: warmup, then K timed steps ----------------------------
-    # A stream of batches alternates over `contexts` contexts (each its own
-    # arenas, stream and captured frame, the tuned shapes of the first): the
-    # batches overlap on the GPU. One context is timed first on its own.
+    # The stream of batches is timed on one context, then alternating over
+    # `contexts` contexts (each its own arenas, stream and captured frame, the
+    # tuned shapes of the first), so consecutive batches overlap on the GPU.
     nctx = max(1, args.contexts)
-    ctxs = [ctx]
-    for _ in range(nctx - 1):
-        cx = capi.Context(max_chunks=n, max_samples=n * dim ** 3, max_vertices=1 << 22, max_indices=1 << 24)
-        cx.set_sampler(**wl.sampler)
-        cx.set_tuning(**tuned)
-        ctxs.append(cx)
-    log(f"[bench] warmup {args.warmup} batched steps of {n} chunks on each of {nctx} context(s)")
-    for cx in ctxs:
-        for _ in range(max(1, args.warmup)):
-            capi.check(cx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        rc, res, _, _ = cx.results(n)
-        capi.check(rc, "bmf_result")
-        cx.sync()
-
-    def timed(cs):
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            capi.check(cs[i % len(cs)].extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        for cx in cs:
-            cx.sync()
-        return time.perf_counter() - t0
-
-    log(f"[bench] timing {args.steps} batched steps on one context")
-    wall1 = timed(ctxs[:1])
-    one_ctx = {"chunks_per_s": round(n * args.steps / wall1, 2), "ms_per_step": round(1e3 * wall1 / args.steps, 4)}
+    log(f"[bench] {args.warmup} warmup and {args.steps} timed batched steps of {n} chunks, on one context and "
+        f"alternating over {nctx}")
+    wall1, wall_b, used, ctx_identical = stream_times(capi, ctx, make, arr, n, args.steps, nctx, args.warmup)
+    one_ctx = {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)}
     log(f"[bench] one context: {one_ctx['chunks_per_s']:.1f} chunks/s ({one_ctx['ms_per_step']:.3f} ms/step)")
-    if nctx > 1:
-        log(f"[bench] timing {args.steps} batched steps alternating over {nctx} contexts")
-        wall = timed(ctxs)
-    else:
-        wall = wall1
+    stream = {"contexts": nctx, "chunks_per_s": round(n / wall_b, 2), "ms_per_step": round(1e3 * wall_b, 4)}
+    wall = min(wall1, wall_b) * args.steps
     value = n * args.steps / wall
     rc, res, _, _ = ctx.results(n)
     capi.check(rc, "bmf_result")
-    # every context's meshes are the first context's, bit for bit
-    ctx_identical = True
-    if nctx > 1:
-        for cx in ctxs[1:]:
-            rc2, res2, _, _ = cx.results(n)
-            capi.check(rc2, "bmf_result")
-            for j in range(n):
-                if res[j].vertex_count != res2[j].vertex_count or res[j].index_count != res2[j].index_count:
-                    ctx_identical = False
-                elif res[j].vertex_count:
-                    a, b = ctx.mesh(j), cx.mesh(j)
-                    if not all(np.array_equal(u, w) for u, w in zip(a, b)):
-                        ctx_identical = False
-        for cx in ctxs[1:]:
-            cx.close()
     work = ctx.frame_stats()
     sampled = sum(r.sampled for r in res)
     V = sum(r.vertex_count for r in res)
     T = sum(r.index_count for r in res) // 3
-    log(f"[bench] batched: {value:.1f} chunks/s ({1e3 * wall / args.steps:.3f} ms/step over {nctx} context(s), "
+    log(f"[bench] batched: {value:.1f} chunks/s ({1e3 * wall / args.steps:.3f} ms/step over {used} context(s), "
         f"meshes identical across contexts: {ctx_identical}), {sampled}/{n} chunks sampled, V={V} T={T}")
 
     # ---- per-stage device times (HIP events on the library stream) -------------
@@ -591,7 +620,7 @@ def main():
     # ---- uneven LOD: 1,536 chunks of 32/64/128 cells, autotuned shapes ------------------
     lod = None
     if args.lod_steps > 0:
-        lod = lod_leg(capi, roofline, workloads, args.lod_steps, args.lod_class)
+        lod = lod_leg(capi, roofline, workloads, args.lod_steps, args.lod_class, max(1, args.contexts))
 
     # ---- the batch prologue at growing batch sizes ---------------------------------------
     scaling = plan_scaling(capi, workloads) if args.lod_steps > 0 else None
@@ -599,17 +628,17 @@ def main():
     # ---- the reference's default world sampler over its LOD leaves ----------------------
     world = None
     if args.world_steps > 0:
-        world = world_leg(capi, roofline, workloads, args.world_steps)
+        world = world_leg(capi, roofline, workloads, args.world_steps, max(1, args.contexts))
 
     # ---- Hermite / QEF: 64 CSG chunks, dual quads placed by FP32 QEF ---------------------
     hermite = None
     if args.hermite_steps > 0:
-        hermite = hermite_leg(capi, roofline, workloads, args.hermite_steps)
+        hermite = hermite_leg(capi, roofline, workloads, args.hermite_steps, max(1, args.contexts))
 
     # ---- config 2 as worded: the same batch in dual-marching-cubes quad mode --------------
     quads = None
     if args.quads_steps > 0:
-        quads = quads_leg(capi, roofline, workloads, args.quads_steps)
+        quads = quads_leg(capi, roofline, workloads, args.quads_steps, max(1, args.contexts))
 
     # ---- single chunk, plain path: latency against one CPU thread ----------------------
     single = None
@@ -788,11 +817,13 @@ def main():
         "config": {"workload": "uniform-512x64: 8x8x8 chunks of 64^3 cells, fbm terrain seed 1337, K=4, batched",
                    "chunks": n, "cells_per_chunk": (dim - 1) ** 3, "smoothing_iters": 4, "octaves": octaves,
                    "sampled_chunks": sampled, "vertices": V, "triangles": T, "tuning": tuned,
-                   "pipeline": {"contexts": nctx, "note": "the timed batches alternate over this many contexts "
+                   "pipeline": {"contexts": used, "note": "the timed batches alternate over this many contexts "
                                 "(own arenas, stream and captured frame each; the same tuned shapes), so "
                                 "consecutive batches overlap on the GPU; ms_per_step is the time per batch "
-                                "of the stream", "meshes_identical_across_contexts": ctx_identical}},
+                                "of the stream; the faster of one context and the stream over several",
+                                "meshes_identical_across_contexts": ctx_identical}},
         "one_context": one_ctx,
+        "stream_over_contexts": stream,
         "cells_per_s": round(value * (dim - 1) ** 3, 1),
         "surface_chunks_per_s": round(sampled * args.steps / wall, 2),
         "plain_chunks_per_s": round(plain, 2) if plain else None,
