@@ -162,6 +162,9 @@ typedef struct {
 } bmf_mesh_view;
 
 /* ---- lifetime ---------------------------------------------------------- */
+/* A context owns its arenas, one stream and its captured frames; several
+ * contexts on one device share nothing, and their batches run concurrently
+ * (a stream of batches alternated over two overlaps on the GPU). */
 int bmf_create(bmf_ctx** out, int device, const bmf_limits* limits);
 void bmf_destroy(bmf_ctx* ctx);
 const char* bmf_strerror(int code);
