@@ -57,10 +57,10 @@ def parse():
                     help="timed batched steps of the uniform batch in DMC quad mode (512 chunks); 0 skips it")
     ap.add_argument("--diff-steps", type=int, default=5,
                     help="timed steps of the forward+loss+backward leg (one 128^3-cell chunk, K=8); 0 skips it")
-    ap.add_argument("--contexts", type=int, default=2,
-                    help="contexts the headline's timed batches alternate over (each with its own arenas and "
-                         "stream, so one batch's sampling overlaps the previous batch's row and smoothing "
-                         "kernels); 1 = one context, batches back to back on one stream")
+    ap.add_argument("--contexts", type=int, default=3,
+                    help="the timed batches of each batched leg alternate over 1 .. this many contexts (each with "
+                         "its own arenas and stream, so one batch's sampling overlaps the previous batch's row and "
+                         "smoothing kernels); the fastest count is reported, with every count's figure")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the library's default launch shapes for the headline batch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -105,14 +105,14 @@ def log(msg):
 
 
 def stream_times(capi, ctx, make_ctx, arr, n, steps, nctx, warmup=2):
-    """A stream of `steps` batched extractions of the same batch: first on
-    `ctx` alone (back to back on its stream), then alternating over `ctx` and
-    nctx - 1 further contexts from make_ctx() (own arenas, stream and captured
-    frame; the same sampler, mesh options and launch shapes), so consecutive
-    batches overlap on the GPU.  Returns the seconds per batch of each, the
-    context count of the faster (a per-workload choice, like the launch
-    shapes: batches whose sampling fills the CUs overlap little) and whether
-    every context's meshes equal the first's bit for bit."""
+    """A stream of `steps` batched extractions of the same batch, timed on
+    `ctx` alone (back to back on its stream) and alternating over k = 2 ..
+    nctx contexts (`ctx` and further ones from make_ctx(): own arenas, stream
+    and captured frame; the same sampler, mesh options and launch shapes), so
+    that consecutive batches overlap on the GPU.  Returns the seconds per
+    batch for each k, the fastest k (a per-workload choice, like the launch
+    shapes) and whether every context's meshes equal the first's bit for
+    bit."""
     extra = []
     try:
         for _ in range(nctx - 1):
@@ -133,8 +133,8 @@ def stream_times(capi, ctx, make_ctx, arr, n, steps, nctx, warmup=2):
                 cx.sync()
             return (time.perf_counter() - t0) / steps
 
-        one = run([ctx])
-        multi = run(cs) if extra else one
+        per = {k: run(cs[:k]) for k in range(1, nctx + 1)}
+        best = min(per, key=per.get)
         same = True
         if extra:
             rc, res, _, _ = ctx.results(n)
@@ -151,7 +151,15 @@ def stream_times(capi, ctx, make_ctx, arr, n, steps, nctx, warmup=2):
     finally:
         for cx in extra:
             cx.close()
-    return one, multi, (nctx if multi < one else 1), same
+    return per, best, same
+
+
+def stream_fields(n, per, best, same):
+    """JSON fields of a leg's stream timings."""
+    return {"contexts": best, "meshes_identical_across_contexts": same,
+            "one_context": {"chunks_per_s": round(n / per[1], 2), "ms_per_step": round(1e3 * per[1], 4)},
+            "stream_over_contexts": {str(k): {"chunks_per_s": round(n / w, 2), "ms_per_step": round(1e3 * w, 4)}
+                                     for k, w in per.items()}}
 
 
 def lod_leg(capi, roofline, workloads, steps, only_class=0, nctx=2):
@@ -177,8 +185,8 @@ def lod_leg(capi, roofline, workloads, steps, only_class=0, nctx=2):
             cx.set_sampler(**wl.sampler)
             return cx
 
-        wall1, wallm, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
-        wall = min(wall1, wallm)
+        per, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
+        wall, wall1 = per[used], per[1]
         rc, res, _, _ = ctx.results(n)
         capi.check(rc, "bmf_result")
         sampled = sum(r.sampled for r in res)
@@ -206,10 +214,7 @@ def lod_leg(capi, roofline, workloads, steps, only_class=0, nctx=2):
             "batched_over_plain": round(plain / wall, 2), "sampled_chunks": sampled, "nonempty_chunks": nonempty,
             "nonempty_fraction": round(nonempty / n, 4), "surface_chunks_per_s": round(sampled / wall, 2),
             "tuning": tuned, "autotune_s": round(t_tune, 3),
-            "contexts": used, "meshes_identical_across_contexts": same,
-            "one_context": {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)},
-            "stream_over_contexts": {"contexts": nctx, "chunks_per_s": round(n / wallm, 2),
-                                     "ms_per_step": round(1e3 * wallm, 4)},
+            **stream_fields(n, per, used, same),
             "roofline": roofline.frame_fractions(flops, nbytes, wall),
             "plain_roofline": roofline.frame_fractions(flops, nbytes, plain),
             "_check": {"workload": wl, "meshes": meshes}}
@@ -237,8 +242,8 @@ def world_leg(capi, roofline, workloads, steps, nctx=2):
         return cx
 
     with make() as ctx:
-        wall1, wallm, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
-        wall = min(wall1, wallm)
+        per, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
+        wall, wall1 = per[used], per[1]
         stages = ctx.profile_stages(cd)
         rc, res, _, _ = ctx.results(n)
         capi.check(rc, "bmf_result")
@@ -261,10 +266,7 @@ def world_leg(capi, roofline, workloads, steps, nctx=2):
                         f"octaves, 13 gradient-perturbation octaves) on the {n} LOD leaves around {focus} of a "
                         f"4096-unit world, 32^3 samples per chunk, K=0",
             "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4),
-            "contexts": used, "meshes_identical_across_contexts": same,
-            "one_context": {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)},
-            "stream_over_contexts": {"contexts": nctx, "chunks_per_s": round(n / wallm, 2),
-                                     "ms_per_step": round(1e3 * wallm, 4)},
+            **stream_fields(n, per, used, same),
             "plain_chunks_per_s": round(n / plain, 2), "batched_over_plain": round(plain / wall, 2),
             "sampled_chunks": sampled, "nonempty_chunks": len(meshes), "vertices": V,
             "surface_chunks_per_s": round(sampled / wall, 2),
@@ -308,8 +310,8 @@ def hermite_leg(capi, roofline, workloads, steps, nctx=2):
         return cx
 
     with make() as ctx:
-        wall1, wallm, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
-        wall = min(wall1, wallm)
+        per, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
+        wall, wall1 = per[used], per[1]
         rc, res, _, _ = ctx.results(n)
         capi.check(rc, "bmf_result")
         V = sum(r.vertex_count for r in res)
@@ -326,10 +328,7 @@ def hermite_leg(capi, roofline, workloads, steps, nctx=2):
         f"stages {', '.join(f'{k}={v:.3f}' for k, v in stages.items())}")
     return {"workload": "csg-64x64: 4x4x4 chunks of 64^3 cells, 200 spheres + 100 boxes (seed 42), DMC quads, QEF tol 0.1",
             "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4), "vertices": V, "quads": Q,
-            "contexts": used, "meshes_identical_across_contexts": same,
-            "one_context": {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)},
-            "stream_over_contexts": {"contexts": nctx, "chunks_per_s": round(n / wallm, 2),
-                                     "ms_per_step": round(1e3 * wallm, 4)},
+            **stream_fields(n, per, used, same),
             "qef_residual": {"definition": "uncentred |ATb - ATA x|^2 of each vertex's QEF (qef_simd.h:434-440), "
                                            "not an error against FP64",
                              "median": float(np.median(err)), "max": float(err.max())},
@@ -398,8 +397,8 @@ def quads_leg(capi, roofline, workloads, steps, nctx=2):
         return cx
 
     with make() as ctx:
-        wall1, wallm, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
-        wall = min(wall1, wallm)
+        per, used, same = stream_times(capi, ctx, make, arr, n, steps, nctx)
+        wall, wall1 = per[used], per[1]
         rc, res, _, _ = ctx.results(n)
         capi.check(rc, "bmf_result")
         sampled = sum(r.sampled for r in res)
@@ -423,10 +422,7 @@ def quads_leg(capi, roofline, workloads, steps, nctx=2):
     return {"workload": "uniform-512x64-quads: 8x8x8 chunks of 64^3 cells, fbm terrain seed 1337, "
                         "DMC quads (mass point), K=4, batched",
             "chunks_per_s": round(n / wall, 2), "ms_per_step": round(1e3 * wall, 4),
-            "contexts": used, "meshes_identical_across_contexts": same,
-            "one_context": {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)},
-            "stream_over_contexts": {"contexts": nctx, "chunks_per_s": round(n / wallm, 2),
-                                     "ms_per_step": round(1e3 * wallm, 4)},
+            **stream_fields(n, per, used, same),
             "cells_per_s": round(n * 64 ** 3 / wall, 1), "surface_chunks_per_s": round(sampled / wall, 2),
             "plain_chunks_per_s": round(n / plain, 2), "batched_over_plain": round(plain / wall, 2),
             "sampled_chunks": sampled, "vertices": V, "quads": Q,
@@ -528,12 +524,12 @@ def main():
     # tuned shapes of the first), so consecutive batches overlap on the GPU.
     nctx = max(1, args.contexts)
     log(f"[bench] {args.warmup} warmup and {args.steps} timed batched steps of {n} chunks, on one context and "
-        f"alternating over {nctx}")
-    wall1, wall_b, used, ctx_identical = stream_times(capi, ctx, make, arr, n, args.steps, nctx, args.warmup)
-    one_ctx = {"chunks_per_s": round(n / wall1, 2), "ms_per_step": round(1e3 * wall1, 4)}
-    log(f"[bench] one context: {one_ctx['chunks_per_s']:.1f} chunks/s ({one_ctx['ms_per_step']:.3f} ms/step)")
-    stream = {"contexts": nctx, "chunks_per_s": round(n / wall_b, 2), "ms_per_step": round(1e3 * wall_b, 4)}
-    wall = min(wall1, wall_b) * args.steps
+        f"alternating over 1 .. {nctx} contexts")
+    per, used, ctx_identical = stream_times(capi, ctx, make, arr, n, args.steps, nctx, args.warmup)
+    streams = stream_fields(n, per, used, ctx_identical)
+    log("[bench] per context count: " + ", ".join(f"{k}: {n / w:.1f} chunks/s ({1e3 * w:.3f} ms/step)"
+                                                  for k, w in per.items()))
+    wall = per[used] * args.steps
     value = n * args.steps / wall
     rc, res, _, _ = ctx.results(n)
     capi.check(rc, "bmf_result")
@@ -820,10 +816,10 @@ def main():
                    "pipeline": {"contexts": used, "note": "the timed batches alternate over this many contexts "
                                 "(own arenas, stream and captured frame each; the same tuned shapes), so "
                                 "consecutive batches overlap on the GPU; ms_per_step is the time per batch "
-                                "of the stream; the faster of one context and the stream over several",
+                                "of the stream; the fastest of 1 .. --contexts contexts",
                                 "meshes_identical_across_contexts": ctx_identical}},
-        "one_context": one_ctx,
-        "stream_over_contexts": stream,
+        "one_context": streams["one_context"],
+        "stream_over_contexts": streams["stream_over_contexts"],
         "cells_per_s": round(value * (dim - 1) ** 3, 1),
         "surface_chunks_per_s": round(sampled * args.steps / wall, 2),
         "plain_chunks_per_s": round(plain, 2) if plain else None,
