@@ -1,8 +1,8 @@
 """Several contexts on one GPU, their batches in flight together: a stream of
-batches alternated over two contexts (each with its own arenas, stream and
-captured frame) overlaps consecutive batches on the GPU.  Bar: after the
-interleaved batches, each context's meshes are bit-identical to a fresh
-context's extraction of the same batch, and the two contexts' different
+batches alternated over two or three contexts (each with its own arenas,
+stream and captured frame) overlaps consecutive batches on the GPU.  Bar:
+after the interleaved batches, each context's meshes are bit-identical to a
+fresh context's extraction of the same batch, and the contexts' different
 batches have not leaked into each other."""
 import numpy as np
 import pytest
@@ -37,7 +37,8 @@ def _same(a, b):
     return True
 
 
-def test_two_contexts_interleaved_match_one_context():
+@pytest.mark.parametrize("nctx", [2, 3])
+def test_contexts_interleaved_match_one_context(nctx):
     # batch A: a 4x4x4 slice of the uniform grid (K = 4); batch B: the same
     # slice 16 units higher (the terrain band still crosses its lowest layer)
     # with K = 2, so the two meshes differ
@@ -53,18 +54,20 @@ def test_two_contexts_interleaved_match_one_context():
         want_b = _meshes(ref, n)
     assert not _same(want_a, want_b)
     assert sum(v for v, _, _ in want_a) > 0 and sum(v for v, _, _ in want_b) > 0
-    c0, c1 = _ctx(n, dim), _ctx(n, dim)
+    cs = [_ctx(n, dim) for _ in range(nctx)]
     try:
-        # eight batches in flight without a synchronisation between them:
-        # context 0 ends on batch A, context 1 on batch B
-        for i in range(8):
-            cx = (c0, c1)[i % 2]
-            arr = (arr_b, arr_a)[(i // 2) % 2] if i < 6 else (arr_a if cx is c0 else arr_b)
-            capi.check(cx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
-        c0.sync()
-        c1.sync()
-        assert _same(_meshes(c0, n), want_a)
-        assert _same(_meshes(c1, n), want_b)
+        # 4 * nctx batches in flight without a synchronisation between them,
+        # round robin over the contexts; a context's batches alternate A / B,
+        # and its last batch is A for even context numbers, B for odd ones
+        rounds = 4
+        for r in range(rounds):
+            for k, cx in enumerate(cs):
+                arr = arr_a if (r + k + rounds - 1) % 2 == 0 else arr_b
+                capi.check(cx.extract_raw(arr, n, capi.MODE_BATCHED), "bmf_extract")
+        for cx in cs:
+            cx.sync()
+        for k, cx in enumerate(cs):
+            assert _same(_meshes(cx, n), want_a if k % 2 == 0 else want_b), k
     finally:
-        c0.close()
-        c1.close()
+        for cx in cs:
+            cx.close()
