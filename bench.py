@@ -518,6 +518,14 @@ def main():
     tuned = ctx.autotune(cd, reps=3) if args.autotune else ctx.tuning()
     log(f"[bench] tuning {tuned} ({'autotuned' if args.autotune else 'defaults'} in {time.perf_counter() - t0:.2f} s)")
 
+    # ---- per-stage device times (HIP events on the library stream), one context,
+    # before the stream timings -------------------------------------------------
+    stages = None
+    for _ in range(max(1, args.profile_reps)):
+        st = ctx.profile_stages(cd)
+        stages = {k: v + (stages or {}).get(k, 0.0) for k, v in st.items()}
+    stages = {k: v / max(1, args.profile_reps) for k, v in stages.items()}
+    log("[bench] stages (ms): " + ", ".join(f"{k}={v:.4f}" for k, v in stages.items()))
     # ---- batched path: warmup, then K timed steps ----------------------------
     # The stream of batches is timed on one context, then alternating over
     # `contexts` contexts (each its own arenas, stream and captured frame, the
@@ -540,13 +548,6 @@ def main():
     log(f"[bench] batched: {value:.1f} chunks/s ({1e3 * wall / args.steps:.3f} ms/step over {used} context(s), "
         f"meshes identical across contexts: {ctx_identical}), {sampled}/{n} chunks sampled, V={V} T={T}")
 
-    # ---- per-stage device times (HIP events on the library stream) -------------
-    stages = None
-    for _ in range(max(1, args.profile_reps)):
-        st = ctx.profile_stages(cd)
-        stages = {k: v + (stages or {}).get(k, 0.0) for k, v in st.items()}
-    stages = {k: v / max(1, args.profile_reps) for k, v in stages.items()}
-    log("[bench] stages (ms): " + ", ".join(f"{k}={v:.4f}" for k, v in stages.items()))
     octaves = wl.sampler["octaves"]
     samples = sampled * dim ** 3
     flops = samples * roofline.terrain_flops_per_sample(octaves)

@@ -2,6 +2,7 @@
 // the PLAIN and BATCHED execution paths and stage inspection.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -936,26 +937,31 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
     if (rc) return rc;
     int maxk = max_steps(c, chunks, n), wclasses = 0;
     for (int i = 0; i < n; i++) wclasses |= row_word_class(chunks[i].dim);
-    // stage time (ms, best of reps) of the pipeline under tuning t
+    // stage times (ms, the median of reps runs: a candidate with one lucky
+    // run does not win) of the pipeline under tuning t
     float st[BMF_NUM_STAGES];
-    auto run = [&](const bmf_tuning& t, float* out) -> int {
+    auto run_n = [&](const bmf_tuning& t, int nrep, float* out) -> int {
         Arenas A = c->A;
         A.sample_tile = t.sample_tile;
         A.row_split = t.row_split;
-        for (int i = 0; i < BMF_NUM_STAGES; i++) out[i] = 1e30f;
-        for (int r = 0; r < reps; r++) {
+        std::vector<float> ms(static_cast<size_t>(nrep) * BMF_NUM_STAGES);
+        for (int r = 0; r < nrep; r++) {
             int e = enqueue_pipeline(c, A, n, maxk, wclasses, 1, 1, t.sample_blocks, t.row_blocks,
                                      scan_blocks(c, n), t.smooth_blocks, c->ev_stage);
             if (e) return e;
             BMF_HIP(hipStreamSynchronize(c->stream));
-            for (int i = 0; i < BMF_NUM_STAGES; i++) {
-                float ms = 0.0f;
-                BMF_HIP(hipEventElapsedTime(&ms, c->ev_stage[i], c->ev_stage[i + 1]));
-                out[i] = std::min(out[i], ms);
-            }
+            for (int i = 0; i < BMF_NUM_STAGES; i++)
+                BMF_HIP(hipEventElapsedTime(&ms[(size_t)i * nrep + r], c->ev_stage[i], c->ev_stage[i + 1]));
+        }
+        for (int i = 0; i < BMF_NUM_STAGES; i++) {
+            float* v = ms.data() + (size_t)i * nrep;
+            std::nth_element(v, v + nrep / 2, v + nrep);
+            out[i] = v[nrep / 2];
         }
         return BMF_OK;
     };
+    auto run = [&](const bmf_tuning& t, float* out) { return run_n(t, reps, out); };
+    const bmf_tuning start = c->tune;
     bmf_tuning cur = c->tune;
     const int cu = c->cu_count;
     // sampling: tile x persistent workgroups (stage 1)
@@ -1018,6 +1024,19 @@ int bmf_autotune(bmf_ctx* c, const bmf_chunk_desc* chunks, int n, int reps, bmf_
             }
         }
         cur.smooth_blocks = bb;
+    }
+    // confirmation: the tuned and the starting shapes over the whole frame,
+    // alternately (median of 3 * reps runs each); the tuned ones are kept
+    // only when they are at least 1 % faster
+    {
+        float frame[2] = {0.0f, 0.0f};
+        const bmf_tuning pair[2] = {start, cur};
+        for (int k = 0; k < 2; k++) {
+            rc = run_n(pair[k], 3 * reps, st);
+            if (rc) return rc;
+            for (int i = 0; i < BMF_NUM_STAGES; i++) frame[k] += st[i];
+        }
+        if (!(frame[1] < 0.99f * frame[0])) cur = start;
     }
     rc = bmf_set_tuning(c, &cur);
     if (rc) return rc;

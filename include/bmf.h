@@ -445,9 +445,11 @@ int bmf_get_tuning(bmf_ctx* ctx, bmf_tuning* tuning);
 int bmf_set_tuning(bmf_ctx* ctx, const bmf_tuning* tuning);
 
 /* Autotuner: runs the BATCHED pipeline on these chunks with candidate
- * shapes (HIP events around each stage, the best of `reps` runs per
- * candidate), keeps the fastest shape of each stage in the context and
- * writes it to `best` (may be NULL).  Outputs of the runs are discarded. */
+ * shapes (HIP events around each stage, the median of `reps` runs per
+ * candidate), takes the fastest shape of each stage, then times those shapes
+ * against the context's starting ones over the whole frame and keeps them
+ * only when they are at least 1 % faster; the result goes to the context and
+ * to `best` (may be NULL).  Outputs of the runs are discarded. */
 int bmf_autotune(bmf_ctx* ctx, const bmf_chunk_desc* chunks, int n, int reps, bmf_tuning* best);
 
 /* ---- differentiable extraction --------------------------------------- */
