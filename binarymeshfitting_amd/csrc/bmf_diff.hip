@@ -451,6 +451,91 @@ __global__ void k_nn_leaves(const float4* __restrict__ st, int64_t n, int64_t P,
     hi[P + l] = b;
 }
 
+// Oriented bound of each leaf: the centre c of its targets, the principal
+// axes of their covariance (cyclic Jacobi, then re-orthonormalised) and the
+// largest |(q - c) . axis| per axis; four float4 per leaf (c, then each axis
+// with its extent in w).  A patch of a curved target set is thin along its
+// normal, where the axis-aligned box of a tilted patch is not.  Empty leaves
+// get a zero record (their axis-aligned box already excludes them).
+__global__ void k_nn_obb(const float4* __restrict__ st, int64_t n, int64_t P, float4* __restrict__ ob) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= P) return;
+    const int64_t b = l * kLeaf, e = std::min<int64_t>(n, (l + 1) * kLeaf);
+    float4* o = ob + 4 * l;
+    if (b >= e) {
+        for (int k = 0; k < 4; k++) o[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (int64_t i = b; i < e; i++) {
+        const float4 q = st[i];
+        cx += q.x;
+        cy += q.y;
+        cz += q.z;
+    }
+    const float inv = 1.0f / (float)(e - b);
+    cx *= inv;
+    cy *= inv;
+    cz *= inv;
+    float a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int64_t i = b; i < e; i++) {
+        const float4 q = st[i];
+        const float d[3] = {q.x - cx, q.y - cy, q.z - cz};
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) a[r][c] += d[r] * d[c];
+    }
+    float v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sw = 0; sw < 8; sw++)
+#pragma unroll
+        for (int pq = 0; pq < 3; pq++) {
+            const int p0 = pq == 2 ? 1 : 0, q0 = pq == 0 ? 1 : 2;
+            const float apq = a[p0][q0];
+            if (!(fabsf(apq) > 1e-30f)) continue;
+            const float th = (a[q0][q0] - a[p0][p0]) / (2.0f * apq);
+            const float t = (th >= 0.0f ? 1.0f : -1.0f) / (fabsf(th) + sqrtf(th * th + 1.0f));
+            const float cs = 1.0f / sqrtf(t * t + 1.0f), sn = t * cs;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {  // columns p0, q0 of A J
+                const float akp = a[k][p0], akq = a[k][q0];
+                a[k][p0] = cs * akp - sn * akq;
+                a[k][q0] = sn * akp + cs * akq;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {  // rows p0, q0 of J^T (A J)
+                const float apk = a[p0][k], aqk = a[q0][k];
+                a[p0][k] = cs * apk - sn * aqk;
+                a[q0][k] = sn * apk + cs * aqk;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float vkp = v[k][p0], vkq = v[k][q0];
+                v[k][p0] = cs * vkp - sn * vkq;
+                v[k][q0] = sn * vkp + cs * vkq;
+            }
+        }
+    // axes: the first two eigenvector columns, Gram-Schmidt, and their cross product
+    float u[3] = {v[0][0], v[1][0], v[2][0]}, w[3] = {v[0][1], v[1][1], v[2][1]};
+    float lu = sqrtf(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int k = 0; k < 3; k++) u[k] /= lu;
+    const float dw = w[0] * u[0] + w[1] * u[1] + w[2] * u[2];
+    for (int k = 0; k < 3; k++) w[k] -= dw * u[k];
+    const float lw = sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int k = 0; k < 3; k++) w[k] /= lw;
+    const float nn[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+    const float* ax[3] = {u, w, nn};
+    float ex[3] = {0.0f, 0.0f, 0.0f};
+    for (int64_t i = b; i < e; i++) {
+        const float4 q = st[i];
+        const float d[3] = {q.x - cx, q.y - cy, q.z - cz};
+#pragma unroll
+        for (int k = 0; k < 3; k++) ex[k] = fmaxf(ex[k], fabsf(d[0] * ax[k][0] + d[1] * ax[k][1] + d[2] * ax[k][2]));
+    }
+    o[0] = make_float4(cx, cy, cz, 0.0f);
+    for (int k = 0; k < 3; k++) o[1 + k] = make_float4(ax[k][0], ax[k][1], ax[k][2], ex[k]);
+}
+
 // Inner boxes of one wide level [level, 2 level).
 __global__ void k_nn_level(int64_t level, float4* __restrict__ lo, float4* __restrict__ hi) {
     const int64_t i = level + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -486,6 +571,25 @@ __device__ __forceinline__ float box_lb2(float px, float py, float pz, float4 l,
     const float dy = fmaxf(fmaxf(l.y - py, py - h.y), 0.0f);
     const float dz = fmaxf(fmaxf(l.z - pz, pz - h.z), 0.0f);
     return (dx * dx + dy * dy) + dz * dz;
+}
+
+// Squared-distance lower bound from p to the targets of a leaf from its
+// oriented record: per axis max(0, |(p - c) . axis| - extent - slack), summed
+// and scaled down.  The slack (1e-5 of |p - c|_1 + extent per axis, then
+// 1 - 1e-5 on the sum) is far above the FP32 errors of the projections, of
+// the axes' orthonormality and of the targets' own FP32 distances (each of
+// order 1e-7 relative), so the bound stays below the FP32 distance of every
+// target of the leaf.
+__device__ __forceinline__ float obb_lb2(float px, float py, float pz, const float4* __restrict__ o) {
+    const float4 c = o[0], u = o[1], w = o[2], nrm = o[3];
+    const float dx = px - c.x, dy = py - c.y, dz = pz - c.z;
+    const float l1 = (fabsf(dx) + fabsf(dy)) + fabsf(dz);
+    const float tu = fabsf((dx * u.x + dy * u.y) + dz * u.z), tw = fabsf((dx * w.x + dy * w.y) + dz * w.z),
+                tn = fabsf((dx * nrm.x + dy * nrm.y) + dz * nrm.z);
+    const float su = fmaxf(tu - u.w - 1e-5f * (l1 + u.w), 0.0f);
+    const float sw = fmaxf(tw - w.w - 1e-5f * (l1 + w.w), 0.0f);
+    const float sn = fmaxf(tn - nrm.w - 1e-5f * (l1 + nrm.w), 0.0f);
+    return ((su * su + sw * sw) + sn * sn) * (1.0f - 1e-5f);
 }
 
 // fmin with the value DPP control CTRL brings from another lane (+inf where
@@ -541,6 +645,7 @@ __device__ __forceinline__ int wave_argmin(float x, uint64_t m, int lane) {
 __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict__ pos, int V,
                                                          const float4* __restrict__ st, int64_t n, int64_t P,
                                                          const float4* __restrict__ lo, const float4* __restrict__ hi,
+                                                         const float4* __restrict__ ob,
                                                          int32_t* __restrict__ nearest, float* __restrict__ dldp,
                                                          double* __restrict__ d2) {
     const int v = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -624,7 +729,10 @@ __global__ void __launch_bounds__(kNNThreads) k_nn_query(const float* __restrict
         s_mk[w][k] = m & ~(1ull << j);
         const int f = k == 0 ? f0 : kNNFan;
         const int c0 = (b0 + j) << f;
-        const float lc = lane < (1 << f) ? box_lb2(px, py, pz, lo[c0 + lane], hi[c0 + lane]) : INFINITY;
+        float lc = lane < (1 << f) ? box_lb2(px, py, pz, lo[c0 + lane], hi[c0 + lane]) : INFINITY;
+        // a frame of leaves also takes each leaf's oriented bound
+        if (f0 + kNNFan * k == D && lane < (1 << f) && lc < INFINITY)
+            lc = fmaxf(lc, obb_lb2(px, py, pz, ob + 4 * (int64_t)(c0 + lane - Pi)));
         s_lb[w][k + 1][lane] = lc;
         s_mk[w][k + 1] = __ballot(lane < (1 << f) && lc <= U);
         s_base[w][k + 1] = c0;
@@ -716,6 +824,7 @@ struct bmf_tape {
     float4* st = nullptr;
     float4* blo = nullptr;  // hierarchy boxes, heap-numbered [2P]
     float4* bhi = nullptr;
+    float4* obb = nullptr;  // oriented bound of each leaf [4P]
     int* bb = nullptr;
     void* cub_tmp = nullptr;
     size_t cub_bytes = 0;
@@ -742,6 +851,7 @@ void free_nn(bmf_tape* t) {
     dfree(t->st);
     dfree(t->blo);
     dfree(t->bhi);
+    dfree(t->obb);
     if (t->cub_tmp) (void)hipFree(t->cub_tmp);
     t->cub_tmp = nullptr;
     t->cub_bytes = 0;
@@ -757,7 +867,7 @@ int reserve_nn(bmf_tape* t, int64_t n) {
     for (int i = 0; i < 2; i++)
         ok = ok && dalloc(&t->keys[i], n) == hipSuccess && dalloc(&t->vals[i], n) == hipSuccess;
     ok = ok && dalloc(&t->st, n) == hipSuccess && dalloc(&t->blo, 2 * P) == hipSuccess &&
-         dalloc(&t->bhi, 2 * P) == hipSuccess;
+         dalloc(&t->bhi, 2 * P) == hipSuccess && dalloc(&t->obb, 4 * P) == hipSuccess;
     if (ok) {
         hipcub::DoubleBuffer<uint32_t> kb(t->keys[0], t->keys[1]);
         hipcub::DoubleBuffer<int32_t> vb(t->vals[0], t->vals[1]);
@@ -776,19 +886,21 @@ int reserve_nn(bmf_tape* t, int64_t n) {
 
 // Buffers of the nearest-target hierarchy of n targets: Morton keys and
 // target ids (double-buffered for the radix sort), the targets in Morton
-// order (x, y, z, id), and the boxes of the heap-numbered nodes 1 .. 2P-1.
+// order (x, y, z, id), the boxes of the heap-numbered nodes 1 .. 2P-1 and
+// the oriented bound of each leaf.
 struct NNBuffers {
     uint32_t* keys[2];
     int32_t* vals[2];
     float4* st;
     float4* blo;
     float4* bhi;
+    float4* ob;
     void* cub_tmp;
     size_t cub_bytes;
 };
 
 NNBuffers tape_nn(bmf_tape* t) {
-    return NNBuffers{{t->keys[0], t->keys[1]}, {t->vals[0], t->vals[1]}, t->st, t->blo, t->bhi, t->cub_tmp,
+    return NNBuffers{{t->keys[0], t->keys[1]}, {t->vals[0], t->vals[1]}, t->st, t->blo, t->bhi, t->obb, t->cub_tmp,
                      t->cub_bytes};
 }
 
@@ -807,6 +919,7 @@ int nn_build(const float* targets, int64_t n, int* bb, const NNBuffers& b, hipSt
     BMF_HIP(hipcub::DeviceRadixSort::SortPairs(b.cub_tmp, bytes, kb, vb, (int)n, 0, 30, s));
     hipLaunchKernelGGL(k_nn_gather, dim3(blocks_for(n)), dim3(kThreads), 0, s, targets, vb.Current(), n, b.st);
     hipLaunchKernelGGL(k_nn_leaves, dim3(blocks_for(P)), dim3(kThreads), 0, s, b.st, n, P, b.blo, b.bhi);
+    hipLaunchKernelGGL(k_nn_obb, dim3(blocks_for(P)), dim3(kThreads), 0, s, b.st, n, P, b.ob);
     int64_t level = P >> 1;
     for (; level >= 2048; level >>= 1)
         hipLaunchKernelGGL(k_nn_level, dim3(blocks_for(level)), dim3(kThreads), 0, s, level, b.blo, b.bhi);
@@ -822,7 +935,7 @@ int nn_query(bmf_tape* t, const NNBuffers& b, int64_t n, float* dldp, int32_t* n
              hipStream_t s) {
     const int64_t P = leaf_pow2(n);
     hipLaunchKernelGGL(k_nn_query, dim3((unsigned)(((int64_t)t->V * 64 + kNNThreads - 1) / kNNThreads)),
-                       dim3(kNNThreads), 0, s, t->pos, t->V, b.st, n, P, b.blo, b.bhi,
+                       dim3(kNNThreads), 0, s, t->pos, t->V, b.st, n, P, b.blo, b.bhi, b.ob,
                        nearest ? nearest : t->nearest_tmp, dldp, t->d2);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, t->d2, t->V, loss_out);
     BMF_HIP(hipGetLastError());
@@ -857,9 +970,9 @@ int smooth_steps(bmf_tape* t, float4* a, float4* b, int k, hipStream_t s, float*
 // The hierarchy of up to BMF_NOALLOC_MAX_TARGETS targets lives in one
 // file-scope device array, so that entry point allocates nothing: Morton
 // keys and target ids (2 x 2 x 1 MiB), the sorted targets (4 MiB), the boxes
-// of 2 x 4096 heap nodes of 64-target leaves (2 x 128 KiB) and the radix
-// sort's temporary storage (2 MiB), 10.25 MiB per GPU for the life of the
-// process.  Host code
+// of 2 x 4096 heap nodes of 64-target leaves (2 x 128 KiB), the oriented
+// bounds of the 4096 leaves (256 KiB) and the radix sort's temporary storage
+// (2 MiB), 10.5 MiB per GPU for the life of the process.  Host code
 // never passes the array's name to a kernel (on the host it is a stand-in,
 // not a device address): each device's address comes from
 // hipGetSymbolAddress, taken once per device with that device current.
@@ -869,10 +982,12 @@ static_assert(kNAMax % kLeaf == 0 && (kNALeaves & (kNALeaves - 1)) == 0, "leaves
 constexpr size_t kNAKeyBytes = sizeof(uint32_t) * kNAMax;
 constexpr size_t kNATgtBytes = sizeof(float4) * kNAMax;
 constexpr size_t kNABoxBytes = sizeof(float4) * 2 * kNALeaves;
+constexpr size_t kNAObbBytes = sizeof(float4) * 4 * kNALeaves;
 constexpr size_t kNACubBytes = 2u << 20;
-constexpr size_t kScratchBytes = 4 * kNAKeyBytes + kNATgtBytes + 2 * kNABoxBytes + kNACubBytes;
-static_assert(kScratchBytes == (41u << 18), "10.25 MiB of scratch with 64-target leaves");
-static_assert(kNAKeyBytes % 256 == 0 && kNATgtBytes % 256 == 0 && kNABoxBytes % 256 == 0, "256-byte aligned parts");
+constexpr size_t kScratchBytes = 4 * kNAKeyBytes + kNATgtBytes + 2 * kNABoxBytes + kNAObbBytes + kNACubBytes;
+static_assert(kScratchBytes == (42u << 18), "10.5 MiB of scratch with 64-target leaves");
+static_assert(kNAKeyBytes % 256 == 0 && kNATgtBytes % 256 == 0 && kNABoxBytes % 256 == 0 && kNAObbBytes % 256 == 0,
+              "256-byte aligned parts");
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
@@ -912,6 +1027,8 @@ NNBuffers scratch_nn(void* base) {
     b.blo = reinterpret_cast<float4*>(p);
     b.bhi = reinterpret_cast<float4*>(p + kNABoxBytes);
     p += 2 * kNABoxBytes;
+    b.ob = reinterpret_cast<float4*>(p);
+    p += kNAObbBytes;
     b.cub_tmp = p;
     b.cub_bytes = kNACubBytes;
     return b;

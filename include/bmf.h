@@ -524,7 +524,7 @@ int bmf_tape_loss_nearest(bmf_tape* tape, float* dloss_dpos, int32_t* nearest, f
 /* bmf_loss_nearest for callers that cannot allocate device memory (code
  * under HIP graph capture, tight optimisation loops, harnesses that forbid
  * hipMalloc).  No counterpart in the reference.  The search hierarchy is
- * built in a static device array of the library (10.25 MiB per GPU, held until
+ * built in a static device array of the library (10.5 MiB per GPU, held until
  * the process ends; its address is looked up once per device by
  * bmf_tape_create) instead of the tape's growable buffers, so the call
  * allocates nothing; n_targets above BMF_NOALLOC_MAX_TARGETS returns
