@@ -1,6 +1,7 @@
 """Several contexts on one GPU, their batches in flight together: a stream of
 batches alternated over two or three contexts (each with its own arenas,
-stream and captured frame) overlaps consecutive batches on the GPU.  Bar:
+stream and captured frame) overlaps consecutive batches on the GPU, in the
+triangle path and in quad mode.  Bar:
 after the interleaved batches, each context's meshes are bit-identical to a
 fresh context's extraction of the same batch, and the contexts' different
 batches have not leaked into each other."""
@@ -14,9 +15,11 @@ pytestmark = pytest.mark.gpu
 TERRAIN = workloads.terrain_sampler()
 
 
-def _ctx(n, dim):
+def _ctx(n, dim, quads=False):
     c = capi.Context(max_chunks=n, max_samples=n * dim ** 3, max_vertices=1 << 22, max_indices=1 << 24)
     c.set_sampler(**TERRAIN)
+    if quads:
+        c.set_mesh_options(mesh=capi.MESH_QUADS)
     return c
 
 
@@ -37,8 +40,8 @@ def _same(a, b):
     return True
 
 
-@pytest.mark.parametrize("nctx", [2, 3])
-def test_contexts_interleaved_match_one_context(nctx):
+@pytest.mark.parametrize("nctx,quads", [(2, False), (3, False), (2, True)])
+def test_contexts_interleaved_match_one_context(nctx, quads):
     # batch A: a 4x4x4 slice of the uniform grid (K = 4); batch B: the same
     # slice 16 units higher (the terrain band still crosses its lowest layer)
     # with K = 2, so the two meshes differ
@@ -47,14 +50,14 @@ def test_contexts_interleaved_match_one_context(nctx):
          for c in a]
     n, dim = len(a), a[0].dim
     arr_a, arr_b = capi.chunk_array(workloads.to_capi(a)), capi.chunk_array(workloads.to_capi(b))
-    with _ctx(n, dim) as ref:
+    with _ctx(n, dim, quads) as ref:
         capi.check(ref.extract_raw(arr_a, n, capi.MODE_BATCHED), "bmf_extract")
         want_a = _meshes(ref, n)
         capi.check(ref.extract_raw(arr_b, n, capi.MODE_BATCHED), "bmf_extract")
         want_b = _meshes(ref, n)
     assert not _same(want_a, want_b)
     assert sum(v for v, _, _ in want_a) > 0 and sum(v for v, _, _ in want_b) > 0
-    cs = [_ctx(n, dim) for _ in range(nctx)]
+    cs = [_ctx(n, dim, quads) for _ in range(nctx)]
     try:
         # 4 * nctx batches in flight without a synchronisation between them,
         # round robin over the contexts; a context's batches alternate A / B,
