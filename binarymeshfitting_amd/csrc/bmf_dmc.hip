@@ -614,17 +614,31 @@ __global__ void __launch_bounds__(kRowThreads) k_dmc_quads(Arenas A, int n) {
                 if (!(f[0] | f[1] | f[2])) continue;
                 uint32_t qid = qbase + prefix_pop<W>(ex, z) + prefix_pop<W>(ey, z) + prefix_pop<W>(ez, z);
                 const bool solid_origin = bit_at<W>(r00, z) == 0u;
+                // the rings of the sample's three edges are cells of the 2 x 2 x 2
+                // block (x-1 .. x, y-1 .. y, z-1 .. z): its eight first-vertex ids
+                // and masks load together (a cell outside the chunk reads cell 0;
+                // only the rings of interior edges, all inside, use them)
+                uint32_t tb8[8], mk8[8];
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const int cx = x - 1 + (c >> 2), cy = y - 1 + ((c >> 1) & 1), cz = z - 1 + (c & 1);
+                    const bool in = cx >= 0 && cy >= 0 && cz >= 0 && cx <= dim - 2 && cy <= dim - 2 && cz <= dim - 2;
+                    const int64_t ci = M.dens_off + (in ? ((int64_t)cx * dim + cy) * dim + cz : 0);
+                    tb8[c] = A.cell_tb[ci];
+                    mk8[c] = A.cell_mask[ci];
+                }
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     if (!f[a]) continue;
                     uint32_t v[4];
                     bool ok = true;
+#pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        const int cx = x + c_ring[a][r][0], cy = y + c_ring[a][r][1], cz = z + c_ring[a][r][2];
-                        const int64_t ci = M.dens_off + ((int64_t)cx * dim + cy) * dim + cz;
+                        // ring cell r of axis a: offsets (c_ring) -1 / 0 per axis -> block cell
+                        const int c = ((c_ring[a][r][0] + 1) << 2) | ((c_ring[a][r][1] + 1) << 1) | (c_ring[a][r][2] + 1);
                         const uint32_t le = (uint32_t)c_ring[a][r][3];
-                        const uint32_t pi = (uint32_t)((s_t.pe[A.cell_mask[ci]] >> (4 * le)) & 15ull);
-                        v[r] = A.cell_tb[ci] + pi;
+                        const uint32_t pi = (uint32_t)((s_t.pe[mk8[c]] >> (4 * le)) & 15ull);
+                        v[r] = tb8[c] + pi;
                         if (pi == 15u || v[r] >= Vc) ok = false;
                     }
                     const uint32_t prim = tris ? 2u * qid : qid;
