@@ -1310,7 +1310,7 @@ __device__ __forceinline__ void emit_ids_row(const Arenas& A, const ChunkMeta& M
         const uint64_t ex = EX.w[k], ey = EY.w[k], ez = EZ.w[k];
         const int z = 64 * k + lane;
         if (z < dim) {
-            const uint32_t fx = lane_bit(ex, lane), fy = lane_bit(ey, lane), fz = lane_bit(ez, lane);
+            const uint32_t fx = lane_bit_u(ex), fy = lane_bit_u(ey), fz = lane_bit_u(ez);
             uint32_t id = kb + mbcnt64(ex) + mbcnt64(ey) + mbcnt64(ez);
             s_idrow[z] = (uint16_t)((id - slab_base) | (fx << 14) | (fy << 15));
             if (own && (fx | fy | fz)) {
@@ -1373,10 +1373,10 @@ __device__ __forceinline__ void emit_tris_row(const Arenas& A, const ChunkMeta& 
         const int z = 64 * k + lane;
         // corner bit 4 dx + 2 dy + dz (DMCChunk.cpp:200-438)
         const uint32_t mask =
-            z < dim - 1 ? lane_bit(R00.w[k], lane) | (lane_bit(S00.w[k], lane) << 1) | (lane_bit(R01.w[k], lane) << 2) |
-                              (lane_bit(S01.w[k], lane) << 3) | (lane_bit(R10.w[k], lane) << 4) |
-                              (lane_bit(S10.w[k], lane) << 5) | (lane_bit(R11.w[k], lane) << 6) |
-                              (lane_bit(S11.w[k], lane) << 7)
+            z < dim - 1 ? lane_bit_u(R00.w[k]) | (lane_bit_u(S00.w[k]) << 1) | (lane_bit_u(R01.w[k]) << 2) |
+                              (lane_bit_u(S01.w[k]) << 3) | (lane_bit_u(R10.w[k]) << 4) |
+                              (lane_bit_u(S10.w[k]) << 5) | (lane_bit_u(R11.w[k]) << 6) |
+                              (lane_bit_u(S11.w[k]) << 7)
                         : 0u;
         const uint32_t nt = s_ntri[mask];
         const uint64_t b0 = __ballot(nt & 1u), b1 = __ballot(nt & 2u), b2 = __ballot(nt & 4u);

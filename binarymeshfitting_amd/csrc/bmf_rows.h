@@ -355,6 +355,16 @@ __device__ __forceinline__ RowBits<W> rshr1(const RowBits<W>& a) {
 // number of set bits below it (v_mbcnt).
 __device__ __forceinline__ uint32_t lane_bit(uint64_t w, int lane) { return (uint32_t)(w >> lane) & 1u; }
 
+// The same for a word held in scalar registers (the same value in every
+// lane), as this lane's bit of the word used as the wave's lane mask: one
+// v_cndmask_b32 with the word as its 64-bit condition, no 64-bit shift.  The
+// caller's lane must be the executing lane.
+__device__ __forceinline__ uint32_t lane_bit_u(uint64_t w) {
+    uint32_t r;
+    asm("v_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(r) : "s"(w));
+    return r;
+}
+
 __device__ __forceinline__ uint32_t mbcnt64(uint64_t w) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(w >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)w, 0u));
 }
