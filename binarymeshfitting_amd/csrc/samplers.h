@@ -76,9 +76,10 @@ __device__ __forceinline__ float snoise3(float x, float y, float z, uint32_t see
     int32_t i = (int32_t)fi, j = (int32_t)fj, k = (int32_t)fk;
     float t = (float)(i + j + k) * G3;
     float x0 = x - (fi - t), y0 = y - (fj - t), z0 = z - (fk - t);
-    bool gxy = x0 >= y0, gyz = y0 >= z0, gxz = x0 >= z0;
-    int i1 = gxy & gxz, j1 = (!gxy) & gyz, k1 = (!gxz) & (!gyz);
-    int i2 = gxy | gxz, j2 = (!gxy) | gyz, k2 = (!gxz) | (!gyz);
+    // the rank flags stay lane masks (v_cmp results combined by scalar ops)
+    const bool gxy = x0 >= y0, gyz = y0 >= z0, gxz = x0 >= z0;
+    const bool i1 = gxy && gxz, j1 = !gxy && gyz, k1 = !gxz && !gyz;
+    const bool i2 = gxy || gxz, j2 = !gxy || gyz, k2 = !gxz || !gyz;
     // x0 - (float)i1 is x0 - 1 or exactly x0 (i1 is 0 or 1), so the middle
     // corners select between x0 and the last corner's x0 - 1
     const float xm = x0 - 1.0f, ym = y0 - 1.0f, zm = z0 - 1.0f;
