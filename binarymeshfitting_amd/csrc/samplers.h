@@ -39,6 +39,13 @@ __device__ __forceinline__ uint32_t mul24_const(uint32_t a) {
     return r;
 }
 
+// a ^ b ^ c as one three-input bitwise operation (truth table 0x96)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // Gradient index of a lattice point from its combined coordinate hash
 // h = mul24(i, P1) ^ mul24(j, P2) ^ mul24(k, P3) ^ seed.
 __device__ __forceinline__ uint32_t grad_mix(uint32_t h) {
@@ -91,10 +98,10 @@ __device__ __forceinline__ float snoise3(float x, float y, float z, uint32_t see
     const uint32_t a0 = mul24((uint32_t)i, kP1), a1 = mul24((uint32_t)(i + 1), kP1);
     const uint32_t b0 = mul24((uint32_t)j, kP2), b1 = mul24((uint32_t)(j + 1), kP2);
     const uint32_t c0 = mul24((uint32_t)k, kP3) ^ seed, c1 = mul24((uint32_t)(k + 1), kP3) ^ seed;
-    float n0 = corner(x0, y0, z0, grad_mix(a0 ^ b0 ^ c0));
-    float n1 = corner(x1, y1, z1, grad_mix((i1 ? a1 : a0) ^ (j1 ? b1 : b0) ^ (k1 ? c1 : c0)));
-    float n2 = corner(x2, y2, z2, grad_mix((i2 ? a1 : a0) ^ (j2 ? b1 : b0) ^ (k2 ? c1 : c0)));
-    float n3 = corner(x3, y3, z3, grad_mix(a1 ^ b1 ^ c1));
+    float n0 = corner(x0, y0, z0, grad_mix(xor3(a0, b0, c0)));
+    float n1 = corner(x1, y1, z1, grad_mix(xor3(i1 ? a1 : a0, j1 ? b1 : b0, k1 ? c1 : c0)));
+    float n2 = corner(x2, y2, z2, grad_mix(xor3(i2 ? a1 : a0, j2 ? b1 : b0, k2 ? c1 : c0)));
+    float n3 = corner(x3, y3, z3, grad_mix(xor3(a1, b1, c1)));
     float n = ((n0 + n1) + n2) + n3;
     float r = kNoiseScale * n;
     return fminf(fmaxf(r, -1.0f), 1.0f);
