@@ -278,7 +278,7 @@ __device__ __forceinline__ float csg_density_lds(const float4* s_csg, int ns, in
 // a, b, c (c with the seed folded in): the simplex lattice hash mixed to 24
 // bits k, then k / 2^23 - 1 (exact).
 __device__ __forceinline__ float lattice_value(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t h = a ^ b ^ c;
+    uint32_t h = xor3(a, b, c);
     h ^= h >> 13;
     h = mul24_const<kM1>(h ^ (h >> 24));
     h ^= h >> 11;
