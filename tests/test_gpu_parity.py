@@ -96,6 +96,27 @@ def test_sample_block_matches_oracle(ctx, oracle, c):
     assert np.array_equal(d < 0, ref < 0)
 
 
+# Chunks whose origin has three equal coordinates: their samples with
+# x == y, y == z or x == z (the diagonal on all three) put the simplex rank
+# comparisons x0 >= y0, y0 >= z0, x0 >= z0 on their ties; negative lattice
+# coordinates included.
+TIES = [
+    workloads.Chunk((-8.0, -8.0, -8.0), 48.0, 49, 0.0, 0, 0),
+    workloads.Chunk((-8.25, -8.25, -8.25), 48.0, 97, 0.0, 0, 0),  # spacing 0.5
+]
+
+
+@pytest.mark.parametrize("c", TIES, ids=lambda c: f"dim{c.dim}")
+def test_sample_block_bit_exact_on_simplex_ties(ctx, oracle, c):
+    ctx.set_sampler(**TERRAIN)
+    d = ctx.sample_block(workloads.to_capi([c])[0])
+    delta, opos = oracle.geometry(_ochunk(oracle, c))
+    ref = oracle.sample_block(_ospec(oracle, TERRAIN), c.dim, opos, delta)
+    diag = np.arange(c.dim)
+    assert np.array_equal(d[diag, diag, diag].view(np.uint32), ref[diag, diag, diag].view(np.uint32))
+    assert np.array_equal(d.view(np.uint32), ref.view(np.uint32))
+
+
 @pytest.mark.parametrize("kind", [1, 2, 3, 4])
 def test_implicit_sample_block(ctx, oracle, kind):
     s = dict(kind=kind, world_size=48.0)
